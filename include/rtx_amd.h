@@ -1,0 +1,420 @@
+/*
+ * rtx_amd.h - C ABI of the MI355X path-tracing library (librtxamd.so).
+ *
+ * The library replaces the device side of FelixFifi/rtx-pathtracer: the
+ * Vulkan ray-tracing pipeline launched by RayTracingApp::raytrace
+ * (src/RayTracingApp.cpp:1120-1204, traceRaysKHR at 1199-1201) with its
+ * raygen/closest-hit/any-hit/miss/intersection shaders
+ * (shaders/raytrace.rgen, raytrace.rchit, raytrace.rahit, raytrace.rmiss,
+ * raytrace.shadow.rmiss, raytrace.sphere.rint/.rchit) and the driver-built
+ * acceleration structures (src/SceneLoader.cpp:1101-1193).
+ *
+ * Conventions
+ *   - every entry point returns int: RTX_OK (0) or a negative RTX_E_* code;
+ *     rtx_last_error() gives the message.  Nothing throws across the ABI.
+ *   - caller-owned host arrays are copied during the call; device memory is
+ *     owned by the library and released by the matching *_destroy.
+ *   - one context per host thread; calls on one context are serialised by
+ *     the caller (the reference drives one frame at a time from
+ *     RayTracingApp::drawCallback, src/RayTracingApp.cpp:79-225).
+ *   - the structs below are byte-identical to the reference's device
+ *     layouts, so a host that fills the reference's buffers can hand the
+ *     same bytes to this library.
+ */
+#ifndef RTX_AMD_H
+#define RTX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+#define RTX_OK 0
+#define RTX_E_INVALID (-1)     /* bad argument or inconsistent scene data */
+#define RTX_E_DEVICE (-2)      /* HIP runtime error                       */
+#define RTX_E_MEMORY (-3)      /* device or host allocation failed        */
+#define RTX_E_UNSUPPORTED (-4) /* feature outside this build              */
+#define RTX_E_STATE (-5)       /* call out of order (e.g. no render yet)  */
+
+/* ------------------------------------------------------- limits / enums */
+/* shaders/limits.glsl:1-7 */
+#define RTX_SIZE_LIGHT_RANDOM 10000
+#define RTX_SIZE_TRI_RANDOM 10000
+#define RTX_MAX_DISTRIBUTIONS 16
+#define RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL 16
+
+/* shaders/raytrace.rgen:21-27; src/SceneLoader.h:31-39 */
+#define RTX_MAT_DIFFUSE 0
+#define RTX_MAT_SPECULAR 1
+#define RTX_MAT_DIELECTRIC 2
+#define RTX_MAT_LIGHT 3
+#define RTX_MAT_PHONG 4
+#define RTX_MAT_CONDUCTOR 5
+#define RTX_MAT_ROUGH_CONDUCTOR 6
+
+/* shaders/raytrace.rgen:29-32; src/SceneLoader.h:41-46 */
+#define RTX_LIGHT_AREA 0
+#define RTX_LIGHT_POINT 1
+#define RTX_LIGHT_SPHERE 2
+#define RTX_LIGHT_ENV_MAP 3
+
+/* ------------------------------------------------- device data layouts */
+/* Vertex, std140, 48 B: shaders/wavefront.glsl:1-7; src/Model.h:24-33 */
+typedef struct rtx_vertex {
+    float pos[3];
+    float _pad0;
+    float normal[3];
+    float _pad1;
+    float texCoord[2];
+    int32_t materialIndex;
+    int32_t _pad2;
+} rtx_vertex;
+
+/* Material, std430, 96 B: shaders/wavefront.glsl:9-23; src/SceneLoader.h:48-62 */
+typedef struct rtx_material {
+    float lightColor[3];
+    float _pad0;
+    float diffuse[3];
+    float _pad1;
+    float specular[3];
+    float specularHighlight;
+    float transparency;
+    float refractionIndex;
+    float refractionIndexInv;
+    float eta;
+    float k;
+    float roughness;
+    int32_t textureIdDiffuse;
+    int32_t textureIdSpecular;
+    int32_t type;
+    int32_t _pad2[3];
+} rtx_material;
+
+/* InstanceInfo, std140, 144 B: shaders/wavefront.glsl:25-30; src/SceneLoader.h:64-69.
+ * Matrices are column-major (GLM / GLSL convention): m[col*4 + row]. */
+typedef struct rtx_instance {
+    float transform[16];
+    float normalTransform[16];
+    int32_t modelIndex;
+    int32_t iLight;
+    int32_t _pad[2];
+} rtx_instance;
+
+/* Light, scalar, 40 B: shaders/wavefront.glsl:32-39; src/SceneLoader.h:71-78 */
+typedef struct rtx_light {
+    float color[3];
+    float pos[3];
+    uint32_t instanceIndex;
+    float sampleProb;
+    float area;
+    int32_t type;
+} rtx_light;
+
+/* FaceSample, 12 B: shaders/wavefront.glsl:41-45; src/SceneLoader.h:80-84 */
+typedef struct rtx_face_sample {
+    int32_t index;
+    float sampleProb;
+    float faceArea;
+} rtx_face_sample;
+
+/* sphere, scalar, 24 B: shaders/raycommon.glsl:71-76; src/Shapes.h:65-75 */
+typedef struct rtx_sphere {
+    float center[3];
+    float radius;
+    int32_t materialIndex;
+    int32_t iLight;
+} rtx_sphere;
+
+/* aabb, scalar, 24 B: shaders/raycommon.glsl:78-81; src/Shapes.h:19-63 */
+typedef struct rtx_aabb {
+    float min[3];
+    float max[3];
+} rtx_aabb;
+
+/* cacheHeader / cacheData, scalar: shaders/raycommon.glsl:83-96;
+ * src/IrradianceCache.h:17-30 */
+typedef struct rtx_cache_header {
+    uint32_t nextCacheSlot;
+    uint32_t maxCaches;
+    uint32_t nextUpdateSlot;
+} rtx_cache_header;
+
+typedef struct rtx_cache_data {
+    float color[3];
+    float normal[3];
+    float rotGrad[3];
+    float transGrad[3];
+    float harmonicR;
+    uint32_t numUpdates;
+} rtx_cache_data;
+
+/* VMF_Theta / VMM_Theta, scalar, 40 B / 720 B: shaders/guiding.glsl:14-51;
+ * src/PathGuiding.h:36-75 */
+typedef struct rtx_vmf_theta {
+    float mu[3];
+    float k;
+    float norm;
+    float eMin2K;
+    float distance;
+    float target[3];
+} rtx_vmf_theta;
+
+typedef struct rtx_vmm_theta {
+    rtx_vmf_theta thetas[RTX_MAX_DISTRIBUTIONS];
+    float pi[RTX_MAX_DISTRIBUTIONS];
+    float meanPosition[3];
+    int32_t usedDistributions;
+} rtx_vmm_theta;
+
+/* DirectionalData, scalar, 40 B: shaders/guiding.glsl:99-113 */
+typedef struct rtx_directional_data {
+    float position[3];
+    float direction[3];
+    float weight;
+    float pdf;
+    float distance;
+    uint32_t flags; /* region id, or 0xFFFFFFFF = invalid */
+} rtx_directional_data;
+
+/* Push-constant block, 192 B, 48 four-byte fields in the reference's order
+ * and with its names (shaders/raycommon.glsl:19-69; src/RayTracingApp.h:81-130).
+ * GLSL bools are 4-byte ints.  rtx_push_constant_defaults() fills the
+ * reference's defaults.  enableRR was declared but never read by the
+ * reference's shaders; here it switches on Russian roulette (see
+ * rtx_frame_params.rrStartDepth). */
+typedef struct rtx_push_constant {
+    uint32_t randomUInt;
+    uint32_t previousFrames;
+    int32_t maxDepth;
+    int32_t maxFollowDiscrete;
+    int32_t samplesPerPixel;
+    int32_t enableRR;
+    int32_t enableNEE;
+    int32_t numNEE;
+    int32_t enableAverageInsteadOfMix;
+    int32_t enableMIS;
+    int32_t usePowerHeuristic;
+    int32_t storeEstimate;
+    int32_t visualizeMode;
+    int32_t showIrradianceCacheOnly;
+    int32_t showIrradianceGradients;
+    int32_t useIrradianceCache;
+    int32_t highlightIrradianceCacheColor;
+    float irradianceA;
+    float irradianceUpdateProb;
+    float irradianceCreateProb;
+    float irradianceVisualizationScale;
+    int32_t useIrradianceGradients;
+    int32_t useIrradianceCacheOnGlossy;
+    float irradianceGradientsMaxLength;
+    int32_t isIrradiancePrepareFrame;
+    int32_t irradianceNumNEE;
+    float irradianceCacheMinRadius;
+    int32_t irradianceCachePerformVisibilityCheck;
+    int32_t useVisibleSphereSampling;
+    int32_t useADRRS;
+    float adrrsS;
+    int32_t adrrsSplit;
+    int32_t splitOnFirst;
+    int32_t useGuiding;
+    float guidingProb;
+    float guidingVisuScale;
+    float guidingVisuMax;
+    int32_t guidingVisuIgnoreOcclusioon;
+    int32_t updateGuiding;
+    int32_t useParallaxCompensation;
+    float time;
+    int32_t guidingVisuMove;
+    float guidingVisuPhiScale;
+    float guidingVisuThetaScale;
+    int32_t numGuidingRegions;
+    int32_t guidingPiPHighlightRegion;
+    int32_t guidingPiPShowSpheres;
+    float guidingPiPSize;
+} rtx_push_constant;
+
+/* Camera uniform block, 256 B: shaders/raytrace.rgen:128-134;
+ * src/RayTracingApp.h:65-70.  Column-major 4x4 matrices. */
+typedef struct rtx_camera {
+    float view[16];
+    float proj[16];
+    float viewInverse[16];
+    float projInverse[16];
+} rtx_camera;
+
+/* ------------------------------------------------------ scene upload */
+/* One model = one vertex and one index buffer of the reference
+ * (descriptor set 1 bindings 1 and 2, src/SceneLoader.cpp:946-984). */
+typedef struct rtx_model_desc {
+    const rtx_vertex *vertices;
+    uint32_t vertexCount;
+    const uint32_t *indices; /* 3 per triangle */
+    uint32_t indexCount;
+} rtx_model_desc;
+
+/* Everything descriptor set 1 bindings 1-8 hold for one scene.  The light
+ * sampling tables are the reference's LightSamplerBuffer
+ * (shaders/raytrace.rgen:107-110): randomLightIndex[10000] followed by
+ * randomTriIndex[rows][10000], one row per area light.  The library
+ * copies the data and builds the bounding-volume hierarchy that replaces
+ * the driver's BLAS/TLAS (src/SceneLoader.cpp:1101-1193). */
+typedef struct rtx_scene_desc {
+    const rtx_model_desc *models;
+    uint32_t modelCount;
+    const rtx_instance *instances;
+    uint32_t instanceCount;
+    const rtx_material *materials;
+    uint32_t materialCount;
+    const rtx_light *lights;
+    uint32_t lightCount;
+    const int32_t *randomLightIndex;        /* RTX_SIZE_LIGHT_RANDOM entries */
+    const rtx_face_sample *randomTriIndex;  /* randomTriRows * RTX_SIZE_TRI_RANDOM */
+    uint32_t randomTriRows;
+    const rtx_sphere *spheres;
+    uint32_t sphereCount;
+    /* Optional lat-long environment map (texture slot 0 of the reference,
+     * raytrace.rmiss:15-30), RGBA32F, envWidth*envHeight texels.  NULL means
+     * the reference's default black texture (src/SceneLoader.cpp:80-86). */
+    const float *envMap;
+    uint32_t envWidth;
+    uint32_t envHeight;
+} rtx_scene_desc;
+
+/* ------------------------------------------------------ frame params */
+/* flags of rtx_frame_params */
+#define RTX_FRAME_KEEP_PATHS 0x1u   /* keep per-path radiance + signature     */
+#define RTX_FRAME_COUNT_VISITS 0x2u /* count BVH node/primitive fetches       */
+#define RTX_FRAME_RESET_ACCUM 0x4u  /* clear the accumulation image first     */
+#define RTX_FRAME_TIME_KERNELS 0x8u /* time every kernel with HIP events      */
+
+/* One call to rtx_render renders `spp` samples per pixel and adds them to the
+ * context's accumulation image.
+ *
+ * Random streams.  The reference seeds one LCG stream per pixel with
+ * tea(pixelIndex, pc.randomUInt) (shaders/raytrace.rgen:1625) where
+ * randomUInt is drawn per frame on the host (src/RayTracingApp.cpp:1121).
+ * Here sample s of a call is rendered exactly as the reference renders one
+ * 1-spp frame whose randomUInt is the first output of PCG32 seeded with
+ * (initstate = seed, initseq = sampleOffset + s).  Each (pixel, sample)
+ * path therefore has its own stream that does not depend on batching. */
+typedef struct rtx_frame_params {
+    rtx_push_constant pc; /* pc.randomUInt and pc.samplesPerPixel are ignored */
+    rtx_camera cam;
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;
+    uint32_t sampleOffset;
+    uint64_t seed;
+    uint32_t flags;        /* RTX_FRAME_* */
+    int32_t rrStartDepth;  /* Russian roulette from this depth on (pc.enableRR) */
+    uint32_t batchPaths;   /* paths in flight per wavefront batch, 0 = auto */
+    uint32_t _reserved[7];
+} rtx_frame_params;
+
+/* Statistics of the last rtx_render (all counts are exact). */
+typedef struct rtx_frame_stats {
+    uint64_t paths;         /* camera paths traced (= samples)            */
+    uint64_t extendRays;    /* closest-hit rays (camera + continuation)   */
+    uint64_t shadowRays;    /* any-hit NEE shadow rays                    */
+    uint64_t probeRays;     /* closest-hit MIS probe rays                 */
+    uint64_t nodeVisits;    /* BVH node fetches (RTX_FRAME_COUNT_VISITS)  */
+    uint64_t primVisits;    /* primitive fetches (RTX_FRAME_COUNT_VISITS) */
+    double msTotal;         /* device time of the whole render (HIP events) */
+    double msExtend;        /* time in closest-hit traversal kernels       */
+    double msShadow;        /* time in any-hit traversal kernels           */
+    double msProbe;         /* time in probe traversal kernels             */
+    double msShade;         /* time in generate/shade/finish/accumulate    */
+    uint32_t batches;
+    uint32_t bounces;       /* wavefront iterations of the last batch     */
+    uint32_t nodeBytes;     /* bytes per BVH node record fetched          */
+    uint32_t primBytes;     /* bytes per triangle record fetched          */
+} rtx_frame_stats;
+
+/* BVH statistics of a scene. */
+typedef struct rtx_scene_info {
+    uint32_t triangles;
+    uint32_t spheres;
+    uint32_t nodes;
+    uint32_t maxDepth;
+    uint64_t deviceBytes;   /* nodes + triangle records + shading data */
+    double buildMs;         /* host build time */
+} rtx_scene_info;
+
+typedef struct rtx_ctx rtx_ctx;
+typedef struct rtx_scene rtx_scene;
+
+/* ------------------------------------------------------- entry points */
+/* Fill the reference's push-constant defaults (src/RayTracingApp.h:81-130). */
+void rtx_push_constant_defaults(rtx_push_constant *pc);
+
+/* Create a context on HIP device `device` (replaces the Vulkan instance,
+ * device and pipeline set-up of src/VulkanWindow.cpp and
+ * RayTracingApp::initRayTracing, src/RayTracingApp.cpp:588-604). */
+int rtx_ctx_create(int device, rtx_ctx **out);
+int rtx_ctx_destroy(rtx_ctx *ctx);
+const char *rtx_last_error(const rtx_ctx *ctx);
+
+/* Upload a scene and build its BVH (replaces SceneLoader::createVulkanObjects,
+ * src/SceneLoader.cpp:67-87, incl. createBottomLevelAS/createTopLevelAS). */
+int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *desc, rtx_scene **out);
+int rtx_scene_destroy(rtx_ctx *ctx, rtx_scene *scene);
+int rtx_scene_get_info(const rtx_scene *scene, rtx_scene_info *info);
+
+/* Render params->spp samples per pixel into the accumulation image
+ * (replaces RayTracingApp::raytrace + traceRaysKHR, src/RayTracingApp.cpp:
+ * 1120-1204, and the per-pixel main() of shaders/raytrace.rgen:1623-1827).
+ * Synchronous: returns when the device work is done. stats may be NULL. */
+int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *params,
+               rtx_frame_stats *stats);
+
+/* Copy the mean image (accumulated sum / accumulated samples) as RGBA32F,
+ * width*height*4 floats, alpha = 1 (PostProcessing::saveOffscreenImage,
+ * src/PostProcessing.cpp:310-338). */
+int rtx_read_image(rtx_ctx *ctx, float *rgba, size_t floatCount);
+
+/* Number of samples per pixel accumulated since the last reset. */
+int rtx_accumulated_samples(const rtx_ctx *ctx, uint32_t *samples);
+
+/* Per-path results of the last render made with RTX_FRAME_KEEP_PATHS:
+ * path i = sample (i / (width*height)) of pixel (i % (width*height)).
+ * rgb: 3 floats per path.  sig: 2 uint32 per path = (final LCG state,
+ * FNV-1a hash of the hit primitive ids along the path); either may be NULL. */
+int rtx_read_paths(rtx_ctx *ctx, float *rgb, uint32_t *sig, size_t pathCount);
+
+/* ---------------------------------------------------- traversal queries */
+/* Rays for the query entry points: origin + tmin, direction + tmax. */
+typedef struct rtx_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} rtx_ray;
+
+/* Closest-hit result: t, primitive id (triangles first, in instance order,
+ * then spheres; -1 = miss) and the hit's barycentrics (u, v) for triangles. */
+typedef struct rtx_hit {
+    float t;
+    int32_t prim;
+    float u;
+    float v;
+} rtx_hit;
+
+/* Closest hit for n rays (the scene traceRayEXT with gl_RayFlagsNoneEXT,
+ * shaders/raytrace.rgen:1011-1022). */
+int rtx_trace_closest(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
+                      size_t n, rtx_hit *hits);
+
+/* Occlusion for n rays (the shadow traceRayEXT with TerminateOnFirstHit and
+ * SkipClosestHit, shaders/raytrace.rgen:622-638): occluded[i] = 1 if any
+ * primitive is hit in [tmin, tmax]. */
+int rtx_trace_occluded(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
+                       size_t n, uint8_t *occluded);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* RTX_AMD_H */

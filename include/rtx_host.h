@@ -1,0 +1,79 @@
+/*
+ * rtx_host.h - host side of the path tracer (librtxhost.so).
+ *
+ * This is the C++ host that stays on the CPU in front of librtxamd.so:
+ * the scene layer (SceneLoader / Model / Shapes / WeightedSampler of the
+ * reference), camera matrices (CameraController) and EXR output
+ * (CommonOps::writeEXR).  It produces rtx_scene_desc / rtx_camera values
+ * whose bytes are laid out exactly as the reference's device buffers.
+ *
+ * The seeded procedural scenes of the benchmark workloads live here too,
+ * since the reference only loads scenes from files.
+ */
+#ifndef RTX_HOST_H
+#define RTX_HOST_H
+
+#include <stdint.h>
+#include "rtx_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rtxh_scene rtxh_scene;
+
+/* Procedural scenes. */
+#define RTXH_SCENE_CORNELL 1 /* 5 walls + 2 boxes (34 tris) + quad light + dielectric sphere */
+#define RTXH_SCENE_ROOM 2    /* 20x10x20 m room, exactly `triangles` triangles, 4 area lights */
+#define RTXH_SCENE_PLATES 3  /* plates-and-sphere-lights test (veachMIS-like) */
+
+/* Camera placement of a scene (SceneLoader origin/target/upDir/vfov,
+ * src/SceneLoader.h:147-150). */
+typedef struct rtxh_view {
+    float origin[3];
+    float target[3];
+    float up[3];
+    float vfov; /* degrees */
+} rtxh_view;
+
+/* Build procedural scene `kind` from PCG32 stream `seed`.  `triangles` is
+ * the exact triangle count for RTXH_SCENE_ROOM (0 = 262144) and ignored
+ * otherwise. */
+int rtxh_scene_generate(int kind, uint64_t seed, uint32_t triangles, rtxh_scene **out);
+
+/* Load a Mitsuba-0.6 XML scene (subset read by SceneLoader::parseMitsubaSceneFile,
+ * src/SceneLoader.cpp:329-348) with its OBJ shapes. */
+int rtxh_scene_load_xml(const char *path, rtxh_scene **out);
+
+void rtxh_scene_destroy(rtxh_scene *scene);
+
+/* Pointers stay valid while the scene lives. */
+int rtxh_scene_desc(const rtxh_scene *scene, rtx_scene_desc *desc);
+int rtxh_scene_view(const rtxh_scene *scene, rtxh_view *view);
+
+/* View/projection matrices and their inverses as RayTracingApp::
+ * updateUniformBuffer computes them (src/RayTracingApp.cpp:559-585,
+ * src/CameraController.cpp:76-107). */
+int rtxh_camera(const rtxh_view *view, uint32_t width, uint32_t height, rtx_camera *cam);
+
+/* Write width*height RGBA32F pixels as an uncompressed RGB FLOAT OpenEXR
+ * file (CommonOps::writeEXR, src/CommonOps.cpp:12-38). */
+int rtxh_write_exr(const char *path, const float *rgba, uint32_t width, uint32_t height);
+
+/* Read an uncompressed or ZIP-compressed scanline EXR with HALF or FLOAT
+ * R,G,B[,A] channels into RGBA32F (CommonOps::readEXR, src/CommonOps.cpp:40-65).
+ * *rgba is malloc'ed; free with rtxh_free. */
+int rtxh_read_exr(const char *path, float **rgba, uint32_t *width, uint32_t *height);
+void rtxh_free(void *p);
+
+/* PCG32 (O'Neill 2014): first output after seeding with (initstate, initseq).
+ * This is how rtx_render derives the per-sample frame seed. */
+uint32_t rtxh_pcg32_first(uint64_t initstate, uint64_t initseq);
+
+const char *rtxh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTX_HOST_H */

@@ -1,0 +1,1226 @@
+// CPU oracle: serial per-path restatement of the reference's path-tracing
+// loop.  TEST INFRASTRUCTURE ONLY (see oracle.h).  Every function names the
+// reference lines it follows; "ref:" citations are paths in
+// FelixFifi/rtx-pathtracer.
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+// ------------------------------------------------------------------ vectors
+template <class R> struct V3 {
+    R x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(R a, R b, R c) : x(a), y(b), z(c) {}
+    R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <class R> V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> V3<R> operator*(V3<R> a, V3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class R> V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> V3<R> operator*(R s, V3<R> a) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> V3<R> operator/(V3<R> a, V3<R> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+template <class R> R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> V3<R> cross(V3<R> a, V3<R> b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+template <class R> R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> V3<R> normalize(V3<R> a) { return a / length(a); }
+// GLSL reflect / refract (GLSL 4.60 spec 8.5)
+template <class R> V3<R> reflect(V3<R> i, V3<R> n) { return i - R(2) * dot(n, i) * n; }
+template <class R> V3<R> refract(V3<R> i, V3<R> n, R eta) {
+    R d = dot(n, i);
+    R k = R(1) - eta * eta * (R(1) - d * d);
+    if (k < R(0)) return V3<R>(0, 0, 0);
+    return eta * i - (eta * d + std::sqrt(k)) * n;
+}
+template <class R> V3<R> f3(const float *p) { return V3<R>((R)p[0], (R)p[1], (R)p[2]); }
+template <class R> V3<R> fromD(const double *p) { return V3<R>((R)p[0], (R)p[1], (R)p[2]); }
+template <class R> void toD(V3<R> v, double *p) {
+    p[0] = (double)v.x;
+    p[1] = (double)v.y;
+    p[2] = (double)v.z;
+}
+
+// column-major mat4 * vec4 (xyz part), GLM order (c0*x + c1*y) + (c2*z + c3*w)
+template <class R> V3<R> xform(const float *m, V3<R> v, R w) {
+    R o[3];
+    for (int r = 0; r < 3; ++r) {
+        R a0 = (R)m[0 * 4 + r] * v.x + (R)m[1 * 4 + r] * v.y;
+        R a1 = (R)m[2 * 4 + r] * v.z + (R)m[3 * 4 + r] * w;
+        o[r] = a0 + a1;
+    }
+    return {o[0], o[1], o[2]};
+}
+
+template <class R> struct Consts {
+    static constexpr R PI = R(3.1415926535897932384626433832795);
+    static constexpr R INV_PI = R(1) / PI;
+    static constexpr R INV_2PI = R(1) / (R(2) * PI);
+    static constexpr R HALF_PI = PI / R(2);
+    static constexpr R TMIN = R(0.001);    // ref: shaders/raytrace.rgen:52
+    static constexpr R TMAX = R(1000000.0); // ref: shaders/raytrace.rgen:53
+};
+
+// ------------------------------------------------------------- RNG (ref: random.glsl)
+uint32_t tea(uint32_t val0, uint32_t val1) { // ref: shaders/random.glsl:13-27
+    uint32_t v0 = val0, v1 = val1, s0 = 0;
+    for (uint32_t n = 0; n < 16; n++) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+
+uint32_t pcg32First(uint64_t initstate, uint64_t initseq) { // O'Neill 2014, pcg32_srandom_r
+    uint64_t state = 0, inc = (initseq << 1u) | 1u;
+    auto step = [&]() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    };
+    step();
+    state += initstate;
+    step();
+    return step();
+}
+
+// Bounded rejection loops: the reference's do/while loops have no bound;
+// 2^16 rejections in a row has probability < 0.48^65536, so the bound only
+// guarantees termination and never changes a result.
+constexpr int kMaxRejections = 65536;
+
+// ------------------------------------------------------------------ scene
+struct Node {
+    double lo[3], hi[3];
+    int left, right; // children (internal) or -1
+    int start, count; // leaf range into primOrder
+};
+
+struct HitRec {
+    double t;
+    int prim;
+    double u, v;
+};
+
+} // namespace
+
+struct orc_scene {
+    std::vector<std::vector<rtx_vertex>> verts;
+    std::vector<std::vector<uint32_t>> idx;
+    std::vector<rtx_instance> inst;
+    std::vector<rtx_material> mats;
+    std::vector<rtx_light> lights;
+    std::vector<int32_t> lightIndex;
+    std::vector<rtx_face_sample> triTable;
+    uint32_t triRows = 0;
+    std::vector<rtx_sphere> spheres;
+    // world-space triangles: 9 doubles each (instance transform applied in FP64)
+    std::vector<double> tri;
+    std::vector<float> triF; // the same in FP32 (transform applied in FP32)
+    std::vector<int> triInst, triLocal;
+    size_t ntri = 0;
+    std::vector<Node> nodes;
+    std::vector<int> primOrder;
+};
+
+namespace {
+
+void triBounds(const orc_scene &s, int p, double lo[3], double hi[3]) {
+    if (p < (int)s.ntri) {
+        const double *t = &s.tri[(size_t)p * 9];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min({t[a], t[3 + a], t[6 + a]});
+            hi[a] = std::max({t[a], t[3 + a], t[6 + a]});
+        }
+    } else {
+        const rtx_sphere &sp = s.spheres[p - s.ntri];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = (double)sp.center[a] - std::fabs((double)sp.radius);
+            hi[a] = (double)sp.center[a] + std::fabs((double)sp.radius);
+        }
+    }
+    // pad so that FP32 hits computed slightly outside the exact triangle are kept
+    for (int a = 0; a < 3; ++a) {
+        double pad = 1e-5 * (std::fabs(lo[a]) + std::fabs(hi[a])) + 1e-7;
+        lo[a] -= pad;
+        hi[a] += pad;
+    }
+}
+
+// Plain median-split BVH (the oracle does not care about speed beyond
+// "finishes in seconds"; closest-hit answers do not depend on the tree).
+int buildNode(orc_scene &s, std::vector<std::array<double, 6>> &box, int start, int count) {
+    Node n;
+    for (int a = 0; a < 3; ++a) {
+        n.lo[a] = std::numeric_limits<double>::infinity();
+        n.hi[a] = -std::numeric_limits<double>::infinity();
+    }
+    for (int i = start; i < start + count; ++i) {
+        const auto &b = box[s.primOrder[i]];
+        for (int a = 0; a < 3; ++a) {
+            n.lo[a] = std::min(n.lo[a], b[a]);
+            n.hi[a] = std::max(n.hi[a], b[3 + a]);
+        }
+    }
+    int id = (int)s.nodes.size();
+    s.nodes.push_back(n);
+    if (count <= 4) {
+        s.nodes[id].left = s.nodes[id].right = -1;
+        s.nodes[id].start = start;
+        s.nodes[id].count = count;
+        return id;
+    }
+    double clo[3], chi[3];
+    for (int a = 0; a < 3; ++a) {
+        clo[a] = std::numeric_limits<double>::infinity();
+        chi[a] = -clo[a];
+    }
+    for (int i = start; i < start + count; ++i) {
+        const auto &b = box[s.primOrder[i]];
+        for (int a = 0; a < 3; ++a) {
+            double c = 0.5 * (b[a] + b[3 + a]);
+            clo[a] = std::min(clo[a], c);
+            chi[a] = std::max(chi[a], c);
+        }
+    }
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+    int mid = start + count / 2;
+    std::nth_element(s.primOrder.begin() + start, s.primOrder.begin() + mid,
+                     s.primOrder.begin() + start + count, [&](int x, int y) {
+                         double cx = box[x][axis] + box[x][3 + axis];
+                         double cy = box[y][axis] + box[y][3 + axis];
+                         return cx < cy || (cx == cy && x < y);
+                     });
+    int l = buildNode(s, box, start, mid - start);
+    int r = buildNode(s, box, mid, start + count - mid);
+    s.nodes[id].left = l;
+    s.nodes[id].right = r;
+    s.nodes[id].start = s.nodes[id].count = 0;
+    return id;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), the test
+// the GPU kernel uses too.  Barycentrics (u, v) weight vertices 1 and 2,
+// as the reference's hit attributes do (raytrace.rchit:28).
+template <class R> struct RayPre {
+    V3<R> o, d;
+    int kx, ky, kz;
+    R Sx, Sy, Sz;
+    explicit RayPre(V3<R> org, V3<R> dir) : o(org), d(dir) {
+        R ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+        kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+        kx = (kz + 1) % 3;
+        ky = (kx + 1) % 3;
+        if (d[kz] < R(0)) std::swap(kx, ky);
+        Sx = d[kx] / d[kz];
+        Sy = d[ky] / d[kz];
+        Sz = R(1) / d[kz];
+    }
+};
+
+template <class R>
+bool isectTri(const RayPre<R> &rp, V3<R> v0, V3<R> v1, V3<R> v2, R tmin, R tmax, R &t, R &u, R &v) {
+    V3<R> A = v0 - rp.o, B = v1 - rp.o, C = v2 - rp.o;
+    R Ax = A[rp.kx] - rp.Sx * A[rp.kz], Ay = A[rp.ky] - rp.Sy * A[rp.kz];
+    R Bx = B[rp.kx] - rp.Sx * B[rp.kz], By = B[rp.ky] - rp.Sy * B[rp.kz];
+    R Cx = C[rp.kx] - rp.Sx * C[rp.kz], Cy = C[rp.ky] - rp.Sy * C[rp.kz];
+    R U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (sizeof(R) == 4 && (U == R(0) || V == R(0) || W == R(0))) {
+        U = (R)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (R)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (R)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < R(0) || V < R(0) || W < R(0)) && (U > R(0) || V > R(0) || W > R(0))) return false;
+    R det = U + V + W;
+    if (det == R(0)) return false;
+    R Az = rp.Sz * A[rp.kz], Bz = rp.Sz * B[rp.kz], Cz = rp.Sz * C[rp.kz];
+    R T = U * Az + V * Bz + W * Cz;
+    R tt = T / det;
+    if (!(tt >= tmin && tt <= tmax)) return false;
+    t = tt;
+    u = V / det;
+    v = W / det;
+    return true;
+}
+
+// Analytic sphere (ref: shaders/raytrace.sphere.rint:11-29): both roots are
+// reported; the nearest root inside [tmin, tmax] is kept.
+template <class R> bool isectSphere(const rtx_sphere &s, V3<R> o, V3<R> d, R tmin, R tmax, R &t) {
+    V3<R> oc = o - f3<R>(s.center);
+    R dotDOC = dot(d, oc);
+    R r = (R)s.radius;
+    R rootTerm = dotDOC * dotDOC - dot(oc, oc) + r * r;
+    if (rootTerm < R(0)) return false;
+    R root = std::sqrt(rootTerm);
+    R t1 = -dotDOC + root, t2 = -dotDOC - root;
+    bool hit = false;
+    R best = tmax;
+    if (t1 >= tmin && t1 <= best) {
+        best = t1;
+        hit = true;
+    }
+    if (t2 >= tmin && t2 <= best) {
+        best = t2;
+        hit = true;
+    }
+    t = best;
+    return hit;
+}
+
+template <class R> V3<R> triVert(const orc_scene &s, int p, int k) {
+    if (sizeof(R) == 8) {
+        const double *q = &s.tri[(size_t)p * 9 + 3 * k];
+        return V3<R>((R)q[0], (R)q[1], (R)q[2]);
+    }
+    const float *q = &s.triF[(size_t)p * 9 + 3 * k];
+    return V3<R>((R)q[0], (R)q[1], (R)q[2]);
+}
+
+template <class R> bool boxHit(const Node &n, const RayPre<R> &rp, double tmin, double tmax) {
+    double t0 = tmin, t1 = tmax;
+    for (int a = 0; a < 3; ++a) {
+        double o = (double)rp.o[a], d = (double)rp.d[a];
+        if (d == 0.0) {
+            if (o < n.lo[a] || o > n.hi[a]) return false;
+            continue;
+        }
+        double inv = 1.0 / d;
+        double tn = (n.lo[a] - o) * inv, tf = (n.hi[a] - o) * inv;
+        if (tn > tf) std::swap(tn, tf);
+        t0 = std::max(t0, tn);
+        t1 = std::min(t1, tf * (1 + 1e-9));
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
+// Closest hit: smallest t; equal t -> smaller primitive id.
+template <class R>
+bool closest(const orc_scene &s, V3<R> o, V3<R> d, R tmin, R tmax, bool brute, HitRec &h) {
+    RayPre<R> rp(o, d);
+    R bestT = tmax;
+    int bestP = -1;
+    R bu = 0, bv = 0;
+    auto testPrim = [&](int p) {
+        R t, u = 0, v = 0;
+        bool hit;
+        if (p < (int)s.ntri)
+            hit = isectTri<R>(rp, triVert<R>(s, p, 0), triVert<R>(s, p, 1), triVert<R>(s, p, 2), tmin,
+                              bestT, t, u, v);
+        else
+            hit = isectSphere<R>(s.spheres[p - s.ntri], o, d, tmin, bestT, t);
+        if (!hit) return;                                // hits satisfy t <= bestT
+        if (bestP >= 0 && t == bestT && p > bestP) return; // tie: smaller id wins
+        bestT = t;
+        bestP = p;
+        bu = u;
+        bv = v;
+    };
+    size_t nprim = s.ntri + s.spheres.size();
+    if (brute || s.nodes.empty()) {
+        for (size_t p = 0; p < nprim; ++p) testPrim((int)p);
+    } else {
+        std::vector<int> stack;
+        stack.push_back(0);
+        while (!stack.empty()) {
+            const Node &n = s.nodes[stack.back()];
+            stack.pop_back();
+            if (!boxHit<R>(n, rp, (double)tmin, (double)bestT)) continue;
+            if (n.left < 0) {
+                for (int i = n.start; i < n.start + n.count; ++i) testPrim(s.primOrder[i]);
+            } else {
+                stack.push_back(n.left);
+                stack.push_back(n.right);
+            }
+        }
+    }
+    if (bestP < 0) return false;
+    h.t = (double)bestT;
+    h.prim = bestP;
+    h.u = (double)bu;
+    h.v = (double)bv;
+    return true;
+}
+
+template <class R> bool occluded(const orc_scene &s, V3<R> o, V3<R> d, R tmin, R tmax, bool brute) {
+    RayPre<R> rp(o, d);
+    auto testPrim = [&](int p) {
+        R t, u, v;
+        if (p < (int)s.ntri)
+            return isectTri<R>(rp, triVert<R>(s, p, 0), triVert<R>(s, p, 1), triVert<R>(s, p, 2), tmin,
+                               tmax, t, u, v);
+        return isectSphere<R>(s.spheres[p - s.ntri], o, d, tmin, tmax, t);
+    };
+    size_t nprim = s.ntri + s.spheres.size();
+    if (brute || s.nodes.empty()) {
+        for (size_t p = 0; p < nprim; ++p)
+            if (testPrim((int)p)) return true;
+        return false;
+    }
+    std::vector<int> stack{0};
+    while (!stack.empty()) {
+        const Node &n = s.nodes[stack.back()];
+        stack.pop_back();
+        if (!boxHit<R>(n, rp, (double)tmin, (double)tmax)) continue;
+        if (n.left < 0) {
+            for (int i = n.start; i < n.start + n.count; ++i)
+                if (testPrim(s.primOrder[i])) return true;
+        } else {
+            stack.push_back(n.left);
+            stack.push_back(n.right);
+        }
+    }
+    return false;
+}
+
+// ------------------------------------------------------------ the tracer
+template <class R> struct HitInfo { // ref: shaders/raycommon.glsl:1-13
+    V3<R> worldPos, normal;
+    R u = 0, v = 0; // textureUV
+    int matIndex = -1;
+    R t = 0;
+    bool isMiss = true, isFrontFace = false, isSphere = false;
+    V3<R> missColor;
+    int instanceIndex = 0;
+    int prim = -1;
+};
+
+template <class R> struct Tracer {
+    using C = Consts<R>;
+    const orc_scene &S;
+    const rtx_frame_params &fp;
+    const rtx_push_constant &pc;
+    uint32_t seed = 0;
+    uint32_t primHash = 2166136261u;
+
+    Tracer(const orc_scene &s, const rtx_frame_params &f) : S(s), fp(f), pc(f.pc) {}
+
+    // ref: shaders/random.glsl:31-57
+    R rnd() {
+        seed = 1664525u * seed + 1013904223u;
+        return R(seed & 0x00FFFFFFu) / R(16777216.0);
+    }
+    R negPos() { return rnd() * R(2) - R(1); }
+    // int(rnd() * (max + 1)) evaluated in FP32 as the shader does: the
+    // product is what decides the integer, so it is kept bit-exact.
+    int randomInteger(int max) {
+        float u = (float)rnd();
+        return (int)(u * (float)(max + 1));
+    }
+
+    // ref: shaders/transform.glsl:7-42
+    static void coordinateAxis(V3<R> z, V3<R> &x, V3<R> &y) {
+        if (std::fabs(z.x) > std::fabs(z.y)) {
+            R invLen = R(1) / std::sqrt(z.x * z.x + z.z * z.z);
+            y = V3<R>(z.z * invLen, R(0), -z.x * invLen);
+        } else {
+            R invLen = R(1) / std::sqrt(z.y * z.y + z.z * z.z);
+            y = V3<R>(R(0), z.z * invLen, -z.y * invLen);
+        }
+        x = cross(y, z);
+    }
+    static V3<R> toWorld(V3<R> v, V3<R> n) {
+        V3<R> x, y;
+        coordinateAxis(n, x, y);
+        return v.x * x + v.y * y + v.z * n;
+    }
+
+    // ref: shaders/random.glsl:60-68
+    V3<R> randomOnUnitSphere() {
+        V3<R> res;
+        for (int it = 0; it < kMaxRejections; ++it) {
+            R a = negPos(), b = negPos(), c = negPos();
+            res = V3<R>(a, b, c);
+            if (!(length(res) > R(1))) break;
+        }
+        return normalize(res);
+    }
+    // ref: shaders/random.glsl:86-94
+    V3<R> randomInHemisphereCosine(V3<R> n) {
+        R u = rnd();
+        R sqrt_u = std::sqrt(u);
+        R phi = R(2) * C::PI * rnd();
+        V3<R> local(sqrt_u * std::cos(phi), sqrt_u * std::sin(phi), std::sqrt(R(1) - u));
+        return toWorld(local, n);
+    }
+    // ref: shaders/random.glsl:97-106
+    V3<R> randomInHemisphereCosinePower(V3<R> refl, R p) {
+        R u = rnd();
+        R cosTheta = std::pow(u, R(1) / (p + R(1)));
+        R phi = R(2) * C::PI * rnd();
+        R sinTheta = std::sqrt(R(1) - cosTheta * cosTheta);
+        V3<R> local(sinTheta * std::cos(phi), sinTheta * std::sin(phi), cosTheta);
+        return toWorld(local, refl);
+    }
+    // ref: shaders/random.glsl:108-115
+    V3<R> randomOnSphere(const rtx_sphere &s, V3<R> &normal) {
+        for (int it = 0; it < kMaxRejections; ++it) {
+            R a = negPos(), b = negPos(), c = negPos();
+            normal = V3<R>(a, b, c);
+            if (!(length(normal) > R(1))) break;
+        }
+        normal = normalize(normal);
+        return f3<R>(s.center) + normal * (R)s.radius;
+    }
+    // ref: shaders/random.glsl:127-136
+    V3<R> randomBeckmannNormal(const rtx_material &m, V3<R> n) {
+        R a = (R)m.roughness;
+        R thetaM = std::atan(std::sqrt(-a * a * std::log(R(1) - rnd())));
+        R phiM = R(2) * C::PI * rnd();
+        R cosThetaNM = std::cos(thetaM);
+        V3<R> localM(std::sin(thetaM) * std::cos(phiM), std::sin(thetaM) * std::sin(phiM), cosThetaNM);
+        return toWorld(localM, n);
+    }
+
+    // ---------------------------------------------------------- BSDF layer
+    // ref: shaders/raytrace.rgen:145-160
+    static R fresnel(R eta, R cosThetaI) {
+        R sinThetaSqr = eta * eta * (R(1) - cosThetaI * cosThetaI);
+        if (sinThetaSqr > R(1)) return R(1);
+        R cosThetaT = std::sqrt(R(1) - sinThetaSqr);
+        R Rs = (eta * cosThetaI - cosThetaT) / (eta * cosThetaI + cosThetaT);
+        R Rp = (cosThetaI - eta * cosThetaT) / (cosThetaI + eta * cosThetaT);
+        return (Rs * Rs + Rp * Rp) / R(2);
+    }
+    // ref: shaders/raytrace.rgen:163-175
+    static R fresnelConductor(R cosThetaI, R eta, R k) {
+        if (cosThetaI < R(0)) cosThetaI = -cosThetaI;
+        R ek = eta * eta + k * k;
+        R c2 = cosThetaI * cosThetaI;
+        R Rs2 = (ek * c2 - R(2) * eta * cosThetaI + R(1)) / (ek * c2 + R(2) * eta * cosThetaI + R(1));
+        R Rp2 = (ek - R(2) * eta * cosThetaI + c2) / (ek + R(2) * eta * cosThetaI + c2);
+        return (Rs2 + Rp2) / R(2);
+    }
+    // Beckmann D, ref: shaders/raytrace.rgen:179-202 (pow(E, x) == exp(x))
+    static R D(const rtx_material &m, V3<R> n, V3<R> h) {
+        R cosTheta = dot(n, h);
+        if (cosTheta <= R(0)) return R(0);
+        R theta = std::acos(cosTheta);
+        if (std::isnan(theta) || std::isinf(theta)) theta = 0;
+        R tanTheta = std::tan(theta);
+        if (std::isnan(tanTheta) || std::isinf(tanTheta)) tanTheta = 0;
+        R alphaSqr = (R)m.roughness * (R)m.roughness;
+        R c4 = cosTheta * cosTheta * cosTheta * cosTheta;
+        return std::exp(-tanTheta * tanTheta / alphaSqr) / (C::PI * alphaSqr * c4);
+    }
+    // ref: shaders/raytrace.rgen:204-222, kept with its two typos: tan() of a
+    // cosine, and 2.577*a where 2.577*a^2 is meant.
+    static R G1(const rtx_material &m, V3<R> n, V3<R> h, V3<R> v) {
+        R thetaV = dot(v, n);
+        R c = dot(v, h) / thetaV;
+        if (c <= R(0)) return R(0);
+        R a = R(1) / ((R)m.roughness * std::tan(thetaV));
+        if (a >= R(1.6)) return R(1);
+        R a2 = a * a;
+        return (R(3.535) * a + R(2.181) * a2) / (R(1) + R(2.276) * a + R(2.577) * a);
+    }
+    // ref: shaders/raytrace.rgen:224-230
+    static R G(const rtx_material &m, V3<R> i, V3<R> o, V3<R> n, V3<R> h) {
+        R result = G1(m, n, h, i) * G1(m, n, h, o);
+        if (result < R(0)) return R(0);
+        return result;
+    }
+    // ref: shaders/raytrace.rgen:232-239 (no textures)
+    static V3<R> diffuse(const rtx_material &m) { return f3<R>(m.diffuse) / C::PI; }
+    // ref: shaders/raytrace.rgen:242-267
+    static V3<R> phong(const rtx_material &m, V3<R> n, V3<R> wi, V3<R> wo) {
+        V3<R> res(0, 0, 0);
+        R cosThetaWo = dot(wo, n);
+        if (cosThetaWo > R(0)) {
+            res = res + f3<R>(m.diffuse) / C::PI;
+            R dotReflDir = dot(reflect(-wo, n), wi);
+            if (dotReflDir > R(0)) {
+                R h = (R)m.specularHighlight;
+                res = res + ((h + R(2)) / (R(2) * C::PI)) * f3<R>(m.specular) * std::pow(dotReflDir, h);
+            }
+        }
+        return cosThetaWo * res;
+    }
+    // ref: shaders/raytrace.rgen:269-280
+    static V3<R> roughConductor(const rtx_material &m, V3<R> n, V3<R> wi, V3<R> wo) {
+        V3<R> hr = normalize(wi + wo);
+        R cosThetaIHr = dot(wi, hr);
+        R cosThetaI = dot(wi, n);
+        R cosThetaO = dot(wo, n);
+        if (cosThetaI <= R(0)) return V3<R>(0, 0, 0);
+        R val = fresnelConductor(cosThetaIHr, (R)m.eta, (R)m.k) * G(m, wi, wo, n, hr) * D(m, n, hr) /
+                (R(4) * cosThetaI * cosThetaO);
+        return V3<R>(cosThetaO * val, cosThetaO * val, cosThetaO * val);
+    }
+    // ref: shaders/raytrace.rgen:284-332
+    static R pdfBSDF(const rtx_material &m, V3<R> n, V3<R> wi, V3<R> wo) {
+        switch (m.type) {
+        case RTX_MAT_ROUGH_CONDUCTOR: {
+            V3<R> hr = normalize(wi + wo);
+            R pm = D(m, n, hr) * std::fabs(dot(hr, n));
+            if (pm <= R(0) || dot(wo, hr) <= R(0)) return R(0);
+            return pm / (R(4) * std::fabs(dot(wo, hr)));
+        }
+        case RTX_MAT_PHONG: {
+            if (dot(n, wo) < R(0)) return R(0);
+            R lD = length(f3<R>(m.diffuse)), lS = length(f3<R>(m.specular));
+            R sum = lD + lS;
+            if (sum == R(0)) return R(0);
+            V3<R> reflected = reflect(-wi, n);
+            R h = (R)m.specularHighlight;
+            R pdf = 0;
+            if (dot(reflected, wo) > R(0)) {
+                pdf = (h + R(1)) * std::pow(dot(reflected, wo), h) / (R(2) * C::PI);
+                pdf *= lS / sum;
+            }
+            pdf += dot(wo, n) / C::PI * lD / sum;
+            return pdf;
+        }
+        default:
+            return dot(wo, n) / C::PI;
+        }
+    }
+    // ref: shaders/raytrace.rgen:483-552
+    R sampleBSDF(const rtx_material &m, V3<R> wi, V3<R> n, bool frontFace, V3<R> &newDir) {
+        switch (m.type) {
+        case RTX_MAT_ROUGH_CONDUCTOR: {
+            V3<R> worldM = randomBeckmannNormal(m, n);
+            newDir = reflect(-wi, worldM);
+            return pdfBSDF(m, n, wi, newDir);
+        }
+        case RTX_MAT_PHONG: {
+            R lD = length(f3<R>(m.diffuse)), lS = length(f3<R>(m.specular));
+            R sum = lD + lS;
+            if (sum == R(0)) return R(0);
+            if (rnd() * sum > lD) {
+                V3<R> reflected = reflect(-wi, n);
+                newDir = randomInHemisphereCosinePower(reflected, (R)m.specularHighlight);
+                if (dot(n, newDir) < R(0)) return R(0);
+                return pdfBSDF(m, n, wi, newDir);
+            }
+            newDir = randomInHemisphereCosine(n);
+            return pdfBSDF(m, n, wi, newDir);
+        }
+        case RTX_MAT_SPECULAR:
+        case RTX_MAT_CONDUCTOR:
+            newDir = reflect(-wi, n);
+            return R(1);
+        case RTX_MAT_DIELECTRIC: {
+            R eta = frontFace ? (R)m.refractionIndexInv : (R)m.refractionIndex;
+            R cosThetaI = dot(wi, n);
+            R F = fresnel(eta, cosThetaI);
+            if (rnd() > F) {
+                newDir = refract(-wi, n, eta);
+                return R(1) - F;
+            }
+            newDir = reflect(-wi, n);
+            return F;
+        }
+        default:
+            newDir = randomInHemisphereCosine(n);
+            return dot(newDir, n) / C::PI;
+        }
+    }
+    // ref: shaders/raytrace.rgen:557-599
+    static V3<R> evalBsdf(const rtx_material &m, V3<R> n, V3<R> wi, V3<R> wo, bool frontFace) {
+        switch (m.type) {
+        case RTX_MAT_DIFFUSE:
+        case RTX_MAT_LIGHT:
+            return dot(wo, n) * diffuse(m);
+        case RTX_MAT_PHONG:
+            return phong(m, n, wi, wo);
+        case RTX_MAT_ROUGH_CONDUCTOR: {
+            V3<R> r = roughConductor(m, n, wi, wo);
+            if (std::isnan(r.x)) return V3<R>(0, 0, 0);
+            return r;
+        }
+        case RTX_MAT_DIELECTRIC: {
+            R cosThetaI = dot(n, wi);
+            R eta = frontFace ? (R)m.refractionIndexInv : (R)m.refractionIndex;
+            if (dot(n, wo) < R(0)) return f3<R>(m.specular) * (R(1) - fresnel(eta, cosThetaI));
+            return f3<R>(m.specular) * fresnel(eta, cosThetaI);
+        }
+        case RTX_MAT_CONDUCTOR: {
+            R F = fresnelConductor(dot(wi, n), (R)m.eta, (R)m.k);
+            return V3<R>(F, F, F);
+        }
+        case RTX_MAT_SPECULAR:
+            return f3<R>(m.specular);
+        default:
+            return V3<R>(0, 0, 0);
+        }
+    }
+    static bool hasDiscreteDirection(const rtx_material &m) { // ref: raytrace.rgen:733-742
+        return m.type == RTX_MAT_DIELECTRIC || m.type == RTX_MAT_SPECULAR || m.type == RTX_MAT_CONDUCTOR;
+    }
+    static R powerHeuristic(R p1, R p2) { // ref: raytrace.rgen:341-344
+        R s = p1 * p1;
+        return s / (s + p2 * p2);
+    }
+    static R balanceHeuristic(R p1, R p2) { return p1 / (p1 + p2); } // ref: raytrace.rgen:346-348
+    R heuristic(R a, R b) const { return pc.usePowerHeuristic ? powerHeuristic(a, b) : balanceHeuristic(a, b); }
+
+    // ------------------------------------------------------ traversal + hit shaders
+    HitInfo<R> trace(V3<R> o, V3<R> d, R tmin, R tmax) {
+        HitInfo<R> info;
+        HitRec h;
+        if (!closest<R>(S, o, d, tmin, tmax, false, h)) { // ref: raytrace.rmiss:18-31
+            info.normal = -d;
+            info.t = tmax;
+            info.isMiss = true;
+            info.missColor = V3<R>(0, 0, 0); // default black env texture
+            info.matIndex = -1;
+            info.prim = -1;
+            return info;
+        }
+        info.isMiss = false;
+        info.prim = h.prim;
+        info.t = (R)h.t;
+        if (h.prim >= (int)S.ntri) { // ref: raytrace.sphere.rchit:13-35
+            int si = h.prim - (int)S.ntri;
+            const rtx_sphere &s = S.spheres[si];
+            V3<R> pos = o + info.t * d;
+            V3<R> normal = normalize(pos - f3<R>(s.center));
+            if (dot(d, normal) < R(0)) {
+                info.isFrontFace = true;
+                info.normal = normal;
+            } else {
+                info.isFrontFace = false;
+                info.normal = -normal;
+            }
+            info.worldPos = pos;
+            info.u = info.v = 0;
+            info.matIndex = s.materialIndex;
+            info.isSphere = true;
+            info.instanceIndex = si;
+            return info;
+        }
+        // ref: raytrace.rchit:16-58
+        int inst = S.triInst[h.prim], local = S.triLocal[h.prim];
+        const rtx_instance &in = S.inst[inst];
+        const auto &vv = S.verts[in.modelIndex];
+        const auto &ii = S.idx[in.modelIndex];
+        const rtx_vertex &v0 = vv[ii[3 * local + 0]], &v1 = vv[ii[3 * local + 1]], &v2 = vv[ii[3 * local + 2]];
+        R b1 = (R)h.u, b2 = (R)h.v, b0 = R(1) - b1 - b2;
+        V3<R> normal = f3<R>(v0.normal) * b0 + f3<R>(v1.normal) * b1 + f3<R>(v2.normal) * b2;
+        normal = normalize(xform<R>(in.normalTransform, normal, R(0)));
+        V3<R> wp = f3<R>(v0.pos) * b0 + f3<R>(v1.pos) * b1 + f3<R>(v2.pos) * b2;
+        wp = xform<R>(in.transform, wp, R(1));
+        R tu = (R)v0.texCoord[0] * b0 + (R)v1.texCoord[0] * b1 + (R)v2.texCoord[0] * b2;
+        R tv = (R)v0.texCoord[1] * b0 + (R)v1.texCoord[1] * b1 + (R)v2.texCoord[1] * b2;
+        if (dot(d, normal) < R(0)) {
+            info.isFrontFace = true;
+            info.normal = normal;
+        } else {
+            info.isFrontFace = false;
+            info.normal = -normal;
+        }
+        info.worldPos = wp;
+        info.u = tu;
+        info.v = tv;
+        info.matIndex = v0.materialIndex;
+        info.isSphere = false;
+        info.instanceIndex = inst;
+        return info;
+    }
+
+    // ref: shaders/raytrace.rgen:358-473
+    R sampleLights(V3<R> origin, V3<R> normal, V3<R> &lightDir, V3<R> &lightColor, R &lightDistance) {
+        int iRandomLight = randomInteger(RTX_SIZE_LIGHT_RANDOM - 1);
+        int iLight = S.lightIndex[iRandomLight];
+        const rtx_light &light = S.lights[iLight];
+        if (light.type == RTX_LIGHT_POINT) {
+            V3<R> toLight = f3<R>(light.pos) - origin;
+            lightDistance = length(toLight);
+            lightDir = toLight / lightDistance;
+            lightColor = f3<R>(light.color) / (lightDistance * lightDistance);
+            return (R)light.sampleProb;
+        } else if (light.type == RTX_LIGHT_SPHERE) {
+            const rtx_sphere &s = S.spheres[light.instanceIndex];
+            lightColor = f3<R>(S.mats[s.materialIndex].lightColor);
+            V3<R> sphereNormal;
+            R area = (R)light.area;
+            V3<R> position = randomOnSphere(s, sphereNormal);
+            V3<R> toLight = position - origin;
+            lightDistance = length(toLight);
+            lightDir = normalize(toLight);
+            R cosThetaLight = dot(-lightDir, sphereNormal);
+            return (R)light.sampleProb * lightDistance * lightDistance / (cosThetaLight * area);
+        } else if (light.type == RTX_LIGHT_ENV_MAP) {
+            return R(0); // rejected at scene creation
+        }
+        int iModel = S.inst[light.instanceIndex].modelIndex;
+        int iRandomTri = randomInteger(RTX_SIZE_TRI_RANDOM - 1);
+        const rtx_face_sample &ts = S.triTable[(size_t)iLight * RTX_SIZE_TRI_RANDOM + iRandomTri];
+        int iTri = ts.index;
+        const auto &vv = S.verts[iModel];
+        const auto &ii = S.idx[iModel];
+        const rtx_vertex &v0 = vv[ii[3 * iTri + 0]], &v1 = vv[ii[3 * iTri + 1]], &v2 = vv[ii[3 * iTri + 2]];
+        R rx = rnd();
+        R ry = rnd();
+        R sqrtx = std::sqrt(rx);
+        R b0 = R(1) - sqrtx, b1 = sqrtx * (R(1) - ry), b2 = ry * sqrtx;
+        V3<R> pos = f3<R>(v0.pos) * b0 + f3<R>(v1.pos) * b1 + f3<R>(v2.pos) * b2;
+        V3<R> nrm = f3<R>(v0.normal) * b0 + f3<R>(v1.normal) * b1 + f3<R>(v2.normal) * b2;
+        const rtx_instance &in = S.inst[light.instanceIndex];
+        pos = xform<R>(in.transform, pos, R(1));
+        nrm = normalize(xform<R>(in.normalTransform, nrm, R(0)));
+        V3<R> toLight = pos - origin;
+        lightDistance = length(toLight);
+        lightDir = toLight / lightDistance;
+        lightColor = f3<R>(S.mats[v0.materialIndex].lightColor);
+        R cosThetaLight = dot(-lightDir, nrm);
+        if (cosThetaLight < R(0)) cosThetaLight = -cosThetaLight;
+        return (R)light.sampleProb * lightDistance * lightDistance / cosThetaLight / (R)light.area;
+    }
+
+    static bool neeCapable(const rtx_material &m) { // ref: raytrace.rgen:602-611
+        return m.type == RTX_MAT_DIFFUSE || m.type == RTX_MAT_PHONG || m.type == RTX_MAT_LIGHT ||
+               m.type == RTX_MAT_ROUGH_CONDUCTOR;
+    }
+
+    // ref: shaders/raytrace.rgen:601-731
+    V3<R> nextEventEstimation(const rtx_material &m, V3<R> origin, V3<R> wi, V3<R> normal, bool frontFace) {
+        if (!neeCapable(m) || S.lights.empty()) return V3<R>(0, 0, 0);
+        V3<R> nee(0, 0, 0);
+        V3<R> lightDir, lightColor;
+        R lightDistance;
+        R pdfLights = sampleLights(origin, normal, lightDir, lightColor, lightDistance);
+        bool isShadowed = true;
+        R cosThetaLight = dot(normal, lightDir);
+        if (cosThetaLight > R(0) && pdfLights > R(0))
+            isShadowed = occluded<R>(S, origin, lightDir, C::TMIN, lightDistance * (R(1) - R(0.0001)), false);
+        if (!isShadowed) {
+            if (pc.enableMIS) {
+                R pdfMat = pdfBSDF(m, normal, wi, lightDir);
+                R h = heuristic(pdfLights, pdfMat);
+                if (std::isnan(h)) return nee;
+                nee = nee + evalBsdf(m, normal, wi, lightDir, true) * lightColor * h / pdfLights;
+            } else {
+                nee = evalBsdf(m, normal, wi, lightDir, true) * lightColor / pdfLights;
+            }
+        }
+        if (pc.enableMIS) {
+            V3<R> bsdfDir;
+            R pdfMat = sampleBSDF(m, wi, normal, frontFace, bsdfDir);
+            if (pdfMat > R(0)) {
+                HitInfo<R> pinfo = trace(origin, bsdfDir, C::TMIN, C::TMAX);
+                if (pinfo.isMiss) {
+                    lightColor = pinfo.missColor;
+                    pdfLights = C::INV_2PI / (R)S.lights.size();
+                    R h = heuristic(pdfMat, pdfLights);
+                    nee = nee + evalBsdf(m, normal, wi, bsdfDir, true) * lightColor * h / pdfMat;
+                } else {
+                    const rtx_material &ms = S.mats[pinfo.matIndex];
+                    if (ms.type == RTX_MAT_LIGHT) {
+                        int iLight = pinfo.isSphere ? S.spheres[pinfo.instanceIndex].iLight
+                                                    : S.inst[pinfo.instanceIndex].iLight;
+                        const rtx_light &light = S.lights[iLight];
+                        lightColor = f3<R>(ms.lightColor);
+                        // pdfLight, ref: raytrace.rgen:334-339
+                        R cosL = dot(-bsdfDir, pinfo.normal);
+                        pdfLights = (R)light.sampleProb * pinfo.t * pinfo.t / cosL / (R)light.area;
+                        R h = heuristic(pdfMat, pdfLights);
+                        if (std::isnan(h)) return nee;
+                        nee = nee + evalBsdf(m, normal, wi, bsdfDir, true) * lightColor * h / pdfMat;
+                    }
+                }
+            }
+        }
+        return nee;
+    }
+
+    // ref: shaders/raytrace.rgen:871-877
+    V3<R> multipleNEE(const rtx_material &m, V3<R> origin, V3<R> wi, V3<R> normal, bool ff, int numNEE) {
+        V3<R> r(0, 0, 0);
+        for (int i = 0; i < numNEE; ++i) r = r + nextEventEstimation(m, origin, wi, normal, ff);
+        return r / (R)numNEE;
+    }
+
+    // The bounce loop, ref: shaders/raytrace.rgen:992-1226 for the plain
+    // method (no irradiance cache, ADRRS, splitting or guiding).  Russian
+    // roulette (absent from the reference) is applied when pc.enableRR is set.
+    V3<R> raytrace(V3<R> origin, V3<R> direction) {
+        const int maxDepth = pc.maxDepth, maxFollowDiscrete = pc.maxFollowDiscrete;
+        const bool useNEE = pc.enableNEE != 0;
+        const bool addDirectLights = !useNEE;
+        const int numNEE = pc.numNEE;
+        bool follow = true;
+        int followCount = 0;
+        bool addNextDirectLights = true; // addFirstHitLight
+        int depth = 0;
+        V3<R> throughput(1, 1, 1), result(0, 0, 0);
+        do {
+            depth++;
+            HitInfo<R> info = trace(origin, direction, C::TMIN, C::TMAX);
+            primHash = (primHash ^ (uint32_t)info.prim) * 16777619u;
+            if (info.isMiss) {
+                if (addDirectLights || addNextDirectLights) result = result + throughput * info.missColor;
+                break;
+            }
+            const rtx_material &m = S.mats[info.matIndex];
+            origin = info.worldPos;
+            V3<R> normal = info.normal;
+            if (m.type == RTX_MAT_LIGHT && (addDirectLights || addNextDirectLights))
+                result = result + throughput * f3<R>(m.lightColor);
+            if (hasDiscreteDirection(m)) {
+                addNextDirectLights = true;
+                follow = true;
+                if (depth >= maxDepth) followCount++;
+            } else {
+                addNextDirectLights = false;
+                follow = false;
+                if (useNEE) {
+                    V3<R> nee = multipleNEE(m, origin, -direction, normal, info.isFrontFace, numNEE);
+                    result = result + throughput * nee;
+                }
+            }
+            V3<R> newDirection;
+            R pdf = sampleBSDF(m, -direction, normal, info.isFrontFace, newDirection);
+            if (pdf <= R(0)) break;
+            V3<R> change = evalBsdf(m, normal, -direction, newDirection, info.isFrontFace) / pdf;
+            throughput = throughput * change;
+            direction = newDirection;
+            if (pc.enableRR && depth >= fp.rrStartDepth) {
+                R q = std::max(std::fabs(throughput.x), std::max(std::fabs(throughput.y), std::fabs(throughput.z)));
+                q = std::min(q, R(0.95));
+                if (rnd() >= q) break;
+                throughput = throughput / q;
+            }
+        } while (depth <= maxDepth || (follow && followCount <= maxFollowDiscrete));
+        return result;
+    }
+
+    // getCameraRay, ref: shaders/raytrace.rgen:1487-1494
+    void cameraRay(uint32_t px, uint32_t py, uint32_t w, uint32_t h, V3<R> &o, V3<R> &d) {
+        const rtx_camera &cam = fp.cam;
+        R jx = negPos();
+        R jy = negPos();
+        R ux = ((R)px + R(0.5) + jx / R(2)) / (R)w;
+        R uy = ((R)py + R(0.5) + jy / R(2)) / (R)h;
+        R dx = ux * R(2) - R(1), dy = uy * R(2) - R(1);
+        o = xform<R>(cam.viewInverse, V3<R>(0, 0, 0), R(1));
+        // target = projInverse * (d.x, d.y, 1, 1); only xyz is used
+        V3<R> target = xform<R>(cam.projInverse, V3<R>(dx, dy, R(1)), R(1));
+        d = normalize(xform<R>(cam.viewInverse, normalize(target), R(0)));
+    }
+
+    // main(), ref: shaders/raytrace.rgen:1623-1711 for one 1-spp frame.
+    V3<R> renderPath(uint64_t pathId) {
+        const uint64_t npix = (uint64_t)fp.width * fp.height;
+        uint32_t pixel = (uint32_t)(pathId % npix);
+        uint64_t sample = pathId / npix + fp.sampleOffset;
+        uint32_t frameSeed = pcg32First(fp.seed, sample);
+        seed = tea(pixel, frameSeed);
+        primHash = 2166136261u;
+        V3<R> o, d;
+        cameraRay(pixel % fp.width, pixel / fp.width, fp.width, fp.height, o, d);
+        return raytrace(o, d);
+    }
+};
+
+int checkScene(const orc_scene &s) {
+    for (const auto &l : s.lights)
+        if (l.type == RTX_LIGHT_ENV_MAP) {
+            g_err = "environment-map lights are not supported by the oracle yet";
+            return RTX_E_UNSUPPORTED;
+        }
+    return RTX_OK;
+}
+
+int checkParams(const rtx_frame_params *fp) {
+    const rtx_push_constant &pc = fp->pc;
+    if (fp->width == 0 || fp->height == 0) {
+        g_err = "empty image";
+        return RTX_E_INVALID;
+    }
+    if (pc.useIrradianceCache || pc.useADRRS || pc.splitOnFirst || pc.useGuiding || pc.updateGuiding ||
+        pc.visualizeMode != 0 || pc.showIrradianceCacheOnly || pc.storeEstimate) {
+        g_err = "oracle: only the plain method is restated so far";
+        return RTX_E_UNSUPPORTED;
+    }
+    if (pc.enableNEE && pc.numNEE < 1) {
+        g_err = "numNEE must be >= 1";
+        return RTX_E_INVALID;
+    }
+    return RTX_OK;
+}
+
+template <class F> void parallelFor(size_t n, int threads, F fn) {
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<size_t> next{0};
+    const size_t chunk = 64;
+    auto worker = [&]() {
+        for (;;) {
+            size_t b = next.fetch_add(chunk);
+            if (b >= n) break;
+            size_t e = std::min(n, b + chunk);
+            for (size_t i = b; i < e; ++i) fn(i);
+        }
+    };
+    if (threads == 1) {
+        worker();
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+}
+
+template <class R> V3<R> vd(const double *p) { return V3<R>((R)p[0], (R)p[1], (R)p[2]); }
+
+} // namespace
+
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+int orc_scene_create(const rtx_scene_desc *d, orc_scene **out) {
+    if (!d || !out) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    auto s = std::make_unique<orc_scene>();
+    for (uint32_t m = 0; m < d->modelCount; ++m) {
+        const rtx_model_desc &md = d->models[m];
+        s->verts.emplace_back(md.vertices, md.vertices + md.vertexCount);
+        s->idx.emplace_back(md.indices, md.indices + md.indexCount);
+    }
+    s->inst.assign(d->instances, d->instances + d->instanceCount);
+    s->mats.assign(d->materials, d->materials + d->materialCount);
+    s->lights.assign(d->lights, d->lights + d->lightCount);
+    if (d->randomLightIndex) s->lightIndex.assign(d->randomLightIndex, d->randomLightIndex + RTX_SIZE_LIGHT_RANDOM);
+    if (d->randomTriIndex)
+        s->triTable.assign(d->randomTriIndex, d->randomTriIndex + (size_t)d->randomTriRows * RTX_SIZE_TRI_RANDOM);
+    s->triRows = d->randomTriRows;
+    s->spheres.assign(d->spheres, d->spheres + d->sphereCount);
+    int rc = checkScene(*s);
+    if (rc != RTX_OK) return rc;
+    // flatten instances to world-space triangles (global primitive order:
+    // instances in order, each model's triangles in order; spheres after)
+    for (uint32_t i = 0; i < d->instanceCount; ++i) {
+        const rtx_instance &in = s->inst[i];
+        const auto &vv = s->verts[in.modelIndex];
+        const auto &ii = s->idx[in.modelIndex];
+        for (size_t f = 0; f * 3 + 2 < ii.size(); ++f) {
+            for (int k = 0; k < 3; ++k) {
+                const rtx_vertex &v = vv[ii[3 * f + k]];
+                V3<double> p = xform<double>(in.transform, f3<double>(v.pos), 1.0);
+                V3<float> pf = xform<float>(in.transform, f3<float>(v.pos), 1.0f);
+                s->tri.push_back(p.x);
+                s->tri.push_back(p.y);
+                s->tri.push_back(p.z);
+                s->triF.push_back(pf.x);
+                s->triF.push_back(pf.y);
+                s->triF.push_back(pf.z);
+            }
+            s->triInst.push_back((int)i);
+            s->triLocal.push_back((int)f);
+        }
+    }
+    s->ntri = s->triInst.size();
+    size_t nprim = s->ntri + s->spheres.size();
+    if (nprim > 0) {
+        std::vector<std::array<double, 6>> box(nprim);
+        for (size_t p = 0; p < nprim; ++p) {
+            double lo[3], hi[3];
+            triBounds(*s, (int)p, lo, hi);
+            box[p] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+        }
+        s->primOrder.resize(nprim);
+        for (size_t p = 0; p < nprim; ++p) s->primOrder[p] = (int)p;
+        buildNode(*s, box, 0, (int)nprim);
+    }
+    *out = s.release();
+    return RTX_OK;
+}
+
+void orc_scene_destroy(orc_scene *s) { delete s; }
+
+int orc_render_paths(const orc_scene *s, const rtx_frame_params *fp, const uint64_t *ids, size_t n, int prec,
+                     int threads, double *rgb, uint32_t *sig) {
+    if (!s || !fp || (!ids && n) || (!rgb && n)) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    int rc = checkParams(fp);
+    if (rc != RTX_OK) return rc;
+    if (prec != 32 && prec != 64) {
+        g_err = "prec must be 32 or 64";
+        return RTX_E_INVALID;
+    }
+    parallelFor(n, threads, [&](size_t i) {
+        double r[3];
+        uint32_t st, hs;
+        if (prec == 64) {
+            Tracer<double> t(*s, *fp);
+            V3<double> v = t.renderPath(ids[i]);
+            toD(v, r);
+            st = t.seed;
+            hs = t.primHash;
+        } else {
+            Tracer<float> t(*s, *fp);
+            V3<float> v = t.renderPath(ids[i]);
+            toD(v, r);
+            st = t.seed;
+            hs = t.primHash;
+        }
+        rgb[3 * i + 0] = r[0];
+        rgb[3 * i + 1] = r[1];
+        rgb[3 * i + 2] = r[2];
+        if (sig) {
+            sig[2 * i + 0] = st;
+            sig[2 * i + 1] = hs;
+        }
+    });
+    return RTX_OK;
+}
+
+int orc_trace_closest(const orc_scene *s, const rtx_ray *rays, size_t n, int brute, double *t, int32_t *prim,
+                      double *u, double *v) {
+    if (!s || (!rays && n)) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    parallelFor(n, 0, [&](size_t i) {
+        const rtx_ray &r = rays[i];
+        HitRec h;
+        bool hit = closest<double>(*s, f3<double>(r.org), f3<double>(r.dir), (double)r.tmin, (double)r.tmax,
+                                   brute != 0, h);
+        if (t) t[i] = hit ? h.t : (double)r.tmax;
+        if (prim) prim[i] = hit ? h.prim : -1;
+        if (u) u[i] = hit ? h.u : 0.0;
+        if (v) v[i] = hit ? h.v : 0.0;
+    });
+    return RTX_OK;
+}
+
+int orc_trace_occluded(const orc_scene *s, const rtx_ray *rays, size_t n, int brute, uint8_t *occ) {
+    if (!s || (!rays && n) || (!occ && n)) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    parallelFor(n, 0, [&](size_t i) {
+        const rtx_ray &r = rays[i];
+        occ[i] = occluded<double>(*s, f3<double>(r.org), f3<double>(r.dir), (double)r.tmin, (double)r.tmax,
+                                  brute != 0)
+                     ? 1
+                     : 0;
+    });
+    return RTX_OK;
+}
+
+uint32_t orc_tea(uint32_t v0, uint32_t v1) { return tea(v0, v1); }
+uint32_t orc_lcg_step(uint32_t s) { return 1664525u * s + 1013904223u; }
+uint32_t orc_pcg32_first(uint64_t a, uint64_t b) { return pcg32First(a, b); }
+
+namespace {
+const orc_scene &emptyScene() {
+    static orc_scene s;
+    return s;
+}
+rtx_frame_params defaultParams() {
+    rtx_frame_params fp;
+    std::memset(&fp, 0, sizeof(fp));
+    fp.pc.usePowerHeuristic = 1; // src/RayTracingApp.h:92
+    return fp;
+}
+} // namespace
+
+void orc_camera_ray(const rtx_camera *cam, uint32_t w, uint32_t h, uint32_t px, uint32_t py, uint32_t *state,
+                    double org[3], double dir[3]) {
+    rtx_frame_params fp = defaultParams();
+    fp.cam = *cam;
+    fp.width = w;
+    fp.height = h;
+    Tracer<double> t(emptyScene(), fp);
+    t.seed = *state;
+    V3<double> o, d;
+    t.cameraRay(px, py, w, h, o, d);
+    *state = t.seed;
+    toD(o, org);
+    toD(d, dir);
+}
+
+double orc_fresnel(double eta, double c) { return Tracer<double>::fresnel(eta, c); }
+double orc_fresnel_conductor(double c, double eta, double k) { return Tracer<double>::fresnelConductor(c, eta, k); }
+double orc_beckmann_d(const rtx_material *m, const double n[3], const double h[3]) {
+    return Tracer<double>::D(*m, vd<double>(n), vd<double>(h));
+}
+double orc_pdf_bsdf(const rtx_material *m, const double n[3], const double wi[3], const double wo[3]) {
+    return Tracer<double>::pdfBSDF(*m, vd<double>(n), vd<double>(wi), vd<double>(wo));
+}
+void orc_eval_bsdf(const rtx_material *m, const double n[3], const double wi[3], const double wo[3], int ff,
+                   double out[3]) {
+    toD(Tracer<double>::evalBsdf(*m, vd<double>(n), vd<double>(wi), vd<double>(wo), ff != 0), out);
+}
+double orc_sample_bsdf(const rtx_material *m, const double wi[3], const double n[3], int ff, uint32_t *state,
+                       double wo[3]) {
+    rtx_frame_params fp = defaultParams();
+    Tracer<double> t(emptyScene(), fp);
+    t.seed = *state;
+    V3<double> d(0, 0, 0);
+    double pdf = t.sampleBSDF(*m, vd<double>(wi), vd<double>(n), ff != 0, d);
+    *state = t.seed;
+    toD(d, wo);
+    return pdf;
+}
+void orc_sample_cosine(const double n[3], uint32_t *state, double out[3]) {
+    rtx_frame_params fp = defaultParams();
+    Tracer<double> t(emptyScene(), fp);
+    t.seed = *state;
+    toD(t.randomInHemisphereCosine(vd<double>(n)), out);
+    *state = t.seed;
+}
+void orc_sample_cosine_power(const double r[3], double p, uint32_t *state, double out[3]) {
+    rtx_frame_params fp = defaultParams();
+    Tracer<double> t(emptyScene(), fp);
+    t.seed = *state;
+    toD(t.randomInHemisphereCosinePower(vd<double>(r), p), out);
+    *state = t.seed;
+}
+void orc_sample_beckmann_normal(const rtx_material *m, const double n[3], uint32_t *state, double out[3]) {
+    rtx_frame_params fp = defaultParams();
+    Tracer<double> t(emptyScene(), fp);
+    t.seed = *state;
+    toD(t.randomBeckmannNormal(*m, vd<double>(n)), out);
+    *state = t.seed;
+}
+void orc_sample_unit_sphere(uint32_t *state, double out[3]) {
+    rtx_frame_params fp = defaultParams();
+    Tracer<double> t(emptyScene(), fp);
+    t.seed = *state;
+    toD(t.randomOnUnitSphere(), out);
+    *state = t.seed;
+}
+double orc_sample_lights(const orc_scene *s, const double origin[3], const double normal[3], uint32_t *state,
+                         double lightDir[3], double lightColor[3], double *lightDistance) {
+    rtx_frame_params fp = defaultParams();
+    Tracer<double> t(*s, fp);
+    t.seed = *state;
+    V3<double> ld, lc;
+    double dist = 0;
+    double pdf = t.sampleLights(vd<double>(origin), vd<double>(normal), ld, lc, dist);
+    *state = t.seed;
+    toD(ld, lightDir);
+    toD(lc, lightColor);
+    *lightDistance = dist;
+    return pdf;
+}
+
+} // extern "C"

@@ -1,0 +1,665 @@
+// C ABI of librtxamd.so (include/rtx_amd.h): context, scene upload + BVH
+// build, wavefront render orchestration, readback and ray queries.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+using namespace rtx;
+
+struct rtx_scene {
+    DevScene view{};
+    std::vector<void *> allocs;
+    rtx_scene_info info{};
+    int device = 0;
+};
+
+namespace {
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    int cls = 0;
+};
+
+enum { kClsExtend = 0, kClsShadow = 1, kClsProbe = 2, kClsShade = 3 };
+
+} // namespace
+
+struct rtx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // accumulation image
+    float4 *accum = nullptr;
+    float4 *resolved = nullptr;
+    uint32_t accW = 0, accH = 0, accSamples = 0;
+    // wavefront buffers
+    size_t cap = 0;
+    uint32_t capNEE = 0;
+    PathBuffers pb{};
+    Queue q[2]{}, qs{}, qp{};
+    int32_t *done = nullptr;
+    Counters *cnt = nullptr;
+    FrameConst *fc = nullptr;
+    std::vector<void *> batchAllocs;
+    // per-path results of the last render
+    float4 *keepRgb = nullptr;
+    uint2 *keepSig = nullptr;
+    size_t keepCap = 0, keptPaths = 0;
+    // timing
+    std::vector<EventPair> events;
+    size_t eventsUsed = 0;
+    hipEvent_t tStart = nullptr, tStop = nullptr;
+    int cuCount = 256;
+};
+
+namespace {
+
+#define CHECK_HIP(ctx, expr)                                                                              \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) {                                                                           \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                              \
+            return RTX_E_DEVICE;                                                                          \
+        }                                                                                                 \
+    } while (0)
+
+int fail(rtx_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+template <class T> hipError_t devAlloc(T **p, size_t count, std::vector<void *> &list) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) {
+        list.push_back(q);
+        *p = (T *)q;
+    }
+    return e;
+}
+
+void freeAll(std::vector<void *> &list) {
+    for (void *p : list) (void)hipFree(p);
+    list.clear();
+}
+
+// mat4 * (v, 1) in FP32 with GLM's summation order
+void xformPoint(const float *m, const float *v, float *o) {
+    for (int r = 0; r < 3; ++r) {
+        float a0 = m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1];
+        float a1 = m[2 * 4 + r] * v[2] + m[3 * 4 + r] * 1.0f;
+        o[r] = a0 + a1;
+    }
+}
+
+int validateScene(rtx_ctx *ctx, const rtx_scene_desc *d, uint64_t &numTris) {
+    if (!d) return fail(ctx, RTX_E_INVALID, "scene desc is NULL");
+    if (d->modelCount && !d->models) return fail(ctx, RTX_E_INVALID, "models is NULL");
+    if (d->instanceCount && !d->instances) return fail(ctx, RTX_E_INVALID, "instances is NULL");
+    if (d->materialCount == 0 || !d->materials) return fail(ctx, RTX_E_INVALID, "a scene needs materials");
+    if (d->lightCount && (!d->lights || !d->randomLightIndex))
+        return fail(ctx, RTX_E_INVALID, "lights without light tables");
+    if (d->sphereCount && !d->spheres) return fail(ctx, RTX_E_INVALID, "spheres is NULL");
+    if (d->envMap) return fail(ctx, RTX_E_UNSUPPORTED, "environment maps are not supported yet");
+    for (uint32_t m = 0; m < d->modelCount; ++m) {
+        const rtx_model_desc &md = d->models[m];
+        if (md.indexCount % 3) return fail(ctx, RTX_E_INVALID, "model index count not a multiple of 3");
+        if ((md.indexCount && !md.indices) || (md.vertexCount && !md.vertices))
+            return fail(ctx, RTX_E_INVALID, "model buffers are NULL");
+        for (uint32_t i = 0; i < md.indexCount; ++i)
+            if (md.indices[i] >= md.vertexCount) return fail(ctx, RTX_E_INVALID, "vertex index out of range");
+        for (uint32_t v = 0; v < md.vertexCount; ++v)
+            if (md.vertices[v].materialIndex < 0 || (uint32_t)md.vertices[v].materialIndex >= d->materialCount)
+                return fail(ctx, RTX_E_INVALID, "vertex material index out of range");
+    }
+    numTris = 0;
+    for (uint32_t i = 0; i < d->instanceCount; ++i) {
+        const rtx_instance &in = d->instances[i];
+        if (in.modelIndex < 0 || (uint32_t)in.modelIndex >= d->modelCount)
+            return fail(ctx, RTX_E_INVALID, "instance model index out of range");
+        if (in.iLight >= (int)d->lightCount) return fail(ctx, RTX_E_INVALID, "instance light index out of range");
+        numTris += d->models[in.modelIndex].indexCount / 3;
+    }
+    if (numTris >= (1ull << 27)) return fail(ctx, RTX_E_UNSUPPORTED, "more than 2^27 triangles");
+    for (uint32_t m = 0; m < d->materialCount; ++m)
+        if (d->materials[m].type < 0 || d->materials[m].type > RTX_MAT_ROUGH_CONDUCTOR)
+            return fail(ctx, RTX_E_INVALID, "unknown material type");
+    for (uint32_t s = 0; s < d->sphereCount; ++s) {
+        const rtx_sphere &sp = d->spheres[s];
+        if (sp.materialIndex < 0 || (uint32_t)sp.materialIndex >= d->materialCount)
+            return fail(ctx, RTX_E_INVALID, "sphere material index out of range");
+        if (sp.iLight >= (int)d->lightCount) return fail(ctx, RTX_E_INVALID, "sphere light index out of range");
+    }
+    for (uint32_t l = 0; l < d->lightCount; ++l) {
+        const rtx_light &li = d->lights[l];
+        switch (li.type) {
+        case RTX_LIGHT_POINT: break;
+        case RTX_LIGHT_SPHERE:
+            if (li.instanceIndex >= d->sphereCount) return fail(ctx, RTX_E_INVALID, "sphere light out of range");
+            break;
+        case RTX_LIGHT_AREA: {
+            if (li.instanceIndex >= d->instanceCount) return fail(ctx, RTX_E_INVALID, "area light instance out of range");
+            // The reference indexes randomTriIndex by the global light index
+            // (shaders/raytrace.rgen:427); rows exist only for area lights.
+            if (l >= d->randomTriRows || !d->randomTriIndex)
+                return fail(ctx, RTX_E_INVALID, "area light has no face-sampling row (area lights must come first)");
+            const rtx_model_desc &md = d->models[d->instances[li.instanceIndex].modelIndex];
+            for (int k = 0; k < RTX_SIZE_TRI_RANDOM; ++k) {
+                int f = d->randomTriIndex[(size_t)l * RTX_SIZE_TRI_RANDOM + k].index;
+                if (f < 0 || (uint32_t)f >= md.indexCount / 3)
+                    return fail(ctx, RTX_E_INVALID, "face sample out of range");
+            }
+            break;
+        }
+        default:
+            return fail(ctx, RTX_E_UNSUPPORTED, "environment-map lights are not supported yet");
+        }
+    }
+    if (d->lightCount)
+        for (int k = 0; k < RTX_SIZE_LIGHT_RANDOM; ++k)
+            if (d->randomLightIndex[k] < 0 || (uint32_t)d->randomLightIndex[k] >= d->lightCount)
+                return fail(ctx, RTX_E_INVALID, "light table entry out of range");
+    return RTX_OK;
+}
+
+int ensureBatchBuffers(rtx_ctx *ctx, size_t paths, uint32_t numNEE) {
+    if (paths <= ctx->cap && numNEE <= ctx->capNEE) return RTX_OK;
+    freeAll(ctx->batchAllocs);
+    size_t cap = std::max(paths, ctx->cap);
+    uint32_t nee = std::max(numNEE, ctx->capNEE);
+    auto &L = ctx->batchAllocs;
+    PathBuffers &pb = ctx->pb;
+    CHECK_HIP(ctx, devAlloc(&pb.rng, cap, L));
+    CHECK_HIP(ctx, devAlloc(&pb.hash, cap, L));
+    CHECK_HIP(ctx, devAlloc(&pb.thr, cap, L));
+    CHECK_HIP(ctx, devAlloc(&pb.result, cap, L));
+    CHECK_HIP(ctx, devAlloc(&pb.pendShadow, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&pb.pendProbe, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&pb.out, cap, L));
+    CHECK_HIP(ctx, devAlloc(&pb.outSig, cap, L));
+    for (int k = 0; k < 2; ++k) {
+        CHECK_HIP(ctx, devAlloc(&ctx->q[k].id, cap, L));
+        CHECK_HIP(ctx, devAlloc(&ctx->q[k].org, cap, L));
+        CHECK_HIP(ctx, devAlloc(&ctx->q[k].dir, cap, L));
+        CHECK_HIP(ctx, devAlloc(&ctx->q[k].aux, cap, L));
+    }
+    CHECK_HIP(ctx, devAlloc(&ctx->qs.id, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->qs.org, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->qs.dir, cap * nee, L));
+    ctx->qs.aux = nullptr;
+    CHECK_HIP(ctx, devAlloc(&ctx->qp.id, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->qp.org, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->qp.dir, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->qp.aux, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->done, cap, L));
+    ctx->cap = cap;
+    ctx->capNEE = nee;
+    return RTX_OK;
+}
+
+const char *unsupportedFeature(const rtx_push_constant &pc) {
+    if (pc.useIrradianceCache) return "useIrradianceCache";
+    if (pc.useADRRS) return "useADRRS";
+    if (pc.splitOnFirst) return "splitOnFirst";
+    if (pc.useGuiding) return "useGuiding";
+    if (pc.updateGuiding) return "updateGuiding";
+    if (pc.storeEstimate) return "storeEstimate";
+    if (pc.showIrradianceCacheOnly) return "showIrradianceCacheOnly";
+    if (pc.visualizeMode != 0) return "visualizeMode";
+    return nullptr;
+}
+
+EventPair *nextEvents(rtx_ctx *ctx, int cls) {
+    if (ctx->eventsUsed == ctx->events.size()) {
+        EventPair p;
+        if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess) return nullptr;
+        ctx->events.push_back(p);
+    }
+    EventPair *p = &ctx->events[ctx->eventsUsed++];
+    p->cls = cls;
+    return p;
+}
+
+} // namespace
+
+extern "C" {
+
+void rtx_push_constant_defaults(rtx_push_constant *pc) {
+    if (!pc) return;
+    std::memset(pc, 0, sizeof(*pc));
+    // src/RayTracingApp.h:81-130
+    pc->previousFrames = (uint32_t)-1;
+    pc->maxDepth = 30;
+    pc->maxFollowDiscrete = 3;
+    pc->samplesPerPixel = 1;
+    pc->enableNEE = 1;
+    pc->numNEE = 1;
+    pc->usePowerHeuristic = 1;
+    pc->irradianceA = 0.2f;
+    pc->irradianceUpdateProb = 0.00001f;
+    pc->irradianceCreateProb = 0.0005f;
+    pc->irradianceVisualizationScale = 1.0f;
+    pc->irradianceGradientsMaxLength = 5;
+    pc->irradianceNumNEE = 1;
+    pc->irradianceCacheMinRadius = 0.1f;
+    pc->adrrsS = 5;
+    pc->adrrsSplit = 1;
+    pc->guidingProb = 0.5f;
+    pc->guidingVisuScale = 0.5f;
+    pc->guidingVisuMax = 1.0f;
+    pc->useParallaxCompensation = 1;
+    pc->guidingVisuPhiScale = 0.004f;
+    pc->guidingVisuThetaScale = 0.01f;
+    pc->guidingPiPShowSpheres = 1;
+    pc->guidingPiPSize = 0.3f;
+}
+
+const char *rtx_last_error(const rtx_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int rtx_ctx_create(int device, rtx_ctx **out) {
+    if (!out) return RTX_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return RTX_E_DEVICE;
+    if (device < 0 || device >= n) return RTX_E_INVALID;
+    auto *ctx = new (std::nothrow) rtx_ctx();
+    if (!ctx) return RTX_E_MEMORY;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&ctx->cnt, sizeof(Counters)) != hipSuccess || hipMalloc(&ctx->fc, sizeof(FrameConst)) != hipSuccess ||
+        hipEventCreate(&ctx->tStart) != hipSuccess || hipEventCreate(&ctx->tStop) != hipSuccess) {
+        rtx_ctx_destroy(ctx);
+        return RTX_E_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->cuCount = prop.multiProcessorCount;
+    *out = ctx;
+    return RTX_OK;
+}
+
+int rtx_ctx_destroy(rtx_ctx *ctx) {
+    if (!ctx) return RTX_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    freeAll(ctx->batchAllocs);
+    if (ctx->accum) (void)hipFree(ctx->accum);
+    if (ctx->resolved) (void)hipFree(ctx->resolved);
+    if (ctx->keepRgb) (void)hipFree(ctx->keepRgb);
+    if (ctx->keepSig) (void)hipFree(ctx->keepSig);
+    if (ctx->cnt) (void)hipFree(ctx->cnt);
+    if (ctx->fc) (void)hipFree(ctx->fc);
+    for (auto &e : ctx->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (ctx->tStart) (void)hipEventDestroy(ctx->tStart);
+    if (ctx->tStop) (void)hipEventDestroy(ctx->tStop);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RTX_OK;
+}
+
+int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
+    if (!ctx || !out) return RTX_E_INVALID;
+    *out = nullptr;
+    uint64_t numTris = 0;
+    int rc = validateScene(ctx, d, numTris);
+    if (rc != RTX_OK) return rc;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+
+    // flatten instances into world-space triangles (global primitive order)
+    std::vector<float> tv((size_t)numTris * 9);
+    std::vector<int2> primRef((size_t)numTris);
+    size_t t = 0;
+    for (uint32_t i = 0; i < d->instanceCount; ++i) {
+        const rtx_instance &in = d->instances[i];
+        const rtx_model_desc &md = d->models[in.modelIndex];
+        for (uint32_t f = 0; f < md.indexCount / 3; ++f, ++t) {
+            for (int k = 0; k < 3; ++k)
+                xformPoint(in.transform, md.vertices[md.indices[3 * f + k]].pos, &tv[9 * t + 3 * k]);
+            primRef[t] = make_int2((int)i, (int)f);
+        }
+    }
+    BvhBuildResult bvh;
+    try {
+        buildBvh(tv.data(), (uint32_t)numTris, bvh);
+    } catch (const std::exception &e) {
+        return fail(ctx, RTX_E_INVALID, std::string("BVH build failed: ") + e.what());
+    }
+    std::vector<float4> tris((size_t)numTris * 3);
+    for (size_t k = 0; k < bvh.primOrder.size(); ++k) {
+        uint32_t p = bvh.primOrder[k];
+        const float *v = &tv[9 * (size_t)p];
+        int pid = (int)p;
+        float pidBits;
+        std::memcpy(&pidBits, &pid, 4);
+        tris[3 * k + 0] = make_float4(v[0], v[1], v[2], pidBits);
+        tris[3 * k + 1] = make_float4(v[3], v[4], v[5], 0.0f);
+        tris[3 * k + 2] = make_float4(v[6], v[7], v[8], 0.0f);
+    }
+    std::vector<rtx_vertex> verts;
+    std::vector<uint32_t> idx, vbase(std::max<uint32_t>(d->modelCount, 1)), ibase(std::max<uint32_t>(d->modelCount, 1));
+    for (uint32_t m = 0; m < d->modelCount; ++m) {
+        vbase[m] = (uint32_t)verts.size();
+        ibase[m] = (uint32_t)idx.size();
+        verts.insert(verts.end(), d->models[m].vertices, d->models[m].vertices + d->models[m].vertexCount);
+        idx.insert(idx.end(), d->models[m].indices, d->models[m].indices + d->models[m].indexCount);
+    }
+
+    auto *sc = new (std::nothrow) rtx_scene();
+    if (!sc) return fail(ctx, RTX_E_MEMORY, "out of memory");
+    sc->device = ctx->device;
+    auto upload = [&](auto **dst, const auto *src, size_t count) -> hipError_t {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+        T *p = nullptr;
+        hipError_t e = devAlloc(&p, count, sc->allocs);
+        if (e != hipSuccess) return e;
+        if (count) e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+        *dst = p;
+        return e;
+    };
+    DevScene &S = sc->view;
+    hipError_t e = hipSuccess;
+    float4 *nodes = nullptr, *trisD = nullptr;
+    int2 *prD = nullptr;
+    rtx_vertex *vD = nullptr;
+    uint32_t *iD = nullptr, *vbD = nullptr, *ibD = nullptr;
+    rtx_instance *instD = nullptr;
+    rtx_material *matD = nullptr;
+    rtx_light *lightD = nullptr;
+    int32_t *liD = nullptr;
+    rtx_face_sample *ftD = nullptr;
+    rtx_sphere *spD = nullptr;
+    if (e == hipSuccess) e = upload(&nodes, reinterpret_cast<const float4 *>(bvh.nodes.data()), bvh.nodes.size() * 4);
+    if (e == hipSuccess) e = upload(&trisD, tris.data(), tris.size());
+    if (e == hipSuccess) e = upload(&prD, primRef.data(), primRef.size());
+    if (e == hipSuccess) e = upload(&vD, verts.data(), verts.size());
+    if (e == hipSuccess) e = upload(&iD, idx.data(), idx.size());
+    if (e == hipSuccess) e = upload(&vbD, vbase.data(), vbase.size());
+    if (e == hipSuccess) e = upload(&ibD, ibase.data(), ibase.size());
+    if (e == hipSuccess) e = upload(&instD, d->instances, d->instanceCount);
+    if (e == hipSuccess) e = upload(&matD, d->materials, d->materialCount);
+    if (e == hipSuccess) e = upload(&lightD, d->lights, d->lightCount);
+    if (e == hipSuccess) e = upload(&liD, d->randomLightIndex, d->lightCount ? RTX_SIZE_LIGHT_RANDOM : 0);
+    if (e == hipSuccess)
+        e = upload(&ftD, d->randomTriIndex, d->randomTriIndex ? (size_t)d->randomTriRows * RTX_SIZE_TRI_RANDOM : 0);
+    if (e == hipSuccess) e = upload(&spD, d->spheres, d->sphereCount);
+    if (e != hipSuccess) {
+        freeAll(sc->allocs);
+        delete sc;
+        return fail(ctx, RTX_E_DEVICE, std::string("scene upload: ") + hipGetErrorString(e));
+    }
+    S.nodes = nodes;
+    S.tris = trisD;
+    S.primRef = prD;
+    S.vertices = vD;
+    S.indices = iD;
+    S.vertexBase = vbD;
+    S.indexBase = ibD;
+    S.instances = instD;
+    S.materials = matD;
+    S.lights = lightD;
+    S.lightIndex = liD;
+    S.triTable = ftD;
+    S.spheres = spD;
+    S.numTris = (uint32_t)numTris;
+    S.numSpheres = d->sphereCount;
+    S.lightCount = d->lightCount;
+    S.triRows = d->randomTriRows;
+    S.numNodes = (uint32_t)bvh.nodes.size();
+    sc->info.triangles = (uint32_t)numTris;
+    sc->info.spheres = d->sphereCount;
+    sc->info.nodes = (uint32_t)bvh.nodes.size();
+    sc->info.maxDepth = bvh.maxDepth;
+    sc->info.deviceBytes = bvh.nodes.size() * sizeof(BvhNode) + tris.size() * sizeof(float4) +
+                           primRef.size() * sizeof(int2) + verts.size() * sizeof(rtx_vertex) +
+                           idx.size() * 4 + (size_t)d->materialCount * sizeof(rtx_material);
+    sc->info.buildMs = bvh.buildMs;
+    *out = sc;
+    return RTX_OK;
+}
+
+int rtx_scene_destroy(rtx_ctx *ctx, rtx_scene *scene) {
+    if (!scene) return RTX_E_INVALID;
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    freeAll(scene->allocs);
+    delete scene;
+    return RTX_OK;
+}
+
+int rtx_scene_get_info(const rtx_scene *scene, rtx_scene_info *info) {
+    if (!scene || !info) return RTX_E_INVALID;
+    *info = scene->info;
+    return RTX_OK;
+}
+
+int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, rtx_frame_stats *stats) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!scene || !p) return fail(ctx, RTX_E_INVALID, "NULL scene or params");
+    if (p->width == 0 || p->height == 0 || p->spp == 0) return fail(ctx, RTX_E_INVALID, "empty render");
+    if (const char *f = unsupportedFeature(p->pc))
+        return fail(ctx, RTX_E_UNSUPPORTED, std::string("feature not supported yet: ") + f);
+    if (p->pc.maxDepth < 0 || p->pc.maxDepth > 200 || p->pc.maxFollowDiscrete < 0 || p->pc.maxFollowDiscrete > 50)
+        return fail(ctx, RTX_E_INVALID, "maxDepth must be in [0, 200] and maxFollowDiscrete in [0, 50]");
+    uint32_t numNEE = p->pc.enableNEE ? (uint32_t)p->pc.numNEE : 1u;
+    if (p->pc.enableNEE && (p->pc.numNEE < 1 || p->pc.numNEE > 8))
+        return fail(ctx, RTX_E_INVALID, "numNEE must be in [1, 8]");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    const uint64_t npix = (uint64_t)p->width * p->height;
+    const uint64_t total = npix * p->spp;
+    if (npix > (1ull << 31)) return fail(ctx, RTX_E_INVALID, "image too large");
+
+    // accumulation image
+    bool reset = (p->flags & RTX_FRAME_RESET_ACCUM) || p->width != ctx->accW || p->height != ctx->accH;
+    if (p->width != ctx->accW || p->height != ctx->accH || !ctx->accum) {
+        if (ctx->accum) (void)hipFree(ctx->accum);
+        if (ctx->resolved) (void)hipFree(ctx->resolved);
+        ctx->accum = ctx->resolved = nullptr;
+        CHECK_HIP(ctx, hipMalloc(&ctx->accum, npix * sizeof(float4)));
+        CHECK_HIP(ctx, hipMalloc(&ctx->resolved, npix * sizeof(float4)));
+        ctx->accW = p->width;
+        ctx->accH = p->height;
+    }
+    if (reset) {
+        CHECK_HIP(ctx, hipMemsetAsync(ctx->accum, 0, npix * sizeof(float4), ctx->stream));
+        ctx->accSamples = 0;
+    }
+
+    // batch size: whole sample planes when they fit, else pixel ranges
+    uint64_t batch = p->batchPaths ? p->batchPaths : (1ull << 22);
+    batch = std::min<uint64_t>(batch, total);
+    if (batch >= npix) batch -= batch % npix;
+    int rc = ensureBatchBuffers(ctx, (size_t)batch, numNEE);
+    if (rc != RTX_OK) return rc;
+
+    const bool keep = (p->flags & RTX_FRAME_KEEP_PATHS) != 0;
+    if (keep) {
+        if (total > ctx->keepCap) {
+            if (ctx->keepRgb) (void)hipFree(ctx->keepRgb);
+            if (ctx->keepSig) (void)hipFree(ctx->keepSig);
+            ctx->keepRgb = nullptr;
+            ctx->keepSig = nullptr;
+            ctx->keepCap = 0;
+            CHECK_HIP(ctx, hipMalloc(&ctx->keepRgb, total * sizeof(float4)));
+            CHECK_HIP(ctx, hipMalloc(&ctx->keepSig, total * sizeof(uint2)));
+            ctx->keepCap = total;
+        }
+        ctx->keptPaths = total;
+    }
+
+    FrameConst fc;
+    std::memset(&fc, 0, sizeof(fc));
+    fc.pc = p->pc;
+    fc.cam = p->cam;
+    fc.width = p->width;
+    fc.height = p->height;
+    fc.seed = p->seed;
+    fc.sampleOffset = p->sampleOffset;
+    fc.rrStartDepth = p->rrStartDepth;
+    fc.numNEE = numNEE;
+    fc.countVisits = (p->flags & RTX_FRAME_COUNT_VISITS) ? 1 : 0;
+    CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));
+    CHECK_HIP(ctx, hipMemsetAsync(ctx->cnt, 0, sizeof(Counters), ctx->stream));
+
+    const bool countVisits = fc.countVisits != 0;
+    const bool timeKernels = (p->flags & RTX_FRAME_TIME_KERNELS) != 0;
+    ctx->eventsUsed = 0;
+    const int traceBlocks = ctx->cuCount * 16;
+    const int shadeBlocks = ctx->cuCount * 8;
+    const int maxIter = p->pc.maxDepth + p->pc.maxFollowDiscrete + 3;
+    const DevScene &S = scene->view;
+    hipStream_t s = ctx->stream;
+
+    auto timed = [&](int cls, auto &&launch) {
+        EventPair *ev = timeKernels ? nextEvents(ctx, cls) : nullptr;
+        if (ev) (void)hipEventRecord(ev->a, s);
+        launch();
+        if (ev) (void)hipEventRecord(ev->b, s);
+    };
+
+    CHECK_HIP(ctx, hipEventRecord(ctx->tStart, s));
+    uint32_t batches = 0;
+    for (uint64_t first = 0; first < total; first += batch) {
+        BatchDesc b;
+        b.firstPath = first;
+        b.numPaths = (uint32_t)std::min<uint64_t>(batch, total - first);
+        b.npix = (uint32_t)npix;
+        int qsel = 0;
+        timed(kClsShade, [&] { launchGenerate(s, ctx->fc, b, ctx->pb, ctx->q[qsel], ctx->cnt, qsel); });
+        for (int it = 0; it < maxIter; ++it) {
+            timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
+            timed(kClsExtend, [&] { launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, countVisits, traceBlocks); });
+            timed(kClsShade, [&] {
+                launchShade(s, S, ctx->fc, b, ctx->pb, ctx->q[qsel], ctx->q[qsel ^ 1], ctx->qs, ctx->qp, ctx->done,
+                            ctx->cnt, qsel, shadeBlocks);
+            });
+            timed(kClsShadow, [&] { launchShadow(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
+            timed(kClsProbe,
+                  [&] { launchProbe(s, S, ctx->fc, ctx->qp, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
+            timed(kClsShade, [&] { launchFinish(s, ctx->fc, ctx->pb, ctx->done, ctx->cnt, shadeBlocks); });
+            qsel ^= 1;
+        }
+        timed(kClsShade, [&] {
+            launchAccumulate(s, b, ctx->pb, ctx->accum, ctx->keepRgb, ctx->keepSig, keep);
+        });
+        CHECK_HIP(ctx, hipGetLastError());
+        batches++;
+    }
+    CHECK_HIP(ctx, hipEventRecord(ctx->tStop, s));
+    CHECK_HIP(ctx, hipStreamSynchronize(s));
+    ctx->accSamples += p->spp;
+
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        Counters c;
+        CHECK_HIP(ctx, hipMemcpy(&c, ctx->cnt, sizeof(c), hipMemcpyDeviceToHost));
+        stats->paths = total;
+        stats->extendRays = c.extendRays;
+        stats->shadowRays = c.shadowRays;
+        stats->probeRays = c.probeRays;
+        stats->nodeVisits = c.nodeVisits;
+        stats->primVisits = c.primVisits;
+        float ms = 0;
+        CHECK_HIP(ctx, hipEventElapsedTime(&ms, ctx->tStart, ctx->tStop));
+        stats->msTotal = ms;
+        for (size_t k = 0; k < ctx->eventsUsed; ++k) {
+            float e = 0;
+            CHECK_HIP(ctx, hipEventElapsedTime(&e, ctx->events[k].a, ctx->events[k].b));
+            switch (ctx->events[k].cls) {
+            case kClsExtend: stats->msExtend += e; break;
+            case kClsShadow: stats->msShadow += e; break;
+            case kClsProbe: stats->msProbe += e; break;
+            default: stats->msShade += e; break;
+            }
+        }
+        stats->batches = batches;
+        stats->bounces = (uint32_t)maxIter;
+        stats->nodeBytes = (uint32_t)sizeof(BvhNode);
+        stats->primBytes = 48;
+    }
+    return RTX_OK;
+}
+
+int rtx_read_image(rtx_ctx *ctx, float *rgba, size_t floatCount) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!rgba) return fail(ctx, RTX_E_INVALID, "rgba is NULL");
+    if (!ctx->accum || ctx->accSamples == 0) return fail(ctx, RTX_E_STATE, "nothing rendered yet");
+    size_t npix = (size_t)ctx->accW * ctx->accH;
+    if (floatCount < npix * 4) return fail(ctx, RTX_E_INVALID, "output buffer too small");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    launchResolveImage(ctx->stream, ctx->accum, (uint32_t)npix, 1.0f / (float)ctx->accSamples, ctx->resolved);
+    CHECK_HIP(ctx, hipGetLastError());
+    CHECK_HIP(ctx, hipMemcpyAsync(rgba, ctx->resolved, npix * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RTX_OK;
+}
+
+int rtx_accumulated_samples(const rtx_ctx *ctx, uint32_t *samples) {
+    if (!ctx || !samples) return RTX_E_INVALID;
+    *samples = ctx->accSamples;
+    return RTX_OK;
+}
+
+int rtx_read_paths(rtx_ctx *ctx, float *rgb, uint32_t *sig, size_t n) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->keepRgb || ctx->keptPaths == 0) return fail(ctx, RTX_E_STATE, "no render with RTX_FRAME_KEEP_PATHS");
+    if (n > ctx->keptPaths) return fail(ctx, RTX_E_INVALID, "more paths requested than rendered");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    if (rgb) {
+        std::vector<float4> tmp(n);
+        CHECK_HIP(ctx, hipMemcpy(tmp.data(), ctx->keepRgb, n * sizeof(float4), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            rgb[3 * i + 0] = tmp[i].x;
+            rgb[3 * i + 1] = tmp[i].y;
+            rgb[3 * i + 2] = tmp[i].z;
+        }
+    }
+    if (sig) CHECK_HIP(ctx, hipMemcpy(sig, ctx->keepSig, n * sizeof(uint2), hipMemcpyDeviceToHost));
+    return RTX_OK;
+}
+
+static int queryCommon(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, void *out,
+                       size_t outBytes, bool closest) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!scene || (n && (!rays || !out))) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    if (n == 0) return RTX_OK;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<void *> tmp;
+    rtx_ray *rD = nullptr;
+    void *oD = nullptr;
+    hipError_t e = devAlloc(&rD, n, tmp);
+    if (e == hipSuccess) e = devAlloc((char **)&oD, n * outBytes, tmp);
+    if (e == hipSuccess) e = hipMemcpyAsync(rD, rays, n * sizeof(rtx_ray), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        if (closest) launchQueryClosest(ctx->stream, scene->view, rD, n, (rtx_hit *)oD);
+        else launchQueryOccluded(ctx->stream, scene->view, rD, n, (uint8_t *)oD);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, oD, n * outBytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    freeAll(tmp);
+    if (e != hipSuccess) return fail(ctx, RTX_E_DEVICE, std::string("ray query: ") + hipGetErrorString(e));
+    return RTX_OK;
+}
+
+int rtx_trace_closest(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, rtx_hit *hits) {
+    return queryCommon(ctx, scene, rays, n, hits, sizeof(rtx_hit), true);
+}
+
+int rtx_trace_occluded(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, uint8_t *occluded) {
+    return queryCommon(ctx, scene, rays, n, occluded, 1, false);
+}
+
+} // extern "C"

@@ -1,0 +1,208 @@
+// Host BVH builder: binned surface-area heuristic (Wald, "On fast
+// construction of SAH-based bounding volume hierarchies", 2007), producing
+// the 64-byte two-box nodes the traversal kernels read.  It replaces the
+// driver's BLAS/TLAS build (src/SceneLoader.cpp:1101-1193), whose source
+// is not available.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+#include "bvh.h"
+
+namespace rtx {
+
+namespace {
+
+struct Box {
+    float lo[3], hi[3];
+    void reset() {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::numeric_limits<float>::infinity();
+            hi[a] = -std::numeric_limits<float>::infinity();
+        }
+    }
+    void grow(const Box &b) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], b.lo[a]);
+            hi[a] = std::max(hi[a], b.hi[a]);
+        }
+    }
+    void grow(const float p[3]) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0 || dy < 0 || dz < 0) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+constexpr int kBins = 32;
+constexpr int kSahDepthLimit = 32; // deeper subtrees use object-median splits
+constexpr int kLeafTarget = 4;
+
+struct Builder {
+    std::vector<Box> primBox;
+    std::vector<float> centroid; // 3 per prim
+    std::vector<uint32_t> order;
+    std::vector<BvhNode> nodes;
+    uint32_t maxDepth = 0;
+
+    // Returns the link for the subtree over order[start, start+count).
+    int32_t build(uint32_t start, uint32_t count, uint32_t depth, Box &outBox) {
+        outBox.reset();
+        Box cbox;
+        cbox.reset();
+        for (uint32_t i = start; i < start + count; ++i) {
+            outBox.grow(primBox[order[i]]);
+            cbox.grow(&centroid[3 * (size_t)order[i]]);
+        }
+        maxDepth = std::max(maxDepth, depth);
+        if (count <= (uint32_t)kLeafTarget && depth + 1 < (uint32_t)kMaxBvhDepth)
+            return makeLeaf(start, count);
+        if (depth + 1 >= (uint32_t)kMaxBvhDepth) {
+            if (count > (uint32_t)kLeafMaxPrims) throw std::runtime_error("BVH depth limit exceeded");
+            return makeLeaf(start, count);
+        }
+        uint32_t mid = start;
+        bool found = false;
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (cbox.hi[a] - cbox.lo[a] > cbox.hi[axis] - cbox.lo[axis]) axis = a;
+        if (depth < (uint32_t)kSahDepthLimit) {
+            float bestCost = std::numeric_limits<float>::infinity();
+            int bestAxis = -1, bestBin = -1;
+            for (int a = 0; a < 3; ++a) {
+                float ext = cbox.hi[a] - cbox.lo[a];
+                if (!(ext > 0.0f)) continue;
+                Box bb[kBins];
+                uint32_t bc[kBins] = {0};
+                for (auto &b : bb) b.reset();
+                float scale = kBins / ext;
+                for (uint32_t i = start; i < start + count; ++i) {
+                    uint32_t p = order[i];
+                    int bin = (int)((centroid[3 * (size_t)p + a] - cbox.lo[a]) * scale);
+                    bin = std::min(std::max(bin, 0), kBins - 1);
+                    bb[bin].grow(primBox[p]);
+                    bc[bin]++;
+                }
+                Box right[kBins];
+                uint32_t rc[kBins];
+                Box acc;
+                acc.reset();
+                uint32_t n = 0;
+                for (int b = kBins - 1; b > 0; --b) {
+                    acc.grow(bb[b]);
+                    n += bc[b];
+                    right[b] = acc;
+                    rc[b] = n;
+                }
+                acc.reset();
+                n = 0;
+                for (int b = 0; b < kBins - 1; ++b) {
+                    acc.grow(bb[b]);
+                    n += bc[b];
+                    if (n == 0 || rc[b + 1] == 0) continue;
+                    float cost = acc.area() * n + right[b + 1].area() * rc[b + 1];
+                    if (cost < bestCost) {
+                        bestCost = cost;
+                        bestAxis = a;
+                        bestBin = b;
+                    }
+                }
+            }
+            float leafCost = outBox.area() * count;
+            // cost model: traversal step = 1, triangle test = 1 (relative)
+            if (bestAxis >= 0 && (count > (uint32_t)kLeafMaxPrims || bestCost + outBox.area() < leafCost)) {
+                float ext = cbox.hi[bestAxis] - cbox.lo[bestAxis];
+                float scale = kBins / ext;
+                auto it = std::partition(order.begin() + start, order.begin() + start + count, [&](uint32_t p) {
+                    int bin = (int)((centroid[3 * (size_t)p + bestAxis] - cbox.lo[bestAxis]) * scale);
+                    bin = std::min(std::max(bin, 0), kBins - 1);
+                    return bin <= bestBin;
+                });
+                mid = (uint32_t)(it - order.begin());
+                found = mid > start && mid < start + count;
+            } else if (bestAxis >= 0 || count <= (uint32_t)kLeafMaxPrims) {
+                if (count <= (uint32_t)kLeafMaxPrims) return makeLeaf(start, count);
+            }
+        }
+        if (!found) { // object median on the widest centroid axis (or by index if degenerate)
+            mid = start + count / 2;
+            std::nth_element(order.begin() + start, order.begin() + mid, order.begin() + start + count,
+                             [&](uint32_t x, uint32_t y) {
+                                 float cx = centroid[3 * (size_t)x + axis], cy = centroid[3 * (size_t)y + axis];
+                                 return cx < cy || (cx == cy && x < y);
+                             });
+        }
+        uint32_t id = (uint32_t)nodes.size();
+        nodes.emplace_back();
+        Box lb, rb;
+        int32_t l = build(start, mid - start, depth + 1, lb);
+        int32_t r = build(mid, start + count - mid, depth + 1, rb);
+        setNode(nodes[id], lb, rb, l, r);
+        return (int32_t)id;
+    }
+
+    static void setNode(BvhNode &n, const Box &l, const Box &r, int32_t ll, int32_t rl) {
+        n.a[0] = l.lo[0]; n.a[1] = l.lo[1]; n.a[2] = l.lo[2]; n.a[3] = l.hi[0];
+        n.b[0] = l.hi[1]; n.b[1] = l.hi[2]; n.b[2] = r.lo[0]; n.b[3] = r.lo[1];
+        n.c[0] = r.lo[2]; n.c[1] = r.hi[0]; n.c[2] = r.hi[1]; n.c[3] = r.hi[2];
+        n.d[0] = ll;
+        n.d[1] = rl;
+        n.d[2] = 0;
+        n.d[3] = 0;
+    }
+};
+
+// Pad a coordinate bound outwards by ~2^-20 relative so that FP32 hits
+// accepted by the watertight test just outside an edge are never culled.
+inline float padLo(float x) { return x - (std::fabs(x) + 1.0f) * 9.5367431640625e-07f; }
+inline float padHi(float x) { return x + (std::fabs(x) + 1.0f) * 9.5367431640625e-07f; }
+
+} // namespace
+
+void buildBvh(const float *tv, uint32_t numTris, BvhBuildResult &out) {
+    auto t0 = std::chrono::steady_clock::now();
+    Builder b;
+    b.primBox.resize(numTris);
+    b.centroid.resize(3 * (size_t)numTris);
+    b.order.resize(numTris);
+    for (uint32_t i = 0; i < numTris; ++i) {
+        Box &bx = b.primBox[i];
+        bx.reset();
+        for (int k = 0; k < 3; ++k) bx.grow(&tv[9 * (size_t)i + 3 * k]);
+        for (int a = 0; a < 3; ++a) {
+            bx.lo[a] = padLo(bx.lo[a]);
+            bx.hi[a] = padHi(bx.hi[a]);
+            b.centroid[3 * (size_t)i + a] = 0.5f * (bx.lo[a] + bx.hi[a]);
+        }
+        b.order[i] = i;
+    }
+    Box all;
+    all.reset();
+    int32_t root = numTris > 0 ? b.build(0, numTris, 0, all) : makeLeaf(0, 1);
+    if (root < 0) {
+        // a single leaf (or nothing): wrap it in an internal root whose
+        // second child has an empty box and is never entered
+        Box empty; // a point far outside any ray's [tmin, tmax]: never entered
+        for (int a = 0; a < 3; ++a) empty.lo[a] = empty.hi[a] = 1e30f;
+        if (numTris == 0) all = empty;
+        b.nodes.emplace_back();
+        Builder::setNode(b.nodes.back(), all, empty, root, makeLeaf(0, 1));
+        b.maxDepth = 0;
+    }
+    out.nodes = std::move(b.nodes);
+    out.primOrder = std::move(b.order);
+    out.maxDepth = b.maxDepth + 1;
+    auto t1 = std::chrono::steady_clock::now();
+    out.buildMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+} // namespace rtx

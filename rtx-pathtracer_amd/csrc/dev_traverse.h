@@ -1,0 +1,220 @@
+// BVH traversal on the device.  Replaces the RT-core traversal behind the
+// reference's traceRayEXT calls (closest hit: shaders/raytrace.rgen:1011-1022;
+// shadow any-hit with TerminateOnFirstHit: raytrace.rgen:622-638) and the
+// sphere intersection shader (raytrace.sphere.rint:11-29).
+//
+// Stack-based traversal (Aila & Laine 2009) over two-box nodes; the
+// stack's first LDS_DEPTH entries live in LDS (one column per thread, so
+// consecutive lanes hit consecutive banks), deeper entries in a private
+// array.  The builder bounds the tree depth by kMaxBvhDepth.
+#pragma once
+
+#include "dev_math.h"
+#include "internal.h"
+
+namespace rtx {
+
+struct TravRay {
+    V3 o, d, inv;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+RTX_DI TravRay makeTravRay(V3 o, V3 d) {
+    TravRay r;
+    r.o = o;
+    r.d = d;
+    // zero direction components get a tiny stand-in so slab tests never see 0*inf
+    float dx = d.x != 0.0f ? d.x : copysignf(1e-30f, d.x);
+    float dy = d.y != 0.0f ? d.y : copysignf(1e-30f, d.y);
+    float dz = d.z != 0.0f ? d.z : copysignf(1e-30f, d.z);
+    r.inv = v3(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    // watertight test set-up (Woop, Benthin, Wald, JCGT 2013)
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    if (sel(d, kz) < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    float dkz = sel(d, kz);
+    r.Sx = sel(d, kx) / dkz;
+    r.Sy = sel(d, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    return r;
+}
+
+// Slab test against [lo, hi]; conservative by the 1 + 2*gamma_3 factor of
+// Ize, "Robust BVH ray traversal" (JCGT 2013).
+RTX_DI bool slab(float lx, float ly, float lz, float hx, float hy, float hz, const TravRay &r, float tmin,
+                 float tmax, float &tnear) {
+    float tx0 = (lx - r.o.x) * r.inv.x, tx1 = (hx - r.o.x) * r.inv.x;
+    float ty0 = (ly - r.o.y) * r.inv.y, ty1 = (hy - r.o.y) * r.inv.y;
+    float tz0 = (lz - r.o.z) * r.inv.z, tz1 = (hz - r.o.z) * r.inv.z;
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.0000004f;
+    tnear = tn;
+    return tn <= fminf(tf, tmax);
+}
+
+// Watertight ray/triangle test.  Products are not contracted into FMAs so
+// that the edge functions of a shared edge are exact negatives.  Returns
+// the hit distance and the barycentrics of vertices 1 and 2 (the
+// reference's hit attributes, raytrace.rchit:28).
+RTX_DI bool isectTri(const TravRay &r, float4 a, float4 b, float4 c, float tmin, float tmax, float &t, float &u,
+                     float &v) {
+#pragma clang fp contract(off)
+    V3 A = v3(a) - r.o, B = v3(b) - r.o, C = v3(c) - r.o;
+    float Az0 = sel(A, r.kz), Bz0 = sel(B, r.kz), Cz0 = sel(C, r.kz);
+    float Ax = sel(A, r.kx) - r.Sx * Az0, Ay = sel(A, r.ky) - r.Sy * Az0;
+    float Bx = sel(B, r.kx) - r.Sx * Bz0, By = sel(B, r.ky) - r.Sy * Bz0;
+    float Cx = sel(C, r.kx) - r.Sx * Cz0, Cy = sel(C, r.ky) - r.Sy * Cz0;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.Sz * Az0, Bz = r.Sz * Bz0, Cz = r.Sz * Cz0;
+    float T = U * Az + V * Bz + W * Cz;
+    float tt = T / det;
+    if (!(tt >= tmin && tt <= tmax)) return false;
+    t = tt;
+    u = V / det;
+    v = W / det;
+    return true;
+}
+
+// raytrace.sphere.rint:11-29: both roots reported, nearest in [tmin, tmax] kept
+RTX_DI bool isectSphere(const rtx_sphere &s, V3 o, V3 d, float tmin, float tmax, float &t) {
+    V3 oc = o - v3(s.center);
+    float dotDOC = dot(d, oc);
+    float rootTerm = dotDOC * dotDOC - dot(oc, oc) + s.radius * s.radius;
+    if (rootTerm < 0.0f) return false;
+    float root = sqrtf(rootTerm);
+    float t1 = -dotDOC + root, t2 = -dotDOC - root;
+    bool hit = false;
+    float best = tmax;
+    if (t1 >= tmin && t1 <= best) {
+        best = t1;
+        hit = true;
+    }
+    if (t2 >= tmin && t2 <= best) {
+        best = t2;
+        hit = true;
+    }
+    t = best;
+    return hit;
+}
+
+// Stack with LDS_DEPTH entries in LDS (column per thread, stride BLOCK)
+// and the remainder in a private array.
+template <int BLOCK, int LDS_DEPTH> struct HybridStack {
+    int *lds;
+    int sp;
+    int priv[kMaxBvhDepth - LDS_DEPTH];
+    RTX_DI void push(int x) {
+        if (sp < LDS_DEPTH) lds[sp * BLOCK] = x;
+        else priv[sp - LDS_DEPTH] = x;
+        sp++;
+    }
+    RTX_DI int pop() {
+        sp--;
+        return sp < LDS_DEPTH ? lds[sp * BLOCK] : priv[sp - LDS_DEPTH];
+    }
+};
+
+struct PrivateStack {
+    int sp;
+    int priv[kMaxBvhDepth];
+    RTX_DI void push(int x) { priv[sp++] = x; }
+    RTX_DI int pop() { return priv[--sp]; }
+};
+
+struct VisitCount {
+    uint32_t nodes = 0, prims = 0;
+};
+
+// Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax].
+// Returns the primitive id (triangles 0..numTris-1, then spheres) or -1.
+// Closest-hit ties in t go to the smaller primitive id.
+template <bool ANY, bool COUNT, class Stack>
+RTX_DI int traverse(const DevScene &S, const TravRay &r, float tmin, float &tmax, float &bu, float &bv,
+                    Stack &st, VisitCount &vc) {
+    int best = -1;
+    for (uint32_t i = 0; i < S.numSpheres; ++i) {
+        float t;
+        if (isectSphere(S.spheres[i], r.o, r.d, tmin, tmax, t)) {
+            int id = (int)(S.numTris + i);
+            if (ANY) return id;
+            if (best < 0 || t < tmax || id < best) {
+                tmax = t;
+                best = id;
+                bu = bv = 0.0f;
+            }
+        }
+    }
+    st.sp = 0;
+    int link = 0;
+    for (;;) {
+        if (link >= 0) {
+            const float4 *np = S.nodes + 4 * (size_t)link;
+            float4 a = np[0], b = np[1], c = np[2];
+            int4 dd = *reinterpret_cast<const int4 *>(np + 3);
+            if (COUNT) vc.nodes++;
+            float t0, t1;
+            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, r, tmin, tmax, t0);
+            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, r, tmin, tmax, t1);
+            if (h0 && h1) {
+                int nearL = dd.x, farL = dd.y;
+                if (t1 < t0) {
+                    nearL = dd.y;
+                    farL = dd.x;
+                }
+                st.push(farL);
+                link = nearL;
+            } else if (h0) {
+                link = dd.x;
+            } else if (h1) {
+                link = dd.y;
+            } else {
+                if (st.sp == 0) break;
+                link = st.pop();
+            }
+        } else {
+            uint32_t leaf = ~(uint32_t)link;
+            uint32_t start = leaf >> 4, cnt = (leaf & 15u) + 1u;
+            const float4 *tp = S.tris + 3 * (size_t)start;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                float4 v0 = tp[3 * k], v1 = tp[3 * k + 1], v2 = tp[3 * k + 2];
+                if (COUNT) vc.prims++;
+                float t, u, v;
+                if (isectTri(r, v0, v1, v2, tmin, tmax, t, u, v)) {
+                    int id = __float_as_int(v0.w);
+                    if (ANY) return id;
+                    if (best < 0 || t < tmax || id < best) {
+                        tmax = t;
+                        best = id;
+                        bu = u;
+                        bv = v;
+                    }
+                }
+            }
+            if (st.sp == 0) break;
+            link = st.pop();
+        }
+    }
+    return best;
+}
+
+} // namespace rtx

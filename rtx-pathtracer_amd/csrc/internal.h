@@ -1,0 +1,116 @@
+// Data shared between the host orchestration (api.cpp), the BVH builder
+// (bvh_build.cpp) and the kernels (kernels.hip).  Plain structs only.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "bvh.h"
+#include "rtx_amd.h"
+
+namespace rtx {
+
+// ----------------------------------------------------------- device scene
+struct DevScene {
+    const float4 *nodes;      // 4 x float4 per BvhNode
+    const float4 *tris;       // 3 x float4 per triangle in leaf order: v0 (w = prim id bits), v1, v2
+    const int2 *primRef;      // per global triangle id: (instance, local triangle)
+    const rtx_vertex *vertices;
+    const uint32_t *indices;
+    const uint32_t *vertexBase; // per model
+    const uint32_t *indexBase;  // per model
+    const rtx_instance *instances;
+    const rtx_material *materials;
+    const rtx_light *lights;
+    const int32_t *lightIndex;
+    const rtx_face_sample *triTable;
+    const rtx_sphere *spheres;
+    uint32_t numTris;
+    uint32_t numSpheres;
+    uint32_t lightCount;
+    uint32_t triRows;
+    uint32_t numNodes;
+};
+
+// ----------------------------------------------------------- path state
+// Queues and per-path state of one wavefront batch (structure of arrays).
+struct Queue {
+    int32_t *id;    // path id (extend queue, done queue) or NEE slot (shadow/probe)
+    float4 *org;    // origin.xyz, tmin
+    float4 *dir;    // direction.xyz, tmax
+    float4 *aux;    // extend: hit (t, prim bits, u, v); probe: (base rgb, pdfMat)
+};
+
+struct Counters {
+    uint32_t extend[2]; // ping-pong extend queue sizes
+    uint32_t shadow;
+    uint32_t probe;
+    uint32_t done;
+    uint32_t pad[3];
+    unsigned long long extendRays, shadowRays, probeRays, nodeVisits, primVisits;
+};
+
+struct FrameConst {
+    rtx_push_constant pc;
+    rtx_camera cam;
+    uint32_t width, height;
+    uint64_t seed;
+    uint32_t sampleOffset;   // first sample of this render call
+    int32_t rrStartDepth;
+    uint32_t numNEE;         // NEE samples per vertex (>= 1)
+    uint32_t countVisits;
+};
+
+struct BatchDesc {
+    uint64_t firstPath;   // global path index (within the call) of local path 0
+    uint32_t numPaths;    // paths in this batch
+    uint32_t npix;        // width * height
+};
+
+struct PathBuffers {
+    uint32_t *rng;
+    uint32_t *hash;
+    float4 *thr;      // throughput xyz, w = flags (uint bits)
+    float4 *result;   // radiance xyz
+    float4 *pendShadow; // [path * numNEE + j]: premultiplied shadow contribution, w = 1 if to add
+    float4 *pendProbe;  // [path * numNEE + j]: premultiplied probe contribution
+    float4 *out;      // per local path: radiance xyz, w unused
+    uint2 *outSig;    // per local path: (final LCG state, primitive hash)
+};
+
+// Path flags packed in thr.w
+constexpr uint32_t kFlagDepthMask = 0xFFu;        // bits 0-7 depth
+constexpr uint32_t kFlagFollowCountShift = 8;     // bits 8-15 followCount
+constexpr uint32_t kFlagFollow = 1u << 16;
+constexpr uint32_t kFlagAddNext = 1u << 17;
+constexpr uint32_t kFlagPending = 1u << 18;       // NEE contributions waiting to be added
+
+// ----------------------------------------------------------- launchers
+struct LaunchCfg {
+    int blocksExtend = 0;   // 0 = auto
+    int blocksShade = 0;
+};
+
+void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue q0,
+                    Counters *cnt, int qsel);
+void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits,
+                  int blocks);
+void launchShade(hipStream_t s, const DevScene &S, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue qin,
+                 Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel, int blocks);
+void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
+                  int blocks);
+void launchProbe(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb, Counters *cnt,
+                 bool countVisits, int blocks);
+void launchFinish(hipStream_t s, const FrameConst *fc, PathBuffers pb, const int32_t *done, Counters *cnt,
+                  int blocks);
+void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,
+                      bool keep);
+void launchResetCounters(hipStream_t s, Counters *cnt, int which);
+void launchQueryClosest(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, rtx_hit *hits);
+void launchQueryOccluded(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, uint8_t *occ);
+void launchResolveImage(hipStream_t s, const float4 *accum, uint32_t npix, float invSamples, float4 *out);
+
+} // namespace rtx

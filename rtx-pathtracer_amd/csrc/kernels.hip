@@ -1,0 +1,499 @@
+// Wavefront path-tracing kernels for CDNA4 (gfx950).
+//
+// The reference runs one recursive megakernel per pixel (raygen main(),
+// shaders/raytrace.rgen:1623-1827, bounce loop raytrace() 992-1226).  Here
+// one bounce of every live path is one pass of five kernels:
+//   extend  - closest-hit traversal of the queued rays
+//   shade   - hit shader + emission + NEE sampling + continuation sampling,
+//             appends shadow / probe / next-extend / done records with
+//             wave-ballot + prefix-sum compaction
+//   shadow  - any-hit traversal of the NEE shadow rays
+//   probe   - closest-hit traversal of the MIS probe rays + their MIS weight
+//   finish  - adds the last pending NEE terms of terminated paths
+// Every path owns its LCG stream, so results do not depend on queue order.
+#include <hip/hip_runtime.h>
+
+#include "dev_math.h"
+#include "dev_shading.h"
+#include "dev_traverse.h"
+#include "internal.h"
+
+namespace rtx {
+
+constexpr int kTraceBlock = 128;
+constexpr int kLdsStack = 24;
+constexpr int kShadeBlock = 256;
+
+// --------------------------------------------------------- wave helpers
+// Append `pred` lanes to a queue: one atomic per wave, order-preserving
+// within the wave (ballot + popcount prefix).  Must be called with the
+// whole wave converged.
+RTX_DI int waveAppend(bool pred, uint32_t *counter) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int lane = (int)__lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return pred ? (int)(base + (uint32_t)__popcll(below)) : -1;
+}
+
+RTX_DI uint32_t waveSum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <bool COUNT> RTX_DI void flushVisits(const VisitCount &vc, Counters *cnt) {
+    if (!COUNT) return;
+    uint32_t n = waveSum(vc.nodes), p = waveSum(vc.prims);
+    if (__lane_id() == 0) {
+        atomicAdd(&cnt->nodeVisits, (unsigned long long)n);
+        atomicAdd(&cnt->primVisits, (unsigned long long)p);
+    }
+}
+
+// ------------------------------------------------------------ generate
+// main() up to the camera ray: seed = tea(pixel, randomUInt) (raytrace.rgen:1625),
+// getCameraRay (raytrace.rgen:1487-1494); randomUInt per sample from PCG32.
+__global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
+                                                 Queue q0, Counters *cnt, int qsel) {
+    const FrameConst &fc = *fcp;
+    uint32_t pid = blockIdx.x * 256 + threadIdx.x;
+    if (pid == 0) cnt->extend[qsel] = b.numPaths;
+    if (pid >= b.numPaths) return;
+    uint64_t g = b.firstPath + pid;
+    uint32_t pixel = (uint32_t)(g % b.npix);
+    uint64_t sample = g / b.npix + fc.sampleOffset;
+    uint32_t frameSeed = pcg32First(fc.seed, sample);
+    Rng rng{tea(pixel, frameSeed)};
+    uint32_t px = pixel % fc.width, py = pixel / fc.width;
+    float jx = rng.negPos();
+    float jy = rng.negPos();
+    float ux = ((float)px + 0.5f + jx / 2.0f) / (float)fc.width;
+    float uy = ((float)py + 0.5f + jy / 2.0f) / (float)fc.height;
+    float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    V3 o = xform(fc.cam.viewInverse, v3(0.0f), 1.0f);
+    V3 target = xform(fc.cam.projInverse, v3(dx, dy, 1.0f), 1.0f);
+    V3 d = normalize(xform(fc.cam.viewInverse, normalize(target), 0.0f));
+    q0.id[pid] = (int)pid;
+    q0.org[pid] = f4(o, kTMin);
+    q0.dir[pid] = f4(d, kTMax);
+    pb.rng[pid] = rng.s;
+    pb.hash[pid] = 2166136261u;
+    // depth 0, followCount 0, follow = true, addNextDirectLights = addFirstHitLight = true
+    pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagFollow | kFlagAddNext));
+    pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// --------------------------------------------------------------- extend
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kExtend(DevScene S, Queue q, Counters *cnt, int qsel) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    const uint32_t n = cnt->extend[qsel];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->extendRays, (unsigned long long)n);
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
+        float4 o = q.org[i], d = q.dir[i];
+        TravRay r = makeTravRay(v3(o), v3(d));
+        float tmax = d.w, bu = 0.0f, bv = 0.0f;
+        int prim = traverse<false, COUNT>(S, r, o.w, tmax, bu, bv, st, vc);
+        q.aux[i] = make_float4(tmax, __int_as_float(prim), bu, bv);
+    }
+    flushVisits<COUNT>(vc, cnt);
+}
+
+// --------------------------------------------------------------- shadow
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kShadow(DevScene S, Queue q, PathBuffers pb, Counters *cnt) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    const uint32_t n = cnt->shadow;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->shadowRays, (unsigned long long)n);
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
+        float4 o = q.org[i], d = q.dir[i];
+        TravRay r = makeTravRay(v3(o), v3(d));
+        float tmax = d.w, bu, bv;
+        int prim = traverse<true, COUNT>(S, r, o.w, tmax, bu, bv, st, vc);
+        if (prim < 0) pb.pendShadow[q.id[i]].w = 1.0f; // unoccluded: raytrace.shadow.rmiss:7-10
+    }
+    flushVisits<COUNT>(vc, cnt);
+}
+
+// ---------------------------------------------------------------- probe
+// The MIS probe of nextEventEstimation (raytrace.rgen:665-729): closest hit
+// plus the BSDF-side MIS weight against the light it hits.
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, const FrameConst *__restrict__ fcp, Queue q,
+                                                      PathBuffers pb, Counters *cnt) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    const FrameConst &fc = *fcp;
+    const uint32_t n = cnt->probe;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->probeRays, (unsigned long long)n);
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    const bool power = fc.pc.usePowerHeuristic != 0;
+    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
+        float4 o4 = q.org[i], d4 = q.dir[i], base = q.aux[i];
+        V3 o = v3(o4), d = v3(d4);
+        TravRay r = makeTravRay(o, d);
+        float tmax = d4.w, bu = 0.0f, bv = 0.0f;
+        int prim = traverse<false, COUNT>(S, r, o4.w, tmax, bu, bv, st, vc);
+        float pdfMat = base.w;
+        V3 contrib = v3(0.0f);
+        if (prim < 0) {
+            // miss: environment colour (default black texture) with pdf 1/(2 pi lightCount)
+            V3 lightColor = v3(0.0f);
+            float pdfL = kInv2Pi / (float)S.lightCount;
+            float h = misWeight(power, pdfMat, pdfL);
+            contrib = v3(base) * lightColor * h;
+        } else {
+            HitInfo hi = hitInfo(S, prim, tmax, bu, bv, o, d);
+            const rtx_material &ms = S.materials[hi.matIndex];
+            if (ms.type == RTX_MAT_LIGHT) {
+                int iLight = hi.isSphere ? S.spheres[hi.instanceIndex].iLight : S.instances[hi.instanceIndex].iLight;
+                const rtx_light &light = S.lights[iLight];
+                float cosL = dot(-d, hi.normal); // pdfLight, raytrace.rgen:334-339
+                float pdfL = light.sampleProb * tmax * tmax / cosL / light.area;
+                float h = misWeight(power, pdfMat, pdfL);
+                if (!isnan(h)) contrib = v3(base) * v3(ms.lightColor) * h;
+            }
+        }
+        pb.pendProbe[q.id[i]] = f4(contrib, 0.0f);
+    }
+    flushVisits<COUNT>(vc, cnt);
+}
+
+RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE) {
+    V3 s = v3(0.0f);
+    for (uint32_t j = 0; j < numNEE; ++j) {
+        float4 sh = pb.pendShadow[pid * numNEE + j];
+        V3 term = v3(pb.pendProbe[pid * numNEE + j]);
+        if (sh.w != 0.0f) term = v3(sh) + term;
+        s = s + term;
+    }
+    return s;
+}
+
+// ----------------------------------------------------------------- shade
+// One iteration of the bounce loop, raytrace.rgen:1008-1215 (plain method).
+__global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameConst *__restrict__ fcp, BatchDesc b,
+                                                      PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp,
+                                                      int32_t *done, Counters *cnt, int qsel) {
+    const FrameConst &fc = *fcp;
+    const uint32_t n = cnt->extend[qsel];
+    const uint32_t numNEE = fc.numNEE;
+    const float invNEE = 1.0f / (float)numNEE;
+    const bool useNEE = fc.pc.enableNEE != 0;
+    const bool addDirect = !useNEE;
+    const bool mis = fc.pc.enableMIS != 0;
+    const bool power = fc.pc.usePowerHeuristic != 0;
+    const int maxDepth = fc.pc.maxDepth, maxFollow = fc.pc.maxFollowDiscrete;
+    const uint32_t stride = gridDim.x * kShadeBlock;
+    uint32_t *nextCount = &cnt->extend[qsel ^ 1];
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        bool cont = false, isDone = false, doNee = false;
+        uint32_t pid = 0;
+        Rng rng{0};
+        V3 thr = v3(0.0f), res = v3(0.0f), d = v3(0.0f);
+        uint32_t flags = 0, hash = 0;
+        HitInfo h;
+        Mat m;
+        m.type = -1;
+        int depth = 0, followCount = 0;
+        bool follow = false, addNext = false;
+        if (active) {
+            pid = (uint32_t)qin.id[i];
+            float4 hit = qin.aux[i];
+            d = v3(qin.dir[i]);
+            V3 o = v3(qin.org[i]);
+            rng.s = pb.rng[pid];
+            float4 t4 = pb.thr[pid];
+            thr = v3(t4);
+            flags = __float_as_uint(t4.w);
+            res = v3(pb.result[pid]);
+            hash = pb.hash[pid];
+            if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE);
+            int prim = __float_as_int(hit.y);
+            hash = (hash ^ (uint32_t)prim) * 16777619u;
+            depth = (int)(flags & kFlagDepthMask) + 1;
+            followCount = (int)((flags >> kFlagFollowCountShift) & 0xFFu);
+            follow = (flags & kFlagFollow) != 0;
+            addNext = (flags & kFlagAddNext) != 0;
+            if (prim < 0) {
+                // raytrace.rmiss: black default environment
+                if (addDirect || addNext) res = res + thr * v3(0.0f);
+                isDone = true;
+            } else {
+                h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
+                m = loadMat(&S.materials[h.matIndex]);
+                if (m.type == RTX_MAT_LIGHT && (addDirect || addNext)) res = res + thr * m.lightColor;
+                if (hasDiscreteDirection(m.type)) {
+                    addNext = true;
+                    follow = true;
+                    if (depth >= maxDepth) followCount++;
+                } else {
+                    addNext = false;
+                    follow = false;
+                    doNee = useNEE && neeCapable(m.type) && S.lightCount > 0;
+                }
+            }
+        }
+        // ---- next event estimation, raytrace.rgen:601-731 (numNEE samples, multipleNEE 871-877)
+        const V3 wi = -d;
+        for (uint32_t j = 0; j < numNEE; ++j) {
+            bool wantShadow = false, wantProbe = false;
+            V3 lightDir = v3(0.0f), bsdfDir = v3(0.0f), probeBase = v3(0.0f);
+            float lightDist = 0.0f, pdfMatProbe = 0.0f;
+            const uint32_t slot = pid * numNEE + j;
+            if (doNee) {
+                V3 lightColor;
+                float pdfL = sampleLights(S, rng, h.worldPos, lightDir, lightColor, lightDist);
+                float cosS = dot(h.normal, lightDir);
+                bool traced = cosS > 0.0f && pdfL > 0.0f;
+                bool probeAllowed = true;
+                V3 sh = v3(0.0f);
+                if (traced) {
+                    if (mis) {
+                        float pdfMat = pdfBSDF(m, h.normal, wi, lightDir);
+                        float hw = misWeight(power, pdfL, pdfMat);
+                        if (isnan(hw)) {
+                            // The reference returns from NEE here if the shadow ray is
+                            // unoccluded (raytrace.rgen:653-655), skipping the probe's draws:
+                            // resolve visibility now to keep the random stream in step.
+                            PrivateStack ps;
+                            VisitCount vc0;
+                            TravRay sr = makeTravRay(h.worldPos, lightDir);
+                            float tm = lightDist * (1.0f - 0.0001f), bu, bv;
+                            bool occl = traverse<true, false>(S, sr, kTMin, tm, bu, bv, ps, vc0) >= 0;
+                            traced = false;
+                            if (!occl) probeAllowed = false;
+                        } else {
+                            sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor * hw / pdfL;
+                        }
+                    } else {
+                        sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor / pdfL;
+                    }
+                }
+                pb.pendShadow[slot] = f4(thr * sh * invNEE, 0.0f);
+                wantShadow = traced;
+                bool probeZero = true;
+                if (mis && probeAllowed) {
+                    float pdfMat = sampleBSDF(rng, m, wi, h.normal, h.frontFace, bsdfDir);
+                    if (pdfMat > 0.0f) {
+                        wantProbe = true;
+                        probeZero = false;
+                        pdfMatProbe = pdfMat;
+                        probeBase = thr * evalBsdf(m, h.normal, wi, bsdfDir, true) * invNEE / pdfMat;
+                    }
+                }
+                if (probeZero) pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            int si = waveAppend(wantShadow, &cnt->shadow);
+            if (wantShadow) {
+                qs.id[si] = (int)slot;
+                qs.org[si] = f4(h.worldPos, kTMin);
+                qs.dir[si] = f4(lightDir, lightDist * (1.0f - 0.0001f));
+            }
+            int pi = waveAppend(wantProbe, &cnt->probe);
+            if (wantProbe) {
+                qp.id[pi] = (int)slot;
+                qp.org[pi] = f4(h.worldPos, kTMin);
+                qp.dir[pi] = f4(bsdfDir, kTMax);
+                qp.aux[pi] = f4(probeBase, pdfMatProbe);
+            }
+        }
+        // ---- continuation, raytrace.rgen:1169-1215
+        V3 newDir = v3(0.0f);
+        if (active && !isDone) {
+            float pdf = sampleBSDF(rng, m, wi, h.normal, h.frontFace, newDir);
+            if (pdf <= 0.0f) {
+                isDone = true;
+            } else {
+                V3 change = evalBsdf(m, h.normal, wi, newDir, h.frontFace) / pdf;
+                thr = thr * change;
+                bool killed = false;
+                if (fc.pc.enableRR && depth >= fc.rrStartDepth) { // Russian roulette (absent from the reference)
+                    float q = fmaxf(fabsf(thr.x), fmaxf(fabsf(thr.y), fabsf(thr.z)));
+                    q = fminf(q, 0.95f);
+                    if (rng.next() >= q) killed = true;
+                    else thr = thr / q;
+                }
+                if (killed) {
+                    isDone = true;
+                } else {
+                    cont = depth <= maxDepth || (follow && followCount <= maxFollow);
+                    isDone = !cont;
+                }
+            }
+        }
+        if (active) {
+            uint32_t nf = (uint32_t)depth | ((uint32_t)followCount << kFlagFollowCountShift) |
+                          (follow ? kFlagFollow : 0u) | (addNext ? kFlagAddNext : 0u) |
+                          (doNee ? kFlagPending : 0u);
+            pb.rng[pid] = rng.s;
+            pb.hash[pid] = hash;
+            pb.thr[pid] = f4(thr, __uint_as_float(nf));
+            pb.result[pid] = f4(res, 0.0f);
+        }
+        int qi = waveAppend(cont, nextCount);
+        if (cont) {
+            qout.id[qi] = (int)pid;
+            qout.org[qi] = f4(h.worldPos, kTMin);
+            qout.dir[qi] = f4(newDir, kTMax);
+        }
+        int di = waveAppend(isDone, &cnt->done);
+        if (isDone) done[di] = (int)pid;
+    }
+}
+
+// ---------------------------------------------------------------- finish
+__global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fcp, PathBuffers pb,
+                                               const int32_t *done, Counters *cnt) {
+    const uint32_t n = cnt->done;
+    const uint32_t numNEE = fcp->numNEE;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        uint32_t pid = (uint32_t)done[i];
+        V3 res = v3(pb.result[pid]);
+        uint32_t flags = __float_as_uint(pb.thr[pid].w);
+        if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE);
+        pb.out[pid] = f4(res, 0.0f);
+        pb.outSig[pid] = make_uint2(pb.rng[pid], pb.hash[pid]);
+    }
+}
+
+// ------------------------------------------------------------ accumulate
+// Adds each pixel's samples of this batch in increasing sample order, so
+// the accumulated image does not depend on how samples are batched.
+__global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb,
+                                                   uint2 *keepSig, int keep) {
+    uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    uint32_t lanes = b.numPaths < b.npix ? b.numPaths : b.npix;
+    if (j >= lanes) return;
+    uint32_t pixel = (uint32_t)((b.firstPath + j) % b.npix);
+    float4 acc = accum[pixel];
+    for (uint32_t pid = j; pid < b.numPaths; pid += b.npix) {
+        float4 v = pb.out[pid];
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += 1.0f;
+        if (keep) {
+            keepRgb[b.firstPath + pid] = v;
+            keepSig[b.firstPath + pid] = pb.outSig[pid];
+        }
+    }
+    accum[pixel] = acc;
+}
+
+__global__ void kResetCounters(Counters *cnt, int which) {
+    if (threadIdx.x == 0) {
+        cnt->extend[which] = 0;
+        cnt->shadow = 0;
+        cnt->probe = 0;
+        cnt->done = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void kResolveImage(const float4 *accum, uint32_t npix, float inv, float4 *out) {
+    uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+    float4 a = accum[p];
+    out[p] = make_float4(a.x * inv, a.y * inv, a.z * inv, 1.0f);
+}
+
+// ---------------------------------------------------------- ray queries
+__global__ __launch_bounds__(kTraceBlock) void kQueryClosest(DevScene S, const rtx_ray *rays, size_t n, rtx_hit *hits) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    size_t i = (size_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    rtx_ray ry = rays[i];
+    TravRay r = makeTravRay(v3(ry.org), v3(ry.dir));
+    float tmax = ry.tmax, bu = 0.0f, bv = 0.0f;
+    int prim = traverse<false, false>(S, r, ry.tmin, tmax, bu, bv, st, vc);
+    rtx_hit h;
+    h.t = prim >= 0 ? tmax : ry.tmax;
+    h.prim = prim;
+    h.u = bu;
+    h.v = bv;
+    hits[i] = h;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void kQueryOccluded(DevScene S, const rtx_ray *rays, size_t n,
+                                                              uint8_t *occ) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    size_t i = (size_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    rtx_ray ry = rays[i];
+    TravRay r = makeTravRay(v3(ry.org), v3(ry.dir));
+    float tmax = ry.tmax, bu, bv;
+    occ[i] = traverse<true, false>(S, r, ry.tmin, tmax, bu, bv, st, vc) >= 0 ? 1 : 0;
+}
+
+// ------------------------------------------------------------- launchers
+static int gridFor(size_t n, int block) { return (int)((n + block - 1) / block); }
+
+void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue q0, Counters *cnt,
+                    int qsel) {
+    hipLaunchKernelGGL(kGenerate, dim3(gridFor(b.numPaths, 256)), dim3(256), 0, s, fc, b, pb, q0, cnt, qsel);
+}
+void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits, int blocks) {
+    if (countVisits) hipLaunchKernelGGL(kExtend<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+    else hipLaunchKernelGGL(kExtend<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+}
+void launchShade(hipStream_t s, const DevScene &S, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue qin,
+                 Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel, int blocks) {
+    hipLaunchKernelGGL(kShade, dim3(blocks), dim3(kShadeBlock), 0, s, S, fc, b, pb, qin, qout, qs, qp, done, cnt,
+                       qsel);
+}
+void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
+                  int blocks) {
+    if (countVisits) hipLaunchKernelGGL(kShadow<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
+    else hipLaunchKernelGGL(kShadow<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
+}
+void launchProbe(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb, Counters *cnt,
+                 bool countVisits, int blocks) {
+    if (countVisits) hipLaunchKernelGGL(kProbe<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, fc, q, pb, cnt);
+    else hipLaunchKernelGGL(kProbe<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, fc, q, pb, cnt);
+}
+void launchFinish(hipStream_t s, const FrameConst *fc, PathBuffers pb, const int32_t *done, Counters *cnt,
+                  int blocks) {
+    hipLaunchKernelGGL(kFinish, dim3(blocks), dim3(256), 0, s, fc, pb, done, cnt);
+}
+void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,
+                      bool keep) {
+    uint32_t lanes = b.numPaths < b.npix ? b.numPaths : b.npix;
+    hipLaunchKernelGGL(kAccumulate, dim3(gridFor(lanes, 256)), dim3(256), 0, s, b, pb, accum, keepRgb, keepSig,
+                       keep ? 1 : 0);
+}
+void launchResetCounters(hipStream_t s, Counters *cnt, int which) {
+    hipLaunchKernelGGL(kResetCounters, dim3(1), dim3(64), 0, s, cnt, which);
+}
+void launchQueryClosest(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, rtx_hit *hits) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(kQueryClosest, dim3(gridFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, S, rays, n, hits);
+}
+void launchQueryOccluded(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, uint8_t *occ) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(kQueryOccluded, dim3(gridFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, S, rays, n, occ);
+}
+void launchResolveImage(hipStream_t s, const float4 *accum, uint32_t npix, float inv, float4 *out) {
+    hipLaunchKernelGGL(kResolveImage, dim3(gridFor(npix, 256)), dim3(256), 0, s, accum, npix, inv, out);
+}
+
+} // namespace rtx

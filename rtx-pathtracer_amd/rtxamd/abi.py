@@ -1,0 +1,177 @@
+"""ctypes mirror of include/rtx_amd.h and include/rtx_host.h.
+
+Every structure asserts the byte size the C header (and the reference's
+device layouts, shaders/raycommon.glsl, shaders/wavefront.glsl) fixes.
+"""
+import ctypes as C
+
+RTX_OK = 0
+RTX_E_INVALID = -1
+RTX_E_DEVICE = -2
+RTX_E_MEMORY = -3
+RTX_E_UNSUPPORTED = -4
+RTX_E_STATE = -5
+
+RTX_FRAME_KEEP_PATHS = 0x1
+RTX_FRAME_COUNT_VISITS = 0x2
+RTX_FRAME_RESET_ACCUM = 0x4
+RTX_FRAME_TIME_KERNELS = 0x8
+
+MAT_DIFFUSE, MAT_SPECULAR, MAT_DIELECTRIC, MAT_LIGHT, MAT_PHONG, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR = range(7)
+LIGHT_AREA, LIGHT_POINT, LIGHT_SPHERE, LIGHT_ENV_MAP = range(4)
+
+SCENE_CORNELL = 1
+SCENE_ROOM = 2
+SCENE_PLATES = 3
+
+F3 = C.c_float * 3
+M4 = C.c_float * 16
+
+
+class Vertex(C.Structure):
+    _fields_ = [("pos", F3), ("_pad0", C.c_float), ("normal", F3), ("_pad1", C.c_float),
+                ("texCoord", C.c_float * 2), ("materialIndex", C.c_int32), ("_pad2", C.c_int32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("lightColor", F3), ("_pad0", C.c_float), ("diffuse", F3), ("_pad1", C.c_float),
+                ("specular", F3), ("specularHighlight", C.c_float), ("transparency", C.c_float),
+                ("refractionIndex", C.c_float), ("refractionIndexInv", C.c_float), ("eta", C.c_float),
+                ("k", C.c_float), ("roughness", C.c_float), ("textureIdDiffuse", C.c_int32),
+                ("textureIdSpecular", C.c_int32), ("type", C.c_int32), ("_pad2", C.c_int32 * 3)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("transform", M4), ("normalTransform", M4), ("modelIndex", C.c_int32),
+                ("iLight", C.c_int32), ("_pad", C.c_int32 * 2)]
+
+
+class Light(C.Structure):
+    _fields_ = [("color", F3), ("pos", F3), ("instanceIndex", C.c_uint32), ("sampleProb", C.c_float),
+                ("area", C.c_float), ("type", C.c_int32)]
+
+
+class FaceSample(C.Structure):
+    _fields_ = [("index", C.c_int32), ("sampleProb", C.c_float), ("faceArea", C.c_float)]
+
+
+class Sphere(C.Structure):
+    _fields_ = [("center", F3), ("radius", C.c_float), ("materialIndex", C.c_int32), ("iLight", C.c_int32)]
+
+
+PUSH_CONSTANT_FIELDS = [
+    ("randomUInt", C.c_uint32), ("previousFrames", C.c_uint32), ("maxDepth", C.c_int32),
+    ("maxFollowDiscrete", C.c_int32), ("samplesPerPixel", C.c_int32), ("enableRR", C.c_int32),
+    ("enableNEE", C.c_int32), ("numNEE", C.c_int32), ("enableAverageInsteadOfMix", C.c_int32),
+    ("enableMIS", C.c_int32), ("usePowerHeuristic", C.c_int32), ("storeEstimate", C.c_int32),
+    ("visualizeMode", C.c_int32), ("showIrradianceCacheOnly", C.c_int32),
+    ("showIrradianceGradients", C.c_int32), ("useIrradianceCache", C.c_int32),
+    ("highlightIrradianceCacheColor", C.c_int32), ("irradianceA", C.c_float),
+    ("irradianceUpdateProb", C.c_float), ("irradianceCreateProb", C.c_float),
+    ("irradianceVisualizationScale", C.c_float), ("useIrradianceGradients", C.c_int32),
+    ("useIrradianceCacheOnGlossy", C.c_int32), ("irradianceGradientsMaxLength", C.c_float),
+    ("isIrradiancePrepareFrame", C.c_int32), ("irradianceNumNEE", C.c_int32),
+    ("irradianceCacheMinRadius", C.c_float), ("irradianceCachePerformVisibilityCheck", C.c_int32),
+    ("useVisibleSphereSampling", C.c_int32), ("useADRRS", C.c_int32), ("adrrsS", C.c_float),
+    ("adrrsSplit", C.c_int32), ("splitOnFirst", C.c_int32), ("useGuiding", C.c_int32),
+    ("guidingProb", C.c_float), ("guidingVisuScale", C.c_float), ("guidingVisuMax", C.c_float),
+    ("guidingVisuIgnoreOcclusioon", C.c_int32), ("updateGuiding", C.c_int32),
+    ("useParallaxCompensation", C.c_int32), ("time", C.c_float), ("guidingVisuMove", C.c_int32),
+    ("guidingVisuPhiScale", C.c_float), ("guidingVisuThetaScale", C.c_float),
+    ("numGuidingRegions", C.c_int32), ("guidingPiPHighlightRegion", C.c_int32),
+    ("guidingPiPShowSpheres", C.c_int32), ("guidingPiPSize", C.c_float),
+]
+
+
+class PushConstant(C.Structure):
+    _fields_ = PUSH_CONSTANT_FIELDS
+
+
+class Camera(C.Structure):
+    _fields_ = [("view", M4), ("proj", M4), ("viewInverse", M4), ("projInverse", M4)]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("vertices", C.POINTER(Vertex)), ("vertexCount", C.c_uint32),
+                ("indices", C.POINTER(C.c_uint32)), ("indexCount", C.c_uint32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("models", C.POINTER(ModelDesc)), ("modelCount", C.c_uint32),
+                ("instances", C.POINTER(Instance)), ("instanceCount", C.c_uint32),
+                ("materials", C.POINTER(Material)), ("materialCount", C.c_uint32),
+                ("lights", C.POINTER(Light)), ("lightCount", C.c_uint32),
+                ("randomLightIndex", C.POINTER(C.c_int32)),
+                ("randomTriIndex", C.POINTER(FaceSample)), ("randomTriRows", C.c_uint32),
+                ("spheres", C.POINTER(Sphere)), ("sphereCount", C.c_uint32),
+                ("envMap", C.POINTER(C.c_float)), ("envWidth", C.c_uint32), ("envHeight", C.c_uint32)]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [("pc", PushConstant), ("cam", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("sampleOffset", C.c_uint32), ("seed", C.c_uint64),
+                ("flags", C.c_uint32), ("rrStartDepth", C.c_int32), ("batchPaths", C.c_uint32),
+                ("_reserved", C.c_uint32 * 7)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("extendRays", C.c_uint64), ("shadowRays", C.c_uint64),
+                ("probeRays", C.c_uint64), ("nodeVisits", C.c_uint64), ("primVisits", C.c_uint64),
+                ("msTotal", C.c_double), ("msExtend", C.c_double), ("msShadow", C.c_double),
+                ("msProbe", C.c_double), ("msShade", C.c_double), ("batches", C.c_uint32),
+                ("bounces", C.c_uint32), ("nodeBytes", C.c_uint32), ("primBytes", C.c_uint32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("triangles", C.c_uint32), ("spheres", C.c_uint32), ("nodes", C.c_uint32),
+                ("maxDepth", C.c_uint32), ("deviceBytes", C.c_uint64), ("buildMs", C.c_double)]
+
+
+class Ray(C.Structure):
+    _fields_ = [("org", F3), ("tmin", C.c_float), ("dir", F3), ("tmax", C.c_float)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("prim", C.c_int32), ("u", C.c_float), ("v", C.c_float)]
+
+
+class View(C.Structure):
+    _fields_ = [("origin", F3), ("target", F3), ("up", F3), ("vfov", C.c_float)]
+
+
+EXPECTED_SIZES = {
+    Vertex: 48, Material: 96, Instance: 144, Light: 40, FaceSample: 12, Sphere: 24,
+    PushConstant: 192, Camera: 256, Ray: 32, Hit: 16,
+}
+for _t, _n in EXPECTED_SIZES.items():
+    assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
+
+
+def push_constant_defaults():
+    """The reference's push-constant defaults, src/RayTracingApp.h:81-130."""
+    pc = PushConstant()
+    pc.previousFrames = 0xFFFFFFFF
+    pc.maxDepth = 30
+    pc.maxFollowDiscrete = 3
+    pc.samplesPerPixel = 1
+    pc.enableNEE = 1
+    pc.numNEE = 1
+    pc.usePowerHeuristic = 1
+    pc.irradianceA = 0.2
+    pc.irradianceUpdateProb = 0.00001
+    pc.irradianceCreateProb = 0.0005
+    pc.irradianceVisualizationScale = 1.0
+    pc.irradianceGradientsMaxLength = 5
+    pc.irradianceNumNEE = 1
+    pc.irradianceCacheMinRadius = 0.1
+    pc.adrrsS = 5
+    pc.adrrsSplit = 1
+    pc.guidingProb = 0.5
+    pc.guidingVisuScale = 0.5
+    pc.guidingVisuMax = 1.0
+    pc.useParallaxCompensation = 1
+    pc.guidingVisuPhiScale = 0.004
+    pc.guidingVisuThetaScale = 0.01
+    pc.guidingPiPShowSpheres = 1
+    pc.guidingPiPSize = 0.3
+    return pc
