@@ -1,0 +1,209 @@
+"""Benchmark: Msamples/s of the plain path tracer at 1920x1080, depth 12.
+
+Workload (one step = one full image): the seeded procedural room of 262,144
+triangles (boxes and 24-sided columns in a 20x10x20 m volume, 50% Lambert /
+30% rough conductor / 20% dielectric, 4 area lights), 1920x1080 at 64 spp,
+maxDepth 12, NEE + MIS (power heuristic), FP32, one MI355X.
+
+Prints one JSON line.  `roofline` prices the dominant kernel (closest-hit
+traversal) by algorithmic bytes per ray = 32 (ray) + 16 (hit) + nodes x 64 B
++ triangles x 48 B, with node/triangle fetch counts from a counting pass of
+the identical (deterministic) workload and kernel times from HIP events on
+the library's stream.  `cpu_baseline` times the FP64 CPU restatement of the
+same logic on a pixel subsample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "rtx-pathtracer_amd"), str(ROOT)]
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 64, 48, 32, 16
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--triangles", type=int, default=262144)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-count", action="store_true", help="skip the node/triangle counting pass")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates rocprofv3 byte counters)")
+    a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        n = a.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+        return
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: every GPU box here has one MI355X (asked for --gpus {a.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import rtxamd
+    from rtxamd import abi
+
+    W, H, spp = a.width, a.height, a.spp
+    hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 1, a.triangles)
+    r = rtxamd.Renderer(0)
+    t0 = time.time()
+    ds = r.upload(hs)
+    info = ds.info()
+    log(f"scene: {info.triangles} triangles, {info.nodes} BVH nodes, depth {info.maxDepth}, "
+        f"build {info.buildMs:.0f} ms, upload {time.time() - t0:.2f} s")
+    cam = hs.camera(W, H)
+    pc = dict(maxDepth=12, enableMIS=1, enableNEE=1, usePowerHeuristic=1)
+
+    def params(flags, n_spp=spp):
+        return rtxamd.frame_params(cam, W, H, n_spp, seed=1, flags=flags | abi.RTX_FRAME_RESET_ACCUM, **pc)
+
+    for _ in range(a.warmup):
+        r.render(ds, params(0))
+    log("warmup done")
+    # timed region: K full images; each render ends with a stream synchronize
+    totals = {"ms": 0.0, "ext": 0.0, "sh": 0.0, "pr": 0.0, "shade": 0.0}
+    counts = None
+    t_start = time.perf_counter()
+    for k in range(a.steps):
+        st = r.render(ds, params(abi.RTX_FRAME_TIME_KERNELS))
+        totals["ms"] += st.msTotal
+        totals["ext"] += st.msExtend
+        totals["sh"] += st.msShadow
+        totals["pr"] += st.msProbe
+        totals["shade"] += st.msShade
+        counts = st
+        log(f"step {k}: {st.msTotal:.1f} ms device, {st.paths / st.msTotal / 1e3:.2f} Msamples/s")
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall * 1e3 / a.steps
+    samples = W * H * spp
+    value = samples / (ms_per_step * 1e3)
+
+    # counting pass (same deterministic workload) for the algorithmic bytes
+    cst = r.render(ds, params(abi.RTX_FRAME_COUNT_VISITS)) if not a.no_count else counts
+    ext_rays = cst.extendRays
+    ext_bytes = (ext_rays * (RAY_BYTES + HIT_BYTES) + cst.nodeVisitsByKind[0] * NODE_BYTES +
+                 cst.primVisitsByKind[0] * TRI_BYTES)
+    all_rays = cst.extendRays + cst.shadowRays + cst.probeRays
+    all_bytes = (all_rays * (RAY_BYTES + HIT_BYTES) + sum(cst.nodeVisitsByKind) * NODE_BYTES +
+                 sum(cst.primVisitsByKind) * TRI_BYTES)
+    launches_ext = counts.launchesByKind[0]
+    ext_ms_step = totals["ext"] / a.steps
+    avg_launch_ms = ext_ms_step / max(1, launches_ext)
+    achieved = ext_bytes / launches_ext / (avg_launch_ms * 1e-3) / 1e9
+    trav_ms = (totals["ext"] + totals["sh"] + totals["pr"]) / a.steps
+    traffic = None
+    tfile = ROOT / "profiles" / "extend_traffic_room262k_1080p.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "kernel": "kExtend (closest-hit BVH traversal)",
+        "algorithmic_bytes_per_launch": int(ext_bytes / max(1, launches_ext)),
+        "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_step": launches_ext,
+        "nodes_per_ray": round(cst.nodeVisitsByKind[0] / max(1, ext_rays), 2),
+        "tris_per_ray": round(cst.primVisitsByKind[0] / max(1, ext_rays), 2),
+        "all_traversal_frac": round(all_bytes / (trav_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    mrays = all_rays / (ms_per_step * 1e3)
+
+    # FP32 device vs FP64 oracle on one sample plane, pixel subsample
+    cpu = None
+    parity = None
+    if not a.no_cpu:
+        from oracle.binding import OracleScene, DEFAULT_THREADS
+        orc = OracleScene(abi, hs.desc())
+        fp1 = rtxamd.frame_params(cam, W, H, 1, seed=1,
+                                  flags=abi.RTX_FRAME_RESET_ACCUM | abi.RTX_FRAME_KEEP_PATHS, **pc)
+        r.render(ds, fp1)
+        rgb, sig = r.read_paths(W * H)
+        # calibrate: time a small sample, then size the measured sample to the budget
+        probe_ids = np.arange(0, W * H, 997, dtype=np.uint64)
+        t1 = time.perf_counter()
+        orc.render_paths(fp1, probe_ids, threads=DEFAULT_THREADS)
+        rate = len(probe_ids) / (time.perf_counter() - t1)
+        n = int(min(W * H, max(2000, rate * a.cpu_seconds)))
+        stride = max(1, (W * H) // n)
+        ids = np.arange(0, W * H, stride, dtype=np.uint64)
+        t1 = time.perf_counter()
+        orgb, osig = orc.render_paths(fp1, ids, threads=DEFAULT_THREADS)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": round(len(ids) / cpu_s / 1e6, 5), "unit": "Msamples/s", "cores": DEFAULT_THREADS,
+               "kind": "port",
+               "sample": f"{len(ids)} paths = every {stride}th pixel of sample 0 of the same workload, "
+                         f"FP64 restatement, {DEFAULT_THREADS} threads, {cpu_s:.1f} s"}
+        def stats(x, xs, ref, refs):
+            same = (xs == refs).all(1)
+            rel = np.abs(x - ref).max(1) / np.maximum(np.abs(ref).max(1), 1e-6)
+            return {"paths": int(len(x)), "non_diverged": round(float(same.mean()), 5),
+                    "within_1e-4": round(float((rel[same] <= 1e-4).mean()), 5),
+                    "rel_err_p50": float(np.median(rel[same])), "rel_err_p99": float(np.percentile(rel[same], 99))}
+
+        sub = ids[::8]
+        r32, s32 = orc.render_paths(fp1, sub, prec=32, threads=DEFAULT_THREADS)
+        parity = {"device_vs_fp64": stats(rgb[ids.astype(np.int64)], sig[ids.astype(np.int64)], orgb, osig),
+                  "reference_logic_in_fp32_vs_fp64": stats(r32, s32, orgb[::8], osig[::8]),
+                  "note": "FP32 shading normals on 2-8 cm smooth columns at ~10 m coordinates make ~20% of "
+                          "13-bounce paths branch differently; the CPU FP32 run of the same logic does the same"}
+
+    out = {
+        "metric": "Msamples/s at 1080p depth 12; equal-time RelMSE vs plain; p99 job latency",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene, PCG32 seed 1)",
+        "config": {
+            "workload": f"room {info.triangles} tris, {W}x{H}, {spp} spp, depth 12, NEE+MIS (plain method)",
+            "triangles": info.triangles, "width": W, "height": H, "spp": spp, "max_depth": 12,
+            "nee": True, "mis": "power", "irradiance_cache": "off (not built yet)",
+            "samples_per_step": samples,
+        },
+        "mrays_per_s": round(mrays, 1),
+        "rays_per_step": {"extend": cst.extendRays, "shadow": cst.shadowRays, "probe": cst.probeRays},
+        "ms_device_per_step": {"total": round(totals["ms"] / a.steps, 1), "extend": round(totals["ext"] / a.steps, 1),
+                               "shadow": round(totals["sh"] / a.steps, 1), "probe": round(totals["pr"] / a.steps, 1),
+                               "shade_and_other": round(totals["shade"] / a.steps, 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp32_vs_fp64": parity,
+    }
+    print(json.dumps(out), flush=True)
+    ds.close()
+    r.close()
+
+
+if __name__ == "__main__":
+    main()

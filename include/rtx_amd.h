@@ -331,6 +331,11 @@ typedef struct rtx_frame_stats {
     uint32_t bounces;       /* wavefront iterations of the last batch     */
     uint32_t nodeBytes;     /* bytes per BVH node record fetched          */
     uint32_t primBytes;     /* bytes per triangle record fetched          */
+    /* per traversal kind: [0] closest-hit extend, [1] shadow, [2] probe */
+    uint64_t nodeVisitsByKind[3];
+    uint64_t primVisitsByKind[3];
+    uint32_t launchesByKind[4]; /* kernel launches: extend, shadow, probe, other */
+    uint32_t _pad;
 } rtx_frame_stats;
 
 /* BVH statistics of a scene. */

@@ -47,6 +47,25 @@ int rtxh_scene_load_xml(const char *path, rtxh_scene **out);
 
 void rtxh_scene_destroy(rtxh_scene *scene);
 
+/* Programmatic scene building, in the order SceneLoader adds objects while
+ * parsing (src/SceneLoader.cpp:405-514, 700-744): a mesh becomes a model
+ * plus an instance and, if any face uses a light material, an area light;
+ * a sphere with a light material becomes a sphere light; point lights are
+ * appended (the reference's JSON loader adds them last, after all area
+ * lights, src/SceneLoader.cpp:686-702).  rtxh_scene_finalize builds the
+ * light-sampling tables.  transform / normalTransform may be NULL
+ * (identity). */
+int rtxh_scene_new(rtxh_scene **out);
+int rtxh_scene_add_material(rtxh_scene *scene, const rtx_material *material, int32_t *index);
+int rtxh_scene_add_mesh(rtxh_scene *scene, const rtx_vertex *vertices, uint32_t vertexCount,
+                        const uint32_t *indices, uint32_t indexCount, const float *transform,
+                        const float *normalTransform, int32_t *instance);
+int rtxh_scene_add_sphere(rtxh_scene *scene, const float center[3], float radius, int32_t material,
+                          int32_t *index);
+int rtxh_scene_add_point_light(rtxh_scene *scene, const float pos[3], const float color[3]);
+int rtxh_scene_set_view(rtxh_scene *scene, const rtxh_view *view);
+int rtxh_scene_finalize(rtxh_scene *scene);
+
 /* Pointers stay valid while the scene lives. */
 int rtxh_scene_desc(const rtxh_scene *scene, rtx_scene_desc *desc);
 int rtxh_scene_view(const rtxh_scene *scene, rtxh_view *view);

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.
 """
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -13,6 +14,9 @@ ORACLE_DIR = Path(__file__).resolve().parent
 LIB = ORACLE_DIR / "_build" / "liborcpt.so"
 
 _lib = None
+
+# a GPU box grants 16 host cores per GPU; never use more
+DEFAULT_THREADS = max(1, min(16, os.cpu_count() or 1))
 
 
 def build():
@@ -35,6 +39,8 @@ def lib(abi):
         dp = C.POINTER(C.c_double)
         L.orc_trace_closest.argtypes = [C.c_void_p, C.POINTER(abi.Ray), C.c_size_t, C.c_int, dp,
                                         C.POINTER(C.c_int32), dp, dp]
+        L.orc_trace_closest_prec.argtypes = [C.c_void_p, C.POINTER(abi.Ray), C.c_size_t, C.c_int, C.c_int, dp,
+                                             C.POINTER(C.c_int32), dp, dp]
         L.orc_trace_occluded.argtypes = [C.c_void_p, C.POINTER(abi.Ray), C.c_size_t, C.c_int, C.POINTER(C.c_uint8)]
         L.orc_tea.argtypes = [C.c_uint32, C.c_uint32]
         L.orc_tea.restype = C.c_uint32
@@ -92,7 +98,8 @@ class OracleScene:
         except Exception:
             pass
 
-    def render_paths(self, params, ids, prec=64, threads=0):
+    def render_paths(self, params, ids, prec=64, threads=None):
+        threads = DEFAULT_THREADS if threads is None else threads
         ids = np.ascontiguousarray(ids, np.uint64)
         n = ids.size
         rgb = np.empty((n, 3), np.float64)
@@ -104,7 +111,7 @@ class OracleScene:
             raise RuntimeError(f"oracle render: {self.L.orc_last_error().decode()}")
         return rgb, sig
 
-    def trace_closest(self, rays, brute=False):
+    def trace_closest(self, rays, brute=False, prec=64):
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
         t = np.empty(n, np.float64)
@@ -112,8 +119,8 @@ class OracleScene:
         u = np.empty(n, np.float64)
         v = np.empty(n, np.float64)
         dp = C.POINTER(C.c_double)
-        rc = self.L.orc_trace_closest(self._h, rays.ctypes.data_as(C.POINTER(self.abi.Ray)), n, int(brute),
-                                      t.ctypes.data_as(dp), prim.ctypes.data_as(C.POINTER(C.c_int32)),
+        rc = self.L.orc_trace_closest_prec(self._h, rays.ctypes.data_as(C.POINTER(self.abi.Ray)), n, int(brute),
+                                      prec, t.ctypes.data_as(dp), prim.ctypes.data_as(C.POINTER(C.c_int32)),
                                       u.ctypes.data_as(dp), v.ctypes.data_as(dp))
         if rc != 0:
             raise RuntimeError(self.L.orc_last_error().decode())

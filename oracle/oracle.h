@@ -41,6 +41,9 @@ int orc_render_paths(const orc_scene *scene, const rtx_frame_params *params, con
  * brute != 0 tests every primitive instead of walking the BVH. */
 int orc_trace_closest(const orc_scene *scene, const rtx_ray *rays, size_t n, int brute,
                       double *t, int32_t *prim, double *u, double *v);
+/* The same with the traversal arithmetic in FP32 (prec = 32) or FP64 (64). */
+int orc_trace_closest_prec(const orc_scene *scene, const rtx_ray *rays, size_t n, int brute, int prec,
+                           double *t, int32_t *prim, double *u, double *v);
 int orc_trace_occluded(const orc_scene *scene, const rtx_ray *rays, size_t n, int brute,
                        uint8_t *occluded);
 

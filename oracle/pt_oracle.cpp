@@ -107,6 +107,14 @@ uint32_t pcg32First(uint64_t initstate, uint64_t initseq) { // O'Neill 2014, pcg
 // guarantees termination and never changes a result.
 constexpr int kMaxRejections = 65536;
 
+// Path signature: FNV-1a-style fold of every discrete traversal outcome in
+// path order (extend-ray primitive ids, shadow-ray visibility, probe-ray
+// primitive ids).  Two runs of one path with equal signatures and equal
+// final LCG states took the same branches everywhere.
+constexpr uint32_t kSigVisible = 0x1b873593u;
+constexpr uint32_t kSigOccluded = 0x5bd1e995u;
+constexpr uint32_t kSigProbe = 0x85ebca6bu;
+
 // ------------------------------------------------------------------ scene
 struct Node {
     double lo[3], hi[3];
@@ -803,8 +811,11 @@ template <class R> struct Tracer {
         R pdfLights = sampleLights(origin, normal, lightDir, lightColor, lightDistance);
         bool isShadowed = true;
         R cosThetaLight = dot(normal, lightDir);
-        if (cosThetaLight > R(0) && pdfLights > R(0))
+        if (cosThetaLight > R(0) && pdfLights > R(0)) {
             isShadowed = occluded<R>(S, origin, lightDir, C::TMIN, lightDistance * (R(1) - R(0.0001)), false);
+            // the outcome of every traced shadow ray joins the path signature
+            primHash = (primHash ^ (isShadowed ? kSigOccluded : kSigVisible)) * 16777619u;
+        }
         if (!isShadowed) {
             if (pc.enableMIS) {
                 R pdfMat = pdfBSDF(m, normal, wi, lightDir);
@@ -820,6 +831,7 @@ template <class R> struct Tracer {
             R pdfMat = sampleBSDF(m, wi, normal, frontFace, bsdfDir);
             if (pdfMat > R(0)) {
                 HitInfo<R> pinfo = trace(origin, bsdfDir, C::TMIN, C::TMAX);
+                primHash = (primHash ^ (kSigProbe ^ (uint32_t)pinfo.prim)) * 16777619u;
                 if (pinfo.isMiss) {
                     lightColor = pinfo.missColor;
                     pdfLights = C::INV_2PI / (R)S.lights.size();
@@ -892,8 +904,11 @@ template <class R> struct Tracer {
             }
             V3<R> newDirection;
             R pdf = sampleBSDF(m, -direction, normal, info.isFrontFace, newDirection);
-            if (pdf <= R(0)) break;
+            // The reference stops on pdf <= 0 only; a NaN pdf (pdfBSDF of wo == -wi
+            // normalises a zero half-vector) would poison the pixel, so it ends the path.
+            if (pdf <= R(0) || std::isnan(pdf)) break;
             V3<R> change = evalBsdf(m, normal, -direction, newDirection, info.isFrontFace) / pdf;
+            if (std::isnan(change.x) || std::isnan(change.y) || std::isnan(change.z)) break;
             throughput = throughput * change;
             direction = newDirection;
             if (pc.enableRR && depth >= fp.rrStartDepth) {
@@ -962,7 +977,7 @@ int checkParams(const rtx_frame_params *fp) {
 }
 
 template <class F> void parallelFor(size_t n, int threads, F fn) {
-    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if (threads <= 0) threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     std::atomic<size_t> next{0};
     const size_t chunk = 64;
     auto worker = [&]() {
@@ -1093,6 +1108,11 @@ int orc_render_paths(const orc_scene *s, const rtx_frame_params *fp, const uint6
 
 int orc_trace_closest(const orc_scene *s, const rtx_ray *rays, size_t n, int brute, double *t, int32_t *prim,
                       double *u, double *v) {
+    return orc_trace_closest_prec(s, rays, n, brute, 64, t, prim, u, v);
+}
+
+int orc_trace_closest_prec(const orc_scene *s, const rtx_ray *rays, size_t n, int brute, int prec, double *t,
+                           int32_t *prim, double *u, double *v) {
     if (!s || (!rays && n)) {
         g_err = "NULL argument";
         return RTX_E_INVALID;
@@ -1100,8 +1120,9 @@ int orc_trace_closest(const orc_scene *s, const rtx_ray *rays, size_t n, int bru
     parallelFor(n, 0, [&](size_t i) {
         const rtx_ray &r = rays[i];
         HitRec h;
-        bool hit = closest<double>(*s, f3<double>(r.org), f3<double>(r.dir), (double)r.tmin, (double)r.tmax,
-                                   brute != 0, h);
+        bool hit = prec == 32 ? closest<float>(*s, f3<float>(r.org), f3<float>(r.dir), r.tmin, r.tmax, brute != 0, h)
+                              : closest<double>(*s, f3<double>(r.org), f3<double>(r.dir), (double)r.tmin,
+                                                (double)r.tmax, brute != 0, h);
         if (t) t[i] = hit ? h.t : (double)r.tmax;
         if (prim) prim[i] = hit ? h.prim : -1;
         if (u) u[i] = hit ? h.u : 0.0;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -510,6 +511,9 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.rrStartDepth = p->rrStartDepth;
     fc.numNEE = numNEE;
     fc.countVisits = (p->flags & RTX_FRAME_COUNT_VISITS) ? 1 : 0;
+    // developer aid: RTX_DEBUG_PATH=<path index within this call> prints that path's bounces
+    fc.debugPath = ~0ull;
+    if (const char *dbg = std::getenv("RTX_DEBUG_PATH")) fc.debugPath = std::strtoull(dbg, nullptr, 10);
     CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));
     CHECK_HIP(ctx, hipMemsetAsync(ctx->cnt, 0, sizeof(Counters), ctx->stream));
 
@@ -522,7 +526,9 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     const DevScene &S = scene->view;
     hipStream_t s = ctx->stream;
 
+    uint32_t launches[4] = {0, 0, 0, 0};
     auto timed = [&](int cls, auto &&launch) {
+        launches[cls]++;
         EventPair *ev = timeKernels ? nextEvents(ctx, cls) : nullptr;
         if (ev) (void)hipEventRecord(ev->a, s);
         launch();
@@ -569,8 +575,13 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         stats->extendRays = c.extendRays;
         stats->shadowRays = c.shadowRays;
         stats->probeRays = c.probeRays;
-        stats->nodeVisits = c.nodeVisits;
-        stats->primVisits = c.primVisits;
+        for (int k = 0; k < 3; ++k) {
+            stats->nodeVisitsByKind[k] = c.nodeVisits[k];
+            stats->primVisitsByKind[k] = c.primVisits[k];
+            stats->nodeVisits += c.nodeVisits[k];
+            stats->primVisits += c.primVisits[k];
+        }
+        for (int k = 0; k < 4; ++k) stats->launchesByKind[k] = launches[k];
         float ms = 0;
         CHECK_HIP(ctx, hipEventElapsedTime(&ms, ctx->tStart, ctx->tStop));
         stats->msTotal = ms;

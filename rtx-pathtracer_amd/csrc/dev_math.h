@@ -56,6 +56,15 @@ constexpr float kTMin = 0.001f;     // shaders/raytrace.rgen:52
 constexpr float kTMax = 1000000.0f; // shaders/raytrace.rgen:53
 constexpr int kMaxRejections = 65536; // bound of the rejection loops (never reached in practice)
 
+// Path signature: a fold of every discrete traversal outcome in path order
+// (extend-ray primitive ids, shadow-ray visibility, probe-ray primitive
+// ids), returned with the final LCG state for per-path parity checks.
+constexpr uint32_t kSigPrime = 16777619u;
+constexpr uint32_t kSigVisible = 0x1b873593u;
+constexpr uint32_t kSigOccluded = 0x5bd1e995u;
+constexpr uint32_t kSigProbe = 0x85ebca6bu;
+RTX_DI uint32_t sigFold(uint32_t h, uint32_t v) { return (h ^ v) * kSigPrime; }
+
 // ------------------------------------------------------------------ RNG
 // tea: 16 rounds of the Tiny Encryption Algorithm (shaders/random.glsl:13-27)
 RTX_DI uint32_t tea(uint32_t val0, uint32_t val1) {

@@ -94,17 +94,19 @@ RTX_DI float fresnelConductor(float cosThetaI, float eta, float k) { // shaders/
     float Rp2 = (ek - 2.0f * eta * cosThetaI + c2) / (ek + 2.0f * eta * cosThetaI + c2);
     return (Rs2 + Rp2) / 2.0f;
 }
-// Beckmann D (shaders/raytrace.rgen:179-202); pow(E, x) written as exp(x)
+// Beckmann D (shaders/raytrace.rgen:179-202); pow(E, x) written as exp(x).
+// The reference forms tan(acos(cos)); here tan^2 = |n x m|^2 / (n.m)^2,
+// the same quantity without acos/tan's cancellation near the pole, which
+// matters in FP32 at small alpha (exp(-tan^2 / alpha^2) magnifies errors
+// in tan^2 by 1/alpha^2).  The reference's guards reduce to tan^2 >= 0.
 RTX_DI float beckmannD(float alpha, V3 n, V3 m) {
     float cosTheta = dot(n, m);
     if (cosTheta <= 0.0f) return 0.0f;
-    float theta = acosf(cosTheta);
-    if (isnan(theta) || isinf(theta)) theta = 0.0f;
-    float tanTheta = tanf(theta);
-    if (isnan(tanTheta) || isinf(tanTheta)) tanTheta = 0.0f;
+    V3 x = cross(n, m);
+    float c2 = cosTheta * cosTheta;
+    float tan2 = fmaxf(dot(x, x) / c2, 0.0f);
     float alphaSqr = alpha * alpha;
-    float c4 = cosTheta * cosTheta * cosTheta * cosTheta;
-    return expf(-tanTheta * tanTheta / alphaSqr) / (kPi * alphaSqr * c4);
+    return expf(-tan2 / alphaSqr) / (kPi * alphaSqr * (c2 * c2));
 }
 // G1 with the reference's two typos kept (tan of a cosine; 2.577*a for
 // 2.577*a^2), shaders/raytrace.rgen:204-222

@@ -50,7 +50,8 @@ struct Counters {
     uint32_t probe;
     uint32_t done;
     uint32_t pad[3];
-    unsigned long long extendRays, shadowRays, probeRays, nodeVisits, primVisits;
+    unsigned long long extendRays, shadowRays, probeRays;
+    unsigned long long nodeVisits[3], primVisits[3]; // extend, shadow, probe
 };
 
 struct FrameConst {
@@ -62,6 +63,7 @@ struct FrameConst {
     int32_t rrStartDepth;
     uint32_t numNEE;         // NEE samples per vertex (>= 1)
     uint32_t countVisits;
+    uint64_t debugPath;      // path whose bounces are printed (RTX_DEBUG_PATH), ~0 = none
 };
 
 struct BatchDesc {

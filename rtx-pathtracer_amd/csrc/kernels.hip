@@ -46,12 +46,12 @@ RTX_DI uint32_t waveSum(uint32_t v) {
     return v;
 }
 
-template <bool COUNT> RTX_DI void flushVisits(const VisitCount &vc, Counters *cnt) {
+template <bool COUNT> RTX_DI void flushVisits(const VisitCount &vc, Counters *cnt, int kind) {
     if (!COUNT) return;
     uint32_t n = waveSum(vc.nodes), p = waveSum(vc.prims);
     if (__lane_id() == 0) {
-        atomicAdd(&cnt->nodeVisits, (unsigned long long)n);
-        atomicAdd(&cnt->primVisits, (unsigned long long)p);
+        atomicAdd(&cnt->nodeVisits[kind], (unsigned long long)n);
+        atomicAdd(&cnt->primVisits[kind], (unsigned long long)p);
     }
 }
 
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(DevScene S, Queue q, Coun
         int prim = traverse<false, COUNT>(S, r, o.w, tmax, bu, bv, st, vc);
         q.aux[i] = make_float4(tmax, __int_as_float(prim), bu, bv);
     }
-    flushVisits<COUNT>(vc, cnt);
+    flushVisits<COUNT>(vc, cnt, 0);
 }
 
 // --------------------------------------------------------------- shadow
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(DevScene S, Queue q, Path
         int prim = traverse<true, COUNT>(S, r, o.w, tmax, bu, bv, st, vc);
         if (prim < 0) pb.pendShadow[q.id[i]].w = 1.0f; // unoccluded: raytrace.shadow.rmiss:7-10
     }
-    flushVisits<COUNT>(vc, cnt);
+    flushVisits<COUNT>(vc, cnt, 1);
 }
 
 // ---------------------------------------------------------------- probe
@@ -166,17 +166,24 @@ __global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, const FrameCon
                 if (!isnan(h)) contrib = v3(base) * v3(ms.lightColor) * h;
             }
         }
-        pb.pendProbe[q.id[i]] = f4(contrib, 0.0f);
+        pb.pendProbe[q.id[i]] = f4(contrib, __int_as_float(prim));
     }
-    flushVisits<COUNT>(vc, cnt);
+    flushVisits<COUNT>(vc, cnt, 2);
 }
 
-RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE) {
+// Pending NEE terms of the previous vertex, in NEE-sample order.
+// pendShadow.w: 0 no shadow ray, -1 traced and occluded, +1 traced and visible.
+// pendProbe.w (int bits): -2 no probe ray, else the probe's hit id (-1 = miss).
+RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint32_t &hash) {
     V3 s = v3(0.0f);
     for (uint32_t j = 0; j < numNEE; ++j) {
         float4 sh = pb.pendShadow[pid * numNEE + j];
-        V3 term = v3(pb.pendProbe[pid * numNEE + j]);
-        if (sh.w != 0.0f) term = v3(sh) + term;
+        float4 pr = pb.pendProbe[pid * numNEE + j];
+        if (sh.w != 0.0f) hash = sigFold(hash, sh.w > 0.0f ? kSigVisible : kSigOccluded);
+        int probePrim = __float_as_int(pr.w);
+        if (probePrim != -2) hash = sigFold(hash, kSigProbe ^ (uint32_t)probePrim);
+        V3 term = v3(pr);
+        if (sh.w > 0.0f) term = v3(sh) + term;
         s = s + term;
     }
     return s;
@@ -222,9 +229,9 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
             flags = __float_as_uint(t4.w);
             res = v3(pb.result[pid]);
             hash = pb.hash[pid];
-            if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE);
+            if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE, hash);
             int prim = __float_as_int(hit.y);
-            hash = (hash ^ (uint32_t)prim) * 16777619u;
+            hash = sigFold(hash, (uint32_t)prim);
             depth = (int)(flags & kFlagDepthMask) + 1;
             followCount = (int)((flags >> kFlagFollowCountShift) & 0xFFu);
             follow = (flags & kFlagFollow) != 0;
@@ -236,6 +243,10 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
             } else {
                 h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
                 m = loadMat(&S.materials[h.matIndex]);
+                if (b.firstPath + pid == fc.debugPath)
+                    printf("[gpu] depth %d prim %d t %.9g mat %d seed %u thr %.6g %.6g %.6g d %.6g %.6g %.6g n %.6g %.6g %.6g ff %d res %.6g\n",
+                           depth, prim, hit.x, m.type, rng.s, thr.x, thr.y, thr.z, d.x, d.y, d.z, h.normal.x,
+                           h.normal.y, h.normal.z, (int)h.frontFace, res.x);
                 if (m.type == RTX_MAT_LIGHT && (addDirect || addNext)) res = res + thr * m.lightColor;
                 if (hasDiscreteDirection(m.type)) {
                     addNext = true;
@@ -261,6 +272,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
                 float cosS = dot(h.normal, lightDir);
                 bool traced = cosS > 0.0f && pdfL > 0.0f;
                 bool probeAllowed = true;
+                float shadowState = traced ? -1.0f : 0.0f; // occluded until the shadow kernel says otherwise
                 V3 sh = v3(0.0f);
                 if (traced) {
                     if (mis) {
@@ -276,6 +288,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
                             float tm = lightDist * (1.0f - 0.0001f), bu, bv;
                             bool occl = traverse<true, false>(S, sr, kTMin, tm, bu, bv, ps, vc0) >= 0;
                             traced = false;
+                            shadowState = occl ? -1.0f : 1.0f; // resolved here; contributes nothing
                             if (!occl) probeAllowed = false;
                         } else {
                             sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor * hw / pdfL;
@@ -284,7 +297,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
                         sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor / pdfL;
                     }
                 }
-                pb.pendShadow[slot] = f4(thr * sh * invNEE, 0.0f);
+                pb.pendShadow[slot] = f4(thr * sh * invNEE, shadowState);
                 wantShadow = traced;
                 bool probeZero = true;
                 if (mis && probeAllowed) {
@@ -296,7 +309,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
                         probeBase = thr * evalBsdf(m, h.normal, wi, bsdfDir, true) * invNEE / pdfMat;
                     }
                 }
-                if (probeZero) pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (probeZero) pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-2));
             }
             int si = waveAppend(wantShadow, &cnt->shadow);
             if (wantShadow) {
@@ -316,10 +329,17 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
         V3 newDir = v3(0.0f);
         if (active && !isDone) {
             float pdf = sampleBSDF(rng, m, wi, h.normal, h.frontFace, newDir);
-            if (pdf <= 0.0f) {
+            if (b.firstPath + pid == fc.debugPath) {
+                V3 ev = evalBsdf(m, h.normal, wi, newDir, h.frontFace);
+                printf("[gpu]   continue pdf %.9g eval %.9g new %.6g %.6g %.6g\n", pdf, ev.x, newDir.x, newDir.y, newDir.z);
+            }
+            // NaN guard beyond the reference (which stops on pdf <= 0 only): a NaN
+            // pdf or BSDF ratio ends the path instead of poisoning the pixel.
+            V3 change = v3(0.0f);
+            if (!(pdf <= 0.0f) && !isnan(pdf)) change = evalBsdf(m, h.normal, wi, newDir, h.frontFace) / pdf;
+            if (pdf <= 0.0f || isnan(pdf) || isnan(change.x) || isnan(change.y) || isnan(change.z)) {
                 isDone = true;
             } else {
-                V3 change = evalBsdf(m, h.normal, wi, newDir, h.frontFace) / pdf;
                 thr = thr * change;
                 bool killed = false;
                 if (fc.pc.enableRR && depth >= fc.rrStartDepth) { // Russian roulette (absent from the reference)
@@ -365,9 +385,10 @@ __global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fc
         uint32_t pid = (uint32_t)done[i];
         V3 res = v3(pb.result[pid]);
         uint32_t flags = __float_as_uint(pb.thr[pid].w);
-        if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE);
+        uint32_t hash = pb.hash[pid];
+        if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE, hash);
         pb.out[pid] = f4(res, 0.0f);
-        pb.outSig[pid] = make_uint2(pb.rng[pid], pb.hash[pid]);
+        pb.outSig[pid] = make_uint2(pb.rng[pid], hash);
     }
 }
 

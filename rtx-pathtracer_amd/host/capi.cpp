@@ -66,6 +66,62 @@ int rtxh_scene_load_xml(const char *path, rtxh_scene **out) {
 
 void rtxh_scene_destroy(rtxh_scene *scene) { delete scene; }
 
+int rtxh_scene_new(rtxh_scene **out) {
+    if (!out) return fail("out is NULL");
+    *out = new (std::nothrow) rtxh_scene();
+    return *out ? RTX_OK : fail("out of memory");
+}
+
+int rtxh_scene_add_material(rtxh_scene *scene, const rtx_material *m, int32_t *index) {
+    if (!scene || !m) return fail("NULL argument");
+    int i = scene->scene.addMaterial(*m);
+    if (index) *index = i;
+    return RTX_OK;
+}
+
+int rtxh_scene_add_mesh(rtxh_scene *scene, const rtx_vertex *v, uint32_t nv, const uint32_t *idx, uint32_t ni,
+                        const float *transform, const float *normalTransform, int32_t *instance) {
+    if (!scene || (nv && !v) || (ni && !idx)) return fail("NULL argument");
+    if (ni % 3) return fail("index count must be a multiple of 3");
+    rtxh::Model m;
+    m.vertices.assign(v, v + nv);
+    m.indices.assign(idx, idx + ni);
+    for (uint32_t i = 0; i < ni; ++i)
+        if (idx[i] >= nv) return fail("vertex index out of range");
+    for (uint32_t i = 0; i < nv; ++i)
+        if (v[i].materialIndex < 0 || v[i].materialIndex >= (int)scene->scene.materials.size())
+            return fail("vertex material index out of range");
+    int inst = scene->scene.addModelInstance(std::move(m), transform, normalTransform);
+    if (instance) *instance = inst;
+    return RTX_OK;
+}
+
+int rtxh_scene_add_sphere(rtxh_scene *scene, const float c[3], float radius, int32_t material, int32_t *index) {
+    if (!scene || !c) return fail("NULL argument");
+    if (material < 0 || material >= (int)scene->scene.materials.size()) return fail("material index out of range");
+    int i = scene->scene.addSphere(rtxh::Vec3(c[0], c[1], c[2]), radius, material);
+    if (index) *index = i;
+    return RTX_OK;
+}
+
+int rtxh_scene_add_point_light(rtxh_scene *scene, const float p[3], const float col[3]) {
+    if (!scene || !p || !col) return fail("NULL argument");
+    scene->scene.addPointLight(rtxh::Vec3(p[0], p[1], p[2]), rtxh::Vec3(col[0], col[1], col[2]));
+    return RTX_OK;
+}
+
+int rtxh_scene_set_view(rtxh_scene *scene, const rtxh_view *view) {
+    if (!scene || !view) return fail("NULL argument");
+    scene->scene.view = *view;
+    return RTX_OK;
+}
+
+int rtxh_scene_finalize(rtxh_scene *scene) {
+    if (!scene) return fail("NULL argument");
+    scene->scene.buildLightTables();
+    return RTX_OK;
+}
+
 int rtxh_scene_desc(const rtxh_scene *scene, rtx_scene_desc *desc) {
     if (!scene || !desc) return fail("NULL argument");
     const_cast<rtxh_scene *>(scene)->scene.fillDesc(desc);

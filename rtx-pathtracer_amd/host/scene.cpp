@@ -64,12 +64,18 @@ float Model::faceArea(int face, const Mat4 &transform) const {
     return l01 * l02 * sinAlpha / 2.0f;
 }
 
+static void set3(float dst[3], Vec3 v) {
+    dst[0] = v.x;
+    dst[1] = v.y;
+    dst[2] = v.z;
+}
+
 static void identityInto(float m[16]) {
     Mat4 id = Mat4::identity();
     std::memcpy(m, id.m, sizeof(id.m));
 }
 
-int Scene::addModelInstance(Model &&model) {
+int Scene::addModelInstance(Model &&model, const float *transform, const float *normalTransform) {
     // emissive faces: faces whose material is a light (src/SceneLoader.cpp:320-323)
     model.emissiveFaces.clear();
     for (size_t f = 0; f * 3 < model.indices.size(); ++f) {
@@ -82,6 +88,8 @@ int Scene::addModelInstance(Model &&model) {
     std::memset(&inst, 0, sizeof(inst));
     identityInto(inst.transform);
     identityInto(inst.normalTransform);
+    if (transform) std::memcpy(inst.transform, transform, sizeof(inst.transform));
+    if (normalTransform) std::memcpy(inst.normalTransform, normalTransform, sizeof(inst.normalTransform));
     inst.modelIndex = (int)models.size() - 1;
     inst.iLight = -1;
     int instanceIndex = (int)instances.size();
@@ -95,6 +103,15 @@ int Scene::addModelInstance(Model &&model) {
     }
     instances.push_back(inst);
     return instanceIndex;
+}
+
+void Scene::addPointLight(Vec3 pos, Vec3 color) { // parsePointLights, src/SceneLoader.cpp:686-702
+    rtx_light light;
+    std::memset(&light, 0, sizeof(light));
+    light.type = RTX_LIGHT_POINT;
+    set3(light.color, color);
+    set3(light.pos, pos);
+    lights.push_back(light);
 }
 
 int Scene::addSphere(Vec3 center, float radius, int material) {
@@ -233,11 +250,6 @@ static rtx_material blankMaterial() {
     return m;
 }
 
-static void set3(float dst[3], Vec3 v) {
-    dst[0] = v.x;
-    dst[1] = v.y;
-    dst[2] = v.z;
-}
 
 // parseXmlBSDF (src/SceneLoader.cpp:558-611) material mappings.
 rtx_material makeDiffuse(Vec3 albedo) {

@@ -69,7 +69,8 @@ struct Scene {
     // Append a model as its own instance (identity transform) the way the
     // Mitsuba path adds an "obj" shape (src/SceneLoader.cpp:455-495).
     // Returns the instance index; creates an area light if any face is emissive.
-    int addModelInstance(Model &&model);
+    int addModelInstance(Model &&model, const float *transform = nullptr, const float *normalTransform = nullptr);
+    void addPointLight(Vec3 pos, Vec3 color);
     int addSphere(Vec3 center, float radius, int material);
     int addMaterial(const rtx_material &m);
 

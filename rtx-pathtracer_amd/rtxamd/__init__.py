@@ -54,6 +54,16 @@ def host_lib():
                                     C.POINTER(C.c_uint32)]
         L.rtxh_free.argtypes = [C.c_void_p]
         L.rtxh_free.restype = None
+        L.rtxh_scene_new.argtypes = [C.POINTER(C.c_void_p)]
+        L.rtxh_scene_add_material.argtypes = [C.c_void_p, C.POINTER(abi.Material), C.POINTER(C.c_int32)]
+        L.rtxh_scene_add_mesh.argtypes = [C.c_void_p, C.POINTER(abi.Vertex), C.c_uint32, C.POINTER(C.c_uint32),
+                                          C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_int32)]
+        L.rtxh_scene_add_sphere.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_int32,
+                                            C.POINTER(C.c_int32)]
+        L.rtxh_scene_add_point_light.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.rtxh_scene_set_view.argtypes = [C.c_void_p, C.POINTER(View)]
+        L.rtxh_scene_finalize.argtypes = [C.c_void_p]
         L.rtxh_pcg32_first.argtypes = [C.c_uint64, C.c_uint64]
         L.rtxh_pcg32_first.restype = C.c_uint32
         _host = L
@@ -100,6 +110,67 @@ class HostScene:
         h = C.c_void_p()
         _host_check(host_lib().rtxh_scene_generate(int(kind), int(seed), int(triangles), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def new(cls):
+        h = C.c_void_p()
+        _host_check(host_lib().rtxh_scene_new(C.byref(h)))
+        return cls(h)
+
+    def add_material(self, mat):
+        i = C.c_int32()
+        _host_check(host_lib().rtxh_scene_add_material(self._h, C.byref(mat), C.byref(i)))
+        return i.value
+
+    def add_mesh(self, positions, indices, materials, normals=None, transform=None, normal_transform=None):
+        """positions (n,3), indices (m,3), materials: per-vertex or scalar material index."""
+        positions = np.asarray(positions, np.float32).reshape(-1, 3)
+        indices = np.ascontiguousarray(np.asarray(indices, np.uint32).reshape(-1))
+        n = positions.shape[0]
+        verts = (abi.Vertex * n)()
+        if normals is None:
+            normals = np.zeros_like(positions)
+            tri = indices.reshape(-1, 3)
+            fn = np.cross(positions[tri[:, 1]] - positions[tri[:, 0]], positions[tri[:, 2]] - positions[tri[:, 0]])
+            fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+            for k in range(3):
+                normals[tri[:, k]] = fn
+        normals = np.asarray(normals, np.float32).reshape(-1, 3)
+        mats = np.broadcast_to(np.asarray(materials, np.int32), (n,))
+        for i in range(n):
+            verts[i].pos[:] = positions[i].tolist()
+            verts[i].normal[:] = normals[i].tolist()
+            verts[i].materialIndex = int(mats[i])
+        fp = C.POINTER(C.c_float)
+        tr = (C.c_float * 16)(*np.asarray(transform, np.float32).reshape(16, order="F")) if transform is not None else None
+        ntr = (C.c_float * 16)(*np.asarray(normal_transform, np.float32).reshape(16, order="F")) \
+            if normal_transform is not None else None
+        inst = C.c_int32()
+        _host_check(host_lib().rtxh_scene_add_mesh(
+            self._h, verts, n, indices.ctypes.data_as(C.POINTER(C.c_uint32)), indices.size,
+            C.cast(tr, fp) if tr is not None else None, C.cast(ntr, fp) if ntr is not None else None, C.byref(inst)))
+        return inst.value
+
+    def add_sphere(self, center, radius, material):
+        c = (C.c_float * 3)(*center)
+        i = C.c_int32()
+        _host_check(host_lib().rtxh_scene_add_sphere(self._h, c, float(radius), int(material), C.byref(i)))
+        return i.value
+
+    def add_point_light(self, pos, color):
+        _host_check(host_lib().rtxh_scene_add_point_light(self._h, (C.c_float * 3)(*pos), (C.c_float * 3)(*color)))
+
+    def set_view(self, origin, target, up, vfov):
+        v = View()
+        v.origin[:] = list(origin)
+        v.target[:] = list(target)
+        v.up[:] = list(up)
+        v.vfov = vfov
+        _host_check(host_lib().rtxh_scene_set_view(self._h, C.byref(v)))
+
+    def finalize(self):
+        _host_check(host_lib().rtxh_scene_finalize(self._h))
+        return self
 
     @classmethod
     def load_xml(cls, path):
@@ -160,6 +231,48 @@ def frame_params(cam, width, height, spp, seed=1, sample_offset=0, flags=0, rr_s
     fp.rrStartDepth = rr_start_depth
     fp.batchPaths = batch_paths
     return fp
+
+
+def _material(mtype, **kw):
+    m = abi.Material()
+    m.textureIdDiffuse = -1
+    m.textureIdSpecular = -1
+    m.type = mtype
+    for k, v in kw.items():
+        if isinstance(v, (list, tuple)):
+            getattr(m, k)[:] = [float(x) for x in v]
+        else:
+            setattr(m, k, v)
+    return m
+
+
+def diffuse(albedo):
+    return _material(abi.MAT_DIFFUSE, diffuse=albedo)
+
+
+def light(radiance, albedo=(0.0, 0.0, 0.0)):
+    return _material(abi.MAT_LIGHT, diffuse=albedo, lightColor=radiance)
+
+
+def dielectric(ior):
+    """parseXmlBSDF 'dielectric', src/SceneLoader.cpp:587-591."""
+    return _material(abi.MAT_DIELECTRIC, specular=(1, 1, 1), refractionIndex=ior, refractionIndexInv=1.0 / ior)
+
+
+def rough_conductor(alpha, eta, k):
+    return _material(abi.MAT_ROUGH_CONDUCTOR, roughness=alpha, eta=eta, k=k)
+
+
+def conductor(eta, k):
+    return _material(abi.MAT_CONDUCTOR, eta=eta, k=k)
+
+
+def mirror():
+    return _material(abi.MAT_SPECULAR, specular=(1, 1, 1))
+
+
+def phong(kd, ks, exponent):
+    return _material(abi.MAT_PHONG, diffuse=kd, specular=ks, specularHighlight=exponent)
 
 
 def write_exr(path, rgba):

@@ -119,7 +119,9 @@ class FrameStats(C.Structure):
                 ("probeRays", C.c_uint64), ("nodeVisits", C.c_uint64), ("primVisits", C.c_uint64),
                 ("msTotal", C.c_double), ("msExtend", C.c_double), ("msShadow", C.c_double),
                 ("msProbe", C.c_double), ("msShade", C.c_double), ("batches", C.c_uint32),
-                ("bounces", C.c_uint32), ("nodeBytes", C.c_uint32), ("primBytes", C.c_uint32)]
+                ("bounces", C.c_uint32), ("nodeBytes", C.c_uint32), ("primBytes", C.c_uint32),
+                ("nodeVisitsByKind", C.c_uint64 * 3), ("primVisitsByKind", C.c_uint64 * 3),
+                ("launchesByKind", C.c_uint32 * 4), ("_pad", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):

@@ -1,0 +1,96 @@
+"""Scenes built through the host scene API for tests (helpers, not tests)."""
+import numpy as np
+
+import rtxamd
+from rtxamd import abi
+
+
+def quad(p0, p1, p2, p3):
+    return np.array([p0, p1, p2, p3], np.float32), np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+def box(lo, hi):
+    lo, hi = np.asarray(lo, np.float32), np.asarray(hi, np.float32)
+    c = lambda x, y, z: [(lo, hi)[x][0], (lo, hi)[y][1], (lo, hi)[z][2]]
+    faces = [
+        [c(0, 0, 0), c(0, 0, 1), c(0, 1, 1), c(0, 1, 0)], [c(1, 0, 0), c(1, 1, 0), c(1, 1, 1), c(1, 0, 1)],
+        [c(0, 0, 0), c(1, 0, 0), c(1, 0, 1), c(0, 0, 1)], [c(0, 1, 0), c(0, 1, 1), c(1, 1, 1), c(1, 1, 0)],
+        [c(0, 0, 0), c(0, 1, 0), c(1, 1, 0), c(1, 0, 0)], [c(0, 0, 1), c(1, 0, 1), c(1, 1, 1), c(0, 1, 1)],
+    ]
+    pos = np.array([p for f in faces for p in f], np.float32)
+    idx = np.array([[4 * i, 4 * i + 1, 4 * i + 2, 4 * i, 4 * i + 2, 4 * i + 3] for i in range(6)], np.uint32)
+    return pos, idx.reshape(-1, 3)
+
+
+def material_zoo(point_light=True):
+    """Cornell-like box exercising every material type and light type:
+    Lambert, Phong, smooth conductor, mirror, rough conductor, dielectric and
+    light materials; area, sphere and point lights (area lights first).
+    Point lights are reachable by NEE only, so estimator-equivalence tests
+    leave them out (point_light=False)."""
+    s = rtxamd.HostScene.new()
+    white = s.add_material(rtxamd.diffuse((0.7, 0.7, 0.7)))
+    red = s.add_material(rtxamd.diffuse((0.6, 0.1, 0.1)))
+    ph = s.add_material(rtxamd.phong((0.3, 0.4, 0.5), (0.4, 0.4, 0.3), 30.0))
+    ph_sharp = s.add_material(rtxamd.phong((0.1, 0.1, 0.1), (0.8, 0.8, 0.8), 300.0))
+    cond = s.add_material(rtxamd.conductor(0.2, 3.0))
+    mir = s.add_material(rtxamd.mirror())
+    rough = s.add_material(rtxamd.rough_conductor(0.25, 0.37, 2.82))
+    rough_glossy = s.add_material(rtxamd.rough_conductor(0.08, 1.2, 5.0))
+    glass = s.add_material(rtxamd.dielectric(1.5))
+    lamp = s.add_material(rtxamd.light((8.0, 8.0, 8.0), (0.5, 0.5, 0.5)))
+    ball = s.add_material(rtxamd.light((20.0, 10.0, 5.0)))
+    # area light first so that its light index owns face-table row 0
+    p, i = quad((-0.3, -0.3, 1.98), (-0.3, 0.3, 1.98), (0.3, 0.3, 1.98), (0.3, -0.3, 1.98))
+    s.add_mesh(p, i, lamp)
+    walls = [
+        (quad((-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0)), white),
+        (quad((-1, -1, 2), (-1, 1, 2), (1, 1, 2), (1, -1, 2)), white),
+        (quad((-1, 1, 0), (1, 1, 0), (1, 1, 2), (-1, 1, 2)), ph),
+        (quad((-1, -1, 0), (-1, 1, 0), (-1, 1, 2), (-1, -1, 2)), red),
+        (quad((1, -1, 0), (1, -1, 2), (1, 1, 2), (1, 1, 0)), mir),
+    ]
+    for (p, i), m in walls:
+        s.add_mesh(p, i, m)
+    for lo, hi, m in [((0.05, -0.6, 0.0), (0.55, -0.1, 0.5), rough), ((-0.7, 0.0, 0.0), (-0.2, 0.5, 1.1), cond),
+                      ((-0.2, -0.7, 0.0), (0.0, -0.5, 0.3), ph_sharp), ((0.5, 0.3, 0.0), (0.8, 0.6, 0.4), rough_glossy)]:
+        p, i = box(lo, hi)
+        s.add_mesh(p, i, m)
+    s.add_sphere((0.3, -0.35, 0.75), 0.25, glass)
+    s.add_sphere((-0.5, -0.5, 1.5), 0.08, ball)
+    if point_light:
+        s.add_point_light((0.6, -0.6, 1.6), (1.5, 1.5, 1.5))
+    s.set_view((0.0, -3.8, 1.0), (0.0, 0.0, 1.0), (0.0, 0.0, 1.0), 40.0)
+    return s.finalize()
+
+
+def emitter_over_plane(side=1.0, height=1.0, radiance=5.0, albedo=0.5):
+    """A large diffuse floor at z=0 under a square area light of side `side`
+    centred at height `height`, facing down; the camera looks straight down
+    at the origin with a very narrow field of view."""
+    s = rtxamd.HostScene.new()
+    lamp = s.add_material(rtxamd.light((radiance,) * 3))
+    floor = s.add_material(rtxamd.diffuse((albedo,) * 3))
+    h = side / 2
+    p, i = quad((-h, -h, height), (-h, h, height), (h, h, height), (h, -h, height))
+    s.add_mesh(p, i, lamp)
+    p, i = quad((-50, -50, 0), (50, -50, 0), (50, 50, 0), (-50, 50, 0))
+    s.add_mesh(p, i, floor)
+    s.set_view((0.0, 0.0, 0.5 * height), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 0.01)
+    return s.finalize()
+
+
+def form_factor_parallel_square(side, height):
+    """Differential-area-to-parallel-square form factor, square centred above
+    the point: four corner rectangles a x b = side/2 x side/2 (Howell's
+    catalogue of configuration factors, case B-3)."""
+    X = Y = (side / 2) / height
+    f = (X / np.sqrt(1 + X * X) * np.arctan(Y / np.sqrt(1 + X * X)) +
+         Y / np.sqrt(1 + Y * Y) * np.arctan(X / np.sqrt(1 + Y * Y))) / (2 * np.pi)
+    return 4 * f
+
+
+def params(hs, w, h, spp, **kw):
+    cam = hs.camera(w, h)
+    flags = kw.pop("flags", abi.RTX_FRAME_RESET_ACCUM)
+    return rtxamd.frame_params(cam, w, h, spp, flags=flags, **kw)
