@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--triangles", type=int, default=262144)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traversal", type=int, default=0, help="1 persistent lane-refill, 2 grid-stride (0: default)")
+    ap.add_argument("--trace-blocks", type=int, default=0, help="traversal blocks per CU (0: default)")
+    ap.add_argument("--shade-blocks", type=int, default=0, help="shading blocks per CU (0: default)")
     ap.add_argument("--no-count", action="store_true", help="skip the node/triangle counting pass")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 byte counters)")
@@ -69,6 +72,10 @@ def main():
     W, H, spp = a.width, a.height, a.spp
     hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 1, a.triangles)
     r = rtxamd.Renderer(0)
+    r.set_tuning(traversalKernel=a.traversal, traceBlocksPerCU=a.trace_blocks, shadeBlocksPerCU=a.shade_blocks)
+    tun = r.tuning()
+    ext_kernel = "kClosestP (persistent closest-hit BVH traversal)" if tun.traversalKernel == 1 else \
+        "kExtend (closest-hit BVH traversal)"
     t0 = time.time()
     ds = r.upload(hs)
     info = ds.info()
@@ -124,7 +131,7 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "kernel": "kExtend (closest-hit BVH traversal)",
+        "kernel": ext_kernel,
         "algorithmic_bytes_per_launch": int(ext_bytes / max(1, launches_ext)),
         "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_step": launches_ext,
         "nodes_per_ray": round(cst.nodeVisitsByKind[0] / max(1, ext_rays), 2),

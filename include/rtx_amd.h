@@ -348,6 +348,17 @@ typedef struct rtx_scene_info {
     double buildMs;         /* host build time */
 } rtx_scene_info;
 
+/* Launch-shape knobs of the wavefront (no counterpart in the reference,
+ * whose traversal is the driver's).  In rtx_ctx_set_tuning a zero field
+ * keeps the context's current value. */
+typedef struct rtx_tuning {
+    int32_t traversalKernel;  /* 1 = persistent lane-refill while-while (default), 2 = grid-stride stack loop */
+    int32_t traceBlocksPerCU; /* resident 128-thread traversal blocks per CU (default 16) */
+    int32_t shadeBlocksPerCU; /* 256-thread shading blocks per CU (default 8) */
+    uint32_t batchPaths;      /* paths in flight per batch when rtx_frame_params.batchPaths is 0 (2^22) */
+    int32_t _reserved[4];
+} rtx_tuning;
+
 typedef struct rtx_ctx rtx_ctx;
 typedef struct rtx_scene rtx_scene;
 
@@ -361,6 +372,11 @@ void rtx_push_constant_defaults(rtx_push_constant *pc);
 int rtx_ctx_create(int device, rtx_ctx **out);
 int rtx_ctx_destroy(rtx_ctx *ctx);
 const char *rtx_last_error(const rtx_ctx *ctx);
+
+/* Set / read the launch-shape knobs of a context (new; no counterpart in
+ * the reference). */
+int rtx_ctx_set_tuning(rtx_ctx *ctx, const rtx_tuning *tuning);
+int rtx_ctx_get_tuning(const rtx_ctx *ctx, rtx_tuning *tuning);
 
 /* Upload a scene and build its BVH (replaces SceneLoader::createVulkanObjects,
  * src/SceneLoader.cpp:67-87, incl. createBottomLevelAS/createTopLevelAS). */

@@ -59,6 +59,7 @@ struct rtx_ctx {
     size_t eventsUsed = 0;
     hipEvent_t tStart = nullptr, tStop = nullptr;
     int cuCount = 256;
+    rtx_tuning tuning{1, 16, 8, 1u << 22, {0, 0, 0, 0}};
 };
 
 namespace {
@@ -190,16 +191,19 @@ int ensureBatchBuffers(rtx_ctx *ctx, size_t paths, uint32_t numNEE) {
         CHECK_HIP(ctx, devAlloc(&ctx->q[k].id, cap, L));
         CHECK_HIP(ctx, devAlloc(&ctx->q[k].org, cap, L));
         CHECK_HIP(ctx, devAlloc(&ctx->q[k].dir, cap, L));
-        CHECK_HIP(ctx, devAlloc(&ctx->q[k].aux, cap, L));
+        CHECK_HIP(ctx, devAlloc(&ctx->q[k].hit, cap, L));
+        ctx->q[k].aux = nullptr;
     }
     CHECK_HIP(ctx, devAlloc(&ctx->qs.id, cap * nee, L));
     CHECK_HIP(ctx, devAlloc(&ctx->qs.org, cap * nee, L));
     CHECK_HIP(ctx, devAlloc(&ctx->qs.dir, cap * nee, L));
     ctx->qs.aux = nullptr;
+    ctx->qs.hit = nullptr;
     CHECK_HIP(ctx, devAlloc(&ctx->qp.id, cap * nee, L));
     CHECK_HIP(ctx, devAlloc(&ctx->qp.org, cap * nee, L));
     CHECK_HIP(ctx, devAlloc(&ctx->qp.dir, cap * nee, L));
     CHECK_HIP(ctx, devAlloc(&ctx->qp.aux, cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->qp.hit, cap * nee, L));
     CHECK_HIP(ctx, devAlloc(&ctx->done, cap, L));
     ctx->cap = cap;
     ctx->capNEE = nee;
@@ -264,6 +268,29 @@ void rtx_push_constant_defaults(rtx_push_constant *pc) {
 }
 
 const char *rtx_last_error(const rtx_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int rtx_ctx_set_tuning(rtx_ctx *ctx, const rtx_tuning *t) {
+    if (!ctx || !t) return RTX_E_INVALID;
+    rtx_tuning v = ctx->tuning;
+    if (t->traversalKernel) v.traversalKernel = t->traversalKernel;
+    if (t->traceBlocksPerCU) v.traceBlocksPerCU = t->traceBlocksPerCU;
+    if (t->shadeBlocksPerCU) v.shadeBlocksPerCU = t->shadeBlocksPerCU;
+    if (t->batchPaths) v.batchPaths = t->batchPaths;
+    if (v.traversalKernel < 1 || v.traversalKernel > 2)
+        return fail(ctx, RTX_E_INVALID, "rtx_tuning.traversalKernel must be 1 or 2");
+    if (v.traceBlocksPerCU < 1 || v.traceBlocksPerCU > 64 || v.shadeBlocksPerCU < 1 || v.shadeBlocksPerCU > 64)
+        return fail(ctx, RTX_E_INVALID, "rtx_tuning blocks per CU must lie in [1, 64]");
+    if (v.batchPaths < 1024 || v.batchPaths > (1u << 26))
+        return fail(ctx, RTX_E_INVALID, "rtx_tuning.batchPaths must lie in [1024, 2^26]");
+    ctx->tuning = v;
+    return RTX_OK;
+}
+
+int rtx_ctx_get_tuning(const rtx_ctx *ctx, rtx_tuning *t) {
+    if (!ctx || !t) return RTX_E_INVALID;
+    *t = ctx->tuning;
+    return RTX_OK;
+}
 
 int rtx_ctx_create(int device, rtx_ctx **out) {
     if (!out) return RTX_E_INVALID;
@@ -479,7 +506,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     }
 
     // batch size: whole sample planes when they fit, else pixel ranges
-    uint64_t batch = p->batchPaths ? p->batchPaths : (1ull << 22);
+    uint64_t batch = p->batchPaths ? p->batchPaths : ctx->tuning.batchPaths;
     batch = std::min<uint64_t>(batch, total);
     if (batch >= npix) batch -= batch % npix;
     int rc = ensureBatchBuffers(ctx, (size_t)batch, numNEE);
@@ -520,8 +547,9 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     const bool countVisits = fc.countVisits != 0;
     const bool timeKernels = (p->flags & RTX_FRAME_TIME_KERNELS) != 0;
     ctx->eventsUsed = 0;
-    const int traceBlocks = ctx->cuCount * 16;
-    const int shadeBlocks = ctx->cuCount * 8;
+    const int traceBlocks = ctx->cuCount * ctx->tuning.traceBlocksPerCU;
+    const int shadeBlocks = ctx->cuCount * ctx->tuning.shadeBlocksPerCU;
+    const bool persistent = ctx->tuning.traversalKernel == 1;
     const int maxIter = p->pc.maxDepth + p->pc.maxFollowDiscrete + 3;
     const DevScene &S = scene->view;
     hipStream_t s = ctx->stream;
@@ -546,14 +574,25 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         timed(kClsShade, [&] { launchGenerate(s, ctx->fc, b, ctx->pb, ctx->q[qsel], ctx->cnt, qsel); });
         for (int it = 0; it < maxIter; ++it) {
             timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
-            timed(kClsExtend, [&] { launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, countVisits, traceBlocks); });
+            timed(kClsExtend, [&] {
+                if (persistent) launchClosestP(s, S, ctx->q[qsel], ctx->cnt, qsel, 0, countVisits, traceBlocks);
+                else launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, countVisits, traceBlocks);
+            });
             timed(kClsShade, [&] {
                 launchShade(s, S, ctx->fc, b, ctx->pb, ctx->q[qsel], ctx->q[qsel ^ 1], ctx->qs, ctx->qp, ctx->done,
                             ctx->cnt, qsel, shadeBlocks);
             });
-            timed(kClsShadow, [&] { launchShadow(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
-            timed(kClsProbe,
-                  [&] { launchProbe(s, S, ctx->fc, ctx->qp, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
+            timed(kClsShadow, [&] {
+                if (persistent) launchShadowP(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks);
+                else launchShadow(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks);
+            });
+            if (persistent) {
+                timed(kClsProbe, [&] { launchClosestP(s, S, ctx->qp, ctx->cnt, qsel, 2, countVisits, traceBlocks); });
+                timed(kClsProbe, [&] { launchProbeResolve(s, S, ctx->fc, ctx->qp, ctx->pb, ctx->cnt, shadeBlocks); });
+            } else {
+                timed(kClsProbe,
+                      [&] { launchProbe(s, S, ctx->fc, ctx->qp, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
+            }
             timed(kClsShade, [&] { launchFinish(s, ctx->fc, ctx->pb, ctx->done, ctx->cnt, shadeBlocks); });
             qsel ^= 1;
         }
@@ -646,15 +685,22 @@ static int queryCommon(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays
     if (!ctx) return RTX_E_INVALID;
     if (!scene || (n && (!rays || !out))) return fail(ctx, RTX_E_INVALID, "NULL argument");
     if (n == 0) return RTX_OK;
+    if (n > (size_t)INT32_MAX) return fail(ctx, RTX_E_INVALID, "ray query: more than 2^31-1 rays");
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<void *> tmp;
     rtx_ray *rD = nullptr;
     void *oD = nullptr;
+    uint32_t *fetch = nullptr;
     hipError_t e = devAlloc(&rD, n, tmp);
     if (e == hipSuccess) e = devAlloc((char **)&oD, n * outBytes, tmp);
+    if (e == hipSuccess) e = devAlloc(&fetch, 1, tmp);
+    if (e == hipSuccess) e = hipMemsetAsync(fetch, 0, sizeof(uint32_t), ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(rD, rays, n * sizeof(rtx_ray), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
-        if (closest) launchQueryClosest(ctx->stream, scene->view, rD, n, (rtx_hit *)oD);
+        if (ctx->tuning.traversalKernel == 1)
+            launchQueryP(ctx->stream, scene->view, rD, (uint32_t)n, oD, closest, fetch,
+                         ctx->cuCount * ctx->tuning.traceBlocksPerCU);
+        else if (closest) launchQueryClosest(ctx->stream, scene->view, rD, n, (rtx_hit *)oD);
         else launchQueryOccluded(ctx->stream, scene->view, rD, n, (uint8_t *)oD);
         e = hipGetLastError();
     }

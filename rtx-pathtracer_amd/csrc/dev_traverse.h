@@ -217,4 +217,170 @@ RTX_DI int traverse(const DevScene &S, const TravRay &r, float tmin, float &tmax
     return best;
 }
 
+// ------------------------------------------------------------------
+// Persistent traversal with lane refill (Aila & Laine 2009, sections 4-5):
+// each wave loops until the queue is drained; lanes whose ray finished
+// fetch a new ray (one atomic per wave), and the "while-while" loop with
+// postponed leaves keeps lanes in step: a lane that reaches a leaf parks
+// it and keeps descending until every lane of the wave has a leaf, then
+// the wave intersects leaves together.
+constexpr int32_t kLinkDone = (int32_t)0x80000000; // no node: a leaf never encodes to this
+
+struct LaneRay {
+    TravRay r;
+    float tmin, tmax, bu, bv;
+    int best;
+    int link, leaf;
+    uint32_t idx;
+};
+
+template <class Stack> RTX_DI int popOrDone(Stack &st) { return st.sp == 0 ? kLinkDone : st.pop(); }
+
+// One inner-node visit: returns the next link (pushing the far child when
+// both are hit), or the next stack entry / kLinkDone when neither is hit.
+template <bool COUNT, class Stack>
+RTX_DI int innerStep(const DevScene &S, LaneRay &L, Stack &st, VisitCount &vc) {
+    const float4 *np = S.nodes + 4 * (size_t)L.link;
+    float4 a = np[0], b = np[1], c = np[2];
+    int4 dd = *reinterpret_cast<const int4 *>(np + 3);
+    if (COUNT) vc.nodes++;
+    float t0, t1;
+    bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, L.r, L.tmin, L.tmax, t0);
+    bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, L.r, L.tmin, L.tmax, t1);
+    if (h0 && h1) {
+        int nearL = dd.x, farL = dd.y;
+        if (t1 < t0) {
+            nearL = dd.y;
+            farL = dd.x;
+        }
+        st.push(farL);
+        return nearL;
+    }
+    if (h0) return dd.x;
+    if (h1) return dd.y;
+    return popOrDone(st);
+}
+
+// Intersects the primitives of leaf link `leaf`; returns true when an any-hit
+// ray is done.
+template <bool ANY, bool COUNT>
+RTX_DI bool leafStep(const DevScene &S, LaneRay &L, int leaf, VisitCount &vc) {
+    uint32_t data = ~(uint32_t)leaf;
+    uint32_t start = data >> 4, cnt = (data & 15u) + 1u;
+    const float4 *tp = S.tris + 3 * (size_t)start;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        float4 v0 = tp[3 * k], v1 = tp[3 * k + 1], v2 = tp[3 * k + 2];
+        if (COUNT) vc.prims++;
+        float t, u, v;
+        if (isectTri(L.r, v0, v1, v2, L.tmin, L.tmax, t, u, v)) {
+            int id = __float_as_int(v0.w);
+            if (ANY) {
+                L.best = id;
+                return true;
+            }
+            if (L.best < 0 || t < L.tmax || id < L.best) {
+                L.tmax = t;
+                L.best = id;
+                L.bu = u;
+                L.bv = v;
+            }
+        }
+    }
+    return false;
+}
+
+// Starts a ray: spheres are tested up front (all workloads have few).
+template <bool ANY> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
+    L.best = -1;
+    L.bu = L.bv = 0.0f;
+    L.link = 0;
+    L.leaf = kLinkDone;
+    for (uint32_t i = 0; i < S.numSpheres; ++i) {
+        float t;
+        if (isectSphere(S.spheres[i], L.r.o, L.r.d, L.tmin, L.tmax, t)) {
+            int id = (int)(S.numTris + i);
+            if (L.best < 0 || t < L.tmax || id < L.best) {
+                L.tmax = t;
+                L.best = id;
+            }
+            if (ANY) {
+                L.link = kLinkDone;
+                return;
+            }
+        }
+    }
+}
+
+// Drives the persistent loop.  load(idx, L) fills L.r/L.tmin/L.tmax;
+// store(L) consumes a finished ray.  `fetch` must be zero at launch.  A
+// wave refills when at least kRefillLanes of its lanes are idle (or all
+// are), with one atomic for the whole wave; once the queue is drained the
+// wave stops fetching, so `fetch` ends below n + 64 * waves.
+constexpr int kRefillLanes = 16;
+
+template <bool ANY, bool COUNT, class Stack, class Load, class Store>
+RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stack &st, VisitCount &vc, Load load,
+                            Store store) {
+    LaneRay L;
+    L.link = L.leaf = kLinkDone;
+    L.idx = 0;
+    bool active = false;
+    bool drained = false; // wave-uniform
+    for (;;) {
+        // refill idle lanes (the whole wave is converged here)
+        const bool idle = !active;
+        const uint64_t m = __ballot(idle);
+        const int nIdle = __popcll(m);
+        const bool anyActive = nIdle < 64; // gfx950 waves are 64 wide; blocks are whole waves
+        if (!drained && (nIdle >= kRefillLanes || !anyActive)) {
+            const int lane = (int)__lane_id();
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(fetch, (uint32_t)nIdle);
+            base = __shfl(base, leader);
+            if (base + (uint32_t)nIdle >= n) drained = true;
+            if (idle) {
+                const uint64_t below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+                const uint32_t idx = base + (uint32_t)__popcll(below);
+                if (idx < n) {
+                    L.idx = idx;
+                    load(idx, L);
+                    st.sp = 0;
+                    beginRay<ANY>(S, L);
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            // inner nodes until every lane of the wave holds a leaf (or is done)
+            while (L.link >= 0) {
+                L.link = innerStep<COUNT>(S, L, st, vc);
+                if (L.link < 0 && L.link != kLinkDone && L.leaf == kLinkDone) {
+                    L.leaf = L.link;
+                    L.link = popOrDone(st);
+                }
+                if (__ballot(L.leaf == kLinkDone) == 0) break;
+            }
+            // leaves
+            while (L.leaf != kLinkDone) {
+                if (leafStep<ANY, COUNT>(S, L, L.leaf, vc)) {
+                    L.link = kLinkDone;
+                    L.leaf = kLinkDone;
+                    break;
+                }
+                L.leaf = kLinkDone;
+                if (L.link < 0 && L.link != kLinkDone) {
+                    L.leaf = L.link;
+                    L.link = popOrDone(st);
+                }
+            }
+            if (L.link == kLinkDone && L.leaf == kLinkDone) {
+                store(L);
+                active = false;
+            }
+        }
+    }
+}
+
 } // namespace rtx

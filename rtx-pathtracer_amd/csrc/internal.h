@@ -41,7 +41,8 @@ struct Queue {
     int32_t *id;    // path id (extend queue, done queue) or NEE slot (shadow/probe)
     float4 *org;    // origin.xyz, tmin
     float4 *dir;    // direction.xyz, tmax
-    float4 *aux;    // extend: hit (t, prim bits, u, v); probe: (base rgb, pdfMat)
+    float4 *hit;    // closest hit (t, prim bits, u, v): extend and probe queues
+    float4 *aux;    // probe: (premultiplied BSDF weight rgb, pdfMat)
 };
 
 struct Counters {
@@ -50,6 +51,8 @@ struct Counters {
     uint32_t probe;
     uint32_t done;
     uint32_t pad[3];
+    uint32_t fetch[3];  // persistent-kernel work counters: extend, shadow, probe
+    uint32_t pad2;
     unsigned long long extendRays, shadowRays, probeRays;
     unsigned long long nodeVisits[3], primVisits[3]; // extend, shadow, probe
 };
@@ -91,15 +94,21 @@ constexpr uint32_t kFlagAddNext = 1u << 17;
 constexpr uint32_t kFlagPending = 1u << 18;       // NEE contributions waiting to be added
 
 // ----------------------------------------------------------- launchers
-struct LaunchCfg {
-    int blocksExtend = 0;   // 0 = auto
-    int blocksShade = 0;
-};
 
 void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue q0,
                     Counters *cnt, int qsel);
 void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits,
                   int blocks);
+// persistent lane-refill traversal: closest hit of an extend (kind 0) or probe (kind 2) queue
+void launchClosestP(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, int kind, bool countVisits,
+                    int blocks);
+void launchShadowP(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
+                   int blocks);
+// persistent variant of the two queries; *fetch must be zero
+void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_t n, void *out, bool closest,
+                  uint32_t *fetch, int blocks);
+void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
+                        Counters *cnt, int blocks);
 void launchShade(hipStream_t s, const DevScene &S, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue qin,
                  Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel, int blocks);
 void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,

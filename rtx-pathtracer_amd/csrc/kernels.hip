@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(DevScene S, Queue q, Coun
         TravRay r = makeTravRay(v3(o), v3(d));
         float tmax = d.w, bu = 0.0f, bv = 0.0f;
         int prim = traverse<false, COUNT>(S, r, o.w, tmax, bu, bv, st, vc);
-        q.aux[i] = make_float4(tmax, __int_as_float(prim), bu, bv);
+        q.hit[i] = make_float4(tmax, __int_as_float(prim), bu, bv);
     }
     flushVisits<COUNT>(vc, cnt, 0);
 }
@@ -126,6 +126,31 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(DevScene S, Queue q, Path
     flushVisits<COUNT>(vc, cnt, 1);
 }
 
+// MIS probe result (raytrace.rgen:684-726): on a miss the environment colour
+// (black default texture) with pdf 1/(2 pi lightCount); on a light the
+// light-side pdf (pdfLight, raytrace.rgen:334-339) and the BSDF-side weight.
+// base = throughput * f * cos / (numNEE * pdfMat), base.w = pdfMat.
+RTX_DI V3 probeContribution(const DevScene &S, bool power, V3 o, V3 d, int prim, float t, float bu, float bv,
+                            float4 base) {
+    float pdfMat = base.w;
+    if (prim < 0) {
+        V3 lightColor = v3(0.0f);
+        float pdfL = kInv2Pi / (float)S.lightCount;
+        float h = misWeight(power, pdfMat, pdfL);
+        return v3(base) * lightColor * h;
+    }
+    HitInfo hi = hitInfo(S, prim, t, bu, bv, o, d);
+    const rtx_material &ms = S.materials[hi.matIndex];
+    if (ms.type != RTX_MAT_LIGHT) return v3(0.0f);
+    int iLight = hi.isSphere ? S.spheres[hi.instanceIndex].iLight : S.instances[hi.instanceIndex].iLight;
+    const rtx_light &light = S.lights[iLight];
+    float cosL = dot(-d, hi.normal);
+    float pdfL = light.sampleProb * t * t / cosL / light.area;
+    float h = misWeight(power, pdfMat, pdfL);
+    if (isnan(h)) return v3(0.0f);
+    return v3(base) * v3(ms.lightColor) * h;
+}
+
 // ---------------------------------------------------------------- probe
 // The MIS probe of nextEventEstimation (raytrace.rgen:665-729): closest hit
 // plus the BSDF-side MIS weight against the light it hits.
@@ -146,29 +171,69 @@ __global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, const FrameCon
         TravRay r = makeTravRay(o, d);
         float tmax = d4.w, bu = 0.0f, bv = 0.0f;
         int prim = traverse<false, COUNT>(S, r, o4.w, tmax, bu, bv, st, vc);
-        float pdfMat = base.w;
-        V3 contrib = v3(0.0f);
-        if (prim < 0) {
-            // miss: environment colour (default black texture) with pdf 1/(2 pi lightCount)
-            V3 lightColor = v3(0.0f);
-            float pdfL = kInv2Pi / (float)S.lightCount;
-            float h = misWeight(power, pdfMat, pdfL);
-            contrib = v3(base) * lightColor * h;
-        } else {
-            HitInfo hi = hitInfo(S, prim, tmax, bu, bv, o, d);
-            const rtx_material &ms = S.materials[hi.matIndex];
-            if (ms.type == RTX_MAT_LIGHT) {
-                int iLight = hi.isSphere ? S.spheres[hi.instanceIndex].iLight : S.instances[hi.instanceIndex].iLight;
-                const rtx_light &light = S.lights[iLight];
-                float cosL = dot(-d, hi.normal); // pdfLight, raytrace.rgen:334-339
-                float pdfL = light.sampleProb * tmax * tmax / cosL / light.area;
-                float h = misWeight(power, pdfMat, pdfL);
-                if (!isnan(h)) contrib = v3(base) * v3(ms.lightColor) * h;
-            }
-        }
+        V3 contrib = probeContribution(S, power, o, d, prim, tmax, bu, bv, base);
         pb.pendProbe[q.id[i]] = f4(contrib, __int_as_float(prim));
     }
     flushVisits<COUNT>(vc, cnt, 2);
+}
+
+// --------------------------------------------------- persistent traversal
+// kind 0: extend queue (count extend[qsel]); kind 2: probe queue (count probe)
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kClosestP(DevScene S, Queue q, Counters *cnt, int qsel, int kind) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    const uint32_t n = kind == 0 ? cnt->extend[qsel] : cnt->probe;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(kind == 0 ? &cnt->extendRays : &cnt->probeRays, (unsigned long long)n);
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    persistentTrace<false, COUNT>(
+        S, n, &cnt->fetch[kind], st, vc,
+        [&](uint32_t i, LaneRay &L) {
+            float4 o = q.org[i], d = q.dir[i];
+            L.r = makeTravRay(v3(o), v3(d));
+            L.tmin = o.w;
+            L.tmax = d.w;
+        },
+        [&](const LaneRay &L) { q.hit[L.idx] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv); });
+    flushVisits<COUNT>(vc, cnt, kind);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kShadowP(DevScene S, Queue q, PathBuffers pb, Counters *cnt) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    const uint32_t n = cnt->shadow;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->shadowRays, (unsigned long long)n);
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    persistentTrace<true, COUNT>(
+        S, n, &cnt->fetch[1], st, vc,
+        [&](uint32_t i, LaneRay &L) {
+            float4 o = q.org[i], d = q.dir[i];
+            L.r = makeTravRay(v3(o), v3(d));
+            L.tmin = o.w;
+            L.tmax = d.w;
+        },
+        [&](const LaneRay &L) {
+            if (L.best < 0) pb.pendShadow[q.id[L.idx]].w = 1.0f; // unoccluded: raytrace.shadow.rmiss:7-10
+        });
+    flushVisits<COUNT>(vc, cnt, 1);
+}
+
+// The BSDF-side half of the MIS probe (raytrace.rgen:684-726) for probe rays
+// whose closest hits kClosestP wrote.
+__global__ __launch_bounds__(256) void kProbeResolve(DevScene S, const FrameConst *__restrict__ fcp, Queue q,
+                                                     PathBuffers pb, Counters *cnt) {
+    const uint32_t n = cnt->probe;
+    const bool power = fcp->pc.usePowerHeuristic != 0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        float4 o4 = q.org[i], d4 = q.dir[i], base = q.aux[i], hit = q.hit[i];
+        int prim = __float_as_int(hit.y);
+        pb.pendProbe[q.id[i]] = f4(probeContribution(S, power, v3(o4), v3(d4), prim, hit.x, hit.z, hit.w, base),
+                                   __int_as_float(prim));
+    }
 }
 
 // Pending NEE terms of the previous vertex, in NEE-sample order.
@@ -220,7 +285,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
         bool follow = false, addNext = false;
         if (active) {
             pid = (uint32_t)qin.id[i];
-            float4 hit = qin.aux[i];
+            float4 hit = qin.hit[i];
             d = v3(qin.dir[i]);
             V3 o = v3(qin.org[i]);
             rng.s = pb.rng[pid];
@@ -422,6 +487,7 @@ __global__ void kResetCounters(Counters *cnt, int which) {
         cnt->shadow = 0;
         cnt->probe = 0;
         cnt->done = 0;
+        cnt->fetch[0] = cnt->fetch[1] = cnt->fetch[2] = 0;
     }
 }
 
@@ -466,6 +532,37 @@ __global__ __launch_bounds__(kTraceBlock) void kQueryOccluded(DevScene S, const 
     occ[i] = traverse<true, false>(S, r, ry.tmin, tmax, bu, bv, st, vc) >= 0 ? 1 : 0;
 }
 
+// Persistent lane-refill variants of the two queries (the traversal the
+// wavefront uses by default).
+template <bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void kQueryP(DevScene S, const rtx_ray *rays, uint32_t n, void *out,
+                                                      uint32_t *fetch) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    persistentTrace<ANY, false>(
+        S, n, fetch, st, vc,
+        [&](uint32_t i, LaneRay &L) {
+            rtx_ray ry = rays[i];
+            L.r = makeTravRay(v3(ry.org), v3(ry.dir));
+            L.tmin = ry.tmin;
+            L.tmax = ry.tmax;
+        },
+        [&](const LaneRay &L) {
+            if (ANY) {
+                static_cast<uint8_t *>(out)[L.idx] = L.best >= 0 ? 1 : 0;
+            } else {
+                rtx_hit h;
+                h.t = L.tmax; // the ray's tmax when nothing was hit
+                h.prim = L.best;
+                h.u = L.bu;
+                h.v = L.bv;
+                static_cast<rtx_hit *>(out)[L.idx] = h;
+            }
+        });
+}
+
 // ------------------------------------------------------------- launchers
 static int gridFor(size_t n, int block) { return (int)((n + block - 1) / block); }
 
@@ -476,6 +573,20 @@ void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffer
 void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits, int blocks) {
     if (countVisits) hipLaunchKernelGGL(kExtend<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
     else hipLaunchKernelGGL(kExtend<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+}
+void launchClosestP(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, int kind, bool countVisits,
+                    int blocks) {
+    if (countVisits) hipLaunchKernelGGL(kClosestP<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel, kind);
+    else hipLaunchKernelGGL(kClosestP<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel, kind);
+}
+void launchShadowP(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
+                   int blocks) {
+    if (countVisits) hipLaunchKernelGGL(kShadowP<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
+    else hipLaunchKernelGGL(kShadowP<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
+}
+void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
+                        Counters *cnt, int blocks) {
+    hipLaunchKernelGGL(kProbeResolve, dim3(blocks), dim3(256), 0, s, S, fc, q, pb, cnt);
 }
 void launchShade(hipStream_t s, const DevScene &S, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue qin,
                  Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel, int blocks) {
@@ -512,6 +623,12 @@ void launchQueryClosest(hipStream_t s, const DevScene &S, const rtx_ray *rays, s
 void launchQueryOccluded(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, uint8_t *occ) {
     if (n == 0) return;
     hipLaunchKernelGGL(kQueryOccluded, dim3(gridFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, S, rays, n, occ);
+}
+void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_t n, void *out, bool closest,
+                  uint32_t *fetch, int blocks) {
+    if (n == 0) return;
+    if (closest) hipLaunchKernelGGL(kQueryP<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, rays, n, out, fetch);
+    else hipLaunchKernelGGL(kQueryP<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, rays, n, out, fetch);
 }
 void launchResolveImage(hipStream_t s, const float4 *accum, uint32_t npix, float inv, float4 *out) {
     hipLaunchKernelGGL(kResolveImage, dim3(gridFor(npix, 256)), dim3(256), 0, s, accum, npix, inv, out);

@@ -90,6 +90,8 @@ def device_lib():
         L.rtx_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Ray), C.c_size_t, C.POINTER(Hit)]
         L.rtx_trace_occluded.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Ray), C.c_size_t,
                                          C.POINTER(C.c_uint8)]
+        L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
+        L.rtx_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         _dev = L
     return _dev
 
@@ -355,6 +357,19 @@ class Renderer:
         h = C.c_void_p()
         self._check(device_lib().rtx_scene_create(self._h, C.byref(d), C.byref(h)))
         return DeviceScene(self, h)
+
+    def tuning(self):
+        t = abi.Tuning()
+        self._check(device_lib().rtx_ctx_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        """Launch-shape knobs (traversalKernel, traceBlocksPerCU, shadeBlocksPerCU,
+        batchPaths); fields left out keep their current values."""
+        t = abi.Tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._check(device_lib().rtx_ctx_set_tuning(self._h, C.byref(t)))
 
     def render(self, scene, params):
         st = FrameStats()

@@ -124,6 +124,11 @@ class FrameStats(C.Structure):
                 ("launchesByKind", C.c_uint32 * 4), ("_pad", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("traversalKernel", C.c_int32), ("traceBlocksPerCU", C.c_int32),
+                ("shadeBlocksPerCU", C.c_int32), ("batchPaths", C.c_uint32), ("_reserved", C.c_int32 * 4)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("triangles", C.c_uint32), ("spheres", C.c_uint32), ("nodes", C.c_uint32),
                 ("maxDepth", C.c_uint32), ("deviceBytes", C.c_uint64), ("buildMs", C.c_double)]
@@ -143,7 +148,7 @@ class View(C.Structure):
 
 EXPECTED_SIZES = {
     Vertex: 48, Material: 96, Instance: 144, Light: 40, FaceSample: 12, Sphere: 24,
-    PushConstant: 192, Camera: 256, Ray: 32, Hit: 16,
+    PushConstant: 192, Camera: 256, Ray: 32, Hit: 16, Tuning: 32,
 }
 for _t, _n in EXPECTED_SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)

@@ -17,7 +17,7 @@ C_NAMES = {
     abi.FaceSample: "rtx_face_sample", abi.Sphere: "rtx_sphere", abi.PushConstant: "rtx_push_constant",
     abi.Camera: "rtx_camera", abi.ModelDesc: "rtx_model_desc", abi.SceneDesc: "rtx_scene_desc",
     abi.FrameParams: "rtx_frame_params", abi.FrameStats: "rtx_frame_stats", abi.SceneInfo: "rtx_scene_info",
-    abi.Ray: "rtx_ray", abi.Hit: "rtx_hit",
+    abi.Ray: "rtx_ray", abi.Hit: "rtx_hit", abi.Tuning: "rtx_tuning",
 }
 
 

