@@ -74,8 +74,8 @@ def main():
     r = rtxamd.Renderer(0)
     r.set_tuning(traversalKernel=a.traversal, traceBlocksPerCU=a.trace_blocks, shadeBlocksPerCU=a.shade_blocks)
     tun = r.tuning()
-    ext_kernel = "kClosestP (persistent closest-hit BVH traversal)" if tun.traversalKernel == 1 else \
-        "kExtend (closest-hit BVH traversal)"
+    ext_kernel = "kClosestP<false, 0> (persistent closest-hit BVH traversal of extend rays)" \
+        if tun.traversalKernel == 1 else "kExtend<false> (closest-hit BVH traversal)"
     t0 = time.time()
     ds = r.upload(hs)
     info = ds.info()

@@ -405,6 +405,43 @@ int rtx_accumulated_samples(const rtx_ctx *ctx, uint32_t *samples);
  * FNV-1a hash of the hit primitive ids along the path); either may be NULL. */
 int rtx_read_paths(rtx_ctx *ctx, float *rgb, uint32_t *sig, size_t pathCount);
 
+/* ------------------------------------------------------ irradiance cache */
+/* One cache per context (the reference keeps one per scene and recreates it
+ * on a scene switch, src/RayTracingApp.cpp:346-347).  rtx_render with
+ * pc.useIrradianceCache renders each sample as one 1-spp frame that reads
+ * the cache as it was at the frame's start; after the frame the frame's
+ * update elections (in pixel order, by the header's nextUpdateSlot rule)
+ * and creation candidates (in pixel order; the first maxEntries-count of
+ * them) are computed with calculateCacheData's 200 strata and committed in
+ * that order.  pc.isIrradiancePrepareFrame frames update the cache only. */
+
+/* Replaces IrradianceCache::IrradianceCache + createBuffers
+ * (src/IrradianceCache.cpp:11-17, 46-75): a cache of maxEntries records. */
+int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries);
+/* Empty the cache (as freshly created). */
+int rtx_ic_reset(rtx_ctx *ctx);
+/* Copy the cache buffers (header, and up to `capacity` records and spheres)
+ * out of / into the device (the reference's binding-13 and binding-12
+ * buffers, src/IrradianceCache.cpp:106-123).  rtx_ic_write takes
+ * hdr->maxCaches records; hdr must match the created cache. */
+int rtx_ic_read(rtx_ctx *ctx, rtx_cache_header *hdr, rtx_cache_data *entries, rtx_sphere *spheres,
+                uint32_t capacity);
+int rtx_ic_write(rtx_ctx *ctx, const rtx_cache_header *hdr, const rtx_cache_data *entries,
+                 const rtx_sphere *spheres);
+
+/* The upkeep jobs of the last cache frame of the last rtx_render made with
+ * RTX_FRAME_KEEP_PATHS (diagnostics; no counterpart in the reference).
+ * sub: 4 floats per sub-path (radiance rgb, first-hit distance), 200 per
+ * job in stratum order k * 10 + j; subSig: 2 uint32 per sub-path. */
+typedef struct rtx_ic_job {
+    int32_t type;      /* 0 = update of record `target`, 1 = creation */
+    uint32_t target;
+    float origin[3];
+    float normal[3];
+} rtx_ic_job;
+int rtx_ic_read_jobs(rtx_ctx *ctx, rtx_ic_job *jobs, float *sub, uint32_t *subSig, uint32_t capacity,
+                     uint32_t *numJobs);
+
 /* ---------------------------------------------------- traversal queries */
 /* Rays for the query entry points: origin + tmin, direction + tmax. */
 typedef struct rtx_ray {

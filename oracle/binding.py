@@ -74,8 +74,75 @@ def lib(abi):
         L.orc_sample_unit_sphere.restype = None
         L.orc_sample_lights.argtypes = [C.c_void_p, d3, d3, C.POINTER(C.c_uint32), d3, d3, C.POINTER(C.c_double)]
         L.orc_sample_lights.restype = C.c_double
+        vp = C.c_void_p
+        L.orc_ic_frame.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.c_int, C.c_int, vp, vp, vp,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_uint32), vp, C.c_size_t,
+                                   C.POINTER(C.c_size_t), C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+        L.orc_ic_reduce.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, d3, C.POINTER(C.c_double), d3, d3,
+                                    d3, C.POINTER(C.c_double)]
+        L.orc_ic_stratum.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+        L.orc_ic_stratum.restype = None
+        L.orc_ic_query.argtypes = [C.c_void_p, C.POINTER(abi.PushConstant), vp, vp, vp, d3, d3, d3]
+        L.orc_ic_apply.argtypes = [C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, vp, C.c_size_t,
+                                   C.POINTER(C.c_double)]
+        L.orc_ic_calculate.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, C.c_uint32, d3,
+                                       d3, d3, d3, d3, C.POINTER(C.c_double)]
         _lib = L
     return _lib
+
+
+class IcJob(C.Structure):
+    _fields_ = [("type", C.c_int32), ("target", C.c_uint32), ("origin", C.c_double * 3),
+                ("normal", C.c_double * 3)]
+
+
+IC_STRATA = 200  # 20 phi x 10 theta strata of calculateCacheData
+
+
+def ic_reduce(abi, seed, sample, job, normal, sub, prec=64):
+    """calculateCacheData's accumulation over one job's sub-path results
+    sub: (200, 4) = L rgb, first-hit distance.  Returns color, rot, trans, harmonicR."""
+    L = lib(abi)
+    sub = np.ascontiguousarray(sub, np.float64)
+    c, r, t = (C.c_double * 3)(), (C.c_double * 3)(), (C.c_double * 3)()
+    h = C.c_double()
+    L.orc_ic_reduce(prec, seed, sample, job, d3(normal), sub.ctypes.data_as(C.POINTER(C.c_double)), c, r, t,
+                    C.byref(h))
+    return np.array(c[:]), np.array(r[:]), np.array(t[:]), h.value
+
+
+def ic_apply(abi, params, state, jobs, sub, prec=32):
+    """Accumulate and commit `jobs` [(type, target, origin, normal)] with their
+    sub-path results sub (J, 200, 4) into `state` (in place)."""
+    L = lib(abi)
+    arr = (IcJob * max(1, len(jobs)))()
+    for i, (t, tg, o, n) in enumerate(jobs):
+        arr[i].type, arr[i].target = int(t), int(tg)
+        arr[i].origin[:] = [float(x) for x in o]
+        arr[i].normal[:] = [float(x) for x in n]
+    sub = np.ascontiguousarray(sub, np.float64).reshape(-1, 4)
+    hdr, ent, sph = state.ctypes_views()
+    rc = L.orc_ic_apply(C.byref(params), prec, hdr, ent, sph, C.cast(arr, C.c_void_p), len(jobs),
+                        sub.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise RuntimeError(L.orc_last_error().decode())
+
+
+def ic_stratum(abi, seed, sample, job, k, j):
+    L = lib(abi)
+    phi, theta, st = C.c_double(), C.c_double(), C.c_uint32()
+    L.orc_ic_stratum(seed, sample, job, k, j, C.byref(phi), C.byref(theta), C.byref(st))
+    return phi.value, theta.value, st.value
+
+
+def ic_query(abi, pc, state, x, n, scene=None):
+    """Cache lookup at x with normal n against `state` (rtxamd.IcState); None on a miss."""
+    L = lib(abi)
+    hdr, ent, sph = state.ctypes_views()
+    out = (C.c_double * 3)()
+    hit = L.orc_ic_query(scene._h if scene is not None else None, C.byref(pc), hdr, ent, sph, d3(x), d3(n), out)
+    return np.array(out[:]) if hit else None
 
 
 def d3(v):
@@ -110,6 +177,43 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError(f"oracle render: {self.L.orc_last_error().decode()}")
         return rgb, sig
+
+    def ic_frame(self, params, state, prec=64, threads=None, job_cap=None):
+        """One frame of the irradiance-cache method over every pixel (sample
+        params.sampleOffset).  Updates `state` (rtxamd.IcState) in place and
+        returns (rgb, sig, jobs, sub, subsig): per pixel, then the frame's
+        jobs with their (200, 4) sub-path results and (200, 2) signatures."""
+        threads = DEFAULT_THREADS if threads is None else threads
+        npix = params.width * params.height
+        rgb = np.empty((npix, 3), np.float64)
+        sig = np.empty((npix, 2), np.uint32)
+        cap = state.max_entries + npix if job_cap is None else job_cap
+        jobs = (IcJob * max(1, cap))()
+        sub = np.zeros((max(1, cap) * IC_STRATA, 4), np.float64)
+        subsig = np.zeros((max(1, cap) * IC_STRATA, 2), np.uint32)
+        nj = C.c_size_t()
+        hdr, ent, sph = state.ctypes_views()
+        rc = self.L.orc_ic_frame(self._h, C.byref(params), prec, threads, hdr, ent, sph,
+                                 rgb.ctypes.data_as(C.POINTER(C.c_double)), sig.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 C.cast(jobs, C.c_void_p), cap, C.byref(nj),
+                                 sub.ctypes.data_as(C.POINTER(C.c_double)), subsig.ctypes.data_as(C.POINTER(C.c_uint32)))
+        if rc != 0:
+            raise RuntimeError(f"oracle ic frame: {self.L.orc_last_error().decode()}")
+        n = min(nj.value, cap)
+        jl = [(jobs[i].type, jobs[i].target, np.array(jobs[i].origin[:]), np.array(jobs[i].normal[:]))
+              for i in range(n)]
+        return rgb, sig, jl, sub[:n * IC_STRATA].reshape(n, IC_STRATA, 4), subsig[:n * IC_STRATA].reshape(n, IC_STRATA, 2)
+
+    def ic_calculate(self, params, state, origin, normal, job=0, prec=64):
+        """calculateCacheData at one point: color, rot, trans, harmonicR."""
+        hdr, ent, sph = state.ctypes_views()
+        c, r, t = (C.c_double * 3)(), (C.c_double * 3)(), (C.c_double * 3)()
+        h = C.c_double()
+        rc = self.L.orc_ic_calculate(self._h, C.byref(params), prec, hdr, ent, sph, job, d3(origin), d3(normal), c, r,
+                                     t, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return np.array(c[:]), np.array(r[:]), np.array(t[:]), h.value
 
     def trace_closest(self, rays, brute=False, prec=64):
         rays = np.ascontiguousarray(rays, np.float32)

@@ -47,6 +47,48 @@ int orc_trace_closest_prec(const orc_scene *scene, const rtx_ray *rays, size_t n
 int orc_trace_occluded(const orc_scene *scene, const rtx_ray *rays, size_t n, int brute,
                        uint8_t *occluded);
 
+/* Irradiance cache (shaders/raytrace.rgen:748-828, 1228-1421).
+ * orc_ic_frame renders one frame - every pixel of params at sample
+ * params->sampleOffset, 1 spp - against the cache state passed in, then
+ * runs the frame's cache updates and creations and writes the new state
+ * back (hdr, and entries/spheres of hdr->maxCaches records).  rgb/sig per
+ * pixel as in orc_render_paths.  The frame's jobs (updates first, in pixel
+ * order, then creations) and their 200 sub-path results each (L rgb, first
+ * hit distance; sig) are returned up to jobCap jobs; any of them may be NULL. */
+typedef struct orc_ic_job {
+    int32_t type;    /* 0 = update of entry `target`, 1 = creation */
+    uint32_t target;
+    double origin[3];
+    double normal[3];
+} orc_ic_job;
+
+int orc_ic_frame(const orc_scene *scene, const rtx_frame_params *params, int prec, int threads,
+                 rtx_cache_header *hdr, rtx_cache_data *entries, rtx_sphere *spheres, double *rgb,
+                 uint32_t *sig, orc_ic_job *jobs, size_t jobCap, size_t *numJobs, double *sub,
+                 uint32_t *subSig);
+/* The accumulation and commit of a frame's jobs, given their sub-path
+ * results (4 doubles each, 200 per job): the state is updated in place. */
+int orc_ic_apply(const rtx_frame_params *params, int prec, rtx_cache_header *hdr, rtx_cache_data *entries,
+                 rtx_sphere *spheres, const orc_ic_job *jobs, size_t numJobs, const double *sub);
+/* calculateCacheData's accumulation over one job's 200 sub-path results
+ * (sub: 4 doubles each, stratum order k * 10 + j). */
+int orc_ic_reduce(int prec, uint64_t seed, uint32_t sample, uint32_t job, const double normal[3],
+                  const double *sub, double color[3], double rot[3], double trans[3], double *harmonicR);
+/* calculateCacheData (raytrace.rgen:1231-1316) for one point: the 200
+ * sub-paths of job `job` of frame params->sampleOffset against the given
+ * cache state, then the accumulation. */
+int orc_ic_calculate(const orc_scene *scene, const rtx_frame_params *params, int prec,
+                     const rtx_cache_header *hdr, const rtx_cache_data *entries, const rtx_sphere *spheres,
+                     uint32_t job, const double origin[3], const double normal[3], double color[3], double rot[3],
+                     double trans[3], double *harmonicR);
+/* phi, theta and the stream state after the theta draw of stratum (k, j). */
+void orc_ic_stratum(uint64_t seed, uint32_t sample, uint32_t job, int k, int j, double *phi, double *theta,
+                    uint32_t *state);
+/* Cache lookup at x with normal n: 1 and the weighted colour, or 0. */
+int orc_ic_query(const orc_scene *scene, const rtx_push_constant *pc, const rtx_cache_header *hdr,
+                 const rtx_cache_data *entries, const rtx_sphere *spheres, const double x[3], const double n[3],
+                 double color[3]);
+
 /* Building blocks, exposed for known-answer and distribution tests. */
 uint32_t orc_tea(uint32_t v0, uint32_t v1);
 uint32_t orc_lcg_step(uint32_t state);

@@ -114,6 +114,27 @@ constexpr int kMaxRejections = 65536;
 constexpr uint32_t kSigVisible = 0x1b873593u;
 constexpr uint32_t kSigOccluded = 0x5bd1e995u;
 constexpr uint32_t kSigProbe = 0x85ebca6bu;
+constexpr uint32_t kSigCacheHit = 0xc2b2ae35u; // an irradiance-cache lookup that ended the path
+
+// Irradiance cache (shaders/raytrace.rgen:1228-1421, src/IrradianceCache.h:17-30)
+constexpr int kMaxNewCacheEntries = 5;  // MAX_NEW_IRRADIANCE_ENTRIES, raytrace.rgen:65
+constexpr int kIcStrataPhi = 20;        // N in calculateCacheData, raytrace.rgen:1232
+constexpr int kIcStrataTheta = 10;      // M, raytrace.rgen:1233
+constexpr int kIcStrata = kIcStrataPhi * kIcStrataTheta;
+// Keys of the per-stratum random streams of calculateCacheData: the reference
+// draws the 20 phi jitters and 200 sub-paths from one sequential per-pixel
+// stream; here each phi column and each stratum owns a TEA-seeded stream so
+// the 200 sub-paths of a cache record run in parallel.
+constexpr uint32_t kIcPhiKey = 0x9e3779b9u;
+constexpr uint32_t kIcStratumKey = 0x7f4a7c15u;
+
+// The cache as a frame reads it: the reference's FP32 buffers.
+struct IcSnapshot {
+    rtx_cache_header hdr;
+    std::vector<rtx_cache_data> e;
+    std::vector<rtx_sphere> sph;
+    uint32_t count() const { return std::min(hdr.nextCacheSlot, hdr.maxCaches); }
+};
 
 // ------------------------------------------------------------------ scene
 struct Node {
@@ -420,6 +441,11 @@ template <class R> struct Tracer {
     const rtx_push_constant &pc;
     uint32_t seed = 0;
     uint32_t primHash = 2166136261u;
+    // irradiance cache: the state this frame reads, and what the path asks of it
+    const IcSnapshot *ic = nullptr;
+    bool icElect = false;
+    int numCand = 0;
+    V3<R> candO[kMaxNewCacheEntries], candN[kMaxNewCacheEntries];
 
     Tracer(const orc_scene &s, const rtx_frame_params &f) : S(s), fp(f), pc(f.pc) {}
 
@@ -451,6 +477,9 @@ template <class R> struct Tracer {
         V3<R> x, y;
         coordinateAxis(n, x, y);
         return v.x * x + v.y * y + v.z * n;
+    }
+    static V3<R> sphericalToCartesian(R theta, R phi) {
+        return V3<R>(std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta));
     }
 
     // ref: shaders/random.glsl:60-68
@@ -864,19 +893,73 @@ template <class R> struct Tracer {
         return r / (R)numNEE;
     }
 
-    // The bounce loop, ref: shaders/raytrace.rgen:992-1226 for the plain
-    // method (no irradiance cache, ADRRS, splitting or guiding).  Russian
-    // roulette (absent from the reference) is applied when pc.enableRR is set.
-    V3<R> raytrace(V3<R> origin, V3<R> direction) {
-        const int maxDepth = pc.maxDepth, maxFollowDiscrete = pc.maxFollowDiscrete;
-        const bool useNEE = pc.enableNEE != 0;
-        const bool addDirectLights = !useNEE;
-        const int numNEE = pc.numNEE;
+    // ---------------------------------------------------- irradiance cache
+    // ref: raytrace.rgen:806-813
+    bool isICCapable(const rtx_material &m) const {
+        if (m.type == RTX_MAT_DIFFUSE || m.type == RTX_MAT_LIGHT) return true;
+        return pc.useIrradianceCacheOnGlossy && !hasDiscreteDirection(m);
+    }
+    // ref: raytrace.rgen:815-828
+    static V3<R> approxDiffuse(const rtx_material &m, V3<R> normal, V3<R> wi) {
+        switch (m.type) {
+        case RTX_MAT_DIFFUSE:
+        case RTX_MAT_LIGHT:
+        case RTX_MAT_PHONG: return diffuse(m);
+        case RTX_MAT_ROUGH_CONDUCTOR: {
+            R f = fresnelConductor(dot(wi, normal), (R)m.eta, (R)m.k);
+            return V3<R>(f, f, f) * C::INV_PI;
+        }
+        }
+        return V3<R>(-1, -1, -1);
+    }
+    // queryIrradianceCache (raytrace.rgen:748-773): every cache sphere that
+    // contains x (raytrace.irradiance.rint:11-21) goes through the any-hit
+    // weighting of raytrace.irradiance.rahit:18-51, here in ascending entry order.
+    bool queryIC(V3<R> x, V3<R> n, V3<R> &color) const {
+        V3<R> sum(0, 0, 0);
+        R total = 0;
+        const uint32_t count = ic->count();
+        for (uint32_t i = 0; i < count; ++i) {
+            const rtx_sphere &sp = ic->sph[i];
+            const rtx_cache_data &e = ic->e[i];
+            V3<R> c = f3<R>(sp.center);
+            if (!(length(x - c) <= (R)sp.radius)) continue;
+            V3<R> ni = f3<R>(e.normal);
+            R w = R(1) / (length(x - c) / (R)e.harmonicR + std::sqrt(R(1) - dot(n, ni)));
+            if (std::isnan(w) || std::isinf(w)) w = R(1000000);
+            if (w <= R(1) / (R)pc.irradianceA) continue;
+            if (pc.irradianceCachePerformVisibilityCheck && !(R(-0.001) <= dot(x - c, (n + ni) / R(2)))) continue;
+            if (pc.useIrradianceGradients) {
+                V3<R> col = f3<R>(e.color);
+                R E = length(col);
+                V3<R> cn = E != R(0) ? normalize(col) : V3<R>(0, 0, 0);
+                V3<R> adjusted = cn * (E + dot(cross(ni, n), f3<R>(e.rotGrad)) + dot(x - c, f3<R>(e.transGrad)));
+                sum = sum + w * adjusted;
+            } else {
+                sum = sum + w * f3<R>(e.color);
+            }
+            total += w;
+        }
+        if (total > R(0)) {
+            color = sum / total;
+            return true;
+        }
+        return false;
+    }
+
+    // The bounce loop, ref: shaders/raytrace.rgen:992-1226: the plain method
+    // plus the irradiance-cache branch (1080-1139); no ADRRS, splitting or
+    // guiding yet.  Russian roulette (absent from the reference) is applied
+    // to camera paths when pc.enableRR is set.
+    V3<R> raytrace(V3<R> origin, V3<R> direction, V3<R> throughput, int maxDepth, int maxFollowDiscrete,
+                   bool addDirectLights, bool addFirstHitLight, bool useNEE, int numNEE, bool useIC, bool createIC,
+                   bool allowRR, R &firstT) {
         bool follow = true;
         int followCount = 0;
-        bool addNextDirectLights = true; // addFirstHitLight
+        bool addNextDirectLights = addFirstHitLight;
         int depth = 0;
-        V3<R> throughput(1, 1, 1), result(0, 0, 0);
+        firstT = C::TMAX;
+        V3<R> result(0, 0, 0);
         do {
             depth++;
             HitInfo<R> info = trace(origin, direction, C::TMIN, C::TMAX);
@@ -888,6 +971,7 @@ template <class R> struct Tracer {
             const rtx_material &m = S.mats[info.matIndex];
             origin = info.worldPos;
             V3<R> normal = info.normal;
+            if (depth == 1) firstT = info.t;
             if (m.type == RTX_MAT_LIGHT && (addDirectLights || addNextDirectLights))
                 result = result + throughput * f3<R>(m.lightColor);
             if (hasDiscreteDirection(m)) {
@@ -897,6 +981,23 @@ template <class R> struct Tracer {
             } else {
                 addNextDirectLights = false;
                 follow = false;
+                if (useIC && isICCapable(m)) {
+                    V3<R> irradiance;
+                    if (queryIC(origin, normal, irradiance)) {
+                        // cache record found: the path ends here (raytrace.rgen:1114-1122)
+                        primHash = (primHash ^ kSigCacheHit) * 16777619u;
+                        result = result + throughput * approxDiffuse(m, normal, -direction) * irradiance;
+                        result = result + throughput * multipleNEE(m, origin, -direction, normal, info.isFrontFace,
+                                                                   numNEE);
+                        break;
+                    } else if (createIC && rnd() < (R)pc.irradianceCreateProb) {
+                        if (numCand < kMaxNewCacheEntries) { // raytrace.rgen:1125-1137
+                            candO[numCand] = origin;
+                            candN[numCand] = normal;
+                            numCand++;
+                        }
+                    }
+                }
                 if (useNEE) {
                     V3<R> nee = multipleNEE(m, origin, -direction, normal, info.isFrontFace, numNEE);
                     result = result + throughput * nee;
@@ -911,7 +1012,7 @@ template <class R> struct Tracer {
             if (std::isnan(change.x) || std::isnan(change.y) || std::isnan(change.z)) break;
             throughput = throughput * change;
             direction = newDirection;
-            if (pc.enableRR && depth >= fp.rrStartDepth) {
+            if (allowRR && pc.enableRR && depth >= fp.rrStartDepth) {
                 R q = std::max(std::fabs(throughput.x), std::max(std::fabs(throughput.y), std::fabs(throughput.z)));
                 q = std::min(q, R(0.95));
                 if (rnd() >= q) break;
@@ -943,9 +1044,14 @@ template <class R> struct Tracer {
         uint32_t frameSeed = pcg32First(fp.seed, sample);
         seed = tea(pixel, frameSeed);
         primHash = 2166136261u;
+        const bool useIC = pc.useIrradianceCache != 0;
+        // raytrace.rgen:1637-1639: the update election is the stream's first draw
+        if (useIC && rnd() < (R)pc.irradianceUpdateProb) icElect = true;
         V3<R> o, d;
         cameraRay(pixel % fp.width, pixel / fp.width, fp.width, fp.height, o, d);
-        return raytrace(o, d);
+        R firstT;
+        return raytrace(o, d, V3<R>(1, 1, 1), pc.maxDepth, pc.maxFollowDiscrete, pc.enableNEE == 0, true,
+                        pc.enableNEE != 0, pc.numNEE, useIC, useIC, true, firstT);
     }
 };
 
@@ -958,13 +1064,21 @@ int checkScene(const orc_scene &s) {
     return RTX_OK;
 }
 
-int checkParams(const rtx_frame_params *fp) {
+int checkParams(const rtx_frame_params *fp, bool cacheFrame = false) {
     const rtx_push_constant &pc = fp->pc;
     if (fp->width == 0 || fp->height == 0) {
         g_err = "empty image";
         return RTX_E_INVALID;
     }
-    if (pc.useIrradianceCache || pc.useADRRS || pc.splitOnFirst || pc.useGuiding || pc.updateGuiding ||
+    if (pc.useIrradianceCache && !cacheFrame) {
+        g_err = "oracle: irradiance-cache frames go through orc_ic_frame";
+        return RTX_E_UNSUPPORTED;
+    }
+    if (pc.useIrradianceCache && (pc.numNEE < 1 || pc.irradianceNumNEE < 1)) {
+        g_err = "numNEE and irradianceNumNEE must be >= 1 with the irradiance cache";
+        return RTX_E_INVALID;
+    }
+    if (pc.useADRRS || pc.splitOnFirst || pc.useGuiding || pc.updateGuiding ||
         pc.visualizeMode != 0 || pc.showIrradianceCacheOnly || pc.storeEstimate) {
         g_err = "oracle: only the plain method is restated so far";
         return RTX_E_UNSUPPORTED;
@@ -998,6 +1112,250 @@ template <class F> void parallelFor(size_t n, int threads, F fn) {
 }
 
 template <class R> V3<R> vd(const double *p) { return V3<R>((R)p[0], (R)p[1], (R)p[2]); }
+
+// ------------------------------------------------- irradiance-cache frame
+// One frame (one 1-spp sample plane) of main() with useIrradianceCache
+// (raytrace.rgen:1623-1827): every pixel's path reads the cache as it was at
+// the start of the frame; afterwards the frame's update elections (in pixel
+// order) and creation candidates (in pixel, then path order) become jobs of
+// calculateCacheData, whose results are committed in job order.  The
+// reference does the same work with atomics from every pixel at once, so
+// its order (and which candidates win the last free slots) is a race; this
+// order is the deterministic restatement of it.
+struct IcJobRec {
+    int type; // 0 = update (updateIrradianceCache, raytrace.rgen:1334-1381), 1 = create (1383-1421)
+    uint32_t target;
+    double o[3], n[3];
+};
+
+template <class R> struct IcResult {
+    V3<R> color, rot, trans;
+    R harmonicR;
+};
+
+// The stratum (k, j) of job `job`: phi from the column stream, theta from
+// the stratum stream, whose state then drives the sub-path.
+template <class R> void icStratum(uint32_t frameSeed, uint32_t job, int k, int j, R &phi, R &theta, uint32_t &state) {
+    using C = Consts<R>;
+    uint32_t ps = tea(job * kIcStrataPhi + (uint32_t)k, frameSeed ^ kIcPhiKey);
+    ps = 1664525u * ps + 1013904223u;
+    R uphi = R(ps & 0x00FFFFFFu) / R(16777216.0);
+    phi = R(2) * C::PI * ((R)k + uphi) / (R)kIcStrataPhi; // raytrace.rgen:1247
+    uint32_t st = tea((job * kIcStrataPhi + (uint32_t)k) * kIcStrataTheta + (uint32_t)j, frameSeed ^ kIcStratumKey);
+    st = 1664525u * st + 1013904223u;
+    R utheta = R(st & 0x00FFFFFFu) / R(16777216.0);
+    theta = std::asin(std::sqrt(((R)j + utheta) / (R)kIcStrataTheta)); // raytrace.rgen:1257
+    state = st;
+}
+
+// One sub-path of calculateCacheData (raytrace.rgen:1255-1263): raytrace with
+// maxDepth 1, maxFollowDiscrete 10, no direct light on hits, NEE with
+// irradianceNumNEE samples, cache lookups on, no new candidates.
+template <class R>
+void icSubPath(const orc_scene &S, const rtx_frame_params &fp, const IcSnapshot &snap, uint32_t frameSeed,
+               uint32_t job, const IcJobRec &jb, int k, int j, double out[4], uint32_t sig[2]) {
+    Tracer<R> t(S, fp);
+    t.ic = &snap;
+    R phi, theta;
+    icStratum<R>(frameSeed, job, k, j, phi, theta, t.seed);
+    t.primHash = 2166136261u;
+    V3<R> n = vd<R>(jb.n), o = vd<R>(jb.o);
+    V3<R> dir = Tracer<R>::toWorld(Tracer<R>::sphericalToCartesian(theta, phi), n);
+    R r;
+    V3<R> L = t.raytrace(o, dir, V3<R>(1, 1, 1), 1, 10, false, false, true, fp.pc.irradianceNumNEE, true, false,
+                         false, r);
+    out[0] = (double)L.x;
+    out[1] = (double)L.y;
+    out[2] = (double)L.z;
+    out[3] = (double)r;
+    sig[0] = t.seed;
+    sig[1] = t.primHash;
+}
+
+// The accumulation half of calculateCacheData (raytrace.rgen:1264-1315) over
+// the 200 sub-path results (L rgb, first-hit distance r) of one job, in the
+// reference's loop order.  Column k-1's values feed the k-gradient as the
+// reference's previousKLs/previousKRs intend.
+template <class R>
+IcResult<R> icReduce(uint32_t frameSeed, uint32_t job, V3<R> normal, const double *sub) {
+    using C = Consts<R>;
+    const int N = kIcStrataPhi, M = kIcStrataTheta;
+    R invDistanceSum = 0;
+    int numDistances = 0;
+    IcResult<R> res;
+    res.rot = V3<R>(0, 0, 0);
+    res.trans = V3<R>(0, 0, 0);
+    V3<R> color(0, 0, 0);
+    R prevKL[kIcStrataTheta] = {}, prevKR[kIcStrataTheta] = {};
+    for (int k = 0; k < N; ++k) {
+        R phi, theta;
+        uint32_t st;
+        icStratum<R>(frameSeed, job, k, 0, phi, theta, st);
+        V3<R> uk = Tracer<R>::toWorld(Tracer<R>::sphericalToCartesian(C::HALF_PI, phi), normal);
+        V3<R> vk = Tracer<R>::toWorld(Tracer<R>::sphericalToCartesian(C::HALF_PI, phi + C::HALF_PI), normal);
+        V3<R> prevVk = Tracer<R>::toWorld(
+            Tracer<R>::sphericalToCartesian(C::HALF_PI, R(2) * C::PI * (R)k / (R)N + C::HALF_PI), normal);
+        R prevJR = 0, prevJL = 0;
+        for (int j = 0; j < M; ++j) {
+            icStratum<R>(frameSeed, job, k, j, phi, theta, st);
+            const double *o = sub + 4 * (size_t)(k * M + j);
+            V3<R> sample((R)o[0], (R)o[1], (R)o[2]);
+            R r = (R)o[3];
+            color = color + sample;
+            if (r < C::TMAX) {
+                invDistanceSum += R(1) / r;
+                numDistances++;
+            }
+            R L = length(sample);
+            R prevJTheta = std::asin(std::sqrt((R)j / (R)M));
+            R nextJTheta = std::asin(std::sqrt((R)(j + 1) / (R)M));
+            R tanTheta = std::tan(theta);
+            if (std::isinf(tanTheta) || std::isnan(tanTheta)) tanTheta = 0;
+            res.rot = res.rot - tanTheta * L * vk;
+            if (j > 0) {
+                R cosPrev = std::cos(prevJTheta);
+                res.trans = res.trans + uk * R(2) * C::PI / (R)N * std::sin(prevJTheta) * cosPrev * cosPrev /
+                                            std::min(r, prevJR) * (L - prevJL);
+            }
+            if (k > 0)
+                res.trans = res.trans + prevVk * (std::sin(nextJTheta) - std::sin(prevJTheta)) /
+                                            std::min(r, prevKR[j]) * (L - prevKL[j]);
+            prevKL[j] = L;
+            prevKR[j] = r;
+            prevJL = L;
+            prevJR = r;
+        }
+    }
+    R normFactor = C::PI / (R)(M * N);
+    res.color = normFactor * color;
+    res.rot = res.rot * normFactor;
+    if (invDistanceSum == R(0) || numDistances == 0) res.harmonicR = R(-1);
+    else res.harmonicR = R(1) / (invDistanceSum / (R)numDistances);
+    return res;
+}
+
+template <class R> void clampGradients(const rtx_push_constant &pc, V3<R> &rot, V3<R> &trans) { // raytrace.rgen:1318-1329
+    R lr = length(rot);
+    if (lr > (R)pc.irradianceGradientsMaxLength) rot = rot * ((R)pc.irradianceGradientsMaxLength / lr);
+    R lt = length(trans);
+    if (lt > (R)pc.irradianceGradientsMaxLength) trans = trans * ((R)pc.irradianceGradientsMaxLength / lt);
+}
+
+template <class R> V3<R> mixv(V3<R> x, V3<R> y, R a) { return x * (R(1) - a) + y * a; }
+template <class R> R mixs(R x, R y, R a) { return x * (R(1) - a) + y * a; }
+template <class R> void store3(float *d, V3<R> v) {
+    d[0] = (float)v.x;
+    d[1] = (float)v.y;
+    d[2] = (float)v.z;
+}
+
+// Commit one job's result into the cache buffers (FP32, as the reference's).
+template <class R> void icCommit(IcSnapshot &live, const rtx_push_constant &pc, const IcJobRec &jb, IcResult<R> r) {
+    if (jb.type == 0) { // updateIrradianceCache, raytrace.rgen:1349-1380
+        rtx_cache_data &e = live.e[jb.target];
+        R a = std::min((R)e.numUpdates / (R)(e.numUpdates + 1), R(0.95));
+        store3(e.color, mixv(r.color, f3<R>(e.color), a));
+        V3<R> rot = mixv(r.rot, f3<R>(e.rotGrad), a), trans = mixv(r.trans, f3<R>(e.transGrad), a);
+        clampGradients(pc, rot, trans);
+        store3(e.rotGrad, rot);
+        store3(e.transGrad, trans);
+        R h = r.harmonicR > R(0) ? mixs(r.harmonicR, (R)e.harmonicR, a) : (R)e.harmonicR;
+        h = std::max(h, (R)pc.irradianceCacheMinRadius);
+        e.harmonicR = (float)h;
+        live.sph[jb.target].radius = (float)((R)pc.irradianceA * h);
+        e.numUpdates += 1;
+        return;
+    }
+    // createIrradianceCache, raytrace.rgen:1383-1421 (slots stay below maxCaches)
+    if (r.harmonicR < R(0)) return;
+    uint32_t slot = live.hdr.nextCacheSlot;
+    if (slot >= live.hdr.maxCaches) return;
+    live.hdr.nextCacheSlot = slot + 1;
+    R h = std::max(r.harmonicR, (R)pc.irradianceCacheMinRadius);
+    clampGradients(pc, r.rot, r.trans);
+    rtx_cache_data &e = live.e[slot];
+    for (int a = 0; a < 3; ++a) e.normal[a] = (float)jb.n[a];
+    store3(e.color, r.color);
+    e.harmonicR = (float)h;
+    store3(e.rotGrad, r.rot);
+    store3(e.transGrad, r.trans);
+    e.numUpdates = 1;
+    rtx_sphere &sp = live.sph[slot];
+    for (int a = 0; a < 3; ++a) sp.center[a] = (float)jb.o[a];
+    sp.radius = (float)((R)pc.irradianceA * h);
+}
+
+template <class R>
+void icFrame(const orc_scene &S, const rtx_frame_params &fp, IcSnapshot &live, int threads, double *rgb,
+             uint32_t *sig, std::vector<IcJobRec> &jobs, std::vector<double> &sub, std::vector<uint32_t> &subSig) {
+    const size_t npix = (size_t)fp.width * fp.height;
+    const IcSnapshot snap = live;
+    const uint32_t frameSeed = pcg32First(fp.seed, fp.sampleOffset);
+    std::vector<uint8_t> elect(npix);
+    std::vector<int> ncand(npix);
+    std::vector<std::array<double, 6 * kMaxNewCacheEntries>> cand(npix);
+    parallelFor(npix, threads, [&](size_t p) {
+        Tracer<R> t(S, fp);
+        t.ic = &snap;
+        V3<R> v = t.renderPath(p);
+        rgb[3 * p + 0] = (double)v.x;
+        rgb[3 * p + 1] = (double)v.y;
+        rgb[3 * p + 2] = (double)v.z;
+        if (sig) {
+            sig[2 * p + 0] = t.seed;
+            sig[2 * p + 1] = t.primHash;
+        }
+        elect[p] = t.icElect ? 1 : 0;
+        ncand[p] = t.numCand;
+        for (int c = 0; c < t.numCand; ++c) {
+            toD(t.candO[c], &cand[p][6 * c]);
+            toD(t.candN[c], &cand[p][6 * c + 3]);
+        }
+    });
+    jobs.clear();
+    const uint32_t count = live.count();
+    uint32_t numUpdates = 0; // at most maxCaches update jobs a frame
+    for (size_t p = 0; p < npix && numUpdates < live.hdr.maxCaches; ++p) { // raytrace.rgen:1335-1345
+        if (!elect[p]) continue;
+        uint32_t idx = live.hdr.nextUpdateSlot++;
+        if (idx >= count) {
+            live.hdr.nextUpdateSlot = 0;
+            continue;
+        }
+        if (live.e[idx].numUpdates < 1) continue;
+        IcJobRec jb{0, idx, {}, {}};
+        for (int a = 0; a < 3; ++a) {
+            jb.o[a] = live.sph[idx].center[a];
+            jb.n[a] = live.e[idx].normal[a];
+        }
+        jobs.push_back(jb);
+        numUpdates++;
+    }
+    if (count < live.hdr.maxCaches) { // createIrradianceCache's early exit, raytrace.rgen:1384-1386
+        uint32_t room = live.hdr.maxCaches - count, made = 0;
+        for (size_t p = 0; p < npix && made < room; ++p)
+            for (int c = 0; c < ncand[p] && made < room; ++c, ++made) {
+                IcJobRec jb{1, 0, {}, {}};
+                for (int a = 0; a < 3; ++a) {
+                    jb.o[a] = cand[p][6 * c + a];
+                    jb.n[a] = cand[p][6 * c + 3 + a];
+                }
+                jobs.push_back(jb);
+            }
+    }
+    const size_t nsub = jobs.size() * kIcStrata;
+    sub.assign(nsub * 4, 0.0);
+    subSig.assign(nsub * 2, 0u);
+    parallelFor(nsub, threads, [&](size_t i) {
+        uint32_t job = (uint32_t)(i / kIcStrata), s = (uint32_t)(i % kIcStrata);
+        icSubPath<R>(S, fp, snap, frameSeed, job, jobs[job], (int)(s / kIcStrataTheta), (int)(s % kIcStrataTheta),
+                     &sub[4 * i], &subSig[2 * i]);
+    });
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        IcResult<R> r = icReduce<R>(frameSeed, (uint32_t)j, vd<R>(jobs[j].n), &sub[4 * j * kIcStrata]);
+        icCommit<R>(live, fp.pc, jobs[j], r);
+    }
+}
 
 } // namespace
 
@@ -1176,6 +1534,147 @@ void orc_camera_ray(const rtx_camera *cam, uint32_t w, uint32_t h, uint32_t px, 
     *state = t.seed;
     toD(o, org);
     toD(d, dir);
+}
+
+int orc_ic_frame(const orc_scene *s, const rtx_frame_params *fp, int prec, int threads, rtx_cache_header *hdr,
+                 rtx_cache_data *entries, rtx_sphere *spheres, double *rgb, uint32_t *sig, orc_ic_job *jobs,
+                 size_t jobCap, size_t *numJobs, double *sub, uint32_t *subSig) {
+    if (!s || !fp || !hdr || !entries || !spheres || !rgb) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    int rc = checkParams(fp, true);
+    if (rc != RTX_OK) return rc;
+    if (!fp->pc.useIrradianceCache) {
+        g_err = "orc_ic_frame needs pc.useIrradianceCache";
+        return RTX_E_INVALID;
+    }
+    if (prec != 32 && prec != 64) {
+        g_err = "prec must be 32 or 64";
+        return RTX_E_INVALID;
+    }
+    IcSnapshot live;
+    live.hdr = *hdr;
+    live.e.assign(entries, entries + hdr->maxCaches);
+    live.sph.assign(spheres, spheres + hdr->maxCaches);
+    std::vector<IcJobRec> jr;
+    std::vector<double> sb;
+    std::vector<uint32_t> ss;
+    if (prec == 64) icFrame<double>(*s, *fp, live, threads, rgb, sig, jr, sb, ss);
+    else icFrame<float>(*s, *fp, live, threads, rgb, sig, jr, sb, ss);
+    *hdr = live.hdr;
+    std::copy(live.e.begin(), live.e.end(), entries);
+    std::copy(live.sph.begin(), live.sph.end(), spheres);
+    if (numJobs) *numJobs = jr.size();
+    const size_t keep = std::min(jobCap, jr.size());
+    for (size_t j = 0; j < keep; ++j) {
+        if (jobs) {
+            jobs[j].type = jr[j].type;
+            jobs[j].target = jr[j].target;
+            for (int a = 0; a < 3; ++a) {
+                jobs[j].origin[a] = jr[j].o[a];
+                jobs[j].normal[a] = jr[j].n[a];
+            }
+        }
+        if (sub) std::copy(sb.begin() + 4 * j * kIcStrata, sb.begin() + 4 * (j + 1) * kIcStrata, sub + 4 * j * kIcStrata);
+        if (subSig)
+            std::copy(ss.begin() + 2 * j * kIcStrata, ss.begin() + 2 * (j + 1) * kIcStrata, subSig + 2 * j * kIcStrata);
+    }
+    return RTX_OK;
+}
+
+int orc_ic_calculate(const orc_scene *s, const rtx_frame_params *fp, int prec, const rtx_cache_header *hdr,
+                     const rtx_cache_data *entries, const rtx_sphere *spheres, uint32_t job, const double origin[3],
+                     const double normal[3], double color[3], double rot[3], double trans[3], double *harmonicR) {
+    if (!s || !fp || !hdr || !entries || !spheres || !origin || !normal) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    int rc = checkParams(fp, true);
+    if (rc != RTX_OK) return rc;
+    IcSnapshot snap;
+    snap.hdr = *hdr;
+    snap.e.assign(entries, entries + hdr->maxCaches);
+    snap.sph.assign(spheres, spheres + hdr->maxCaches);
+    IcJobRec jb{1, 0, {origin[0], origin[1], origin[2]}, {normal[0], normal[1], normal[2]}};
+    const uint32_t frameSeed = pcg32First(fp->seed, fp->sampleOffset);
+    std::vector<double> sub(4 * kIcStrata);
+    uint32_t sig[2];
+    for (int st = 0; st < kIcStrata; ++st) {
+        if (prec == 64)
+            icSubPath<double>(*s, *fp, snap, frameSeed, job, jb, st / kIcStrataTheta, st % kIcStrataTheta, &sub[4 * st], sig);
+        else
+            icSubPath<float>(*s, *fp, snap, frameSeed, job, jb, st / kIcStrataTheta, st % kIcStrataTheta, &sub[4 * st], sig);
+    }
+    return orc_ic_reduce(prec, fp->seed, fp->sampleOffset, job, normal, sub.data(), color, rot, trans, harmonicR);
+}
+
+int orc_ic_apply(const rtx_frame_params *fp, int prec, rtx_cache_header *hdr, rtx_cache_data *entries,
+                 rtx_sphere *spheres, const orc_ic_job *jobs, size_t numJobs, const double *sub) {
+    if (!fp || !hdr || !entries || !spheres || (numJobs && (!jobs || !sub))) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    IcSnapshot live;
+    live.hdr = *hdr;
+    live.e.assign(entries, entries + hdr->maxCaches);
+    live.sph.assign(spheres, spheres + hdr->maxCaches);
+    const uint32_t frameSeed = pcg32First(fp->seed, fp->sampleOffset);
+    for (size_t j = 0; j < numJobs; ++j) {
+        IcJobRec jb{jobs[j].type, jobs[j].target, {}, {}};
+        for (int a = 0; a < 3; ++a) {
+            jb.o[a] = jobs[j].origin[a];
+            jb.n[a] = jobs[j].normal[a];
+        }
+        const double *sj = sub + 4 * j * kIcStrata;
+        if (prec == 64) icCommit<double>(live, fp->pc, jb, icReduce<double>(frameSeed, (uint32_t)j, vd<double>(jb.n), sj));
+        else icCommit<float>(live, fp->pc, jb, icReduce<float>(frameSeed, (uint32_t)j, vd<float>(jb.n), sj));
+    }
+    *hdr = live.hdr;
+    std::copy(live.e.begin(), live.e.end(), entries);
+    std::copy(live.sph.begin(), live.sph.end(), spheres);
+    return RTX_OK;
+}
+
+int orc_ic_reduce(int prec, uint64_t seed, uint32_t sample, uint32_t job, const double normal[3], const double *sub,
+                  double color[3], double rot[3], double trans[3], double *harmonicR) {
+    uint32_t frameSeed = pcg32First(seed, sample);
+    if (prec == 64) {
+        IcResult<double> r = icReduce<double>(frameSeed, job, vd<double>(normal), sub);
+        toD(r.color, color);
+        toD(r.rot, rot);
+        toD(r.trans, trans);
+        *harmonicR = r.harmonicR;
+    } else {
+        IcResult<float> r = icReduce<float>(frameSeed, job, vd<float>(normal), sub);
+        toD(r.color, color);
+        toD(r.rot, rot);
+        toD(r.trans, trans);
+        *harmonicR = r.harmonicR;
+    }
+    return RTX_OK;
+}
+
+void orc_ic_stratum(uint64_t seed, uint32_t sample, uint32_t job, int k, int j, double *phi, double *theta,
+                    uint32_t *state) {
+    icStratum<double>(pcg32First(seed, sample), job, k, j, *phi, *theta, *state);
+}
+
+int orc_ic_query(const orc_scene *s, const rtx_push_constant *pc, const rtx_cache_header *hdr,
+                 const rtx_cache_data *entries, const rtx_sphere *spheres, const double x[3], const double n[3],
+                 double color[3]) {
+    IcSnapshot snap;
+    snap.hdr = *hdr;
+    snap.e.assign(entries, entries + hdr->maxCaches);
+    snap.sph.assign(spheres, spheres + hdr->maxCaches);
+    rtx_frame_params fp = defaultParams();
+    fp.pc = *pc;
+    Tracer<double> t(s ? *s : emptyScene(), fp);
+    t.ic = &snap;
+    V3<double> c;
+    if (!t.queryIC(vd<double>(x), vd<double>(n), c)) return 0;
+    toD(c, color);
+    return 1;
 }
 
 double orc_fresnel(double eta, double c) { return Tracer<double>::fresnel(eta, c); }

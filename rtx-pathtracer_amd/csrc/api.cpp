@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "dev_math.h"
 #include "internal.h"
 
 using namespace rtx;
@@ -60,6 +61,18 @@ struct rtx_ctx {
     hipEvent_t tStart = nullptr, tStop = nullptr;
     int cuCount = 256;
     rtx_tuning tuning{1, 16, 8, 1u << 22, {0, 0, 0, 0}};
+    // irradiance cache (rtx_ic_create)
+    IcBuffers ic{};
+    std::vector<void *> icAllocs;
+    bool icValid = false;
+    FrameConst *fcIC = nullptr;      // constants of the cache's stratum sub-paths
+    float4 *candBuf = nullptr;       // per-path creation candidates of a batch
+    size_t candCap = 0;
+    // sub-path results of the last cache frame of a render with RTX_FRAME_KEEP_PATHS
+    float4 *keepSub = nullptr;
+    uint2 *keepSubSig = nullptr;
+    size_t keepSubCap = 0;
+    uint32_t keptJobs = 0, keptUpdates = 0;
 };
 
 namespace {
@@ -210,8 +223,28 @@ int ensureBatchBuffers(rtx_ctx *ctx, size_t paths, uint32_t numNEE) {
     return RTX_OK;
 }
 
+DevIC icView(const IcBuffers &ic) {
+    DevIC v;
+    v.entries = ic.entries;
+    v.spheres = ic.spheres;
+    v.bucketStart = ic.bucketStart;
+    v.items = ic.items;
+    v.meta = ic.meta;
+    v.nb = ic.nb;
+    return v;
+}
+
+// Level-0 cell of the lookup index: the diameter of the smallest record
+// (irradianceA * irradianceCacheMinRadius), or 1e-3 when that is 0.
+float icBaseCell(const rtx_push_constant &pc) {
+    float c0 = 2.0f * pc.irradianceA * pc.irradianceCacheMinRadius;
+    return c0 > 1e-6f ? c0 : 1e-3f;
+}
+
+// Paths of one wavefront batch of the cache's sub-paths: whole jobs.
+constexpr uint32_t kMaxBlockSums = (1u << 26) / 256 + 1;
+
 const char *unsupportedFeature(const rtx_push_constant &pc) {
-    if (pc.useIrradianceCache) return "useIrradianceCache";
     if (pc.useADRRS) return "useADRRS";
     if (pc.splitOnFirst) return "splitOnFirst";
     if (pc.useGuiding) return "useGuiding";
@@ -319,6 +352,11 @@ int rtx_ctx_destroy(rtx_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     freeAll(ctx->batchAllocs);
+    freeAll(ctx->icAllocs);
+    if (ctx->fcIC) (void)hipFree(ctx->fcIC);
+    if (ctx->candBuf) (void)hipFree(ctx->candBuf);
+    if (ctx->keepSub) (void)hipFree(ctx->keepSub);
+    if (ctx->keepSubSig) (void)hipFree(ctx->keepSubSig);
     if (ctx->accum) (void)hipFree(ctx->accum);
     if (ctx->resolved) (void)hipFree(ctx->resolved);
     if (ctx->keepRgb) (void)hipFree(ctx->keepRgb);
@@ -481,9 +519,21 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         return fail(ctx, RTX_E_UNSUPPORTED, std::string("feature not supported yet: ") + f);
     if (p->pc.maxDepth < 0 || p->pc.maxDepth > 200 || p->pc.maxFollowDiscrete < 0 || p->pc.maxFollowDiscrete > 50)
         return fail(ctx, RTX_E_INVALID, "maxDepth must be in [0, 200] and maxFollowDiscrete in [0, 50]");
-    uint32_t numNEE = p->pc.enableNEE ? (uint32_t)p->pc.numNEE : 1u;
-    if (p->pc.enableNEE && (p->pc.numNEE < 1 || p->pc.numNEE > 8))
+    if (p->batchPaths > (1u << 26)) return fail(ctx, RTX_E_INVALID, "batchPaths must be at most 2^26");
+    const bool useIC = p->pc.useIrradianceCache != 0;
+    // a cache hit runs multipleNEE even with NEE off (raytrace.rgen:1120-1121)
+    const bool needNEE = p->pc.enableNEE || useIC;
+    uint32_t numNEE = needNEE ? (uint32_t)p->pc.numNEE : 1u;
+    if (needNEE && (p->pc.numNEE < 1 || p->pc.numNEE > 8))
         return fail(ctx, RTX_E_INVALID, "numNEE must be in [1, 8]");
+    uint32_t neeCap = numNEE;
+    if (useIC) {
+        if (!ctx->icValid) return fail(ctx, RTX_E_STATE, "useIrradianceCache needs rtx_ic_create first");
+        if (p->pc.irradianceNumNEE < 1 || p->pc.irradianceNumNEE > 8)
+            return fail(ctx, RTX_E_INVALID, "irradianceNumNEE must be in [1, 8]");
+        if (!(p->pc.irradianceA > 0.0f)) return fail(ctx, RTX_E_INVALID, "irradianceA must be > 0");
+        neeCap = std::max(neeCap, (uint32_t)p->pc.irradianceNumNEE);
+    }
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     const uint64_t npix = (uint64_t)p->width * p->height;
     const uint64_t total = npix * p->spp;
@@ -504,13 +554,27 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         CHECK_HIP(ctx, hipMemsetAsync(ctx->accum, 0, npix * sizeof(float4), ctx->stream));
         ctx->accSamples = 0;
     }
+    // irradiance-cache prepare frames only build the cache (src/RayTracingApp.cpp:160-161)
+    const bool accumulate = !(useIC && p->pc.isIrradiancePrepareFrame);
 
-    // batch size: whole sample planes when they fit, else pixel ranges
+    // batch size: whole sample planes when they fit, else pixel ranges; with
+    // the cache one frame (sample plane) at a time, each followed by its upkeep
     uint64_t batch = p->batchPaths ? p->batchPaths : ctx->tuning.batchPaths;
     batch = std::min<uint64_t>(batch, total);
+    if (useIC) batch = std::min<uint64_t>(batch, npix);
     if (batch >= npix) batch -= batch % npix;
-    int rc = ensureBatchBuffers(ctx, (size_t)batch, numNEE);
+    size_t bufPaths = (size_t)batch;
+    if (useIC) bufPaths = std::max<size_t>(bufPaths, kIcStrata); // at least one job of sub-paths
+    int rc = ensureBatchBuffers(ctx, bufPaths, neeCap);
     if (rc != RTX_OK) return rc;
+    if (useIC && ctx->candCap < bufPaths) {
+        if (ctx->candBuf) (void)hipFree(ctx->candBuf);
+        ctx->candBuf = nullptr;
+        ctx->candCap = 0;
+        CHECK_HIP(ctx, hipMalloc(&ctx->candBuf, bufPaths * kMaxNewCacheEntries * 2 * sizeof(float4)));
+        ctx->candCap = bufPaths;
+    }
+    ctx->pb.cand = ctx->candBuf;
 
     const bool keep = (p->flags & RTX_FRAME_KEEP_PATHS) != 0;
     if (keep) {
@@ -541,7 +605,29 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     // developer aid: RTX_DEBUG_PATH=<path index within this call> prints that path's bounces
     fc.debugPath = ~0ull;
     if (const char *dbg = std::getenv("RTX_DEBUG_PATH")) fc.debugPath = std::strtoull(dbg, nullptr, 10);
+    // camera paths: raytrace(..., maxDepth, maxFollowDiscrete, !enableNEE, true, enableNEE, numNEE,
+    // useIrradianceCache, createIC = true, ...) (raytrace.rgen:1671)
+    fc.maxDepth = p->pc.maxDepth;
+    fc.maxFollow = p->pc.maxFollowDiscrete;
+    fc.addDirect = p->pc.enableNEE ? 0u : 1u;
+    fc.useNEE = p->pc.enableNEE ? 1u : 0u;
+    fc.useIC = useIC ? 1u : 0u;
+    fc.createIC = useIC ? 1u : 0u;
+    fc.allowRR = 1;
     CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));
+    // the cache's stratum sub-paths: raytrace(..., 1, 10, false, false, true, irradianceNumNEE,
+    // true, false, ...) (raytrace.rgen:1260)
+    FrameConst fcIC = fc;
+    fcIC.maxDepth = 1;
+    fcIC.maxFollow = 10;
+    fcIC.addDirect = 0;
+    fcIC.useNEE = 1;
+    fcIC.useIC = 1;
+    fcIC.createIC = 0;
+    fcIC.allowRR = 0;
+    fcIC.numNEE = useIC ? (uint32_t)p->pc.irradianceNumNEE : 1u;
+    fcIC.debugPath = ~0ull;
+    if (useIC) CHECK_HIP(ctx, hipMemcpyAsync(ctx->fcIC, &fcIC, sizeof(fcIC), hipMemcpyHostToDevice, ctx->stream));
     CHECK_HIP(ctx, hipMemsetAsync(ctx->cnt, 0, sizeof(Counters), ctx->stream));
 
     const bool countVisits = fc.countVisits != 0;
@@ -551,7 +637,10 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     const int shadeBlocks = ctx->cuCount * ctx->tuning.shadeBlocksPerCU;
     const bool persistent = ctx->tuning.traversalKernel == 1;
     const int maxIter = p->pc.maxDepth + p->pc.maxFollowDiscrete + 3;
+    const int maxIterIC = 1 + 10 + 3;
     const DevScene &S = scene->view;
+    const DevIC icv = icView(ctx->ic);
+    const float c0 = icBaseCell(p->pc);
     hipStream_t s = ctx->stream;
 
     uint32_t launches[4] = {0, 0, 0, 0};
@@ -562,24 +651,17 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         launch();
         if (ev) (void)hipEventRecord(ev->b, s);
     };
-
-    CHECK_HIP(ctx, hipEventRecord(ctx->tStart, s));
-    uint32_t batches = 0;
-    for (uint64_t first = 0; first < total; first += batch) {
-        BatchDesc b;
-        b.firstPath = first;
-        b.numPaths = (uint32_t)std::min<uint64_t>(batch, total - first);
-        b.npix = (uint32_t)npix;
+    // the bounce iterations of one batch whose extend queue 0 is filled
+    auto bounces = [&](const FrameConst *fcd, BatchDesc b, int iters) {
         int qsel = 0;
-        timed(kClsShade, [&] { launchGenerate(s, ctx->fc, b, ctx->pb, ctx->q[qsel], ctx->cnt, qsel); });
-        for (int it = 0; it < maxIter; ++it) {
+        for (int it = 0; it < iters; ++it) {
             timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
             timed(kClsExtend, [&] {
                 if (persistent) launchClosestP(s, S, ctx->q[qsel], ctx->cnt, qsel, 0, countVisits, traceBlocks);
                 else launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, countVisits, traceBlocks);
             });
             timed(kClsShade, [&] {
-                launchShade(s, S, ctx->fc, b, ctx->pb, ctx->q[qsel], ctx->q[qsel ^ 1], ctx->qs, ctx->qp, ctx->done,
+                launchShade(s, S, icv, fcd, b, ctx->pb, ctx->q[qsel], ctx->q[qsel ^ 1], ctx->qs, ctx->qp, ctx->done,
                             ctx->cnt, qsel, shadeBlocks);
             });
             timed(kClsShadow, [&] {
@@ -588,23 +670,88 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             });
             if (persistent) {
                 timed(kClsProbe, [&] { launchClosestP(s, S, ctx->qp, ctx->cnt, qsel, 2, countVisits, traceBlocks); });
-                timed(kClsProbe, [&] { launchProbeResolve(s, S, ctx->fc, ctx->qp, ctx->pb, ctx->cnt, shadeBlocks); });
+                timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, shadeBlocks); });
             } else {
-                timed(kClsProbe,
-                      [&] { launchProbe(s, S, ctx->fc, ctx->qp, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
+                timed(kClsProbe, [&] { launchProbe(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
             }
-            timed(kClsShade, [&] { launchFinish(s, ctx->fc, ctx->pb, ctx->done, ctx->cnt, shadeBlocks); });
+            timed(kClsShade, [&] { launchFinish(s, fcd, ctx->pb, ctx->done, ctx->cnt, shadeBlocks); });
             qsel ^= 1;
         }
-        timed(kClsShade, [&] {
-            launchAccumulate(s, b, ctx->pb, ctx->accum, ctx->keepRgb, ctx->keepSig, keep);
-        });
+    };
+
+    CHECK_HIP(ctx, hipEventRecord(ctx->tStart, s));
+    uint32_t batches = 0;
+    if (useIC) timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
+    const uint64_t frameLen = useIC ? npix : total; // paths per upkeep step
+    for (uint64_t f0 = 0; f0 < total; f0 += frameLen) {
+        const uint64_t fEnd = std::min(total, f0 + frameLen);
+        if (useIC) timed(kClsShade, [&] { launchIcFrameBegin(s, ctx->ic); });
+        for (uint64_t first = f0; first < fEnd; first += batch) {
+            BatchDesc b;
+            b.firstPath = first;
+            b.numPaths = (uint32_t)std::min<uint64_t>(batch, fEnd - first);
+            b.npix = (uint32_t)npix;
+            timed(kClsShade, [&] { launchGenerate(s, ctx->fc, b, ctx->pb, ctx->q[0], ctx->cnt, 0, ctx->ic.frame); });
+            bounces(ctx->fc, b, maxIter);
+            timed(kClsShade, [&] {
+                launchAccumulate(s, b, ctx->pb, accumulate ? ctx->accum : nullptr, ctx->keepRgb, ctx->keepSig, keep);
+            });
+            if (useIC) timed(kClsShade, [&] { launchIcCollect(s, ctx->ic, ctx->pb, b.numPaths); });
+            CHECK_HIP(ctx, hipGetLastError());
+            batches++;
+        }
+        if (!useIC) continue;
+        // the frame's cache upkeep
+        const uint64_t sample = f0 / npix + p->sampleOffset;
+        const uint32_t frameSeed = pcg32First(p->seed, sample);
+        timed(kClsShade, [&] { launchIcPlan(s, ctx->ic); });
+        uint32_t jobInfo[2];
+        CHECK_HIP(ctx, hipMemcpyAsync(jobInfo, ctx->ic.frame + 3, sizeof(jobInfo), hipMemcpyDeviceToHost, s));
+        CHECK_HIP(ctx, hipStreamSynchronize(s));
+        const uint32_t numJobs = jobInfo[1];
+        const bool keepSub = keep && fEnd == total;
+        if (keepSub) {
+            size_t need = (size_t)numJobs * kIcStrata;
+            if (need > ctx->keepSubCap) {
+                if (ctx->keepSub) (void)hipFree(ctx->keepSub);
+                if (ctx->keepSubSig) (void)hipFree(ctx->keepSubSig);
+                ctx->keepSub = nullptr;
+                ctx->keepSubSig = nullptr;
+                ctx->keepSubCap = 0;
+                CHECK_HIP(ctx, hipMalloc(&ctx->keepSub, need * sizeof(float4)));
+                CHECK_HIP(ctx, hipMalloc(&ctx->keepSubSig, need * sizeof(uint2)));
+                ctx->keepSubCap = need;
+            }
+            ctx->keptJobs = numJobs;
+            ctx->keptUpdates = jobInfo[0];
+        }
+        const uint32_t jobsPerBatch = (uint32_t)std::max<size_t>(1, ctx->cap / kIcStrata);
+        for (uint32_t j0 = 0; j0 < numJobs; j0 += jobsPerBatch) {
+            const uint32_t nj = std::min(jobsPerBatch, numJobs - j0);
+            const uint32_t np = nj * kIcStrata;
+            BatchDesc b;
+            b.firstPath = 0;
+            b.numPaths = np;
+            b.npix = np;
+            timed(kClsShade,
+                  [&] { launchGenerateIC(s, ctx->ic, frameSeed, j0, np, ctx->pb, ctx->q[0], ctx->cnt, 0); });
+            bounces(ctx->fcIC, b, maxIterIC);
+            timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, j0, nj, ctx->pb); });
+            if (keepSub) {
+                CHECK_HIP(ctx, hipMemcpyAsync(ctx->keepSub + (size_t)j0 * kIcStrata, ctx->pb.out, np * sizeof(float4),
+                                              hipMemcpyDeviceToDevice, s));
+                CHECK_HIP(ctx, hipMemcpyAsync(ctx->keepSubSig + (size_t)j0 * kIcStrata, ctx->pb.outSig,
+                                              np * sizeof(uint2), hipMemcpyDeviceToDevice, s));
+            }
+            CHECK_HIP(ctx, hipGetLastError());
+        }
+        timed(kClsShade, [&] { launchIcCommit(s, ctx->ic, ctx->fcIC); });
+        timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
         CHECK_HIP(ctx, hipGetLastError());
-        batches++;
     }
     CHECK_HIP(ctx, hipEventRecord(ctx->tStop, s));
     CHECK_HIP(ctx, hipStreamSynchronize(s));
-    ctx->accSamples += p->spp;
+    if (accumulate) ctx->accSamples += p->spp;
 
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -639,6 +786,126 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         stats->nodeBytes = (uint32_t)sizeof(BvhNode);
         stats->primBytes = 48;
     }
+    return RTX_OK;
+}
+
+// ------------------------------------------------------- irradiance cache
+static void icFree(rtx_ctx *ctx) {
+    freeAll(ctx->icAllocs);
+    ctx->ic = IcBuffers{};
+    ctx->icValid = false;
+}
+
+int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) {
+    if (!ctx) return RTX_E_INVALID;
+    if (maxEntries == 0 || maxEntries > (1u << 24))
+        return fail(ctx, RTX_E_INVALID, "maxEntries must be in [1, 2^24]");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    icFree(ctx);
+    IcBuffers &ic = ctx->ic;
+    auto &L = ctx->icAllocs;
+    ic.maxEntries = maxEntries;
+    ic.nb = 1024;
+    while (ic.nb < 4u * maxEntries) ic.nb <<= 1;
+    ic.itemCap = 27u * maxEntries;
+    ic.jobCap = 2u * maxEntries;
+    ic.blockCap = kMaxBlockSums;
+    hipError_t e = devAlloc(&ic.hdr, 1, L);
+    if (e == hipSuccess) e = devAlloc(&ic.entries, maxEntries, L);
+    if (e == hipSuccess) e = devAlloc(&ic.spheres, maxEntries, L);
+    if (e == hipSuccess) e = devAlloc(&ic.bucketCount, ic.nb, L);
+    if (e == hipSuccess) e = devAlloc(&ic.bucketStart, ic.nb + 1, L);
+    if (e == hipSuccess) e = devAlloc(&ic.cursor, ic.nb, L);
+    if (e == hipSuccess) e = devAlloc(&ic.meta, 4, L);
+    if (e == hipSuccess) e = devAlloc(&ic.items, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.frame, 8, L);
+    if (e == hipSuccess) e = devAlloc(&ic.cand, 2 * (size_t)maxEntries, L);
+    if (e == hipSuccess) e = devAlloc(&ic.jobs, 2 * (size_t)ic.jobCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.jobRes, 10 * (size_t)ic.jobCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.blockSums, ic.blockCap, L);
+    if (e == hipSuccess && !ctx->fcIC) e = hipMalloc(&ctx->fcIC, sizeof(FrameConst));
+    if (e != hipSuccess) {
+        icFree(ctx);
+        return fail(ctx, RTX_E_DEVICE, std::string("irradiance cache allocation: ") + hipGetErrorString(e));
+    }
+    return rtx_ic_reset(ctx);
+}
+
+int rtx_ic_reset(rtx_ctx *ctx) {
+    if (!ctx) return RTX_E_INVALID;
+    if (ctx->ic.maxEntries == 0) return fail(ctx, RTX_E_STATE, "no irradiance cache (rtx_ic_create)");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    launchIcInit(ctx->stream, ctx->ic);
+    CHECK_HIP(ctx, hipGetLastError());
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->icValid = true;
+    ctx->keptJobs = ctx->keptUpdates = 0;
+    return RTX_OK;
+}
+
+int rtx_ic_read(rtx_ctx *ctx, rtx_cache_header *hdr, rtx_cache_data *entries, rtx_sphere *spheres,
+                uint32_t capacity) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->icValid) return fail(ctx, RTX_E_STATE, "no irradiance cache (rtx_ic_create)");
+    const uint32_t n = std::min(capacity, ctx->ic.maxEntries);
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hdr) CHECK_HIP(ctx, hipMemcpy(hdr, ctx->ic.hdr, sizeof(rtx_cache_header), hipMemcpyDeviceToHost));
+    if (entries && n) CHECK_HIP(ctx, hipMemcpy(entries, ctx->ic.entries, n * sizeof(rtx_cache_data),
+                                               hipMemcpyDeviceToHost));
+    if (spheres && n) CHECK_HIP(ctx, hipMemcpy(spheres, ctx->ic.spheres, n * sizeof(rtx_sphere),
+                                               hipMemcpyDeviceToHost));
+    return RTX_OK;
+}
+
+int rtx_ic_write(rtx_ctx *ctx, const rtx_cache_header *hdr, const rtx_cache_data *entries,
+                 const rtx_sphere *spheres) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->icValid) return fail(ctx, RTX_E_STATE, "no irradiance cache (rtx_ic_create)");
+    if (!hdr || !entries || !spheres) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    if (hdr->maxCaches != ctx->ic.maxEntries || hdr->nextCacheSlot > hdr->maxCaches)
+        return fail(ctx, RTX_E_INVALID, "header does not match the cache (maxCaches, nextCacheSlot)");
+    const size_t n = ctx->ic.maxEntries;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CHECK_HIP(ctx, hipMemcpy(ctx->ic.hdr, hdr, sizeof(rtx_cache_header), hipMemcpyHostToDevice));
+    CHECK_HIP(ctx, hipMemcpy(ctx->ic.entries, entries, n * sizeof(rtx_cache_data), hipMemcpyHostToDevice));
+    CHECK_HIP(ctx, hipMemcpy(ctx->ic.spheres, spheres, n * sizeof(rtx_sphere), hipMemcpyHostToDevice));
+    return RTX_OK;
+}
+
+int rtx_ic_read_jobs(rtx_ctx *ctx, rtx_ic_job *jobs, float *sub, uint32_t *subSig, uint32_t capacity,
+                     uint32_t *numJobs) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->icValid) return fail(ctx, RTX_E_STATE, "no irradiance cache (rtx_ic_create)");
+    if (numJobs) *numJobs = ctx->keptJobs;
+    const uint32_t n = std::min(capacity, ctx->keptJobs);
+    if (n == 0) return RTX_OK;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (jobs) {
+        std::vector<float4> j(2 * (size_t)n);
+        CHECK_HIP(ctx, hipMemcpy(j.data(), ctx->ic.jobs, j.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            float4 o = j[2 * i], nn = j[2 * i + 1];
+            int32_t type;
+            uint32_t target;
+            std::memcpy(&type, &o.w, 4);
+            std::memcpy(&target, &nn.w, 4);
+            jobs[i].type = type;
+            jobs[i].target = target;
+            jobs[i].origin[0] = o.x;
+            jobs[i].origin[1] = o.y;
+            jobs[i].origin[2] = o.z;
+            jobs[i].normal[0] = nn.x;
+            jobs[i].normal[1] = nn.y;
+            jobs[i].normal[2] = nn.z;
+        }
+    }
+    const size_t ns = (size_t)n * kIcStrata;
+    if (sub) CHECK_HIP(ctx, hipMemcpy(sub, ctx->keepSub, ns * sizeof(float4), hipMemcpyDeviceToHost));
+    if (subSig) CHECK_HIP(ctx, hipMemcpy(subSig, ctx->keepSubSig, ns * sizeof(uint2), hipMemcpyDeviceToHost));
     return RTX_OK;
 }
 

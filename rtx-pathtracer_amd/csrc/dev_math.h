@@ -63,6 +63,7 @@ constexpr uint32_t kSigPrime = 16777619u;
 constexpr uint32_t kSigVisible = 0x1b873593u;
 constexpr uint32_t kSigOccluded = 0x5bd1e995u;
 constexpr uint32_t kSigProbe = 0x85ebca6bu;
+constexpr uint32_t kSigCacheHit = 0xc2b2ae35u; // an irradiance-cache lookup that ended the path
 RTX_DI uint32_t sigFold(uint32_t h, uint32_t v) { return (h ^ v) * kSigPrime; }
 
 // ------------------------------------------------------------------ RNG
@@ -81,7 +82,7 @@ RTX_DI uint32_t tea(uint32_t val0, uint32_t val1) {
 // PCG32 first output after pcg32_srandom(initstate, initseq) (O'Neill 2014):
 // the per-sample frame seed that replaces the reference's per-frame
 // glm::linearRand (src/RayTracingApp.cpp:1121).
-RTX_DI uint32_t pcg32First(uint64_t initstate, uint64_t initseq) {
+__host__ __device__ inline uint32_t pcg32First(uint64_t initstate, uint64_t initseq) {
     uint64_t inc = (initseq << 1u) | 1u;
     uint64_t state = 0;
     state = state * 6364136223846793005ULL + inc;

@@ -67,7 +67,55 @@ struct FrameConst {
     uint32_t numNEE;         // NEE samples per vertex (>= 1)
     uint32_t countVisits;
     uint64_t debugPath;      // path whose bounces are printed (RTX_DEBUG_PATH), ~0 = none
+    // The arguments of raytrace() (raytrace.rgen:992) for this class of paths:
+    // camera paths (main, raytrace.rgen:1671) or the hemisphere strata of
+    // the irradiance cache (calculateCacheData, raytrace.rgen:1260).
+    int32_t maxDepth, maxFollow;
+    uint32_t addDirect, useNEE, useIC, createIC, allowRR;
 };
+
+// ------------------------------------------------------ irradiance cache
+struct IcItem {              // one (record, grid cell) pair of the lookup index
+    uint32_t entry;
+    uint32_t pad;
+    uint64_t key;
+};
+
+// What lookups read (dev_icache.h).
+struct DevIC {
+    const rtx_cache_data *entries;
+    const rtx_sphere *spheres;
+    const uint32_t *bucketStart; // nb + 1
+    const IcItem *items;
+    const uint32_t *meta;        // [0] occupied levels, [1] items, [2] level-0 cell size (float bits)
+    uint32_t nb;
+};
+
+// The cache, its index and the per-frame work lists (device memory).
+struct IcBuffers {
+    rtx_cache_header *hdr;
+    rtx_cache_data *entries;
+    rtx_sphere *spheres;
+    uint32_t maxEntries;
+    uint32_t nb, itemCap;
+    uint32_t *bucketCount, *bucketStart, *cursor, *meta;
+    IcItem *items;
+    // frame: [0] update elections, [1] creation candidates, [2] free slots at
+    // frame start, [3] update jobs, [4] jobs
+    uint32_t *frame;
+    float4 *cand;                // 2 per candidate: origin, normal (first maxEntries kept)
+    float4 *jobs;                // 2 per job: (origin, type), (normal, target)
+    uint32_t jobCap;
+    float *jobRes;               // 10 per job: color, rot, trans, harmonicR
+    uint32_t *blockSums;
+    uint32_t blockCap;
+};
+constexpr int kMaxNewCacheEntries = 5; // MAX_NEW_IRRADIANCE_ENTRIES, raytrace.rgen:65
+constexpr int kIcStrataPhi = 20;       // N, raytrace.rgen:1232
+constexpr int kIcStrataTheta = 10;     // M, raytrace.rgen:1233
+constexpr int kIcStrata = kIcStrataPhi * kIcStrataTheta;
+constexpr uint32_t kIcPhiKey = 0x9e3779b9u;     // keys of the stratum streams
+constexpr uint32_t kIcStratumKey = 0x7f4a7c15u;
 
 struct BatchDesc {
     uint64_t firstPath;   // global path index (within the call) of local path 0
@@ -82,8 +130,9 @@ struct PathBuffers {
     float4 *result;   // radiance xyz
     float4 *pendShadow; // [path * numNEE + j]: premultiplied shadow contribution, w = 1 if to add
     float4 *pendProbe;  // [path * numNEE + j]: premultiplied probe contribution
-    float4 *out;      // per local path: radiance xyz, w unused
+    float4 *out;      // per local path: radiance xyz, w = first hit distance
     uint2 *outSig;    // per local path: (final LCG state, primitive hash)
+    float4 *cand;     // 2 per candidate, kMaxNewCacheEntries per path: origin, normal
 };
 
 // Path flags packed in thr.w
@@ -92,11 +141,13 @@ constexpr uint32_t kFlagFollowCountShift = 8;     // bits 8-15 followCount
 constexpr uint32_t kFlagFollow = 1u << 16;
 constexpr uint32_t kFlagAddNext = 1u << 17;
 constexpr uint32_t kFlagPending = 1u << 18;       // NEE contributions waiting to be added
+constexpr uint32_t kFlagCandShift = 19;           // bits 19-21 irradiance-cache candidates
+constexpr uint32_t kFlagCandMask = 7u << kFlagCandShift;
 
 // ----------------------------------------------------------- launchers
 
 void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue q0,
-                    Counters *cnt, int qsel);
+                    Counters *cnt, int qsel, uint32_t *electCounter);
 void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits,
                   int blocks);
 // persistent lane-refill traversal: closest hit of an extend (kind 0) or probe (kind 2) queue
@@ -109,14 +160,27 @@ void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_
                   uint32_t *fetch, int blocks);
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
                         Counters *cnt, int blocks);
-void launchShade(hipStream_t s, const DevScene &S, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue qin,
-                 Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel, int blocks);
+void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const FrameConst *fc, BatchDesc b,
+                 PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel,
+                 int blocks);
+void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numPaths,
+                      PathBuffers pb, Queue q0, Counters *cnt, int qsel);
+// irradiance-cache frame bookkeeping (icache.hip)
+void launchIcFrameBegin(hipStream_t s, IcBuffers ic);
+void launchIcCollect(hipStream_t s, IcBuffers ic, PathBuffers pb, uint32_t numPaths);
+void launchIcPlan(hipStream_t s, IcBuffers ic);
+void launchIcReduce(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numJobs,
+                    PathBuffers pb);
+void launchIcCommit(hipStream_t s, IcBuffers ic, const FrameConst *fc);
+void launchIcIndex(hipStream_t s, IcBuffers ic, float c0);
+void launchIcInit(hipStream_t s, IcBuffers ic);
 void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
                   int blocks);
 void launchProbe(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb, Counters *cnt,
                  bool countVisits, int blocks);
 void launchFinish(hipStream_t s, const FrameConst *fc, PathBuffers pb, const int32_t *done, Counters *cnt,
                   int blocks);
+// accum may be NULL (irradiance-cache prepare frames): then only the kept paths are written
 void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,
                       bool keep);
 void launchResetCounters(hipStream_t s, Counters *cnt, int which);

@@ -13,6 +13,7 @@
 // Every path owns its LCG stream, so results do not depend on queue order.
 #include <hip/hip_runtime.h>
 
+#include "dev_icache.h"
 #include "dev_math.h"
 #include "dev_shading.h"
 #include "dev_traverse.h"
@@ -57,18 +58,27 @@ template <bool COUNT> RTX_DI void flushVisits(const VisitCount &vc, Counters *cn
 
 // ------------------------------------------------------------ generate
 // main() up to the camera ray: seed = tea(pixel, randomUInt) (raytrace.rgen:1625),
+// the irradiance-cache update election (1637-1639, the stream's first draw),
 // getCameraRay (raytrace.rgen:1487-1494); randomUInt per sample from PCG32.
 __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
-                                                 Queue q0, Counters *cnt, int qsel) {
+                                                 Queue q0, Counters *cnt, int qsel, uint32_t *electCounter) {
     const FrameConst &fc = *fcp;
     uint32_t pid = blockIdx.x * 256 + threadIdx.x;
     if (pid == 0) cnt->extend[qsel] = b.numPaths;
-    if (pid >= b.numPaths) return;
+    const bool active = pid < b.numPaths;
     uint64_t g = b.firstPath + pid;
     uint32_t pixel = (uint32_t)(g % b.npix);
     uint64_t sample = g / b.npix + fc.sampleOffset;
     uint32_t frameSeed = pcg32First(fc.seed, sample);
     Rng rng{tea(pixel, frameSeed)};
+    bool elect = false;
+    if (fc.pc.useIrradianceCache) elect = rng.next() < fc.pc.irradianceUpdateProb;
+    if (fc.pc.useIrradianceCache) {
+        // only the number of elections matters: they are served in order
+        uint64_t m = __ballot(active && elect);
+        if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(electCounter, (uint32_t)__popcll(m));
+    }
+    if (!active) return;
     uint32_t px = pixel % fc.width, py = pixel / fc.width;
     float jx = rng.negPos();
     float jy = rng.negPos();
@@ -85,7 +95,38 @@ __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ 
     pb.hash[pid] = 2166136261u;
     // depth 0, followCount 0, follow = true, addNextDirectLights = addFirstHitLight = true
     pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagFollow | kFlagAddNext));
-    pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
+}
+
+// The sub-paths of calculateCacheData (raytrace.rgen:1246-1263) for jobs
+// firstJob.. of the frame's job list: path p is stratum (k, j) = (s / 10,
+// s % 10), s = p % 200, of job firstJob + p / 200.  phi comes from the
+// column's stream, theta from the stratum's, whose state then drives the
+// sub-path; raytrace(..., addFirstHitLight = false, ...) starts with
+// addNextDirectLights off.
+__global__ __launch_bounds__(256) void kGenerateIC(IcBuffers ic, uint32_t frameSeed, uint32_t firstJob,
+                                                   uint32_t numPaths, PathBuffers pb, Queue q0, Counters *cnt,
+                                                   int qsel) {
+    uint32_t pid = blockIdx.x * 256 + threadIdx.x;
+    if (pid == 0) cnt->extend[qsel] = numPaths;
+    if (pid >= numPaths) return;
+    const uint32_t job = firstJob + pid / kIcStrata, st = pid % kIcStrata;
+    const uint32_t k = st / kIcStrataTheta, j = st % kIcStrataTheta;
+    float4 o4 = ic.jobs[2 * job], n4 = ic.jobs[2 * job + 1];
+    V3 n = v3(n4);
+    Rng col{tea(job * kIcStrataPhi + k, frameSeed ^ kIcPhiKey)};
+    float phi = 2.0f * kPi * ((float)k + col.next()) / (float)kIcStrataPhi;
+    Rng rng{tea((job * kIcStrataPhi + k) * kIcStrataTheta + j, frameSeed ^ kIcStratumKey)};
+    float theta = asinf(sqrtf(((float)j + rng.next()) / (float)kIcStrataTheta));
+    V3 local = v3(sinf(theta) * cosf(phi), sinf(theta) * sinf(phi), cosf(theta));
+    V3 d = toWorld(local, n);
+    q0.id[pid] = (int)pid;
+    q0.org[pid] = f4(v3(o4), kTMin);
+    q0.dir[pid] = f4(d, kTMax);
+    pb.rng[pid] = rng.s;
+    pb.hash[pid] = 2166136261u;
+    pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagFollow));
+    pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
 }
 
 // --------------------------------------------------------------- extend
@@ -178,9 +219,11 @@ __global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, const FrameCon
 }
 
 // --------------------------------------------------- persistent traversal
-// kind 0: extend queue (count extend[qsel]); kind 2: probe queue (count probe)
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kClosestP(DevScene S, Queue q, Counters *cnt, int qsel, int kind) {
+// KIND 0: extend queue (count extend[qsel]); KIND 2: probe queue (count probe).
+// KIND is a template parameter so that profiles tell the two apart.
+template <bool COUNT, int KIND>
+__global__ __launch_bounds__(kTraceBlock) void kClosestP(DevScene S, Queue q, Counters *cnt, int qsel) {
+    constexpr int kind = KIND;
     __shared__ int stackMem[kLdsStack * kTraceBlock];
     const uint32_t n = kind == 0 ? cnt->extend[qsel] : cnt->probe;
     if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -256,18 +299,24 @@ RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint3
 
 // ----------------------------------------------------------------- shade
 // One iteration of the bounce loop, raytrace.rgen:1008-1215 (plain method).
-__global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameConst *__restrict__ fcp, BatchDesc b,
-                                                      PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp,
-                                                      int32_t *done, Counters *cnt, int qsel) {
+__global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, const FrameConst *__restrict__ fcp,
+                                                      BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs,
+                                                      Queue qp, int32_t *done, Counters *cnt, int qsel) {
     const FrameConst &fc = *fcp;
     const uint32_t n = cnt->extend[qsel];
     const uint32_t numNEE = fc.numNEE;
     const float invNEE = 1.0f / (float)numNEE;
-    const bool useNEE = fc.pc.enableNEE != 0;
-    const bool addDirect = !useNEE;
+    const bool useNEE = fc.useNEE != 0;
+    const bool addDirect = fc.addDirect != 0;
     const bool mis = fc.pc.enableMIS != 0;
     const bool power = fc.pc.usePowerHeuristic != 0;
-    const int maxDepth = fc.pc.maxDepth, maxFollow = fc.pc.maxFollowDiscrete;
+    const int maxDepth = fc.maxDepth, maxFollow = fc.maxFollow;
+    const bool useIC = fc.useIC != 0, createIC = fc.createIC != 0;
+    const bool icOnGlossy = fc.pc.useIrradianceCacheOnGlossy != 0;
+    IcQueryParams icp;
+    icp.invA = 1.0f / fc.pc.irradianceA;
+    icp.gradients = fc.pc.useIrradianceGradients != 0;
+    icp.visibility = fc.pc.irradianceCachePerformVisibilityCheck != 0;
     const uint32_t stride = gridDim.x * kShadeBlock;
     uint32_t *nextCount = &cnt->extend[qsel ^ 1];
     for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += stride) {
@@ -282,7 +331,9 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
         Mat m;
         m.type = -1;
         int depth = 0, followCount = 0;
-        bool follow = false, addNext = false;
+        bool follow = false, addNext = false, icHit = false;
+        uint32_t ncand = 0;
+        float firstT = kTMax;
         if (active) {
             pid = (uint32_t)qin.id[i];
             float4 hit = qin.hit[i];
@@ -292,7 +343,10 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
             float4 t4 = pb.thr[pid];
             thr = v3(t4);
             flags = __float_as_uint(t4.w);
-            res = v3(pb.result[pid]);
+            float4 r4 = pb.result[pid];
+            res = v3(r4);
+            firstT = r4.w;
+            ncand = (flags & kFlagCandMask) >> kFlagCandShift;
             hash = pb.hash[pid];
             if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE, hash);
             int prim = __float_as_int(hit.y);
@@ -308,6 +362,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
             } else {
                 h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
                 m = loadMat(&S.materials[h.matIndex]);
+                if (depth == 1) firstT = hit.x;
                 if (b.firstPath + pid == fc.debugPath)
                     printf("[gpu] depth %d prim %d t %.9g mat %d seed %u thr %.6g %.6g %.6g d %.6g %.6g %.6g n %.6g %.6g %.6g ff %d res %.6g\n",
                            depth, prim, hit.x, m.type, rng.s, thr.x, thr.y, thr.z, d.x, d.y, d.z, h.normal.x,
@@ -320,7 +375,25 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
                 } else {
                     addNext = false;
                     follow = false;
-                    doNee = useNEE && neeCapable(m.type) && S.lightCount > 0;
+                    bool nee = useNEE;
+                    if (useIC && isICCapable(m.type, icOnGlossy)) { // raytrace.rgen:1081-1139
+                        V3 irr;
+                        if (icQuery(IC, icp, h.worldPos, h.normal, irr)) {
+                            // a cache record ends the path after its NEE (1114-1122)
+                            hash = sigFold(hash, kSigCacheHit);
+                            res = res + thr * approxDiffuse(m, h.normal, -d) * irr;
+                            icHit = true;
+                            nee = true;
+                        } else if (createIC && rng.next() < fc.pc.irradianceCreateProb) {
+                            if (ncand < (uint32_t)kMaxNewCacheEntries) {
+                                float4 *c = pb.cand + 2 * ((size_t)pid * kMaxNewCacheEntries + ncand);
+                                c[0] = f4(h.worldPos, 0.0f);
+                                c[1] = f4(h.normal, 0.0f);
+                                ncand++;
+                            }
+                        }
+                    }
+                    doNee = nee && neeCapable(m.type) && S.lightCount > 0;
                 }
             }
         }
@@ -392,6 +465,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
         }
         // ---- continuation, raytrace.rgen:1169-1215
         V3 newDir = v3(0.0f);
+        if (icHit) isDone = true;
         if (active && !isDone) {
             float pdf = sampleBSDF(rng, m, wi, h.normal, h.frontFace, newDir);
             if (b.firstPath + pid == fc.debugPath) {
@@ -407,7 +481,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
             } else {
                 thr = thr * change;
                 bool killed = false;
-                if (fc.pc.enableRR && depth >= fc.rrStartDepth) { // Russian roulette (absent from the reference)
+                if (fc.allowRR && fc.pc.enableRR && depth >= fc.rrStartDepth) { // Russian roulette (not in the reference)
                     float q = fmaxf(fabsf(thr.x), fmaxf(fabsf(thr.y), fabsf(thr.z)));
                     q = fminf(q, 0.95f);
                     if (rng.next() >= q) killed = true;
@@ -424,11 +498,11 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, const FrameCon
         if (active) {
             uint32_t nf = (uint32_t)depth | ((uint32_t)followCount << kFlagFollowCountShift) |
                           (follow ? kFlagFollow : 0u) | (addNext ? kFlagAddNext : 0u) |
-                          (doNee ? kFlagPending : 0u);
+                          (doNee ? kFlagPending : 0u) | (ncand << kFlagCandShift);
             pb.rng[pid] = rng.s;
             pb.hash[pid] = hash;
             pb.thr[pid] = f4(thr, __uint_as_float(nf));
-            pb.result[pid] = f4(res, 0.0f);
+            pb.result[pid] = f4(res, firstT);
         }
         int qi = waveAppend(cont, nextCount);
         if (cont) {
@@ -448,11 +522,12 @@ __global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fc
     const uint32_t numNEE = fcp->numNEE;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         uint32_t pid = (uint32_t)done[i];
-        V3 res = v3(pb.result[pid]);
+        float4 r4 = pb.result[pid];
+        V3 res = v3(r4);
         uint32_t flags = __float_as_uint(pb.thr[pid].w);
         uint32_t hash = pb.hash[pid];
         if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE, hash);
-        pb.out[pid] = f4(res, 0.0f);
+        pb.out[pid] = f4(res, r4.w);
         pb.outSig[pid] = make_uint2(pb.rng[pid], hash);
     }
 }
@@ -466,7 +541,7 @@ __global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, PathBuffers pb, 
     uint32_t lanes = b.numPaths < b.npix ? b.numPaths : b.npix;
     if (j >= lanes) return;
     uint32_t pixel = (uint32_t)((b.firstPath + j) % b.npix);
-    float4 acc = accum[pixel];
+    float4 acc = accum ? accum[pixel] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (uint32_t pid = j; pid < b.numPaths; pid += b.npix) {
         float4 v = pb.out[pid];
         acc.x += v.x;
@@ -478,7 +553,7 @@ __global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, PathBuffers pb, 
             keepSig[b.firstPath + pid] = pb.outSig[pid];
         }
     }
-    accum[pixel] = acc;
+    if (accum) accum[pixel] = acc;
 }
 
 __global__ void kResetCounters(Counters *cnt, int which) {
@@ -567,8 +642,15 @@ __global__ __launch_bounds__(kTraceBlock) void kQueryP(DevScene S, const rtx_ray
 static int gridFor(size_t n, int block) { return (int)((n + block - 1) / block); }
 
 void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue q0, Counters *cnt,
-                    int qsel) {
-    hipLaunchKernelGGL(kGenerate, dim3(gridFor(b.numPaths, 256)), dim3(256), 0, s, fc, b, pb, q0, cnt, qsel);
+                    int qsel, uint32_t *electCounter) {
+    hipLaunchKernelGGL(kGenerate, dim3(gridFor(b.numPaths, 256)), dim3(256), 0, s, fc, b, pb, q0, cnt, qsel,
+                       electCounter);
+}
+void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numPaths,
+                      PathBuffers pb, Queue q0, Counters *cnt, int qsel) {
+    if (numPaths == 0) return;
+    hipLaunchKernelGGL(kGenerateIC, dim3(gridFor(numPaths, 256)), dim3(256), 0, s, ic, frameSeed, firstJob, numPaths,
+                       pb, q0, cnt, qsel);
 }
 void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits, int blocks) {
     if (countVisits) hipLaunchKernelGGL(kExtend<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
@@ -576,8 +658,13 @@ void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int 
 }
 void launchClosestP(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, int kind, bool countVisits,
                     int blocks) {
-    if (countVisits) hipLaunchKernelGGL(kClosestP<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel, kind);
-    else hipLaunchKernelGGL(kClosestP<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel, kind);
+    if (kind == 0) {
+        if (countVisits) hipLaunchKernelGGL((kClosestP<true, 0>), dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+        else hipLaunchKernelGGL((kClosestP<false, 0>), dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+    } else {
+        if (countVisits) hipLaunchKernelGGL((kClosestP<true, 2>), dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+        else hipLaunchKernelGGL((kClosestP<false, 2>), dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+    }
 }
 void launchShadowP(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
                    int blocks) {
@@ -588,9 +675,10 @@ void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, 
                         Counters *cnt, int blocks) {
     hipLaunchKernelGGL(kProbeResolve, dim3(blocks), dim3(256), 0, s, S, fc, q, pb, cnt);
 }
-void launchShade(hipStream_t s, const DevScene &S, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue qin,
-                 Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel, int blocks) {
-    hipLaunchKernelGGL(kShade, dim3(blocks), dim3(kShadeBlock), 0, s, S, fc, b, pb, qin, qout, qs, qp, done, cnt,
+void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const FrameConst *fc, BatchDesc b,
+                 PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel,
+                 int blocks) {
+    hipLaunchKernelGGL(kShade, dim3(blocks), dim3(kShadeBlock), 0, s, S, ic, fc, b, pb, qin, qout, qs, qp, done, cnt,
                        qsel);
 }
 void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,

@@ -90,6 +90,12 @@ def device_lib():
         L.rtx_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Ray), C.c_size_t, C.POINTER(Hit)]
         L.rtx_trace_occluded.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Ray), C.c_size_t,
                                          C.POINTER(C.c_uint8)]
+        L.rtx_ic_create.argtypes = [C.c_void_p, C.c_uint32]
+        L.rtx_ic_reset.argtypes = [C.c_void_p]
+        L.rtx_ic_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
+        L.rtx_ic_write.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rtx_ic_read_jobs.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32),
+                                       C.c_uint32, C.POINTER(C.c_uint32)]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         L.rtx_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         _dev = L
@@ -311,6 +317,42 @@ def rays_array(org, dirs, tmin=1e-3, tmax=1e6):
     return r
 
 
+class IcState:
+    """Host copy of an irradiance cache: the header and the entry and sphere
+    buffers of src/IrradianceCache.cpp:46-75, as numpy arrays."""
+
+    def __init__(self, hdr, entries, spheres):
+        self.hdr = hdr            # CacheHeader
+        self.entries = entries    # CACHE_DTYPE[maxCaches]
+        self.spheres = spheres    # SPHERE_DTYPE[maxCaches]
+
+    @classmethod
+    def empty(cls, max_entries):
+        """The freshly created cache (IrradianceCache::createBuffers): no
+        entries, harmonicR -1, spheres of radius 0 at the origin."""
+        hdr = abi.CacheHeader(0, max_entries, 0)
+        e = np.zeros(max_entries, abi.CACHE_DTYPE)
+        e["harmonicR"] = -1.0
+        sp = np.zeros(max_entries, abi.SPHERE_DTYPE)
+        return cls(hdr, e, sp)
+
+    @property
+    def max_entries(self):
+        return int(self.hdr.maxCaches)
+
+    @property
+    def count(self):
+        return int(min(self.hdr.nextCacheSlot, self.hdr.maxCaches))
+
+    def copy(self):
+        return IcState(abi.CacheHeader(self.hdr.nextCacheSlot, self.hdr.maxCaches, self.hdr.nextUpdateSlot),
+                       self.entries.copy(), self.spheres.copy())
+
+    def ctypes_views(self):
+        return (C.cast(C.pointer(self.hdr), C.c_void_p), self.entries.ctypes.data_as(C.c_void_p),
+                self.spheres.ctypes.data_as(C.c_void_p))
+
+
 class DeviceScene:
     def __init__(self, ctx, handle):
         self.ctx = ctx
@@ -370,6 +412,41 @@ class Renderer:
         for k, v in kw.items():
             setattr(t, k, v)
         self._check(device_lib().rtx_ctx_set_tuning(self._h, C.byref(t)))
+
+    # ------------------------------------------------ irradiance cache
+    def ic_create(self, max_entries):
+        self._check(device_lib().rtx_ic_create(self._h, int(max_entries)))
+        self._ic_max = int(max_entries)
+
+    def ic_reset(self):
+        self._check(device_lib().rtx_ic_reset(self._h))
+
+    def ic_read(self):
+        """The device cache as an IcState."""
+        st = IcState.empty(self._ic_max)
+        hdr, ent, sph = st.ctypes_views()
+        self._check(device_lib().rtx_ic_read(self._h, hdr, ent, sph, self._ic_max))
+        return st
+
+    def ic_write(self, state):
+        hdr, ent, sph = state.ctypes_views()
+        self._check(device_lib().rtx_ic_write(self._h, hdr, ent, sph))
+
+    def ic_read_jobs(self):
+        """Jobs of the last kept cache frame: list of (type, target, origin,
+        normal), sub-path results (jobs, 200, 4) and signatures (jobs, 200, 2)."""
+        n = C.c_uint32()
+        self._check(device_lib().rtx_ic_read_jobs(self._h, None, None, None, 0, C.byref(n)))
+        nj = n.value
+        jobs = (abi.IcJob * max(1, nj))()
+        sub = np.zeros((max(1, nj) * 200, 4), np.float32)
+        sig = np.zeros((max(1, nj) * 200, 2), np.uint32)
+        self._check(device_lib().rtx_ic_read_jobs(self._h, C.cast(jobs, C.c_void_p),
+                                                  sub.ctypes.data_as(C.POINTER(C.c_float)),
+                                                  sig.ctypes.data_as(C.POINTER(C.c_uint32)), nj, C.byref(n)))
+        jl = [(jobs[i].type, jobs[i].target, np.array(jobs[i].origin[:], np.float64),
+               np.array(jobs[i].normal[:], np.float64)) for i in range(nj)]
+        return jl, sub[:nj * 200].reshape(nj, 200, 4), sig[:nj * 200].reshape(nj, 200, 2)
 
     def render(self, scene, params):
         st = FrameStats()

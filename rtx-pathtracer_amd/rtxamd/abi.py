@@ -5,6 +5,8 @@ device layouts, shaders/raycommon.glsl, shaders/wavefront.glsl) fixes.
 """
 import ctypes as C
 
+import numpy as np
+
 RTX_OK = 0
 RTX_E_INVALID = -1
 RTX_E_DEVICE = -2
@@ -57,6 +59,25 @@ class FaceSample(C.Structure):
 
 class Sphere(C.Structure):
     _fields_ = [("center", F3), ("radius", C.c_float), ("materialIndex", C.c_int32), ("iLight", C.c_int32)]
+
+
+class CacheHeader(C.Structure):
+    _fields_ = [("nextCacheSlot", C.c_uint32), ("maxCaches", C.c_uint32), ("nextUpdateSlot", C.c_uint32)]
+
+
+class CacheData(C.Structure):
+    _fields_ = [("color", F3), ("normal", F3), ("rotGrad", F3), ("transGrad", F3), ("harmonicR", C.c_float),
+                ("numUpdates", C.c_uint32)]
+
+
+class IcJob(C.Structure):
+    _fields_ = [("type", C.c_int32), ("target", C.c_uint32), ("origin", F3), ("normal", F3)]
+
+
+# numpy views of the cache buffers (same bytes as CacheData / Sphere arrays)
+CACHE_DTYPE = np.dtype([("color", "<f4", 3), ("normal", "<f4", 3), ("rotGrad", "<f4", 3), ("transGrad", "<f4", 3),
+                        ("harmonicR", "<f4"), ("numUpdates", "<u4")])
+SPHERE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("materialIndex", "<i4"), ("iLight", "<i4")])
 
 
 PUSH_CONSTANT_FIELDS = [
@@ -148,7 +169,7 @@ class View(C.Structure):
 
 EXPECTED_SIZES = {
     Vertex: 48, Material: 96, Instance: 144, Light: 40, FaceSample: 12, Sphere: 24,
-    PushConstant: 192, Camera: 256, Ray: 32, Hit: 16, Tuning: 32,
+    PushConstant: 192, Camera: 256, Ray: 32, Hit: 16, Tuning: 32, CacheHeader: 12, CacheData: 56, IcJob: 32,
 }
 for _t, _n in EXPECTED_SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)

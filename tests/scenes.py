@@ -94,3 +94,38 @@ def params(hs, w, h, spp, **kw):
     cam = hs.camera(w, h)
     flags = kw.pop("flags", abi.RTX_FRAME_RESET_ACCUM)
     return rtxamd.frame_params(cam, w, h, spp, flags=flags, **kw)
+
+
+def ceiling_over_point_light(height=1.0, albedo=0.6, intensity=2.0, eps=1e-4):
+    """A black floor at z=0, a Lambert ceiling of albedo `albedo` at z=height
+    (100 heights wide) and one point light of intensity `intensity` just above
+    the floor's origin.  The ceiling's radiance seen from the origin is
+    (albedo I / (pi h^2)) cos^3(theta), so the origin's irradiance is
+    2 albedo I / (5 h^2) and the harmonic mean distance to the ceiling over
+    cosine-weighted directions is 1.5 h."""
+    s = rtxamd.HostScene.new()
+    black = s.add_material(rtxamd.diffuse((0.0, 0.0, 0.0)))
+    ceil = s.add_material(rtxamd.diffuse((albedo,) * 3))
+    w = 100.0 * height
+    p, i = quad((-w, -w, height), (-w, w, height), (w, w, height), (w, -w, height))
+    s.add_mesh(p, i, ceil)
+    p, i = quad((-w, -w, 0), (w, -w, 0), (w, w, 0), (-w, w, 0))
+    s.add_mesh(p, i, black)
+    s.add_point_light((0.0, 0.0, eps), (intensity,) * 3)
+    s.set_view((0.0, -2.0, 0.5 * height), (0.0, 0.0, 0.5 * height), (0.0, 0.0, 1.0), 60.0)
+    return s.finalize()
+
+
+def cache_state(records, max_entries, irradiance_a=0.2):
+    """An IcState holding `records` = [(center, normal, color, harmonicR), ...]
+    with the reference's radius a * harmonicR and numUpdates 1."""
+    st = rtxamd.IcState.empty(max_entries)
+    for i, (c, n, col, hr) in enumerate(records):
+        st.entries[i]["normal"] = n
+        st.entries[i]["color"] = col
+        st.entries[i]["harmonicR"] = hr
+        st.entries[i]["numUpdates"] = 1
+        st.spheres[i]["center"] = c
+        st.spheres[i]["radius"] = np.float32(irradiance_a) * np.float32(hr)
+    st.hdr.nextCacheSlot = len(records)
+    return st

@@ -17,7 +17,8 @@ C_NAMES = {
     abi.FaceSample: "rtx_face_sample", abi.Sphere: "rtx_sphere", abi.PushConstant: "rtx_push_constant",
     abi.Camera: "rtx_camera", abi.ModelDesc: "rtx_model_desc", abi.SceneDesc: "rtx_scene_desc",
     abi.FrameParams: "rtx_frame_params", abi.FrameStats: "rtx_frame_stats", abi.SceneInfo: "rtx_scene_info",
-    abi.Ray: "rtx_ray", abi.Hit: "rtx_hit", abi.Tuning: "rtx_tuning",
+    abi.Ray: "rtx_ray", abi.Hit: "rtx_hit", abi.Tuning: "rtx_tuning", abi.CacheHeader: "rtx_cache_header",
+    abi.CacheData: "rtx_cache_data", abi.IcJob: "rtx_ic_job",
 }
 
 
@@ -58,6 +59,9 @@ REFERENCE_LAYOUT = {
     abi.FaceSample: (12, {"index": 0, "sampleProb": 4, "faceArea": 8}),
     abi.Sphere: (24, {"center": 0, "radius": 12, "materialIndex": 16, "iLight": 20}),
     abi.Camera: (256, {"view": 0, "proj": 64, "viewInverse": 128, "projInverse": 192}),
+    abi.CacheHeader: (12, {"nextCacheSlot": 0, "maxCaches": 4, "nextUpdateSlot": 8}),
+    abi.CacheData: (56, {"color": 0, "normal": 12, "rotGrad": 24, "transGrad": 36, "harmonicR": 48,
+                         "numUpdates": 52}),
 }
 
 
