@@ -442,6 +442,16 @@ typedef struct rtx_ic_job {
 int rtx_ic_read_jobs(rtx_ctx *ctx, rtx_ic_job *jobs, float *sub, uint32_t *subSig, uint32_t capacity,
                      uint32_t *numJobs);
 
+/* ------------------------------------------------------------ path guiding */
+/* The guiding regions (boxes) and one vMF mixture per region: replaces
+ * PathGuiding's AABB and VMM_Theta buffers and its acceleration structure
+ * (bindings 15-17, src/PathGuiding.cpp:141-197).  Renders with pc.useGuiding
+ * sample non-discrete materials from the mixture of the region holding the
+ * vertex with probability pc.guidingProb (getNewDirection,
+ * shaders/raytrace.rgen:923-960).  n = 0 removes them. */
+int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n);
+int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32_t capacity, uint32_t *n);
+
 /* ---------------------------------------------------- traversal queries */
 /* Rays for the query entry points: origin + tmin, direction + tmax. */
 typedef struct rtx_ray {

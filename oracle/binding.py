@@ -86,6 +86,11 @@ def lib(abi):
         L.orc_ic_query.argtypes = [C.c_void_p, C.POINTER(abi.PushConstant), vp, vp, vp, d3, d3, d3]
         L.orc_ic_apply.argtypes = [C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, vp, C.c_size_t,
                                    C.POINTER(C.c_double)]
+        L.orc_guiding_set.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32]
+        L.orc_vmm_pdf.argtypes = [C.POINTER(abi.VmmTheta), d3, d3, C.c_int]
+        L.orc_vmm_pdf.restype = C.c_double
+        L.orc_sample_vmm.argtypes = [C.POINTER(abi.VmmTheta), d3, C.c_int, C.POINTER(C.c_uint32), d3]
+        L.orc_sample_vmm.restype = None
         L.orc_ic_calculate.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, C.c_uint32, d3,
                                        d3, d3, d3, d3, C.POINTER(C.c_double)]
         _lib = L
@@ -203,6 +208,13 @@ class OracleScene:
         jl = [(jobs[i].type, jobs[i].target, np.array(jobs[i].origin[:]), np.array(jobs[i].normal[:]))
               for i in range(n)]
         return rgb, sig, jl, sub[:n * IC_STRATA].reshape(n, IC_STRATA, 4), subsig[:n * IC_STRATA].reshape(n, IC_STRATA, 2)
+
+    def guiding_set(self, regions, vmms):
+        from rtxamd import guiding_arrays
+        boxes, arr = guiding_arrays(regions, vmms)
+        rc = self.L.orc_guiding_set(self._h, boxes, arr, len(vmms))
+        if rc != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
 
     def ic_calculate(self, params, state, origin, normal, job=0, prec=64):
         """calculateCacheData at one point: color, rot, trans, harmonicR."""

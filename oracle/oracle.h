@@ -37,6 +37,13 @@ const char *orc_last_error(void);
 int orc_render_paths(const orc_scene *scene, const rtx_frame_params *params, const uint64_t *ids,
                      size_t n, int prec, int threads, double *rgb, uint32_t *sig);
 
+/* Path-guiding regions (boxes) and their vMF mixtures, read by renders
+ * with pc.useGuiding (the reference's bindings 15-17). */
+int orc_guiding_set(orc_scene *scene, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n);
+/* VMM pdf (guiding.glsl:57-65) and sampleVMM (91-104) in FP64. */
+double orc_vmm_pdf(const rtx_vmm_theta *m, const double x[3], const double wo[3], int parallax);
+void orc_sample_vmm(const rtx_vmm_theta *m, const double x[3], int parallax, uint32_t *state, double out[3]);
+
 /* Closest hit / occlusion with the oracle's FP64 traversal.
  * brute != 0 tests every primitive instead of walking the BVH. */
 int orc_trace_closest(const orc_scene *scene, const rtx_ray *rays, size_t n, int brute,

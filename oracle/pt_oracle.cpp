@@ -168,6 +168,9 @@ struct orc_scene {
     size_t ntri = 0;
     std::vector<Node> nodes;
     std::vector<int> primOrder;
+    // path-guiding regions and their mixtures (orc_guiding_set)
+    std::vector<rtx_aabb> regions;
+    std::vector<rtx_vmm_theta> vmms;
 };
 
 namespace {
@@ -893,6 +896,77 @@ template <class R> struct Tracer {
         return r / (R)numNEE;
     }
 
+    // ------------------------------------------------------------ guiding
+    // getGuidingRegion (raytrace.rgen:904-921, raytrace.guiding.rint:6-10):
+    // the region box holding x (bounds inclusive); of several (a point on a
+    // shared face) the smallest index.  -1 outside every region.
+    int guidingRegion(V3<R> x) const {
+        for (size_t i = 0; i < S.regions.size(); ++i) {
+            const rtx_aabb &b = S.regions[i];
+            if ((R)b.min[0] <= x.x && x.x <= (R)b.max[0] && (R)b.min[1] <= x.y && x.y <= (R)b.max[1] &&
+                (R)b.min[2] <= x.z && x.z <= (R)b.max[2])
+                return (int)i;
+        }
+        return -1;
+    }
+    // vMF (guiding.glsl:34-47) with parallax compensation
+    static R vmf(V3<R> wo, const rtx_vmf_theta &t, V3<R> x, bool parallax) {
+        if (t.k == 0.0f) return R(0.07957747155); // M_INV_4PI
+        V3<R> mu = f3<R>(t.mu);
+        if (parallax && t.distance > 0.0f) mu = normalize(f3<R>(t.target) - x);
+        return (R)t.norm * std::exp((R)t.k * (dot(mu, wo) - R(1)));
+    }
+    static R vmm(V3<R> wo, const rtx_vmm_theta &m, V3<R> x, bool parallax) { // guiding.glsl:57-65
+        R res = 0;
+        for (int i = 0; i < m.usedDistributions; ++i) res += (R)m.pi[i] * vmf(wo, m.thetas[i], x, parallax);
+        return res;
+    }
+    V3<R> sampleVMF(const rtx_vmf_theta &t, V3<R> x, bool parallax) { // guiding.glsl:67-89
+        if (t.k > 0.0f) {
+            R r1 = rnd();
+            R r2 = rnd();
+            R cosTheta = R(1) + std::log(R(1) + (R)t.eMin2K * r1 - r1) / (R)t.k;
+            R sinTheta = R(1) - cosTheta * cosTheta <= R(0) ? R(0) : std::sqrt(R(1) - cosTheta * cosTheta);
+            R phi = R(2) * C::PI * r2;
+            V3<R> mu = f3<R>(t.mu);
+            if (parallax && t.distance > 0.0f) mu = normalize(f3<R>(t.target) - x);
+            return toWorld(V3<R>(sinTheta * std::cos(phi), sinTheta * std::sin(phi), cosTheta), mu);
+        }
+        return randomOnUnitSphere();
+    }
+    V3<R> sampleVMM(const rtx_vmm_theta &m, V3<R> x, bool parallax) { // guiding.glsl:91-104
+        R u = rnd();
+        int i = 0;
+        const int last = m.usedDistributions - 1;
+        R piSum = (R)m.pi[0];
+        while (piSum < u && i < last) {
+            i++;
+            piSum += (R)m.pi[i];
+        }
+        return sampleVMF(m.thetas[i], x, parallax);
+    }
+    // getNewDirection (raytrace.rgen:923-960): with guiding on, non-discrete
+    // materials sample the region's mixture with probability guidingProb
+    // and the BSDF otherwise; the pdf is the mix of both.
+    R getNewDirection(const rtx_material &m, V3<R> x, V3<R> normal, V3<R> wi, bool frontFace, V3<R> &newDir) {
+        if (!(pc.useGuiding && !hasDiscreteDirection(m))) return sampleBSDF(m, wi, normal, frontFace, newDir);
+        int region = guidingRegion(x);
+        if (region < 0) return R(0);
+        const rtx_vmm_theta &vm = S.vmms[region];
+        const bool parallax = pc.useParallaxCompensation != 0;
+        R pdfMat;
+        if (rnd() < (R)pc.guidingProb) {
+            newDir = sampleVMM(vm, x, parallax);
+            pdfMat = pdfBSDF(m, normal, wi, newDir);
+        } else {
+            pdfMat = sampleBSDF(m, wi, normal, frontFace, newDir);
+        }
+        if (dot(newDir, normal) < R(0) || pdfMat <= R(0)) return R(0);
+        R pdfGuiding = vmm(newDir, vm, x, parallax);
+        if (std::isnan(pdfGuiding)) return sampleBSDF(m, wi, normal, frontFace, newDir);
+        return pdfMat * (R(1) - (R)pc.guidingProb) + pdfGuiding * (R)pc.guidingProb; // mix()
+    }
+
     // ---------------------------------------------------- irradiance cache
     // ref: raytrace.rgen:806-813
     bool isICCapable(const rtx_material &m) const {
@@ -1004,7 +1078,7 @@ template <class R> struct Tracer {
                 }
             }
             V3<R> newDirection;
-            R pdf = sampleBSDF(m, -direction, normal, info.isFrontFace, newDirection);
+            R pdf = getNewDirection(m, origin, normal, -direction, info.isFrontFace, newDirection);
             // The reference stops on pdf <= 0 only; a NaN pdf (pdfBSDF of wo == -wi
             // normalises a zero half-vector) would poison the pixel, so it ends the path.
             if (pdf <= R(0) || std::isnan(pdf)) break;
@@ -1078,7 +1152,7 @@ int checkParams(const rtx_frame_params *fp, bool cacheFrame = false) {
         g_err = "numNEE and irradianceNumNEE must be >= 1 with the irradiance cache";
         return RTX_E_INVALID;
     }
-    if (pc.useADRRS || pc.splitOnFirst || pc.useGuiding || pc.updateGuiding ||
+    if (pc.useADRRS || pc.splitOnFirst || pc.updateGuiding ||
         pc.visualizeMode != 0 || pc.showIrradianceCacheOnly || pc.storeEstimate) {
         g_err = "oracle: only the plain method is restated so far";
         return RTX_E_UNSUPPORTED;
@@ -1425,6 +1499,23 @@ int orc_scene_create(const rtx_scene_desc *d, orc_scene **out) {
 
 void orc_scene_destroy(orc_scene *s) { delete s; }
 
+int orc_guiding_set(orc_scene *s, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) {
+    if (!s || (n && (!regions || !vmms))) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (vmms[i].usedDistributions < 1 || vmms[i].usedDistributions > RTX_MAX_DISTRIBUTIONS) {
+            g_err = "usedDistributions out of range";
+            return RTX_E_INVALID;
+        }
+    s->regions.assign(regions, regions + n);
+    s->vmms.assign(vmms, vmms + n);
+    return RTX_OK;
+}
+
+
+
 int orc_render_paths(const orc_scene *s, const rtx_frame_params *fp, const uint64_t *ids, size_t n, int prec,
                      int threads, double *rgb, uint32_t *sig) {
     if (!s || !fp || (!ids && n) || (!rgb && n)) {
@@ -1436,6 +1527,10 @@ int orc_render_paths(const orc_scene *s, const rtx_frame_params *fp, const uint6
     if (prec != 32 && prec != 64) {
         g_err = "prec must be 32 or 64";
         return RTX_E_INVALID;
+    }
+    if (fp->pc.useGuiding && s->regions.empty()) {
+        g_err = "useGuiding without guiding regions (orc_guiding_set)";
+        return RTX_E_STATE;
     }
     parallelFor(n, threads, [&](size_t i) {
         double r[3];
@@ -1548,6 +1643,10 @@ int orc_ic_frame(const orc_scene *s, const rtx_frame_params *fp, int prec, int t
     if (!fp->pc.useIrradianceCache) {
         g_err = "orc_ic_frame needs pc.useIrradianceCache";
         return RTX_E_INVALID;
+    }
+    if (fp->pc.useGuiding && s->regions.empty()) {
+        g_err = "useGuiding without guiding regions (orc_guiding_set)";
+        return RTX_E_STATE;
     }
     if (prec != 32 && prec != 64) {
         g_err = "prec must be 32 or 64";
@@ -1675,6 +1774,17 @@ int orc_ic_query(const orc_scene *s, const rtx_push_constant *pc, const rtx_cach
     if (!t.queryIC(vd<double>(x), vd<double>(n), c)) return 0;
     toD(c, color);
     return 1;
+}
+
+double orc_vmm_pdf(const rtx_vmm_theta *m, const double x[3], const double wo[3], int parallax) {
+    return Tracer<double>::vmm(vd<double>(wo), *m, vd<double>(x), parallax != 0);
+}
+
+void orc_sample_vmm(const rtx_vmm_theta *m, const double x[3], int parallax, uint32_t *state, double out[3]) {
+    Tracer<double> t(emptyScene(), defaultParams());
+    t.seed = *state;
+    toD(t.sampleVMM(*m, vd<double>(x), parallax != 0), out);
+    *state = t.seed;
 }
 
 double orc_fresnel(double eta, double c) { return Tracer<double>::fresnel(eta, c); }

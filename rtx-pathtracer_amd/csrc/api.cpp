@@ -73,6 +73,11 @@ struct rtx_ctx {
     uint2 *keepSubSig = nullptr;
     size_t keepSubCap = 0;
     uint32_t keptJobs = 0, keptUpdates = 0;
+    // path guiding (rtx_guiding_set)
+    std::vector<void *> guideAllocs;
+    DevGuiding guide{};
+    std::vector<rtx_aabb> guideBoxes;
+    std::vector<rtx_vmm_theta> guideVmms;
 };
 
 namespace {
@@ -223,6 +228,47 @@ int ensureBatchBuffers(rtx_ctx *ctx, size_t paths, uint32_t numNEE) {
     return RTX_OK;
 }
 
+// Median split on the widest axis of the region centres, depth first.
+static int buildGuideRec(std::vector<GuideNode> &out, const rtx_aabb *boxes, std::vector<uint32_t> &ids, size_t lo,
+                         size_t hi) {
+    const int me = (int)out.size();
+    out.push_back(GuideNode{});
+    GuideNode nd{};
+    for (int a = 0; a < 3; ++a) {
+        nd.lo[a] = boxes[ids[lo]].min[a];
+        nd.hi[a] = boxes[ids[lo]].max[a];
+    }
+    float clo[3], chi[3];
+    for (int a = 0; a < 3; ++a) clo[a] = chi[a] = 0.5f * (boxes[ids[lo]].min[a] + boxes[ids[lo]].max[a]);
+    for (size_t i = lo; i < hi; ++i)
+        for (int a = 0; a < 3; ++a) {
+            nd.lo[a] = std::min(nd.lo[a], boxes[ids[i]].min[a]);
+            nd.hi[a] = std::max(nd.hi[a], boxes[ids[i]].max[a]);
+            float c = 0.5f * (boxes[ids[i]].min[a] + boxes[ids[i]].max[a]);
+            clo[a] = std::min(clo[a], c);
+            chi[a] = std::max(chi[a], c);
+        }
+    if (hi - lo == 1) {
+        nd.region = (int32_t)ids[lo];
+        nd.right = -1;
+        out[me] = nd;
+        return me;
+    }
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+    const size_t mid = (lo + hi) / 2;
+    std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi, [&](uint32_t x, uint32_t y) {
+        float cx = boxes[x].min[axis] + boxes[x].max[axis], cy = boxes[y].min[axis] + boxes[y].max[axis];
+        return cx < cy || (cx == cy && x < y);
+    });
+    nd.region = -1;
+    buildGuideRec(out, boxes, ids, lo, mid);
+    nd.right = buildGuideRec(out, boxes, ids, mid, hi);
+    out[me] = nd;
+    return me;
+}
+
 DevIC icView(const IcBuffers &ic) {
     DevIC v;
     v.entries = ic.entries;
@@ -247,7 +293,6 @@ constexpr uint32_t kMaxBlockSums = (1u << 26) / 256 + 1;
 const char *unsupportedFeature(const rtx_push_constant &pc) {
     if (pc.useADRRS) return "useADRRS";
     if (pc.splitOnFirst) return "splitOnFirst";
-    if (pc.useGuiding) return "useGuiding";
     if (pc.updateGuiding) return "updateGuiding";
     if (pc.storeEstimate) return "storeEstimate";
     if (pc.showIrradianceCacheOnly) return "showIrradianceCacheOnly";
@@ -267,6 +312,18 @@ EventPair *nextEvents(rtx_ctx *ctx, int cls) {
 }
 
 } // namespace
+
+namespace rtx {
+std::vector<GuideNode> buildGuideNodes(const rtx_aabb *boxes, uint32_t n) {
+    std::vector<GuideNode> out;
+    if (n == 0) return out;
+    std::vector<uint32_t> ids(n);
+    for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+    out.reserve(2 * (size_t)n);
+    buildGuideRec(out, boxes, ids, 0, n);
+    return out;
+}
+} // namespace rtx
 
 extern "C" {
 
@@ -353,6 +410,7 @@ int rtx_ctx_destroy(rtx_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     freeAll(ctx->batchAllocs);
     freeAll(ctx->icAllocs);
+    freeAll(ctx->guideAllocs);
     if (ctx->fcIC) (void)hipFree(ctx->fcIC);
     if (ctx->candBuf) (void)hipFree(ctx->candBuf);
     if (ctx->keepSub) (void)hipFree(ctx->keepSub);
@@ -534,6 +592,10 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         if (!(p->pc.irradianceA > 0.0f)) return fail(ctx, RTX_E_INVALID, "irradianceA must be > 0");
         neeCap = std::max(neeCap, (uint32_t)p->pc.irradianceNumNEE);
     }
+    if (p->pc.useGuiding && ctx->guide.numRegions == 0)
+        return fail(ctx, RTX_E_STATE, "useGuiding needs guiding regions (rtx_guiding_set)");
+    if (p->pc.useGuiding && !(p->pc.guidingProb >= 0.0f && p->pc.guidingProb <= 1.0f))
+        return fail(ctx, RTX_E_INVALID, "guidingProb must be in [0, 1]");
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     const uint64_t npix = (uint64_t)p->width * p->height;
     const uint64_t total = npix * p->spp;
@@ -661,8 +723,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
                 else launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, countVisits, traceBlocks);
             });
             timed(kClsShade, [&] {
-                launchShade(s, S, icv, fcd, b, ctx->pb, ctx->q[qsel], ctx->q[qsel ^ 1], ctx->qs, ctx->qp, ctx->done,
-                            ctx->cnt, qsel, shadeBlocks);
+                launchShade(s, S, icv, ctx->guide, fcd, b, ctx->pb, ctx->q[qsel], ctx->q[qsel ^ 1], ctx->qs, ctx->qp,
+                            ctx->done, ctx->cnt, qsel, shadeBlocks);
             });
             timed(kClsShadow, [&] {
                 if (persistent) launchShadowP(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks);
@@ -786,6 +848,56 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         stats->nodeBytes = (uint32_t)sizeof(BvhNode);
         stats->primBytes = 48;
     }
+    return RTX_OK;
+}
+
+// ---------------------------------------------------------- path guiding
+int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) {
+    if (!ctx) return RTX_E_INVALID;
+    if (n && (!regions || !vmms)) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    if (n > (1u << 20)) return fail(ctx, RTX_E_INVALID, "more than 2^20 guiding regions");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (vmms[i].usedDistributions < 1 || vmms[i].usedDistributions > RTX_MAX_DISTRIBUTIONS)
+            return fail(ctx, RTX_E_INVALID, "usedDistributions must be in [1, 16]");
+        for (int a = 0; a < 3; ++a)
+            if (!(regions[i].min[a] <= regions[i].max[a])) return fail(ctx, RTX_E_INVALID, "empty guiding region box");
+    }
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    freeAll(ctx->guideAllocs);
+    ctx->guide = DevGuiding{};
+    ctx->guideBoxes.assign(regions, regions + n);
+    ctx->guideVmms.assign(vmms, vmms + n);
+    if (n == 0) return RTX_OK;
+    std::vector<GuideNode> nodes = buildGuideNodes(regions, n);
+    rtx_aabb *dBoxes = nullptr;
+    rtx_vmm_theta *dVmms = nullptr;
+    GuideNode *dNodes = nullptr;
+    hipError_t e = devAlloc(&dBoxes, n, ctx->guideAllocs);
+    if (e == hipSuccess) e = devAlloc(&dVmms, n, ctx->guideAllocs);
+    if (e == hipSuccess) e = devAlloc(&dNodes, nodes.size(), ctx->guideAllocs);
+    if (e == hipSuccess) e = hipMemcpy(dBoxes, regions, n * sizeof(rtx_aabb), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dVmms, vmms, n * sizeof(rtx_vmm_theta), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dNodes, nodes.data(), nodes.size() * sizeof(GuideNode), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        freeAll(ctx->guideAllocs);
+        return fail(ctx, RTX_E_DEVICE, std::string("guiding upload: ") + hipGetErrorString(e));
+    }
+    ctx->guide.boxes = dBoxes;
+    ctx->guide.vmms = dVmms;
+    ctx->guide.nodes = dNodes;
+    ctx->guide.numRegions = n;
+    ctx->guide.numNodes = (uint32_t)nodes.size();
+    return RTX_OK;
+}
+
+int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32_t capacity, uint32_t *n) {
+    if (!ctx) return RTX_E_INVALID;
+    const uint32_t count = (uint32_t)ctx->guideBoxes.size();
+    if (n) *n = count;
+    const uint32_t k = std::min(capacity, count);
+    if (regions) std::copy(ctx->guideBoxes.begin(), ctx->guideBoxes.begin() + k, regions);
+    if (vmms) std::copy(ctx->guideVmms.begin(), ctx->guideVmms.begin() + k, vmms);
     return RTX_OK;
 }
 

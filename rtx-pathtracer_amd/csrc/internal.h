@@ -74,6 +74,27 @@ struct FrameConst {
     uint32_t addDirect, useNEE, useIC, createIC, allowRR;
 };
 
+// ------------------------------------------------------------ path guiding
+// Box hierarchy over the guiding regions, depth-first: an inner node's left
+// child follows it, `right` names the other; a leaf holds one region.
+struct GuideNode {
+    float lo[3];
+    int32_t right;
+    float hi[3];
+    int32_t region; // >= 0: leaf
+};
+
+struct DevGuiding {
+    const rtx_aabb *boxes;
+    const rtx_vmm_theta *vmms;
+    const GuideNode *nodes;
+    uint32_t numRegions, numNodes;
+};
+
+// Builds the hierarchy over region boxes (median splits on the widest axis
+// of the centres); api.cpp.
+std::vector<GuideNode> buildGuideNodes(const rtx_aabb *boxes, uint32_t n);
+
 // ------------------------------------------------------ irradiance cache
 struct IcItem {              // one (record, grid cell) pair of the lookup index
     uint32_t entry;
@@ -160,9 +181,9 @@ void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_
                   uint32_t *fetch, int blocks);
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
                         Counters *cnt, int blocks);
-void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const FrameConst *fc, BatchDesc b,
-                 PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel,
-                 int blocks);
+void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const DevGuiding &g, const FrameConst *fc,
+                 BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt,
+                 int qsel, int blocks);
 void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numPaths,
                       PathBuffers pb, Queue q0, Counters *cnt, int qsel);
 // irradiance-cache frame bookkeeping (icache.hip)

@@ -13,6 +13,7 @@
 // Every path owns its LCG stream, so results do not depend on queue order.
 #include <hip/hip_runtime.h>
 
+#include "dev_guiding.h"
 #include "dev_icache.h"
 #include "dev_math.h"
 #include "dev_shading.h"
@@ -299,9 +300,10 @@ RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint3
 
 // ----------------------------------------------------------------- shade
 // One iteration of the bounce loop, raytrace.rgen:1008-1215 (plain method).
-__global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, const FrameConst *__restrict__ fcp,
-                                                      BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs,
-                                                      Queue qp, int32_t *done, Counters *cnt, int qsel) {
+__global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevGuiding G,
+                                                      const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
+                                                      Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done,
+                                                      Counters *cnt, int qsel) {
     const FrameConst &fc = *fcp;
     const uint32_t n = cnt->extend[qsel];
     const uint32_t numNEE = fc.numNEE;
@@ -317,6 +319,10 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, cons
     icp.invA = 1.0f / fc.pc.irradianceA;
     icp.gradients = fc.pc.useIrradianceGradients != 0;
     icp.visibility = fc.pc.irradianceCachePerformVisibilityCheck != 0;
+    GuidingParams gp;
+    gp.on = fc.pc.useGuiding != 0;
+    gp.parallax = fc.pc.useParallaxCompensation != 0;
+    gp.prob = fc.pc.guidingProb;
     const uint32_t stride = gridDim.x * kShadeBlock;
     uint32_t *nextCount = &cnt->extend[qsel ^ 1];
     for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += stride) {
@@ -467,7 +473,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, cons
         V3 newDir = v3(0.0f);
         if (icHit) isDone = true;
         if (active && !isDone) {
-            float pdf = sampleBSDF(rng, m, wi, h.normal, h.frontFace, newDir);
+            float pdf = getNewDirection(rng, G, gp, m, h.worldPos, h.normal, wi, h.frontFace, newDir);
             if (b.firstPath + pid == fc.debugPath) {
                 V3 ev = evalBsdf(m, h.normal, wi, newDir, h.frontFace);
                 printf("[gpu]   continue pdf %.9g eval %.9g new %.6g %.6g %.6g\n", pdf, ev.x, newDir.x, newDir.y, newDir.z);
@@ -675,11 +681,11 @@ void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, 
                         Counters *cnt, int blocks) {
     hipLaunchKernelGGL(kProbeResolve, dim3(blocks), dim3(256), 0, s, S, fc, q, pb, cnt);
 }
-void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const FrameConst *fc, BatchDesc b,
-                 PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel,
-                 int blocks) {
-    hipLaunchKernelGGL(kShade, dim3(blocks), dim3(kShadeBlock), 0, s, S, ic, fc, b, pb, qin, qout, qs, qp, done, cnt,
-                       qsel);
+void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const DevGuiding &g, const FrameConst *fc,
+                 BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt,
+                 int qsel, int blocks) {
+    hipLaunchKernelGGL(kShade, dim3(blocks), dim3(kShadeBlock), 0, s, S, ic, g, fc, b, pb, qin, qout, qs, qp, done,
+                       cnt, qsel);
 }
 void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
                   int blocks) {

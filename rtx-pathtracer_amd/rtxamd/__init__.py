@@ -96,6 +96,9 @@ def device_lib():
         L.rtx_ic_write.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.rtx_ic_read_jobs.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32),
                                        C.c_uint32, C.POINTER(C.c_uint32)]
+        L.rtx_guiding_set.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32]
+        L.rtx_guiding_get.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32,
+                                      C.POINTER(C.c_uint32)]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         L.rtx_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         _dev = L
@@ -317,6 +320,45 @@ def rays_array(org, dirs, tmin=1e-3, tmax=1e6):
     return r
 
 
+def vmf_theta(mu, kappa, distance=-1.0, target=(0.0, 0.0, 0.0)):
+    """One lobe as VMF_Theta::setK fills it (src/PathGuiding.h:44-50): kappa
+    below 1e-3 becomes 0, norm = k / (2 pi (1 - e^-2k)), eMin2K = e^-2k."""
+    t = abi.VmfTheta()
+    mu = np.asarray(mu, np.float64)
+    mu = mu / np.linalg.norm(mu)
+    t.mu[:] = [float(x) for x in mu]
+    k = np.float32(0.0 if kappa < 1e-3 else kappa)
+    t.k = float(k)
+    t.norm = float(np.float32(k / (2 * np.pi * (1 - np.exp(-2.0 * float(k)))))) if k > 0 else 0.0
+    t.eMin2K = float(np.float32(np.exp(-2.0 * float(k))))
+    t.distance = float(distance)
+    t.target[:] = [float(x) for x in target]
+    return t
+
+
+def vmm_theta(lobes, weights, mean_position=(0.0, 0.0, 0.0)):
+    """A mixture: lobes = [vmf_theta, ...], weights summing to 1."""
+    m = abi.VmmTheta()
+    m.usedDistributions = len(lobes)
+    for i, (t, w) in enumerate(zip(lobes, weights)):
+        m.thetas[i] = t
+        m.pi[i] = float(w)
+    m.meanPosition[:] = [float(x) for x in mean_position]
+    return m
+
+
+def guiding_arrays(regions, vmms):
+    regions = np.asarray(regions, np.float32).reshape(-1, 2, 3)
+    boxes = (abi.Aabb * max(1, len(regions)))()
+    for i, (lo, hi) in enumerate(regions):
+        boxes[i].min[:] = [float(x) for x in lo]
+        boxes[i].max[:] = [float(x) for x in hi]
+    arr = (abi.VmmTheta * max(1, len(vmms)))()
+    for i, v in enumerate(vmms):
+        arr[i] = v
+    return boxes, arr
+
+
 class IcState:
     """Host copy of an irradiance cache: the header and the entry and sphere
     buffers of src/IrradianceCache.cpp:46-75, as numpy arrays."""
@@ -412,6 +454,12 @@ class Renderer:
         for k, v in kw.items():
             setattr(t, k, v)
         self._check(device_lib().rtx_ctx_set_tuning(self._h, C.byref(t)))
+
+    # ------------------------------------------------------ guiding
+    def guiding_set(self, regions, vmms):
+        """regions: (n, 2, 3) boxes; vmms: sequence of abi.VmmTheta."""
+        boxes, arr = guiding_arrays(regions, vmms)
+        self._check(device_lib().rtx_guiding_set(self._h, boxes, arr, len(vmms)))
 
     # ------------------------------------------------ irradiance cache
     def ic_create(self, max_entries):

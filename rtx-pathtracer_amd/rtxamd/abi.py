@@ -61,6 +61,23 @@ class Sphere(C.Structure):
     _fields_ = [("center", F3), ("radius", C.c_float), ("materialIndex", C.c_int32), ("iLight", C.c_int32)]
 
 
+class Aabb(C.Structure):
+    _fields_ = [("min", F3), ("max", F3)]
+
+
+MAX_DISTRIBUTIONS = 16
+
+
+class VmfTheta(C.Structure):
+    _fields_ = [("mu", F3), ("k", C.c_float), ("norm", C.c_float), ("eMin2K", C.c_float), ("distance", C.c_float),
+                ("target", F3)]
+
+
+class VmmTheta(C.Structure):
+    _fields_ = [("thetas", VmfTheta * MAX_DISTRIBUTIONS), ("pi", C.c_float * MAX_DISTRIBUTIONS),
+                ("meanPosition", F3), ("usedDistributions", C.c_int32)]
+
+
 class CacheHeader(C.Structure):
     _fields_ = [("nextCacheSlot", C.c_uint32), ("maxCaches", C.c_uint32), ("nextUpdateSlot", C.c_uint32)]
 
@@ -170,6 +187,7 @@ class View(C.Structure):
 EXPECTED_SIZES = {
     Vertex: 48, Material: 96, Instance: 144, Light: 40, FaceSample: 12, Sphere: 24,
     PushConstant: 192, Camera: 256, Ray: 32, Hit: 16, Tuning: 32, CacheHeader: 12, CacheData: 56, IcJob: 32,
+    Aabb: 24, VmfTheta: 40, VmmTheta: 720,
 }
 for _t, _n in EXPECTED_SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)

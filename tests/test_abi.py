@@ -18,7 +18,8 @@ C_NAMES = {
     abi.Camera: "rtx_camera", abi.ModelDesc: "rtx_model_desc", abi.SceneDesc: "rtx_scene_desc",
     abi.FrameParams: "rtx_frame_params", abi.FrameStats: "rtx_frame_stats", abi.SceneInfo: "rtx_scene_info",
     abi.Ray: "rtx_ray", abi.Hit: "rtx_hit", abi.Tuning: "rtx_tuning", abi.CacheHeader: "rtx_cache_header",
-    abi.CacheData: "rtx_cache_data", abi.IcJob: "rtx_ic_job",
+    abi.CacheData: "rtx_cache_data", abi.IcJob: "rtx_ic_job", abi.Aabb: "rtx_aabb", abi.VmfTheta: "rtx_vmf_theta",
+    abi.VmmTheta: "rtx_vmm_theta",
 }
 
 
