@@ -451,6 +451,11 @@ int rtx_ic_read_jobs(rtx_ctx *ctx, rtx_ic_job *jobs, float *sub, uint32_t *subSi
  * shaders/raytrace.rgen:923-960).  n = 0 removes them. */
 int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n);
 int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32_t capacity, uint32_t *n);
+/* The guiding samples of the last frame rendered with pc.updateGuiding: the
+ * reference's DirectionalData buffer (binding 18, src/SampleCollector.cpp:
+ * 29-51), RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL slots per pixel in pixel order;
+ * unused slots have flags 0xFFFFFFFF.  count <= width*height*16. */
+int rtx_guiding_read_samples(rtx_ctx *ctx, rtx_directional_data *out, size_t count);
 
 /* ---------------------------------------------------- traversal queries */
 /* Rays for the query entry points: origin + tmin, direction + tmax. */

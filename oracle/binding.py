@@ -86,6 +86,9 @@ def lib(abi):
         L.orc_ic_query.argtypes = [C.c_void_p, C.POINTER(abi.PushConstant), vp, vp, vp, d3, d3, d3]
         L.orc_ic_apply.argtypes = [C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, vp, C.c_size_t,
                                    C.POINTER(C.c_double)]
+        L.orc_render_paths_record.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.POINTER(C.c_uint64),
+                                              C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_uint32), C.c_void_p]
         L.orc_guiding_set.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32]
         L.orc_vmm_pdf.argtypes = [C.POINTER(abi.VmmTheta), d3, d3, C.c_int]
         L.orc_vmm_pdf.restype = C.c_double
@@ -226,6 +229,21 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError(self.L.orc_last_error().decode())
         return np.array(c[:]), np.array(r[:]), np.array(t[:]), h.value
+
+    def render_paths_record(self, params, ids, prec=64, threads=None):
+        """render_paths with pc.updateGuiding: also each path's 16 DirectionalData slots."""
+        threads = DEFAULT_THREADS if threads is None else threads
+        ids = np.ascontiguousarray(ids, np.uint64)
+        n = ids.size
+        rgb = np.empty((n, 3), np.float64)
+        sig = np.empty((n, 2), np.uint32)
+        dd = np.zeros((n, self.abi.MAX_DIRECTIONAL_DATA_PER_PIXEL), self.abi.DIRECTIONAL_DTYPE)
+        rc = self.L.orc_render_paths_record(self._h, C.byref(params), ids.ctypes.data_as(C.POINTER(C.c_uint64)), n,
+                                            prec, threads, rgb.ctypes.data_as(C.POINTER(C.c_double)),
+                                            sig.ctypes.data_as(C.POINTER(C.c_uint32)), dd.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError(f"oracle render: {self.L.orc_last_error().decode()}")
+        return rgb, sig, dd
 
     def trace_closest(self, rays, brute=False, prec=64):
         rays = np.ascontiguousarray(rays, np.float32)

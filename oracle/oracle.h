@@ -44,6 +44,13 @@ int orc_guiding_set(orc_scene *scene, const rtx_aabb *regions, const rtx_vmm_the
 double orc_vmm_pdf(const rtx_vmm_theta *m, const double x[3], const double wo[3], int parallax);
 void orc_sample_vmm(const rtx_vmm_theta *m, const double x[3], int parallax, uint32_t *state, double out[3]);
 
+/* orc_render_paths with pc.updateGuiding: also returns each path's guiding
+ * sample record, RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL DirectionalData per path
+ * (the slots of its pixel in the reference's buffer, binding 18). */
+int orc_render_paths_record(const orc_scene *scene, const rtx_frame_params *params, const uint64_t *ids,
+                            size_t n, int prec, int threads, double *rgb, uint32_t *sig,
+                            rtx_directional_data *dd);
+
 /* Closest hit / occlusion with the oracle's FP64 traversal.
  * brute != 0 tests every primitive instead of walking the BVH. */
 int orc_trace_closest(const orc_scene *scene, const rtx_ray *rays, size_t n, int brute,

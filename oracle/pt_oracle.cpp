@@ -50,6 +50,11 @@ template <class R> V3<R> refract(V3<R> i, V3<R> n, R eta) {
 }
 template <class R> V3<R> f3(const float *p) { return V3<R>((R)p[0], (R)p[1], (R)p[2]); }
 template <class R> V3<R> fromD(const double *p) { return V3<R>((R)p[0], (R)p[1], (R)p[2]); }
+template <class R> void store3(float *d, V3<R> v) {
+    d[0] = (float)v.x;
+    d[1] = (float)v.y;
+    d[2] = (float)v.z;
+}
 template <class R> void toD(V3<R> v, double *p) {
     p[0] = (double)v.x;
     p[1] = (double)v.y;
@@ -115,6 +120,7 @@ constexpr uint32_t kSigVisible = 0x1b873593u;
 constexpr uint32_t kSigOccluded = 0x5bd1e995u;
 constexpr uint32_t kSigProbe = 0x85ebca6bu;
 constexpr uint32_t kSigCacheHit = 0xc2b2ae35u; // an irradiance-cache lookup that ended the path
+constexpr uint32_t kDirInvalid = 0xFFFFFFFFu;  // DirectionalData.flags INVALID, guiding.glsl:112
 
 // Irradiance cache (shaders/raytrace.rgen:1228-1421, src/IrradianceCache.h:17-30)
 constexpr int kMaxNewCacheEntries = 5;  // MAX_NEW_IRRADIANCE_ENTRIES, raytrace.rgen:65
@@ -449,6 +455,13 @@ template <class R> struct Tracer {
     bool icElect = false;
     int numCand = 0;
     V3<R> candO[kMaxNewCacheEntries], candN[kMaxNewCacheEntries];
+    // guiding sample record of this path (raytrace.rgen:966-990, 1179-1212):
+    // its RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL slots, flags INVALID on entry
+    rtx_directional_data *dd = nullptr;
+    int curSlot = 0; // currentSampleOffset; sampleOffset is 0 (one path per pixel and frame)
+    int iUpdateDistance = -1;
+    R distanceFactor = R(1);
+    V3<R> lightSums[RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL], sampleThroughputs[RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL];
 
     Tracer(const orc_scene &s, const rtx_frame_params &f) : S(s), fp(f), pc(f.pc) {}
 
@@ -967,6 +980,26 @@ template <class R> struct Tracer {
         return pdfMat * (R(1) - (R)pc.guidingProb) + pdfGuiding * (R)pc.guidingProb; // mix()
     }
 
+    // ------------------------------------------------- guiding sample record
+    static bool isMatAlmostDiscrete(const rtx_material &m) { // raytrace.rgen:962-964
+        return (m.type == RTX_MAT_ROUGH_CONDUCTOR && m.roughness <= 0.3f) ||
+               (m.type == RTX_MAT_PHONG && m.specularHighlight >= 250.0f);
+    }
+    void updateSamples(V3<R> light) { // raytrace.rgen:973-978
+        for (int i = curSlot - 1; i >= 0; --i) {
+            lightSums[i] = lightSums[i] + light;
+            light = light * sampleThroughputs[i];
+        }
+    }
+    void commitSamples() { // raytrace.rgen:980-990
+        for (int i = 0; i < curSlot; ++i) {
+            R w = length(lightSums[i]) / (R)dd[i].pdf;
+            if (w <= R(0)) dd[i].flags = kDirInvalid;
+            else dd[i].weight = (float)w;
+        }
+    }
+    void addDistance(R t) { dd[iUpdateDistance].distance = (float)((R)dd[iUpdateDistance].distance + t * distanceFactor); }
+
     // ---------------------------------------------------- irradiance cache
     // ref: raytrace.rgen:806-813
     bool isICCapable(const rtx_material &m) const {
@@ -1027,7 +1060,7 @@ template <class R> struct Tracer {
     // to camera paths when pc.enableRR is set.
     V3<R> raytrace(V3<R> origin, V3<R> direction, V3<R> throughput, int maxDepth, int maxFollowDiscrete,
                    bool addDirectLights, bool addFirstHitLight, bool useNEE, int numNEE, bool useIC, bool createIC,
-                   bool allowRR, R &firstT) {
+                   bool allowRR, R &firstT, bool saveSamples = false) {
         bool follow = true;
         int followCount = 0;
         bool addNextDirectLights = addFirstHitLight;
@@ -1039,19 +1072,28 @@ template <class R> struct Tracer {
             HitInfo<R> info = trace(origin, direction, C::TMIN, C::TMAX);
             primHash = (primHash ^ (uint32_t)info.prim) * 16777619u;
             if (info.isMiss) {
-                if (addDirectLights || addNextDirectLights) result = result + throughput * info.missColor;
+                if (addDirectLights || addNextDirectLights) {
+                    result = result + throughput * info.missColor;
+                    if (saveSamples) {
+                        updateSamples(info.missColor);
+                        if (iUpdateDistance != -1) dd[iUpdateDistance].distance = 0.0f; // infinitely far
+                    }
+                }
                 break;
             }
             const rtx_material &m = S.mats[info.matIndex];
             origin = info.worldPos;
             V3<R> normal = info.normal;
             if (depth == 1) firstT = info.t;
-            if (m.type == RTX_MAT_LIGHT && (addDirectLights || addNextDirectLights))
+            if (m.type == RTX_MAT_LIGHT && (addDirectLights || addNextDirectLights)) {
                 result = result + throughput * f3<R>(m.lightColor);
+                if (saveSamples) updateSamples(f3<R>(m.lightColor));
+            }
             if (hasDiscreteDirection(m)) {
                 addNextDirectLights = true;
                 follow = true;
                 if (depth >= maxDepth) followCount++;
+                if (saveSamples && iUpdateDistance != -1) addDistance(info.t);
             } else {
                 addNextDirectLights = false;
                 follow = false;
@@ -1072,9 +1114,14 @@ template <class R> struct Tracer {
                         }
                     }
                 }
+                if (saveSamples && iUpdateDistance != -1) {
+                    addDistance(info.t);
+                    if (!isMatAlmostDiscrete(m)) iUpdateDistance = -1; // the record's distance is final
+                }
                 if (useNEE) {
                     V3<R> nee = multipleNEE(m, origin, -direction, normal, info.isFrontFace, numNEE);
                     result = result + throughput * nee;
+                    if (saveSamples) updateSamples(nee);
                 }
             }
             V3<R> newDirection;
@@ -1085,6 +1132,29 @@ template <class R> struct Tracer {
             V3<R> change = evalBsdf(m, normal, -direction, newDirection, info.isFrontFace) / pdf;
             if (std::isnan(change.x) || std::isnan(change.y) || std::isnan(change.z)) break;
             throughput = throughput * change;
+            if (saveSamples && curSlot < RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL) {
+                rtx_directional_data &d = dd[curSlot];
+                if (hasDiscreteDirection(m) || isMatAlmostDiscrete(m)) {
+                    d.flags = kDirInvalid; // probabilities, not densities: unusable for fitting
+                } else {
+                    store3(d.position, origin);
+                    store3(d.direction, newDirection);
+                    d.pdf = (float)pdf;
+                    d.weight = 0.0f;
+                    d.distance = 0.0f;
+                    d.flags = (uint32_t)guidingRegion(origin);
+                    iUpdateDistance = curSlot;
+                    distanceFactor = R(1);
+                }
+                lightSums[curSlot] = V3<R>(0, 0, 0);
+                sampleThroughputs[curSlot] = change;
+                curSlot++;
+            } else if (saveSamples && iUpdateDistance != -1 && m.type == RTX_MAT_DIELECTRIC &&
+                       dot(newDirection, normal) < R(0)) {
+                // refraction: the next distance is the apparent one (raytrace.rgen:1200-1210)
+                R eta = info.isFrontFace ? (R)m.refractionIndex : (R)m.refractionIndexInv;
+                distanceFactor = std::fabs(dot(normal, direction) / dot(normal, newDirection)) * eta;
+            }
             direction = newDirection;
             if (allowRR && pc.enableRR && depth >= fp.rrStartDepth) {
                 R q = std::max(std::fabs(throughput.x), std::max(std::fabs(throughput.y), std::fabs(throughput.z)));
@@ -1124,8 +1194,11 @@ template <class R> struct Tracer {
         V3<R> o, d;
         cameraRay(pixel % fp.width, pixel / fp.width, fp.width, fp.height, o, d);
         R firstT;
-        return raytrace(o, d, V3<R>(1, 1, 1), pc.maxDepth, pc.maxFollowDiscrete, pc.enableNEE == 0, true,
-                        pc.enableNEE != 0, pc.numNEE, useIC, useIC, true, firstT);
+        const bool save = pc.updateGuiding != 0 && dd != nullptr;
+        V3<R> res = raytrace(o, d, V3<R>(1, 1, 1), pc.maxDepth, pc.maxFollowDiscrete, pc.enableNEE == 0, true,
+                             pc.enableNEE != 0, pc.numNEE, useIC, useIC, true, firstT, save);
+        if (save) commitSamples();
+        return res;
     }
 };
 
@@ -1152,7 +1225,7 @@ int checkParams(const rtx_frame_params *fp, bool cacheFrame = false) {
         g_err = "numNEE and irradianceNumNEE must be >= 1 with the irradiance cache";
         return RTX_E_INVALID;
     }
-    if (pc.useADRRS || pc.splitOnFirst || pc.updateGuiding ||
+    if (pc.useADRRS || pc.splitOnFirst ||
         pc.visualizeMode != 0 || pc.showIrradianceCacheOnly || pc.storeEstimate) {
         g_err = "oracle: only the plain method is restated so far";
         return RTX_E_UNSUPPORTED;
@@ -1317,11 +1390,6 @@ template <class R> void clampGradients(const rtx_push_constant &pc, V3<R> &rot, 
 
 template <class R> V3<R> mixv(V3<R> x, V3<R> y, R a) { return x * (R(1) - a) + y * a; }
 template <class R> R mixs(R x, R y, R a) { return x * (R(1) - a) + y * a; }
-template <class R> void store3(float *d, V3<R> v) {
-    d[0] = (float)v.x;
-    d[1] = (float)v.y;
-    d[2] = (float)v.z;
-}
 
 // Commit one job's result into the cache buffers (FP32, as the reference's).
 template <class R> void icCommit(IcSnapshot &live, const rtx_push_constant &pc, const IcJobRec &jb, IcResult<R> r) {
@@ -1528,8 +1596,8 @@ int orc_render_paths(const orc_scene *s, const rtx_frame_params *fp, const uint6
         g_err = "prec must be 32 or 64";
         return RTX_E_INVALID;
     }
-    if (fp->pc.useGuiding && s->regions.empty()) {
-        g_err = "useGuiding without guiding regions (orc_guiding_set)";
+    if ((fp->pc.useGuiding || fp->pc.updateGuiding) && s->regions.empty()) {
+        g_err = "guiding without guiding regions (orc_guiding_set)";
         return RTX_E_STATE;
     }
     parallelFor(n, threads, [&](size_t i) {
@@ -1553,6 +1621,48 @@ int orc_render_paths(const orc_scene *s, const rtx_frame_params *fp, const uint6
         rgb[3 * i + 2] = r[2];
         if (sig) {
             sig[2 * i + 0] = st;
+            sig[2 * i + 1] = hs;
+        }
+    });
+    return RTX_OK;
+}
+
+int orc_render_paths_record(const orc_scene *s, const rtx_frame_params *fp, const uint64_t *ids, size_t n, int prec,
+                            int threads, double *rgb, uint32_t *sig, rtx_directional_data *dd) {
+    if (!s || !fp || (!ids && n) || (!rgb && n) || (!dd && n)) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    int rc = checkParams(fp);
+    if (rc != RTX_OK) return rc;
+    if ((fp->pc.useGuiding || fp->pc.updateGuiding) && s->regions.empty()) {
+        g_err = "guiding without guiding regions (orc_guiding_set)";
+        return RTX_E_STATE;
+    }
+    parallelFor(n, threads, [&](size_t i) {
+        rtx_directional_data *slots = dd + (size_t)RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL * i;
+        for (int k = 0; k < RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL; ++k) {
+            slots[k] = rtx_directional_data{};
+            slots[k].flags = kDirInvalid; // resetSamplesToInvalid, raytrace.rgen:1615-1621
+        }
+        double r[3];
+        uint32_t st, hs;
+        if (prec == 64) {
+            Tracer<double> t(*s, *fp);
+            t.dd = slots;
+            toD(t.renderPath(ids[i]), r);
+            st = t.seed;
+            hs = t.primHash;
+        } else {
+            Tracer<float> t(*s, *fp);
+            t.dd = slots;
+            toD(t.renderPath(ids[i]), r);
+            st = t.seed;
+            hs = t.primHash;
+        }
+        for (int a = 0; a < 3; ++a) rgb[3 * i + a] = r[a];
+        if (sig) {
+            sig[2 * i] = st;
             sig[2 * i + 1] = hs;
         }
     });

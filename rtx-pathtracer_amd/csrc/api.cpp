@@ -73,6 +73,11 @@ struct rtx_ctx {
     uint2 *keepSubSig = nullptr;
     size_t keepSubCap = 0;
     uint32_t keptJobs = 0, keptUpdates = 0;
+    // guiding sample records (pc.updateGuiding)
+    std::vector<void *> recAllocs;    // per-path record state, recCap paths
+    size_t recCap = 0;
+    rtx_directional_data *dirData = nullptr;
+    size_t dirCap = 0, dirCount = 0;  // slots allocated / slots of the last recording render
     // path guiding (rtx_guiding_set)
     std::vector<void *> guideAllocs;
     DevGuiding guide{};
@@ -293,7 +298,6 @@ constexpr uint32_t kMaxBlockSums = (1u << 26) / 256 + 1;
 const char *unsupportedFeature(const rtx_push_constant &pc) {
     if (pc.useADRRS) return "useADRRS";
     if (pc.splitOnFirst) return "splitOnFirst";
-    if (pc.updateGuiding) return "updateGuiding";
     if (pc.storeEstimate) return "storeEstimate";
     if (pc.showIrradianceCacheOnly) return "showIrradianceCacheOnly";
     if (pc.visualizeMode != 0) return "visualizeMode";
@@ -411,6 +415,8 @@ int rtx_ctx_destroy(rtx_ctx *ctx) {
     freeAll(ctx->batchAllocs);
     freeAll(ctx->icAllocs);
     freeAll(ctx->guideAllocs);
+    freeAll(ctx->recAllocs);
+    if (ctx->dirData) (void)hipFree(ctx->dirData);
     if (ctx->fcIC) (void)hipFree(ctx->fcIC);
     if (ctx->candBuf) (void)hipFree(ctx->candBuf);
     if (ctx->keepSub) (void)hipFree(ctx->keepSub);
@@ -592,8 +598,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         if (!(p->pc.irradianceA > 0.0f)) return fail(ctx, RTX_E_INVALID, "irradianceA must be > 0");
         neeCap = std::max(neeCap, (uint32_t)p->pc.irradianceNumNEE);
     }
-    if (p->pc.useGuiding && ctx->guide.numRegions == 0)
-        return fail(ctx, RTX_E_STATE, "useGuiding needs guiding regions (rtx_guiding_set)");
+    if ((p->pc.useGuiding || p->pc.updateGuiding) && ctx->guide.numRegions == 0)
+        return fail(ctx, RTX_E_STATE, "useGuiding/updateGuiding need guiding regions (rtx_guiding_set)");
     if (p->pc.useGuiding && !(p->pc.guidingProb >= 0.0f && p->pc.guidingProb <= 1.0f))
         return fail(ctx, RTX_E_INVALID, "guidingProb must be in [0, 1]");
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
@@ -623,7 +629,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     // the cache one frame (sample plane) at a time, each followed by its upkeep
     uint64_t batch = p->batchPaths ? p->batchPaths : ctx->tuning.batchPaths;
     batch = std::min<uint64_t>(batch, total);
-    if (useIC) batch = std::min<uint64_t>(batch, npix);
+    const bool record = p->pc.updateGuiding != 0;
+    if (useIC || record) batch = std::min<uint64_t>(batch, npix);
     if (batch >= npix) batch -= batch % npix;
     size_t bufPaths = (size_t)batch;
     if (useIC) bufPaths = std::max<size_t>(bufPaths, kIcStrata); // at least one job of sub-paths
@@ -637,6 +644,28 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         ctx->candCap = bufPaths;
     }
     ctx->pb.cand = ctx->candBuf;
+    if (record) {
+        if (ctx->recCap < bufPaths) {
+            freeAll(ctx->recAllocs);
+            ctx->recCap = 0;
+            auto &L = ctx->recAllocs;
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.grec, bufPaths, L));
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.grecThr, bufPaths, L));
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.lightSum, bufPaths * kMaxRecords, L));
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.sampleThr, bufPaths * kMaxRecords, L));
+            ctx->recCap = bufPaths;
+        }
+        const size_t slots = (size_t)npix * kMaxRecords;
+        if (ctx->dirCap < slots) {
+            if (ctx->dirData) (void)hipFree(ctx->dirData);
+            ctx->dirData = nullptr;
+            ctx->dirCap = 0;
+            CHECK_HIP(ctx, hipMalloc(&ctx->dirData, slots * sizeof(rtx_directional_data)));
+            ctx->dirCap = slots;
+        }
+        ctx->pb.dirData = ctx->dirData;
+        ctx->dirCount = slots;
+    }
 
     const bool keep = (p->flags & RTX_FRAME_KEEP_PATHS) != 0;
     if (keep) {
@@ -676,6 +705,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.useIC = useIC ? 1u : 0u;
     fc.createIC = useIC ? 1u : 0u;
     fc.allowRR = 1;
+    fc.record = record ? 1u : 0u;
     CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));
     // the cache's stratum sub-paths: raytrace(..., 1, 10, false, false, true, irradianceNumNEE,
     // true, false, ...) (raytrace.rgen:1260)
@@ -687,6 +717,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fcIC.useIC = 1;
     fcIC.createIC = 0;
     fcIC.allowRR = 0;
+    fcIC.record = 0;
     fcIC.numNEE = useIC ? (uint32_t)p->pc.irradianceNumNEE : 1u;
     fcIC.debugPath = ~0ull;
     if (useIC) CHECK_HIP(ctx, hipMemcpyAsync(ctx->fcIC, &fcIC, sizeof(fcIC), hipMemcpyHostToDevice, ctx->stream));
@@ -744,7 +775,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     CHECK_HIP(ctx, hipEventRecord(ctx->tStart, s));
     uint32_t batches = 0;
     if (useIC) timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
-    const uint64_t frameLen = useIC ? npix : total; // paths per upkeep step
+    const uint64_t frameLen = (useIC || record) ? npix : total; // paths per frame with per-frame state
     for (uint64_t f0 = 0; f0 < total; f0 += frameLen) {
         const uint64_t fEnd = std::min(total, f0 + frameLen);
         if (useIC) timed(kClsShade, [&] { launchIcFrameBegin(s, ctx->ic); });
@@ -753,6 +784,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.firstPath = first;
             b.numPaths = (uint32_t)std::min<uint64_t>(batch, fEnd - first);
             b.npix = (uint32_t)npix;
+            ctx->pb.dirFirstPixel = (uint32_t)(first % npix);
             timed(kClsShade, [&] { launchGenerate(s, ctx->fc, b, ctx->pb, ctx->q[0], ctx->cnt, 0, ctx->ic.frame); });
             bounces(ctx->fc, b, maxIter);
             timed(kClsShade, [&] {
@@ -898,6 +930,18 @@ int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32
     const uint32_t k = std::min(capacity, count);
     if (regions) std::copy(ctx->guideBoxes.begin(), ctx->guideBoxes.begin() + k, regions);
     if (vmms) std::copy(ctx->guideVmms.begin(), ctx->guideVmms.begin() + k, vmms);
+    return RTX_OK;
+}
+
+int rtx_guiding_read_samples(rtx_ctx *ctx, rtx_directional_data *out, size_t count) {
+    if (!ctx) return RTX_E_INVALID;
+    if (ctx->dirCount == 0) return fail(ctx, RTX_E_STATE, "no render with updateGuiding yet");
+    if (count > ctx->dirCount) return fail(ctx, RTX_E_INVALID, "more samples asked for than recorded");
+    if (count == 0) return RTX_OK;
+    if (!out) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CHECK_HIP(ctx, hipMemcpy(out, ctx->dirData, count * sizeof(rtx_directional_data), hipMemcpyDeviceToHost));
     return RTX_OK;
 }
 

@@ -72,6 +72,7 @@ struct FrameConst {
     // the irradiance cache (calculateCacheData, raytrace.rgen:1260).
     int32_t maxDepth, maxFollow;
     uint32_t addDirect, useNEE, useIC, createIC, allowRR;
+    uint32_t record;          // saveSamples: write guiding sample records (pc.updateGuiding)
 };
 
 // ------------------------------------------------------------ path guiding
@@ -154,7 +155,16 @@ struct PathBuffers {
     float4 *out;      // per local path: radiance xyz, w = first hit distance
     uint2 *outSig;    // per local path: (final LCG state, primitive hash)
     float4 *cand;     // 2 per candidate, kMaxNewCacheEntries per path: origin, normal
+    // guiding sample record (pc.updateGuiding)
+    int4 *grec;       // per path: (current slot, slot whose distance grows, distanceFactor bits, NEE slot base)
+    float4 *grecThr;  // per path: throughput at the pending NEE
+    float4 *lightSum; // kMaxRecords per path
+    float4 *sampleThr;// kMaxRecords per path: throughput change of the slot's vertex
+    rtx_directional_data *dirData; // per pixel of the frame: kMaxRecords slots
+    uint32_t dirFirstPixel;        // pixel of local path 0 (frames are batched by pixel ranges)
 };
+constexpr int kMaxRecords = RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL;
+constexpr uint32_t kDirInvalid = 0xFFFFFFFFu; // DirectionalData INVALID, guiding.glsl:112
 
 // Path flags packed in thr.w
 constexpr uint32_t kFlagDepthMask = 0xFFu;        // bits 0-7 depth

@@ -57,6 +57,28 @@ template <bool COUNT> RTX_DI void flushVisits(const VisitCount &vc, Counters *cn
     }
 }
 
+// ------------------------------------------------------ guiding records
+// updateSamples (raytrace.rgen:973-978): light reaching a vertex is added to
+// the light sums of the path's earlier records, through their throughputs.
+RTX_DI void updateSamples(const PathBuffers &pb, uint32_t pid, int base, V3 light) {
+    float4 *ls = pb.lightSum + (size_t)pid * kMaxRecords;
+    const float4 *st = pb.sampleThr + (size_t)pid * kMaxRecords;
+    for (int i = base - 1; i >= 0; --i) {
+        float4 a = ls[i];
+        ls[i] = f4(v3(a) + light, 0.0f);
+        light = light * v3(st[i]);
+    }
+}
+
+RTX_DI rtx_directional_data *recordSlots(const PathBuffers &pb, uint32_t pid) {
+    return pb.dirData + (size_t)(pb.dirFirstPixel + pid) * kMaxRecords;
+}
+
+RTX_DI bool isMatAlmostDiscrete(const Mat &m) { // raytrace.rgen:962-964
+    return (m.type == RTX_MAT_ROUGH_CONDUCTOR && m.roughness <= 0.3f) ||
+           (m.type == RTX_MAT_PHONG && m.highlight >= 250.0f);
+}
+
 // ------------------------------------------------------------ generate
 // main() up to the camera ray: seed = tea(pixel, randomUInt) (raytrace.rgen:1625),
 // the irradiance-cache update election (1637-1639, the stream's first draw),
@@ -94,6 +116,11 @@ __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ 
     q0.dir[pid] = f4(d, kTMax);
     pb.rng[pid] = rng.s;
     pb.hash[pid] = 2166136261u;
+    if (fc.record) { // resetSamplesToInvalid, raytrace.rgen:1615-1621
+        rtx_directional_data *dd = recordSlots(pb, pid);
+        for (int k = 0; k < kMaxRecords; ++k) dd[k].flags = kDirInvalid;
+        pb.grec[pid] = make_int4(0, -1, __float_as_int(1.0f), 0);
+    }
     // depth 0, followCount 0, follow = true, addNextDirectLights = addFirstHitLight = true
     pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagFollow | kFlagAddNext));
     pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
@@ -314,6 +341,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
     const bool power = fc.pc.usePowerHeuristic != 0;
     const int maxDepth = fc.maxDepth, maxFollow = fc.maxFollow;
     const bool useIC = fc.useIC != 0, createIC = fc.createIC != 0;
+    const bool record = fc.record != 0;
     const bool icOnGlossy = fc.pc.useIrradianceCacheOnGlossy != 0;
     IcQueryParams icp;
     icp.invA = 1.0f / fc.pc.irradianceA;
@@ -339,6 +367,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
         int depth = 0, followCount = 0;
         bool follow = false, addNext = false, icHit = false;
         uint32_t ncand = 0;
+        int4 grec = make_int4(0, -1, 0, 0); // record state: slot, distance slot, distanceFactor bits, NEE base
         float firstT = kTMax;
         if (active) {
             pid = (uint32_t)qin.id[i];
@@ -354,7 +383,16 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             firstT = r4.w;
             ncand = (flags & kFlagCandMask) >> kFlagCandShift;
             hash = pb.hash[pid];
-            if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE, hash);
+            if (record) grec = pb.grec[pid];
+            if (flags & kFlagPending) {
+                V3 nee = pendingSum(pb, pid, numNEE, hash);
+                if (record) { // recorded NEE terms are kept without the throughput
+                    res = res + v3(pb.grecThr[pid]) * nee;
+                    updateSamples(pb, pid, grec.w, nee);
+                } else {
+                    res = res + nee;
+                }
+            }
             int prim = __float_as_int(hit.y);
             hash = sigFold(hash, (uint32_t)prim);
             depth = (int)(flags & kFlagDepthMask) + 1;
@@ -363,7 +401,13 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             addNext = (flags & kFlagAddNext) != 0;
             if (prim < 0) {
                 // raytrace.rmiss: black default environment
-                if (addDirect || addNext) res = res + thr * v3(0.0f);
+                if (addDirect || addNext) {
+                    res = res + thr * v3(0.0f);
+                    if (record) {
+                        updateSamples(pb, pid, grec.x, v3(0.0f));
+                        if (grec.y != -1) recordSlots(pb, pid)[grec.y].distance = 0.0f; // infinitely far
+                    }
+                }
                 isDone = true;
             } else {
                 h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
@@ -373,11 +417,15 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                     printf("[gpu] depth %d prim %d t %.9g mat %d seed %u thr %.6g %.6g %.6g d %.6g %.6g %.6g n %.6g %.6g %.6g ff %d res %.6g\n",
                            depth, prim, hit.x, m.type, rng.s, thr.x, thr.y, thr.z, d.x, d.y, d.z, h.normal.x,
                            h.normal.y, h.normal.z, (int)h.frontFace, res.x);
-                if (m.type == RTX_MAT_LIGHT && (addDirect || addNext)) res = res + thr * m.lightColor;
+                if (m.type == RTX_MAT_LIGHT && (addDirect || addNext)) {
+                    res = res + thr * m.lightColor;
+                    if (record) updateSamples(pb, pid, grec.x, m.lightColor);
+                }
                 if (hasDiscreteDirection(m.type)) {
                     addNext = true;
                     follow = true;
                     if (depth >= maxDepth) followCount++;
+                    if (record && grec.y != -1) recordSlots(pb, pid)[grec.y].distance += hit.x * __int_as_float(grec.z);
                 } else {
                     addNext = false;
                     follow = false;
@@ -399,12 +447,18 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                             }
                         }
                     }
+                    if (record && !icHit && grec.y != -1) { // raytrace.rgen:1153-1160
+                        recordSlots(pb, pid)[grec.y].distance += hit.x * __int_as_float(grec.z);
+                        if (!isMatAlmostDiscrete(m)) grec.y = -1;
+                    }
                     doNee = nee && neeCapable(m.type) && S.lightCount > 0;
+                    if (record) grec.w = grec.x; // the NEE light reaches the records made so far
                 }
             }
         }
         // ---- next event estimation, raytrace.rgen:601-731 (numNEE samples, multipleNEE 871-877)
         const V3 wi = -d;
+        const V3 thrNee = thr; // the throughput NEE terms of this vertex carry
         for (uint32_t j = 0; j < numNEE; ++j) {
             bool wantShadow = false, wantProbe = false;
             V3 lightDir = v3(0.0f), bsdfDir = v3(0.0f), probeBase = v3(0.0f);
@@ -441,7 +495,8 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                         sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor / pdfL;
                     }
                 }
-                pb.pendShadow[slot] = f4(thr * sh * invNEE, shadowState);
+                const V3 tw = record ? v3(1.0f) : thr;
+                pb.pendShadow[slot] = f4(tw * sh * invNEE, shadowState);
                 wantShadow = traced;
                 bool probeZero = true;
                 if (mis && probeAllowed) {
@@ -450,7 +505,8 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                         wantProbe = true;
                         probeZero = false;
                         pdfMatProbe = pdfMat;
-                        probeBase = thr * evalBsdf(m, h.normal, wi, bsdfDir, true) * invNEE / pdfMat;
+                        probeBase = (record ? v3(1.0f) : thr) * evalBsdf(m, h.normal, wi, bsdfDir, true) * invNEE /
+                                    pdfMat;
                     }
                 }
                 if (probeZero) pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-2));
@@ -486,6 +542,32 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                 isDone = true;
             } else {
                 thr = thr * change;
+                if (record && grec.x < kMaxRecords) { // raytrace.rgen:1179-1199
+                    rtx_directional_data &dd = recordSlots(pb, pid)[grec.x];
+                    if (hasDiscreteDirection(m.type) || isMatAlmostDiscrete(m)) {
+                        dd.flags = kDirInvalid;
+                    } else {
+                        dd.position[0] = h.worldPos.x;
+                        dd.position[1] = h.worldPos.y;
+                        dd.position[2] = h.worldPos.z;
+                        dd.direction[0] = newDir.x;
+                        dd.direction[1] = newDir.y;
+                        dd.direction[2] = newDir.z;
+                        dd.pdf = pdf;
+                        dd.weight = 0.0f;
+                        dd.distance = 0.0f;
+                        dd.flags = (uint32_t)guidingRegion(G, h.worldPos);
+                        grec.y = grec.x;
+                        grec.z = __float_as_int(1.0f);
+                    }
+                    pb.lightSum[(size_t)pid * kMaxRecords + grec.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    pb.sampleThr[(size_t)pid * kMaxRecords + grec.x] = f4(change, 0.0f);
+                    grec.x++;
+                } else if (record && grec.y != -1 && m.type == RTX_MAT_DIELECTRIC && dot(newDir, h.normal) < 0.0f) {
+                    // refraction: the next distance is the apparent one (raytrace.rgen:1200-1210)
+                    float eta = h.frontFace ? m.ior : m.iorInv;
+                    grec.z = __float_as_int(fabsf(dot(h.normal, d) / dot(h.normal, newDir)) * eta);
+                }
                 bool killed = false;
                 if (fc.allowRR && fc.pc.enableRR && depth >= fc.rrStartDepth) { // Russian roulette (not in the reference)
                     float q = fmaxf(fabsf(thr.x), fmaxf(fabsf(thr.y), fabsf(thr.z)));
@@ -509,6 +591,10 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             pb.hash[pid] = hash;
             pb.thr[pid] = f4(thr, __uint_as_float(nf));
             pb.result[pid] = f4(res, firstT);
+            if (record) {
+                if (doNee) pb.grecThr[pid] = f4(thrNee, 0.0f); // the throughput the pending NEE terms carry
+                pb.grec[pid] = grec;
+            }
         }
         int qi = waveAppend(cont, nextCount);
         if (cont) {
@@ -526,13 +612,32 @@ __global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fc
                                                const int32_t *done, Counters *cnt) {
     const uint32_t n = cnt->done;
     const uint32_t numNEE = fcp->numNEE;
+    const bool record = fcp->record != 0;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         uint32_t pid = (uint32_t)done[i];
         float4 r4 = pb.result[pid];
         V3 res = v3(r4);
         uint32_t flags = __float_as_uint(pb.thr[pid].w);
         uint32_t hash = pb.hash[pid];
-        if (flags & kFlagPending) res = res + pendingSum(pb, pid, numNEE, hash);
+        int4 grec = record ? pb.grec[pid] : make_int4(0, -1, 0, 0);
+        if (flags & kFlagPending) {
+            V3 nee = pendingSum(pb, pid, numNEE, hash);
+            if (record) {
+                res = res + v3(pb.grecThr[pid]) * nee;
+                updateSamples(pb, pid, grec.w, nee);
+            } else {
+                res = res + nee;
+            }
+        }
+        if (record) { // commitSamples, raytrace.rgen:980-990
+            rtx_directional_data *dd = recordSlots(pb, pid);
+            const float4 *ls = pb.lightSum + (size_t)pid * kMaxRecords;
+            for (int k = 0; k < grec.x; ++k) {
+                float w = length(v3(ls[k])) / dd[k].pdf;
+                if (w <= 0.0f) dd[k].flags = kDirInvalid;
+                else dd[k].weight = w;
+            }
+        }
         pb.out[pid] = f4(res, r4.w);
         pb.outSig[pid] = make_uint2(pb.rng[pid], hash);
     }

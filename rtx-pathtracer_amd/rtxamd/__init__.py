@@ -99,6 +99,7 @@ def device_lib():
         L.rtx_guiding_set.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32]
         L.rtx_guiding_get.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32,
                                       C.POINTER(C.c_uint32)]
+        L.rtx_guiding_read_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         L.rtx_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         _dev = L
@@ -460,6 +461,12 @@ class Renderer:
         """regions: (n, 2, 3) boxes; vmms: sequence of abi.VmmTheta."""
         boxes, arr = guiding_arrays(regions, vmms)
         self._check(device_lib().rtx_guiding_set(self._h, boxes, arr, len(vmms)))
+
+    def guiding_read_samples(self, width, height):
+        """The last recording frame's samples: (width*height, 16) DIRECTIONAL_DTYPE."""
+        out = np.zeros((width * height, abi.MAX_DIRECTIONAL_DATA_PER_PIXEL), abi.DIRECTIONAL_DTYPE)
+        self._check(device_lib().rtx_guiding_read_samples(self._h, out.ctypes.data_as(C.c_void_p), out.size))
+        return out
 
     # ------------------------------------------------ irradiance cache
     def ic_create(self, max_entries):

@@ -78,6 +78,12 @@ class VmmTheta(C.Structure):
                 ("meanPosition", F3), ("usedDistributions", C.c_int32)]
 
 
+DIRECTIONAL_DTYPE = np.dtype([("position", "<f4", 3), ("direction", "<f4", 3), ("weight", "<f4"), ("pdf", "<f4"),
+                              ("distance", "<f4"), ("flags", "<u4")])
+DIR_INVALID = 0xFFFFFFFF
+MAX_DIRECTIONAL_DATA_PER_PIXEL = 16
+
+
 class CacheHeader(C.Structure):
     _fields_ = [("nextCacheSlot", C.c_uint32), ("maxCaches", C.c_uint32), ("nextUpdateSlot", C.c_uint32)]
 

@@ -86,3 +86,42 @@ def test_guiding_errors(renderer):
     with pytest.raises(rtxamd.RtxError):
         renderer.guiding_set([[[0, 0, 0], [1, 1, 1]]], [bad])
     ds.close()
+
+
+@pytest.mark.parametrize("guided", [0, 1])
+def test_sample_records_match_oracle(renderer, guided):
+    """updateGuiding: the DirectionalData slots of every pixel (position,
+    direction, pdf, weight = |light sum| / pdf, distance, region) against the
+    oracle's for the paths that take the same branches."""
+    hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
+    regions, vmms = regions_and_mixtures(1)
+    renderer.guiding_set(regions, vmms)
+    orc = OracleScene(abi, hs.desc())
+    orc.guiding_set(regions, vmms)
+    ds = renderer.upload(hs)
+    w = h = 96
+    fp = scenes.params(hs, w, h, 1, maxDepth=8, enableMIS=1, updateGuiding=1, useGuiding=guided, guidingProb=0.5,
+                       sample_offset=5, flags=abi.RTX_FRAME_KEEP_PATHS | abi.RTX_FRAME_RESET_ACCUM)
+    renderer.render(ds, fp)
+    rgb, sig = renderer.read_paths(w * h)
+    dd = renderer.guiding_read_samples(w, h)
+    ids = np.arange(w * h, dtype=np.uint64)
+    orgb, osig, odd = orc.render_paths_record(fp, ids)
+    same, _ = check_paths(f"recording paths (guided {guided})", rgb, sig, orgb, osig, abs_same=0.97,
+                          abs_within=0.99)
+    g, o = dd[same], odd[same]
+    gv, ov = g["flags"] != abi.DIR_INVALID, o["flags"] != abi.DIR_INVALID
+    assert np.array_equal(gv, ov)
+    assert np.array_equal(g["flags"][gv], o["flags"][ov])
+    log(f"  valid records {gv.sum()} in {same.sum()} paths")
+    for f in ("position", "direction"):
+        assert np.allclose(g[f][gv], o[f][ov], rtol=1e-4, atol=2e-5), f
+    for f, tol in (("pdf", 1e-4), ("distance", 1e-4)):
+        a, b = g[f][gv].astype(np.float64), o[f][ov].astype(np.float64)
+        assert np.all(np.abs(a - b) <= tol * np.abs(b) + 1e-5), f
+    a, b = g["weight"][gv].astype(np.float64), o["weight"][ov].astype(np.float64)
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-6)
+    log(f"  weight rel err p50 {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e}")
+    assert np.mean(rel <= 1e-4) >= 0.99
+    renderer.guiding_set([], [])
+    ds.close()

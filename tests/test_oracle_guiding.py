@@ -144,3 +144,74 @@ def test_guiding_without_regions_is_an_error():
     orc = OracleScene(abi, hs.desc())
     with pytest.raises(RuntimeError):
         orc.render_paths(scenes.params(hs, 4, 4, 1, useGuiding=1), np.arange(4, dtype=np.uint64))
+
+
+# ------------------------------------------------------------- sample record
+def record_scene():
+    hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
+    orc = OracleScene(abi, hs.desc())
+    regions = [[[-1.01, -1.01, -0.01], [0.0, 1.01, 2.01]], [[0.0, -1.01, -0.01], [1.01, 1.01, 2.01]]]
+    vmms = [rtxamd.vmm_theta([rtxamd.vmf_theta((0, 0, 1), 3.0)], [1.0])] * 2
+    orc.guiding_set(regions, vmms)
+    return hs, orc, regions
+
+
+def wall_albedo(p):
+    # the Cornell generator's walls: red at x = -1, green at x = +1, white elsewhere
+    if abs(p[0] + 1.0) < 1e-4:
+        return np.array([0.63, 0.065, 0.05])
+    if abs(p[0] - 1.0) < 1e-4:
+        return np.array([0.14, 0.45, 0.09])
+    return np.array([0.73, 0.73, 0.73])
+
+
+def test_record_weights_carry_the_path_radiance():
+    """NEE off, one bounce: every light reaching the camera path after vertex 0
+    passes through slot 0, so lightSums[0] = result / albedo_0 (the Lambert
+    throughput change f cos / pdf is the albedo) and weight = |.| / pdf."""
+    hs, orc, regions = record_scene()
+    n = 64 * 64 * 8
+    fp = scenes.params(hs, 64, 64, 8, maxDepth=1, maxFollowDiscrete=0, enableNEE=0, updateGuiding=1)
+    rgb, sig, dd = orc.render_paths_record(fp, np.arange(n, dtype=np.uint64))
+    checked = 0
+    for i in range(n):
+        s0 = dd[i, 0]
+        if s0["flags"] == abi.DIR_INVALID or rgb[i].max() <= 0:
+            continue
+        p = s0["position"].astype(np.float64)
+        if abs(abs(p[0]) - 1.0) > 1e-4 and abs(p[2]) > 1e-4 and abs(p[1] - 1.0) > 1e-4:
+            continue  # a box face or the ceiling: keep to the walls and floor
+        want = np.linalg.norm(rgb[i] / wall_albedo(p)) / s0["pdf"]
+        assert s0["weight"] == pytest.approx(want, rel=2e-5)
+        checked += 1
+    assert checked > 100
+
+
+def test_record_distances_regions_and_invalid_slots():
+    hs, orc, regions = record_scene()
+    n = 32 * 32 * 2
+    fp = scenes.params(hs, 32, 32, 2, maxDepth=6, enableMIS=1, updateGuiding=1)
+    rgb, sig, dd = orc.render_paths_record(fp, np.arange(n, dtype=np.uint64))
+    valid = dd["flags"] != abi.DIR_INVALID
+    assert valid.any(axis=1).mean() > 0.5
+    # the region of a record is the region holding its position (smallest index on the shared face)
+    pos = dd["position"][valid]
+    want = np.where(pos[:, 0] <= 0.0, 0, 1)
+    assert np.array_equal(dd["flags"][valid], want)
+    # directions are unit vectors in the upper hemisphere of where they start; pdfs positive
+    d = dd["direction"][valid]
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-5)
+    assert np.all(dd["pdf"][valid] > 0) and np.all(dd["weight"][valid] > 0)
+    # consecutive Lambert vertices: a record's distance is the segment to the next vertex
+    ok = 0
+    for i in range(n):
+        for k in range(15):
+            a, b = dd[i, k], dd[i, k + 1]
+            if a["flags"] == abi.DIR_INVALID or b["flags"] == abi.DIR_INVALID or a["distance"] <= 0:
+                continue
+            seg = np.linalg.norm(b["position"].astype(np.float64) - a["position"])
+            assert a["distance"] == pytest.approx(seg, rel=1e-4, abs=1e-5)
+            # and the next vertex lies along the recorded direction
+            assert np.allclose(a["position"] + a["distance"] * a["direction"], b["position"], atol=1e-4)
+            ok += 1
+    assert ok > 100
