@@ -94,6 +94,15 @@ def lib(abi):
         L.orc_vmm_pdf.restype = C.c_double
         L.orc_sample_vmm.argtypes = [C.POINTER(abi.VmmTheta), d3, C.c_int, C.POINTER(C.c_uint32), d3]
         L.orc_sample_vmm.restype = None
+        L.orc_guide_create.argtypes = [C.POINTER(abi.Aabb), C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.orc_guide_destroy.argtypes = [C.c_void_p]
+        L.orc_guide_destroy.restype = None
+        L.orc_guide_regions.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32]
+        L.orc_guide_regions.restype = C.c_uint32
+        L.orc_guide_update.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.orc_guide_last_error.restype = C.c_char_p
+        L.orc_fastexp.argtypes = [C.c_float]
+        L.orc_fastexp.restype = C.c_float
         L.orc_ic_calculate.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, C.c_uint32, d3,
                                        d3, d3, d3, d3, C.POINTER(C.c_double)]
         _lib = L
@@ -269,3 +278,39 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError(self.L.orc_last_error().decode())
         return occ
+
+
+class OracleGuide:
+    """PathGuiding's regions and mixtures, refit with the oracle's EM."""
+
+    def __init__(self, abi, scene_box, splits, parallax=True, prec=64):
+        self.abi = abi
+        self.L = lib(abi)
+        box = abi.Aabb()
+        box.min[:] = [float(x) for x in scene_box[0]]
+        box.max[:] = [float(x) for x in scene_box[1]]
+        self._h = C.c_void_p()
+        rc = self.L.orc_guide_create(C.byref(box), splits, int(parallax), prec, C.byref(self._h))
+        if rc != 0:
+            raise RuntimeError(self.L.orc_guide_last_error().decode())
+
+    def __del__(self):
+        try:
+            if self._h:
+                self.L.orc_guide_destroy(self._h)
+        except Exception:
+            pass
+
+    def regions(self):
+        n = self.L.orc_guide_regions(self._h, None, None, 0)
+        boxes = (self.abi.Aabb * n)()
+        vmms = (self.abi.VmmTheta * n)()
+        self.L.orc_guide_regions(self._h, boxes, vmms, n)
+        arr = np.array([[b.min[:], b.max[:]] for b in boxes], np.float32)
+        return arr, [vmms[i] for i in range(n)]
+
+    def update(self, samples):
+        samples = np.ascontiguousarray(samples, self.abi.DIRECTIONAL_DTYPE).reshape(-1)
+        rc = self.L.orc_guide_update(self._h, samples.ctypes.data_as(C.c_void_p), samples.size)
+        if rc != 0:
+            raise RuntimeError(self.L.orc_guide_last_error().decode())

@@ -103,6 +103,20 @@ int orc_ic_query(const orc_scene *scene, const rtx_push_constant *pc, const rtx_
                  const rtx_cache_data *entries, const rtx_sphere *spheres, const double x[3], const double n[3],
                  double color[3]);
 
+/* Path-guiding update (em_oracle.cpp): PathGuiding's regions and per-region
+ * vMF mixtures, refit from recorded samples with lightpmm's MAP-EM (fit on
+ * the first update, updateFit after), parallax-aware when `parallax`.
+ * Component split/merge is not restated. */
+typedef struct orc_guide orc_guide;
+int orc_guide_create(const rtx_aabb *sceneBox, uint32_t splitCount, int parallax, int prec, orc_guide **out);
+void orc_guide_destroy(orc_guide *g);
+/* Region boxes and VMM_Theta (as syncPMMsToVMM_Thetas fills them); returns the region count. */
+uint32_t orc_guide_regions(const orc_guide *g, rtx_aabb *boxes, rtx_vmm_theta *vmms, uint32_t capacity);
+/* PathGuiding::update with these samples (flags = region, 0xFFFFFFFF = unused). */
+int orc_guide_update(orc_guide *g, const rtx_directional_data *samples, size_t n);
+const char *orc_guide_last_error(void);
+float orc_fastexp(float x);
+
 /* Building blocks, exposed for known-answer and distribution tests. */
 uint32_t orc_tea(uint32_t v0, uint32_t v1);
 uint32_t orc_lcg_step(uint32_t state);
