@@ -346,6 +346,10 @@ typedef struct rtx_scene_info {
     uint32_t maxDepth;
     uint64_t deviceBytes;   /* nodes + triangle records + shading data */
     double buildMs;         /* host build time */
+    /* SceneLoader::calculateSceneSize (src/SceneLoader.cpp:350-367): every
+     * instance's model vertices transformed, and the spheres' boxes */
+    float boundsMin[3];
+    float boundsMax[3];
 } rtx_scene_info;
 
 /* Launch-shape knobs of the wavefront (no counterpart in the reference,
@@ -451,6 +455,19 @@ int rtx_ic_read_jobs(rtx_ctx *ctx, rtx_ic_job *jobs, float *sub, uint32_t *subSi
  * shaders/raytrace.rgen:923-960).  n = 0 removes them. */
 int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n);
 int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32_t capacity, uint32_t *n);
+/* Replaces PathGuiding's constructor (src/PathGuiding.cpp:11-31): regions
+ * from splitting the scene box (grown by Aabb::addEpsilon) splitCount times
+ * along the largest axis, each with lightpmm's initial 8-lobe mixture; the
+ * context then owns and refits them.  parallax: useParallaxCompensation of
+ * the fit (lobe distances and targets). */
+int rtx_guiding_init(rtx_ctx *ctx, const rtx_aabb *sceneBox, uint32_t splitCount, int parallax);
+/* PathGuiding::update (src/PathGuiding.cpp:276-312) on the samples of the
+ * last frame rendered with pc.updateGuiding: per region the samples move to
+ * the region's centre (parallax), lightpmm's MAP-EM fits (first update) or
+ * refits (later) the mixture, and the lobe distances are updated.  Renders
+ * with pc.updateGuiding do this after every frame by themselves when the
+ * context owns the mixtures.  Component split/merge is not done. */
+int rtx_guiding_update(rtx_ctx *ctx);
 /* The guiding samples of the last frame rendered with pc.updateGuiding: the
  * reference's DirectionalData buffer (binding 18, src/SampleCollector.cpp:
  * 29-51), RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL slots per pixel in pixel order;

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 #include <vector>
@@ -78,11 +79,16 @@ struct rtx_ctx {
     size_t recCap = 0;
     rtx_directional_data *dirData = nullptr;
     size_t dirCap = 0, dirCount = 0;  // slots allocated / slots of the last recording render
-    // path guiding (rtx_guiding_set)
+    // path guiding (rtx_guiding_set / rtx_guiding_init)
     std::vector<void *> guideAllocs;
     DevGuiding guide{};
     std::vector<rtx_aabb> guideBoxes;
     std::vector<rtx_vmm_theta> guideVmms;
+    bool guideVmmsStale = false;      // the device mixtures moved on (updates)
+    // the update (rtx_guiding_init): per-region mixtures and work buffers
+    std::vector<void *> fitAllocs, fitWorkAllocs;
+    GuideFitBuffers fit{};
+    bool fitValid = false, fitParallax = true, fitFirst = true;
 };
 
 namespace {
@@ -415,6 +421,8 @@ int rtx_ctx_destroy(rtx_ctx *ctx) {
     freeAll(ctx->batchAllocs);
     freeAll(ctx->icAllocs);
     freeAll(ctx->guideAllocs);
+    freeAll(ctx->fitAllocs);
+    freeAll(ctx->fitWorkAllocs);
     freeAll(ctx->recAllocs);
     if (ctx->dirData) (void)hipFree(ctx->dirData);
     if (ctx->fcIC) (void)hipFree(ctx->fcIC);
@@ -554,6 +562,27 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
                            primRef.size() * sizeof(int2) + verts.size() * sizeof(rtx_vertex) +
                            idx.size() * 4 + (size_t)d->materialCount * sizeof(rtx_material);
     sc->info.buildMs = bvh.buildMs;
+    for (int a = 0; a < 3; ++a) {
+        sc->info.boundsMin[a] = std::numeric_limits<float>::infinity();
+        sc->info.boundsMax[a] = -std::numeric_limits<float>::infinity();
+    }
+    for (uint32_t i = 0; i < d->instanceCount; ++i) {
+        const rtx_instance &in = d->instances[i];
+        const rtx_model_desc &md = d->models[in.modelIndex];
+        for (uint32_t v = 0; v < md.vertexCount; ++v) {
+            float w[3];
+            xformPoint(in.transform, md.vertices[v].pos, w);
+            for (int a = 0; a < 3; ++a) {
+                sc->info.boundsMin[a] = std::min(sc->info.boundsMin[a], w[a]);
+                sc->info.boundsMax[a] = std::max(sc->info.boundsMax[a], w[a]);
+            }
+        }
+    }
+    for (uint32_t i = 0; i < d->sphereCount; ++i)
+        for (int a = 0; a < 3; ++a) {
+            sc->info.boundsMin[a] = std::min(sc->info.boundsMin[a], d->spheres[i].center[a] - d->spheres[i].radius);
+            sc->info.boundsMax[a] = std::max(sc->info.boundsMax[a], d->spheres[i].center[a] + d->spheres[i].radius);
+        }
     *out = sc;
     return RTX_OK;
 }
@@ -574,6 +603,8 @@ int rtx_scene_get_info(const rtx_scene *scene, rtx_scene_info *info) {
     *info = scene->info;
     return RTX_OK;
 }
+
+static int guideUpdateAsync(rtx_ctx *ctx);
 
 int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, rtx_frame_stats *stats) {
     if (!ctx) return RTX_E_INVALID;
@@ -794,6 +825,13 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             CHECK_HIP(ctx, hipGetLastError());
             batches++;
         }
+        // PathGuiding::update after every recording frame when the context owns the mixtures
+        // (src/RayTracingApp.cpp:143-151)
+        if (record && ctx->fitValid) {
+            int rcu = 0;
+            timed(kClsShade, [&] { rcu = guideUpdateAsync(ctx); });
+            if (rcu != RTX_OK) return rcu;
+        }
         if (!useIC) continue;
         // the frame's cache upkeep
         const uint64_t sample = f0 / npix + p->sampleOffset;
@@ -884,22 +922,14 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
 }
 
 // ---------------------------------------------------------- path guiding
-int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) {
-    if (!ctx) return RTX_E_INVALID;
-    if (n && (!regions || !vmms)) return fail(ctx, RTX_E_INVALID, "NULL argument");
-    if (n > (1u << 20)) return fail(ctx, RTX_E_INVALID, "more than 2^20 guiding regions");
-    for (uint32_t i = 0; i < n; ++i) {
-        if (vmms[i].usedDistributions < 1 || vmms[i].usedDistributions > RTX_MAX_DISTRIBUTIONS)
-            return fail(ctx, RTX_E_INVALID, "usedDistributions must be in [1, 16]");
-        for (int a = 0; a < 3; ++a)
-            if (!(regions[i].min[a] <= regions[i].max[a])) return fail(ctx, RTX_E_INVALID, "empty guiding region box");
-    }
+static int guideUpload(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) {
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     freeAll(ctx->guideAllocs);
     ctx->guide = DevGuiding{};
     ctx->guideBoxes.assign(regions, regions + n);
     ctx->guideVmms.assign(vmms, vmms + n);
+    ctx->guideVmmsStale = false;
     if (n == 0) return RTX_OK;
     std::vector<GuideNode> nodes = buildGuideNodes(regions, n);
     rtx_aabb *dBoxes = nullptr;
@@ -923,13 +953,150 @@ int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *
     return RTX_OK;
 }
 
+int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) {
+    if (!ctx) return RTX_E_INVALID;
+    if (n && (!regions || !vmms)) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    if (n > (1u << 20)) return fail(ctx, RTX_E_INVALID, "more than 2^20 guiding regions");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (vmms[i].usedDistributions < 1 || vmms[i].usedDistributions > RTX_MAX_DISTRIBUTIONS)
+            return fail(ctx, RTX_E_INVALID, "usedDistributions must be in [1, 16]");
+        for (int a = 0; a < 3; ++a)
+            if (!(regions[i].min[a] <= regions[i].max[a])) return fail(ctx, RTX_E_INVALID, "empty guiding region box");
+    }
+    // hand-set mixtures replace any fitted ones
+    freeAll(ctx->fitAllocs);
+    freeAll(ctx->fitWorkAllocs);
+    ctx->fit = GuideFitBuffers{};
+    ctx->fitValid = false;
+    return guideUpload(ctx, regions, vmms, n);
+}
+
 int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32_t capacity, uint32_t *n) {
     if (!ctx) return RTX_E_INVALID;
     const uint32_t count = (uint32_t)ctx->guideBoxes.size();
     if (n) *n = count;
+    if (ctx->guideVmmsStale && count) {
+        CHECK_HIP(ctx, hipSetDevice(ctx->device));
+        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        CHECK_HIP(ctx, hipMemcpy(ctx->guideVmms.data(), ctx->guide.vmms, count * sizeof(rtx_vmm_theta),
+                                 hipMemcpyDeviceToHost));
+        ctx->guideVmmsStale = false;
+    }
     const uint32_t k = std::min(capacity, count);
     if (regions) std::copy(ctx->guideBoxes.begin(), ctx->guideBoxes.begin() + k, regions);
     if (vmms) std::copy(ctx->guideVmms.begin(), ctx->guideVmms.begin() + k, vmms);
+    return RTX_OK;
+}
+
+// PathGuiding's constructor (src/PathGuiding.cpp:11-31, createRegions 82-106):
+// the scene box grown by Aabb::addEpsilon, split splitCount times along the
+// largest axis of every box; each region starts with lightpmm's initial
+// mixture (8 lobes, kappa 5, spherical-Fibonacci directions).
+int rtx_guiding_init(rtx_ctx *ctx, const rtx_aabb *sceneBox, uint32_t splitCount, int parallax) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!sceneBox) return fail(ctx, RTX_E_INVALID, "NULL scene box");
+    if (splitCount > 16) return fail(ctx, RTX_E_INVALID, "splitCount must be at most 16");
+    rtx_aabb e;
+    for (int c = 0; c < 3; ++c) {
+        if (!(sceneBox->min[c] <= sceneBox->max[c])) return fail(ctx, RTX_E_INVALID, "empty scene box");
+        float ext = sceneBox->max[c] - sceneBox->min[c], ctr = sceneBox->min[c] + 0.5f * ext;
+        e.min[c] = ctr - 0.50001f * ext;
+        e.max[c] = ctr + 0.50001f * ext;
+    }
+    std::vector<rtx_aabb> boxes{e};
+    for (uint32_t it = 0; it < splitCount; ++it) {
+        std::vector<rtx_aabb> next;
+        next.reserve(boxes.size() * 2);
+        for (const rtx_aabb &x : boxes) {
+            float size[3] = {x.max[0] - x.min[0], x.max[1] - x.min[1], x.max[2] - x.min[2]};
+            int axis = size[0] > size[1] ? (size[0] > size[2] ? 0 : 2) : (size[1] > size[2] ? 1 : 2);
+            rtx_aabb l = x, r = x;
+            l.max[axis] -= 0.5f * size[axis];
+            r.min[axis] += 0.5f * size[axis];
+            next.push_back(l);
+            next.push_back(r);
+        }
+        boxes.swap(next);
+    }
+    const uint32_t n = (uint32_t)boxes.size();
+    std::vector<rtx_vmm_theta> placeholder(n);
+    for (auto &v : placeholder) v.usedDistributions = 1;
+    int rc = rtx_guiding_set(ctx, boxes.data(), placeholder.data(), n);
+    if (rc != RTX_OK) return rc;
+    GuideFitBuffers &g = ctx->fit;
+    g = GuideFitBuffers{};
+    g.numRegions = n;
+    g.boxes = ctx->guide.boxes;
+    hipError_t err = devAlloc(&g.regions, n, ctx->fitAllocs);
+    if (err == hipSuccess) err = devAlloc(&g.regionStart, n + 1, ctx->fitAllocs);
+    if (err == hipSuccess) err = devAlloc(&g.counters, 4, ctx->fitAllocs);
+    if (err != hipSuccess) {
+        freeAll(ctx->fitAllocs);
+        return fail(ctx, RTX_E_DEVICE, std::string("guiding state: ") + hipGetErrorString(err));
+    }
+    ctx->fitParallax = parallax != 0;
+    ctx->fitFirst = true;
+    ctx->fitValid = true;
+    launchGuideInit(ctx->stream, g, 8, 5.0f);
+    launchGuideToVmm(ctx->stream, g, ctx->fitParallax, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
+    CHECK_HIP(ctx, hipGetLastError());
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->guideVmmsStale = true;
+    return RTX_OK;
+}
+
+static int guideFitWork(rtx_ctx *ctx, size_t slots) {
+    GuideFitBuffers &g = ctx->fit;
+    if (g.slotCap >= slots) return RTX_OK;
+    freeAll(ctx->fitWorkAllocs);
+    auto &L = ctx->fitWorkAllocs;
+    const uint32_t nblk = (uint32_t)((slots + 1023) / 1024);
+    CHECK_HIP(ctx, devAlloc(&g.keysA, slots, L));
+    CHECK_HIP(ctx, devAlloc(&g.keysB, slots, L));
+    CHECK_HIP(ctx, devAlloc(&g.valsA, slots, L));
+    CHECK_HIP(ctx, devAlloc(&g.valsB, slots, L));
+    CHECK_HIP(ctx, devAlloc(&g.samples, 2 * slots, L));
+    CHECK_HIP(ctx, devAlloc(&g.blockSums, nblk, L));
+    CHECK_HIP(ctx, devAlloc(&g.hist, (size_t)nblk * 64, L));
+    g.slotCap = (uint32_t)slots;
+    g.blockCap = nblk;
+    return RTX_OK;
+}
+
+static EmProps guideProps() { // src/PathGuiding.cpp:33-40 over VMMFactory.h:45-61
+    EmProps P;
+    P.minItr = 1;
+    P.maxItr = 100;
+    P.relLogLikelihoodThreshold = 0.005f;
+    P.maxKappa = 50000.0f;
+    P.vPrior = 0.01f;
+    P.rPrior = 0.0f;
+    P.rPriorWeight = 1.0f;
+    return P;
+}
+
+// PathGuiding::update (src/PathGuiding.cpp:276-312) on the samples the last
+// recording frame left (stream-ordered, no host wait).
+static int guideUpdateAsync(rtx_ctx *ctx) {
+    if (ctx->dirCount > (size_t)UINT32_MAX) return fail(ctx, RTX_E_INVALID, "too many guiding samples");
+    int rc = guideFitWork(ctx, ctx->dirCount);
+    if (rc != RTX_OK) return rc;
+    launchGuideUpdate(ctx->stream, ctx->fit, ctx->dirData, (uint32_t)ctx->dirCount, guideProps(), ctx->fitParallax,
+                      ctx->fitFirst, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
+    CHECK_HIP(ctx, hipGetLastError());
+    ctx->fitFirst = false;
+    ctx->guideVmmsStale = true;
+    return RTX_OK;
+}
+
+int rtx_guiding_update(rtx_ctx *ctx) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->fitValid) return fail(ctx, RTX_E_STATE, "no guiding state to update (rtx_guiding_init)");
+    if (ctx->dirCount == 0) return fail(ctx, RTX_E_STATE, "no render with updateGuiding yet");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = guideUpdateAsync(ctx);
+    if (rc != RTX_OK) return rc;
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RTX_OK;
 }
 

@@ -96,6 +96,48 @@ struct DevGuiding {
 // of the centres); api.cpp.
 std::vector<GuideNode> buildGuideNodes(const rtx_aabb *boxes, uint32_t n);
 
+// One guiding region's mixture as lightpmm keeps it (ParametricMixtureModel
+// of VMFKernels, external/lightpmm/include/pmm/ParametricMixtureModel.h:56-73)
+// plus PathGuiding's per-region extras (parallax mean, harmonic-mean lobe
+// distances; src/PathGuiding.h:77-84).
+struct GuideRegion {
+    int32_t K;
+    uint32_t numEMIterations;
+    float sampleWeight, numSamples;
+    uint64_t totalNumSamples;
+    float w[RTX_MAX_DISTRIBUTIONS], kappa[RTX_MAX_DISTRIBUTIONS], r[RTX_MAX_DISTRIBUTIONS];
+    float norm[RTX_MAX_DISTRIBUTIONS], e2k[RTX_MAX_DISTRIBUTIONS];
+    float mu[RTX_MAX_DISTRIBUTIONS][3];
+    float dist[RTX_MAX_DISTRIBUTIONS], distW[RTX_MAX_DISTRIBUTIONS];
+    float mean[3], lastMean[3];
+};
+
+// lightpmm factory properties (VMMFactory.h:45-61 as PathGuiding sets them)
+struct EmProps {
+    uint32_t minItr, maxItr;
+    float relLogLikelihoodThreshold;
+    float maxKappa, vPrior, rPrior, rPriorWeight;
+};
+
+// Work buffers of one guiding update (device).
+struct GuideFitBuffers {
+    GuideRegion *regions;
+    const rtx_aabb *boxes;
+    uint32_t numRegions;
+    uint32_t *keysA, *keysB, *valsA, *valsB; // region / slot pairs while sorting
+    uint32_t *blockSums, *hist;
+    uint32_t *regionStart;                   // numRegions + 1
+    uint32_t *counters;                      // [0] valid samples
+    float4 *samples;                         // 2 per sample: (direction, weight), (distance, -, -, -)
+    uint32_t slotCap, blockCap;
+};
+
+void launchGuideInit(hipStream_t s, GuideFitBuffers g, uint32_t numInitialComponents, float initKappa);
+// PathGuiding::update on the recorded samples (slots = pixels * 16)
+void launchGuideUpdate(hipStream_t s, GuideFitBuffers g, const rtx_directional_data *dd, uint32_t slots,
+                       EmProps props, bool parallax, bool firstFit, rtx_vmm_theta *vmmsOut);
+void launchGuideToVmm(hipStream_t s, GuideFitBuffers g, bool parallax, rtx_vmm_theta *vmmsOut);
+
 // ------------------------------------------------------ irradiance cache
 struct IcItem {              // one (record, grid cell) pair of the lookup index
     uint32_t entry;

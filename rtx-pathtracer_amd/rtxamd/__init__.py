@@ -100,6 +100,8 @@ def device_lib():
         L.rtx_guiding_get.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32,
                                       C.POINTER(C.c_uint32)]
         L.rtx_guiding_read_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.rtx_guiding_init.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.c_uint32, C.c_int]
+        L.rtx_guiding_update.argtypes = [C.c_void_p]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         L.rtx_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         _dev = L
@@ -397,6 +399,10 @@ class IcState:
 
 
 class DeviceScene:
+    def bounds(self):
+        i = self.info()
+        return np.array([i.boundsMin[:], i.boundsMax[:]], np.float32)
+
     def __init__(self, ctx, handle):
         self.ctx = ctx
         self._h = handle
@@ -461,6 +467,25 @@ class Renderer:
         """regions: (n, 2, 3) boxes; vmms: sequence of abi.VmmTheta."""
         boxes, arr = guiding_arrays(regions, vmms)
         self._check(device_lib().rtx_guiding_set(self._h, boxes, arr, len(vmms)))
+
+    def guiding_init(self, scene_box, splits, parallax=True):
+        box = abi.Aabb()
+        box.min[:] = [float(x) for x in scene_box[0]]
+        box.max[:] = [float(x) for x in scene_box[1]]
+        self._check(device_lib().rtx_guiding_init(self._h, C.byref(box), int(splits), int(parallax)))
+
+    def guiding_update(self):
+        self._check(device_lib().rtx_guiding_update(self._h))
+
+    def guiding_get(self):
+        """(boxes (n, 2, 3), [abi.VmmTheta]) of the context."""
+        n = C.c_uint32()
+        self._check(device_lib().rtx_guiding_get(self._h, None, None, 0, C.byref(n)))
+        boxes = (abi.Aabb * max(1, n.value))()
+        vmms = (abi.VmmTheta * max(1, n.value))()
+        self._check(device_lib().rtx_guiding_get(self._h, boxes, vmms, n.value, C.byref(n)))
+        arr = np.array([[boxes[i].min[:], boxes[i].max[:]] for i in range(n.value)], np.float32)
+        return arr, [vmms[i] for i in range(n.value)]
 
     def guiding_read_samples(self, width, height):
         """The last recording frame's samples: (width*height, 16) DIRECTIONAL_DTYPE."""

@@ -175,7 +175,8 @@ class Tuning(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("triangles", C.c_uint32), ("spheres", C.c_uint32), ("nodes", C.c_uint32),
-                ("maxDepth", C.c_uint32), ("deviceBytes", C.c_uint64), ("buildMs", C.c_double)]
+                ("maxDepth", C.c_uint32), ("deviceBytes", C.c_uint64), ("buildMs", C.c_double),
+                ("boundsMin", F3), ("boundsMax", F3)]
 
 
 class Ray(C.Structure):

@@ -125,3 +125,99 @@ def test_sample_records_match_oracle(renderer, guided):
     assert np.mean(rel <= 1e-4) >= 0.99
     renderer.guiding_set([], [])
     ds.close()
+
+
+def mixture_pdf_grid(vmm, dirs, x):
+    """The mixture pdf (guiding.glsl VMM, with parallax) at each direction."""
+    K = vmm.usedDistributions
+    out = np.zeros(len(dirs))
+    for k in range(K):
+        t = vmm.thetas[k]
+        if t.k == 0:
+            out += vmm.pi[k] / (4 * np.pi)
+            continue
+        mu = np.array(t.mu[:], np.float64)
+        if t.distance > 0:
+            mu = np.array(t.target[:]) - x
+            mu /= np.linalg.norm(mu)
+        out += vmm.pi[k] * t.norm * np.exp(t.k * (dirs @ mu - 1))
+    return out
+
+
+def fib_sphere(n):
+    i = np.arange(n) + 0.5
+    z = 1 - 2 * i / n
+    phi = np.pi * (1 + 5 ** 0.5) * i
+    r = np.sqrt(1 - z * z)
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+
+
+def test_guiding_update_matches_oracle_em(renderer):
+    """Two training frames: the device's region mixtures after each update
+    against the oracle's MAP-EM on the same recorded samples (first update:
+    fit, second: updateFit).  Compared as densities over the sphere, from
+    each region's mean position."""
+    from oracle.binding import OracleGuide
+    hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
+    ds = renderer.upload(hs)
+    box = ds.bounds()
+    renderer.guiding_init(box, 3, parallax=True)
+    og = OracleGuide(abi, box, 3, parallax=True, prec=64)
+    o32 = OracleGuide(abi, box, 3, parallax=True, prec=32)
+    w = h = 128
+    dirs = fib_sphere(4000)
+    for frame in range(2):
+        fp = scenes.params(hs, w, h, 1, maxDepth=8, enableMIS=1, updateGuiding=1, useGuiding=frame,
+                           guidingProb=0.5, sample_offset=frame)
+        renderer.render(ds, fp)  # records the frame, then updates the mixtures
+        dd = renderer.guiding_read_samples(w, h)
+        og.update(dd)
+        o32.update(dd)
+        boxes, gv = renderer.guiding_get()
+        _, ov = og.regions()
+        _, fv = o32.regions()
+        errs, errs32 = [], []
+        for r in range(len(gv)):
+            if (dd["flags"] == r).sum() < 50:
+                continue
+            x = np.array(ov[r].meanPosition[:], np.float64)
+            pg, po, pf = (mixture_pdf_grid(v, dirs, x) for v in (gv[r], ov[r], fv[r]))
+            errs.append(np.abs(pg - po).mean() / po.mean())
+            errs32.append(np.abs(pf - po).mean() / po.mean())
+            assert gv[r].usedDistributions == ov[r].usedDistributions
+            assert np.allclose(gv[r].meanPosition[:], ov[r].meanPosition[:], atol=1e-6)
+        errs, errs32 = np.array(errs), np.array(errs32)
+        log(f"update {frame}: {len(errs)} regions; density L1 vs FP64 oracle: device median {np.median(errs):.2e} "
+            f"max {errs.max():.2e}; FP32 oracle median {np.median(errs32):.2e} max {errs32.max():.2e}")
+        assert np.median(errs) <= max(1e-3, 2 * np.median(errs32))
+        assert errs.max() <= max(0.05, 2 * errs32.max())
+    renderer.guiding_set([], [])
+    ds.close()
+
+
+def test_trained_guiding_is_unbiased_and_reduces_variance(renderer):
+    """After training frames the guided estimator keeps the plain mean and,
+    on the plates-and-lights scene, its per-pixel variance drops."""
+    hs = rtxamd.HostScene.generate(abi.SCENE_PLATES)
+    ds = renderer.upload(hs)
+    renderer.guiding_init(ds.bounds(), 6, parallax=True)
+    w, h = 160, 90
+    base = dict(maxDepth=8, enableMIS=1)
+    for f in range(4):
+        renderer.render(ds, scenes.params(hs, w, h, 1, updateGuiding=1, useGuiding=int(f > 0), guidingProb=0.5,
+                                          sample_offset=1000 + f, **base))
+    n = w * h * 32
+    keep = abi.RTX_FRAME_KEEP_PATHS | abi.RTX_FRAME_RESET_ACCUM
+    renderer.render(ds, scenes.params(hs, w, h, 32, seed=11, flags=keep, **base))
+    a, _ = renderer.read_paths(n)
+    renderer.render(ds, scenes.params(hs, w, h, 32, seed=12, flags=keep, useGuiding=1, guidingProb=0.5, **base))
+    b, _ = renderer.read_paths(n)
+    la, lb = a.mean(1).astype(np.float64), b.mean(1).astype(np.float64)
+    se = np.sqrt(la.var() / n + lb.var() / n)
+    va = la.reshape(32, -1).var(0).mean()
+    vb = lb.reshape(32, -1).var(0).mean()
+    log(f"plates: plain mean {la.mean():.5f} guided {lb.mean():.5f} (se {se:.1e}); per-pixel variance plain "
+        f"{va:.3e} guided {vb:.3e}")
+    assert abs(la.mean() - lb.mean()) < 4 * se
+    renderer.guiding_set([], [])
+    ds.close()

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 W = H = 64
 
 
-def cornell_state(orc, hs, frames=4, max_entries=400):
+def cornell_state(orc, hs, frames=4, max_entries=3000):
     """A cache built by the oracle's own frames on the Cornell scene."""
     st = rtxamd.IcState.empty(max_entries)
     for f in range(frames):
@@ -98,7 +98,10 @@ def test_frame_upkeep_matches_oracle(renderer, cornell):
     # accumulation + commit of the device's own sub-path results
     ref = st0.copy()
     ic_apply(abi, fp, ref, gjobs, gsub, prec=32)
-    assert ref.count == gstate.count and ref.hdr.nextUpdateSlot == gstate.hdr.nextUpdateSlot
+    assert ref.count == gstate.count
+    # the plan's slot counter (not part of the commit) follows the oracle's frame
+    assert gstate.hdr.nextUpdateSlot == ostate.hdr.nextUpdateSlot
+    assert sum(1 for j in gjobs if j[0] == 1) > 10 and nu > 0  # both kinds of job ran
     c = gstate.count
     assert np.array_equal(ref.entries["numUpdates"][:c], gstate.entries["numUpdates"][:c])
     for f in ("color", "rotGrad", "transGrad"):
@@ -131,29 +134,41 @@ def test_cache_render_is_deterministic(renderer):
     ds.close()
 
 
-def test_cache_image_close_to_plain(renderer):
-    """Prepare frames, then 64 spp with the cache against 64 spp plain: the
-    cache's bias stays small and paths get shorter."""
+def test_cache_render_matches_oracle_sequence(renderer):
+    """Prepare frames and cache frames, device against the oracle running the
+    same frames on the same streams: the images agree in the mean; the cache
+    darkens the image against plain path tracing (its records hold
+    irradiance of depth-1 sub-paths, calculateCacheData, raytrace.rgen:1260)
+    and shortens paths."""
     hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
+    orc = OracleScene(abi, hs.desc())
     ds = renderer.upload(hs)
-    w, h = 128, 128
-    base = dict(maxDepth=8, enableMIS=1)
-    plain = renderer.render(ds, scenes.params(hs, w, h, 256, seed=5, **base))
-    ref = renderer.read_image(w, h)[..., :3]
-    renderer.ic_create(10000)
-    icp = dict(useIrradianceCache=1, irradianceCreateProb=0.0005, irradianceUpdateProb=0.00001, **base)
-    renderer.render(ds, scenes.params(hs, w, h, 30, isIrradiancePrepareFrame=1, **icp))
-    n = renderer.ic_read().count
-    cst = renderer.render(ds, scenes.params(hs, w, h, 64, seed=9, **icp))
-    img = renderer.read_image(w, h)[..., :3]
-    bias = img.mean() / ref.mean() - 1.0
-    e = relmse(img, ref)
-    log(f"cache: {n} records after prepare frames; mean {img.mean():.4f} vs plain {ref.mean():.4f} "
-        f"(bias {bias:+.3%}), relmse {e:.3e}; extend rays/path {cst.extendRays / cst.paths:.2f} vs "
-        f"{plain.extendRays / plain.paths:.2f}")
-    assert n > 100
-    assert abs(bias) < 0.05
-    assert e < 0.05
+    w = h = 48
+    icp = dict(maxDepth=8, enableMIS=1, useIrradianceCache=1, irradianceCreateProb=0.01,
+               irradianceUpdateProb=0.002)
+    renderer.ic_create(4000)
+    st = rtxamd.IcState.empty(4000)
+    prep, frames = 12, 8
+    renderer.render(ds, scenes.params(hs, w, h, prep, isIrradiancePrepareFrame=1, **icp))
+    cst = renderer.render(ds, scenes.params(hs, w, h, frames, sample_offset=prep, **icp))
+    img = renderer.read_image(w, h)[..., :3].astype(np.float64)
+    acc = np.zeros((w * h, 3))
+    for f in range(prep + frames):
+        rgb, _, _, _, _ = orc.ic_frame(scenes.params(hs, w, h, 1, sample_offset=f, **icp), st, prec=64)
+        if f >= prep:
+            acc += rgb
+    oimg = acc / frames
+    dev = renderer.ic_read()
+    plain = renderer.render(ds, scenes.params(hs, w, h, 256, maxDepth=8, enableMIS=1,
+                                              flags=abi.RTX_FRAME_RESET_ACCUM))
+    pimg = renderer.read_image(w, h)[..., :3].astype(np.float64)
+    se = np.sqrt(oimg.reshape(-1).var() / oimg.size)
+    log(f"cache sequence: records device {dev.count} oracle {st.count}; mean device {img.mean():.4f} "
+        f"oracle {oimg.mean():.4f} plain {pimg.mean():.4f}; extend rays/path {cst.extendRays / cst.paths:.2f} "
+        f"vs plain {plain.extendRays / plain.paths:.2f}")
+    assert abs(dev.count - st.count) <= max(3, st.count // 50)
+    assert abs(img.mean() - oimg.mean()) < 0.02 * oimg.mean() + 4 * se
+    assert -0.25 < img.mean() / pimg.mean() - 1.0 < 0.02
     assert cst.extendRays / cst.paths < plain.extendRays / plain.paths
     ds.close()
 
