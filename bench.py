@@ -1,16 +1,25 @@
-"""Benchmark: Msamples/s of the plain path tracer at 1920x1080, depth 12.
+"""Benchmark: Msamples/s at 1920x1080, depth 12, on one MI355X.
 
-Workload (one step = one full image): the seeded procedural room of 262,144
-triangles (boxes and 24-sided columns in a 20x10x20 m volume, 50% Lambert /
-30% rough conductor / 20% dielectric, 4 area lights), 1920x1080 at 64 spp,
-maxDepth 12, NEE + MIS (power heuristic), FP32, one MI355X.
+Workload: the seeded procedural room of 262,144 triangles (boxes and
+24-sided columns in a 20x10x20 m volume, 50% Lambert / 30% rough conductor /
+20% dielectric, 4 area lights), 1920x1080 at 64 spp, maxDepth 12, NEE + MIS
+(power heuristic), irradiance cache of 10,000 records, FP32.
+
+One step = one full image as the reference produces it with the cache on:
+the cache is emptied, 50 prepare frames (1 spp each, cache upkeep only,
+src/RayTracingApp.cpp:1131-1145 with irradianceCachePrepareFrames = 50) build
+it, then 64 samples per pixel are rendered with the cache (creation and
+update probabilities at the reference's defaults).  `value` counts the 64
+spp only; the prepare frames are inside the timed step.  --method plain
+measures the plain path tracer instead (no cache); its figure is reported
+beside the cache's in every run.
 
 Prints one JSON line.  `roofline` prices the dominant kernel (closest-hit
-traversal) by algorithmic bytes per ray = 32 (ray) + 16 (hit) + nodes x 64 B
-+ triangles x 48 B, with node/triangle fetch counts from a counting pass of
-the identical (deterministic) workload and kernel times from HIP events on
-the library's stream.  `cpu_baseline` times the FP64 CPU restatement of the
-same logic on a pixel subsample of the same workload.
+traversal of extend rays) by algorithmic bytes per ray = 32 (ray) + 16 (hit)
++ nodes x 64 B + triangles x 48 B, with node/triangle fetch counts from a
+counting pass of the identical (deterministic) workload and kernel times
+from HIP events on the library's stream.  `cpu_baseline` times the FP64 CPU
+restatement of the plain method on a pixel subsample of one sample plane.
 """
 import argparse
 import ctypes as C
@@ -42,12 +51,16 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--triangles", type=int, default=262144)
+    ap.add_argument("--method", choices=["ic", "plain"], default="ic")
+    ap.add_argument("--ic-size", type=int, default=10000)
+    ap.add_argument("--prepare-frames", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traversal", type=int, default=0, help="1 persistent lane-refill, 2 grid-stride (0: default)")
     ap.add_argument("--trace-blocks", type=int, default=0, help="traversal blocks per CU (0: default)")
     ap.add_argument("--shade-blocks", type=int, default=0, help="shading blocks per CU (0: default)")
     ap.add_argument("--no-count", action="store_true", help="skip the node/triangle counting pass")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-method comparison step")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 byte counters)")
     a = ap.parse_args()
@@ -83,46 +96,63 @@ def main():
         f"build {info.buildMs:.0f} ms, upload {time.time() - t0:.2f} s")
     cam = hs.camera(W, H)
     pc = dict(maxDepth=12, enableMIS=1, enableNEE=1, usePowerHeuristic=1)
+    icpc = dict(pc, useIrradianceCache=1)  # reference defaults: create 0.0005, update 0.00001
+    if a.method == "ic":
+        r.ic_create(a.ic_size)
 
-    def params(flags, n_spp=spp):
-        return rtxamd.frame_params(cam, W, H, n_spp, seed=1, flags=flags | abi.RTX_FRAME_RESET_ACCUM, **pc)
+    def params(flags, n_spp=spp, **kw):
+        return rtxamd.frame_params(cam, W, H, n_spp, seed=1, flags=flags | abi.RTX_FRAME_RESET_ACCUM, **kw)
+
+    def step(flags, method):
+        """One image; returns the frame stats of its renders."""
+        if method == "plain":
+            return [r.render(ds, params(flags, **pc))]
+        r.ic_reset()
+        prep = r.render(ds, params(flags, a.prepare_frames, isIrradiancePrepareFrame=1, **icpc))
+        img = r.render(ds, rtxamd.frame_params(cam, W, H, spp, seed=1, sample_offset=a.prepare_frames,
+                                               flags=flags | abi.RTX_FRAME_RESET_ACCUM, **icpc))
+        return [prep, img]
 
     for _ in range(a.warmup):
-        r.render(ds, params(0))
+        step(0, a.method)
     log("warmup done")
     # timed region: K full images; each render ends with a stream synchronize
-    totals = {"ms": 0.0, "ext": 0.0, "sh": 0.0, "pr": 0.0, "shade": 0.0}
-    counts = None
+    keys = ("msTotal", "msExtend", "msShadow", "msProbe", "msShade")
+    totals = dict.fromkeys(keys, 0.0)
+    launches = [0, 0, 0, 0]
     t_start = time.perf_counter()
     for k in range(a.steps):
-        st = r.render(ds, params(abi.RTX_FRAME_TIME_KERNELS))
-        totals["ms"] += st.msTotal
-        totals["ext"] += st.msExtend
-        totals["sh"] += st.msShadow
-        totals["pr"] += st.msProbe
-        totals["shade"] += st.msShade
-        counts = st
-        log(f"step {k}: {st.msTotal:.1f} ms device, {st.paths / st.msTotal / 1e3:.2f} Msamples/s")
+        sts = step(abi.RTX_FRAME_TIME_KERNELS, a.method)
+        for st in sts:
+            for key in keys:
+                totals[key] += getattr(st, key)
+            for c in range(4):
+                launches[c] += st.launchesByKind[c]
+        log(f"step {k}: {sum(st.msTotal for st in sts):.1f} ms device")
     wall = time.perf_counter() - t_start
     ms_per_step = wall * 1e3 / a.steps
     samples = W * H * spp
     value = samples / (ms_per_step * 1e3)
 
     # counting pass (same deterministic workload) for the algorithmic bytes
-    cst = r.render(ds, params(abi.RTX_FRAME_COUNT_VISITS)) if not a.no_count else counts
-    ext_rays = cst.extendRays
-    ext_bytes = (ext_rays * (RAY_BYTES + HIT_BYTES) + cst.nodeVisitsByKind[0] * NODE_BYTES +
-                 cst.primVisitsByKind[0] * TRI_BYTES)
-    all_rays = cst.extendRays + cst.shadowRays + cst.probeRays
-    all_bytes = (all_rays * (RAY_BYTES + HIT_BYTES) + sum(cst.nodeVisitsByKind) * NODE_BYTES +
-                 sum(cst.primVisitsByKind) * TRI_BYTES)
-    launches_ext = counts.launchesByKind[0]
-    ext_ms_step = totals["ext"] / a.steps
+    if a.no_count:
+        csts = sts
+    else:
+        csts = step(abi.RTX_FRAME_COUNT_VISITS, a.method)
+    cnt = {f: sum(getattr(st, f) for st in csts) for f in ("extendRays", "shadowRays", "probeRays")}
+    nodes = [sum(st.nodeVisitsByKind[k] for st in csts) for k in range(3)]
+    prims = [sum(st.primVisitsByKind[k] for st in csts) for k in range(3)]
+    ext_rays = cnt["extendRays"]
+    ext_bytes = ext_rays * (RAY_BYTES + HIT_BYTES) + nodes[0] * NODE_BYTES + prims[0] * TRI_BYTES
+    all_rays = cnt["extendRays"] + cnt["shadowRays"] + cnt["probeRays"]
+    all_bytes = all_rays * (RAY_BYTES + HIT_BYTES) + sum(nodes) * NODE_BYTES + sum(prims) * TRI_BYTES
+    launches_ext = launches[0] // a.steps
+    ext_ms_step = totals["msExtend"] / a.steps
     avg_launch_ms = ext_ms_step / max(1, launches_ext)
-    achieved = ext_bytes / launches_ext / (avg_launch_ms * 1e-3) / 1e9
-    trav_ms = (totals["ext"] + totals["sh"] + totals["pr"]) / a.steps
+    achieved = ext_bytes / max(1, launches_ext) / (avg_launch_ms * 1e-3) / 1e9
+    trav_ms = (totals["msExtend"] + totals["msShadow"] + totals["msProbe"]) / a.steps
     traffic = None
-    tfile = ROOT / "profiles" / "extend_traffic_room262k_1080p.json"
+    tfile = ROOT / "profiles" / f"extend_traffic_room262k_1080p_{a.method}.json"
     if tfile.exists():
         try:
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
@@ -134,11 +164,21 @@ def main():
         "kernel": ext_kernel,
         "algorithmic_bytes_per_launch": int(ext_bytes / max(1, launches_ext)),
         "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_step": launches_ext,
-        "nodes_per_ray": round(cst.nodeVisitsByKind[0] / max(1, ext_rays), 2),
-        "tris_per_ray": round(cst.primVisitsByKind[0] / max(1, ext_rays), 2),
+        "nodes_per_ray": round(nodes[0] / max(1, ext_rays), 2),
+        "tris_per_ray": round(prims[0] / max(1, ext_rays), 2),
         "all_traversal_frac": round(all_bytes / (trav_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
     mrays = all_rays / (ms_per_step * 1e3)
+
+    # the other method, one step, for comparison
+    other = "plain" if a.method == "ic" and not a.no_plain else None
+    other_value = None
+    if other:
+        step(0, other)
+        t1 = time.perf_counter()
+        step(0, other)
+        other_value = samples / ((time.perf_counter() - t1) * 1e6)
+        log(f"plain method: {other_value:.2f} Msamples/s")
 
     # FP32 device vs FP64 oracle on one sample plane, pixel subsample
     cpu = None
@@ -147,7 +187,7 @@ def main():
         from oracle.binding import OracleScene, DEFAULT_THREADS
         orc = OracleScene(abi, hs.desc())
         fp1 = rtxamd.frame_params(cam, W, H, 1, seed=1,
-                                  flags=abi.RTX_FRAME_RESET_ACCUM | abi.RTX_FRAME_KEEP_PATHS, **pc)
+                                  flags=abi.RTX_FRAME_RESET_ACCUM | abi.RTX_FRAME_KEEP_PATHS, **pc)  # plain
         r.render(ds, fp1)
         rgb, sig = r.read_paths(W * H)
         # calibrate: time a small sample, then size the measured sample to the budget
@@ -163,8 +203,8 @@ def main():
         cpu_s = time.perf_counter() - t1
         cpu = {"value": round(len(ids) / cpu_s / 1e6, 5), "unit": "Msamples/s", "cores": DEFAULT_THREADS,
                "kind": "port",
-               "sample": f"{len(ids)} paths = every {stride}th pixel of sample 0 of the same workload, "
-                         f"FP64 restatement, {DEFAULT_THREADS} threads, {cpu_s:.1f} s"}
+               "sample": f"{len(ids)} paths = every {stride}th pixel of sample 0 of the same scene and camera, "
+                         f"plain method, FP64 restatement, {DEFAULT_THREADS} threads, {cpu_s:.1f} s"}
         def stats(x, xs, ref, refs):
             same = (xs == refs).all(1)
             rel = np.abs(x - ref).max(1) / np.maximum(np.abs(ref).max(1), 1e-6)
@@ -193,16 +233,23 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene, PCG32 seed 1)",
         "config": {
-            "workload": f"room {info.triangles} tris, {W}x{H}, {spp} spp, depth 12, NEE+MIS (plain method)",
+            "workload": (f"room {info.triangles} tris, {W}x{H}, {spp} spp, depth 12, NEE+MIS, " +
+                         (f"irradiance cache {a.ic_size} records ({a.prepare_frames} prepare frames in the step)"
+                          if a.method == "ic" else "plain method")),
             "triangles": info.triangles, "width": W, "height": H, "spp": spp, "max_depth": 12,
-            "nee": True, "mis": "power", "irradiance_cache": "off (not built yet)",
+            "nee": True, "mis": "power",
+            "irradiance_cache": ({"size": a.ic_size, "prepare_frames": a.prepare_frames,
+                                  "create_prob": 0.0005, "update_prob": 0.00001} if a.method == "ic" else None),
             "samples_per_step": samples,
         },
+        "plain_method_msamples_per_s": round(other_value, 3) if other_value else None,
         "mrays_per_s": round(mrays, 1),
-        "rays_per_step": {"extend": cst.extendRays, "shadow": cst.shadowRays, "probe": cst.probeRays},
-        "ms_device_per_step": {"total": round(totals["ms"] / a.steps, 1), "extend": round(totals["ext"] / a.steps, 1),
-                               "shadow": round(totals["sh"] / a.steps, 1), "probe": round(totals["pr"] / a.steps, 1),
-                               "shade_and_other": round(totals["shade"] / a.steps, 1)},
+        "rays_per_step": cnt,
+        "ms_device_per_step": {"total": round(totals["msTotal"] / a.steps, 1),
+                               "extend": round(totals["msExtend"] / a.steps, 1),
+                               "shadow": round(totals["msShadow"] / a.steps, 1),
+                               "probe": round(totals["msProbe"] / a.steps, 1),
+                               "shade_and_other": round(totals["msShade"] / a.steps, 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32_vs_fp64": parity,
