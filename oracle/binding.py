@@ -105,6 +105,11 @@ def lib(abi):
         L.orc_fastexp.restype = C.c_float
         L.orc_ic_calculate.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, C.c_uint32, d3,
                                        d3, d3, d3, d3, C.POINTER(C.c_double)]
+        L.orc_estimate_set.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
+        L.orc_weight_window.argtypes = [d3, d3, d3, C.c_double, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]
+        L.orc_weight_window.restype = C.c_double
+        L.orc_adrrs_counters.argtypes = [C.POINTER(C.c_uint64)]
+        L.orc_adrrs_counters.restype = None
         _lib = L
     return _lib
 
@@ -115,6 +120,22 @@ class IcJob(C.Structure):
 
 
 IC_STRATA = 200  # 20 phi x 10 theta strata of calculateCacheData
+
+
+def weight_window(abi, estimate, adjoint, throughput, adrrs_s, state):
+    """applyWeightWindow in FP64: (q, n, state after)."""
+    L = lib(abi)
+    st = C.c_uint32(state)
+    n = C.c_int(0)
+    q = L.orc_weight_window(d3(estimate), d3(adjoint), d3(throughput), adrrs_s, C.byref(st), C.byref(n))
+    return q, n.value, st.value
+
+
+def adrrs_counters(abi):
+    """(splits queued, roulette deaths, split paths traced) since the last call."""
+    out = (C.c_uint64 * 3)()
+    lib(abi).orc_adrrs_counters(out)
+    return tuple(int(x) for x in out)
 
 
 def ic_reduce(abi, seed, sample, job, normal, sub, prec=64):
@@ -220,6 +241,14 @@ class OracleScene:
         jl = [(jobs[i].type, jobs[i].target, np.array(jobs[i].origin[:]), np.array(jobs[i].normal[:]))
               for i in range(n)]
         return rgb, sig, jl, sub[:n * IC_STRATA].reshape(n, IC_STRATA, 4), subsig[:n * IC_STRATA].reshape(n, IC_STRATA, 2)
+
+    def estimate_set(self, rgba, width, height):
+        """The ADRRS pixel estimate (RGBA float32, width*height) of later frames."""
+        self._estimate = np.ascontiguousarray(rgba, np.float32).reshape(-1)
+        assert self._estimate.size == 4 * width * height
+        rc = self.L.orc_estimate_set(self._h, self._estimate.ctypes.data_as(C.POINTER(C.c_float)), width, height)
+        if rc != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
 
     def guiding_set(self, regions, vmms):
         from rtxamd import guiding_arrays

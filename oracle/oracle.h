@@ -30,12 +30,26 @@ void orc_scene_destroy(orc_scene *scene);
 const char *orc_last_error(void);
 
 /* Render the listed paths.  Path id = pixel + width*height*sample, where
- * sample counts from params->sampleOffset.  prec = 64 (oracle) or 32 (the
+ * sample counts from params->sampleOffset; one path is one frame of one
+ * pixel (main(), raytrace.rgen:1623-1711): pc.samplesPerPixel camera
+ * samples on one random stream, then the frame's queued splits, averaged.  prec = 64 (oracle) or 32 (the
  * FP32 instantiation).  rgb: 3 doubles per path; sig: 2 uint32 per path
  * (final LCG state, hash of hit primitive ids), may be NULL.
  * threads <= 0 uses all hardware threads. */
 int orc_render_paths(const orc_scene *scene, const rtx_frame_params *params, const uint64_t *ids,
                      size_t n, int prec, int threads, double *rgb, uint32_t *sig);
+
+/* ADRRS: the per-pixel estimate (RGBA, width*height) that frames with
+ * pc.useADRRS read (estimateImage, raytrace.rgen:1632-1636). */
+int orc_estimate_set(orc_scene *scene, const float *rgba, uint32_t width, uint32_t height);
+/* Counts since the last call (then reset): split records queued, paths
+ * ended by the weight window's roulette, split paths traced. */
+void orc_adrrs_counters(uint64_t out[3]);
+/* applyWeightWindow (raytrace.rgen:830-869) in FP64 with the given pixel
+ * estimate, adjoint (cache irradiance) and path throughput; *state is the
+ * LCG state (one draw when the window splits).  Returns q, sets *n. */
+double orc_weight_window(const double estimate[3], const double adjoint[3], const double throughput[3],
+                         double adrrsS, uint32_t *state, int *n);
 
 /* Path-guiding regions (boxes) and their vMF mixtures, read by renders
  * with pc.useGuiding (the reference's bindings 15-17). */
@@ -63,7 +77,8 @@ int orc_trace_occluded(const orc_scene *scene, const rtx_ray *rays, size_t n, in
 
 /* Irradiance cache (shaders/raytrace.rgen:748-828, 1228-1421).
  * orc_ic_frame renders one frame - every pixel of params at sample
- * params->sampleOffset, 1 spp - against the cache state passed in, then
+ * params->sampleOffset - against the cache state passed in (frames with
+ * pc.useIrradianceCache, or pc.useADRRS, which looks the cache up), then
  * runs the frame's cache updates and creations and writes the new state
  * back (hdr, and entries/spheres of hdr->maxCaches records).  rgb/sig per
  * pixel as in orc_render_paths.  The frame's jobs (updates first, in pixel

@@ -120,6 +120,8 @@ constexpr uint32_t kSigVisible = 0x1b873593u;
 constexpr uint32_t kSigOccluded = 0x5bd1e995u;
 constexpr uint32_t kSigProbe = 0x85ebca6bu;
 constexpr uint32_t kSigCacheHit = 0xc2b2ae35u; // an irradiance-cache lookup that ended the path
+constexpr uint32_t kSigRRKill = 0x27d4eb2fu;   // a weight-window roulette that ended the path
+constexpr uint32_t kSigSplit = 0x165667b1u;    // ^ number of split records a vertex queued
 constexpr uint32_t kDirInvalid = 0xFFFFFFFFu;  // DirectionalData.flags INVALID, guiding.glsl:112
 
 // Irradiance cache (shaders/raytrace.rgen:1228-1421, src/IrradianceCache.h:17-30)
@@ -133,6 +135,14 @@ constexpr int kIcStrata = kIcStrataPhi * kIcStrataTheta;
 // the 200 sub-paths of a cache record run in parallel.
 constexpr uint32_t kIcPhiKey = 0x9e3779b9u;
 constexpr uint32_t kIcStratumKey = 0x7f4a7c15u;
+
+// ADRRS: at most this many queued split records per pixel and frame
+// (MAX_SPLITS, raytrace.rgen:82).
+constexpr int kMaxSplits = 10;
+// Test statistics over all frames rendered since the last orc_adrrs_counters
+// call: split records queued, paths ended by the window's roulette, split
+// paths traced.
+std::atomic<uint64_t> g_adrrsCount[3];
 
 // The cache as a frame reads it: the reference's FP32 buffers.
 struct IcSnapshot {
@@ -177,6 +187,9 @@ struct orc_scene {
     // path-guiding regions and their mixtures (orc_guiding_set)
     std::vector<rtx_aabb> regions;
     std::vector<rtx_vmm_theta> vmms;
+    // ADRRS pixel estimates (estimateImage, raytrace.rgen:120), RGBA per pixel
+    std::vector<float> estimate;
+    uint32_t estW = 0, estH = 0;
 };
 
 namespace {
@@ -458,10 +471,21 @@ template <class R> struct Tracer {
     // guiding sample record of this path (raytrace.rgen:966-990, 1179-1212):
     // its RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL slots, flags INVALID on entry
     rtx_directional_data *dd = nullptr;
-    int curSlot = 0; // currentSampleOffset; sampleOffset is 0 (one path per pixel and frame)
+    int curSlot = 0;      // currentSampleOffset of the running raytrace()
+    int sampleOffset = 0; // first slot of the running raytrace() (raytrace.rgen:93, 1221)
     int iUpdateDistance = -1;
     R distanceFactor = R(1);
     V3<R> lightSums[RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL], sampleThroughputs[RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL];
+    // ADRRS (raytrace.rgen:69-88): the pixel's estimate and its queued splits
+    V3<R> estimate = V3<R>(0, 0, 0);
+    struct SplitRec { // splitInfo, raytrace.rgen:70-80 (no texture coordinates: untextured scenes)
+        V3<R> origin, normal, wi, throughput;
+        int matIndex, depth;
+        bool frontFace;
+    };
+    SplitRec splits[kMaxSplits];
+    int nSplits = 0;
+    R lastFirstT = C::TMAX;
 
     Tracer(const orc_scene &s, const rtx_frame_params &f) : S(s), fp(f), pc(f.pc) {}
 
@@ -986,13 +1010,13 @@ template <class R> struct Tracer {
                (m.type == RTX_MAT_PHONG && m.specularHighlight >= 250.0f);
     }
     void updateSamples(V3<R> light) { // raytrace.rgen:973-978
-        for (int i = curSlot - 1; i >= 0; --i) {
+        for (int i = curSlot - 1; i >= sampleOffset; --i) {
             lightSums[i] = lightSums[i] + light;
             light = light * sampleThroughputs[i];
         }
     }
     void commitSamples() { // raytrace.rgen:980-990
-        for (int i = 0; i < curSlot; ++i) {
+        for (int i = sampleOffset; i < curSlot; ++i) {
             R w = length(lightSums[i]) / (R)dd[i].pdf;
             if (w <= R(0)) dd[i].flags = kDirInvalid;
             else dd[i].weight = (float)w;
@@ -1054,18 +1078,59 @@ template <class R> struct Tracer {
         return false;
     }
 
-    // The bounce loop, ref: shaders/raytrace.rgen:992-1226: the plain method
-    // plus the irradiance-cache branch (1080-1139); no ADRRS, splitting or
-    // guiding yet.  Russian roulette (absent from the reference) is applied
-    // to camera paths when pc.enableRR is set.
-    V3<R> raytrace(V3<R> origin, V3<R> direction, V3<R> throughput, int maxDepth, int maxFollowDiscrete,
-                   bool addDirectLights, bool addFirstHitLight, bool useNEE, int numNEE, bool useIC, bool createIC,
-                   bool allowRR, R &firstT, bool saveSamples = false) {
+    // ------------------------------------------------------------- ADRRS
+    // applyWeightWindow (raytrace.rgen:830-869): the window's centre is
+    // |estimate / adjoint| (component-wise quotient), lower = 2c/(1+s),
+    // upper = s*lower; below it roulette with survival max(v/lower, 0.1),
+    // above it split into q = v/upper copies rounded up with probability
+    // q - floor(q).  A float-to-int conversion past INT_MAX saturates, as
+    // the GPU conversion instructions do.
+    R weightWindow(V3<R> throughput, V3<R> adjoint, int &n) {
+        R center = length(estimate / adjoint);
+        R lower = R(2) * center / (R(1) + (R)pc.adrrsS);
+        R upper = (R)pc.adrrsS * lower;
+        R v = length(throughput);
+        if (std::isnan(lower) || lower <= R(0)) {
+            n = 1;
+            return R(1);
+        }
+        if (lower <= v && v <= upper) {
+            n = 1;
+            return R(1);
+        } else if (v <= lower) {
+            n = 1;
+            return std::max(v / lower, R(0.1));
+        }
+        R q = v / upper;
+        n = q >= R(2147483520.0) ? 2147483646 : (int)q;
+        if (rnd() > (R)(n + 1) - q) n++;
+        return q;
+    }
+    // split (raytrace.rgen:883-902): false when MAX_SPLITS records are queued
+    bool split(V3<R> origin, V3<R> normal, V3<R> wi, V3<R> throughput, int matIndex, int depth, bool frontFace) {
+        if (nSplits >= kMaxSplits) return false;
+        splits[nSplits++] = SplitRec{origin, normal, wi, throughput, matIndex, depth, frontFace};
+        g_adrrsCount[0]++;
+        return true;
+    }
+
+    // The bounce loop, ref: shaders/raytrace.rgen:992-1226: the plain method,
+    // the irradiance-cache branch and ADRRS (1080-1146), guiding samples and
+    // records.  currentDepth is the depth the path starts from (the split
+    // loop of main() resumes paths at the depth of their split vertex) and
+    // returns the depth reached.  Russian roulette (absent from the
+    // reference) is applied to camera paths when pc.enableRR is set.
+    V3<R> raytrace(V3<R> origin, V3<R> direction, V3<R> throughput, int &currentDepth, int maxDepth,
+                   int maxFollowDiscrete, bool addDirectLights, bool addFirstHitLight, bool useNEE, int numNEE,
+                   bool useIC, bool createIC, bool useADRRS, bool allowRR, R &firstT, bool saveSamples = false) {
         bool follow = true;
         int followCount = 0;
         bool addNextDirectLights = addFirstHitLight;
-        int depth = 0;
+        int depth = currentDepth;
         firstT = C::TMAX;
+        curSlot = sampleOffset;
+        iUpdateDistance = -1;
+        distanceFactor = R(1);
         V3<R> result(0, 0, 0);
         do {
             depth++;
@@ -1097,21 +1162,52 @@ template <class R> struct Tracer {
             } else {
                 addNextDirectLights = false;
                 follow = false;
-                if (useIC && isICCapable(m)) {
+                if ((useIC || (useADRRS && depth > 1)) && isICCapable(m)) {
                     V3<R> irradiance;
                     if (queryIC(origin, normal, irradiance)) {
-                        // cache record found: the path ends here (raytrace.rgen:1114-1122)
-                        primHash = (primHash ^ kSigCacheHit) * 16777619u;
-                        result = result + throughput * approxDiffuse(m, normal, -direction) * irradiance;
-                        result = result + throughput * multipleNEE(m, origin, -direction, normal, info.isFrontFace,
-                                                                   numNEE);
-                        break;
+                        if (useADRRS) { // raytrace.rgen:1087-1116
+                            int n;
+                            R q = weightWindow(throughput, irradiance, n);
+                            if (n == 1) {
+                                if (rnd() > q) { // the path dies
+                                    primHash = (primHash ^ kSigRRKill) * 16777619u;
+                                    g_adrrsCount[1]++;
+                                    break;
+                                }
+                                throughput = throughput / q;
+                            } else if (pc.adrrsSplit) {
+                                int possible = kMaxSplits - nSplits;
+                                if (n - 1 > possible) {
+                                    n = possible + 1;
+                                    q = (R)n;
+                                }
+                                throughput = throughput / q;
+                                // n copies: this path and n - 1 queued splits
+                                for (int k = 0; k < n - 1; ++k)
+                                    split(origin, normal, -direction, throughput, info.matIndex, depth, info.isFrontFace);
+                                primHash = (primHash ^ (kSigSplit ^ (uint32_t)(n - 1))) * 16777619u;
+                            }
+                        } else {
+                            // cache record found: the path ends here (raytrace.rgen:1114-1122)
+                            primHash = (primHash ^ kSigCacheHit) * 16777619u;
+                            result = result + throughput * approxDiffuse(m, normal, -direction) * irradiance;
+                            result = result + throughput * multipleNEE(m, origin, -direction, normal, info.isFrontFace,
+                                                                       numNEE);
+                            break;
+                        }
                     } else if (createIC && rnd() < (R)pc.irradianceCreateProb) {
                         if (numCand < kMaxNewCacheEntries) { // raytrace.rgen:1125-1137
                             candO[numCand] = origin;
                             candN[numCand] = normal;
                             numCand++;
                         }
+                    }
+                }
+                if (pc.splitOnFirst && depth == 1) { // raytrace.rgen:1142-1146
+                    if (split(origin, normal, -direction, throughput * R(0.5), info.matIndex, depth,
+                              info.isFrontFace)) {
+                        throughput = throughput * R(0.5);
+                        primHash = (primHash ^ (kSigSplit ^ 1u)) * 16777619u;
                     }
                 }
                 if (saveSamples && iUpdateDistance != -1) {
@@ -1163,6 +1259,13 @@ template <class R> struct Tracer {
                 throughput = throughput / q;
             }
         } while (depth <= maxDepth || (follow && followCount <= maxFollowDiscrete));
+        if (saveSamples) {
+            commitSamples();
+            // the slots of a path that found no light are reused by the next one;
+            // otherwise the next starts at sampleOffset + currentSampleOffset (1220-1222)
+            if (length(result) > R(0)) sampleOffset += curSlot;
+        }
+        currentDepth = depth;
         return result;
     }
 
@@ -1180,7 +1283,19 @@ template <class R> struct Tracer {
         d = normalize(xform<R>(cam.viewInverse, normalize(target), R(0)));
     }
 
-    // main(), ref: shaders/raytrace.rgen:1623-1711 for one 1-spp frame.
+    // Camera samples of one frame (samplesPerPixel, 1 in prepare frames;
+    // raytrace.rgen:1662).
+    int samplesThisFrame() const {
+        if (pc.useIrradianceCache && pc.isIrradiancePrepareFrame) return 1;
+        return pc.samplesPerPixel > 1 ? pc.samplesPerPixel : 1;
+    }
+
+    // main(), ref: shaders/raytrace.rgen:1623-1711 for one pixel and frame:
+    // the estimate, the cache-update election (the stream's first draw), the
+    // camera samples, then the queued splits (each: NEE at its vertex, a new
+    // direction, a path resumed at its depth; a split whose direction has
+    // pdf <= 0 ends the split loop), the mean over the camera samples.
+    // Sample `pathId / (width*height)` of the call is one frame.
     V3<R> renderPath(uint64_t pathId) {
         const uint64_t npix = (uint64_t)fp.width * fp.height;
         uint32_t pixel = (uint32_t)(pathId % npix);
@@ -1189,16 +1304,45 @@ template <class R> struct Tracer {
         seed = tea(pixel, frameSeed);
         primHash = 2166136261u;
         const bool useIC = pc.useIrradianceCache != 0;
+        const bool useADRRS = pc.useADRRS != 0;
+        if (useADRRS && !S.estimate.empty()) estimate = f3<R>(&S.estimate[4 * (size_t)pixel]);
         // raytrace.rgen:1637-1639: the update election is the stream's first draw
         if (useIC && rnd() < (R)pc.irradianceUpdateProb) icElect = true;
-        V3<R> o, d;
-        cameraRay(pixel % fp.width, pixel / fp.width, fp.width, fp.height, o, d);
-        R firstT;
         const bool save = pc.updateGuiding != 0 && dd != nullptr;
-        V3<R> res = raytrace(o, d, V3<R>(1, 1, 1), pc.maxDepth, pc.maxFollowDiscrete, pc.enableNEE == 0, true,
-                             pc.enableNEE != 0, pc.numNEE, useIC, useIC, true, firstT, save);
-        if (save) commitSamples();
-        return res;
+        // recording frames render nothing when splitting is on (raytrace.rgen:1644-1648)
+        if (pc.updateGuiding && ((useADRRS && pc.adrrsSplit) || pc.splitOnFirst)) return V3<R>(0, 0, 0);
+        const bool useNEE = pc.enableNEE != 0;
+        const int spp = samplesThisFrame();
+        // createIC is true for every camera path (raytrace.rgen:1676); candidates
+        // only arise where the cache is looked up
+        const bool createIC = useIC || useADRRS;
+        V3<R> result(0, 0, 0);
+        R firstT = C::TMAX;
+        for (int k = 0; k < spp; ++k) {
+            V3<R> o, d;
+            cameraRay(pixel % fp.width, pixel / fp.width, fp.width, fp.height, o, d);
+            int depth = 0;
+            result = result + raytrace(o, d, V3<R>(1, 1, 1), depth, pc.maxDepth, pc.maxFollowDiscrete, !useNEE, true,
+                                       useNEE, pc.numNEE, useIC, createIC, useADRRS, true, firstT, save);
+        }
+        for (int k = 0; k < nSplits; ++k) { // raytrace.rgen:1684-1708
+            const SplitRec sp = splits[k];
+            const rtx_material &m = S.mats[sp.matIndex];
+            if (useNEE) result = result + sp.throughput * multipleNEE(m, sp.origin, sp.wi, sp.normal, sp.frontFace,
+                                                                      pc.numNEE);
+            V3<R> dir;
+            R pdf = getNewDirection(m, sp.origin, sp.normal, sp.wi, sp.frontFace, dir);
+            // a NaN pdf or BSDF ratio is treated as pdf <= 0, as in the bounce loop
+            if (pdf <= R(0) || std::isnan(pdf)) break;
+            V3<R> thr = sp.throughput * evalBsdf(m, sp.normal, sp.wi, dir, sp.frontFace) / pdf;
+            if (std::isnan(thr.x) || std::isnan(thr.y) || std::isnan(thr.z)) break;
+            int depth = sp.depth;
+            g_adrrsCount[2]++;
+            result = result + raytrace(sp.origin, dir, thr, depth, pc.maxDepth, pc.maxFollowDiscrete, !useNEE, false,
+                                       useNEE, pc.numNEE, useIC, true, useADRRS, true, firstT, save);
+        }
+        lastFirstT = firstT;
+        return result / (R)spp;
     }
 };
 
@@ -1211,24 +1355,42 @@ int checkScene(const orc_scene &s) {
     return RTX_OK;
 }
 
-int checkParams(const rtx_frame_params *fp, bool cacheFrame = false) {
+int checkParams(const orc_scene &s, const rtx_frame_params *fp, bool cacheFrame = false) {
     const rtx_push_constant &pc = fp->pc;
     if (fp->width == 0 || fp->height == 0) {
         g_err = "empty image";
         return RTX_E_INVALID;
     }
-    if (pc.useIrradianceCache && !cacheFrame) {
-        g_err = "oracle: irradiance-cache frames go through orc_ic_frame";
+    if ((pc.useIrradianceCache || pc.useADRRS) && !cacheFrame) {
+        g_err = "oracle: irradiance-cache and ADRRS frames go through orc_ic_frame";
         return RTX_E_UNSUPPORTED;
     }
     if (pc.useIrradianceCache && (pc.numNEE < 1 || pc.irradianceNumNEE < 1)) {
         g_err = "numNEE and irradianceNumNEE must be >= 1 with the irradiance cache";
         return RTX_E_INVALID;
     }
-    if (pc.useADRRS || pc.splitOnFirst ||
-        pc.visualizeMode != 0 || pc.showIrradianceCacheOnly || pc.storeEstimate) {
-        g_err = "oracle: only the plain method is restated so far";
+    if (pc.visualizeMode != 0 || pc.showIrradianceCacheOnly) {
+        g_err = "oracle: visualisation modes are not restated";
         return RTX_E_UNSUPPORTED;
+    }
+    if (pc.splitOnFirst && (pc.useIrradianceCache || pc.useADRRS)) {
+        // the cache's sub-paths would queue splits on the pixel's list
+        g_err = "splitOnFirst together with the irradiance cache or ADRRS is not supported";
+        return RTX_E_UNSUPPORTED;
+    }
+    if (pc.useADRRS) {
+        if (s.estimate.empty() || s.estW != fp->width || s.estH != fp->height) {
+            g_err = "useADRRS needs an estimate of the frame's size (orc_estimate_set)";
+            return RTX_E_STATE;
+        }
+        if (!(pc.adrrsS >= 1.0f)) { // the reference's range is [1, 5]
+            g_err = "adrrsS must be >= 1";
+            return RTX_E_INVALID;
+        }
+    }
+    if (pc.samplesPerPixel > 4096) {
+        g_err = "samplesPerPixel must be at most 4096";
+        return RTX_E_INVALID;
     }
     if (pc.enableNEE && pc.numNEE < 1) {
         g_err = "numNEE must be >= 1";
@@ -1309,8 +1471,9 @@ void icSubPath(const orc_scene &S, const rtx_frame_params &fp, const IcSnapshot 
     V3<R> n = vd<R>(jb.n), o = vd<R>(jb.o);
     V3<R> dir = Tracer<R>::toWorld(Tracer<R>::sphericalToCartesian(theta, phi), n);
     R r;
-    V3<R> L = t.raytrace(o, dir, V3<R>(1, 1, 1), 1, 10, false, false, true, fp.pc.irradianceNumNEE, true, false,
-                         false, r);
+    int depth = 0;
+    V3<R> L = t.raytrace(o, dir, V3<R>(1, 1, 1), depth, 1, 10, false, false, true, fp.pc.irradianceNumNEE, true, false,
+                         false, false, r);
     out[0] = (double)L.x;
     out[1] = (double)L.y;
     out[2] = (double)L.z;
@@ -1567,6 +1730,21 @@ int orc_scene_create(const rtx_scene_desc *d, orc_scene **out) {
 
 void orc_scene_destroy(orc_scene *s) { delete s; }
 
+void orc_adrrs_counters(uint64_t out[3]) {
+    for (int k = 0; k < 3; ++k) out[k] = g_adrrsCount[k].exchange(0);
+}
+
+int orc_estimate_set(orc_scene *s, const float *rgba, uint32_t width, uint32_t height) {
+    if (!s || (!rgba && width && height)) {
+        g_err = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    s->estimate.assign(rgba, rgba + 4 * (size_t)width * height);
+    s->estW = width;
+    s->estH = height;
+    return RTX_OK;
+}
+
 int orc_guiding_set(orc_scene *s, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) {
     if (!s || (n && (!regions || !vmms))) {
         g_err = "NULL argument";
@@ -1590,7 +1768,7 @@ int orc_render_paths(const orc_scene *s, const rtx_frame_params *fp, const uint6
         g_err = "NULL argument";
         return RTX_E_INVALID;
     }
-    int rc = checkParams(fp);
+    int rc = checkParams(*s, fp);
     if (rc != RTX_OK) return rc;
     if (prec != 32 && prec != 64) {
         g_err = "prec must be 32 or 64";
@@ -1633,7 +1811,7 @@ int orc_render_paths_record(const orc_scene *s, const rtx_frame_params *fp, cons
         g_err = "NULL argument";
         return RTX_E_INVALID;
     }
-    int rc = checkParams(fp);
+    int rc = checkParams(*s, fp);
     if (rc != RTX_OK) return rc;
     if ((fp->pc.useGuiding || fp->pc.updateGuiding) && s->regions.empty()) {
         g_err = "guiding without guiding regions (orc_guiding_set)";
@@ -1726,6 +1904,21 @@ rtx_frame_params defaultParams() {
 }
 } // namespace
 
+double orc_weight_window(const double estimate[3], const double adjoint[3], const double throughput[3],
+                         double adrrsS, uint32_t *state, int *n) {
+    rtx_frame_params fp = defaultParams();
+    fp.pc.adrrsS = (float)adrrsS;
+    Tracer<double> t(emptyScene(), fp);
+    t.seed = *state;
+    t.estimate = V3<double>(estimate[0], estimate[1], estimate[2]);
+    int nn = 1;
+    double q = t.weightWindow(V3<double>(throughput[0], throughput[1], throughput[2]),
+                              V3<double>(adjoint[0], adjoint[1], adjoint[2]), nn);
+    *state = t.seed;
+    *n = nn;
+    return q;
+}
+
 void orc_camera_ray(const rtx_camera *cam, uint32_t w, uint32_t h, uint32_t px, uint32_t py, uint32_t *state,
                     double org[3], double dir[3]) {
     rtx_frame_params fp = defaultParams();
@@ -1748,10 +1941,10 @@ int orc_ic_frame(const orc_scene *s, const rtx_frame_params *fp, int prec, int t
         g_err = "NULL argument";
         return RTX_E_INVALID;
     }
-    int rc = checkParams(fp, true);
+    int rc = checkParams(*s, fp, true);
     if (rc != RTX_OK) return rc;
-    if (!fp->pc.useIrradianceCache) {
-        g_err = "orc_ic_frame needs pc.useIrradianceCache";
+    if (!fp->pc.useIrradianceCache && !fp->pc.useADRRS) {
+        g_err = "orc_ic_frame needs pc.useIrradianceCache or pc.useADRRS";
         return RTX_E_INVALID;
     }
     if (fp->pc.useGuiding && s->regions.empty()) {
@@ -1799,7 +1992,7 @@ int orc_ic_calculate(const orc_scene *s, const rtx_frame_params *fp, int prec, c
         g_err = "NULL argument";
         return RTX_E_INVALID;
     }
-    int rc = checkParams(fp, true);
+    int rc = checkParams(*s, fp, true);
     if (rc != RTX_OK) return rc;
     IcSnapshot snap;
     snap.hdr = *hdr;
