@@ -446,6 +446,16 @@ typedef struct rtx_ic_job {
 int rtx_ic_read_jobs(rtx_ctx *ctx, rtx_ic_job *jobs, float *sub, uint32_t *subSig, uint32_t capacity,
                      uint32_t *numJobs);
 
+/* ------------------------------------------------------------------- ADRRS */
+/* The per-pixel estimate that frames with pc.useADRRS read: the reference's
+ * estimateImage (binding 14, shaders/raytrace.rgen:120, 1632-1636), written by
+ * a render with pc.storeEstimate (saveEstimate, raytrace.rgen:1602-1605:
+ * each pixel's frame result, RGBA with A = 1) or by rtx_estimate_write.
+ * rgba holds 4 floats per pixel, width * height pixels; rtx_estimate_read
+ * fails with RTX_E_STATE when no estimate exists or its size differs. */
+int rtx_estimate_write(rtx_ctx *ctx, const float *rgba, uint32_t width, uint32_t height);
+int rtx_estimate_read(rtx_ctx *ctx, float *rgba, uint32_t width, uint32_t height);
+
 /* ------------------------------------------------------------ path guiding */
 /* The guiding regions (boxes) and one vMF mixture per region: replaces
  * PathGuiding's AABB and VMM_Theta buffers and its acceleration structure

@@ -89,6 +89,12 @@ struct rtx_ctx {
     std::vector<void *> fitAllocs, fitWorkAllocs;
     GuideFitBuffers fit{};
     bool fitValid = false, fitParallax = true, fitFirst = true;
+    // frames of several segments: per-path frame state, split records
+    std::vector<void *> segAllocs, splitAllocs;
+    size_t segCap = 0, splitCap = 0;
+    // ADRRS pixel estimate (estimateImage)
+    float4 *estimate = nullptr;
+    uint32_t estW = 0, estH = 0;
 };
 
 namespace {
@@ -302,9 +308,6 @@ float icBaseCell(const rtx_push_constant &pc) {
 constexpr uint32_t kMaxBlockSums = (1u << 26) / 256 + 1;
 
 const char *unsupportedFeature(const rtx_push_constant &pc) {
-    if (pc.useADRRS) return "useADRRS";
-    if (pc.splitOnFirst) return "splitOnFirst";
-    if (pc.storeEstimate) return "storeEstimate";
     if (pc.showIrradianceCacheOnly) return "showIrradianceCacheOnly";
     if (pc.visualizeMode != 0) return "visualizeMode";
     return nullptr;
@@ -424,6 +427,9 @@ int rtx_ctx_destroy(rtx_ctx *ctx) {
     freeAll(ctx->fitAllocs);
     freeAll(ctx->fitWorkAllocs);
     freeAll(ctx->recAllocs);
+    freeAll(ctx->segAllocs);
+    freeAll(ctx->splitAllocs);
+    if (ctx->estimate) (void)hipFree(ctx->estimate);
     if (ctx->dirData) (void)hipFree(ctx->dirData);
     if (ctx->fcIC) (void)hipFree(ctx->fcIC);
     if (ctx->candBuf) (void)hipFree(ctx->candBuf);
@@ -616,14 +622,32 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         return fail(ctx, RTX_E_INVALID, "maxDepth must be in [0, 200] and maxFollowDiscrete in [0, 50]");
     if (p->batchPaths > (1u << 26)) return fail(ctx, RTX_E_INVALID, "batchPaths must be at most 2^26");
     const bool useIC = p->pc.useIrradianceCache != 0;
+    const bool useADRRS = p->pc.useADRRS != 0;
+    // frames that look the cache up and grow it: useIrradianceCache, or ADRRS
+    // (lookups from depth 2 for the weight window, creations where none is found)
+    const bool icFrames = useIC || useADRRS;
+    const bool splitting = (useADRRS && p->pc.adrrsSplit) || p->pc.splitOnFirst;
     // a cache hit runs multipleNEE even with NEE off (raytrace.rgen:1120-1121)
     const bool needNEE = p->pc.enableNEE || useIC;
     uint32_t numNEE = needNEE ? (uint32_t)p->pc.numNEE : 1u;
     if (needNEE && (p->pc.numNEE < 1 || p->pc.numNEE > 8))
         return fail(ctx, RTX_E_INVALID, "numNEE must be in [1, 8]");
+    if (p->pc.samplesPerPixel < 0 || p->pc.samplesPerPixel > 4096)
+        return fail(ctx, RTX_E_INVALID, "samplesPerPixel must be in [0, 4096]");
+    if (p->pc.splitOnFirst && icFrames)
+        // the cache's own sub-paths would queue splits on the pixel's list
+        return fail(ctx, RTX_E_UNSUPPORTED, "splitOnFirst together with the irradiance cache or ADRRS");
+    if (useADRRS) {
+        if (!(p->pc.adrrsS >= 1.0f)) return fail(ctx, RTX_E_INVALID, "adrrsS must be >= 1");
+        if (!ctx->estimate || ctx->estW != p->width || ctx->estH != p->height)
+            return fail(ctx, RTX_E_STATE,
+                        "useADRRS needs a pixel estimate of the frame's size (a storeEstimate render or "
+                        "rtx_estimate_write)");
+    }
     uint32_t neeCap = numNEE;
-    if (useIC) {
-        if (!ctx->icValid) return fail(ctx, RTX_E_STATE, "useIrradianceCache needs rtx_ic_create first");
+    if (icFrames) {
+        if (!ctx->icValid)
+            return fail(ctx, RTX_E_STATE, "useIrradianceCache and useADRRS need rtx_ic_create first");
         if (p->pc.irradianceNumNEE < 1 || p->pc.irradianceNumNEE > 8)
             return fail(ctx, RTX_E_INVALID, "irradianceNumNEE must be in [1, 8]");
         if (!(p->pc.irradianceA > 0.0f)) return fail(ctx, RTX_E_INVALID, "irradianceA must be > 0");
@@ -655,19 +679,47 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     }
     // irradiance-cache prepare frames only build the cache (src/RayTracingApp.cpp:160-161)
     const bool accumulate = !(useIC && p->pc.isIrradiancePrepareFrame);
+    // camera samples per pixel and frame (raytrace.rgen:1662)
+    const uint32_t samplesPerFrame =
+        (useIC && p->pc.isIrradiancePrepareFrame) ? 1u : (uint32_t)std::max(1, p->pc.samplesPerPixel);
+    const bool segments = samplesPerFrame > 1 || splitting;
 
     // batch size: whole sample planes when they fit, else pixel ranges; with
     // the cache one frame (sample plane) at a time, each followed by its upkeep
     uint64_t batch = p->batchPaths ? p->batchPaths : ctx->tuning.batchPaths;
     batch = std::min<uint64_t>(batch, total);
     const bool record = p->pc.updateGuiding != 0;
-    if (useIC || record) batch = std::min<uint64_t>(batch, npix);
+    if (icFrames || record) batch = std::min<uint64_t>(batch, npix);
     if (batch >= npix) batch -= batch % npix;
     size_t bufPaths = (size_t)batch;
-    if (useIC) bufPaths = std::max<size_t>(bufPaths, kIcStrata); // at least one job of sub-paths
+    if (icFrames) bufPaths = std::max<size_t>(bufPaths, kIcStrata); // at least one job of sub-paths
     int rc = ensureBatchBuffers(ctx, bufPaths, neeCap);
     if (rc != RTX_OK) return rc;
-    if (useIC && ctx->candCap < bufPaths) {
+    if (segments && ctx->segCap < bufPaths) {
+        freeAll(ctx->segAllocs);
+        ctx->segCap = 0;
+        CHECK_HIP(ctx, devAlloc(&ctx->pb.task, bufPaths, ctx->segAllocs));
+        CHECK_HIP(ctx, devAlloc(&ctx->pb.taskRes, bufPaths, ctx->segAllocs));
+        ctx->segCap = bufPaths;
+    }
+    if (splitting && ctx->splitCap < bufPaths) {
+        freeAll(ctx->splitAllocs);
+        ctx->splitCap = 0;
+        CHECK_HIP(ctx, devAlloc(&ctx->pb.splitRec, bufPaths * kMaxSplits * 4, ctx->splitAllocs));
+        ctx->splitCap = bufPaths;
+    }
+    ctx->pb.estimate = useADRRS ? ctx->estimate : nullptr;
+    const bool storeEstimate = p->pc.storeEstimate != 0;
+    if (storeEstimate && (!ctx->estimate || ctx->estW != p->width || ctx->estH != p->height)) {
+        if (ctx->estimate) (void)hipFree(ctx->estimate);
+        ctx->estimate = nullptr;
+        ctx->estW = ctx->estH = 0;
+        CHECK_HIP(ctx, hipMalloc(&ctx->estimate, npix * sizeof(float4)));
+        CHECK_HIP(ctx, hipMemsetAsync(ctx->estimate, 0, npix * sizeof(float4), ctx->stream));
+        ctx->estW = p->width;
+        ctx->estH = p->height;
+    }
+    if (icFrames && ctx->candCap < bufPaths) {
         if (ctx->candBuf) (void)hipFree(ctx->candBuf);
         ctx->candBuf = nullptr;
         ctx->candCap = 0;
@@ -713,6 +765,23 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         ctx->keptPaths = total;
     }
 
+    if (record && splitting) {
+        // main() returns right after resetting the records when splitting is on
+        // (raytrace.rgen:1641-1648): no pixel is traced or written
+        CHECK_HIP(ctx, hipMemsetAsync(ctx->dirData, 0xFF, (size_t)npix * kMaxRecords * sizeof(rtx_directional_data),
+                                      ctx->stream));
+        if (keep) {
+            CHECK_HIP(ctx, hipMemsetAsync(ctx->keepRgb, 0, total * sizeof(float4), ctx->stream));
+            CHECK_HIP(ctx, hipMemsetAsync(ctx->keepSig, 0, total * sizeof(uint2), ctx->stream));
+        }
+        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->paths = total;
+        }
+        return RTX_OK;
+    }
+
     FrameConst fc;
     std::memset(&fc, 0, sizeof(fc));
     fc.pc = p->pc;
@@ -734,9 +803,13 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.addDirect = p->pc.enableNEE ? 0u : 1u;
     fc.useNEE = p->pc.enableNEE ? 1u : 0u;
     fc.useIC = useIC ? 1u : 0u;
-    fc.createIC = useIC ? 1u : 0u;
+    fc.createIC = icFrames ? 1u : 0u; // candidates arise only where the cache is looked up
     fc.allowRR = 1;
     fc.record = record ? 1u : 0u;
+    fc.samplesPerFrame = samplesPerFrame;
+    fc.segments = segments ? 1u : 0u;
+    fc.useADRRS = useADRRS ? 1u : 0u;
+    fc.splitOnFirst = p->pc.splitOnFirst ? 1u : 0u;
     CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));
     // the cache's stratum sub-paths: raytrace(..., 1, 10, false, false, true, irradianceNumNEE,
     // true, false, ...) (raytrace.rgen:1260)
@@ -749,9 +822,13 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fcIC.createIC = 0;
     fcIC.allowRR = 0;
     fcIC.record = 0;
-    fcIC.numNEE = useIC ? (uint32_t)p->pc.irradianceNumNEE : 1u;
+    fcIC.samplesPerFrame = 1;
+    fcIC.segments = 0;
+    fcIC.useADRRS = 0;
+    fcIC.splitOnFirst = 0;
+    fcIC.numNEE = icFrames ? (uint32_t)p->pc.irradianceNumNEE : 1u;
     fcIC.debugPath = ~0ull;
-    if (useIC) CHECK_HIP(ctx, hipMemcpyAsync(ctx->fcIC, &fcIC, sizeof(fcIC), hipMemcpyHostToDevice, ctx->stream));
+    if (icFrames) CHECK_HIP(ctx, hipMemcpyAsync(ctx->fcIC, &fcIC, sizeof(fcIC), hipMemcpyHostToDevice, ctx->stream));
     CHECK_HIP(ctx, hipMemsetAsync(ctx->cnt, 0, sizeof(Counters), ctx->stream));
 
     const bool countVisits = fc.countVisits != 0;
@@ -775,10 +852,24 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         launch();
         if (ev) (void)hipEventRecord(ev->b, s);
     };
-    // the bounce iterations of one batch whose extend queue 0 is filled
-    auto bounces = [&](const FrameConst *fcd, BatchDesc b, int iters) {
+    // The bounce iterations of one batch whose extend queue 0 is filled: a
+    // path ends within `iters` iterations of its start.  With segments a path
+    // slot runs a pixel's whole frame (several camera samples, queued splits),
+    // so the loop goes on in rounds of `iters` until the extend queue is empty.
+    int itersRun = 0;
+    auto bounces = [&](const FrameConst *fcd, BatchDesc b, int iters, bool multi) -> int {
         int qsel = 0;
-        for (int it = 0; it < iters; ++it) {
+        const int cap = multi ? (int)(samplesPerFrame + kMaxSplits + 1) * iters : iters;
+        for (int it = 0;; ++it) {
+            if (it == iters || (multi && it > iters && it % iters == 0)) {
+                if (!multi) break;
+                uint32_t left = 0;
+                CHECK_HIP(ctx, hipMemcpyAsync(&left, &ctx->cnt->extend[qsel], sizeof(left), hipMemcpyDeviceToHost, s));
+                CHECK_HIP(ctx, hipStreamSynchronize(s));
+                if (left == 0) break;
+                if (it >= cap) return fail(ctx, RTX_E_DEVICE, "paths did not finish within the iteration bound");
+            }
+            itersRun = std::max(itersRun, it + 1);
             timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
             timed(kClsExtend, [&] {
                 if (persistent) launchClosestP(s, S, ctx->q[qsel], ctx->cnt, qsel, 0, countVisits, traceBlocks);
@@ -798,18 +889,21 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             } else {
                 timed(kClsProbe, [&] { launchProbe(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
             }
-            timed(kClsShade, [&] { launchFinish(s, fcd, ctx->pb, ctx->done, ctx->cnt, shadeBlocks); });
+            timed(kClsShade, [&] {
+                launchFinish(s, fcd, b, ctx->pb, ctx->done, ctx->cnt, ctx->q[qsel ^ 1], qsel, shadeBlocks);
+            });
             qsel ^= 1;
         }
+        return RTX_OK;
     };
 
     CHECK_HIP(ctx, hipEventRecord(ctx->tStart, s));
     uint32_t batches = 0;
-    if (useIC) timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
-    const uint64_t frameLen = (useIC || record) ? npix : total; // paths per frame with per-frame state
+    if (icFrames) timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
+    const uint64_t frameLen = (icFrames || record) ? npix : total; // paths per frame with per-frame state
     for (uint64_t f0 = 0; f0 < total; f0 += frameLen) {
         const uint64_t fEnd = std::min(total, f0 + frameLen);
-        if (useIC) timed(kClsShade, [&] { launchIcFrameBegin(s, ctx->ic); });
+        if (icFrames) timed(kClsShade, [&] { launchIcFrameBegin(s, ctx->ic); });
         for (uint64_t first = f0; first < fEnd; first += batch) {
             BatchDesc b;
             b.firstPath = first;
@@ -817,11 +911,13 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.npix = (uint32_t)npix;
             ctx->pb.dirFirstPixel = (uint32_t)(first % npix);
             timed(kClsShade, [&] { launchGenerate(s, ctx->fc, b, ctx->pb, ctx->q[0], ctx->cnt, 0, ctx->ic.frame); });
-            bounces(ctx->fc, b, maxIter);
+            int rcb = bounces(ctx->fc, b, maxIter, segments);
+            if (rcb != RTX_OK) return rcb;
             timed(kClsShade, [&] {
-                launchAccumulate(s, b, ctx->pb, accumulate ? ctx->accum : nullptr, ctx->keepRgb, ctx->keepSig, keep);
+                launchAccumulate(s, b, ctx->pb, accumulate ? ctx->accum : nullptr, ctx->keepRgb, ctx->keepSig, keep,
+                                 storeEstimate ? ctx->estimate : nullptr);
             });
-            if (useIC) timed(kClsShade, [&] { launchIcCollect(s, ctx->ic, ctx->pb, b.numPaths); });
+            if (icFrames) timed(kClsShade, [&] { launchIcCollect(s, ctx->ic, ctx->pb, b.numPaths); });
             CHECK_HIP(ctx, hipGetLastError());
             batches++;
         }
@@ -832,7 +928,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             timed(kClsShade, [&] { rcu = guideUpdateAsync(ctx); });
             if (rcu != RTX_OK) return rcu;
         }
-        if (!useIC) continue;
+        if (!icFrames) continue;
         // the frame's cache upkeep
         const uint64_t sample = f0 / npix + p->sampleOffset;
         const uint32_t frameSeed = pcg32First(p->seed, sample);
@@ -867,7 +963,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.npix = np;
             timed(kClsShade,
                   [&] { launchGenerateIC(s, ctx->ic, frameSeed, j0, np, ctx->pb, ctx->q[0], ctx->cnt, 0); });
-            bounces(ctx->fcIC, b, maxIterIC);
+            int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
+            if (rcs != RTX_OK) return rcs;
             timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, j0, nj, ctx->pb); });
             if (keepSub) {
                 CHECK_HIP(ctx, hipMemcpyAsync(ctx->keepSub + (size_t)j0 * kIcStrata, ctx->pb.out, np * sizeof(float4),
@@ -914,7 +1011,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             }
         }
         stats->batches = batches;
-        stats->bounces = (uint32_t)maxIter;
+        stats->bounces = (uint32_t)itersRun;
         stats->nodeBytes = (uint32_t)sizeof(BvhNode);
         stats->primBytes = 48;
     }
@@ -1164,6 +1261,35 @@ int rtx_ic_reset(rtx_ctx *ctx) {
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->icValid = true;
     ctx->keptJobs = ctx->keptUpdates = 0;
+    return RTX_OK;
+}
+
+int rtx_estimate_write(rtx_ctx *ctx, const float *rgba, uint32_t width, uint32_t height) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!rgba || width == 0 || height == 0) return fail(ctx, RTX_E_INVALID, "NULL or empty estimate");
+    const size_t npix = (size_t)width * height;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!ctx->estimate || ctx->estW != width || ctx->estH != height) {
+        if (ctx->estimate) (void)hipFree(ctx->estimate);
+        ctx->estimate = nullptr;
+        ctx->estW = ctx->estH = 0;
+        CHECK_HIP(ctx, hipMalloc(&ctx->estimate, npix * sizeof(float4)));
+        ctx->estW = width;
+        ctx->estH = height;
+    }
+    CHECK_HIP(ctx, hipMemcpy(ctx->estimate, rgba, npix * sizeof(float4), hipMemcpyHostToDevice));
+    return RTX_OK;
+}
+
+int rtx_estimate_read(rtx_ctx *ctx, float *rgba, uint32_t width, uint32_t height) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!rgba) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    if (!ctx->estimate || ctx->estW != width || ctx->estH != height)
+        return fail(ctx, RTX_E_STATE, "no estimate of this size");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CHECK_HIP(ctx, hipMemcpy(rgba, ctx->estimate, (size_t)width * height * sizeof(float4), hipMemcpyDeviceToHost));
     return RTX_OK;
 }
 

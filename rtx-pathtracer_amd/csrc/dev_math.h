@@ -24,6 +24,7 @@ RTX_DI V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
 RTX_DI V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 RTX_DI V3 operator*(float s, V3 a) { return {a.x * s, a.y * s, a.z * s}; }
 RTX_DI V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+RTX_DI V3 operator/(V3 a, V3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
 RTX_DI float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 RTX_DI V3 cross(V3 a, V3 b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
 RTX_DI float length(V3 a) { return sqrtf(dot(a, a)); }
@@ -64,6 +65,8 @@ constexpr uint32_t kSigVisible = 0x1b873593u;
 constexpr uint32_t kSigOccluded = 0x5bd1e995u;
 constexpr uint32_t kSigProbe = 0x85ebca6bu;
 constexpr uint32_t kSigCacheHit = 0xc2b2ae35u; // an irradiance-cache lookup that ended the path
+constexpr uint32_t kSigRRKill = 0x27d4eb2fu;   // a weight-window roulette that ended the path
+constexpr uint32_t kSigSplit = 0x165667b1u;    // ^ number of split records a vertex queued
 RTX_DI uint32_t sigFold(uint32_t h, uint32_t v) { return (h ^ v) * kSigPrime; }
 
 // ------------------------------------------------------------------ RNG

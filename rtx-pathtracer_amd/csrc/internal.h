@@ -73,6 +73,11 @@ struct FrameConst {
     int32_t maxDepth, maxFollow;
     uint32_t addDirect, useNEE, useIC, createIC, allowRR;
     uint32_t record;          // saveSamples: write guiding sample records (pc.updateGuiding)
+    // main()'s frame structure (raytrace.rgen:1662-1708): camera samples per
+    // pixel and frame, and whether a finished path continues with the
+    // frame's next camera sample or queued split (segments)
+    uint32_t samplesPerFrame, segments;
+    uint32_t useADRRS, splitOnFirst; // pc's, for camera paths only
 };
 
 // ------------------------------------------------------------ path guiding
@@ -204,7 +209,15 @@ struct PathBuffers {
     float4 *sampleThr;// kMaxRecords per path: throughput change of the slot's vertex
     rtx_directional_data *dirData; // per pixel of the frame: kMaxRecords slots
     uint32_t dirFirstPixel;        // pixel of local path 0 (frames are batched by pixel ranges)
+    // frames of several segments (FrameConst::segments): one path slot is one
+    // pixel's frame, main() of the reference
+    int4 *task;       // per path: (camera samples left, splits queued, next split, record sampleOffset)
+    float4 *taskRes;  // per path: main()'s result so far (the segment's own result is in `result`)
+    float4 *splitRec; // kMaxSplits x 4 per path: (origin, depth bits), (normal, material | frontFace << 31),
+                      // (wi, 0), (throughput, 0)  (splitInfo, raytrace.rgen:70-80)
+    const float4 *estimate; // per pixel: the ADRRS estimate (estimateImage)
 };
+constexpr int kMaxSplits = 10; // MAX_SPLITS, raytrace.rgen:82
 constexpr int kMaxRecords = RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL;
 constexpr uint32_t kDirInvalid = 0xFFFFFFFFu; // DirectionalData INVALID, guiding.glsl:112
 
@@ -216,6 +229,10 @@ constexpr uint32_t kFlagAddNext = 1u << 17;
 constexpr uint32_t kFlagPending = 1u << 18;       // NEE contributions waiting to be added
 constexpr uint32_t kFlagCandShift = 19;           // bits 19-21 irradiance-cache candidates
 constexpr uint32_t kFlagCandMask = 7u << kFlagCandShift;
+constexpr uint32_t kFlagPendTask = 1u << 22;      // the pending NEE terms belong to main()'s result (a split vertex)
+// Extend-queue id bit: the entry is a queued split to resume (its ray is
+// empty); kShade takes the vertex from the path's split record.
+constexpr uint32_t kSplitVertex = 1u << 31;
 
 // ----------------------------------------------------------- launchers
 
@@ -251,11 +268,14 @@ void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Cou
                   int blocks);
 void launchProbe(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb, Counters *cnt,
                  bool countVisits, int blocks);
-void launchFinish(hipStream_t s, const FrameConst *fc, PathBuffers pb, const int32_t *done, Counters *cnt,
-                  int blocks);
+// Ends the segments of the done queue; with FrameConst::segments a path
+// with camera samples left or splits queued goes on in qnext instead.
+void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, const int32_t *done,
+                  Counters *cnt, Queue qnext, int qsel, int blocks);
 // accum may be NULL (irradiance-cache prepare frames): then only the kept paths are written
+// estimate != NULL (pc.storeEstimate): each pixel's last frame result is stored there too
 void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,
-                      bool keep);
+                      bool keep, float4 *estimate);
 void launchResetCounters(hipStream_t s, Counters *cnt, int which);
 void launchQueryClosest(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, rtx_hit *hits);
 void launchQueryOccluded(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, uint8_t *occ);

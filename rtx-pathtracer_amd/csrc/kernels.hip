@@ -59,11 +59,12 @@ template <bool COUNT> RTX_DI void flushVisits(const VisitCount &vc, Counters *cn
 
 // ------------------------------------------------------ guiding records
 // updateSamples (raytrace.rgen:973-978): light reaching a vertex is added to
-// the light sums of the path's earlier records, through their throughputs.
-RTX_DI void updateSamples(const PathBuffers &pb, uint32_t pid, int base, V3 light) {
+// the light sums of the running segment's earlier records (slots lo..base-1),
+// through their throughputs.
+RTX_DI void updateSamples(const PathBuffers &pb, uint32_t pid, int lo, int base, V3 light) {
     float4 *ls = pb.lightSum + (size_t)pid * kMaxRecords;
     const float4 *st = pb.sampleThr + (size_t)pid * kMaxRecords;
-    for (int i = base - 1; i >= 0; --i) {
+    for (int i = base - 1; i >= lo; --i) {
         float4 a = ls[i];
         ls[i] = f4(v3(a) + light, 0.0f);
         light = light * v3(st[i]);
@@ -79,10 +80,27 @@ RTX_DI bool isMatAlmostDiscrete(const Mat &m) { // raytrace.rgen:962-964
            (m.type == RTX_MAT_PHONG && m.highlight >= 250.0f);
 }
 
+// getCameraRay (raytrace.rgen:1487-1494): a jittered pinhole ray, two draws.
+RTX_DI void cameraRay(const FrameConst &fc, uint32_t pixel, Rng &rng, V3 &o, V3 &d) {
+    uint32_t px = pixel % fc.width, py = pixel / fc.width;
+    float jx = rng.negPos();
+    float jy = rng.negPos();
+    float ux = ((float)px + 0.5f + jx / 2.0f) / (float)fc.width;
+    float uy = ((float)py + 0.5f + jy / 2.0f) / (float)fc.height;
+    float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    o = xform(fc.cam.viewInverse, v3(0.0f), 1.0f);
+    V3 target = xform(fc.cam.projInverse, v3(dx, dy, 1.0f), 1.0f);
+    d = normalize(xform(fc.cam.viewInverse, normalize(target), 0.0f));
+}
+
+// State of a path at the start of a raytrace() call from a camera sample:
+// depth 0, followCount 0, follow = true, addNextDirectLights = addFirstHitLight = true.
+constexpr uint32_t kFlagsCameraStart = kFlagFollow | kFlagAddNext;
+
 // ------------------------------------------------------------ generate
 // main() up to the camera ray: seed = tea(pixel, randomUInt) (raytrace.rgen:1625),
 // the irradiance-cache update election (1637-1639, the stream's first draw),
-// getCameraRay (raytrace.rgen:1487-1494); randomUInt per sample from PCG32.
+// the first camera sample; randomUInt per frame from PCG32.
 __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
                                                  Queue q0, Counters *cnt, int qsel, uint32_t *electCounter) {
     const FrameConst &fc = *fcp;
@@ -102,15 +120,8 @@ __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ 
         if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(electCounter, (uint32_t)__popcll(m));
     }
     if (!active) return;
-    uint32_t px = pixel % fc.width, py = pixel / fc.width;
-    float jx = rng.negPos();
-    float jy = rng.negPos();
-    float ux = ((float)px + 0.5f + jx / 2.0f) / (float)fc.width;
-    float uy = ((float)py + 0.5f + jy / 2.0f) / (float)fc.height;
-    float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
-    V3 o = xform(fc.cam.viewInverse, v3(0.0f), 1.0f);
-    V3 target = xform(fc.cam.projInverse, v3(dx, dy, 1.0f), 1.0f);
-    V3 d = normalize(xform(fc.cam.viewInverse, normalize(target), 0.0f));
+    V3 o, d;
+    cameraRay(fc, pixel, rng, o, d);
     q0.id[pid] = (int)pid;
     q0.org[pid] = f4(o, kTMin);
     q0.dir[pid] = f4(d, kTMax);
@@ -121,8 +132,11 @@ __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ 
         for (int k = 0; k < kMaxRecords; ++k) dd[k].flags = kDirInvalid;
         pb.grec[pid] = make_int4(0, -1, __float_as_int(1.0f), 0);
     }
-    // depth 0, followCount 0, follow = true, addNextDirectLights = addFirstHitLight = true
-    pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagFollow | kFlagAddNext));
+    if (fc.segments) {
+        pb.task[pid] = make_int4((int)fc.samplesPerFrame - 1, 0, 0, 0);
+        pb.taskRes[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagsCameraStart));
     pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
 }
 
@@ -325,8 +339,41 @@ RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint3
     return s;
 }
 
+// ---------------------------------------------------------------- ADRRS
+// applyWeightWindow (raytrace.rgen:830-869): window centre |estimate / adjoint|,
+// lower = 2c/(1+s), upper = s*lower; roulette below with survival
+// max(v/lower, 0.1), split above into q = v/upper copies (rounded up with
+// probability q - floor(q), one draw).  Float-to-int saturates past INT_MAX.
+RTX_DI float weightWindow(Rng &rng, V3 estimate, V3 adjoint, V3 thr, float s, int &n) {
+    const float center = length(estimate / adjoint);
+    const float lower = 2.0f * center / (1.0f + s);
+    const float upper = s * lower;
+    const float v = length(thr);
+    n = 1;
+    if (isnan(lower) || lower <= 0.0f) return 1.0f;
+    if (lower <= v && v <= upper) return 1.0f;
+    if (v <= lower) return fmaxf(v / lower, 0.1f);
+    const float q = v / upper;
+    n = q >= 2147483520.0f ? 2147483646 : (int)q;
+    if (rng.next() > (float)(n + 1) - q) n++;
+    return q;
+}
+
+// split (raytrace.rgen:883-902): queue slot task.y (the caller checks room).
+RTX_DI void pushSplit(const PathBuffers &pb, uint32_t pid, int4 &task, const HitInfo &h, V3 wi, V3 thr, int depth) {
+    float4 *sr = pb.splitRec + ((size_t)pid * kMaxSplits + (uint32_t)task.y) * 4;
+    sr[0] = f4(h.worldPos, __int_as_float(depth));
+    sr[1] = f4(h.normal, __uint_as_float((uint32_t)h.matIndex | (h.frontFace ? 0x80000000u : 0u)));
+    sr[2] = f4(wi, 0.0f);
+    sr[3] = f4(thr, 0.0f);
+    task.y++;
+}
+
 // ----------------------------------------------------------------- shade
-// One iteration of the bounce loop, raytrace.rgen:1008-1215 (plain method).
+// One iteration of the bounce loop, raytrace.rgen:1008-1215, for the paths of
+// the extend queue; an entry marked kSplitVertex resumes a queued split
+// instead (raytrace.rgen:1684-1703: NEE at its vertex, a new direction, then
+// the resumed path at the split's depth).
 __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevGuiding G,
                                                       const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
                                                       Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done,
@@ -343,6 +390,10 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
     const bool useIC = fc.useIC != 0, createIC = fc.createIC != 0;
     const bool record = fc.record != 0;
     const bool icOnGlossy = fc.pc.useIrradianceCacheOnGlossy != 0;
+    const bool segments = fc.segments != 0;
+    const bool useADRRS = fc.useADRRS != 0, splitOnFirst = fc.splitOnFirst != 0;
+    const bool adrrsSplit = fc.pc.adrrsSplit != 0;
+    const float adrrsS = fc.pc.adrrsS;
     IcQueryParams icp;
     icp.invA = 1.0f / fc.pc.irradianceA;
     icp.gradients = fc.pc.useIrradianceGradients != 0;
@@ -366,11 +417,15 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
         m.type = -1;
         int depth = 0, followCount = 0;
         bool follow = false, addNext = false, icHit = false;
+        bool splitV = false, pendTask = false, dead = false, taskDirty = false;
+        int4 task = make_int4(0, 0, 0, 0); // (camera samples left, splits queued, next split, record sampleOffset)
         uint32_t ncand = 0;
         int4 grec = make_int4(0, -1, 0, 0); // record state: slot, distance slot, distanceFactor bits, NEE base
         float firstT = kTMax;
         if (active) {
-            pid = (uint32_t)qin.id[i];
+            const uint32_t raw = (uint32_t)qin.id[i];
+            splitV = (raw & kSplitVertex) != 0;
+            pid = raw & ~kSplitVertex;
             float4 hit = qin.hit[i];
             d = v3(qin.dir[i]);
             V3 o = v3(qin.org[i]);
@@ -384,16 +439,47 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             ncand = (flags & kFlagCandMask) >> kFlagCandShift;
             hash = pb.hash[pid];
             if (record) grec = pb.grec[pid];
+            if (segments) task = pb.task[pid];
             if (flags & kFlagPending) {
                 V3 nee = pendingSum(pb, pid, numNEE, hash);
-                if (record) { // recorded NEE terms are kept without the throughput
+                if (flags & kFlagPendTask) { // a split vertex's NEE joins main()'s result
+                    float4 tr = pb.taskRes[pid];
+                    pb.taskRes[pid] = f4(v3(tr) + nee, tr.w);
+                } else if (record) { // recorded NEE terms are kept without the throughput
                     res = res + v3(pb.grecThr[pid]) * nee;
-                    updateSamples(pb, pid, grec.w, nee);
+                    updateSamples(pb, pid, task.w, grec.w, nee);
                 } else {
                     res = res + nee;
                 }
             }
             int prim = __float_as_int(hit.y);
+            if (splitV) {
+                // the next queued split: its vertex, throughput and depth; raytrace(...,
+                // addFirstHitLight = false, ...) from there (raytrace.rgen:1684-1703)
+                const float4 *sr = pb.splitRec + ((size_t)pid * kMaxSplits + (uint32_t)task.z) * 4;
+                const float4 a = sr[0], nb = sr[1], c = sr[2], t = sr[3];
+                task.z++;
+                taskDirty = true;
+                const uint32_t mf = __float_as_uint(nb.w);
+                h.worldPos = v3(a);
+                h.normal = v3(nb);
+                h.matIndex = (int)(mf & 0x7FFFFFFFu);
+                h.frontFace = (mf >> 31) != 0;
+                h.isSphere = false;
+                h.instanceIndex = 0;
+                h.u = h.v = 0.0f;
+                d = -v3(c);
+                thr = v3(t);
+                m = loadMat(&S.materials[h.matIndex]);
+                depth = __float_as_int(a.w);
+                followCount = 0;
+                follow = true;
+                addNext = false;
+                res = v3(0.0f);
+                firstT = kTMax;
+                doNee = useNEE && neeCapable(m.type) && S.lightCount > 0;
+                pendTask = doNee;
+            } else {
             hash = sigFold(hash, (uint32_t)prim);
             depth = (int)(flags & kFlagDepthMask) + 1;
             followCount = (int)((flags >> kFlagFollowCountShift) & 0xFFu);
@@ -404,7 +490,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                 if (addDirect || addNext) {
                     res = res + thr * v3(0.0f);
                     if (record) {
-                        updateSamples(pb, pid, grec.x, v3(0.0f));
+                        updateSamples(pb, pid, task.w, grec.x, v3(0.0f));
                         if (grec.y != -1) recordSlots(pb, pid)[grec.y].distance = 0.0f; // infinitely far
                     }
                 }
@@ -419,7 +505,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                            h.normal.y, h.normal.z, (int)h.frontFace, res.x);
                 if (m.type == RTX_MAT_LIGHT && (addDirect || addNext)) {
                     res = res + thr * m.lightColor;
-                    if (record) updateSamples(pb, pid, grec.x, m.lightColor);
+                    if (record) updateSamples(pb, pid, task.w, grec.x, m.lightColor);
                 }
                 if (hasDiscreteDirection(m.type)) {
                     addNext = true;
@@ -430,14 +516,38 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                     addNext = false;
                     follow = false;
                     bool nee = useNEE;
-                    if (useIC && isICCapable(m.type, icOnGlossy)) { // raytrace.rgen:1081-1139
+                    if ((useIC || (useADRRS && depth > 1)) && isICCapable(m.type, icOnGlossy)) { // raytrace.rgen:1080-1139
                         V3 irr;
                         if (icQuery(IC, icp, h.worldPos, h.normal, irr)) {
-                            // a cache record ends the path after its NEE (1114-1122)
-                            hash = sigFold(hash, kSigCacheHit);
-                            res = res + thr * approxDiffuse(m, h.normal, -d) * irr;
-                            icHit = true;
-                            nee = true;
+                            if (useADRRS) { // weight window against the cache's irradiance (1087-1116)
+                                const uint32_t pixel = (uint32_t)((b.firstPath + pid) % b.npix);
+                                int nw;
+                                float q = weightWindow(rng, v3(pb.estimate[pixel]), irr, thr, adrrsS, nw);
+                                if (nw == 1) {
+                                    if (rng.next() > q) { // the path dies here, before its NEE
+                                        hash = sigFold(hash, kSigRRKill);
+                                        dead = true;
+                                    } else {
+                                        thr = thr / q;
+                                    }
+                                } else if (adrrsSplit) {
+                                    const int room = kMaxSplits - task.y;
+                                    if (nw - 1 > room) {
+                                        nw = room + 1;
+                                        q = (float)nw;
+                                    }
+                                    thr = thr / q;
+                                    for (int k = 0; k < nw - 1; ++k) pushSplit(pb, pid, task, h, -d, thr, depth);
+                                    taskDirty = true;
+                                    hash = sigFold(hash, kSigSplit ^ (uint32_t)(nw - 1));
+                                }
+                            } else {
+                                // a cache record ends the path after its NEE (1114-1122)
+                                hash = sigFold(hash, kSigCacheHit);
+                                res = res + thr * approxDiffuse(m, h.normal, -d) * irr;
+                                icHit = true;
+                                nee = true;
+                            }
                         } else if (createIC && rng.next() < fc.pc.irradianceCreateProb) {
                             if (ncand < (uint32_t)kMaxNewCacheEntries) {
                                 float4 *c = pb.cand + 2 * ((size_t)pid * kMaxNewCacheEntries + ncand);
@@ -447,14 +557,25 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                             }
                         }
                     }
-                    if (record && !icHit && grec.y != -1) { // raytrace.rgen:1153-1160
+                    if (!dead && splitOnFirst && depth == 1 && task.y < kMaxSplits) { // raytrace.rgen:1142-1146
+                        pushSplit(pb, pid, task, h, -d, thr * 0.5f, depth);
+                        thr = thr * 0.5f;
+                        taskDirty = true;
+                        hash = sigFold(hash, kSigSplit ^ 1u);
+                    }
+                    if (record && !icHit && !dead && grec.y != -1) { // raytrace.rgen:1153-1160
                         recordSlots(pb, pid)[grec.y].distance += hit.x * __int_as_float(grec.z);
                         if (!isMatAlmostDiscrete(m)) grec.y = -1;
+                    }
+                    if (dead) {
+                        isDone = true;
+                        nee = false;
                     }
                     doNee = nee && neeCapable(m.type) && S.lightCount > 0;
                     if (record) grec.w = grec.x; // the NEE light reaches the records made so far
                 }
             }
+            } // not a split vertex
         }
         // ---- next event estimation, raytrace.rgen:601-731 (numNEE samples, multipleNEE 871-877)
         const V3 wi = -d;
@@ -540,6 +661,13 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             if (!(pdf <= 0.0f) && !isnan(pdf)) change = evalBsdf(m, h.normal, wi, newDir, h.frontFace) / pdf;
             if (pdf <= 0.0f || isnan(pdf) || isnan(change.x) || isnan(change.y) || isnan(change.z)) {
                 isDone = true;
+                if (splitV) { // a split without a direction ends the frame's split loop (raytrace.rgen:1698-1700)
+                    task.z = task.y;
+                    taskDirty = true;
+                }
+            } else if (splitV) {
+                thr = thr * change;
+                cont = true; // raytrace() traces at least once
             } else {
                 thr = thr * change;
                 if (record && grec.x < kMaxRecords) { // raytrace.rgen:1179-1199
@@ -586,7 +714,9 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
         if (active) {
             uint32_t nf = (uint32_t)depth | ((uint32_t)followCount << kFlagFollowCountShift) |
                           (follow ? kFlagFollow : 0u) | (addNext ? kFlagAddNext : 0u) |
-                          (doNee ? kFlagPending : 0u) | (ncand << kFlagCandShift);
+                          (doNee ? kFlagPending : 0u) | (doNee && pendTask ? kFlagPendTask : 0u) |
+                          (ncand << kFlagCandShift);
+            if (taskDirty) pb.task[pid] = task;
             pb.rng[pid] = rng.s;
             pb.hash[pid] = hash;
             pb.thr[pid] = f4(thr, __uint_as_float(nf));
@@ -608,38 +738,94 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
 }
 
 // ---------------------------------------------------------------- finish
-__global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fcp, PathBuffers pb,
-                                               const int32_t *done, Counters *cnt) {
+// The end of a raytrace() call: the last pending NEE terms, commitSamples
+// (raytrace.rgen:980-990, 1217-1222).  With segments the segment's result
+// joins main()'s, and the path goes on with the frame's next camera sample
+// (raytrace.rgen:1666-1681) or its next queued split (1684-1708), appended to
+// the next iteration's extend queue; after the last one the pixel's frame
+// result is main()'s sum over the camera samples (1710).
+__global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
+                                               const int32_t *done, Counters *cnt, Queue qn, int qsel) {
+    const FrameConst &fc = *fcp;
     const uint32_t n = cnt->done;
-    const uint32_t numNEE = fcp->numNEE;
-    const bool record = fcp->record != 0;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        uint32_t pid = (uint32_t)done[i];
-        float4 r4 = pb.result[pid];
-        V3 res = v3(r4);
-        uint32_t flags = __float_as_uint(pb.thr[pid].w);
-        uint32_t hash = pb.hash[pid];
-        int4 grec = record ? pb.grec[pid] : make_int4(0, -1, 0, 0);
-        if (flags & kFlagPending) {
-            V3 nee = pendingSum(pb, pid, numNEE, hash);
-            if (record) {
-                res = res + v3(pb.grecThr[pid]) * nee;
-                updateSamples(pb, pid, grec.w, nee);
+    const uint32_t numNEE = fc.numNEE;
+    const bool record = fc.record != 0;
+    const bool segments = fc.segments != 0;
+    uint32_t *nextCount = &cnt->extend[qsel ^ 1];
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const uint32_t i = base + threadIdx.x;
+        bool again = false, splitNext = false;
+        uint32_t pid = 0;
+        V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+        if (i < n) {
+            pid = (uint32_t)done[i];
+            float4 r4 = pb.result[pid];
+            V3 res = v3(r4);
+            uint32_t flags = __float_as_uint(pb.thr[pid].w);
+            uint32_t hash = pb.hash[pid];
+            int4 grec = record ? pb.grec[pid] : make_int4(0, -1, 0, 0);
+            int4 task = segments ? pb.task[pid] : make_int4(0, 0, 0, 0);
+            V3 tr = segments ? v3(pb.taskRes[pid]) : v3(0.0f);
+            if (flags & kFlagPending) {
+                V3 nee = pendingSum(pb, pid, numNEE, hash);
+                if (flags & kFlagPendTask) {
+                    tr = tr + nee;
+                } else if (record) {
+                    res = res + v3(pb.grecThr[pid]) * nee;
+                    updateSamples(pb, pid, task.w, grec.w, nee);
+                } else {
+                    res = res + nee;
+                }
+            }
+            if (record) { // commitSamples
+                rtx_directional_data *dd = recordSlots(pb, pid);
+                const float4 *ls = pb.lightSum + (size_t)pid * kMaxRecords;
+                for (int k = task.w; k < grec.x; ++k) {
+                    float w = length(v3(ls[k])) / dd[k].pdf;
+                    if (w <= 0.0f) dd[k].flags = kDirInvalid;
+                    else dd[k].weight = w;
+                }
+                // a path without light leaves its slots to the next one; otherwise
+                // the next starts at sampleOffset + currentSampleOffset
+                if (length(res) > 0.0f) task.w += grec.x;
+            }
+            if (!segments) {
+                pb.out[pid] = f4(res, r4.w);
+                pb.outSig[pid] = make_uint2(pb.rng[pid], hash);
             } else {
-                res = res + nee;
+                tr = tr + res; // result += raytrace(...)
+                const uint32_t keepCand = flags & kFlagCandMask;
+                Rng rng{pb.rng[pid]};
+                if (task.x > 0) {
+                    task.x--;
+                    const uint32_t pixel = (uint32_t)((b.firstPath + pid) % b.npix);
+                    cameraRay(fc, pixel, rng, o, dir);
+                    again = true;
+                    pb.rng[pid] = rng.s;
+                    pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagsCameraStart | keepCand));
+                    if (record) pb.grec[pid] = make_int4(task.w, -1, __float_as_int(1.0f), task.w);
+                } else if (task.z < task.y) {
+                    again = splitNext = true;
+                    pb.thr[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(keepCand));
+                } else {
+                    pb.out[pid] = f4(tr / (float)fc.samplesPerFrame, r4.w);
+                    pb.outSig[pid] = make_uint2(rng.s, hash);
+                }
+                if (again) {
+                    pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
+                    pb.hash[pid] = hash;
+                }
+                pb.task[pid] = task;
+                pb.taskRes[pid] = f4(tr, 0.0f);
             }
         }
-        if (record) { // commitSamples, raytrace.rgen:980-990
-            rtx_directional_data *dd = recordSlots(pb, pid);
-            const float4 *ls = pb.lightSum + (size_t)pid * kMaxRecords;
-            for (int k = 0; k < grec.x; ++k) {
-                float w = length(v3(ls[k])) / dd[k].pdf;
-                if (w <= 0.0f) dd[k].flags = kDirInvalid;
-                else dd[k].weight = w;
-            }
+        int qi = waveAppend(again, nextCount);
+        if (again) {
+            // a split's entry carries an empty ray (tmax < tmin): traversal finds nothing
+            qn.id[qi] = (int)(splitNext ? (pid | kSplitVertex) : pid);
+            qn.org[qi] = f4(o, splitNext ? 1.0f : kTMin);
+            qn.dir[qi] = f4(dir, splitNext ? -1.0f : kTMax);
         }
-        pb.out[pid] = f4(res, r4.w);
-        pb.outSig[pid] = make_uint2(pb.rng[pid], hash);
     }
 }
 
@@ -647,14 +833,15 @@ __global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fc
 // Adds each pixel's samples of this batch in increasing sample order, so
 // the accumulated image does not depend on how samples are batched.
 __global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb,
-                                                   uint2 *keepSig, int keep) {
+                                                   uint2 *keepSig, int keep, float4 *estimate) {
     uint32_t j = blockIdx.x * 256 + threadIdx.x;
     uint32_t lanes = b.numPaths < b.npix ? b.numPaths : b.npix;
     if (j >= lanes) return;
     uint32_t pixel = (uint32_t)((b.firstPath + j) % b.npix);
     float4 acc = accum ? accum[pixel] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (uint32_t pid = j; pid < b.numPaths; pid += b.npix) {
-        float4 v = pb.out[pid];
+        v = pb.out[pid];
         acc.x += v.x;
         acc.y += v.y;
         acc.z += v.z;
@@ -665,6 +852,7 @@ __global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, PathBuffers pb, 
         }
     }
     if (accum) accum[pixel] = acc;
+    if (estimate) estimate[pixel] = make_float4(v.x, v.y, v.z, 1.0f); // saveEstimate, raytrace.rgen:1602-1605
 }
 
 __global__ void kResetCounters(Counters *cnt, int which) {
@@ -802,15 +990,15 @@ void launchProbe(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q
     if (countVisits) hipLaunchKernelGGL(kProbe<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, fc, q, pb, cnt);
     else hipLaunchKernelGGL(kProbe<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, fc, q, pb, cnt);
 }
-void launchFinish(hipStream_t s, const FrameConst *fc, PathBuffers pb, const int32_t *done, Counters *cnt,
-                  int blocks) {
-    hipLaunchKernelGGL(kFinish, dim3(blocks), dim3(256), 0, s, fc, pb, done, cnt);
+void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, const int32_t *done,
+                  Counters *cnt, Queue qnext, int qsel, int blocks) {
+    hipLaunchKernelGGL(kFinish, dim3(blocks), dim3(256), 0, s, fc, b, pb, done, cnt, qnext, qsel);
 }
 void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,
-                      bool keep) {
+                      bool keep, float4 *estimate) {
     uint32_t lanes = b.numPaths < b.npix ? b.numPaths : b.npix;
     hipLaunchKernelGGL(kAccumulate, dim3(gridFor(lanes, 256)), dim3(256), 0, s, b, pb, accum, keepRgb, keepSig,
-                       keep ? 1 : 0);
+                       keep ? 1 : 0, estimate);
 }
 void launchResetCounters(hipStream_t s, Counters *cnt, int which) {
     hipLaunchKernelGGL(kResetCounters, dim3(1), dim3(64), 0, s, cnt, which);

@@ -102,6 +102,8 @@ def device_lib():
         L.rtx_guiding_read_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.rtx_guiding_init.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.c_uint32, C.c_int]
         L.rtx_guiding_update.argtypes = [C.c_void_p]
+        L.rtx_estimate_write.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
+        L.rtx_estimate_read.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         L.rtx_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         _dev = L
@@ -511,6 +513,20 @@ class Renderer:
     def ic_write(self, state):
         hdr, ent, sph = state.ctypes_views()
         self._check(device_lib().rtx_ic_write(self._h, hdr, ent, sph))
+
+    def estimate_write(self, rgba, width, height):
+        """The ADRRS pixel estimate (RGBA float32 per pixel) that frames with
+        pc.useADRRS read (the reference's estimateImage)."""
+        a = np.ascontiguousarray(rgba, np.float32).reshape(-1)
+        if a.size != 4 * width * height:
+            raise ValueError("estimate must hold 4 floats per pixel")
+        self._check(device_lib().rtx_estimate_write(self._h, a.ctypes.data_as(C.POINTER(C.c_float)), width, height))
+
+    def estimate_read(self, width, height):
+        """The pixel estimate as (width*height, 4) float32."""
+        a = np.zeros((width * height, 4), np.float32)
+        self._check(device_lib().rtx_estimate_read(self._h, a.ctypes.data_as(C.POINTER(C.c_float)), width, height))
+        return a
 
     def ic_read_jobs(self):
         """Jobs of the last kept cache frame: list of (type, target, origin,
