@@ -476,8 +476,25 @@ int rtx_guiding_init(rtx_ctx *ctx, const rtx_aabb *sceneBox, uint32_t splitCount
  * the region's centre (parallax), lightpmm's MAP-EM fits (first update) or
  * refits (later) the mixture, and the lobe distances are updated.  Renders
  * with pc.updateGuiding do this after every frame by themselves when the
- * context owns the mixtures.  Component split/merge is not done. */
+ * context owns the mixtures.  With options.splitAndMerge (the default) the
+ * fit is followed by postFit's component merge and split. */
 int rtx_guiding_update(rtx_ctx *ctx);
+/* PathGuiding's public fitting switches (src/PathGuiding.h:124-129), with
+ * their defaults: postFit merges pairs whose chi-square merge metric is at
+ * most mergeMaxDivergence once more than minSamplesForMerging samples came
+ * in since the last merge, and splits components whose weighted chi-square
+ * divergence reaches splitMinDivergence (refitting them when the batch has
+ * more than minSamplesForPostSplitFitting samples or is the first). */
+typedef struct rtx_guiding_options {
+    int32_t splitAndMerge;                  /* 1 */
+    uint32_t minSamplesForMerging;          /* 8192 */
+    uint32_t minSamplesForSplitting;        /* 4096 */
+    uint32_t minSamplesForPostSplitFitting; /* 4096 */
+    float splitMinDivergence;               /* 0.5 */
+    float mergeMaxDivergence;               /* 0.025 */
+} rtx_guiding_options;
+int rtx_guiding_set_options(rtx_ctx *ctx, const rtx_guiding_options *options);
+int rtx_guiding_get_options(rtx_ctx *ctx, rtx_guiding_options *options);
 /* The guiding samples of the last frame rendered with pc.updateGuiding: the
  * reference's DirectionalData buffer (binding 18, src/SampleCollector.cpp:
  * 29-51), RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL slots per pixel in pixel order;

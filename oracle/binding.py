@@ -101,6 +101,16 @@ def lib(abi):
         L.orc_guide_regions.restype = C.c_uint32
         L.orc_guide_update.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.orc_guide_last_error.restype = C.c_char_p
+        L.orc_guide_set_options.argtypes = [C.c_void_p, C.POINTER(abi.GuidingOptions)]
+        d8 = C.c_double * 8
+        L.orc_vmf_combine.argtypes = [d8, d8, C.c_int, d8]
+        L.orc_vmf_combine.restype = None
+        L.orc_vmf_merge.argtypes = [d8, d8, d8]
+        L.orc_vmf_merge.restype = None
+        L.orc_merge_metric.argtypes = [d8, d8]
+        L.orc_merge_metric.restype = C.c_double
+        L.orc_eigen2x2.argtypes = [C.c_double * 4, C.c_double * 2, C.c_double * 4]
+        L.orc_eigen2x2.restype = None
         L.orc_fastexp.argtypes = [C.c_float]
         L.orc_fastexp.restype = C.c_float
         L.orc_ic_calculate.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, C.c_uint32, d3,
@@ -136,6 +146,41 @@ def adrrs_counters(abi):
     out = (C.c_uint64 * 3)()
     lib(abi).orc_adrrs_counters(out)
     return tuple(int(x) for x in out)
+
+
+def vmf_lobe(w, mu, kappa):
+    """A lobe as the oracle's test hooks take it (8 doubles), normalised as
+    lightpmm's calNormalization does (with exact exp here)."""
+    mu = np.asarray(mu, np.float64)
+    mu = mu / np.linalg.norm(mu)
+    e2k = np.exp(-2 * kappa)
+    norm = kappa / (2 * np.pi * (1 - e2k)) if kappa > 0 else 1 / (4 * np.pi)
+    r = 1 / np.tanh(kappa) - 1 / kappa if kappa > 0 else 0.0
+    return (C.c_double * 8)(w, *mu, kappa, r, norm, e2k)
+
+
+def vmf_combine(abi, a, b, divide):
+    out = (C.c_double * 8)()
+    lib(abi).orc_vmf_combine(a, b, int(divide), out)
+    return np.array(out[:])
+
+
+def vmf_merge(abi, a, b):
+    out = (C.c_double * 8)()
+    lib(abi).orc_vmf_merge(a, b, out)
+    return np.array(out[:])
+
+
+def merge_metric(abi, a, b):
+    return lib(abi).orc_merge_metric(a, b)
+
+
+def eigen2x2(abi, m):
+    """(eigenvalues, eigenvector matrix with the vectors as columns) in Eigen's order."""
+    lam = (C.c_double * 2)()
+    vec = (C.c_double * 4)()
+    lib(abi).orc_eigen2x2((C.c_double * 4)(*[float(x) for x in np.asarray(m).reshape(-1)]), lam, vec)
+    return np.array(lam[:]), np.array(vec[:]).reshape(2, 2)
 
 
 def ic_reduce(abi, seed, sample, job, normal, sub, prec=64):
@@ -312,7 +357,8 @@ class OracleScene:
 class OracleGuide:
     """PathGuiding's regions and mixtures, refit with the oracle's EM."""
 
-    def __init__(self, abi, scene_box, splits, parallax=True, prec=64):
+    def __init__(self, abi, scene_box, splits, parallax=True, prec=64, options=None):
+        """options: abi.GuidingOptions (default: the reference's, split/merge on)."""
         self.abi = abi
         self.L = lib(abi)
         box = abi.Aabb()
@@ -322,6 +368,10 @@ class OracleGuide:
         rc = self.L.orc_guide_create(C.byref(box), splits, int(parallax), prec, C.byref(self._h))
         if rc != 0:
             raise RuntimeError(self.L.orc_guide_last_error().decode())
+        if options is not None:
+            rc = self.L.orc_guide_set_options(self._h, C.byref(options))
+            if rc != 0:
+                raise RuntimeError(self.L.orc_guide_last_error().decode())
 
     def __del__(self):
         try:

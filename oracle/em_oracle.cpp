@@ -12,8 +12,16 @@
 // statistics of external/guiding/incrementaldistance.h (updateDistances
 // 58-104, reposition 106-128).  lightpmm evaluates exp with the fastapprox
 // "fastexp" of Paul Mineiro (pmm-vcl.h:158-184); it is restated here from that
-// published method so that the fits follow the same numbers.  Component
-// split and merge (postFit's splitAll/mergeAll, 494-788) are not restated.
+// published method so that the fits follow the same numbers.  postFit's
+// component merge and split (src/PathGuiding.cpp:414-441, 494-788, on by
+// default: splitAndMerge, src/PathGuiding.h:124-129) follow below with the
+// statistics they read: external/guiding/incrementalpearsonchisquared.h
+// (54-213) and incrementalcovariance2d.h (54-263), lightpmm's VMF product,
+// division and mergeComponent (VMFKernel.h:315-407, 478-532), PMM
+// mergeComponents / weight priors (ParametricMixtureModel.h:298-375) and
+// VMMFactory::maskedFit (VMMFactory.h:225-297, 458-486).  The PCA split reads
+// a 2x2 eigen-decomposition from Eigen (absent here): restated as the one
+// Givens step of Eigen 3's RealSchur::splitOffTwoRows for a 2x2 matrix.
 #include "oracle.h"
 
 #include <algorithm>
@@ -130,7 +138,7 @@ template <class R> struct Stats {
 
 struct Sample {
     double pos[3], dir[3];
-    double weight, distance;
+    double weight, distance, pdf;
 };
 
 template <class R>
@@ -222,6 +230,11 @@ template <class R> struct Region {
     Pmm<R> pmm;
     R mean[3] = {}, lastMean[3] = {};
     R dist[kMaxK], distW[kMaxK];
+    // IncrementalPearsonChiSquared: divergencePlusOneTimesIntegralSqr, numSamples
+    R chi[kMaxK] = {}, chiN[kMaxK] = {};
+    // IncrementalCovariance2D: (var x, var y, cov xy), sumWeights
+    R cov[kMaxK][3] = {}, covW[kMaxK] = {};
+    uint32_t samplesSinceLastMerge = 0;
     Region() {
         for (int k = 0; k < kMaxK; ++k) {
             dist[k] = std::numeric_limits<R>::infinity();
@@ -275,6 +288,570 @@ template <class R> void reposition(Region<R> &g, const R newPosToOldPos[3]) {
     }
 }
 
+// ======================================================= split and merge
+// PathGuiding's switches (src/PathGuiding.h:124-129)
+struct SplitMergeProps {
+    bool splitAndMerge = true;
+    uint32_t minSamplesForMerging = 8192, minSamplesForSplitting = 4096, minSamplesForPostSplitFitting = 4096;
+    float splitMinDivergence = 0.5f, mergeMaxDivergence = 0.025f;
+};
+
+// PMM::removeWeightPrior / applyWeightPrior (ParametricMixtureModel.h:339-375):
+// unclamped, over the K active components.
+template <class R> void pmmRemovePrior(Pmm<R> &m, float vPrior) {
+    const R inv = (R)vPrior * (R)m.K + R(1), nrm = R(1) / inv, prior = (R)vPrior * nrm;
+    for (int k = 0; k < m.K; ++k) m.w[k] = (m.w[k] - prior) * inv;
+}
+template <class R> void pmmApplyPrior(Pmm<R> &m, float vPrior) {
+    const R inv = (R)vPrior * (R)m.K + R(1), nrm = R(1) / inv, prior = (R)vPrior * nrm;
+    for (int k = 0; k < m.K; ++k) m.w[k] = m.w[k] * nrm + prior;
+}
+
+// Component pdfs and the mixture pdf of one direction; false when the soft
+// assignment is invalid (mixture pdf <= 1e-8, SoftAssignmentWeights::valid).
+template <class R> bool componentPdfs(const Pmm<R> &m, const double dir[3], R p[kMaxK], R &mix) {
+    R d[3] = {(R)dir[0], (R)dir[1], (R)dir[2]};
+    mix = 0;
+    for (int k = 0; k < m.K; ++k) {
+        p[k] = m.pdf(k, d);
+        mix += m.w[k] * p[k];
+    }
+    return mix > R(1e-8f);
+}
+
+// IncrementalPearsonChiSquared::updateDivergence(Masked) (54-148): running
+// mean over batches of sum_samples pdf_k * weight^2 * pdf / mixturePdf^2.
+template <class R> void chiUpdate(Region<R> &g, const std::vector<Sample> &s, const bool *mask) {
+    if (s.empty()) return;
+    const Pmm<R> &m = g.pmm;
+    R batch[kMaxK] = {};
+    for (const Sample &x : s) {
+        R p[kMaxK], mix;
+        R d[3] = {(R)x.dir[0], (R)x.dir[1], (R)x.dir[2]};
+        mix = 0;
+        for (int k = 0; k < m.K; ++k) {
+            p[k] = m.pdf(k, d);
+            mix += m.w[k] * p[k];
+        }
+        if (!(mix > R(1e-8f))) continue; // PMM_EPSILON
+        const R f = (R)x.weight * (R)x.weight * (R)x.pdf / (mix * mix);
+        for (int k = 0; k < m.K; ++k)
+            if (!mask || mask[k]) batch[k] += p[k] * f;
+    }
+    for (int k = 0; k < m.K; ++k) {
+        if (mask && !mask[k]) continue;
+        const R n = g.chiN[k] + (R)s.size();
+        g.chi[k] = (g.chi[k] * g.chiN[k] + batch[k]) / n;
+        g.chiN[k] = n;
+    }
+}
+
+// computeDivergence (150-160)
+template <class R> R chiDivergence(const Region<R> &g, int k) {
+    const Pmm<R> &m = g.pmm;
+    const R invAvgSampleWeightSqr = (m.numSamples * m.numSamples) / (m.sampleWeight * m.sampleWeight);
+    return g.chi[k] * invAvgSampleWeightSqr - R(1);
+}
+
+// The tangent frame of a direction: Nori's coordinateAxis as lightpmm uses
+// it (pmm-glm.h:107-133): frame (x, y, z) with z = n.
+template <class R> void tangentFrame(const R n[3], R x[3], R y[3]) {
+    if (std::fabs(n[0]) > std::fabs(n[1])) {
+        R inv = R(1) / std::sqrt(n[0] * n[0] + n[2] * n[2]);
+        y[0] = n[2] * inv, y[1] = 0, y[2] = -n[0] * inv;
+    } else {
+        R inv = R(1) / std::sqrt(n[1] * n[1] + n[2] * n[2]);
+        y[0] = 0, y[1] = n[2] * inv, y[2] = -n[1] * inv;
+    }
+    x[0] = y[1] * n[2] - y[2] * n[1]; // cross(y, z)
+    x[1] = y[2] * n[0] - y[0] * n[2];
+    x[2] = y[0] * n[1] - y[1] * n[0];
+}
+
+// IncrementalCovariance2D::updateStatistics(Masked) (54-166): per component,
+// the weighted second moments of the sample directions in the component's
+// tangent frame (s = x, t = y), soft-assignment times sample weight.
+template <class R> void covUpdate(Region<R> &g, const std::vector<Sample> &s, const bool *mask) {
+    const Pmm<R> &m = g.pmm;
+    R fs[kMaxK][3], ft[kMaxK][3];
+    for (int k = 0; k < m.K; ++k) {
+        // the covariance code compares squares where coordinateAxis compares abs values
+        const R *n = m.mu[k];
+        const bool xGreater = n[0] * n[0] > n[1] * n[1];
+        const R inv = R(1) / std::sqrt((xGreater ? n[0] * n[0] : n[1] * n[1]) + n[2] * n[2]);
+        if (xGreater) ft[k][0] = n[2] * inv, ft[k][1] = 0, ft[k][2] = -n[0] * inv;
+        else ft[k][0] = 0, ft[k][1] = n[2] * inv, ft[k][2] = -n[1] * inv;
+        fs[k][0] = ft[k][1] * n[2] - ft[k][2] * n[1];
+        fs[k][1] = ft[k][2] * n[0] - ft[k][0] * n[2];
+        fs[k][2] = ft[k][0] * n[1] - ft[k][1] * n[0];
+    }
+    R bw[kMaxK] = {}, bv[kMaxK][3] = {};
+    for (const Sample &x : s) {
+        R d[3] = {(R)x.dir[0], (R)x.dir[1], (R)x.dir[2]};
+        R a[kMaxK], total = 0;
+        for (int k = 0; k < m.K; ++k) {
+            a[k] = m.w[k] * m.pdf(k, d);
+            total += a[k];
+        }
+        if (!(total > R(1e-8f))) continue;
+        const R inv = R(1) / total;
+        for (int k = 0; k < m.K; ++k) {
+            if (mask && !mask[k]) continue;
+            const R ws = (R)x.weight * (a[k] * inv);
+            bw[k] += ws;
+            const R lx = fs[k][0] * d[0] + fs[k][1] * d[1] + fs[k][2] * d[2];
+            const R ly = ft[k][0] * d[0] + ft[k][1] * d[1] + ft[k][2] * d[2];
+            bv[k][0] += lx * lx * ws;
+            bv[k][1] += ly * ly * ws;
+            bv[k][2] += lx * ly * ws;
+        }
+    }
+    for (int k = 0; k < m.K; ++k) {
+        if (mask && !mask[k]) continue;
+        const R nw = g.covW[k] + bw[k];
+        for (int c = 0; c < 3; ++c) g.cov[k][c] = nw > R(0) ? (g.cov[k][c] * g.covW[k] + bv[k][c]) / nw : R(0);
+        g.covW[k] = nw;
+    }
+}
+
+// One vMF lobe as lightpmm's kernels hold it, for the merge metric.
+template <class R> struct Lobe {
+    R w, mu[3], kappa, r, norm, e2k;
+};
+template <class R> Lobe<R> lobeOf(const Pmm<R> &m, int k) {
+    return Lobe<R>{m.w[k], {m.mu[k][0], m.mu[k][1], m.mu[k][2]}, m.kappa[k], m.r[k], m.norm[k], m.e2k[k]};
+}
+constexpr float kVmfMinKappa = 1e-3f;
+// VMFKernel::product (315-355) and division (362-402): the product/quotient
+// lobe; its weight is the integral (scaled by the mixture weights).
+template <class R> Lobe<R> lobeCombine(const Lobe<R> &a, const Lobe<R> &b, bool divide) {
+    const R sgn = divide ? R(-1) : R(1);
+    R nm[3];
+    for (int c = 0; c < 3; ++c) nm[c] = a.mu[c] * a.kappa + sgn * b.mu[c] * b.kappa;
+    R nk = std::sqrt(nm[0] * nm[0] + nm[1] * nm[1] + nm[2] * nm[2]);
+    const bool small = nk < R(kVmfMinKappa);
+    if (small) nk = 0;
+    for (int c = 0; c < 3; ++c) nm[c] = small ? a.mu[c] : nm[c] / nk;
+    const R ne2k = fastexp(R(-2) * nk);
+    const R cij = small ? (R)(1.0f / (4.0f * (float)kPi)) : (R)(1.0f / (2.0f * (float)kPi)) * nk / (R(1) - ne2k);
+    const R ti = nm[0] * a.mu[0] + nm[1] * a.mu[1] + nm[2] * a.mu[2];
+    const R tj = nm[0] * b.mu[0] + nm[1] * b.mu[1] + nm[2] * b.mu[2];
+    const R t = a.kappa * (ti - R(1)) + sgn * b.kappa * (tj - R(1));
+    const R e = fastexp(t);
+    Lobe<R> o;
+    o.w = divide ? e * a.w * a.norm / (b.w * b.norm * cij) : e * a.w * b.w * (a.norm * b.norm) / cij;
+    o.kappa = nk;
+    o.r = kappaToMeanCosine(nk);
+    for (int c = 0; c < 3; ++c) o.mu[c] = nm[c];
+    o.norm = cij;
+    o.e2k = ne2k;
+    return o;
+}
+// VMFKernel::mergeComponent (478-532): moment-matched merge of two lobes.
+template <class R> Lobe<R> lobeMerge(const Lobe<R> &a, const Lobe<R> &b) {
+    Lobe<R> o;
+    R kappa = 0, norm = (R)(1.0f / (4.0f * (float)kPi)), e2k = 1;
+    const R weight = a.w + b.w;
+    R mu[3];
+    for (int c = 0; c < 3; ++c) mu[c] = (a.w * a.r * a.mu[c] + b.w * b.r * b.mu[c]) / weight;
+    R r = mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2];
+    if (r > R(0)) {
+        r = std::sqrt(r);
+        kappa = meanCosineToKappa(r);
+        if (kappa < R(kVmfMinKappa)) kappa = 0;
+        e2k = std::exp(R(-2) * kappa);
+        norm = kappa / (R(2) * (R)(float)kPi * (R(1) - e2k));
+        for (int c = 0; c < 3; ++c) mu[c] /= r;
+    } else {
+        for (int c = 0; c < 3; ++c) mu[c] = a.mu[c];
+    }
+    o.w = weight;
+    o.kappa = kappa;
+    o.r = r;
+    for (int c = 0; c < 3; ++c) o.mu[c] = mu[c];
+    o.norm = norm;
+    o.e2k = e2k;
+    return o;
+}
+
+// computePearsonChiSquaredMergeMetric (src/PathGuiding.cpp:712-762): for
+// each pair i < j the chi-square divergence between the pair and its
+// merged lobe, A^2/M + 2 AB/M + B^2/M - w_M.
+template <class R> R mergeMetric(const Pmm<R> &m, int i, int j) {
+    const Lobe<R> a = lobeOf(m, i), b = lobeOf(m, j);
+    const Lobe<R> aa = lobeCombine(a, a, false), bb = lobeCombine(b, b, false), ab = lobeCombine(a, b, false);
+    const Lobe<R> mg = lobeMerge(a, b);
+    return lobeCombine(aa, mg, true).w + R(2) * lobeCombine(ab, mg, true).w + lobeCombine(bb, mg, true).w - mg.w;
+}
+
+// std::partition as libstdc++ implements it for bidirectional iterators
+// (the order of the kept elements decides which split gets which index).
+template <class T, class P> size_t partitionLikeLibstdcxx(std::vector<T> &v, P pred) {
+    size_t first = 0, last = v.size();
+    for (;;) {
+        for (;;) {
+            if (first == last) return first;
+            if (pred(v[first])) ++first;
+            else break;
+        }
+        --last;
+        for (;;) {
+            if (first == last) return first;
+            if (!pred(v[last])) --last;
+            else break;
+        }
+        std::swap(v[first], v[last]);
+        ++first;
+    }
+}
+
+// PathGuiding::mergeComponents (765-788): the statistics first (they read
+// the weights before the merge), then PMM::mergeComponents (321-333): b is
+// merged into a, the last component moves into b's slot.
+template <class R> void mergeComponents(Region<R> &g, int a, int b) {
+    Pmm<R> &m = g.pmm;
+    const int last = m.K - 1;
+    const R wA = m.w[a], wB = m.w[b], wM = wA + wB;
+    { // IncrementalDistance::merge (incrementaldistance.h:165-204): harmonic means
+        const R sw = wM / (wA / g.distW[a] + wB / g.distW[b]);
+        const R d = wM / (wA / g.dist[a] + wB / g.dist[b]);
+        g.dist[a] = d;
+        g.distW[a] = sw;
+        if (b != last) {
+            g.dist[b] = g.dist[last];
+            g.distW[b] = g.distW[last];
+        }
+    }
+    { // IncrementalPearsonChiSquared::merge (172-198)
+        const R inv = R(1) / wM;
+        if (!std::isinf(inv)) {
+            const R sa = wA * inv, sb = wB * inv;
+            g.chi[a] = sa * g.chi[a] + sb * g.chi[b];
+            g.chiN[a] = sa * g.chiN[a] + sb * g.chiN[b];
+        } else {
+            g.chi[a] = g.chiN[a] = 0;
+        }
+        if (b != last) {
+            g.chi[b] = g.chi[last];
+            g.chiN[b] = g.chiN[last];
+        }
+    }
+    { // IncrementalCovariance2D::merge (178-224)
+        const R inv = R(1) / wM;
+        if (!std::isinf(inv)) {
+            R mm[3];
+            for (int c = 0; c < 3; ++c) mm[c] = wA * m.mu[a][c] + wB * m.mu[b][c];
+            const R il = R(1) / std::sqrt(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]);
+            for (int c = 0; c < 3; ++c) mm[c] *= il;
+            R fx[3], fy[3];
+            tangentFrame(mm, fx, fy);
+            const R xa = fx[0] * m.mu[a][0] + fx[1] * m.mu[a][1] + fx[2] * m.mu[a][2];
+            const R ya = fy[0] * m.mu[a][0] + fy[1] * m.mu[a][1] + fy[2] * m.mu[a][2];
+            const R xb = fx[0] * m.mu[b][0] + fx[1] * m.mu[b][1] + fx[2] * m.mu[b][2];
+            const R yb = fy[0] * m.mu[b][0] + fy[1] * m.mu[b][1] + fy[2] * m.mu[b][2];
+            const R mix = wA * wB / wM;
+            const R dv[3] = {(xa - xb) * (xa - xb), (ya - yb) * (ya - yb), (xa - xb) * (ya - yb)};
+            for (int c = 0; c < 3; ++c) g.cov[a][c] = (wA * g.cov[a][c] + wB * g.cov[b][c] + mix * dv[c]) * inv;
+            g.covW[a] = g.covW[a] + g.covW[b];
+        } else {
+            g.cov[a][0] = g.cov[a][1] = g.cov[a][2] = 0;
+            g.covW[a] = 0;
+        }
+        if (b != last) {
+            for (int c = 0; c < 3; ++c) g.cov[b][c] = g.cov[last][c];
+            g.covW[b] = g.covW[last];
+        }
+    }
+    const Lobe<R> mg = lobeMerge(lobeOf(m, a), lobeOf(m, b));
+    m.w[a] = mg.w;
+    m.kappa[a] = mg.kappa;
+    m.r[a] = mg.r;
+    for (int c = 0; c < 3; ++c) m.mu[a][c] = mg.mu[c];
+    m.norm[a] = mg.norm;
+    m.e2k[a] = mg.e2k;
+    // reset b, then swap it with the last component
+    m.w[b] = 0, m.kappa[b] = 0, m.r[b] = 0, m.mu[b][0] = 0, m.mu[b][1] = 0, m.mu[b][2] = 1;
+    m.norm[b] = (R)(1.0f / (4.0f * (float)kPi)), m.e2k[b] = 1;
+    m.K -= 1;
+    if (b != m.K) {
+        std::swap(m.w[b], m.w[m.K]);
+        std::swap(m.kappa[b], m.kappa[m.K]);
+        std::swap(m.r[b], m.r[m.K]);
+        for (int c = 0; c < 3; ++c) std::swap(m.mu[b][c], m.mu[m.K][c]);
+        std::swap(m.norm[b], m.norm[m.K]);
+        std::swap(m.e2k[b], m.e2k[m.K]);
+    }
+}
+
+// PathGuiding::mergeAll (635-709): repeatedly merge the disjoint pairs with
+// metric <= mergeMaxDivergence, smallest metric first.
+template <class R> uint32_t mergeAll(const SplitMergeProps &SP, Region<R> &g) {
+    uint32_t total = 0;
+    for (;;) {
+        const int K = g.pmm.K;
+        if (K <= 1) return total;
+        struct Cand {
+            R metric;
+            int a, b;
+        };
+        std::vector<Cand> c;
+        for (int i = 0; i < K; ++i)
+            for (int j = i + 1; j < K; ++j) c.push_back(Cand{mergeMetric(g.pmm, i, j), i, j});
+        const size_t nValid = partitionLikeLibstdcxx(c, [&](const Cand &x) { return x.metric <= (R)SP.mergeMaxDivergence; });
+        std::stable_sort(c.begin(), c.begin() + nValid, [](const Cand &x, const Cand &y) { return x.metric < y.metric; });
+        uint32_t used = 0;
+        std::vector<std::pair<int, int>> merges;
+        for (size_t q = 0; q < nValid; ++q) {
+            const int a = std::min(c[q].a, c[q].b), b = std::max(c[q].a, c[q].b);
+            if ((used >> a & 1u) || (used >> b & 1u)) continue;
+            merges.push_back({a, b});
+            used |= (1u << a) | (1u << b);
+        }
+        if (merges.empty()) break;
+        std::sort(merges.begin(), merges.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+            return x.second > y.second;
+        });
+        for (const auto &mg : merges) mergeComponents(g, mg.first, mg.second);
+        total += (uint32_t)merges.size();
+    }
+    return total;
+}
+
+// Eigen 3's EigenSolver on a real 2x2 matrix, as RealSchur computes it: the
+// matrix scaled by its largest entry, one Givens rotation (makeGivens of
+// Anderson's continuous plane rotations) splitting off the 2x2 block unless
+// the subdiagonal is already negligible, eigenvalues from the diagonal,
+// eigenvectors by back-substitution, columns normalised.  lam[i] and the
+// column V[.][i] belong together, in Eigen's order.
+template <class R> void eigen2x2(R a00, R a01, R a10, R a11, R lam[2], R V[2][2]) {
+    R T[2][2] = {{a00, a01}, {a10, a11}};
+    R U[2][2] = {{1, 0}, {0, 1}};
+    const R scale = std::max(std::max(std::fabs(a00), std::fabs(a01)), std::max(std::fabs(a10), std::fabs(a11)));
+    if (scale < std::numeric_limits<R>::min()) {
+        lam[0] = lam[1] = 0;
+        V[0][0] = V[1][1] = 1;
+        V[0][1] = V[1][0] = 0;
+        return;
+    }
+    for (auto &row : T)
+        for (R &x : row) x /= scale;
+    const R eps = std::numeric_limits<R>::epsilon();
+    R sd = std::fabs(T[0][0]) + std::fabs(T[1][1]);
+    if (sd == R(0)) sd = std::fabs(T[0][0]) + std::fabs(T[0][1]) + std::fabs(T[1][0]) + std::fabs(T[1][1]);
+    if (!(std::fabs(T[1][0]) < eps * sd)) { // splitOffTwoRows(1)
+        const R p = R(0.5) * (T[0][0] - T[1][1]);
+        const R q = p * p + T[1][0] * T[0][1];
+        if (q >= R(0)) {
+            const R z = std::sqrt(std::fabs(q));
+            const R gp = p >= R(0) ? p + z : p - z, gq = T[1][0];
+            R c, s;
+            if (gq == R(0)) {
+                c = gp < R(0) ? R(-1) : R(1);
+                s = 0;
+            } else if (gp == R(0)) {
+                c = 0;
+                s = gq < R(0) ? R(1) : R(-1);
+            } else if (std::fabs(gp) > std::fabs(gq)) {
+                R t = gq / gp, u = std::sqrt(R(1) + t * t);
+                if (gp < R(0)) u = -u;
+                c = R(1) / u;
+                s = -t * c;
+            } else {
+                R t = gp / gq, u = std::sqrt(R(1) + t * t);
+                if (gq < R(0)) u = -u;
+                s = -R(1) / u;
+                c = -t * s;
+            }
+            // rows: T <- G^T T (adjoint rotation on the left), columns: T <- T G, U <- U G
+            for (int j = 0; j < 2; ++j) {
+                const R x = T[0][j], y = T[1][j];
+                T[0][j] = c * x - s * y;
+                T[1][j] = s * x + c * y;
+            }
+            for (int i = 0; i < 2; ++i) {
+                R x = T[i][0], y = T[i][1];
+                T[i][0] = c * x - s * y;
+                T[i][1] = s * x + c * y;
+                x = U[i][0], y = U[i][1];
+                U[i][0] = c * x - s * y;
+                U[i][1] = s * x + c * y;
+            }
+            T[1][0] = 0;
+        }
+    } else {
+        T[1][0] = 0;
+    }
+    for (auto &row : T)
+        for (R &x : row) x *= scale;
+    lam[0] = T[0][0];
+    lam[1] = T[1][1];
+    // back-substitution: X = [[1, x01], [0, 1]], x01 = -T01 / (T00 - lam1)
+    const R w = T[0][0] - lam[1];
+    const R nrm = std::fabs(T[0][0]) + std::fabs(T[0][1]) + std::fabs(T[1][1]);
+    const R x01 = w != R(0) ? -T[0][1] / w : -T[0][1] / (eps * nrm);
+    R v0[2] = {U[0][0], U[1][0]};
+    R v1[2] = {U[0][0] * x01 + U[0][1], U[1][0] * x01 + U[1][1]};
+    const R n0 = std::sqrt(v0[0] * v0[0] + v0[1] * v0[1]), n1 = std::sqrt(v1[0] * v1[0] + v1[1] * v1[1]);
+    V[0][0] = v0[0] / n0, V[1][0] = v0[1] / n0;
+    V[0][1] = v1[0] / n1, V[1][1] = v1[1] / n1;
+}
+
+// splitComponentUsingPCA (src/PathGuiding.cpp:494-561): the component is
+// replaced by two at +-(half a standard deviation along the principal axis
+// of its directions' covariance), each with half the weight.  The reference
+// takes the principal axis from a ROW of Eigen's eigenvector matrix
+// (computeEigenValuesVectors, 453-486); kept.
+template <class R> void splitComponent(Region<R> &g, int comp, float maxKappa) {
+    Pmm<R> &m = g.pmm;
+    if (m.K == kMaxK) return;
+    const int tgt = m.K;
+    R fx[3], fy[3];
+    tangentFrame(m.mu[comp], fx, fy);
+    R vx = 0, vy = 0, cxy = 0;
+    if (g.covW[comp] > R(0)) vx = g.cov[comp][0], vy = g.cov[comp][1], cxy = g.cov[comp][2];
+    R lam[2], V[2][2];
+    eigen2x2(vx, cxy, cxy, vy, lam, V); // Matrix2f << c[0][0], c[1][0], c[0][1], c[1][1]
+    // (value, row) pairs sorted by value with the comparator <=, then the larger
+    int order[2] = {0, 1};
+    if (lam[1] <= lam[0]) order[0] = 1, order[1] = 0;
+    const int pick = lam[order[1]] > lam[order[0]] ? order[1] : order[0];
+    const R ev = lam[pick];
+    const R pc[2] = {V[pick][0], V[pick][1]};
+    const R off = std::min(R(1), R(0.5) * std::sqrt(ev));
+    const R px = pc[0] * off, py = pc[1] * off;
+    const R z = std::sqrt(R(1) - off * off);
+    R muA[3], muB[3];
+    for (int c = 0; c < 3; ++c) {
+        muA[c] = px * fx[c] + py * fy[c] + z * m.mu[comp][c];
+        muB[c] = -px * fx[c] - py * fy[c] + z * m.mu[comp][c];
+    }
+    m.K = tgt + 1;
+    const R srcR = m.r[comp], srcW = m.w[comp];
+    const R splitW = srcW * R(0.5);
+    const R maxAvgCos = (R)kappaToMeanCosine(maxKappa);
+    const R splitR = z > R(0) ? std::min(maxAvgCos, srcR / z) : maxAvgCos;
+    const R splitK = meanCosineToKappa(splitR);
+    m.setKappaAndR(comp, splitK, splitR);
+    for (int c = 0; c < 3; ++c) m.mu[comp][c] = muA[c];
+    m.w[comp] = splitW;
+    m.setKappaAndR(tgt, splitK, splitR);
+    for (int c = 0; c < 3; ++c) m.mu[tgt][c] = muB[c];
+    m.w[tgt] = splitW;
+    // IncrementalDistance::split: the new component inherits the distance, half the weight each
+    g.dist[tgt] = g.dist[comp];
+    g.distW[comp] = g.distW[tgt] = R(0.5) * g.distW[comp];
+    g.chi[comp] = g.chi[tgt] = g.chiN[comp] = g.chiN[tgt] = 0;
+    for (int c = 0; c < 3; ++c) g.cov[comp][c] = g.cov[tgt][c] = 0;
+    g.covW[comp] = g.covW[tgt] = 0;
+}
+
+// VMMFactory::maskedFit (225-297): MAP-EM on the masked components only;
+// their total weight is kept, the others are left as they are.
+template <class R> void maskedFit(const Props &P, Pmm<R> &m, const std::vector<Sample> &s, const bool *mask) {
+    const int K = m.K;
+    R oldActive = 0;
+    for (int k = 0; k < K; ++k)
+        if (mask[k]) oldActive += removeWeightPrior(P, m.w[k], K);
+    const uint32_t maxItr = s.size() > (size_t)(K * 2) ? P.maxItr : 0;
+    R sumWeight = 0, ll = 0, lastLL = 0, absInvLastLL = 0;
+    const R maxMeanCosine = kappaToMeanCosine((R)P.maxKappa);
+    for (uint32_t i = 0; i < maxItr; ++i) {
+        R sw = 0;
+        Stats<R> st = statsFromSamples(m, s, i == 0 ? &sw : nullptr, i >= P.minItr ? &ll : nullptr);
+        if (i == 0) sumWeight = sw;
+        R activeSum = 0;
+        for (int k = 0; k < K; ++k)
+            if (mask[k]) activeSum += st.W[k];
+        const R norm = oldActive * (sumWeight / activeSum);
+        for (int k = 0; k < K; ++k) {
+            if (!mask[k]) continue;
+            for (int c = 0; c < 3; ++c) st.rW[k][c] *= norm;
+            st.W[k] *= norm;
+        }
+        for (int k = 0; k < K; ++k) { // maskedParameterUpdate (458-486)
+            if (!mask[k]) continue;
+            const R mixtureWeight = applyWeightPrior(P, st.W[k] / sumWeight, K);
+            const R rLength = std::sqrt(st.rW[k][0] * st.rW[k][0] + st.rW[k][1] * st.rW[k][1] + st.rW[k][2] * st.rW[k][2]);
+            if (rLength > R(0))
+                for (int c = 0; c < 3; ++c) m.mu[k][c] = st.rW[k][c] / rLength;
+            const R avgCosine = rLength / st.W[k];
+            R withPrior = applyAvgCosinePrior(P, avgCosine, (R)m.totalNumSamples * mixtureWeight);
+            withPrior = std::isnan(withPrior) ? maxMeanCosine : std::min(withPrior, maxMeanCosine);
+            m.w[k] = mixtureWeight;
+            m.setKappaAndR(k, meanCosineToKappa(withPrior), withPrior);
+        }
+        if (i >= P.minItr) {
+            if (i > P.minItr) {
+                const R rel = (ll - lastLL) * absInvLastLL;
+                if (rel < (R)P.relLogLikelihoodThreshold) break;
+            }
+            lastLL = ll;
+            absInvLastLL = R(1) / std::fabs(ll);
+        }
+    }
+}
+
+// PathGuiding::splitAll (563-632): split every component whose weighted
+// chi-square divergence reaches splitMinDivergence (largest first, up to the
+// free slots), refit the split components, repeat until none qualifies.
+template <class R>
+uint32_t splitAll(const Props &P, const SplitMergeProps &SP, Region<R> &g, const std::vector<Sample> &s, bool fit) {
+    Pmm<R> &m = g.pmm;
+    if (m.K >= kMaxK) return 0;
+    const bool firstFit = s.size() == m.totalNumSamples;
+    uint32_t total = 0;
+    for (;;) {
+        std::vector<std::pair<R, int>> c;
+        for (int k = 0; k < m.K; ++k) {
+            const bool seenEnough = g.chiN[k] > (R)SP.minSamplesForSplitting;
+            const R wd = (firstFit || seenEnough) ? chiDivergence(g, k) * m.w[k] : R(0);
+            c.push_back({std::fabs(wd), k});
+        }
+        const size_t nCand = partitionLikeLibstdcxx(c, [&](const std::pair<R, int> &x) {
+            return x.first >= (R)SP.splitMinDivergence;
+        });
+        const uint32_t numSplits = (uint32_t)std::min<size_t>(nCand, (size_t)(kMaxK - m.K));
+        if (numSplits == 0) break;
+        if (nCand > numSplits)
+            std::partial_sort(c.begin(), c.begin() + numSplits, c.begin() + nCand,
+                              [](const std::pair<R, int> &x, const std::pair<R, int> &y) { return x.first > y.first; });
+        bool mask[kMaxK] = {};
+        for (uint32_t i = 0; i < numSplits; ++i) {
+            mask[c[i].second] = true;
+            mask[m.K] = true;
+            splitComponent(g, c[i].second, P.maxKappa);
+        }
+        if (fit) {
+            pmmApplyPrior(m, P.vPrior);
+            maskedFit(P, m, s, mask);
+            chiUpdate(g, s, mask);
+            covUpdate(g, s, mask);
+            pmmRemovePrior(m, P.vPrior);
+        }
+        total += numSplits;
+    }
+    return total;
+}
+
+// postFit with splitAndMerge (src/PathGuiding.cpp:414-441)
+template <class R> void splitAndMerge(const Props &P, const SplitMergeProps &SP, Region<R> &g, const std::vector<Sample> &s) {
+    Pmm<R> &m = g.pmm;
+    g.samplesSinceLastMerge += (uint32_t)s.size();
+    if (g.samplesSinceLastMerge > SP.minSamplesForMerging) {
+        pmmRemovePrior(m, P.vPrior);
+        mergeAll(SP, g);
+        pmmApplyPrior(m, P.vPrior);
+        g.samplesSinceLastMerge = 0;
+    }
+    chiUpdate(g, s, nullptr);
+    covUpdate(g, s, nullptr);
+    const bool firstFit = s.size() == m.totalNumSamples;
+    const bool fit = firstFit || s.size() > (size_t)SP.minSamplesForPostSplitFitting;
+    pmmRemovePrior(m, P.vPrior);
+    splitAll(P, SP, g, s, fit);
+    pmmApplyPrior(m, P.vPrior);
+}
+
 template <class R> void toVmm(const Region<R> &g, bool parallax, rtx_vmm_theta &v) { // syncPMMsToVMM_Thetas
     v = rtx_vmm_theta{};
     v.usedDistributions = g.pmm.K;
@@ -305,6 +882,7 @@ template <class R> void toVmm(const Region<R> &g, bool parallax, rtx_vmm_theta &
 
 struct orc_guide {
     Props props;
+    SplitMergeProps sm;
     int prec = 64;
     bool parallax = true;
     bool firstFit = true;
@@ -334,6 +912,7 @@ template <class R> void updateAll(orc_guide *g, const rtx_directional_data *dd, 
         }
         x.weight = d.weight;
         x.distance = d.distance;
+        x.pdf = d.pdf;
         per[d.flags].push_back(x);
     }
     for (size_t r = 0; r < regs.size(); ++r) {
@@ -370,6 +949,7 @@ template <class R> void updateAll(orc_guide *g, const rtx_directional_data *dd, 
             }
         }
         emFit(g->props, reg.pmm, s, !g->firstFit);
+        if (g->sm.splitAndMerge) splitAndMerge(g->props, g->sm, reg, s);
         if (g->parallax) updateDistances(reg, s);
     }
     g->firstFit = false;
@@ -448,5 +1028,49 @@ int orc_guide_update(orc_guide *g, const rtx_directional_data *samples, size_t n
 }
 
 float orc_fastexp(float x) { return fastexp<float>(x); }
+
+int orc_guide_set_options(orc_guide *g, const rtx_guiding_options *o) {
+    if (!g || !o) {
+        g_guideErr = "NULL argument";
+        return RTX_E_INVALID;
+    }
+    g->sm.splitAndMerge = o->splitAndMerge != 0;
+    g->sm.minSamplesForMerging = o->minSamplesForMerging;
+    g->sm.minSamplesForSplitting = o->minSamplesForSplitting;
+    g->sm.minSamplesForPostSplitFitting = o->minSamplesForPostSplitFitting;
+    g->sm.splitMinDivergence = o->splitMinDivergence;
+    g->sm.mergeMaxDivergence = o->mergeMaxDivergence;
+    return RTX_OK;
+}
+
+// Test hooks (FP64): lobes as 8 doubles (weight, mu xyz, kappa, r, norm, e^-2kappa).
+static Lobe<double> lobeFrom(const double *v) {
+    return Lobe<double>{v[0], {v[1], v[2], v[3]}, v[4], v[5], v[6], v[7]};
+}
+static void lobeTo(const Lobe<double> &l, double *v) {
+    v[0] = l.w, v[1] = l.mu[0], v[2] = l.mu[1], v[3] = l.mu[2], v[4] = l.kappa, v[5] = l.r, v[6] = l.norm, v[7] = l.e2k;
+}
+void orc_vmf_combine(const double a[8], const double b[8], int divide, double out[8]) {
+    lobeTo(lobeCombine(lobeFrom(a), lobeFrom(b), divide != 0), out);
+}
+void orc_vmf_merge(const double a[8], const double b[8], double out[8]) {
+    lobeTo(lobeMerge(lobeFrom(a), lobeFrom(b)), out);
+}
+double orc_merge_metric(const double a[8], const double b[8]) {
+    Pmm<double> m;
+    m.K = 2;
+    const Lobe<double> la = lobeFrom(a), lb = lobeFrom(b);
+    const Lobe<double> *l[2] = {&la, &lb};
+    for (int k = 0; k < 2; ++k) {
+        m.w[k] = l[k]->w, m.kappa[k] = l[k]->kappa, m.r[k] = l[k]->r, m.norm[k] = l[k]->norm, m.e2k[k] = l[k]->e2k;
+        for (int c = 0; c < 3; ++c) m.mu[k][c] = l[k]->mu[c];
+    }
+    return mergeMetric(m, 0, 1);
+}
+void orc_eigen2x2(const double a[4], double lam[2], double vec[4]) {
+    double V[2][2];
+    eigen2x2(a[0], a[1], a[2], a[3], lam, V);
+    vec[0] = V[0][0], vec[1] = V[0][1], vec[2] = V[1][0], vec[3] = V[1][1];
+}
 
 } // extern "C"

@@ -131,6 +131,19 @@ uint32_t orc_guide_regions(const orc_guide *g, rtx_aabb *boxes, rtx_vmm_theta *v
 int orc_guide_update(orc_guide *g, const rtx_directional_data *samples, size_t n);
 const char *orc_guide_last_error(void);
 float orc_fastexp(float x);
+/* postFit's split/merge switches (default: the reference's). */
+int orc_guide_set_options(orc_guide *g, const rtx_guiding_options *options);
+/* Test hooks in FP64.  A lobe is 8 doubles: weight, mu xyz, kappa, mean
+ * cosine r, norm, e^-2kappa.  orc_vmf_combine: VMFKernel::product (divide =
+ * 0) or ::division (1) of a by b; orc_vmf_merge: ::mergeComponent;
+ * orc_merge_metric: the chi-square merge metric of the pair;
+ * orc_eigen2x2: eigenvalues and eigenvector matrix (row-major, columns are
+ * the vectors) of the 2x2 matrix a (row-major) as Eigen's EigenSolver
+ * computes them. */
+void orc_vmf_combine(const double a[8], const double b[8], int divide, double out[8]);
+void orc_vmf_merge(const double a[8], const double b[8], double out[8]);
+double orc_merge_metric(const double a[8], const double b[8]);
+void orc_eigen2x2(const double a[4], double lam[2], double vec[4]);
 
 /* Building blocks, exposed for known-answer and distribution tests. */
 uint32_t orc_tea(uint32_t v0, uint32_t v1);

@@ -12,6 +12,9 @@ import pytest
 from rtxamd import abi
 from oracle.binding import OracleGuide, lib
 
+# the MAP-EM alone; postFit's split and merge have their own tests
+EM_ONLY = abi.guiding_options(splitAndMerge=0)
+
 
 def sample_vmf(mu, kappa, n, rng):
     mu = np.asarray(mu, float)
@@ -79,7 +82,7 @@ def test_fit_keeps_the_first_moment_and_invariants(prec):
     rng = np.random.default_rng(1)
     d = np.concatenate([sample_vmf((0.3, 0.5, 0.8), 20.0, 12000, rng), sample_vmf((-1, 0, 0.2), 4.0, 8000, rng)])
     w = rng.uniform(0.5, 2.0, len(d))
-    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=False, prec=prec)
+    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=False, prec=prec, options=EM_ONLY)
     g.update(samples(d, weight=w))
     pis, ks, mus = mixture(g.regions()[1][0])
     assert pis.sum() == pytest.approx(1.0, abs=1e-5)
@@ -96,7 +99,7 @@ def test_fit_of_one_lobe_concentrates_on_it():
     rng = np.random.default_rng(2)
     mu = np.array([0.0, 0.6, 0.8])
     d = sample_vmf(mu, 50.0, 20000, rng)
-    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=False)
+    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=False, options=EM_ONLY)
     g.update(samples(d))
     pis, ks, mus = mixture(g.regions()[1][0])
     # most of the mass sits in lobes aligned with mu, as concentrated as the source
@@ -111,7 +114,7 @@ def test_update_fit_accumulates():
     rng = np.random.default_rng(3)
     a = sample_vmf((0, 0, 1), 4.0, 6000, rng)
     b = sample_vmf((0.6, 0.0, 0.8), 4.0, 6000, rng)
-    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=False)
+    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=False, options=EM_ONLY)
     g.update(samples(a))
     pis, ks, mus = mixture(g.regions()[1][0])
     first = (pis[:, None] * mean_cosine(ks)[:, None] * mus).sum(0)
@@ -127,7 +130,7 @@ def test_uncovered_samples_are_skipped():
     (SoftAssignmentWeights::valid): a mixture fitted to one pole ignores a
     batch from the other."""
     rng = np.random.default_rng(5)
-    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=False)
+    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=False, options=EM_ONLY)
     g.update(samples(sample_vmf((0, 0, 1), 200.0, 5000, rng)))
     before = mixture(g.regions()[1][0])
     g.update(samples(sample_vmf((0, 0, -1), 200.0, 5000, rng)))
@@ -138,7 +141,7 @@ def test_uncovered_samples_are_skipped():
 def test_parallax_targets_find_a_point_light():
     rng = np.random.default_rng(4)
     light = np.array([0.2, -0.3, 3.0])
-    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=True)
+    g = OracleGuide(abi, ([-1, -1, -1], [1, 1, 1]), 0, parallax=True, options=EM_ONLY)
     for it in range(3):
         pos = rng.uniform(-0.9, 0.9, (8000, 3))
         v = light - pos
