@@ -89,6 +89,7 @@ struct rtx_ctx {
     std::vector<void *> fitAllocs, fitWorkAllocs;
     GuideFitBuffers fit{};
     bool fitValid = false, fitParallax = true, fitFirst = true;
+    rtx_guiding_options guideOpts{1, 8192, 4096, 4096, 0.5f, 0.025f}; // src/PathGuiding.h:124-129
     // frames of several segments: per-path frame state, split records
     std::vector<void *> segAllocs, splitAllocs;
     size_t segCap = 0, splitCap = 0;
@@ -1178,8 +1179,8 @@ static int guideUpdateAsync(rtx_ctx *ctx) {
     if (ctx->dirCount > (size_t)UINT32_MAX) return fail(ctx, RTX_E_INVALID, "too many guiding samples");
     int rc = guideFitWork(ctx, ctx->dirCount);
     if (rc != RTX_OK) return rc;
-    launchGuideUpdate(ctx->stream, ctx->fit, ctx->dirData, (uint32_t)ctx->dirCount, guideProps(), ctx->fitParallax,
-                      ctx->fitFirst, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
+    launchGuideUpdate(ctx->stream, ctx->fit, ctx->dirData, (uint32_t)ctx->dirCount, guideProps(), ctx->guideOpts,
+                      ctx->fitParallax, ctx->fitFirst, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
     CHECK_HIP(ctx, hipGetLastError());
     ctx->fitFirst = false;
     ctx->guideVmmsStale = true;
@@ -1194,6 +1195,22 @@ int rtx_guiding_update(rtx_ctx *ctx) {
     int rc = guideUpdateAsync(ctx);
     if (rc != RTX_OK) return rc;
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RTX_OK;
+}
+
+int rtx_guiding_set_options(rtx_ctx *ctx, const rtx_guiding_options *options) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!options) return fail(ctx, RTX_E_INVALID, "NULL options");
+    if (!(options->splitMinDivergence >= 0.0f) || !(options->mergeMaxDivergence >= 0.0f))
+        return fail(ctx, RTX_E_INVALID, "divergence thresholds must be >= 0");
+    ctx->guideOpts = *options;
+    return RTX_OK;
+}
+
+int rtx_guiding_get_options(rtx_ctx *ctx, rtx_guiding_options *options) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!options) return fail(ctx, RTX_E_INVALID, "NULL options");
+    *options = ctx->guideOpts;
     return RTX_OK;
 }
 

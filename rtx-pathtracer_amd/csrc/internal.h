@@ -115,6 +115,12 @@ struct GuideRegion {
     float mu[RTX_MAX_DISTRIBUTIONS][3];
     float dist[RTX_MAX_DISTRIBUTIONS], distW[RTX_MAX_DISTRIBUTIONS];
     float mean[3], lastMean[3];
+    // postFit's statistics (external/guiding/incrementalpearsonchisquared.h,
+    // incrementalcovariance2d.h): divergence + 1 times the squared integral and
+    // its sample count; tangent-plane (var x, var y, cov xy) and weight sum
+    float chi[RTX_MAX_DISTRIBUTIONS], chiN[RTX_MAX_DISTRIBUTIONS];
+    float cov[RTX_MAX_DISTRIBUTIONS][3], covW[RTX_MAX_DISTRIBUTIONS];
+    uint32_t samplesSinceLastMerge;
 };
 
 // lightpmm factory properties (VMMFactory.h:45-61 as PathGuiding sets them)
@@ -140,7 +146,8 @@ struct GuideFitBuffers {
 void launchGuideInit(hipStream_t s, GuideFitBuffers g, uint32_t numInitialComponents, float initKappa);
 // PathGuiding::update on the recorded samples (slots = pixels * 16)
 void launchGuideUpdate(hipStream_t s, GuideFitBuffers g, const rtx_directional_data *dd, uint32_t slots,
-                       EmProps props, bool parallax, bool firstFit, rtx_vmm_theta *vmmsOut);
+                       EmProps props, const rtx_guiding_options &sm, bool parallax, bool firstFit,
+                       rtx_vmm_theta *vmmsOut);
 void launchGuideToVmm(hipStream_t s, GuideFitBuffers g, bool parallax, rtx_vmm_theta *vmmsOut);
 
 // ------------------------------------------------------ irradiance cache

@@ -102,6 +102,8 @@ def device_lib():
         L.rtx_guiding_read_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.rtx_guiding_init.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.c_uint32, C.c_int]
         L.rtx_guiding_update.argtypes = [C.c_void_p]
+        L.rtx_guiding_set_options.argtypes = [C.c_void_p, C.POINTER(abi.GuidingOptions)]
+        L.rtx_guiding_get_options.argtypes = [C.c_void_p, C.POINTER(abi.GuidingOptions)]
         L.rtx_estimate_write.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
         L.rtx_estimate_read.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
@@ -513,6 +515,17 @@ class Renderer:
     def ic_write(self, state):
         hdr, ent, sph = state.ctypes_views()
         self._check(device_lib().rtx_ic_write(self._h, hdr, ent, sph))
+
+    def guiding_options(self):
+        o = abi.GuidingOptions()
+        self._check(device_lib().rtx_guiding_get_options(self._h, C.byref(o)))
+        return o
+
+    def set_guiding_options(self, options=None, **kw):
+        """PathGuiding's split/merge switches: an abi.GuidingOptions, or
+        keyword overrides of the reference's defaults."""
+        o = options if options is not None else abi.guiding_options(**kw)
+        self._check(device_lib().rtx_guiding_set_options(self._h, C.byref(o)))
 
     def estimate_write(self, rgba, width, height):
         """The ADRRS pixel estimate (RGBA float32 per pixel) that frames with

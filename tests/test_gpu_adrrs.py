@@ -62,7 +62,8 @@ def test_multi_sample_frames_match_oracle(renderer, cornell, spp):
     ids = np.arange(W * H, dtype=np.uint64)
     orgb, osig = orc.render_paths(fp, ids)
     frgb, fsig = orc.render_paths(fp, ids, prec=32)
-    check_paths(f"{spp} camera samples per frame", rgb, sig, orgb, osig, (frgb, fsig), abs_same=0.95,
+    # a frame of spp samples diverges when any of its paths does
+    check_paths(f"{spp} camera samples per frame", rgb, sig, orgb, osig, (frgb, fsig), abs_same=1 - 0.02 * spp,
                 abs_within=0.99)
     ds.close()
 
@@ -145,7 +146,9 @@ def test_adrrs_image_mean_equals_plain(renderer):
     ds = renderer.upload(hs)
     w = h = 128
     frames = 32
-    mk = lambda sample, n, **kw: scenes.params(hs, w, h, n, sample_offset=sample, maxDepth=8, enableMIS=1, **kw)
+    def mk(sample, n, **kw):
+        return scenes.params(hs, w, h, n, sample_offset=sample, **dict(dict(maxDepth=8, enableMIS=1), **kw))
+
     renderer.ic_create(5000)
     renderer.render(ds, mk(0, 30, useIrradianceCache=1, useIrradianceCacheOnGlossy=1, isIrradiancePrepareFrame=1,
                            irradianceCreateProb=0.01))

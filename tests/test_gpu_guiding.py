@@ -158,49 +158,60 @@ def fib_sphere(n):
     return np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
 
 
-def test_guiding_update_matches_oracle_em(renderer):
-    """Two training frames: the device's region mixtures after each update
-    against the oracle's MAP-EM on the same recorded samples (first update:
-    fit, second: updateFit).  Compared as densities over the sphere, from
-    each region's mean position."""
+@pytest.mark.parametrize("split_merge", [0, 1])
+def test_guiding_update_matches_oracle_em(renderer, split_merge):
+    """Three training frames: the device's region mixtures after each update
+    against the oracle's on the same recorded samples (first update: fit,
+    later: updateFit), with and without postFit's component merge and split.
+    Compared as densities over the sphere from each region's mean position,
+    and by component count; the FP32 run of the oracle is the yardstick."""
     from oracle.binding import OracleGuide
     hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
     ds = renderer.upload(hs)
     box = ds.bounds()
+    opts = abi.guiding_options(splitAndMerge=split_merge)
+    renderer.set_guiding_options(opts)
     renderer.guiding_init(box, 3, parallax=True)
-    # the device update does no component split/merge yet
-    off = abi.guiding_options(splitAndMerge=0)
-    og = OracleGuide(abi, box, 3, parallax=True, prec=64, options=off)
-    o32 = OracleGuide(abi, box, 3, parallax=True, prec=32, options=off)
+    og = OracleGuide(abi, box, 3, parallax=True, prec=64, options=opts)
+    o32 = OracleGuide(abi, box, 3, parallax=True, prec=32, options=opts)
     w = h = 128
     dirs = fib_sphere(4000)
-    for frame in range(2):
-        fp = scenes.params(hs, w, h, 1, maxDepth=8, enableMIS=1, updateGuiding=1, useGuiding=frame,
-                           guidingProb=0.5, sample_offset=frame)
-        renderer.render(ds, fp)  # records the frame, then updates the mixtures
-        dd = renderer.guiding_read_samples(w, h)
-        og.update(dd)
-        o32.update(dd)
-        boxes, gv = renderer.guiding_get()
-        _, ov = og.regions()
-        _, fv = o32.regions()
-        errs, errs32 = [], []
-        for r in range(len(gv)):
-            if (dd["flags"] == r).sum() < 50:
-                continue
-            x = np.array(ov[r].meanPosition[:], np.float64)
-            pg, po, pf = (mixture_pdf_grid(v, dirs, x) for v in (gv[r], ov[r], fv[r]))
-            errs.append(np.abs(pg - po).mean() / po.mean())
-            errs32.append(np.abs(pf - po).mean() / po.mean())
-            assert gv[r].usedDistributions == ov[r].usedDistributions
-            assert np.allclose(gv[r].meanPosition[:], ov[r].meanPosition[:], atol=1e-6)
-        errs, errs32 = np.array(errs), np.array(errs32)
-        log(f"update {frame}: {len(errs)} regions; density L1 vs FP64 oracle: device median {np.median(errs):.2e} "
-            f"max {errs.max():.2e}; FP32 oracle median {np.median(errs32):.2e} max {errs32.max():.2e}")
-        assert np.median(errs) <= max(1e-3, 2 * np.median(errs32))
-        assert errs.max() <= max(0.05, 2 * errs32.max())
-    renderer.guiding_set([], [])
-    ds.close()
+    try:
+        for frame in range(3):
+            fp = scenes.params(hs, w, h, 1, maxDepth=8, enableMIS=1, updateGuiding=1, useGuiding=int(frame > 0),
+                               guidingProb=0.5, sample_offset=frame)
+            renderer.render(ds, fp)  # records the frame, then updates the mixtures
+            dd = renderer.guiding_read_samples(w, h)
+            og.update(dd)
+            o32.update(dd)
+            boxes, gv = renderer.guiding_get()
+            _, ov = og.regions()
+            _, fv = o32.regions()
+            errs, errs32, sameK, sameK32 = [], [], [], []
+            for r in range(len(gv)):
+                if (dd["flags"] == r).sum() < 50:
+                    continue
+                x = np.array(ov[r].meanPosition[:], np.float64)
+                pg, po, pf = (mixture_pdf_grid(v, dirs, x) for v in (gv[r], ov[r], fv[r]))
+                errs.append(np.abs(pg - po).mean() / po.mean())
+                errs32.append(np.abs(pf - po).mean() / po.mean())
+                sameK.append(gv[r].usedDistributions == ov[r].usedDistributions)
+                sameK32.append(fv[r].usedDistributions == ov[r].usedDistributions)
+                assert np.allclose(gv[r].meanPosition[:], ov[r].meanPosition[:], atol=1e-6)
+            errs, errs32 = np.array(errs), np.array(errs32)
+            ks = [(gv[r].usedDistributions, ov[r].usedDistributions) for r in range(len(gv))]
+            log(f"update {frame} (split/merge {split_merge}): {len(errs)} regions, K device/oracle {ks}; density L1 vs "
+                f"FP64 oracle: device median {np.median(errs):.2e} max {errs.max():.2e}; FP32 oracle median "
+                f"{np.median(errs32):.2e} max {errs32.max():.2e}")
+            assert np.mean(sameK) >= np.mean(sameK32) - 0.13
+            assert np.median(errs) <= max(1e-3, 2 * np.median(errs32))
+            assert errs.max() <= max(0.05, 2 * errs32.max())
+            if not split_merge:
+                assert all(sameK)
+    finally:
+        renderer.set_guiding_options()
+        renderer.guiding_set([], [])
+        ds.close()
 
 
 def test_trained_guiding_is_unbiased_and_reduces_variance(renderer):

@@ -280,7 +280,7 @@ def test_pixel_range_batches_match(renderer):
 def test_error_codes(renderer):
     hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
     ds = renderer.upload(hs)
-    for feat in ("useADRRS", "splitOnFirst", "visualizeMode"):
+    for feat in ("visualizeMode", "showIrradianceCacheOnly"):
         fp = scenes.params(hs, 8, 8, 1, **{feat: 1})
         with pytest.raises(rtxamd.RtxError) as e:
             renderer.render(ds, fp)
