@@ -312,10 +312,13 @@ template <bool ANY> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
 }
 
 // Drives the persistent loop.  load(idx, L) fills L.r/L.tmin/L.tmax;
-// store(L) consumes a finished ray.  `fetch` must be zero at launch.  A
-// wave refills when at least kRefillLanes of its lanes are idle (or all
-// are), with one atomic for the whole wave; once the queue is drained the
-// wave stops fetching, so `fetch` ends below n + 64 * waves.
+// store(L) consumes a finished ray.  `fetch` must be zero at launch.  Each
+// wave first takes its own 64 rays of the grid's static span (thread i of
+// the grid: ray i) without an atomic, so a launch over a short or empty queue
+// costs no contention; after that a wave refills when at least kRefillLanes
+// of its lanes are idle (or all are), with one atomic for the whole wave on
+// the rays past the static span.  Once the queue is drained the wave stops
+// fetching, so `fetch` ends below n + 64 * waves.
 constexpr int kRefillLanes = 16;
 
 template <bool ANY, bool COUNT, class Stack, class Load, class Store>
@@ -326,6 +329,9 @@ RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stac
     L.idx = 0;
     bool active = false;
     bool drained = false; // wave-uniform
+    bool first = true;    // wave-uniform
+    const uint32_t staticSpan = gridDim.x * blockDim.x; // blockDim.x is a multiple of 64
+    const uint32_t waveBase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
     for (;;) {
         // refill idle lanes (the whole wave is converged here)
         const bool idle = !active;
@@ -334,11 +340,17 @@ RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stac
         const bool anyActive = nIdle < 64; // gfx950 waves are 64 wide; blocks are whole waves
         if (!drained && (nIdle >= kRefillLanes || !anyActive)) {
             const int lane = (int)__lane_id();
-            const int leader = __ffsll((unsigned long long)m) - 1;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(fetch, (uint32_t)nIdle);
-            base = __shfl(base, leader);
-            if (base + (uint32_t)nIdle >= n) drained = true;
+            if (first) { // every lane idle: the wave's static slice
+                first = false;
+                base = waveBase;
+                if (staticSpan >= n) drained = true;
+            } else {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                if (lane == leader) base = atomicAdd(fetch, (uint32_t)nIdle);
+                base = staticSpan + __shfl(base, leader);
+                if (base + (uint32_t)nIdle >= n) drained = true;
+            }
             if (idle) {
                 const uint64_t below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
                 const uint32_t idx = base + (uint32_t)__popcll(below);
