@@ -479,7 +479,7 @@ int rtx_guiding_init(rtx_ctx *ctx, const rtx_aabb *sceneBox, uint32_t splitCount
  * context owns the mixtures.  With options.splitAndMerge (the default) the
  * fit is followed by postFit's component merge and split. */
 int rtx_guiding_update(rtx_ctx *ctx);
-/* PathGuiding's public fitting switches (src/PathGuiding.h:124-129), with
+/* PathGuiding's public fitting switches (src/PathGuiding.h:118-129), with
  * their defaults: postFit merges pairs whose chi-square merge metric is at
  * most mergeMaxDivergence once more than minSamplesForMerging samples came
  * in since the last merge, and splits components whose weighted chi-square
@@ -492,6 +492,13 @@ typedef struct rtx_guiding_options {
     uint32_t minSamplesForPostSplitFitting; /* 4096 */
     float splitMinDivergence;               /* 0.5 */
     float mergeMaxDivergence;               /* 0.025 */
+    /* region splitting (PathGuiding::update, src/PathGuiding.cpp:295-301,
+     * splitRegion 328-348): after an update every region whose mixture has
+     * seen more than samplesForRegionSplit samples is halved along the largest
+     * axis of its box; both halves keep its mixture and statistics, with the
+     * sample counts decayed by 0.25, the new half appended as the last region */
+    int32_t splitRegions;                   /* 0 */
+    uint32_t samplesForRegionSplit;         /* 10000 */
 } rtx_guiding_options;
 int rtx_guiding_set_options(rtx_ctx *ctx, const rtx_guiding_options *options);
 int rtx_guiding_get_options(rtx_ctx *ctx, rtx_guiding_options *options);

@@ -294,6 +294,8 @@ struct SplitMergeProps {
     bool splitAndMerge = true;
     uint32_t minSamplesForMerging = 8192, minSamplesForSplitting = 4096, minSamplesForPostSplitFitting = 4096;
     float splitMinDivergence = 0.5f, mergeMaxDivergence = 0.025f;
+    bool splitRegions = false; // src/PathGuiding.h:118-119
+    uint32_t samplesForRegionSplit = 10000;
 };
 
 // PMM::removeWeightPrior / applyWeightPrior (ParametricMixtureModel.h:339-375):
@@ -953,6 +955,25 @@ template <class R> void updateAll(orc_guide *g, const rtx_directional_data *dd, 
         if (g->parallax) updateDistances(reg, s);
     }
     g->firstFit = false;
+    // region splitting (src/PathGuiding.cpp:295-301, splitRegion 328-348)
+    if (g->sm.splitRegions) {
+        const size_t count = regs.size();
+        for (size_t r = 0; r < count; ++r) {
+            if (!(regs[r].pmm.numSamples > (R)g->sm.samplesForRegionSplit)) continue;
+            const rtx_aabb b = g->boxes[r];
+            const float size[3] = {b.max[0] - b.min[0], b.max[1] - b.min[1], b.max[2] - b.min[2]};
+            const int axis = size[0] > size[1] ? (size[0] > size[2] ? 0 : 2) : (size[1] > size[2] ? 1 : 2);
+            rtx_aabb left = b, right = b;
+            left.max[axis] -= 0.5f * size[axis];
+            right.min[axis] += 0.5f * size[axis];
+            g->boxes[r] = left;
+            g->boxes.push_back(right);
+            regs[r].pmm.numSamples *= R(0.25f);
+            regs[r].pmm.sampleWeight *= R(0.25f);
+            const Region<R> copy = regs[r];
+            regs.push_back(copy);
+        }
+    }
 }
 } // namespace
 
@@ -1040,6 +1061,8 @@ int orc_guide_set_options(orc_guide *g, const rtx_guiding_options *o) {
     g->sm.minSamplesForPostSplitFitting = o->minSamplesForPostSplitFitting;
     g->sm.splitMinDivergence = o->splitMinDivergence;
     g->sm.mergeMaxDivergence = o->mergeMaxDivergence;
+    g->sm.splitRegions = o->splitRegions != 0;
+    g->sm.samplesForRegionSplit = o->samplesForRegionSplit;
     return RTX_OK;
 }
 

@@ -89,7 +89,7 @@ struct rtx_ctx {
     std::vector<void *> fitAllocs, fitWorkAllocs;
     GuideFitBuffers fit{};
     bool fitValid = false, fitParallax = true, fitFirst = true;
-    rtx_guiding_options guideOpts{1, 8192, 4096, 4096, 0.5f, 0.025f}; // src/PathGuiding.h:124-129
+    rtx_guiding_options guideOpts{1, 8192, 4096, 4096, 0.5f, 0.025f, 0, 10000}; // src/PathGuiding.h:118-129
     // frames of several segments: per-path frame state, split records
     std::vector<void *> segAllocs, splitAllocs;
     size_t segCap = 0, splitCap = 0;
@@ -122,6 +122,17 @@ template <class T> hipError_t devAlloc(T **p, size_t count, std::vector<void *> 
         *p = (T *)q;
     }
     return e;
+}
+
+// Replaces one allocation of a context list by a new one of `count` items.
+template <class T> hipError_t reallocIn(std::vector<void *> &list, T *&p, size_t count) {
+    auto it = std::find(list.begin(), list.end(), (void *)p);
+    if (it != list.end()) {
+        (void)hipFree(*it);
+        list.erase(it);
+    }
+    p = nullptr;
+    return devAlloc(&p, count, list);
 }
 
 void freeAll(std::vector<void *> &list) {
@@ -1173,8 +1184,57 @@ static EmProps guideProps() { // src/PathGuiding.cpp:33-40 over VMMFactory.h:45-
     return P;
 }
 
+// Region splitting after an update (PathGuiding::update, src/PathGuiding.cpp:
+// 295-301; splitRegion 328-348): a region whose mixture saw more than
+// samplesForRegionSplit samples is halved along the largest axis of its box
+// (Aabb::splitAabb, src/Shapes.h:30-46); its sample counts are decayed by
+// 0.25 and the whole region state is copied to a new last region.  Decided
+// on the host from the regions' states, which are small; the region
+// hierarchy and mixtures are then rebuilt.
+static int guideSplitRegions(rtx_ctx *ctx) {
+    GuideFitBuffers &g = ctx->fit;
+    const uint32_t n = g.numRegions;
+    std::vector<GuideRegion> regs(n);
+    CHECK_HIP(ctx, hipMemcpyAsync(regs.data(), g.regions, n * sizeof(GuideRegion), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<rtx_aabb> boxes = ctx->guideBoxes;
+    bool any = false;
+    for (uint32_t r = 0; r < n; ++r) {
+        if (!(regs[r].numSamples > (float)ctx->guideOpts.samplesForRegionSplit)) continue;
+        if (boxes.size() >= (1u << 20)) return fail(ctx, RTX_E_INVALID, "more than 2^20 guiding regions");
+        const rtx_aabb b = boxes[r];
+        const float size[3] = {b.max[0] - b.min[0], b.max[1] - b.min[1], b.max[2] - b.min[2]};
+        const int axis = size[0] > size[1] ? (size[0] > size[2] ? 0 : 2) : (size[1] > size[2] ? 1 : 2);
+        rtx_aabb left = b, right = b;
+        left.max[axis] -= 0.5f * size[axis];
+        right.min[axis] += 0.5f * size[axis];
+        boxes[r] = left;
+        boxes.push_back(right);
+        regs[r].numSamples *= 0.25f;
+        regs[r].sampleWeight *= 0.25f;
+        const GuideRegion copy = regs[r];
+        regs.push_back(copy);
+        any = true;
+    }
+    if (!any) return RTX_OK;
+    const uint32_t m = (uint32_t)regs.size();
+    std::vector<rtx_vmm_theta> placeholder(m);
+    for (auto &v : placeholder) v.usedDistributions = 1;
+    int rc = guideUpload(ctx, boxes.data(), placeholder.data(), m); // boxes, hierarchy
+    if (rc != RTX_OK) return rc;
+    CHECK_HIP(ctx, reallocIn(ctx->fitAllocs, g.regions, m));
+    CHECK_HIP(ctx, reallocIn(ctx->fitAllocs, g.regionStart, (size_t)m + 1));
+    CHECK_HIP(ctx, hipMemcpy(g.regions, regs.data(), m * sizeof(GuideRegion), hipMemcpyHostToDevice));
+    g.numRegions = m;
+    g.boxes = ctx->guide.boxes;
+    launchGuideToVmm(ctx->stream, g, ctx->fitParallax, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
+    CHECK_HIP(ctx, hipGetLastError());
+    return RTX_OK;
+}
+
 // PathGuiding::update (src/PathGuiding.cpp:276-312) on the samples the last
-// recording frame left (stream-ordered, no host wait).
+// recording frame left (stream-ordered, no host wait unless regions split).
 static int guideUpdateAsync(rtx_ctx *ctx) {
     if (ctx->dirCount > (size_t)UINT32_MAX) return fail(ctx, RTX_E_INVALID, "too many guiding samples");
     int rc = guideFitWork(ctx, ctx->dirCount);
@@ -1183,6 +1243,10 @@ static int guideUpdateAsync(rtx_ctx *ctx) {
                       ctx->fitParallax, ctx->fitFirst, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
     CHECK_HIP(ctx, hipGetLastError());
     ctx->fitFirst = false;
+    if (ctx->guideOpts.splitRegions) {
+        rc = guideSplitRegions(ctx);
+        if (rc != RTX_OK) return rc;
+    }
     ctx->guideVmmsStale = true;
     return RTX_OK;
 }

@@ -176,12 +176,13 @@ class Tuning(C.Structure):
 class GuidingOptions(C.Structure):
     _fields_ = [("splitAndMerge", C.c_int32), ("minSamplesForMerging", C.c_uint32),
                 ("minSamplesForSplitting", C.c_uint32), ("minSamplesForPostSplitFitting", C.c_uint32),
-                ("splitMinDivergence", C.c_float), ("mergeMaxDivergence", C.c_float)]
+                ("splitMinDivergence", C.c_float), ("mergeMaxDivergence", C.c_float),
+                ("splitRegions", C.c_int32), ("samplesForRegionSplit", C.c_uint32)]
 
 
 def guiding_options(**kw):
-    """PathGuiding's defaults (src/PathGuiding.h:124-129), overridden by kw."""
-    o = GuidingOptions(1, 8192, 4096, 4096, 0.5, 0.025)
+    """PathGuiding's defaults (src/PathGuiding.h:118-129), overridden by kw."""
+    o = GuidingOptions(1, 8192, 4096, 4096, 0.5, 0.025, 0, 10000)
     for k, v in kw.items():
         if not hasattr(o, k):
             raise KeyError(k)
@@ -210,7 +211,7 @@ class View(C.Structure):
 EXPECTED_SIZES = {
     Vertex: 48, Material: 96, Instance: 144, Light: 40, FaceSample: 12, Sphere: 24,
     PushConstant: 192, Camera: 256, Ray: 32, Hit: 16, Tuning: 32, CacheHeader: 12, CacheData: 56, IcJob: 32,
-    Aabb: 24, VmfTheta: 40, VmmTheta: 720, GuidingOptions: 24,
+    Aabb: 24, VmfTheta: 40, VmmTheta: 720, GuidingOptions: 32,
 }
 for _t, _n in EXPECTED_SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)

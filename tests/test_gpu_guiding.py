@@ -214,6 +214,44 @@ def test_guiding_update_matches_oracle_em(renderer, split_merge):
         ds.close()
 
 
+def test_region_splitting_matches_oracle(renderer):
+    """splitRegions: busy regions are halved after each update; the device's
+    boxes equal the oracle's and the split regions' mixtures match."""
+    from oracle.binding import OracleGuide
+    hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
+    ds = renderer.upload(hs)
+    box = ds.bounds()
+    opts = abi.guiding_options(splitRegions=1, samplesForRegionSplit=12000)
+    renderer.set_guiding_options(opts)
+    renderer.guiding_init(box, 2, parallax=True)
+    og = OracleGuide(abi, box, 2, parallax=True, prec=64, options=opts)
+    w = h = 96
+    dirs = fib_sphere(2000)
+    try:
+        for frame in range(3):
+            fp = scenes.params(hs, w, h, 1, maxDepth=6, enableMIS=1, updateGuiding=1, useGuiding=int(frame > 0),
+                               guidingProb=0.5, sample_offset=20 + frame)
+            renderer.render(ds, fp)
+            dd = renderer.guiding_read_samples(w, h)
+            og.update(dd)
+            gb, gv = renderer.guiding_get()
+            ob, ov = og.regions()
+            log(f"frame {frame}: {len(gb)} regions on the device, {len(ob)} in the oracle")
+            assert len(gb) == len(ob) and np.array_equal(np.asarray(gb, np.float32), ob)
+            errs = []
+            for r in range(len(gv)):
+                x = np.array(ov[r].meanPosition[:], np.float64)
+                pg, po = mixture_pdf_grid(gv[r], dirs, x), mixture_pdf_grid(ov[r], dirs, x)
+                errs.append(np.abs(pg - po).mean() / po.mean())
+            log(f"  density L1 vs FP64 oracle: median {np.median(errs):.2e} max {np.max(errs):.2e}")
+            assert np.median(errs) < 1e-3 and np.max(errs) < 0.05
+        assert len(gb) > 4  # regions were split
+    finally:
+        renderer.set_guiding_options()
+        renderer.guiding_set([], [])
+        ds.close()
+
+
 def test_trained_guiding_is_unbiased_and_reduces_variance(renderer):
     """After training frames the guided estimator keeps the plain mean and,
     on the plates-and-lights scene, its per-pixel variance drops."""

@@ -218,3 +218,34 @@ def test_split_merge_with_parallax_keeps_distances():
     t = vmm.thetas[k]
     assert t.distance > 0
     assert np.allclose(np.array(t.target[:]), light, atol=0.15 * np.linalg.norm(light)), (t.target[:], light)
+
+
+def test_region_splitting_halves_busy_regions():
+    """splitRegions: a region that saw more than samplesForRegionSplit
+    samples is halved along its largest axis; both halves carry its mixture
+    (with the sample counts decayed) and the new half is appended."""
+    rng = np.random.default_rng(14)
+    opts = abi.guiding_options(splitRegions=1, samplesForRegionSplit=3000)
+    g = OracleGuide(abi, ([0, 0, 0], [4, 2, 1]), 1, parallax=False, options=opts)
+    boxes0, _ = g.regions()
+    assert len(boxes0) == 2
+    d = sample_vmf((0, 0, 1), 20.0, 5000, rng)
+    s = samples(d, region=0)
+    g.update(s)  # region 0 has 5000 > 3000 samples, region 1 none
+    boxes, vmms = g.regions()
+    assert len(boxes) == 3
+    vol = lambda b: np.prod(b[1] - b[0])
+    assert vol(boxes[0]) + vol(boxes[2]) == pytest.approx(vol(boxes0[0]), rel=1e-6)
+    assert np.allclose(boxes[1], boxes0[1])
+    # halves of the old region 0 along its largest axis (x and y tie at 2.00002: y, the later one)
+    size = boxes0[0][1] - boxes0[0][0]
+    axis = 0 if size[0] > size[1] else (1 if size[1] > size[2] else 2)
+    assert axis == 1
+    assert np.allclose(boxes[0][0], boxes0[0][0]) and np.allclose(boxes[2][1], boxes0[0][1])
+    assert boxes[0][1][axis] == pytest.approx(boxes[2][0][axis]) == pytest.approx(boxes0[0][0][axis] + size[axis] / 2)
+    pa, ka, ma = mixture(vmms[0])
+    pb, kb, mb = mixture(vmms[2])
+    assert np.array_equal(pa, pb) and np.array_equal(ka, kb) and np.array_equal(ma, mb)
+    # the decayed counts: a second batch of the same size has four times the old weight in the refit
+    g.update(samples(d, region=2))
+    assert len(g.regions()[0]) == 4  # region 2 split again (1250 + 5000 > 3000)
