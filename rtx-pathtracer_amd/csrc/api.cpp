@@ -868,13 +868,17 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     // path ends within `iters` iterations of its start.  With segments a path
     // slot runs a pixel's whole frame (several camera samples, queued splits),
     // so the loop goes on in rounds of `iters` until the extend queue is empty.
+    // Every kPollEvery iterations the extend queue's length is read back and
+    // the loop ends once it is empty (later iterations would only launch
+    // kernels over empty queues).
+    constexpr int kPollEvery = 4;
     int itersRun = 0;
     auto bounces = [&](const FrameConst *fcd, BatchDesc b, int iters, bool multi) -> int {
         int qsel = 0;
         const int cap = multi ? (int)(samplesPerFrame + kMaxSplits + 1) * iters : iters;
         for (int it = 0;; ++it) {
-            if (it == iters || (multi && it > iters && it % iters == 0)) {
-                if (!multi) break;
+            if (it == cap && !multi) break;
+            if (it > 0 && it % kPollEvery == 0) {
                 uint32_t left = 0;
                 CHECK_HIP(ctx, hipMemcpyAsync(&left, &ctx->cnt->extend[qsel], sizeof(left), hipMemcpyDeviceToHost, s));
                 CHECK_HIP(ctx, hipStreamSynchronize(s));
