@@ -255,6 +255,16 @@ typedef struct rtx_model_desc {
     uint32_t indexCount;
 } rtx_model_desc;
 
+/* An 8-bit RGBA texture, rows top to bottom as stbi_load returns them
+ * (src/SceneLoader.cpp:199): RGB sRGB-encoded, alpha linear, sampled like
+ * the reference's R8G8B8A8_SRGB images with a bilinear, repeating sampler
+ * (src/SceneLoader.cpp:209-227). */
+typedef struct rtx_texture_desc {
+    const uint8_t *rgba;
+    uint32_t width;
+    uint32_t height;
+} rtx_texture_desc;
+
 /* Everything descriptor set 1 bindings 1-8 hold for one scene.  The light
  * sampling tables are the reference's LightSamplerBuffer
  * (shaders/raytrace.rgen:107-110): randomLightIndex[10000] followed by
@@ -281,6 +291,12 @@ typedef struct rtx_scene_desc {
     const float *envMap;
     uint32_t envWidth;
     uint32_t envHeight;
+    /* Material textures by slot, as rtx_material.textureIdDiffuse /
+     * textureIdSpecular number them (binding 7, textureSamplers[]): slot 0 is
+     * the environment map's, given above, so textures[0] is not read.
+     * textureCount = 0 when no material has a texture. */
+    const rtx_texture_desc *textures;
+    uint32_t textureCount;
 } rtx_scene_desc;
 
 /* ------------------------------------------------------ frame params */

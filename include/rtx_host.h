@@ -45,6 +45,15 @@ int rtxh_scene_generate(int kind, uint64_t seed, uint32_t triangles, rtxh_scene 
  * src/SceneLoader.cpp:329-348) with its OBJ shapes. */
 int rtxh_scene_load_xml(const char *path, rtxh_scene **out);
 
+/* Load a scene file the way the reference's SceneLoader does
+ * (src/SceneLoader.cpp:29-63): ".xml" is read as Mitsuba 0.6 with its OBJ,
+ * MTL, texture and EXR environment-map files beside it; ".json" as the
+ * reference's own format ({"models": [{name: obj}], "instances":
+ * [{name: {translate, rotate, scale}}], "lights": [{position, color}],
+ * "camera": {origin, target, up}}) with models/, materials/ and textures/
+ * below the scene's folder.  Light tables are built. */
+int rtxh_scene_load(const char *path, rtxh_scene **out);
+
 void rtxh_scene_destroy(rtxh_scene *scene);
 
 /* Programmatic scene building, in the order SceneLoader adds objects while
@@ -84,6 +93,11 @@ int rtxh_write_exr(const char *path, const float *rgba, uint32_t width, uint32_t
  * *rgba is malloc'ed; free with rtxh_free. */
 int rtxh_read_exr(const char *path, float **rgba, uint32_t *width, uint32_t *height);
 void rtxh_free(void *p);
+
+/* Decode a PNG or baseline JPEG file to width*height 8-bit RGBA texels
+ * (stbi_load(path, ..., 4) of the reference, src/SceneLoader.cpp:199).
+ * *rgba is malloc'ed; free with rtxh_free. */
+int rtxh_decode_image(const char *path, uint8_t **rgba, uint32_t *width, uint32_t *height);
 
 /* PCG32 (O'Neill 2014): first output after seeding with (initstate, initseq).
  * This is how rtx_render derives the per-sample frame seed. */

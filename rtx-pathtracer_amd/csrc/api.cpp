@@ -178,9 +178,12 @@ int validateScene(rtx_ctx *ctx, const rtx_scene_desc *d, uint64_t &numTris) {
         numTris += d->models[in.modelIndex].indexCount / 3;
     }
     if (numTris >= (1ull << 27)) return fail(ctx, RTX_E_UNSUPPORTED, "more than 2^27 triangles");
-    for (uint32_t m = 0; m < d->materialCount; ++m)
+    for (uint32_t m = 0; m < d->materialCount; ++m) {
         if (d->materials[m].type < 0 || d->materials[m].type > RTX_MAT_ROUGH_CONDUCTOR)
             return fail(ctx, RTX_E_INVALID, "unknown material type");
+        if (d->materials[m].textureIdDiffuse != -1 || d->materials[m].textureIdSpecular != -1)
+            return fail(ctx, RTX_E_UNSUPPORTED, "material textures are not supported yet");
+    }
     for (uint32_t s = 0; s < d->sphereCount; ++s) {
         const rtx_sphere &sp = d->spheres[s];
         if (sp.materialIndex < 0 || (uint32_t)sp.materialIndex >= d->materialCount)

@@ -64,6 +64,26 @@ int rtxh_scene_load_xml(const char *path, rtxh_scene **out) {
     return RTX_OK;
 }
 
+int rtxh_scene_load(const char *path, rtxh_scene **out) {
+    if (!path || !out) return fail("NULL argument");
+    *out = nullptr;
+    auto *s = new (std::nothrow) rtxh_scene();
+    if (!s) return fail("out of memory");
+    std::string err;
+    bool ok = false;
+    try {
+        ok = rtxh::loadSceneFile(path, s->scene, err);
+    } catch (const std::exception &e) {
+        err = e.what();
+    }
+    if (!ok) {
+        delete s;
+        return fail(err);
+    }
+    *out = s;
+    return RTX_OK;
+}
+
 void rtxh_scene_destroy(rtxh_scene *scene) { delete scene; }
 
 int rtxh_scene_new(rtxh_scene **out) {
@@ -159,6 +179,19 @@ int rtxh_read_exr(const char *path, float **rgba, uint32_t *width, uint32_t *hei
 }
 
 void rtxh_free(void *p) { std::free(p); }
+
+int rtxh_decode_image(const char *path, uint8_t **rgba, uint32_t *width, uint32_t *height) {
+    if (!path || !rgba || !width || !height) return fail("NULL argument");
+    rtxh::Texture t;
+    std::string err;
+    if (!rtxh::loadImageRgba8(path, t, err)) return fail(err);
+    *rgba = (uint8_t *)std::malloc(t.rgba.size() ? t.rgba.size() : 1);
+    if (!*rgba) return fail("out of memory");
+    std::memcpy(*rgba, t.rgba.data(), t.rgba.size());
+    *width = t.width;
+    *height = t.height;
+    return RTX_OK;
+}
 
 uint32_t rtxh_pcg32_first(uint64_t initstate, uint64_t initseq) {
     rtxh::Pcg32 r(initstate, initseq);

@@ -76,6 +76,10 @@ static void identityInto(float m[16]) {
 }
 
 int Scene::addModelInstance(Model &&model, const float *transform, const float *normalTransform) {
+    return addInstance(addModel(std::move(model)), transform, normalTransform);
+}
+
+int Scene::addModel(Model &&model) {
     // emissive faces: faces whose material is a light (src/SceneLoader.cpp:320-323)
     model.emissiveFaces.clear();
     for (size_t f = 0; f * 3 < model.indices.size(); ++f) {
@@ -84,16 +88,20 @@ int Scene::addModelInstance(Model &&model, const float *transform, const float *
             model.emissiveFaces.push_back((int)f);
     }
     models.push_back(std::move(model));
+    return (int)models.size() - 1;
+}
+
+int Scene::addInstance(int model, const float *transform, const float *normalTransform) {
     rtx_instance inst;
     std::memset(&inst, 0, sizeof(inst));
     identityInto(inst.transform);
     identityInto(inst.normalTransform);
     if (transform) std::memcpy(inst.transform, transform, sizeof(inst.transform));
     if (normalTransform) std::memcpy(inst.normalTransform, normalTransform, sizeof(inst.normalTransform));
-    inst.modelIndex = (int)models.size() - 1;
+    inst.modelIndex = model;
     inst.iLight = -1;
     int instanceIndex = (int)instances.size();
-    if (!models.back().emissiveFaces.empty()) {
+    if (!models[model].emissiveFaces.empty()) {
         rtx_light light;
         std::memset(&light, 0, sizeof(light));
         light.type = RTX_LIGHT_AREA;
@@ -103,6 +111,16 @@ int Scene::addModelInstance(Model &&model, const float *transform, const float *
     }
     instances.push_back(inst);
     return instanceIndex;
+}
+
+int Scene::addTexture(Texture &&texture, const std::string &path) {
+    if (textures.empty()) { // the environment map's slot
+        textures.emplace_back();
+        texturePaths.emplace_back();
+    }
+    textures.push_back(std::move(texture));
+    texturePaths.push_back(path);
+    return (int)textures.size() - 1;
 }
 
 void Scene::addPointLight(Vec3 pos, Vec3 color) { // parsePointLights, src/SceneLoader.cpp:686-702
@@ -240,6 +258,14 @@ void Scene::fillDesc(rtx_scene_desc *d) {
     d->envMap = envMap.empty() ? nullptr : envMap.data();
     d->envWidth = envWidth;
     d->envHeight = envHeight;
+    textureDescs.resize(textures.size());
+    for (size_t i = 0; i < textures.size(); ++i) {
+        textureDescs[i].rgba = textures[i].rgba.empty() ? nullptr : textures[i].rgba.data();
+        textureDescs[i].width = textures[i].width;
+        textureDescs[i].height = textures[i].height;
+    }
+    d->textures = textureDescs.empty() ? nullptr : textureDescs.data();
+    d->textureCount = (uint32_t)textureDescs.size();
 }
 
 static rtx_material blankMaterial() {

@@ -50,6 +50,13 @@ struct Model {
     float faceArea(int face, const Mat4 &transform) const; // Model::getFaceArea, src/Model.cpp:34-56
 };
 
+// An 8-bit RGBA texture as stb_image hands it to the reference
+// (src/SceneLoader.cpp:184-236): RGB sRGB-encoded, alpha linear.
+struct Texture {
+    uint32_t width = 0, height = 0;
+    std::vector<uint8_t> rgba;
+};
+
 struct Scene {
     std::vector<Model> models;
     std::vector<rtx_instance> instances;
@@ -61,15 +68,28 @@ struct Scene {
     uint32_t randomTriRows = 0;
     std::vector<float> envMap;
     uint32_t envWidth = 0, envHeight = 0;
+    // Texture slots as the reference numbers them: slot 0 is the environment
+    // map's (src/SceneLoader.cpp:52-53), material textures from 1 on.
+    std::vector<Texture> textures;
+    std::vector<std::string> texturePaths;
     rtxh_view view{{0, -10, 4}, {0, 0, 4}, {0, 0, 1}, 28.0f}; // src/SceneLoader.h:147-150
 
     // scratch for rtx_scene_desc
     std::vector<rtx_model_desc> modelDescs;
+    std::vector<rtx_texture_desc> textureDescs;
 
     // Append a model as its own instance (identity transform) the way the
     // Mitsuba path adds an "obj" shape (src/SceneLoader.cpp:455-495).
     // Returns the instance index; creates an area light if any face is emissive.
     int addModelInstance(Model &&model, const float *transform = nullptr, const float *normalTransform = nullptr);
+    // A model without an instance (SceneLoader::addModel, src/SceneLoader.cpp:233-243);
+    // returns the model index.
+    int addModel(Model &&model);
+    // An instance of a model; an area light when the model has emissive
+    // faces (src/SceneLoader.cpp:717-728).  Returns the instance index.
+    int addInstance(int model, const float *transform, const float *normalTransform);
+    // Registers a decoded texture; returns its slot (1, 2, ...).
+    int addTexture(Texture &&texture, const std::string &path);
     void addPointLight(Vec3 pos, Vec3 color);
     int addSphere(Vec3 center, float radius, int material);
     int addMaterial(const rtx_material &m);
@@ -95,6 +115,13 @@ bool generateCornell(uint64_t seed, Scene &s, std::string &err);
 bool generateRoom(uint64_t seed, uint32_t triangles, Scene &s, std::string &err);
 bool generatePlates(uint64_t seed, Scene &s, std::string &err);
 bool loadMitsubaXml(const std::string &path, Scene &s, std::string &err);
+bool loadJsonScene(const std::string &path, Scene &s, std::string &err);
+// By extension: ".xml" Mitsuba, ".json" the reference's own format.
+bool loadSceneFile(const std::string &path, Scene &s, std::string &err);
+
+// 8-bit PNG / baseline JPEG decoding to RGBA (4 channels, alpha 255 when the
+// file has none), rows top to bottom as stbi_load returns them.
+bool loadImageRgba8(const std::string &path, Texture &out, std::string &err);
 
 bool cameraMatrices(const rtxh_view &view, uint32_t w, uint32_t h, rtx_camera *cam);
 

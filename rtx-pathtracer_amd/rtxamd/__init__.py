@@ -44,6 +44,9 @@ def host_lib():
         L.rtxh_last_error.restype = C.c_char_p
         L.rtxh_scene_generate.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]
         L.rtxh_scene_load_xml.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.rtxh_scene_load.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.rtxh_decode_image.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint32)]
         L.rtxh_scene_destroy.argtypes = [C.c_void_p]
         L.rtxh_scene_destroy.restype = None
         L.rtxh_scene_desc.argtypes = [C.c_void_p, C.POINTER(SceneDesc)]
@@ -196,6 +199,35 @@ class HostScene:
         _host_check(host_lib().rtxh_scene_load_xml(str(path).encode(), C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def load(cls, path):
+        """A Mitsuba .xml or the reference's .json scene file (rtxh_scene_load)."""
+        h = C.c_void_p()
+        _host_check(host_lib().rtxh_scene_load(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    def textures(self):
+        """Material textures by slot as (height, width, 4) uint8 arrays; slot 0
+        (the environment map's) is None."""
+        d = self.desc()
+        out = []
+        for i in range(d.textureCount):
+            t = d.textures[i]
+            if not t.rgba or t.width == 0:
+                out.append(None)
+                continue
+            a = np.ctypeslib.as_array(t.rgba, shape=(t.height * t.width * 4,))
+            out.append(a.reshape(t.height, t.width, 4).copy())
+        return out
+
+    def env_map(self):
+        """The environment map as a (height, width, 4) float32 array, or None."""
+        d = self.desc()
+        if not d.envMap:
+            return None
+        a = np.ctypeslib.as_array(d.envMap, shape=(d.envHeight * d.envWidth * 4,))
+        return a.reshape(d.envHeight, d.envWidth, 4).copy()
+
     def close(self):
         if self._h:
             host_lib().rtxh_scene_destroy(self._h)
@@ -308,6 +340,17 @@ def read_exr(path):
     finally:
         host_lib().rtxh_free(p)
     return arr
+
+
+def decode_image(path):
+    """PNG / baseline JPEG file -> (height, width, 4) uint8 (rtxh_decode_image)."""
+    p = C.POINTER(C.c_uint8)()
+    w, h = C.c_uint32(), C.c_uint32()
+    _host_check(host_lib().rtxh_decode_image(str(path).encode(), C.byref(p), C.byref(w), C.byref(h)))
+    try:
+        return np.ctypeslib.as_array(p, shape=(h.value * w.value * 4,)).reshape(h.value, w.value, 4).copy()
+    finally:
+        host_lib().rtxh_free(p)
 
 
 def pcg32_first(initstate, initseq):

@@ -140,6 +140,10 @@ class ModelDesc(C.Structure):
                 ("indices", C.POINTER(C.c_uint32)), ("indexCount", C.c_uint32)]
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("rgba", C.POINTER(C.c_uint8)), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("models", C.POINTER(ModelDesc)), ("modelCount", C.c_uint32),
                 ("instances", C.POINTER(Instance)), ("instanceCount", C.c_uint32),
@@ -148,7 +152,8 @@ class SceneDesc(C.Structure):
                 ("randomLightIndex", C.POINTER(C.c_int32)),
                 ("randomTriIndex", C.POINTER(FaceSample)), ("randomTriRows", C.c_uint32),
                 ("spheres", C.POINTER(Sphere)), ("sphereCount", C.c_uint32),
-                ("envMap", C.POINTER(C.c_float)), ("envWidth", C.c_uint32), ("envHeight", C.c_uint32)]
+                ("envMap", C.POINTER(C.c_float)), ("envWidth", C.c_uint32), ("envHeight", C.c_uint32),
+                ("textures", C.POINTER(TextureDesc)), ("textureCount", C.c_uint32)]
 
 
 class FrameParams(C.Structure):

@@ -15,7 +15,7 @@ INC = ROOT / "include"
 C_NAMES = {
     abi.Vertex: "rtx_vertex", abi.Material: "rtx_material", abi.Instance: "rtx_instance", abi.Light: "rtx_light",
     abi.FaceSample: "rtx_face_sample", abi.Sphere: "rtx_sphere", abi.PushConstant: "rtx_push_constant",
-    abi.Camera: "rtx_camera", abi.ModelDesc: "rtx_model_desc", abi.SceneDesc: "rtx_scene_desc",
+    abi.Camera: "rtx_camera", abi.ModelDesc: "rtx_model_desc", abi.SceneDesc: "rtx_scene_desc", abi.TextureDesc: "rtx_texture_desc",
     abi.FrameParams: "rtx_frame_params", abi.FrameStats: "rtx_frame_stats", abi.SceneInfo: "rtx_scene_info",
     abi.Ray: "rtx_ray", abi.Hit: "rtx_hit", abi.Tuning: "rtx_tuning", abi.CacheHeader: "rtx_cache_header",
     abi.CacheData: "rtx_cache_data", abi.IcJob: "rtx_ic_job", abi.Aabb: "rtx_aabb", abi.VmfTheta: "rtx_vmf_theta",
