@@ -73,6 +73,15 @@ int rtxh_scene_add_sphere(rtxh_scene *scene, const float center[3], float radius
                           int32_t *index);
 int rtxh_scene_add_point_light(rtxh_scene *scene, const float pos[3], const float color[3]);
 int rtxh_scene_set_view(rtxh_scene *scene, const rtxh_view *view);
+/* An 8-bit RGBA texture (rows top to bottom, RGB sRGB-encoded) for
+ * rtx_material.textureIdDiffuse / textureIdSpecular; *slot receives its id
+ * (1, 2, ...; slot 0 is the environment map's). */
+int rtxh_scene_add_texture(rtxh_scene *scene, const uint8_t *rgba, uint32_t width, uint32_t height, int32_t *slot);
+/* The lat-long environment map (RGBA32F, width*height texels, rows from
+ * +y down) seen by rays that miss; addLight != 0 also appends the
+ * environment light of SceneLoader::parseEnvMap (src/SceneLoader.cpp:370-403),
+ * which must come after all area lights. */
+int rtxh_scene_set_environment(rtxh_scene *scene, const float *rgba, uint32_t width, uint32_t height, int addLight);
 int rtxh_scene_finalize(rtxh_scene *scene);
 
 /* Pointers stay valid while the scene lives. */

@@ -74,6 +74,9 @@ def lib(abi):
         L.orc_sample_unit_sphere.restype = None
         L.orc_sample_lights.argtypes = [C.c_void_p, d3, d3, C.POINTER(C.c_uint32), d3, d3, C.POINTER(C.c_double)]
         L.orc_sample_lights.restype = C.c_double
+        L.orc_tex_sample.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.orc_env_color.argtypes = [C.c_void_p, d3, C.POINTER(C.c_double)]
+        L.orc_env_color.restype = None
         vp = C.c_void_p
         L.orc_ic_frame.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.c_int, C.c_int, vp, vp, vp,
                                    C.POINTER(C.c_double), C.POINTER(C.c_uint32), vp, C.c_size_t,
@@ -286,6 +289,17 @@ class OracleScene:
         jl = [(jobs[i].type, jobs[i].target, np.array(jobs[i].origin[:]), np.array(jobs[i].normal[:]))
               for i in range(n)]
         return rgb, sig, jl, sub[:n * IC_STRATA].reshape(n, IC_STRATA, 4), subsig[:n * IC_STRATA].reshape(n, IC_STRATA, 2)
+
+    def tex_sample(self, slot, u, v):
+        out = (C.c_double * 4)()
+        if self.L.orc_tex_sample(self._h, int(slot), float(u), float(v), out) != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return np.array(out[:])
+
+    def env_color(self, direction):
+        out = (C.c_double * 3)()
+        self.L.orc_env_color(self._h, (C.c_double * 3)(*[float(x) for x in direction]), out)
+        return np.array(out[:])
 
     def estimate_set(self, rgba, width, height):
         """The ADRRS pixel estimate (RGBA float32, width*height) of later frames."""

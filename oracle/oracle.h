@@ -174,6 +174,10 @@ void orc_sample_unit_sphere(uint32_t *state, double out[3]);
 double orc_sample_lights(const orc_scene *scene, const double origin[3], const double normal[3],
                          uint32_t *state, double lightDir[3], double lightColor[3],
                          double *lightDistance);
+/* Texture reads: a material texture (slot >= 1) at (u, v) as RGBA, and the
+ * environment map's colour in direction dir (black without a map). */
+int orc_tex_sample(const orc_scene *scene, int slot, double u, double v, double out[4]);
+void orc_env_color(const orc_scene *scene, const double dir[3], double out[3]);
 
 #ifdef __cplusplus
 }

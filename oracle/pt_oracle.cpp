@@ -190,6 +190,17 @@ struct orc_scene {
     // ADRRS pixel estimates (estimateImage, raytrace.rgen:120), RGBA per pixel
     std::vector<float> estimate;
     uint32_t estW = 0, estH = 0;
+    // textures (binding 7): the environment map (slot 0, RGBA32F) and the
+    // 8-bit RGBA material textures (slots 1..), sRGB decoded through srgb[]
+    std::vector<float> env;
+    uint32_t envW = 0, envH = 0;
+    struct Texture {
+        uint32_t w = 0, h = 0;
+        std::vector<uint8_t> rgba;
+    };
+    std::vector<Texture> textures;
+    float srgb[256];
+    std::vector<uint8_t> triAlpha; // per world triangle: the any-hit alpha test applies
 };
 
 namespace {
@@ -338,6 +349,95 @@ template <class R> bool isectSphere(const rtx_sphere &s, V3<R> o, V3<R> d, R tmi
     return hit;
 }
 
+// ---------------------------------------------------------------- textures
+// Vulkan linear filtering with REPEAT addressing (the samplers of
+// src/SceneLoader.cpp:209-227): texel centres at (i + 0.5) / size, the four
+// nearest texels weighted bilinearly, indices wrapped; one axis.
+template <class R> void bilerpAxis(R u, int n, int &i0, int &i1, R &w) {
+    R x = u * (R)n - R(0.5);
+    if (!(std::fabs(x) < R(1e30))) {
+        i0 = i1 = 0;
+        w = 0;
+        return;
+    }
+    R xf = std::floor(x);
+    w = x - xf;
+    int i = (int)std::fmod(xf, (R)n);
+    if (i < 0) i += n;
+    i0 = i;
+    i1 = i + 1 == n ? 0 : i + 1;
+}
+template <class R> void lerp4(const R *a, const R *b, const R *c, const R *d, R fx, R fy, R *out) {
+    R gx = R(1) - fx, gy = R(1) - fy;
+    for (int k = 0; k < 4; ++k) {
+        R t = a[k] * gx + b[k] * fx, u = c[k] * gx + d[k] * fx;
+        out[k] = t * gy + u * fy;
+    }
+}
+// texture(textureSamplers[slot], uv) of an R8G8B8A8_SRGB material texture:
+// RGB decoded to linear before filtering, alpha / 255.
+template <class R> void texSample(const orc_scene &S, int slot, R u, R v, R out[4]) {
+    const orc_scene::Texture &t = S.textures[slot];
+    int x0, x1, y0, y1;
+    R fx, fy;
+    bilerpAxis<R>(u, (int)t.w, x0, x1, fx);
+    bilerpAxis<R>(v, (int)t.h, y0, y1, fy);
+    R q[4][4];
+    const int xs[4] = {x0, x1, x0, x1}, ys[4] = {y0, y0, y1, y1};
+    for (int k = 0; k < 4; ++k) {
+        const uint8_t *p = &t.rgba[((size_t)ys[k] * t.w + xs[k]) * 4];
+        for (int c = 0; c < 3; ++c) q[k][c] = (R)S.srgb[p[c]];
+        q[k][3] = (R)((float)p[3] / 255.0f);
+    }
+    lerp4<R>(q[0], q[1], q[2], q[3], fx, fy, out);
+}
+// The environment map in direction `dir`, raytrace.rmiss:24-28 and
+// raytrace.rgen:410-416: u = atan(x, -z) / 2pi, v = acos(y) / pi; black
+// without a map (the reference's default texture, SceneLoader.cpp:80-86).
+template <class R> V3<R> envColor(const orc_scene &S, V3<R> dir) {
+    if (S.env.empty()) return V3<R>(0, 0, 0);
+    R u = std::atan2(dir.x, -dir.z) / (R(2) * Consts<R>::PI);
+    R v = std::acos(dir.y) / Consts<R>::PI;
+    int x0, x1, y0, y1;
+    R fx, fy;
+    bilerpAxis<R>(u, (int)S.envW, x0, x1, fx);
+    bilerpAxis<R>(v, (int)S.envH, y0, y1, fy);
+    R q[4][4];
+    const int xs[4] = {x0, x1, x0, x1}, ys[4] = {y0, y0, y1, y1};
+    for (int k = 0; k < 4; ++k)
+        for (int c = 0; c < 4; ++c) q[k][c] = (R)S.env[((size_t)ys[k] * S.envW + xs[k]) * 4 + c];
+    R o[4];
+    lerp4<R>(q[0], q[1], q[2], q[3], fx, fy, o);
+    return V3<R>(o[0], o[1], o[2]);
+}
+// GLSL uint(float) on the reference's hardware: saturating, NaN -> 0.
+inline uint32_t satU32(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+// raytrace.rahit:21-47: a candidate hit on world triangle p (barycentrics
+// bu, bv of vertices 1 and 2, distance t) of a material with a diffuse
+// texture is ignored when rnd(seed) > alpha at the candidate's texture
+// coordinates; seed = tea(uint(u * 1e8 + origin.x * t), randomUInt), computed
+// in FP32 as the shader does.
+template <class R> bool alphaAccept(const orc_scene &S, int p, R ox, R t, R bu, R bv, uint32_t randomUInt) {
+    const rtx_instance &in = S.inst[S.triInst[p]];
+    const auto &vv = S.verts[in.modelIndex];
+    const auto &ii = S.idx[in.modelIndex];
+    int local = S.triLocal[p];
+    const rtx_vertex &v0 = vv[ii[3 * local]], &v1 = vv[ii[3 * local + 1]], &v2 = vv[ii[3 * local + 2]];
+    R b0 = R(1) - bu - bv;
+    R u = (R)v0.texCoord[0] * b0 + (R)v1.texCoord[0] * bu + (R)v2.texCoord[0] * bv;
+    R v = (R)v0.texCoord[1] * b0 + (R)v1.texCoord[1] * bu + (R)v2.texCoord[1] * bv;
+    uint32_t st = tea(satU32((float)u * 100000000.0f + (float)ox * (float)t), randomUInt);
+    st = 1664525u * st + 1013904223u;
+    R r = R(st & 0x00FFFFFFu) / R(16777216.0);
+    R px[4];
+    texSample<R>(S, S.mats[v0.materialIndex].textureIdDiffuse, u, v, px);
+    return !(r > px[3]);
+}
+
 template <class R> V3<R> triVert(const orc_scene &s, int p, int k) {
     if (sizeof(R) == 8) {
         const double *q = &s.tri[(size_t)p * 9 + 3 * k];
@@ -365,9 +465,10 @@ template <class R> bool boxHit(const Node &n, const RayPre<R> &rp, double tmin, 
     return true;
 }
 
-// Closest hit: smallest t; equal t -> smaller primitive id.
+// Closest hit: smallest t; equal t -> smaller primitive id.  Triangles of
+// textured materials pass the any-hit alpha test (randomUInt: the frame's).
 template <class R>
-bool closest(const orc_scene &s, V3<R> o, V3<R> d, R tmin, R tmax, bool brute, HitRec &h) {
+bool closest(const orc_scene &s, V3<R> o, V3<R> d, R tmin, R tmax, bool brute, HitRec &h, uint32_t randomUInt = 0) {
     RayPre<R> rp(o, d);
     R bestT = tmax;
     int bestP = -1;
@@ -382,6 +483,7 @@ bool closest(const orc_scene &s, V3<R> o, V3<R> d, R tmin, R tmax, bool brute, H
             hit = isectSphere<R>(s.spheres[p - s.ntri], o, d, tmin, bestT, t);
         if (!hit) return;                                // hits satisfy t <= bestT
         if (bestP >= 0 && t == bestT && p > bestP) return; // tie: smaller id wins
+        if (p < (int)s.ntri && s.triAlpha[p] && !alphaAccept<R>(s, p, o.x, t, u, v, randomUInt)) return;
         bestT = t;
         bestP = p;
         bu = u;
@@ -413,13 +515,15 @@ bool closest(const orc_scene &s, V3<R> o, V3<R> d, R tmin, R tmax, bool brute, H
     return true;
 }
 
-template <class R> bool occluded(const orc_scene &s, V3<R> o, V3<R> d, R tmin, R tmax, bool brute) {
+template <class R>
+bool occluded(const orc_scene &s, V3<R> o, V3<R> d, R tmin, R tmax, bool brute, uint32_t randomUInt = 0) {
     RayPre<R> rp(o, d);
     auto testPrim = [&](int p) {
         R t, u, v;
         if (p < (int)s.ntri)
             return isectTri<R>(rp, triVert<R>(s, p, 0), triVert<R>(s, p, 1), triVert<R>(s, p, 2), tmin,
-                               tmax, t, u, v);
+                               tmax, t, u, v) &&
+                   !(s.triAlpha[p] && !alphaAccept<R>(s, p, o.x, t, u, v, randomUInt));
         return isectSphere<R>(s.spheres[p - s.ntri], o, d, tmin, tmax, t);
     };
     size_t nprim = s.ntri + s.spheres.size();
@@ -478,14 +582,19 @@ template <class R> struct Tracer {
     V3<R> lightSums[RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL], sampleThroughputs[RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL];
     // ADRRS (raytrace.rgen:69-88): the pixel's estimate and its queued splits
     V3<R> estimate = V3<R>(0, 0, 0);
-    struct SplitRec { // splitInfo, raytrace.rgen:70-80 (no texture coordinates: untextured scenes)
+    struct SplitRec { // splitInfo, raytrace.rgen:70-80
         V3<R> origin, normal, wi, throughput;
         int matIndex, depth;
         bool frontFace;
+        R u, v;
     };
     SplitRec splits[kMaxSplits];
     int nSplits = 0;
     R lastFirstT = C::TMAX;
+    // the frame's randomUInt (any-hit alpha test) and the texture
+    // coordinates of the vertex being shaded (diffuse(), phong())
+    uint32_t randomUInt = 0;
+    R texU = 0, texV = 0;
 
     Tracer(const orc_scene &s, const rtx_frame_params &f) : S(s), fp(f), pc(f.pc) {}
 
@@ -617,18 +726,28 @@ template <class R> struct Tracer {
         if (result < R(0)) return R(0);
         return result;
     }
-    // ref: shaders/raytrace.rgen:232-239 (no textures)
-    static V3<R> diffuse(const rtx_material &m) { return f3<R>(m.diffuse) / C::PI; }
+    V3<R> texRgb(int slot) const {
+        R px[4];
+        texSample<R>(S, slot, texU, texV, px);
+        return V3<R>(px[0], px[1], px[2]);
+    }
+    // ref: shaders/raytrace.rgen:232-239, at the shaded vertex's texture coordinates
+    V3<R> diffuse(const rtx_material &m) const {
+        if (m.textureIdDiffuse != -1) return f3<R>(m.diffuse) * texRgb(m.textureIdDiffuse) / C::PI;
+        return f3<R>(m.diffuse) / C::PI;
+    }
     // ref: shaders/raytrace.rgen:242-267
-    static V3<R> phong(const rtx_material &m, V3<R> n, V3<R> wi, V3<R> wo) {
+    V3<R> phong(const rtx_material &m, V3<R> n, V3<R> wi, V3<R> wo) const {
         V3<R> res(0, 0, 0);
         R cosThetaWo = dot(wo, n);
         if (cosThetaWo > R(0)) {
-            res = res + f3<R>(m.diffuse) / C::PI;
+            res = res + diffuse(m);
             R dotReflDir = dot(reflect(-wo, n), wi);
             if (dotReflDir > R(0)) {
                 R h = (R)m.specularHighlight;
-                res = res + ((h + R(2)) / (R(2) * C::PI)) * f3<R>(m.specular) * std::pow(dotReflDir, h);
+                V3<R> spec = ((h + R(2)) / (R(2) * C::PI)) * f3<R>(m.specular) * std::pow(dotReflDir, h);
+                if (m.textureIdSpecular != -1) spec = spec * texRgb(m.textureIdSpecular);
+                res = res + spec;
             }
         }
         return cosThetaWo * res;
@@ -714,7 +833,7 @@ template <class R> struct Tracer {
         }
     }
     // ref: shaders/raytrace.rgen:557-599
-    static V3<R> evalBsdf(const rtx_material &m, V3<R> n, V3<R> wi, V3<R> wo, bool frontFace) {
+    V3<R> evalBsdf(const rtx_material &m, V3<R> n, V3<R> wi, V3<R> wo, bool frontFace) const {
         switch (m.type) {
         case RTX_MAT_DIFFUSE:
         case RTX_MAT_LIGHT:
@@ -756,11 +875,11 @@ template <class R> struct Tracer {
     HitInfo<R> trace(V3<R> o, V3<R> d, R tmin, R tmax) {
         HitInfo<R> info;
         HitRec h;
-        if (!closest<R>(S, o, d, tmin, tmax, false, h)) { // ref: raytrace.rmiss:18-31
+        if (!closest<R>(S, o, d, tmin, tmax, false, h, randomUInt)) { // ref: raytrace.rmiss:18-31
             info.normal = -d;
             info.t = tmax;
             info.isMiss = true;
-            info.missColor = V3<R>(0, 0, 0); // default black env texture
+            info.missColor = envColor<R>(S, normalize(d)); // black without an environment map
             info.matIndex = -1;
             info.prim = -1;
             return info;
@@ -838,8 +957,12 @@ template <class R> struct Tracer {
             lightDir = normalize(toLight);
             R cosThetaLight = dot(-lightDir, sphereNormal);
             return (R)light.sampleProb * lightDistance * lightDistance / (cosThetaLight * area);
-        } else if (light.type == RTX_LIGHT_ENV_MAP) {
-            return R(0); // rejected at scene creation
+        } else if (light.type == RTX_LIGHT_ENV_MAP) { // uniform hemisphere, pdf 1 / 2pi
+            lightDir = randomOnUnitSphere();
+            if (dot(normal, lightDir) < R(0)) lightDir = reflect(lightDir, normal);
+            lightColor = envColor<R>(S, lightDir);
+            lightDistance = C::TMAX;
+            return (R)light.sampleProb / (R(2) * C::PI);
         }
         int iModel = S.inst[light.instanceIndex].modelIndex;
         int iRandomTri = randomInteger(RTX_SIZE_TRI_RANDOM - 1);
@@ -881,7 +1004,8 @@ template <class R> struct Tracer {
         bool isShadowed = true;
         R cosThetaLight = dot(normal, lightDir);
         if (cosThetaLight > R(0) && pdfLights > R(0)) {
-            isShadowed = occluded<R>(S, origin, lightDir, C::TMIN, lightDistance * (R(1) - R(0.0001)), false);
+            isShadowed =
+                occluded<R>(S, origin, lightDir, C::TMIN, lightDistance * (R(1) - R(0.0001)), false, randomUInt);
             // the outcome of every traced shadow ray joins the path signature
             primHash = (primHash ^ (isShadowed ? kSigOccluded : kSigVisible)) * 16777619u;
         }
@@ -1031,7 +1155,7 @@ template <class R> struct Tracer {
         return pc.useIrradianceCacheOnGlossy && !hasDiscreteDirection(m);
     }
     // ref: raytrace.rgen:815-828
-    static V3<R> approxDiffuse(const rtx_material &m, V3<R> normal, V3<R> wi) {
+    V3<R> approxDiffuse(const rtx_material &m, V3<R> normal, V3<R> wi) const {
         switch (m.type) {
         case RTX_MAT_DIFFUSE:
         case RTX_MAT_LIGHT:
@@ -1109,7 +1233,7 @@ template <class R> struct Tracer {
     // split (raytrace.rgen:883-902): false when MAX_SPLITS records are queued
     bool split(V3<R> origin, V3<R> normal, V3<R> wi, V3<R> throughput, int matIndex, int depth, bool frontFace) {
         if (nSplits >= kMaxSplits) return false;
-        splits[nSplits++] = SplitRec{origin, normal, wi, throughput, matIndex, depth, frontFace};
+        splits[nSplits++] = SplitRec{origin, normal, wi, throughput, matIndex, depth, frontFace, texU, texV};
         g_adrrsCount[0]++;
         return true;
     }
@@ -1149,6 +1273,8 @@ template <class R> struct Tracer {
             const rtx_material &m = S.mats[info.matIndex];
             origin = info.worldPos;
             V3<R> normal = info.normal;
+            texU = info.u;
+            texV = info.v;
             if (depth == 1) firstT = info.t;
             if (m.type == RTX_MAT_LIGHT && (addDirectLights || addNextDirectLights)) {
                 result = result + throughput * f3<R>(m.lightColor);
@@ -1302,6 +1428,7 @@ template <class R> struct Tracer {
         uint64_t sample = pathId / npix + fp.sampleOffset;
         uint32_t frameSeed = pcg32First(fp.seed, sample);
         seed = tea(pixel, frameSeed);
+        randomUInt = frameSeed;
         primHash = 2166136261u;
         const bool useIC = pc.useIrradianceCache != 0;
         const bool useADRRS = pc.useADRRS != 0;
@@ -1328,6 +1455,8 @@ template <class R> struct Tracer {
         for (int k = 0; k < nSplits; ++k) { // raytrace.rgen:1684-1708
             const SplitRec sp = splits[k];
             const rtx_material &m = S.mats[sp.matIndex];
+            texU = sp.u;
+            texV = sp.v;
             if (useNEE) result = result + sp.throughput * multipleNEE(m, sp.origin, sp.wi, sp.normal, sp.frontFace,
                                                                       pc.numNEE);
             V3<R> dir;
@@ -1348,10 +1477,16 @@ template <class R> struct Tracer {
 
 int checkScene(const orc_scene &s) {
     for (const auto &l : s.lights)
-        if (l.type == RTX_LIGHT_ENV_MAP) {
-            g_err = "environment-map lights are not supported by the oracle yet";
-            return RTX_E_UNSUPPORTED;
+        if (l.type == RTX_LIGHT_ENV_MAP && s.env.empty()) {
+            g_err = "environment-map light without an environment map";
+            return RTX_E_INVALID;
         }
+    for (const auto &m : s.mats)
+        for (int32_t tid : {m.textureIdDiffuse, m.textureIdSpecular})
+            if (tid != -1 && (tid < 1 || (size_t)tid >= s.textures.size() || s.textures[tid].w == 0)) {
+                g_err = "material texture id " + std::to_string(tid) + " names no texture";
+                return RTX_E_INVALID;
+            }
     return RTX_OK;
 }
 
@@ -1465,6 +1600,7 @@ void icSubPath(const orc_scene &S, const rtx_frame_params &fp, const IcSnapshot 
                uint32_t job, const IcJobRec &jb, int k, int j, double out[4], uint32_t sig[2]) {
     Tracer<R> t(S, fp);
     t.ic = &snap;
+    t.randomUInt = frameSeed;
     R phi, theta;
     icStratum<R>(frameSeed, job, k, j, phi, theta, t.seed);
     t.primHash = 2166136261u;
@@ -1687,6 +1823,23 @@ int orc_scene_create(const rtx_scene_desc *d, orc_scene **out) {
         s->triTable.assign(d->randomTriIndex, d->randomTriIndex + (size_t)d->randomTriRows * RTX_SIZE_TRI_RANDOM);
     s->triRows = d->randomTriRows;
     s->spheres.assign(d->spheres, d->spheres + d->sphereCount);
+    if (d->envMap && d->envWidth && d->envHeight) {
+        s->env.assign(d->envMap, d->envMap + (size_t)d->envWidth * d->envHeight * 4);
+        s->envW = d->envWidth;
+        s->envH = d->envHeight;
+    }
+    s->textures.resize(d->textures ? d->textureCount : 0);
+    for (uint32_t t = 1; t < s->textures.size(); ++t) {
+        const rtx_texture_desc &td = d->textures[t];
+        if (!td.rgba) continue;
+        s->textures[t].w = td.width;
+        s->textures[t].h = td.height;
+        s->textures[t].rgba.assign(td.rgba, td.rgba + (size_t)td.width * td.height * 4);
+    }
+    for (int i = 0; i < 256; ++i) { // IEC 61966-2-1 decoding of the SRGB formats
+        double c = i / 255.0;
+        s->srgb[i] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
     int rc = checkScene(*s);
     if (rc != RTX_OK) return rc;
     // flatten instances to world-space triangles (global primitive order:
@@ -1712,6 +1865,12 @@ int orc_scene_create(const rtx_scene_desc *d, orc_scene **out) {
         }
     }
     s->ntri = s->triInst.size();
+    s->triAlpha.assign(s->ntri, 0);
+    for (size_t p = 0; p < s->ntri; ++p) { // raytrace.rahit:37: the first vertex's material
+        const rtx_instance &in = s->inst[s->triInst[p]];
+        const rtx_vertex &v0 = s->verts[in.modelIndex][s->idx[in.modelIndex][3 * (size_t)s->triLocal[p]]];
+        s->triAlpha[p] = s->mats[v0.materialIndex].textureIdDiffuse != -1;
+    }
     size_t nprim = s->ntri + s->spheres.size();
     if (nprim > 0) {
         std::vector<std::array<double, 6>> box(nprim);
@@ -2100,7 +2259,24 @@ double orc_pdf_bsdf(const rtx_material *m, const double n[3], const double wi[3]
 }
 void orc_eval_bsdf(const rtx_material *m, const double n[3], const double wi[3], const double wo[3], int ff,
                    double out[3]) {
-    toD(Tracer<double>::evalBsdf(*m, vd<double>(n), vd<double>(wi), vd<double>(wo), ff != 0), out);
+    rtx_material mm = *m; // without a scene: textures are not applied
+    mm.textureIdDiffuse = mm.textureIdSpecular = -1;
+    Tracer<double> t(emptyScene(), defaultParams());
+    toD(t.evalBsdf(mm, vd<double>(n), vd<double>(wi), vd<double>(wo), ff != 0), out);
+}
+// Texture look-ups of a scene: texture(textureSamplers[slot], uv) for a
+// material texture (slot >= 1) and the environment map's colour in `dir`
+// (miss shader form: dir is normalized first).
+int orc_tex_sample(const orc_scene *s, int slot, double u, double v, double out[4]) {
+    if (!s || slot < 1 || (size_t)slot >= s->textures.size() || s->textures[slot].w == 0) {
+        g_err = "no texture in that slot";
+        return RTX_E_INVALID;
+    }
+    texSample<double>(*s, slot, u, v, out);
+    return RTX_OK;
+}
+void orc_env_color(const orc_scene *s, const double dir[3], double out[3]) {
+    toD(envColor<double>(*s, normalize(vd<double>(dir))), out);
 }
 double orc_sample_bsdf(const rtx_material *m, const double wi[3], const double n[3], int ff, uint32_t *state,
                        double wo[3]) {

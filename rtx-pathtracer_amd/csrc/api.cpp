@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -141,6 +142,13 @@ void freeAll(std::vector<void *> &list) {
 }
 
 // mat4 * (v, 1) in FP32 with GLM's summation order
+// The sRGB transfer function's inverse (IEC 61966-2-1), as Vulkan applies it
+// to each colour component of an SRGB-format texel before filtering.
+float srgbDecode(int i) {
+    double c = i / 255.0;
+    return (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+}
+
 void xformPoint(const float *m, const float *v, float *o) {
     for (int r = 0; r < 3; ++r) {
         float a0 = m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1];
@@ -157,7 +165,15 @@ int validateScene(rtx_ctx *ctx, const rtx_scene_desc *d, uint64_t &numTris) {
     if (d->lightCount && (!d->lights || !d->randomLightIndex))
         return fail(ctx, RTX_E_INVALID, "lights without light tables");
     if (d->sphereCount && !d->spheres) return fail(ctx, RTX_E_INVALID, "spheres is NULL");
-    if (d->envMap) return fail(ctx, RTX_E_UNSUPPORTED, "environment maps are not supported yet");
+    if (d->envMap && (d->envWidth == 0 || d->envHeight == 0))
+        return fail(ctx, RTX_E_INVALID, "environment map without a size");
+    if (d->textureCount && !d->textures) return fail(ctx, RTX_E_INVALID, "textures is NULL");
+    for (uint32_t t = 1; t < d->textureCount; ++t) {
+        const rtx_texture_desc &td = d->textures[t];
+        if (!td.rgba || td.width == 0 || td.height == 0)
+            return fail(ctx, RTX_E_INVALID, "texture slot " + std::to_string(t) + " has no texels");
+        if ((uint64_t)td.width * td.height >= (1ull << 31)) return fail(ctx, RTX_E_UNSUPPORTED, "texture too large");
+    }
     for (uint32_t m = 0; m < d->modelCount; ++m) {
         const rtx_model_desc &md = d->models[m];
         if (md.indexCount % 3) return fail(ctx, RTX_E_INVALID, "model index count not a multiple of 3");
@@ -181,8 +197,9 @@ int validateScene(rtx_ctx *ctx, const rtx_scene_desc *d, uint64_t &numTris) {
     for (uint32_t m = 0; m < d->materialCount; ++m) {
         if (d->materials[m].type < 0 || d->materials[m].type > RTX_MAT_ROUGH_CONDUCTOR)
             return fail(ctx, RTX_E_INVALID, "unknown material type");
-        if (d->materials[m].textureIdDiffuse != -1 || d->materials[m].textureIdSpecular != -1)
-            return fail(ctx, RTX_E_UNSUPPORTED, "material textures are not supported yet");
+        for (int32_t tid : {d->materials[m].textureIdDiffuse, d->materials[m].textureIdSpecular})
+            if (tid != -1 && (tid < 1 || (uint32_t)tid >= d->textureCount))
+                return fail(ctx, RTX_E_INVALID, "material texture id " + std::to_string(tid) + " names no texture");
     }
     for (uint32_t s = 0; s < d->sphereCount; ++s) {
         const rtx_sphere &sp = d->spheres[s];
@@ -211,8 +228,11 @@ int validateScene(rtx_ctx *ctx, const rtx_scene_desc *d, uint64_t &numTris) {
             }
             break;
         }
+        case RTX_LIGHT_ENV_MAP:
+            if (!d->envMap) return fail(ctx, RTX_E_INVALID, "environment-map light without an environment map");
+            break;
         default:
-            return fail(ctx, RTX_E_UNSUPPORTED, "environment-map lights are not supported yet");
+            return fail(ctx, RTX_E_INVALID, "unknown light type");
         }
     }
     if (d->lightCount)
@@ -231,6 +251,7 @@ int ensureBatchBuffers(rtx_ctx *ctx, size_t paths, uint32_t numNEE) {
     PathBuffers &pb = ctx->pb;
     CHECK_HIP(ctx, devAlloc(&pb.rng, cap, L));
     CHECK_HIP(ctx, devAlloc(&pb.hash, cap, L));
+    CHECK_HIP(ctx, devAlloc(&pb.frameSeed, cap, L));
     CHECK_HIP(ctx, devAlloc(&pb.thr, cap, L));
     CHECK_HIP(ctx, devAlloc(&pb.result, cap, L));
     CHECK_HIP(ctx, devAlloc(&pb.pendShadow, cap * nee, L));
@@ -494,11 +515,24 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     } catch (const std::exception &e) {
         return fail(ctx, RTX_E_INVALID, std::string("BVH build failed: ") + e.what());
     }
+    // triangles of materials with a diffuse texture run the any-hit alpha
+    // test (raytrace.rahit:37-46 reads the material of the first vertex)
+    std::vector<uint8_t> alpha((size_t)numTris, 0);
+    uint32_t alphaTris = 0;
+    for (size_t p = 0; p < (size_t)numTris; ++p) {
+        const rtx_instance &in = d->instances[primRef[p].x];
+        const rtx_model_desc &md = d->models[in.modelIndex];
+        int mat = md.vertices[md.indices[3 * (size_t)primRef[p].y]].materialIndex;
+        if (d->materials[mat].textureIdDiffuse != -1) {
+            alpha[p] = 1;
+            alphaTris++;
+        }
+    }
     std::vector<float4> tris((size_t)numTris * 3);
     for (size_t k = 0; k < bvh.primOrder.size(); ++k) {
         uint32_t p = bvh.primOrder[k];
         const float *v = &tv[9 * (size_t)p];
-        int pid = (int)p;
+        int pid = (int)p | (alpha[p] ? kAlphaBit : 0);
         float pidBits;
         std::memcpy(&pidBits, &pid, 4);
         tris[3 * k + 0] = make_float4(v[0], v[1], v[2], pidBits);
@@ -552,6 +586,28 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     if (e == hipSuccess)
         e = upload(&ftD, d->randomTriIndex, d->randomTriIndex ? (size_t)d->randomTriRows * RTX_SIZE_TRI_RANDOM : 0);
     if (e == hipSuccess) e = upload(&spD, d->spheres, d->sphereCount);
+    // textures: the environment map as float4 texels, material textures packed
+    // one after another, the sRGB decoding table of the R8G8B8A8_SRGB images
+    float4 *envD = nullptr;
+    uint32_t *texD = nullptr;
+    int4 *texInfoD = nullptr;
+    float *srgbD = nullptr;
+    std::vector<int4> texInfo(std::max<uint32_t>(d->textureCount, 1), make_int4(0, 0, 0, 0));
+    std::vector<uint32_t> texels;
+    for (uint32_t t = 1; t < d->textureCount; ++t) {
+        const rtx_texture_desc &td = d->textures[t];
+        texInfo[t] = make_int4((int)texels.size(), (int)td.width, (int)td.height, 0);
+        const size_t n = (size_t)td.width * td.height;
+        texels.resize(texels.size() + n);
+        std::memcpy(texels.data() + texels.size() - n, td.rgba, n * 4);
+    }
+    std::vector<float> srgb(256);
+    for (int i = 0; i < 256; ++i) srgb[i] = srgbDecode(i);
+    if (e == hipSuccess && d->envMap)
+        e = upload(&envD, reinterpret_cast<const float4 *>(d->envMap), (size_t)d->envWidth * d->envHeight);
+    if (e == hipSuccess) e = upload(&texD, texels.data(), texels.size());
+    if (e == hipSuccess) e = upload(&texInfoD, texInfo.data(), texInfo.size());
+    if (e == hipSuccess) e = upload(&srgbD, srgb.data(), srgb.size());
     if (e != hipSuccess) {
         freeAll(sc->allocs);
         delete sc;
@@ -575,6 +631,14 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     S.lightCount = d->lightCount;
     S.triRows = d->randomTriRows;
     S.numNodes = (uint32_t)bvh.nodes.size();
+    S.env = envD;
+    S.envW = d->envMap ? d->envWidth : 0;
+    S.envH = d->envMap ? d->envHeight : 0;
+    S.texels = texD;
+    S.texInfo = texInfoD;
+    S.numTextures = d->textureCount;
+    S.alphaTris = alphaTris;
+    S.srgbToLinear = srgbD;
     sc->info.triangles = (uint32_t)numTris;
     sc->info.spheres = d->sphereCount;
     sc->info.nodes = (uint32_t)bvh.nodes.size();
@@ -876,6 +940,14 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     // kernels over empty queues).
     constexpr int kPollEvery = 4;
     int itersRun = 0;
+    // where the any-hit alpha test finds each ray's frame seed: extend entries
+    // are path ids, shadow and probe entries path * numNEE + j
+    auto setRaySeeds = [&](uint32_t nee) {
+        for (Queue *q : {&ctx->q[0], &ctx->q[1], &ctx->qs, &ctx->qp}) {
+            q->pathSeed = ctx->pb.frameSeed;
+            q->seedDiv = q == &ctx->qs || q == &ctx->qp ? nee : 1u;
+        }
+    };
     auto bounces = [&](const FrameConst *fcd, BatchDesc b, int iters, bool multi) -> int {
         int qsel = 0;
         const int cap = multi ? (int)(samplesPerFrame + kMaxSplits + 1) * iters : iters;
@@ -930,6 +1002,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.npix = (uint32_t)npix;
             ctx->pb.dirFirstPixel = (uint32_t)(first % npix);
             timed(kClsShade, [&] { launchGenerate(s, ctx->fc, b, ctx->pb, ctx->q[0], ctx->cnt, 0, ctx->ic.frame); });
+            setRaySeeds(fc.numNEE);
             int rcb = bounces(ctx->fc, b, maxIter, segments);
             if (rcb != RTX_OK) return rcb;
             timed(kClsShade, [&] {
@@ -982,6 +1055,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.npix = np;
             timed(kClsShade,
                   [&] { launchGenerateIC(s, ctx->ic, frameSeed, j0, np, ctx->pb, ctx->q[0], ctx->cnt, 0); });
+            setRaySeeds(fcIC.numNEE);
             int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
             if (rcs != RTX_OK) return rcs;
             timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, j0, nj, ctx->pb); });

@@ -67,7 +67,7 @@ RTX_DI V3 approxDiffuse(const Mat &m, V3 normal, V3 wi) {
     switch (m.type) {
     case RTX_MAT_DIFFUSE:
     case RTX_MAT_LIGHT:
-    case RTX_MAT_PHONG: return m.diffuse * (1.0f / kPi);
+    case RTX_MAT_PHONG: return m.kd * (1.0f / kPi);
     case RTX_MAT_ROUGH_CONDUCTOR: return v3(fresnelConductor(dot(wi, normal), m.eta, m.k) * (1.0f / kPi));
     default: return v3(-1.0f);
     }

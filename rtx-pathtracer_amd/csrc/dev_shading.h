@@ -5,6 +5,7 @@
 #pragma once
 
 #include "dev_math.h"
+#include "dev_texture.h"
 #include "internal.h"
 
 namespace rtx {
@@ -38,6 +39,11 @@ RTX_DI V3 randomOnUnitSphere(Rng &r) { // shaders/random.glsl:60-68
         if (!(length(res) > 1.0f)) break;
     }
     return normalize(res);
+}
+RTX_DI V3 randomInHemisphere(Rng &r, V3 n) { // shaders/random.glsl:71-83
+    V3 res = randomOnUnitSphere(r);
+    if (dot(n, res) < 0.0f) res = reflect(res, n);
+    return res;
 }
 RTX_DI V3 randomInHemisphereCosine(Rng &r, V3 n) { // shaders/random.glsl:86-94
     float u = r.next();
@@ -124,11 +130,14 @@ RTX_DI float Gterm(float alpha, V3 i, V3 o, V3 n, V3 m) { // shaders/raytrace.rg
     return r < 0.0f ? 0.0f : r;
 }
 
-// Material fields the shading reads, gathered once per vertex.
+// Material fields the shading reads, gathered once per vertex.  kd and ks
+// are the diffuse and specular colours with the vertex's textures applied
+// (diffuse() and phong(), raytrace.rgen:232-267); `diffuse` and `specular`
+// stay untextured for the lobe choice (raytrace.rgen:302-303, 493-494).
 struct Mat {
-    V3 lightColor, diffuse, specular;
+    V3 lightColor, diffuse, specular, kd, ks;
     float highlight, ior, iorInv, eta, k, roughness;
-    int type;
+    int type, texD, texS;
 };
 RTX_DI Mat loadMat(const rtx_material *m) {
     const float4 *p = reinterpret_cast<const float4 *>(m);
@@ -145,8 +154,23 @@ RTX_DI Mat loadMat(const rtx_material *m) {
     // e = (k, roughness, texIdDiffuse, texIdSpecular)
     r.k = e.x;
     r.roughness = e.y;
+    r.texD = __float_as_int(e.z);
+    r.texS = __float_as_int(e.w);
     r.type = __float_as_int(f.x);
+    r.kd = r.diffuse;
+    r.ks = r.specular;
     return r;
+}
+// The texture look-ups of diffuse() / phong() at texture coordinates (u, v).
+RTX_DI void applyTextures(const DevScene &S, Mat &m, float u, float v) {
+    if (m.texD != -1) {
+        float4 t = texSample(S, m.texD, u, v);
+        m.kd = m.diffuse * v3(t.x, t.y, t.z);
+    }
+    if (m.texS != -1) {
+        float4 t = texSample(S, m.texS, u, v);
+        m.ks = m.specular * v3(t.x, t.y, t.z);
+    }
 }
 
 RTX_DI bool hasDiscreteDirection(int type) { // shaders/raytrace.rgen:733-742
@@ -161,10 +185,10 @@ RTX_DI V3 phong(const Mat &m, V3 n, V3 wi, V3 wo) { // shaders/raytrace.rgen:242
     V3 res = v3(0.0f);
     float cosThetaWo = dot(wo, n);
     if (cosThetaWo > 0.0f) {
-        res = res + m.diffuse / kPi;
+        res = res + m.kd / kPi;
         float dotReflDir = dot(reflect(-wo, n), wi);
         if (dotReflDir > 0.0f)
-            res = res + ((m.highlight + 2.0f) / (2.0f * kPi)) * m.specular * powf(dotReflDir, m.highlight);
+            res = res + ((m.highlight + 2.0f) / (2.0f * kPi)) * m.ks * powf(dotReflDir, m.highlight);
     }
     return cosThetaWo * res;
 }
@@ -245,7 +269,7 @@ RTX_DI V3 evalBsdf(const Mat &m, V3 n, V3 wi, V3 wo, bool frontFace) { // shader
     switch (m.type) {
     case RTX_MAT_DIFFUSE:
     case RTX_MAT_LIGHT:
-        return dot(wo, n) * (m.diffuse / kPi);
+        return dot(wo, n) * (m.kd / kPi);
     case RTX_MAT_PHONG:
         return phong(m, n, wi, wo);
     case RTX_MAT_ROUGH_CONDUCTOR: {
@@ -328,10 +352,17 @@ RTX_DI HitInfo hitInfo(const DevScene &S, int prim, float t, float bu, float bv,
 }
 
 // sampleLights, shaders/raytrace.rgen:358-473
-RTX_DI float sampleLights(const DevScene &S, Rng &r, V3 origin, V3 &lightDir, V3 &lightColor, float &lightDist) {
+RTX_DI float sampleLights(const DevScene &S, Rng &r, V3 origin, V3 normal, V3 &lightDir, V3 &lightColor,
+                          float &lightDist) {
     int iRandomLight = r.integer(RTX_SIZE_LIGHT_RANDOM - 1);
     int iLight = S.lightIndex[iRandomLight];
     const rtx_light &light = S.lights[iLight];
+    if (light.type == RTX_LIGHT_ENV_MAP) { // raytrace.rgen:406-421: uniform hemisphere, pdf 1 / 2pi
+        lightDir = randomInHemisphere(r, normal);
+        lightColor = envColor(S, lightDir);
+        lightDist = kTMax;
+        return light.sampleProb / (2.0f * kPi);
+    }
     if (light.type == RTX_LIGHT_POINT) {
         V3 toLight = v3(light.pos) - origin;
         lightDist = length(toLight);

@@ -10,6 +10,7 @@
 #pragma once
 
 #include "dev_math.h"
+#include "dev_texture.h"
 #include "internal.h"
 
 namespace rtx {
@@ -147,10 +148,11 @@ struct VisitCount {
 
 // Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax].
 // Returns the primitive id (triangles 0..numTris-1, then spheres) or -1.
-// Closest-hit ties in t go to the smaller primitive id.
+// Closest-hit ties in t go to the smaller primitive id.  Triangles marked
+// with kAlphaBit pass the any-hit alpha test first (randomUInt: the frame's).
 template <bool ANY, bool COUNT, class Stack>
 RTX_DI int traverse(const DevScene &S, const TravRay &r, float tmin, float &tmax, float &bu, float &bv,
-                    Stack &st, VisitCount &vc) {
+                    Stack &st, VisitCount &vc, uint32_t randomUInt = 0) {
     int best = -1;
     for (uint32_t i = 0; i < S.numSpheres; ++i) {
         float t;
@@ -201,6 +203,10 @@ RTX_DI int traverse(const DevScene &S, const TravRay &r, float tmin, float &tmax
                 float t, u, v;
                 if (isectTri(r, v0, v1, v2, tmin, tmax, t, u, v)) {
                     int id = __float_as_int(v0.w);
+                    if (id & kAlphaBit) {
+                        id &= ~kAlphaBit;
+                        if (!alphaAccept(S, id, r.o.x, t, u, v, randomUInt)) continue;
+                    }
                     if (ANY) return id;
                     if (best < 0 || t < tmax || id < best) {
                         tmax = t;
@@ -232,6 +238,7 @@ struct LaneRay {
     int best;
     int link, leaf;
     uint32_t idx;
+    uint32_t randomUInt; // the frame's, for the any-hit alpha test
 };
 
 template <class Stack> RTX_DI int popOrDone(Stack &st) { return st.sp == 0 ? kLinkDone : st.pop(); }
@@ -263,7 +270,7 @@ RTX_DI int innerStep(const DevScene &S, LaneRay &L, Stack &st, VisitCount &vc) {
 
 // Intersects the primitives of leaf link `leaf`; returns true when an any-hit
 // ray is done.
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, bool ALPHA>
 RTX_DI bool leafStep(const DevScene &S, LaneRay &L, int leaf, VisitCount &vc) {
     uint32_t data = ~(uint32_t)leaf;
     uint32_t start = data >> 4, cnt = (data & 15u) + 1u;
@@ -274,6 +281,10 @@ RTX_DI bool leafStep(const DevScene &S, LaneRay &L, int leaf, VisitCount &vc) {
         float t, u, v;
         if (isectTri(L.r, v0, v1, v2, L.tmin, L.tmax, t, u, v)) {
             int id = __float_as_int(v0.w);
+            if (ALPHA && (id & kAlphaBit)) {
+                id &= ~kAlphaBit;
+                if (!alphaAccept(S, id, L.r.o.x, t, u, v, L.randomUInt)) continue;
+            }
             if (ANY) {
                 L.best = id;
                 return true;
@@ -321,7 +332,7 @@ template <bool ANY> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
 // fetching, so `fetch` ends below n + 64 * waves.
 constexpr int kRefillLanes = 16;
 
-template <bool ANY, bool COUNT, class Stack, class Load, class Store>
+template <bool ANY, bool COUNT, bool ALPHA, class Stack, class Load, class Store>
 RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stack &st, VisitCount &vc, Load load,
                             Store store) {
     LaneRay L;
@@ -376,7 +387,7 @@ RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stac
             }
             // leaves
             while (L.leaf != kLinkDone) {
-                if (leafStep<ANY, COUNT>(S, L, L.leaf, vc)) {
+                if (leafStep<ANY, COUNT, ALPHA>(S, L, L.leaf, vc)) {
                     L.link = kLinkDone;
                     L.leaf = kLinkDone;
                     break;

@@ -33,7 +33,20 @@ struct DevScene {
     uint32_t lightCount;
     uint32_t triRows;
     uint32_t numNodes;
+    // textures (binding 7 of the reference): slot 0 the environment map
+    // (RGBA32F lat-long), slots 1.. 8-bit RGBA material textures
+    const float4 *env;          // envW x envH texels, or null (black)
+    uint32_t envW, envH;
+    const uint32_t *texels;     // all material textures, RGBA8 packed little-endian
+    const int4 *texInfo;        // per slot: (first texel, width, height, 0)
+    uint32_t numTextures;
+    uint32_t alphaTris;         // triangles whose material has a diffuse texture (any-hit alpha test)
+    const float *srgbToLinear;  // 256 entries
 };
+
+// Bit of a triangle's id (tris[3k].w) marking it for the any-hit alpha test
+// (raytrace.rahit:21-47); ids stay below 2^27.
+constexpr int32_t kAlphaBit = 1 << 30;
 
 // ----------------------------------------------------------- path state
 // Queues and per-path state of one wavefront batch (structure of arrays).
@@ -43,6 +56,10 @@ struct Queue {
     float4 *dir;    // direction.xyz, tmax
     float4 *hit;    // closest hit (t, prim bits, u, v): extend and probe queues
     float4 *aux;    // probe: (premultiplied BSDF weight rgb, pdfMat)
+    // the frame's randomUInt of entry i's path is pathSeed[(id[i] & 0x7fffffff) / seedDiv]
+    // (read by the any-hit alpha test only)
+    const uint32_t *pathSeed;
+    uint32_t seedDiv;
 };
 
 struct Counters {
@@ -221,8 +238,9 @@ struct PathBuffers {
     int4 *task;       // per path: (camera samples left, splits queued, next split, record sampleOffset)
     float4 *taskRes;  // per path: main()'s result so far (the segment's own result is in `result`)
     float4 *splitRec; // kMaxSplits x 4 per path: (origin, depth bits), (normal, material | frontFace << 31),
-                      // (wi, 0), (throughput, 0)  (splitInfo, raytrace.rgen:70-80)
+                      // (wi, texture u), (throughput, texture v)  (splitInfo, raytrace.rgen:70-80)
     const float4 *estimate; // per pixel: the ADRRS estimate (estimateImage)
+    uint32_t *frameSeed;    // per path: the frame's randomUInt (any-hit alpha test seeds)
 };
 constexpr int kMaxSplits = 10; // MAX_SPLITS, raytrace.rgen:82
 constexpr int kMaxRecords = RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL;

@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ 
     q0.dir[pid] = f4(d, kTMax);
     pb.rng[pid] = rng.s;
     pb.hash[pid] = 2166136261u;
+    pb.frameSeed[pid] = frameSeed;
     if (fc.record) { // resetSamplesToInvalid, raytrace.rgen:1615-1621
         rtx_directional_data *dd = recordSlots(pb, pid);
         for (int k = 0; k < kMaxRecords; ++k) dd[k].flags = kDirInvalid;
@@ -167,8 +168,14 @@ __global__ __launch_bounds__(256) void kGenerateIC(IcBuffers ic, uint32_t frameS
     q0.dir[pid] = f4(d, kTMax);
     pb.rng[pid] = rng.s;
     pb.hash[pid] = 2166136261u;
+    pb.frameSeed[pid] = frameSeed;
     pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagFollow));
     pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
+}
+
+// The frame's randomUInt for queue entry i, when the any-hit alpha test needs it.
+RTX_DI uint32_t raySeed(const DevScene &S, const Queue &q, uint32_t i) {
+    return S.alphaTris ? q.pathSeed[((uint32_t)q.id[i] & 0x7fffffffu) / q.seedDiv] : 0u;
 }
 
 // --------------------------------------------------------------- extend
@@ -184,7 +191,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(DevScene S, Queue q, Coun
         float4 o = q.org[i], d = q.dir[i];
         TravRay r = makeTravRay(v3(o), v3(d));
         float tmax = d.w, bu = 0.0f, bv = 0.0f;
-        int prim = traverse<false, COUNT>(S, r, o.w, tmax, bu, bv, st, vc);
+        int prim = traverse<false, COUNT>(S, r, o.w, tmax, bu, bv, st, vc, raySeed(S, q, i));
         q.hit[i] = make_float4(tmax, __int_as_float(prim), bu, bv);
     }
     flushVisits<COUNT>(vc, cnt, 0);
@@ -203,21 +210,21 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(DevScene S, Queue q, Path
         float4 o = q.org[i], d = q.dir[i];
         TravRay r = makeTravRay(v3(o), v3(d));
         float tmax = d.w, bu, bv;
-        int prim = traverse<true, COUNT>(S, r, o.w, tmax, bu, bv, st, vc);
+        int prim = traverse<true, COUNT>(S, r, o.w, tmax, bu, bv, st, vc, raySeed(S, q, i));
         if (prim < 0) pb.pendShadow[q.id[i]].w = 1.0f; // unoccluded: raytrace.shadow.rmiss:7-10
     }
     flushVisits<COUNT>(vc, cnt, 1);
 }
 
 // MIS probe result (raytrace.rgen:684-726): on a miss the environment colour
-// (black default texture) with pdf 1/(2 pi lightCount); on a light the
+// (black without a map) with pdf 1/(2 pi lightCount); on a light the
 // light-side pdf (pdfLight, raytrace.rgen:334-339) and the BSDF-side weight.
 // base = throughput * f * cos / (numNEE * pdfMat), base.w = pdfMat.
 RTX_DI V3 probeContribution(const DevScene &S, bool power, V3 o, V3 d, int prim, float t, float bu, float bv,
                             float4 base) {
     float pdfMat = base.w;
     if (prim < 0) {
-        V3 lightColor = v3(0.0f);
+        V3 lightColor = envColor(S, normalize(d));
         float pdfL = kInv2Pi / (float)S.lightCount;
         float h = misWeight(power, pdfMat, pdfL);
         return v3(base) * lightColor * h;
@@ -253,7 +260,7 @@ __global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, const FrameCon
         V3 o = v3(o4), d = v3(d4);
         TravRay r = makeTravRay(o, d);
         float tmax = d4.w, bu = 0.0f, bv = 0.0f;
-        int prim = traverse<false, COUNT>(S, r, o4.w, tmax, bu, bv, st, vc);
+        int prim = traverse<false, COUNT>(S, r, o4.w, tmax, bu, bv, st, vc, raySeed(S, q, i));
         V3 contrib = probeContribution(S, power, o, d, prim, tmax, bu, bv, base);
         pb.pendProbe[q.id[i]] = f4(contrib, __int_as_float(prim));
     }
@@ -263,48 +270,71 @@ __global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, const FrameCon
 // --------------------------------------------------- persistent traversal
 // KIND 0: extend queue (count extend[qsel]); KIND 2: probe queue (count probe).
 // KIND is a template parameter so that profiles tell the two apart.
-template <bool COUNT, int KIND>
-__global__ __launch_bounds__(kTraceBlock) void kClosestP(DevScene S, Queue q, Counters *cnt, int qsel) {
+// The persistent kernels come in two builds: without the any-hit alpha test
+// (scenes with no textured triangles, the hot path) and with it (kClosestPA,
+// kShadowPA), so that the test's code stays out of the common traversal loop.
+template <bool COUNT, int KIND, bool ALPHA>
+RTX_DI void closestBody(const DevScene &S, const Queue &q, Counters *cnt, int qsel, int *stackMem) {
     constexpr int kind = KIND;
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
     const uint32_t n = kind == 0 ? cnt->extend[qsel] : cnt->probe;
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(kind == 0 ? &cnt->extendRays : &cnt->probeRays, (unsigned long long)n);
     HybridStack<kTraceBlock, kLdsStack> st;
     st.lds = stackMem + threadIdx.x;
     VisitCount vc;
-    persistentTrace<false, COUNT>(
+    persistentTrace<false, COUNT, ALPHA>(
         S, n, &cnt->fetch[kind], st, vc,
         [&](uint32_t i, LaneRay &L) {
             float4 o = q.org[i], d = q.dir[i];
             L.r = makeTravRay(v3(o), v3(d));
             L.tmin = o.w;
             L.tmax = d.w;
+            if (ALPHA) L.randomUInt = raySeed(S, q, i);
         },
         [&](const LaneRay &L) { q.hit[L.idx] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv); });
     flushVisits<COUNT>(vc, cnt, kind);
 }
-
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kShadowP(DevScene S, Queue q, PathBuffers pb, Counters *cnt) {
+template <bool COUNT, int KIND>
+__global__ __launch_bounds__(kTraceBlock) void kClosestP(DevScene S, Queue q, Counters *cnt, int qsel) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
+    closestBody<COUNT, KIND, false>(S, q, cnt, qsel, stackMem);
+}
+template <bool COUNT, int KIND>
+__global__ __launch_bounds__(kTraceBlock) void kClosestPA(DevScene S, Queue q, Counters *cnt, int qsel) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    closestBody<COUNT, KIND, true>(S, q, cnt, qsel, stackMem);
+}
+
+template <bool COUNT, bool ALPHA>
+RTX_DI void shadowBody(const DevScene &S, const Queue &q, const PathBuffers &pb, Counters *cnt, int *stackMem) {
     const uint32_t n = cnt->shadow;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->shadowRays, (unsigned long long)n);
     HybridStack<kTraceBlock, kLdsStack> st;
     st.lds = stackMem + threadIdx.x;
     VisitCount vc;
-    persistentTrace<true, COUNT>(
+    persistentTrace<true, COUNT, ALPHA>(
         S, n, &cnt->fetch[1], st, vc,
         [&](uint32_t i, LaneRay &L) {
             float4 o = q.org[i], d = q.dir[i];
             L.r = makeTravRay(v3(o), v3(d));
             L.tmin = o.w;
             L.tmax = d.w;
+            if (ALPHA) L.randomUInt = raySeed(S, q, i);
         },
         [&](const LaneRay &L) {
             if (L.best < 0) pb.pendShadow[q.id[L.idx]].w = 1.0f; // unoccluded: raytrace.shadow.rmiss:7-10
         });
     flushVisits<COUNT>(vc, cnt, 1);
+}
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kShadowP(DevScene S, Queue q, PathBuffers pb, Counters *cnt) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    shadowBody<COUNT, false>(S, q, pb, cnt, stackMem);
+}
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kShadowPA(DevScene S, Queue q, PathBuffers pb, Counters *cnt) {
+    __shared__ int stackMem[kLdsStack * kTraceBlock];
+    shadowBody<COUNT, true>(S, q, pb, cnt, stackMem);
 }
 
 // The BSDF-side half of the MIS probe (raytrace.rgen:684-726) for probe rays
@@ -364,8 +394,8 @@ RTX_DI void pushSplit(const PathBuffers &pb, uint32_t pid, int4 &task, const Hit
     float4 *sr = pb.splitRec + ((size_t)pid * kMaxSplits + (uint32_t)task.y) * 4;
     sr[0] = f4(h.worldPos, __int_as_float(depth));
     sr[1] = f4(h.normal, __uint_as_float((uint32_t)h.matIndex | (h.frontFace ? 0x80000000u : 0u)));
-    sr[2] = f4(wi, 0.0f);
-    sr[3] = f4(thr, 0.0f);
+    sr[2] = f4(wi, h.u);
+    sr[3] = f4(thr, h.v);
     task.y++;
 }
 
@@ -467,10 +497,12 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                 h.frontFace = (mf >> 31) != 0;
                 h.isSphere = false;
                 h.instanceIndex = 0;
-                h.u = h.v = 0.0f;
+                h.u = c.w;
+                h.v = t.w;
                 d = -v3(c);
                 thr = v3(t);
                 m = loadMat(&S.materials[h.matIndex]);
+                applyTextures(S, m, h.u, h.v);
                 depth = __float_as_int(a.w);
                 followCount = 0;
                 follow = true;
@@ -486,11 +518,12 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             follow = (flags & kFlagFollow) != 0;
             addNext = (flags & kFlagAddNext) != 0;
             if (prim < 0) {
-                // raytrace.rmiss: black default environment
+                // raytrace.rmiss: the environment map (black when the scene has none)
                 if (addDirect || addNext) {
-                    res = res + thr * v3(0.0f);
+                    const V3 missColor = envColor(S, normalize(d));
+                    res = res + thr * missColor;
                     if (record) {
-                        updateSamples(pb, pid, task.w, grec.x, v3(0.0f));
+                        updateSamples(pb, pid, task.w, grec.x, missColor);
                         if (grec.y != -1) recordSlots(pb, pid)[grec.y].distance = 0.0f; // infinitely far
                     }
                 }
@@ -498,6 +531,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             } else {
                 h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
                 m = loadMat(&S.materials[h.matIndex]);
+                applyTextures(S, m, h.u, h.v);
                 if (depth == 1) firstT = hit.x;
                 if (b.firstPath + pid == fc.debugPath)
                     printf("[gpu] depth %d prim %d t %.9g mat %d seed %u thr %.6g %.6g %.6g d %.6g %.6g %.6g n %.6g %.6g %.6g ff %d res %.6g\n",
@@ -587,7 +621,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             const uint32_t slot = pid * numNEE + j;
             if (doNee) {
                 V3 lightColor;
-                float pdfL = sampleLights(S, rng, h.worldPos, lightDir, lightColor, lightDist);
+                float pdfL = sampleLights(S, rng, h.worldPos, h.normal, lightDir, lightColor, lightDist);
                 float cosS = dot(h.normal, lightDir);
                 bool traced = cosS > 0.0f && pdfL > 0.0f;
                 bool probeAllowed = true;
@@ -605,7 +639,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                             VisitCount vc0;
                             TravRay sr = makeTravRay(h.worldPos, lightDir);
                             float tm = lightDist * (1.0f - 0.0001f), bu, bv;
-                            bool occl = traverse<true, false>(S, sr, kTMin, tm, bu, bv, ps, vc0) >= 0;
+                            bool occl = traverse<true, false>(S, sr, kTMin, tm, bu, bv, ps, vc0, pb.frameSeed[pid]) >= 0;
                             traced = false;
                             shadowState = occl ? -1.0f : 1.0f; // resolved here; contributes nothing
                             if (!occl) probeAllowed = false;
@@ -915,13 +949,14 @@ __global__ __launch_bounds__(kTraceBlock) void kQueryP(DevScene S, const rtx_ray
     HybridStack<kTraceBlock, kLdsStack> st;
     st.lds = stackMem + threadIdx.x;
     VisitCount vc;
-    persistentTrace<ANY, false>(
+    persistentTrace<ANY, false, true>( // alpha-tested triangles with randomUInt 0
         S, n, fetch, st, vc,
         [&](uint32_t i, LaneRay &L) {
             rtx_ray ry = rays[i];
             L.r = makeTravRay(v3(ry.org), v3(ry.dir));
             L.tmin = ry.tmin;
             L.tmax = ry.tmax;
+            L.randomUInt = 0;
         },
         [&](const LaneRay &L) {
             if (ANY) {
@@ -957,18 +992,36 @@ void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int 
 }
 void launchClosestP(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, int kind, bool countVisits,
                     int blocks) {
+    const dim3 g(blocks), t(kTraceBlock);
+    const bool a = S.alphaTris != 0;
     if (kind == 0) {
-        if (countVisits) hipLaunchKernelGGL((kClosestP<true, 0>), dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
-        else hipLaunchKernelGGL((kClosestP<false, 0>), dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+        if (a) {
+            if (countVisits) hipLaunchKernelGGL((kClosestPA<true, 0>), g, t, 0, s, S, q, cnt, qsel);
+            else hipLaunchKernelGGL((kClosestPA<false, 0>), g, t, 0, s, S, q, cnt, qsel);
+        } else {
+            if (countVisits) hipLaunchKernelGGL((kClosestP<true, 0>), g, t, 0, s, S, q, cnt, qsel);
+            else hipLaunchKernelGGL((kClosestP<false, 0>), g, t, 0, s, S, q, cnt, qsel);
+        }
     } else {
-        if (countVisits) hipLaunchKernelGGL((kClosestP<true, 2>), dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
-        else hipLaunchKernelGGL((kClosestP<false, 2>), dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+        if (a) {
+            if (countVisits) hipLaunchKernelGGL((kClosestPA<true, 2>), g, t, 0, s, S, q, cnt, qsel);
+            else hipLaunchKernelGGL((kClosestPA<false, 2>), g, t, 0, s, S, q, cnt, qsel);
+        } else {
+            if (countVisits) hipLaunchKernelGGL((kClosestP<true, 2>), g, t, 0, s, S, q, cnt, qsel);
+            else hipLaunchKernelGGL((kClosestP<false, 2>), g, t, 0, s, S, q, cnt, qsel);
+        }
     }
 }
 void launchShadowP(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
                    int blocks) {
-    if (countVisits) hipLaunchKernelGGL(kShadowP<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
-    else hipLaunchKernelGGL(kShadowP<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
+    const dim3 g(blocks), t(kTraceBlock);
+    if (S.alphaTris != 0) {
+        if (countVisits) hipLaunchKernelGGL(kShadowPA<true>, g, t, 0, s, S, q, pb, cnt);
+        else hipLaunchKernelGGL(kShadowPA<false>, g, t, 0, s, S, q, pb, cnt);
+    } else {
+        if (countVisits) hipLaunchKernelGGL(kShadowP<true>, g, t, 0, s, S, q, pb, cnt);
+        else hipLaunchKernelGGL(kShadowP<false>, g, t, 0, s, S, q, pb, cnt);
+    }
 }
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
                         Counters *cnt, int blocks) {

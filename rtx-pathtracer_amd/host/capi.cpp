@@ -136,6 +136,33 @@ int rtxh_scene_set_view(rtxh_scene *scene, const rtxh_view *view) {
     return RTX_OK;
 }
 
+int rtxh_scene_add_texture(rtxh_scene *scene, const uint8_t *rgba, uint32_t width, uint32_t height, int32_t *slot) {
+    if (!scene || !rgba) return fail("NULL argument");
+    if (width == 0 || height == 0 || (uint64_t)width * height >= (1ull << 31)) return fail("bad texture size");
+    rtxh::Texture t;
+    t.width = width;
+    t.height = height;
+    t.rgba.assign(rgba, rgba + (size_t)width * height * 4);
+    int s = scene->scene.addTexture(std::move(t), std::string());
+    if (slot) *slot = s;
+    return RTX_OK;
+}
+
+int rtxh_scene_set_environment(rtxh_scene *scene, const float *rgba, uint32_t width, uint32_t height, int addLight) {
+    if (!scene || !rgba) return fail("NULL argument");
+    if (width == 0 || height == 0) return fail("bad environment map size");
+    scene->scene.envMap.assign(rgba, rgba + (size_t)width * height * 4);
+    scene->scene.envWidth = width;
+    scene->scene.envHeight = height;
+    if (addLight) {
+        rtx_light light;
+        std::memset(&light, 0, sizeof(light));
+        light.type = RTX_LIGHT_ENV_MAP;
+        scene->scene.lights.push_back(light);
+    }
+    return RTX_OK;
+}
+
 int rtxh_scene_finalize(rtxh_scene *scene) {
     if (!scene) return fail("NULL argument");
     scene->scene.buildLightTables();

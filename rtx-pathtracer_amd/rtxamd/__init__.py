@@ -66,6 +66,9 @@ def host_lib():
                                             C.POINTER(C.c_int32)]
         L.rtxh_scene_add_point_light.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.rtxh_scene_set_view.argtypes = [C.c_void_p, C.POINTER(View)]
+        L.rtxh_scene_add_texture.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32,
+                                             C.POINTER(C.c_int32)]
+        L.rtxh_scene_set_environment.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]
         L.rtxh_scene_finalize.argtypes = [C.c_void_p]
         L.rtxh_pcg32_first.argtypes = [C.c_uint64, C.c_uint64]
         L.rtxh_pcg32_first.restype = C.c_uint32
@@ -143,8 +146,10 @@ class HostScene:
         _host_check(host_lib().rtxh_scene_add_material(self._h, C.byref(mat), C.byref(i)))
         return i.value
 
-    def add_mesh(self, positions, indices, materials, normals=None, transform=None, normal_transform=None):
-        """positions (n,3), indices (m,3), materials: per-vertex or scalar material index."""
+    def add_mesh(self, positions, indices, materials, normals=None, transform=None, normal_transform=None,
+                 texcoords=None):
+        """positions (n,3), indices (m,3), materials: per-vertex or scalar material index,
+        texcoords (n,2) or None."""
         positions = np.asarray(positions, np.float32).reshape(-1, 3)
         indices = np.ascontiguousarray(np.asarray(indices, np.uint32).reshape(-1))
         n = positions.shape[0]
@@ -158,10 +163,13 @@ class HostScene:
                 normals[tri[:, k]] = fn
         normals = np.asarray(normals, np.float32).reshape(-1, 3)
         mats = np.broadcast_to(np.asarray(materials, np.int32), (n,))
+        uv = None if texcoords is None else np.asarray(texcoords, np.float32).reshape(-1, 2)
         for i in range(n):
             verts[i].pos[:] = positions[i].tolist()
             verts[i].normal[:] = normals[i].tolist()
             verts[i].materialIndex = int(mats[i])
+            if uv is not None:
+                verts[i].texCoord[:] = uv[i].tolist()
         fp = C.POINTER(C.c_float)
         tr = (C.c_float * 16)(*np.asarray(transform, np.float32).reshape(16, order="F")) if transform is not None else None
         ntr = (C.c_float * 16)(*np.asarray(normal_transform, np.float32).reshape(16, order="F")) \
@@ -188,6 +196,22 @@ class HostScene:
         v.up[:] = list(up)
         v.vfov = vfov
         _host_check(host_lib().rtxh_scene_set_view(self._h, C.byref(v)))
+
+    def add_texture(self, rgba):
+        """(h, w, 4) uint8 RGBA (RGB sRGB-encoded) -> texture slot for Material.textureIdDiffuse."""
+        a = np.ascontiguousarray(rgba, np.uint8)
+        h, w = a.shape[:2]
+        slot = C.c_int32()
+        _host_check(host_lib().rtxh_scene_add_texture(self._h, a.ctypes.data_as(C.POINTER(C.c_uint8)), w, h,
+                                                      C.byref(slot)))
+        return slot.value
+
+    def set_environment(self, rgba, add_light=True):
+        """(h, w, 4) float32 lat-long environment map; add_light appends the environment light."""
+        a = np.ascontiguousarray(rgba, np.float32)
+        h, w = a.shape[:2]
+        _host_check(host_lib().rtxh_scene_set_environment(self._h, a.ctypes.data_as(C.POINTER(C.c_float)), w, h,
+                                                          1 if add_light else 0))
 
     def finalize(self):
         _host_check(host_lib().rtxh_scene_finalize(self._h))

@@ -129,3 +129,74 @@ def cache_state(records, max_entries, irradiance_a=0.2):
         st.spheres[i]["radius"] = np.float32(irradiance_a) * np.float32(hr)
     st.hdr.nextCacheSlot = len(records)
     return st
+
+
+def constant_env(radiance, w=8, h=4):
+    return np.full((h, w, 4), radiance, np.float32)
+
+
+def furnace(albedo=0.6, radiance=1.5, env_light=True, point_light=False):
+    """A diffuse sphere of albedo `albedo` (radius 1 at the origin) under a
+    constant environment of radiance `radiance`.  A convex object never sees
+    itself, so every pixel that hits it has expected value albedo *
+    radiance and every other pixel exactly `radiance`."""
+    s = rtxamd.HostScene.new()
+    m = s.add_material(rtxamd.diffuse((albedo,) * 3))
+    s.add_sphere((0.0, 0.0, 0.0), 1.0, m)
+    if point_light:  # a second light: the environment light's sample probability halves
+        s.add_point_light((0.0, 0.0, 50.0), (0.0, 0.0, 0.0))
+    s.set_environment(constant_env(radiance), add_light=env_light)
+    s.set_view((0.0, -4.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 40.0)
+    return s.finalize()
+
+
+def textured_plate(texture, diffuse=(1.0, 1.0, 1.0), radiance=2.0, alpha_black=False):
+    """A 2 x 2 quad at z = 0 facing +z with texture coordinates spanning
+    [0,1]^2 and a diffuse texture, under a constant environment; the camera
+    looks straight down from z = 3.  With alpha_black the quad's diffuse
+    colour is 0, so a pixel is `radiance` where the any-hit test lets the
+    camera ray through and 0 where it does not."""
+    s = rtxamd.HostScene.new()
+    slot = s.add_texture(texture)
+    m = rtxamd.diffuse((0.0, 0.0, 0.0) if alpha_black else diffuse)
+    m.textureIdDiffuse = slot
+    mi = s.add_material(m)
+    p, i = quad((-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0))
+    s.add_mesh(p, i, mi, texcoords=[(0, 0), (1, 0), (1, 1), (0, 1)])
+    s.set_environment(constant_env(radiance), add_light=True)
+    s.set_view((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 30.0)
+    return s.finalize()
+
+
+def env_gallery():
+    """A non-constant environment (smooth colour bands), a textured Phong
+    box, a Lambert floor and an area light: every environment and texture
+    path of the shaders (miss colour, environment NEE, MIS probe misses,
+    textured diffuse and Phong) in one scene."""
+    s = rtxamd.HostScene.new()
+    lamp = s.add_material(rtxamd.light((6.0, 6.0, 6.0)))
+    floor = s.add_material(rtxamd.diffuse((0.5, 0.5, 0.5)))
+    y, x = np.mgrid[0:16, 0:32]
+    tex = np.stack([(x * 8) % 256, (y * 16) % 256, ((x + y) * 5) % 256, np.full_like(x, 255)], -1).astype(np.uint8)
+    slot = s.add_texture(tex)
+    ph = rtxamd.phong((0.6, 0.6, 0.6), (0.3, 0.3, 0.3), 20.0)
+    ph.textureIdDiffuse = slot
+    phm = s.add_material(ph)
+    dm = rtxamd.diffuse((0.9, 0.9, 0.9))
+    dm.textureIdDiffuse = slot
+    dmi = s.add_material(dm)
+    p, i = quad((-0.3, -0.3, 2.5), (-0.3, 0.3, 2.5), (0.3, 0.3, 2.5), (0.3, -0.3, 2.5))
+    s.add_mesh(p, i, lamp)
+    p, i = quad((-3, -3, 0), (3, -3, 0), (3, 3, 0), (-3, 3, 0))
+    s.add_mesh(p, i, floor)
+    p, i = box((-0.5, -0.5, 0.0), (0.1, 0.1, 0.8))
+    uv = np.tile(np.array([(0, 0), (1, 0), (1, 1), (0, 1)], np.float32), (6, 1))
+    s.add_mesh(p, i, phm, texcoords=uv)
+    p, i = quad((0.3, -0.8, 0.01), (1.3, -0.8, 0.01), (1.3, 0.2, 0.01), (0.3, 0.2, 0.01))
+    s.add_mesh(p, i, dmi, texcoords=[(0, 0), (2, 0), (2, 2), (0, 2)])
+    ey, ex = np.mgrid[0:8, 0:16]
+    env = np.stack([0.2 + 0.1 * ex, 0.3 + 0.2 * ey, np.full_like(ex, 0.5, dtype=np.float64) + 0.05 * (ex % 3),
+                    np.ones_like(ex, dtype=np.float64)], -1).astype(np.float32)
+    s.set_environment(env, add_light=True)
+    s.set_view((0.0, -4.0, 2.0), (0.0, 0.0, 0.4), (0.0, 0.0, 1.0), 45.0)
+    return s.finalize()
