@@ -305,9 +305,13 @@ typedef struct rtx_scene_desc {
 #define RTX_FRAME_COUNT_VISITS 0x2u /* count BVH node/primitive fetches       */
 #define RTX_FRAME_RESET_ACCUM 0x4u  /* clear the accumulation image first     */
 #define RTX_FRAME_TIME_KERNELS 0x8u /* time every kernel with HIP events      */
+#define RTX_FRAME_NO_GUIDING_UPDATE 0x10u /* record (pc.updateGuiding) but skip the
+                                           * guiding update after the frame     */
 
-/* One call to rtx_render renders `spp` samples per pixel and adds them to the
- * context's accumulation image.
+/* One call to rtx_render renders `spp` frames and adds each frame's result
+ * (per pixel: the mean of pc.samplesPerPixel camera samples and their
+ * splits, main() of shaders/raytrace.rgen:1623-1711) to the context's
+ * accumulation image.
  *
  * Random streams.  The reference seeds one LCG stream per pixel with
  * tea(pixelIndex, pc.randomUInt) (shaders/raytrace.rgen:1625) where
@@ -317,7 +321,7 @@ typedef struct rtx_scene_desc {
  * (initstate = seed, initseq = sampleOffset + s).  Each (pixel, sample)
  * path therefore has its own stream that does not depend on batching. */
 typedef struct rtx_frame_params {
-    rtx_push_constant pc; /* pc.randomUInt and pc.samplesPerPixel are ignored */
+    rtx_push_constant pc; /* pc.randomUInt and pc.previousFrames are ignored */
     rtx_camera cam;
     uint32_t width;
     uint32_t height;
@@ -404,7 +408,7 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *desc, rtx_scene **out);
 int rtx_scene_destroy(rtx_ctx *ctx, rtx_scene *scene);
 int rtx_scene_get_info(const rtx_scene *scene, rtx_scene_info *info);
 
-/* Render params->spp samples per pixel into the accumulation image
+/* Render params->spp frames into the accumulation image
  * (replaces RayTracingApp::raytrace + traceRaysKHR, src/RayTracingApp.cpp:
  * 1120-1204, and the per-pixel main() of shaders/raytrace.rgen:1623-1827).
  * Synchronous: returns when the device work is done. stats may be NULL. */
@@ -552,6 +556,73 @@ int rtx_trace_closest(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
  * primitive is hit in [tmin, tmax]. */
 int rtx_trace_occluded(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
                        size_t n, uint8_t *occluded);
+
+/* ------------------------------------------------------ frame schedule */
+/* The host loop around the render of the reference's application,
+ * RayTracingApp::drawFrame and RayTracingApp::raytrace
+ * (src/RayTracingApp.cpp:91-215, 1120-1163, 1206-1217), as a state machine
+ * that plans each frame's push constants and accumulation:
+ *   - progressive accumulation: previousFrames counts frames since the last
+ *     change (rtx_schedule_input_changed, rtx_schedule_set_push_constants);
+ *   - irradiance-cache prepare frames: the first prepareFrames frames with
+ *     useIrradianceCache or useADRRS are prepare frames (ADRRS runs them as
+ *     cache frames with useIrradianceCacheOnGlossy), then ADRRS renders the
+ *     estimate frame (setEstimateRTSettings) and returns to the saved
+ *     settings with useADRRS;
+ *   - guiding cadence: recording frames (pc.updateGuiding) are followed by a
+ *     guiding update until guidingOptimizations updates have run; the next
+ *     recording frame is rendered without one and recording stops;
+ *   - eval modes: evalSamples stops after that many samples per pixel
+ *     outside prepare frames and reports the wall time since the first eval
+ *     frame; evalSeconds counts the samples rendered in that many seconds.
+ * A frame's randomUInt comes from PCG32 (seed, sampleOffset = frames so far)
+ * as in rtx_render.  The structs have no counterpart in the reference: their
+ * fields are RayTracingApp's (src/RayTracingApp.h:147-190). */
+typedef struct rtx_schedule rtx_schedule;
+
+typedef struct rtx_schedule_config {
+    int32_t prepareFrames;        /* irradianceCachePrepareFrames (reference: 50) */
+    int32_t guidingOptimizations; /* numGuidingOptimizations (reference: 6; -1 = no limit) */
+    int32_t accumulate;           /* accumulateResults */
+    int32_t evalSamples;          /* evalXSamples with evalNumSamples = this; 0 = off */
+    int32_t evalSeconds;          /* evalXSeconds with evalNumSeconds = this; 0 = off */
+    int32_t sppToOneAfterEval;    /* setSPPToOneAfterEval */
+} rtx_schedule_config;
+
+typedef struct rtx_schedule_state {
+    uint64_t frames;                     /* frames rendered so far */
+    int32_t currentPrepareFrames;
+    int32_t currentGuidingOptimizations; /* -1 when not recording */
+    uint32_t previousFrames;             /* of the last planned frame */
+    int32_t evalSamplesRunning;          /* evalXSamples */
+    int32_t evalSecondsRunning;          /* evalXSeconds */
+    int32_t evalCurrentSamples;
+    int64_t evalSecondsTotalSamples;     /* evalXSecondsTotalSamples */
+    int64_t evalSecondsResult;           /* evalXSecondsLastResult */
+    double evalSamplesMs;                /* evalXSamplesTime; -1 until an eval finished */
+    double guidingUpdateMs;              /* guidingUpdateTimeMs; -1 until recording stopped */
+    rtx_push_constant pc;                /* rtPushConstants for the next frame */
+} rtx_schedule_state;
+
+void rtx_schedule_config_defaults(rtx_schedule_config *config);
+/* pc: the application's settings (rtPushConstants) at start. */
+int rtx_schedule_create(const rtx_schedule_config *config, const rtx_push_constant *pc, rtx_schedule **out);
+int rtx_schedule_destroy(rtx_schedule *schedule);
+/* A settings change (a UI edit: hasInputChanged); restarts accumulation. */
+int rtx_schedule_set_push_constants(rtx_schedule *schedule, const rtx_push_constant *pc);
+/* The camera or scene changed: accumulation restarts with the next frame. */
+int rtx_schedule_input_changed(rtx_schedule *schedule);
+/* Plans the next frame: *frame = *base (camera, size, seed, flags, ...) with
+ * the frame's push constants, spp = 1, sampleOffset = frames so far and
+ * RTX_FRAME_RESET_ACCUM when accumulation restarts.  No device work. */
+int rtx_schedule_begin_frame(rtx_schedule *schedule, const rtx_frame_params *base, rtx_frame_params *frame);
+/* The bookkeeping after the planned frame was rendered: guiding cadence and
+ * the eval counters. */
+int rtx_schedule_end_frame(rtx_schedule *schedule);
+/* begin_frame, rtx_render, end_frame. */
+int rtx_schedule_frame(rtx_ctx *ctx, const rtx_scene *scene, rtx_schedule *schedule,
+                       const rtx_frame_params *base, rtx_frame_stats *stats);
+int rtx_schedule_get_state(const rtx_schedule *schedule, rtx_schedule_state *state);
 
 #ifdef __cplusplus
 } /* extern "C" */

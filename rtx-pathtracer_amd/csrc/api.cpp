@@ -1015,7 +1015,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         }
         // PathGuiding::update after every recording frame when the context owns the mixtures
         // (src/RayTracingApp.cpp:143-151)
-        if (record && ctx->fitValid) {
+        if (record && ctx->fitValid && !(p->flags & RTX_FRAME_NO_GUIDING_UPDATE)) {
             int rcu = 0;
             timed(kClsShade, [&] { rcu = guideUpdateAsync(ctx); });
             if (rcu != RTX_OK) return rcu;

@@ -110,6 +110,18 @@ def device_lib():
         L.rtx_guiding_update.argtypes = [C.c_void_p]
         L.rtx_guiding_set_options.argtypes = [C.c_void_p, C.POINTER(abi.GuidingOptions)]
         L.rtx_guiding_get_options.argtypes = [C.c_void_p, C.POINTER(abi.GuidingOptions)]
+        L.rtx_schedule_config_defaults.argtypes = [C.POINTER(abi.ScheduleConfig)]
+        L.rtx_schedule_config_defaults.restype = None
+        L.rtx_schedule_create.argtypes = [C.POINTER(abi.ScheduleConfig), C.POINTER(abi.PushConstant),
+                                          C.POINTER(C.c_void_p)]
+        L.rtx_schedule_destroy.argtypes = [C.c_void_p]
+        L.rtx_schedule_set_push_constants.argtypes = [C.c_void_p, C.POINTER(abi.PushConstant)]
+        L.rtx_schedule_input_changed.argtypes = [C.c_void_p]
+        L.rtx_schedule_begin_frame.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.POINTER(abi.FrameParams)]
+        L.rtx_schedule_end_frame.argtypes = [C.c_void_p]
+        L.rtx_schedule_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(abi.FrameParams),
+                                         C.POINTER(abi.FrameStats)]
+        L.rtx_schedule_get_state.argtypes = [C.c_void_p, C.POINTER(abi.ScheduleState)]
         L.rtx_estimate_write.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
         L.rtx_estimate_read.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
@@ -658,3 +670,65 @@ class Renderer:
         self._check(device_lib().rtx_trace_occluded(self._h, scene._h, rays.ctypes.data_as(C.POINTER(Ray)), n,
                                                     occ.ctypes.data_as(C.POINTER(C.c_uint8))))
         return occ
+
+
+def schedule_config(**kw):
+    """rtx_schedule_config with the reference's defaults (50 prepare frames,
+    6 guiding optimizations, accumulation on), fields overridden by kw."""
+    c = abi.ScheduleConfig()
+    device_lib().rtx_schedule_config_defaults(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Schedule:
+    """The frame schedule of RayTracingApp (rtx_schedule_*): plans each
+    frame's push constants and accumulation, renders it, keeps the counters."""
+
+    def __init__(self, pc, config=None, **kw):
+        self._h = C.c_void_p()
+        cfg = config if config is not None else schedule_config(**kw)
+        rc = device_lib().rtx_schedule_create(C.byref(cfg), C.byref(pc), C.byref(self._h))
+        if rc != abi.RTX_OK:
+            raise RtxError(rc, "rtx_schedule_create")
+
+    def close(self):
+        if self._h:
+            device_lib().rtx_schedule_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != abi.RTX_OK:
+            raise RtxError(rc, what)
+
+    def set_push_constants(self, pc):
+        self._check(device_lib().rtx_schedule_set_push_constants(self._h, C.byref(pc)), "set_push_constants")
+
+    def input_changed(self):
+        self._check(device_lib().rtx_schedule_input_changed(self._h), "input_changed")
+
+    def begin_frame(self, base):
+        out = abi.FrameParams()
+        self._check(device_lib().rtx_schedule_begin_frame(self._h, C.byref(base), C.byref(out)), "begin_frame")
+        return out
+
+    def end_frame(self):
+        self._check(device_lib().rtx_schedule_end_frame(self._h), "end_frame")
+
+    def frame(self, renderer, scene, base):
+        st = abi.FrameStats()
+        rc = device_lib().rtx_schedule_frame(renderer._h, scene._h, self._h, C.byref(base), C.byref(st))
+        renderer._check(rc)
+        return st
+
+    def state(self):
+        st = abi.ScheduleState()
+        self._check(device_lib().rtx_schedule_get_state(self._h, C.byref(st)), "get_state")
+        return st

@@ -18,6 +18,7 @@ RTX_FRAME_KEEP_PATHS = 0x1
 RTX_FRAME_COUNT_VISITS = 0x2
 RTX_FRAME_RESET_ACCUM = 0x4
 RTX_FRAME_TIME_KERNELS = 0x8
+RTX_FRAME_NO_GUIDING_UPDATE = 0x10
 
 MAT_DIFFUSE, MAT_SPECULAR, MAT_DIELECTRIC, MAT_LIGHT, MAT_PHONG, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR = range(7)
 LIGHT_AREA, LIGHT_POINT, LIGHT_SPHERE, LIGHT_ENV_MAP = range(4)
@@ -176,6 +177,20 @@ class FrameStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("traversalKernel", C.c_int32), ("traceBlocksPerCU", C.c_int32),
                 ("shadeBlocksPerCU", C.c_int32), ("batchPaths", C.c_uint32), ("_reserved", C.c_int32 * 4)]
+
+
+class ScheduleConfig(C.Structure):
+    _fields_ = [("prepareFrames", C.c_int32), ("guidingOptimizations", C.c_int32), ("accumulate", C.c_int32),
+                ("evalSamples", C.c_int32), ("evalSeconds", C.c_int32), ("sppToOneAfterEval", C.c_int32)]
+
+
+class ScheduleState(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("currentPrepareFrames", C.c_int32),
+                ("currentGuidingOptimizations", C.c_int32), ("previousFrames", C.c_uint32),
+                ("evalSamplesRunning", C.c_int32), ("evalSecondsRunning", C.c_int32),
+                ("evalCurrentSamples", C.c_int32), ("evalSecondsTotalSamples", C.c_int64),
+                ("evalSecondsResult", C.c_int64), ("evalSamplesMs", C.c_double), ("guidingUpdateMs", C.c_double),
+                ("pc", PushConstant)]
 
 
 class GuidingOptions(C.Structure):

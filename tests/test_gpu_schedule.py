@@ -1,0 +1,100 @@
+"""The frame schedule on the device: rtx_schedule_frame renders exactly the
+frames its plan describes (the same images as planning with
+begin_frame/end_frame and calling rtx_render by hand), the ADRRS sequence
+(prepare frames, estimate frame, ADRRS frames) runs end to end with the
+estimate frame in the average as in the reference, and the guiding cadence
+skips the update after the last recording frame."""
+import numpy as np
+import pytest
+
+import rtxamd
+import scenes
+from parity import log
+from rtxamd import abi
+
+pytestmark = pytest.mark.gpu
+
+W, H = 48, 32
+
+
+def base(hs, **kw):
+    fp = scenes.params(hs, W, H, 1, flags=0, **kw)
+    return fp
+
+
+def samples(renderer):
+    n = rtxamd.C.c_uint32()
+    renderer._check(rtxamd.device_lib().rtx_accumulated_samples(renderer._h, rtxamd.C.byref(n)))
+    return n.value
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    return rtxamd.HostScene.generate(abi.SCENE_CORNELL)
+
+
+def test_scheduled_frames_equal_planned_renders(renderer, cornell):
+    hs = cornell
+    ds = renderer.upload(hs)
+    fp = base(hs, useIrradianceCache=1, useIrradianceCacheOnGlossy=1, maxDepth=6, enableMIS=1,
+              irradianceCreateProb=0.01)
+    images = []
+    for mode in ("schedule", "manual"):
+        renderer.ic_create(2000)
+        s = rtxamd.Schedule(fp.pc, prepareFrames=3)
+        for _ in range(6):
+            if mode == "schedule":
+                s.frame(renderer, ds, fp)
+            else:
+                f = s.begin_frame(fp)
+                renderer.render(ds, f)
+                s.end_frame()
+        images.append(renderer.read_image(W, H))
+        assert samples(renderer) == 3  # prepare frames are not accumulated
+        assert s.state().frames == 6
+    assert np.array_equal(images[0], images[1])
+    ds.close()
+
+
+def test_adrrs_schedule_end_to_end(renderer, cornell):
+    hs = cornell
+    ds = renderer.upload(hs)
+    fp = base(hs, useADRRS=1, adrrsS=5.0, adrrsSplit=1, maxDepth=8, enableMIS=1, irradianceCreateProb=0.01)
+    renderer.ic_create(3000)
+    s = rtxamd.Schedule(fp.pc, prepareFrames=4)
+    for k in range(8):
+        s.frame(renderer, ds, fp)
+    img = renderer.read_image(W, H)
+    # the estimate frame and 3 ADRRS frames make the average (previousFrames 0..3)
+    assert samples(renderer) == 4
+    assert np.isfinite(img).all() and img[..., :3].mean() > 0
+    st = s.state()
+    assert st.pc.useADRRS == 1 and st.pc.storeEstimate == 0 and st.currentPrepareFrames == 5
+    est = renderer.estimate_read(W, H)
+    assert np.isfinite(est).all() and est[:, :3].mean() > 0
+    log(f"ADRRS schedule: image mean {img[..., :3].mean():.4f}, estimate mean {est[:, :3].mean():.4f}")
+    ds.close()
+
+
+def test_guiding_cadence_skips_the_last_update(renderer, cornell):
+    hs = cornell
+    ds = renderer.upload(hs)
+    box = ds.bounds()
+    fp = base(hs, updateGuiding=1, useGuiding=1, guidingProb=0.5, maxDepth=6, enableMIS=1)
+    # scheduled: two recording frames, the update after the first only
+    renderer.guiding_init(box, 2)
+    s = rtxamd.Schedule(fp.pc, guidingOptimizations=1)
+    s.frame(renderer, ds, fp)
+    s.frame(renderer, ds, fp)
+    assert s.state().pc.updateGuiding == 0
+    a_boxes, a_vmms = renderer.guiding_get()
+    # by hand: one recording frame (the context updates after it)
+    renderer.guiding_init(box, 2)
+    f = rtxamd.Schedule(fp.pc, guidingOptimizations=1).begin_frame(fp)
+    renderer.render(ds, f)
+    b_boxes, b_vmms = renderer.guiding_get()
+    assert np.array_equal(np.asarray(a_boxes), np.asarray(b_boxes))
+    assert len(a_vmms) == len(b_vmms)
+    for x, y in zip(a_vmms, b_vmms):
+        assert bytes(x) == bytes(y)
+    ds.close()
