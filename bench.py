@@ -35,7 +35,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT / "rtx-pathtracer_amd"), str(ROOT)]
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 64, 48, 32, 16
+RAY_BYTES, HIT_BYTES = 32, 16  # ray record in, hit record out; node and triangle records: the library's figures
 
 
 def log(*a):
@@ -142,6 +142,7 @@ def main():
     cnt = {f: sum(getattr(st, f) for st in csts) for f in ("extendRays", "shadowRays", "probeRays")}
     nodes = [sum(st.nodeVisitsByKind[k] for st in csts) for k in range(3)]
     prims = [sum(st.primVisitsByKind[k] for st in csts) for k in range(3)]
+    NODE_BYTES, TRI_BYTES = csts[0].nodeBytes, csts[0].primBytes  # bytes per node / triangle record read
     ext_rays = cnt["extendRays"]
     ext_bytes = ext_rays * (RAY_BYTES + HIT_BYTES) + nodes[0] * NODE_BYTES + prims[0] * TRI_BYTES
     all_rays = cnt["extendRays"] + cnt["shadowRays"] + cnt["probeRays"]

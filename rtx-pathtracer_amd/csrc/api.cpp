@@ -572,7 +572,7 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     int32_t *liD = nullptr;
     rtx_face_sample *ftD = nullptr;
     rtx_sphere *spD = nullptr;
-    if (e == hipSuccess) e = upload(&nodes, reinterpret_cast<const float4 *>(bvh.nodes.data()), bvh.nodes.size() * 4);
+    if (e == hipSuccess) e = upload(&nodes, reinterpret_cast<const float4 *>(bvh.nodes.data()), bvh.nodes.size() * 8);
     if (e == hipSuccess) e = upload(&trisD, tris.data(), tris.size());
     if (e == hipSuccess) e = upload(&prD, primRef.data(), primRef.size());
     if (e == hipSuccess) e = upload(&vD, verts.data(), verts.size());
@@ -643,7 +643,7 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     sc->info.spheres = d->sphereCount;
     sc->info.nodes = (uint32_t)bvh.nodes.size();
     sc->info.maxDepth = bvh.maxDepth;
-    sc->info.deviceBytes = bvh.nodes.size() * sizeof(BvhNode) + tris.size() * sizeof(float4) +
+    sc->info.deviceBytes = bvh.nodes.size() * sizeof(Bvh4Node) + tris.size() * sizeof(float4) +
                            primRef.size() * sizeof(int2) + verts.size() * sizeof(rtx_vertex) +
                            idx.size() * 4 + (size_t)d->materialCount * sizeof(rtx_material);
     sc->info.buildMs = bvh.buildMs;
@@ -1105,7 +1105,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         }
         stats->batches = batches;
         stats->bounces = (uint32_t)itersRun;
-        stats->nodeBytes = (uint32_t)sizeof(BvhNode);
+        stats->nodeBytes = kBvh4NodeBytesRead;
         stats->primBytes = 48;
     }
     return RTX_OK;

@@ -7,29 +7,42 @@
 namespace rtx {
 
 // ----------------------------------------------------------------- BVH
-// Binary BVH node holding both children's boxes (64 B, four 16-byte loads):
-//   a = (c0.lo.x, c0.lo.y, c0.lo.z, c0.hi.x)
-//   b = (c0.hi.y, c0.hi.z, c1.lo.x, c1.lo.y)
-//   c = (c1.lo.z, c1.hi.x, c1.hi.y, c1.hi.z)
-//   d = (link0, link1, -, -)
+// The builder makes a binary SAH tree (64-byte two-box nodes) and collapses
+// it into four-wide nodes, the layout the kernels read (128 B, eight 16-byte
+// slots, the first seven read):
+//   f[0] = lo.x of children 0-3   f[1] = hi.x
+//   f[2] = lo.y                   f[3] = hi.y
+//   f[4] = lo.z                   f[5] = hi.z
+//   link[0..3] (slot 6), pad (slot 7)
 // link >= 0: internal node index; link < 0: leaf, ~link = start << 4 | (count - 1).
+// An unused child slot has a point box at 1e30 that no ray reaches.
 struct BvhNode {
     float a[4], b[4], c[4];
     int32_t d[4];
 };
 static_assert(sizeof(BvhNode) == 64, "BvhNode must be 64 bytes");
 
+struct Bvh4Node {
+    float f[6][4];
+    int32_t link[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node must be 128 bytes");
+constexpr uint32_t kBvh4NodeBytesRead = 112; // the seven slots the traversal loads
+
 constexpr int kLeafMaxPrims = 16;
-constexpr int kMaxBvhDepth = 48; // traversal stack capacity (24 in LDS + 24 in registers/scratch)
+constexpr int kMaxBvhDepth = 48;   // depth limit of the binary build
+constexpr int kStackCapacity = 64; // traversal stack entries (24 in LDS, the rest in registers/scratch)
 
 inline int32_t makeLeaf(uint32_t start, uint32_t count) {
     return ~(int32_t)((start << 4) | (count - 1));
 }
 
 struct BvhBuildResult {
-    std::vector<BvhNode> nodes;
+    std::vector<Bvh4Node> nodes;
     std::vector<uint32_t> primOrder; // triangle ids in leaf order
-    uint32_t maxDepth = 0;
+    uint32_t maxDepth = 0;   // of the four-wide tree (levels of nodes)
+    uint32_t binaryDepth = 0;
     double buildMs = 0;
 };
 

@@ -1,6 +1,6 @@
 // Host BVH builder: binned surface-area heuristic (Wald, "On fast
-// construction of SAH-based bounding volume hierarchies", 2007), producing
-// the 64-byte two-box nodes the traversal kernels read.  It replaces the
+// construction of SAH-based bounding volume hierarchies", 2007) for a
+// binary tree, collapsed into the four-wide nodes the traversal kernels read.  It replaces the
 // driver's BLAS/TLAS build (src/SceneLoader.cpp:1101-1193), whose source
 // is not available.
 #include <algorithm>
@@ -161,6 +161,80 @@ struct Builder {
     }
 };
 
+// Collapse of the binary tree into four-wide nodes: a node's children start
+// as the binary node's two; the internal child with the largest surface
+// area is replaced by its two children until there are four or only leaves
+// remain.  Nodes are emitted parent first (depth-first), children in the
+// binary tree's left-to-right order.
+struct Collapser {
+    const std::vector<BvhNode> &in;
+    std::vector<Bvh4Node> out;
+    uint32_t maxDepth;
+
+    struct Child {
+        Box box;
+        int32_t link;
+    };
+    static Child childOf(const BvhNode &n, int k) {
+        Child c;
+        if (k == 0) {
+            c.box.lo[0] = n.a[0]; c.box.lo[1] = n.a[1]; c.box.lo[2] = n.a[2];
+            c.box.hi[0] = n.a[3]; c.box.hi[1] = n.b[0]; c.box.hi[2] = n.b[1];
+        } else {
+            c.box.lo[0] = n.b[2]; c.box.lo[1] = n.b[3]; c.box.lo[2] = n.c[0];
+            c.box.hi[0] = n.c[1]; c.box.hi[1] = n.c[2]; c.box.hi[2] = n.c[3];
+        }
+        c.link = n.d[k];
+        return c;
+    }
+    // Emits the four-wide node for binary node `id`; returns its index.
+    int32_t collapse(int32_t id, uint32_t depth) {
+        maxDepth = std::max(maxDepth, depth);
+        Child ch[4];
+        int n = 2;
+        ch[0] = childOf(in[id], 0);
+        ch[1] = childOf(in[id], 1);
+        while (n < 4) {
+            int best = -1;
+            float bestArea = -1.0f;
+            for (int k = 0; k < n; ++k)
+                if (ch[k].link >= 0 && ch[k].box.area() > bestArea) {
+                    bestArea = ch[k].box.area();
+                    best = k;
+                }
+            if (best < 0) break;
+            const BvhNode &bn = in[ch[best].link];
+            for (int k = n; k > best + 1; --k) ch[k] = ch[k - 1];
+            ch[best] = childOf(bn, 0);
+            ch[best + 1] = childOf(bn, 1);
+            n++;
+        }
+        int32_t self = (int32_t)out.size();
+        out.emplace_back();
+        int32_t links[4];
+        for (int k = 0; k < 4; ++k) links[k] = makeLeaf(0, 1);
+        for (int k = 0; k < n; ++k) links[k] = ch[k].link >= 0 ? collapse(ch[k].link, depth + 1) : ch[k].link;
+        Bvh4Node &o = out[self];
+        std::memset(&o, 0, sizeof(o));
+        for (int k = 0; k < 4; ++k) {
+            Box bx;
+            if (k < n) {
+                bx = ch[k].box;
+            } else {
+                for (int a = 0; a < 3; ++a) bx.lo[a] = bx.hi[a] = 1e30f; // unused slot
+            }
+            o.f[0][k] = bx.lo[0];
+            o.f[1][k] = bx.hi[0];
+            o.f[2][k] = bx.lo[1];
+            o.f[3][k] = bx.hi[1];
+            o.f[4][k] = bx.lo[2];
+            o.f[5][k] = bx.hi[2];
+            o.link[k] = links[k];
+        }
+        return self;
+    }
+};
+
 // Pad a coordinate bound outwards by ~2^-20 relative so that FP32 hits
 // accepted by the watertight test just outside an edge are never culled.
 inline float padLo(float x) { return x - (std::fabs(x) + 1.0f) * 9.5367431640625e-07f; }
@@ -198,9 +272,13 @@ void buildBvh(const float *tv, uint32_t numTris, BvhBuildResult &out) {
         Builder::setNode(b.nodes.back(), all, empty, root, makeLeaf(0, 1));
         b.maxDepth = 0;
     }
-    out.nodes = std::move(b.nodes);
+    out.binaryDepth = b.maxDepth + 1;
+    Collapser c{b.nodes, {}, 0};
+    c.collapse(0, 1);
+    if (3u * c.maxDepth > (uint32_t)kStackCapacity) throw std::runtime_error("BVH too deep for the traversal stack");
+    out.nodes = std::move(c.out);
     out.primOrder = std::move(b.order);
-    out.maxDepth = b.maxDepth + 1;
+    out.maxDepth = c.maxDepth;
     auto t1 = std::chrono::steady_clock::now();
     out.buildMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
 }

@@ -3,10 +3,11 @@
 // shadow any-hit with TerminateOnFirstHit: raytrace.rgen:622-638) and the
 // sphere intersection shader (raytrace.sphere.rint:11-29).
 //
-// Stack-based traversal (Aila & Laine 2009) over two-box nodes; the
+// Stack-based traversal (Aila & Laine 2009) over four-wide nodes; the
 // stack's first LDS_DEPTH entries live in LDS (one column per thread, so
 // consecutive lanes hit consecutive banks), deeper entries in a private
-// array.  The builder bounds the tree depth by kMaxBvhDepth.
+// array.  The builder bounds the stack use (3 entries per level) by
+// kStackCapacity.
 #pragma once
 
 #include "dev_math.h"
@@ -123,7 +124,7 @@ RTX_DI bool isectSphere(const rtx_sphere &s, V3 o, V3 d, float tmin, float tmax,
 template <int BLOCK, int LDS_DEPTH> struct HybridStack {
     int *lds;
     int sp;
-    int priv[kMaxBvhDepth - LDS_DEPTH];
+    int priv[kStackCapacity - LDS_DEPTH];
     RTX_DI void push(int x) {
         if (sp < LDS_DEPTH) lds[sp * BLOCK] = x;
         else priv[sp - LDS_DEPTH] = x;
@@ -137,7 +138,7 @@ template <int BLOCK, int LDS_DEPTH> struct HybridStack {
 
 struct PrivateStack {
     int sp;
-    int priv[kMaxBvhDepth];
+    int priv[kStackCapacity];
     RTX_DI void push(int x) { priv[sp++] = x; }
     RTX_DI int pop() { return priv[--sp]; }
 };
@@ -145,6 +146,48 @@ struct PrivateStack {
 struct VisitCount {
     uint32_t nodes = 0, prims = 0;
 };
+
+constexpr int32_t kLinkDone = (int32_t)0x80000000; // no node: a leaf never encodes to this
+
+template <class Stack> RTX_DI int popOrDone(Stack &st) { return st.sp == 0 ? kLinkDone : st.pop(); }
+
+// One four-wide node visit: of the children whose boxes the ray enters within
+// [tmin, tmax], the nearest is returned and the others are pushed, farthest
+// first; with none entered, the next stack entry (kLinkDone when empty).
+template <class Stack>
+RTX_DI int nodeStep4(const DevScene &S, const TravRay &r, float tmin, float tmax, int link, Stack &st) {
+    const float4 *np = S.nodes + 8 * (size_t)link;
+    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    const int4 ln = *reinterpret_cast<const int4 *>(np + 6);
+    const float inf = __int_as_float(0x7f800000);
+    float t[4], tn;
+    int c[4] = {ln.x, ln.y, ln.z, ln.w};
+    t[0] = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, tmin, tmax, tn) ? tn : inf;
+    t[1] = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, tmin, tmax, tn) ? tn : inf;
+    t[2] = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, tmin, tmax, tn) ? tn : inf;
+    t[3] = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, tmin, tmax, tn) ? tn : inf;
+    // sorting network on (t, link), ascending t
+    auto cs = [&](int i, int j) {
+        if (t[j] < t[i]) {
+            float tt = t[i];
+            t[i] = t[j];
+            t[j] = tt;
+            int cc = c[i];
+            c[i] = c[j];
+            c[j] = cc;
+        }
+    };
+    cs(0, 1);
+    cs(2, 3);
+    cs(0, 2);
+    cs(1, 3);
+    cs(1, 2);
+    if (t[3] < inf) st.push(c[3]);
+    if (t[2] < inf) st.push(c[2]);
+    if (t[1] < inf) st.push(c[1]);
+    if (t[0] < inf) return c[0];
+    return popOrDone(st);
+}
 
 // Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax].
 // Returns the primitive id (triangles 0..numTris-1, then spheres) or -1.
@@ -170,29 +213,9 @@ RTX_DI int traverse(const DevScene &S, const TravRay &r, float tmin, float &tmax
     int link = 0;
     for (;;) {
         if (link >= 0) {
-            const float4 *np = S.nodes + 4 * (size_t)link;
-            float4 a = np[0], b = np[1], c = np[2];
-            int4 dd = *reinterpret_cast<const int4 *>(np + 3);
             if (COUNT) vc.nodes++;
-            float t0, t1;
-            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, r, tmin, tmax, t0);
-            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, r, tmin, tmax, t1);
-            if (h0 && h1) {
-                int nearL = dd.x, farL = dd.y;
-                if (t1 < t0) {
-                    nearL = dd.y;
-                    farL = dd.x;
-                }
-                st.push(farL);
-                link = nearL;
-            } else if (h0) {
-                link = dd.x;
-            } else if (h1) {
-                link = dd.y;
-            } else {
-                if (st.sp == 0) break;
-                link = st.pop();
-            }
+            link = nodeStep4(S, r, tmin, tmax, link, st);
+            if (link == kLinkDone) break;
         } else {
             uint32_t leaf = ~(uint32_t)link;
             uint32_t start = leaf >> 4, cnt = (leaf & 15u) + 1u;
@@ -230,8 +253,6 @@ RTX_DI int traverse(const DevScene &S, const TravRay &r, float tmin, float &tmax
 // postponed leaves keeps lanes in step: a lane that reaches a leaf parks
 // it and keeps descending until every lane of the wave has a leaf, then
 // the wave intersects leaves together.
-constexpr int32_t kLinkDone = (int32_t)0x80000000; // no node: a leaf never encodes to this
-
 struct LaneRay {
     TravRay r;
     float tmin, tmax, bu, bv;
@@ -241,31 +262,12 @@ struct LaneRay {
     uint32_t randomUInt; // the frame's, for the any-hit alpha test
 };
 
-template <class Stack> RTX_DI int popOrDone(Stack &st) { return st.sp == 0 ? kLinkDone : st.pop(); }
-
-// One inner-node visit: returns the next link (pushing the far child when
-// both are hit), or the next stack entry / kLinkDone when neither is hit.
+// One inner-node visit: returns the next link (pushing the farther entered
+// children), or the next stack entry / kLinkDone when none is entered.
 template <bool COUNT, class Stack>
 RTX_DI int innerStep(const DevScene &S, LaneRay &L, Stack &st, VisitCount &vc) {
-    const float4 *np = S.nodes + 4 * (size_t)L.link;
-    float4 a = np[0], b = np[1], c = np[2];
-    int4 dd = *reinterpret_cast<const int4 *>(np + 3);
     if (COUNT) vc.nodes++;
-    float t0, t1;
-    bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, L.r, L.tmin, L.tmax, t0);
-    bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, L.r, L.tmin, L.tmax, t1);
-    if (h0 && h1) {
-        int nearL = dd.x, farL = dd.y;
-        if (t1 < t0) {
-            nearL = dd.y;
-            farL = dd.x;
-        }
-        st.push(farL);
-        return nearL;
-    }
-    if (h0) return dd.x;
-    if (h1) return dd.y;
-    return popOrDone(st);
+    return nodeStep4(S, L.r, L.tmin, L.tmax, L.link, st);
 }
 
 // Intersects the primitives of leaf link `leaf`; returns true when an any-hit

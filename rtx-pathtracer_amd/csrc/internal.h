@@ -15,7 +15,7 @@ namespace rtx {
 
 // ----------------------------------------------------------- device scene
 struct DevScene {
-    const float4 *nodes;      // 4 x float4 per BvhNode
+    const float4 *nodes;      // 8 x float4 per Bvh4Node
     const float4 *tris;       // 3 x float4 per triangle in leaf order: v0 (w = prim id bits), v1, v2
     const int2 *primRef;      // per global triangle id: (instance, local triangle)
     const rtx_vertex *vertices;
