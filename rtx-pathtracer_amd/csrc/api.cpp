@@ -563,7 +563,7 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     DevScene &S = sc->view;
     hipError_t e = hipSuccess;
     float4 *nodes = nullptr, *trisD = nullptr;
-    int2 *prD = nullptr;
+    int4 *prD = nullptr;
     rtx_vertex *vD = nullptr;
     uint32_t *iD = nullptr, *vbD = nullptr, *ibD = nullptr;
     rtx_instance *instD = nullptr;
@@ -574,7 +574,15 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     rtx_sphere *spD = nullptr;
     if (e == hipSuccess) e = upload(&nodes, reinterpret_cast<const float4 *>(bvh.nodes.data()), bvh.nodes.size() * 8);
     if (e == hipSuccess) e = upload(&trisD, tris.data(), tris.size());
-    if (e == hipSuccess) e = upload(&prD, primRef.data(), primRef.size());
+    // per world triangle: its instance and the global indices of its vertices
+    std::vector<int4> triRef((size_t)numTris);
+    for (size_t p = 0; p < (size_t)numTris; ++p) {
+        const int inst = primRef[p].x;
+        const int model = d->instances[inst].modelIndex;
+        const uint32_t *ix = d->models[model].indices + 3 * (size_t)primRef[p].y;
+        triRef[p] = make_int4(inst, (int)(vbase[model] + ix[0]), (int)(vbase[model] + ix[1]), (int)(vbase[model] + ix[2]));
+    }
+    if (e == hipSuccess) e = upload(&prD, triRef.data(), triRef.size());
     if (e == hipSuccess) e = upload(&vD, verts.data(), verts.size());
     if (e == hipSuccess) e = upload(&iD, idx.data(), idx.size());
     if (e == hipSuccess) e = upload(&vbD, vbase.data(), vbase.size());
@@ -615,7 +623,7 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     }
     S.nodes = nodes;
     S.tris = trisD;
-    S.primRef = prD;
+    S.triRef = prD;
     S.vertices = vD;
     S.indices = iD;
     S.vertexBase = vbD;
@@ -644,7 +652,7 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
     sc->info.nodes = (uint32_t)bvh.nodes.size();
     sc->info.maxDepth = bvh.maxDepth;
     sc->info.deviceBytes = bvh.nodes.size() * sizeof(Bvh4Node) + tris.size() * sizeof(float4) +
-                           primRef.size() * sizeof(int2) + verts.size() * sizeof(rtx_vertex) +
+                           primRef.size() * sizeof(int4) + verts.size() * sizeof(rtx_vertex) +
                            idx.size() * 4 + (size_t)d->materialCount * sizeof(rtx_material);
     sc->info.buildMs = bvh.buildMs;
     for (int a = 0; a < 3; ++a) {
@@ -978,7 +986,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
                 timed(kClsProbe, [&] { launchClosestP(s, S, ctx->qp, ctx->cnt, qsel, 2, countVisits, traceBlocks); });
                 timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, shadeBlocks); });
             } else {
-                timed(kClsProbe, [&] { launchProbe(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
+                timed(kClsProbe, [&] { launchProbe(s, S, ctx->qp, ctx->cnt, countVisits, traceBlocks); });
+                timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, shadeBlocks); });
             }
             timed(kClsShade, [&] {
                 launchFinish(s, fcd, b, ctx->pb, ctx->done, ctx->cnt, ctx->q[qsel ^ 1], qsel, shadeBlocks);

@@ -324,12 +324,9 @@ RTX_DI HitInfo hitInfo(const DevScene &S, int prim, float t, float bu, float bv,
         h.isSphere = true;
         h.instanceIndex = si;
     } else {
-        int2 ref = S.primRef[prim];
+        const int4 ref = S.triRef[prim]; // one load instead of the instance -> index -> vertex chain
         const rtx_instance &in = S.instances[ref.x];
-        int model = in.modelIndex;
-        const uint32_t *idx = S.indices + S.indexBase[model] + 3 * ref.y;
-        const rtx_vertex *vb = S.vertices + S.vertexBase[model];
-        const rtx_vertex &v0 = vb[idx[0]], &v1 = vb[idx[1]], &v2 = vb[idx[2]];
+        const rtx_vertex &v0 = S.vertices[ref.y], &v1 = S.vertices[ref.z], &v2 = S.vertices[ref.w];
         float b0 = 1.0f - bu - bv;
         normal = v3(v0.normal) * b0 + v3(v1.normal) * bu + v3(v2.normal) * bv;
         normal = normalize(xform(in.normalTransform, normal, 0.0f));

@@ -83,11 +83,8 @@ RTX_DI uint32_t satU32(float x) {
 // t), randomUInt) (GLSL's uint(vec2) takes the first component).
 RTX_DI bool alphaAccept(const DevScene &S, int prim, float ox, float t, float bu, float bv, uint32_t randomUInt) {
 #pragma clang fp contract(off)
-    int2 ref = S.primRef[prim];
-    const rtx_instance &in = S.instances[ref.x];
-    const uint32_t *idx = S.indices + S.indexBase[in.modelIndex] + 3 * ref.y;
-    const rtx_vertex *vb = S.vertices + S.vertexBase[in.modelIndex];
-    const rtx_vertex &v0 = vb[idx[0]], &v1 = vb[idx[1]], &v2 = vb[idx[2]];
+    const int4 ref = S.triRef[prim];
+    const rtx_vertex &v0 = S.vertices[ref.y], &v1 = S.vertices[ref.z], &v2 = S.vertices[ref.w];
     float b0 = 1.0f - bu - bv;
     float u = v0.texCoord[0] * b0 + v1.texCoord[0] * bu + v2.texCoord[0] * bv;
     float v = v0.texCoord[1] * b0 + v1.texCoord[1] * bu + v2.texCoord[1] * bv;

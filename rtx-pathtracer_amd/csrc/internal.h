@@ -17,7 +17,7 @@ namespace rtx {
 struct DevScene {
     const float4 *nodes;      // 8 x float4 per Bvh4Node
     const float4 *tris;       // 3 x float4 per triangle in leaf order: v0 (w = prim id bits), v1, v2
-    const int2 *primRef;      // per global triangle id: (instance, local triangle)
+    const int4 *triRef;       // per global triangle id: (instance, global indices of its three vertices)
     const rtx_vertex *vertices;
     const uint32_t *indices;
     const uint32_t *vertexBase; // per model
@@ -291,8 +291,7 @@ void launchIcIndex(hipStream_t s, IcBuffers ic, float c0);
 void launchIcInit(hipStream_t s, IcBuffers ic);
 void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
                   int blocks);
-void launchProbe(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb, Counters *cnt,
-                 bool countVisits, int blocks);
+void launchProbe(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, bool countVisits, int blocks);
 // Ends the segments of the done queue; with FrameConst::segments a path
 // with camera samples left or splits queued goes on in qnext instead.
 void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, const int32_t *done,

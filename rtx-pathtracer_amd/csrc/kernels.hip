@@ -242,27 +242,23 @@ RTX_DI V3 probeContribution(const DevScene &S, bool power, V3 o, V3 d, int prim,
 }
 
 // ---------------------------------------------------------------- probe
-// The MIS probe of nextEventEstimation (raytrace.rgen:665-729): closest hit
-// plus the BSDF-side MIS weight against the light it hits.
+// The MIS probe rays of nextEventEstimation (raytrace.rgen:665-729), grid-
+// stride traversal: closest hits into q.hit; kProbeResolve adds the
+// BSDF-side MIS weight against the light hit.
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, const FrameConst *__restrict__ fcp, Queue q,
-                                                      PathBuffers pb, Counters *cnt) {
+__global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, Queue q, Counters *cnt) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
-    const FrameConst &fc = *fcp;
     const uint32_t n = cnt->probe;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->probeRays, (unsigned long long)n);
     HybridStack<kTraceBlock, kLdsStack> st;
     st.lds = stackMem + threadIdx.x;
     VisitCount vc;
-    const bool power = fc.pc.usePowerHeuristic != 0;
     for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
-        float4 o4 = q.org[i], d4 = q.dir[i], base = q.aux[i];
-        V3 o = v3(o4), d = v3(d4);
-        TravRay r = makeTravRay(o, d);
+        float4 o4 = q.org[i], d4 = q.dir[i];
+        TravRay r = makeTravRay(v3(o4), v3(d4));
         float tmax = d4.w, bu = 0.0f, bv = 0.0f;
         int prim = traverse<false, COUNT>(S, r, o4.w, tmax, bu, bv, st, vc, raySeed(S, q, i));
-        V3 contrib = probeContribution(S, power, o, d, prim, tmax, bu, bv, base);
-        pb.pendProbe[q.id[i]] = f4(contrib, __int_as_float(prim));
+        q.hit[i] = make_float4(tmax, __int_as_float(prim), bu, bv);
     }
     flushVisits<COUNT>(vc, cnt, 2);
 }
@@ -1038,10 +1034,9 @@ void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Cou
     if (countVisits) hipLaunchKernelGGL(kShadow<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
     else hipLaunchKernelGGL(kShadow<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
 }
-void launchProbe(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb, Counters *cnt,
-                 bool countVisits, int blocks) {
-    if (countVisits) hipLaunchKernelGGL(kProbe<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, fc, q, pb, cnt);
-    else hipLaunchKernelGGL(kProbe<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, fc, q, pb, cnt);
+void launchProbe(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, bool countVisits, int blocks) {
+    if (countVisits) hipLaunchKernelGGL(kProbe<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt);
+    else hipLaunchKernelGGL(kProbe<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt);
 }
 void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, const int32_t *done,
                   Counters *cnt, Queue qnext, int qsel, int blocks) {
