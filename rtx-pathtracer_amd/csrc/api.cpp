@@ -1400,6 +1400,9 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) {
     ic.nb = 1024;
     while (ic.nb < 4u * maxEntries) ic.nb <<= 1;
     ic.itemCap = 27u * maxEntries;
+    ic.bucketBits = 0;
+    while ((1u << ic.bucketBits) < ic.nb) ic.bucketBits++;
+    ic.sortTempBytes = icSortTempBytes(ic.itemCap, ic.bucketBits);
     ic.jobCap = 2u * maxEntries;
     ic.blockCap = kMaxBlockSums;
     hipError_t e = devAlloc(&ic.hdr, 1, L);
@@ -1410,6 +1413,10 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) {
     if (e == hipSuccess) e = devAlloc(&ic.cursor, ic.nb, L);
     if (e == hipSuccess) e = devAlloc(&ic.meta, 4, L);
     if (e == hipSuccess) e = devAlloc(&ic.items, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.slotKeys, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.slotKeysSorted, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.slotItems, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(reinterpret_cast<uint8_t **>(&ic.sortTemp), std::max<size_t>(ic.sortTempBytes, 1), L);
     if (e == hipSuccess) e = devAlloc(&ic.frame, 8, L);
     if (e == hipSuccess) e = devAlloc(&ic.cand, 2 * (size_t)maxEntries, L);
     if (e == hipSuccess) e = devAlloc(&ic.jobs, 2 * (size_t)ic.jobCap, L);

@@ -19,6 +19,8 @@
 //   kIcIndex*      - rebuild of the lookup index (dev_icache.h)
 #include <hip/hip_runtime.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "dev_icache.h"
 #include "dev_math.h"
 #include "dev_shading.h"
@@ -337,32 +339,41 @@ RTX_DI void icCells(const rtx_sphere &sp, float c0, int &level, int32_t lo[3], i
     }
 }
 
+// Pass 1 counts the (record, cell) items per bucket; pass 2 writes record
+// i's items into its 27 fixed slots in cell order (unused slots keyed nb),
+// so that a stable sort by bucket leaves each bucket's items in record order
+// and lookups add the same records in the same order every time.
 template <bool SCATTER> __global__ __launch_bounds__(256) void kIcIndexFill(IcBuffers ic, float c0) {
     const uint32_t count = min(ic.hdr->nextCacheSlot, ic.hdr->maxCaches);
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= count) return;
+    if (i >= ic.maxEntries) return;
+    uint32_t k = 0;
     const rtx_sphere sp = ic.spheres[i];
-    if (!(sp.radius > 0.0f)) return;
-    int level;
-    int32_t lo[3], hi[3];
-    icCells(sp, c0, level, lo, hi);
-    if (!SCATTER) atomicOr(&ic.meta[0], 1u << level);
-    for (int32_t x = lo[0]; x <= hi[0]; ++x)
-        for (int32_t y = lo[1]; y <= hi[1]; ++y)
-            for (int32_t z = lo[2]; z <= hi[2]; ++z) {
-                const uint64_t key = icKey(level, x, y, z);
-                const uint32_t b = icBucket(key, ic.nb);
-                if (!SCATTER) {
-                    atomicAdd(&ic.bucketCount[b], 1u);
-                } else {
-                    uint32_t pos = ic.bucketStart[b] + atomicAdd(&ic.cursor[b], 1u);
-                    IcItem it;
-                    it.entry = i;
-                    it.pad = 0;
-                    it.key = key;
-                    ic.items[pos] = it;
+    if (i < count && sp.radius > 0.0f) {
+        int level;
+        int32_t lo[3], hi[3];
+        icCells(sp, c0, level, lo, hi);
+        if (!SCATTER) atomicOr(&ic.meta[0], 1u << level);
+        for (int32_t x = lo[0]; x <= hi[0]; ++x)
+            for (int32_t y = lo[1]; y <= hi[1]; ++y)
+                for (int32_t z = lo[2]; z <= hi[2]; ++z) {
+                    const uint64_t key = icKey(level, x, y, z);
+                    const uint32_t b = icBucket(key, ic.nb);
+                    if (!SCATTER) {
+                        atomicAdd(&ic.bucketCount[b], 1u);
+                    } else {
+                        IcItem it;
+                        it.entry = i;
+                        it.pad = 0;
+                        it.key = key;
+                        ic.slotKeys[27 * (size_t)i + k] = b;
+                        ic.slotItems[27 * (size_t)i + k] = it;
+                    }
+                    ++k;
                 }
-            }
+    }
+    if (SCATTER)
+        for (; k < 27; ++k) ic.slotKeys[27 * (size_t)i + k] = ic.nb;
 }
 
 __global__ __launch_bounds__(1024) void kIcIndexScan(IcBuffers ic) {
@@ -379,24 +390,6 @@ __global__ __launch_bounds__(1024) void kIcIndexScan(IcBuffers ic) {
     if (threadIdx.x == 0) {
         ic.bucketStart[ic.nb] = carry;
         ic.meta[1] = carry;
-    }
-}
-
-// Buckets filled by atomics come out in any order: sort each by record id
-// so that lookups add the same records in the same order every time.
-__global__ __launch_bounds__(256) void kIcIndexSort(IcBuffers ic) {
-    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= ic.nb) return;
-    const uint32_t s = ic.bucketStart[b], e = ic.bucketStart[b + 1];
-    for (uint32_t i = s + 1; i < e; ++i) {
-        IcItem x = ic.items[i];
-        uint32_t j = i;
-        while (j > s && (ic.items[j - 1].entry > x.entry ||
-                         (ic.items[j - 1].entry == x.entry && ic.items[j - 1].key > x.key))) {
-            ic.items[j] = ic.items[j - 1];
-            --j;
-        }
-        ic.items[j] = x;
     }
 }
 
@@ -432,7 +425,18 @@ void launchIcIndex(hipStream_t s, IcBuffers ic, float c0) {
     hipLaunchKernelGGL(kIcIndexFill<false>, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
     hipLaunchKernelGGL(kIcIndexScan, dim3(1), dim3(1024), 0, s, ic);
     hipLaunchKernelGGL(kIcIndexFill<true>, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
-    hipLaunchKernelGGL(kIcIndexSort, dim3(blocksFor(ic.nb, 256)), dim3(256), 0, s, ic);
+    // stable LSD radix sort by bucket (keys below 2^(bucketBits + 1): nb itself marks unused slots)
+    size_t bytes = ic.sortTempBytes;
+    (void)hipcub::DeviceRadixSort::SortPairs(ic.sortTemp, bytes, ic.slotKeys, ic.slotKeysSorted, ic.slotItems,
+                                             ic.items, (int)ic.itemCap, 0, (int)ic.bucketBits + 1, s);
+}
+
+size_t icSortTempBytes(uint32_t items, uint32_t bucketBits) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                             (const IcItem *)nullptr, (IcItem *)nullptr, (int)items, 0,
+                                             (int)bucketBits + 1, (hipStream_t)0);
+    return bytes;
 }
 
 } // namespace rtx

@@ -193,6 +193,13 @@ struct IcBuffers {
     uint32_t nb, itemCap;
     uint32_t *bucketCount, *bucketStart, *cursor, *meta;
     IcItem *items;
+    // the index build's sort: 27 fixed slots per record (bucket keys and
+    // items, unused slots keyed nb), sorted stably by bucket into `items`
+    uint32_t *slotKeys, *slotKeysSorted;
+    IcItem *slotItems;
+    void *sortTemp;
+    size_t sortTempBytes;
+    uint32_t bucketBits; // log2(nb)
     // frame: [0] update elections, [1] creation candidates, [2] free slots at
     // frame start, [3] update jobs, [4] jobs
     uint32_t *frame;
@@ -288,6 +295,8 @@ void launchIcReduce(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t fi
                     PathBuffers pb);
 void launchIcCommit(hipStream_t s, IcBuffers ic, const FrameConst *fc);
 void launchIcIndex(hipStream_t s, IcBuffers ic, float c0);
+// Device temp bytes the index build's radix sort needs for `items` slots.
+size_t icSortTempBytes(uint32_t items, uint32_t bucketBits);
 void launchIcInit(hipStream_t s, IcBuffers ic);
 void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
                   int blocks);
