@@ -14,11 +14,13 @@ spp only; the prepare frames are inside the timed step.  --method plain
 measures the plain path tracer instead (no cache); its figure is reported
 beside the cache's in every run.
 
-Prints one JSON line.  `roofline` prices the dominant kernel (closest-hit
-traversal of extend rays) by algorithmic bytes per ray = 32 (ray) + 16 (hit)
-+ nodes x 64 B + triangles x 48 B, with node/triangle fetch counts from a
-counting pass of the identical (deterministic) workload and kernel times
-from HIP events on the library's stream.  `cpu_baseline` times the FP64 CPU
+Prints one JSON line.  `roofline` prices the dominant kernel (kTraceP, the
+persistent traversal that traces one bounce's extend rays together with the
+previous bounce's probe and shadow rays) by algorithmic bytes per ray = 32
+(ray) + 16 (hit) + nodes x node record + triangles x triangle record (the
+library's figures: 112 B read of a 128 B four-wide node, 48 B), with node and
+triangle fetch counts from a counting pass of the identical (deterministic)
+workload and kernel times from HIP events on the library's stream.  `cpu_baseline` times the FP64 CPU
 restatement of the plain method on a pixel subsample of one sample plane.
 """
 import argparse
@@ -87,8 +89,9 @@ def main():
     r = rtxamd.Renderer(0)
     r.set_tuning(traversalKernel=a.traversal, traceBlocksPerCU=a.trace_blocks, shadeBlocksPerCU=a.shade_blocks)
     tun = r.tuning()
-    ext_kernel = "kClosestP<false, 0> (persistent closest-hit BVH traversal of extend rays)" \
-        if tun.traversalKernel == 1 else "kExtend<false> (closest-hit BVH traversal)"
+    persistent = tun.traversalKernel == 1
+    trace_kernel = "kTraceP<false> (persistent BVH traversal per bounce: extend, probe and shadow rays)" \
+        if persistent else "kExtend<false> (grid-stride closest-hit BVH traversal of extend rays)"
     t0 = time.time()
     ds = r.upload(hs)
     info = ds.info()
@@ -147,14 +150,19 @@ def main():
     ext_bytes = ext_rays * (RAY_BYTES + HIT_BYTES) + nodes[0] * NODE_BYTES + prims[0] * TRI_BYTES
     all_rays = cnt["extendRays"] + cnt["shadowRays"] + cnt["probeRays"]
     all_bytes = all_rays * (RAY_BYTES + HIT_BYTES) + sum(nodes) * NODE_BYTES + sum(prims) * TRI_BYTES
-    launches_ext = launches[0] // a.steps
-    ext_ms_step = totals["msExtend"] / a.steps
-    avg_launch_ms = ext_ms_step / max(1, launches_ext)
-    achieved = ext_bytes / max(1, launches_ext) / (avg_launch_ms * 1e-3) / 1e9
-    trav_ms = (totals["msExtend"] + totals["msShadow"] + totals["msProbe"]) / a.steps
+    # the dominant kernel: the persistent per-bounce trace covers every ray;
+    # the grid-stride build's extend kernel covers the extend rays
+    launches_tr = launches[0] // a.steps
+    tr_bytes = all_bytes if persistent else ext_bytes
+    tr_rays = all_rays if persistent else ext_rays
+    tr_nodes = sum(nodes) if persistent else nodes[0]
+    tr_prims = sum(prims) if persistent else prims[0]
+    avg_launch_ms = totals["msExtend"] / a.steps / max(1, launches_tr)
+    achieved = tr_bytes / max(1, launches_tr) / (avg_launch_ms * 1e-3) / 1e9
+    trav_ms = (totals["msExtend"] + totals["msShadow"]) / a.steps
     traffic = None
-    tfile = ROOT / "profiles" / f"extend_traffic_room262k_1080p_{a.method}.json"
-    if tfile.exists():
+    tfile = ROOT / "profiles" / f"trace_traffic_room262k_1080p_{a.method}.json"
+    if persistent and tfile.exists():
         try:
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except Exception:
@@ -162,11 +170,12 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "kernel": ext_kernel,
-        "algorithmic_bytes_per_launch": int(ext_bytes / max(1, launches_ext)),
-        "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_step": launches_ext,
-        "nodes_per_ray": round(nodes[0] / max(1, ext_rays), 2),
-        "tris_per_ray": round(prims[0] / max(1, ext_rays), 2),
+        "kernel": trace_kernel,
+        "algorithmic_bytes_per_launch": int(tr_bytes / max(1, launches_tr)),
+        "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_step": launches_tr,
+        "rays_per_launch": int(tr_rays / max(1, launches_tr)),
+        "nodes_per_ray": round(tr_nodes / max(1, tr_rays), 2),
+        "tris_per_ray": round(tr_prims / max(1, tr_rays), 2),
         "all_traversal_frac": round(all_bytes / (trav_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
     mrays = all_rays / (ms_per_step * 1e3)
@@ -247,9 +256,8 @@ def main():
         "mrays_per_s": round(mrays, 1),
         "rays_per_step": cnt,
         "ms_device_per_step": {"total": round(totals["msTotal"] / a.steps, 1),
-                               "extend": round(totals["msExtend"] / a.steps, 1),
-                               "shadow": round(totals["msShadow"] / a.steps, 1),
-                               "probe": round(totals["msProbe"] / a.steps, 1),
+                               "trace": round(trav_ms, 1),
+                               "probe_resolve": round(totals["msProbe"] / a.steps, 1),
                                "shade_and_other": round(totals["msShade"] / a.steps, 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -343,9 +343,13 @@ typedef struct rtx_frame_stats {
     uint64_t nodeVisits;    /* BVH node fetches (RTX_FRAME_COUNT_VISITS)  */
     uint64_t primVisits;    /* primitive fetches (RTX_FRAME_COUNT_VISITS) */
     double msTotal;         /* device time of the whole render (HIP events) */
-    double msExtend;        /* time in closest-hit traversal kernels       */
-    double msShadow;        /* time in any-hit traversal kernels           */
-    double msProbe;         /* time in probe traversal kernels             */
+    double msExtend;        /* time in traversal: the persistent build's
+                               one launch per bounce (extend rays with the
+                               previous bounce's probe and shadow rays), or
+                               the grid-stride build's extend kernel      */
+    double msShadow;        /* grid-stride build: shadow and probe ray
+                               traversal (0 in the persistent build)      */
+    double msProbe;         /* time in the probe's BSDF-side resolve        */
     double msShade;         /* time in generate/shade/finish/accumulate    */
     uint32_t batches;
     uint32_t bounces;       /* wavefront iterations of the last batch     */
@@ -354,7 +358,8 @@ typedef struct rtx_frame_stats {
     /* per traversal kind: [0] closest-hit extend, [1] shadow, [2] probe */
     uint64_t nodeVisitsByKind[3];
     uint64_t primVisitsByKind[3];
-    uint32_t launchesByKind[4]; /* kernel launches: extend, shadow, probe, other */
+    uint32_t launchesByKind[4]; /* kernel launches, by the four times above:
+                                   extend, shadow, probe, other           */
     uint32_t _pad;
 } rtx_frame_stats;
 

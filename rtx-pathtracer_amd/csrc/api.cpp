@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -939,13 +940,17 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         launch();
         if (ev) (void)hipEventRecord(ev->b, s);
     };
-    // The bounce iterations of one batch whose extend queue 0 is filled: a
-    // path ends within `iters` iterations of its start.  With segments a path
-    // slot runs a pixel's whole frame (several camera samples, queued splits),
-    // so the loop goes on in rounds of `iters` until the extend queue is empty.
-    // Every kPollEvery iterations the extend queue's length is read back and
-    // the loop ends once it is empty (later iterations would only launch
-    // kernels over empty queues).
+    // The bounce iterations of one batch whose extend queue 0 is filled.
+    // Bounce `it` traces its extend queue together with the previous
+    // bounce's shadow and probe rays (one persistent launch), resolves the
+    // probes, finishes the paths the previous bounce ended, then shades its
+    // own hits, which fills the next extend queue and the secondary queues.
+    // A path ends within `iters` iterations of its start, and one more
+    // bounce drains its last secondary rays.  With segments a path slot runs
+    // a pixel's whole frame (several camera samples, queued splits), so the
+    // loop goes on until every queue is empty.  Every kPollEvery iterations
+    // the queue lengths are read back and the loop ends once they are all
+    // empty (later iterations would only launch kernels over empty queues).
     constexpr int kPollEvery = 4;
     int itersRun = 0;
     // where the any-hit alpha test finds each ray's frame seed: extend entries
@@ -958,39 +963,36 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     };
     auto bounces = [&](const FrameConst *fcd, BatchDesc b, int iters, bool multi) -> int {
         int qsel = 0;
-        const int cap = multi ? (int)(samplesPerFrame + kMaxSplits + 1) * iters : iters;
+        const int cap = multi ? (int)(samplesPerFrame + kMaxSplits + 1) * (iters + 1) : iters + 1;
         for (int it = 0;; ++it) {
             if (it == cap && !multi) break;
             if (it > 0 && it % kPollEvery == 0) {
-                uint32_t left = 0;
-                CHECK_HIP(ctx, hipMemcpyAsync(&left, &ctx->cnt->extend[qsel], sizeof(left), hipMemcpyDeviceToHost, s));
+                uint32_t c[6] = {0, 0, 0, 0, 0, 0}; // extend[2], shadow, probe, done[2]
+                static_assert(offsetof(Counters, done) == 4 * sizeof(uint32_t), "counter layout");
+                CHECK_HIP(ctx, hipMemcpyAsync(c, ctx->cnt, sizeof(c), hipMemcpyDeviceToHost, s));
                 CHECK_HIP(ctx, hipStreamSynchronize(s));
-                if (left == 0) break;
+                if (c[qsel] == 0 && c[2] == 0 && c[3] == 0 && c[4 + (qsel ^ 1)] == 0) break;
                 if (it >= cap) return fail(ctx, RTX_E_DEVICE, "paths did not finish within the iteration bound");
             }
             itersRun = std::max(itersRun, it + 1);
-            timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
-            timed(kClsExtend, [&] {
-                if (persistent) launchClosestP(s, S, ctx->q[qsel], ctx->cnt, qsel, 0, countVisits, traceBlocks);
-                else launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, countVisits, traceBlocks);
+            if (it == 0) timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
+            if (persistent) {
+                timed(kClsExtend, [&] {
+                    launchTraceP(s, S, ctx->q[qsel], qsel, ctx->qs, ctx->qp, ctx->pb, ctx->cnt, countVisits,
+                                 traceBlocks);
+                });
+            } else {
+                timed(kClsExtend, [&] { launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, countVisits, traceBlocks); });
+                timed(kClsShadow, [&] { launchShadow(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
+                timed(kClsShadow, [&] { launchProbe(s, S, ctx->qp, ctx->cnt, countVisits, traceBlocks); });
+            }
+            timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, qsel, shadeBlocks); });
+            timed(kClsShade, [&] {
+                launchFinish(s, fcd, b, ctx->pb, ctx->done, ctx->cnt, ctx->q[qsel ^ 1], qsel, shadeBlocks);
             });
             timed(kClsShade, [&] {
                 launchShade(s, S, icv, ctx->guide, fcd, b, ctx->pb, ctx->q[qsel], ctx->q[qsel ^ 1], ctx->qs, ctx->qp,
                             ctx->done, ctx->cnt, qsel, shadeBlocks);
-            });
-            timed(kClsShadow, [&] {
-                if (persistent) launchShadowP(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks);
-                else launchShadow(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks);
-            });
-            if (persistent) {
-                timed(kClsProbe, [&] { launchClosestP(s, S, ctx->qp, ctx->cnt, qsel, 2, countVisits, traceBlocks); });
-                timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, shadeBlocks); });
-            } else {
-                timed(kClsProbe, [&] { launchProbe(s, S, ctx->qp, ctx->cnt, countVisits, traceBlocks); });
-                timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, shadeBlocks); });
-            }
-            timed(kClsShade, [&] {
-                launchFinish(s, fcd, b, ctx->pb, ctx->done, ctx->cnt, ctx->q[qsel ^ 1], qsel, shadeBlocks);
             });
             qsel ^= 1;
         }

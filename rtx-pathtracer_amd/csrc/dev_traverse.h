@@ -260,7 +260,13 @@ struct LaneRay {
     int link, leaf;
     uint32_t idx;
     uint32_t randomUInt; // the frame's, for the any-hit alpha test
+    bool any;            // kHitLane: this lane's ray wants any hit
 };
+
+// How a persistent launch ends its rays: at the closest hit, at the first
+// hit, or per lane (LaneRay::any), so that one launch traces occlusion and
+// closest-hit rays side by side.
+constexpr int kHitClosest = 0, kHitAny = 1, kHitLane = 2;
 
 // One inner-node visit: returns the next link (pushing the farther entered
 // children), or the next stack entry / kLinkDone when none is entered.
@@ -272,8 +278,9 @@ RTX_DI int innerStep(const DevScene &S, LaneRay &L, Stack &st, VisitCount &vc) {
 
 // Intersects the primitives of leaf link `leaf`; returns true when an any-hit
 // ray is done.
-template <bool ANY, bool COUNT, bool ALPHA>
+template <int HIT, bool COUNT, bool ALPHA>
 RTX_DI bool leafStep(const DevScene &S, LaneRay &L, int leaf, VisitCount &vc) {
+    const bool ANY = HIT == kHitAny || (HIT == kHitLane && L.any);
     uint32_t data = ~(uint32_t)leaf;
     uint32_t start = data >> 4, cnt = (data & 15u) + 1u;
     const float4 *tp = S.tris + 3 * (size_t)start;
@@ -303,7 +310,8 @@ RTX_DI bool leafStep(const DevScene &S, LaneRay &L, int leaf, VisitCount &vc) {
 }
 
 // Starts a ray: spheres are tested up front (all workloads have few).
-template <bool ANY> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
+template <int HIT> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
+    const bool ANY = HIT == kHitAny || (HIT == kHitLane && L.any);
     L.best = -1;
     L.bu = L.bv = 0.0f;
     L.link = 0;
@@ -324,7 +332,8 @@ template <bool ANY> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
     }
 }
 
-// Drives the persistent loop.  load(idx, L) fills L.r/L.tmin/L.tmax;
+// Drives the persistent loop.  load(idx, L) fills L.r/L.tmin/L.tmax (and
+// L.any under kHitLane);
 // store(L) consumes a finished ray.  `fetch` must be zero at launch.  Each
 // wave first takes its own 64 rays of the grid's static span (thread i of
 // the grid: ray i) without an atomic, so a launch over a short or empty queue
@@ -334,12 +343,13 @@ template <bool ANY> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
 // fetching, so `fetch` ends below n + 64 * waves.
 constexpr int kRefillLanes = 16;
 
-template <bool ANY, bool COUNT, bool ALPHA, class Stack, class Load, class Store>
+template <int HIT, bool COUNT, bool ALPHA, class Stack, class Load, class Store>
 RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stack &st, VisitCount &vc, Load load,
                             Store store) {
     LaneRay L;
     L.link = L.leaf = kLinkDone;
     L.idx = 0;
+    L.any = HIT == kHitAny;
     bool active = false;
     bool drained = false; // wave-uniform
     bool first = true;    // wave-uniform
@@ -371,7 +381,7 @@ RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stac
                     L.idx = idx;
                     load(idx, L);
                     st.sp = 0;
-                    beginRay<ANY>(S, L);
+                    beginRay<HIT>(S, L);
                     active = true;
                 }
             }
@@ -389,7 +399,7 @@ RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stac
             }
             // leaves
             while (L.leaf != kLinkDone) {
-                if (leafStep<ANY, COUNT, ALPHA>(S, L, L.leaf, vc)) {
+                if (leafStep<HIT, COUNT, ALPHA>(S, L, L.leaf, vc)) {
                     L.link = kLinkDone;
                     L.leaf = kLinkDone;
                     break;

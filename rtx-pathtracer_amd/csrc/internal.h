@@ -66,9 +66,9 @@ struct Counters {
     uint32_t extend[2]; // ping-pong extend queue sizes
     uint32_t shadow;
     uint32_t probe;
-    uint32_t done;
-    uint32_t pad[3];
-    uint32_t fetch[3];  // persistent-kernel work counters: extend, shadow, probe
+    uint32_t done[2];   // done lists, by the bounce's extend queue
+    uint32_t pad[2];
+    uint32_t fetch[3];  // persistent-kernel work counters ([1]: the per-bounce trace)
     uint32_t pad2;
     unsigned long long extendRays, shadowRays, probeRays;
     unsigned long long nodeVisits[3], primVisits[3]; // extend, shadow, probe
@@ -272,16 +272,15 @@ void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffer
                     Counters *cnt, int qsel, uint32_t *electCounter);
 void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits,
                   int blocks);
-// persistent lane-refill traversal: closest hit of an extend (kind 0) or probe (kind 2) queue
-void launchClosestP(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, int kind, bool countVisits,
-                    int blocks);
-void launchShadowP(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
-                   int blocks);
+// persistent lane-refill traversal of one bounce: extend queue qe (count
+// extend[esel]), then the probe and shadow queues; fetch[1] must be zero
+void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
+                  Counters *cnt, bool countVisits, int blocks);
 // persistent variant of the two queries; *fetch must be zero
 void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_t n, void *out, bool closest,
                   uint32_t *fetch, int blocks);
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
-                        Counters *cnt, int blocks);
+                        Counters *cnt, int qsel, int blocks);
 void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const DevGuiding &g, const FrameConst *fc,
                  BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt,
                  int qsel, int blocks);
@@ -301,8 +300,9 @@ void launchIcInit(hipStream_t s, IcBuffers ic);
 void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
                   int blocks);
 void launchProbe(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, bool countVisits, int blocks);
-// Ends the segments of the done queue; with FrameConst::segments a path
-// with camera samples left or splits queued goes on in qnext instead.
+// Ends the segments of the previous bounce's done list (done[qsel ^ 1]);
+// with FrameConst::segments a path with camera samples left or splits queued
+// goes on in qnext (count extend[qsel ^ 1]) instead.
 void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, const int32_t *done,
                   Counters *cnt, Queue qnext, int qsel, int blocks);
 // accum may be NULL (irradiance-cache prepare frames): then only the kept paths are written

@@ -264,79 +264,115 @@ __global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, Queue q, Count
 }
 
 // --------------------------------------------------- persistent traversal
-// KIND 0: extend queue (count extend[qsel]); KIND 2: probe queue (count probe).
-// KIND is a template parameter so that profiles tell the two apart.
-// The persistent kernels come in two builds: without the any-hit alpha test
-// (scenes with no textured triangles, the hot path) and with it (kClosestPA,
-// kShadowPA), so that the test's code stays out of the common traversal loop.
-template <bool COUNT, int KIND, bool ALPHA>
-RTX_DI void closestBody(const DevScene &S, const Queue &q, Counters *cnt, int qsel, int *stackMem) {
-    constexpr int kind = KIND;
-    const uint32_t n = kind == 0 ? cnt->extend[qsel] : cnt->probe;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(kind == 0 ? &cnt->extendRays : &cnt->probeRays, (unsigned long long)n);
-    HybridStack<kTraceBlock, kLdsStack> st;
-    st.lds = stackMem + threadIdx.x;
-    VisitCount vc;
-    persistentTrace<false, COUNT, ALPHA>(
-        S, n, &cnt->fetch[kind], st, vc,
-        [&](uint32_t i, LaneRay &L) {
-            float4 o = q.org[i], d = q.dir[i];
-            L.r = makeTravRay(v3(o), v3(d));
-            L.tmin = o.w;
-            L.tmax = d.w;
-            if (ALPHA) L.randomUInt = raySeed(S, q, i);
-        },
-        [&](const LaneRay &L) { q.hit[L.idx] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv); });
-    flushVisits<COUNT>(vc, cnt, kind);
-}
-template <bool COUNT, int KIND>
-__global__ __launch_bounds__(kTraceBlock) void kClosestP(DevScene S, Queue q, Counters *cnt, int qsel) {
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
-    closestBody<COUNT, KIND, false>(S, q, cnt, qsel, stackMem);
-}
-template <bool COUNT, int KIND>
-__global__ __launch_bounds__(kTraceBlock) void kClosestPA(DevScene S, Queue q, Counters *cnt, int qsel) {
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
-    closestBody<COUNT, KIND, true>(S, q, cnt, qsel, stackMem);
-}
-
+// One launch per bounce traces three queues: the bounce's extend rays
+// (closest hit into qe.hit), then the previous bounce's MIS probe rays
+// (closest hit into qp.hit), then its shadow rays (first hit; visibility into
+// pendShadow).  Ray i of the launch is extend ray i for i < ne, probe ray
+// i - ne below ne + np, shadow ray i - ne - np after that.  The closest-hit
+// rays come first so that the launch's tail is made of short occlusion rays,
+// and a bounce pays one launch and one tail.  Work counter: fetch[1].
+// COUNT builds count node and triangle visits per queue (visit kinds 0
+// extend, 1 shadow, 2 probe).  The kernels come in two builds: without the
+// any-hit alpha test (scenes with no textured triangles, the hot path) and
+// with it (kTracePA), so that the test's code stays out of the common loop.
 template <bool COUNT, bool ALPHA>
-RTX_DI void shadowBody(const DevScene &S, const Queue &q, const PathBuffers &pb, Counters *cnt, int *stackMem) {
-    const uint32_t n = cnt->shadow;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->shadowRays, (unsigned long long)n);
+RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue &qs, const Queue &qp,
+                      const PathBuffers &pb, Counters *cnt, int *stackMem) {
+    const uint32_t ne = cnt->extend[esel], np = cnt->probe, ns = cnt->shadow;
+    const uint32_t nep = ne + np;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&cnt->extendRays, (unsigned long long)ne);
+        atomicAdd(&cnt->probeRays, (unsigned long long)np);
+        atomicAdd(&cnt->shadowRays, (unsigned long long)ns);
+    }
     HybridStack<kTraceBlock, kLdsStack> st;
     st.lds = stackMem + threadIdx.x;
-    VisitCount vc;
-    persistentTrace<true, COUNT, ALPHA>(
-        S, n, &cnt->fetch[1], st, vc,
+    VisitCount vc, vk[3];
+    // the queues' arrays as plain pointers, read in per-queue branches (a
+    // select among them becomes a table in scratch and flat loads)
+    const float4 *eOrg = qe.org, *eDir = qe.dir, *pOrg = qp.org, *pDir = qp.dir, *sOrg = qs.org, *sDir = qs.dir;
+    const int *eId = qe.id, *pId = qp.id, *sId = qs.id;
+    float4 *eHit = qe.hit, *pHit = qp.hit;
+    float4 *pendShadow = pb.pendShadow;
+    const uint32_t *pathSeed = qe.pathSeed;
+    const uint32_t secDiv = qs.seedDiv; // probe and shadow entries: path * numNEE + j
+    persistentTrace<kHitLane, COUNT, ALPHA>(
+        S, nep + ns, &cnt->fetch[1], st, vc,
         [&](uint32_t i, LaneRay &L) {
-            float4 o = q.org[i], d = q.dir[i];
+            const bool ext = i < ne, shadow = i >= nep;
+            const uint32_t j = ext ? i : shadow ? i - nep : i - ne;
+            float4 o, d;
+            uint32_t id = 0;
+            if (ext) {
+                o = eOrg[j];
+                d = eDir[j];
+                if (ALPHA) id = (uint32_t)eId[j];
+            } else if (shadow) {
+                o = sOrg[j];
+                d = sDir[j];
+                if (ALPHA) id = (uint32_t)sId[j];
+            } else {
+                o = pOrg[j];
+                d = pDir[j];
+                if (ALPHA) id = (uint32_t)pId[j];
+            }
             L.r = makeTravRay(v3(o), v3(d));
             L.tmin = o.w;
             L.tmax = d.w;
-            if (ALPHA) L.randomUInt = raySeed(S, q, i);
+            L.any = shadow;
+            if (ALPHA) {
+                id &= 0x7fffffffu;
+                L.randomUInt = pathSeed[ext ? id : id / secDiv];
+            }
         },
         [&](const LaneRay &L) {
-            if (L.best < 0) pb.pendShadow[q.id[L.idx]].w = 1.0f; // unoccluded: raytrace.shadow.rmiss:7-10
+            const bool ext = L.idx < ne;
+            if (L.any) {
+                if (L.best < 0) pendShadow[sId[L.idx - nep]].w = 1.0f; // raytrace.shadow.rmiss:7-10
+            } else if (ext) {
+                eHit[L.idx] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv);
+            } else {
+                pHit[L.idx - ne] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv);
+            }
+            if (COUNT) {
+                const int k = ext ? 0 : L.any ? 1 : 2;
+                for (int m = 0; m < 3; ++m) {
+                    if (m == k) {
+                        vk[m].nodes += vc.nodes;
+                        vk[m].prims += vc.prims;
+                    }
+                }
+                vc = VisitCount{};
+            }
         });
-    flushVisits<COUNT>(vc, cnt, 1);
+    flushVisits<COUNT>(vk[0], cnt, 0);
+    flushVisits<COUNT>(vk[1], cnt, 1);
+    flushVisits<COUNT>(vk[2], cnt, 2);
 }
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kShadowP(DevScene S, Queue q, PathBuffers pb, Counters *cnt) {
+__global__ __launch_bounds__(kTraceBlock) void kTraceP(DevScene S, Queue qe, int esel, Queue qs, Queue qp,
+                                                      PathBuffers pb, Counters *cnt) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
-    shadowBody<COUNT, false>(S, q, pb, cnt, stackMem);
+    traceBody<COUNT, false>(S, qe, esel, qs, qp, pb, cnt, stackMem);
 }
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kShadowPA(DevScene S, Queue q, PathBuffers pb, Counters *cnt) {
+__global__ __launch_bounds__(kTraceBlock) void kTracePA(DevScene S, Queue qe, int esel, Queue qs, Queue qp,
+                                                       PathBuffers pb, Counters *cnt) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
-    shadowBody<COUNT, true>(S, q, pb, cnt, stackMem);
+    traceBody<COUNT, true>(S, qe, esel, qs, qp, pb, cnt, stackMem);
 }
 
 // The BSDF-side half of the MIS probe (raytrace.rgen:684-726) for probe rays
-// whose closest hits kClosestP wrote.
+// whose closest hits the trace kernels wrote.  Also clears, for bounce qsel,
+// the counters of the next bounce's extend queue and of this bounce's done
+// list: their previous contents (two bounces back) are consumed, and the
+// kernels that append to them (kFinish, kShade) run after this one.
 __global__ __launch_bounds__(256) void kProbeResolve(DevScene S, const FrameConst *__restrict__ fcp, Queue q,
-                                                     PathBuffers pb, Counters *cnt) {
+                                                     PathBuffers pb, Counters *cnt, int qsel) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt->extend[qsel ^ 1] = 0;
+        cnt->done[qsel] = 0;
+    }
     const uint32_t n = cnt->probe;
     const bool power = fcp->pc.usePowerHeuristic != 0;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -762,7 +798,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             qout.org[qi] = f4(h.worldPos, kTMin);
             qout.dir[qi] = f4(newDir, kTMax);
         }
-        int di = waveAppend(isDone, &cnt->done);
+        int di = waveAppend(isDone, &cnt->done[qsel]);
         if (isDone) done[di] = (int)pid;
     }
 }
@@ -773,11 +809,20 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
 // joins main()'s, and the path goes on with the frame's next camera sample
 // (raytrace.rgen:1666-1681) or its next queued split (1684-1708), appended to
 // the next iteration's extend queue; after the last one the pixel's frame
-// result is main()'s sum over the camera samples (1710).
+// result is main()'s sum over the camera samples (1710).  Runs in bounce
+// qsel, before its kShade, on the paths the previous bounce's kShade ended
+// (done list qsel ^ 1, whose NEE rays this bounce traced), appending to the
+// next bounce's extend queue (qn, count extend[qsel ^ 1]).
 __global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
                                                const int32_t *done, Counters *cnt, Queue qn, int qsel) {
     const FrameConst &fc = *fcp;
-    const uint32_t n = cnt->done;
+    const uint32_t n = cnt->done[qsel ^ 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the secondary queues were traced and resolved; kShade refills them
+        cnt->shadow = 0;
+        cnt->probe = 0;
+        cnt->fetch[0] = cnt->fetch[1] = cnt->fetch[2] = 0;
+    }
     const uint32_t numNEE = fc.numNEE;
     const bool record = fc.record != 0;
     const bool segments = fc.segments != 0;
@@ -890,7 +935,7 @@ __global__ void kResetCounters(Counters *cnt, int which) {
         cnt->extend[which] = 0;
         cnt->shadow = 0;
         cnt->probe = 0;
-        cnt->done = 0;
+        cnt->done[0] = cnt->done[1] = 0;
         cnt->fetch[0] = cnt->fetch[1] = cnt->fetch[2] = 0;
     }
 }
@@ -945,7 +990,7 @@ __global__ __launch_bounds__(kTraceBlock) void kQueryP(DevScene S, const rtx_ray
     HybridStack<kTraceBlock, kLdsStack> st;
     st.lds = stackMem + threadIdx.x;
     VisitCount vc;
-    persistentTrace<ANY, false, true>( // alpha-tested triangles with randomUInt 0
+    persistentTrace<ANY ? kHitAny : kHitClosest, false, true>( // alpha-tested triangles with randomUInt 0
         S, n, fetch, st, vc,
         [&](uint32_t i, LaneRay &L) {
             rtx_ray ry = rays[i];
@@ -986,42 +1031,20 @@ void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int 
     if (countVisits) hipLaunchKernelGGL(kExtend<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
     else hipLaunchKernelGGL(kExtend<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
 }
-void launchClosestP(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, int kind, bool countVisits,
-                    int blocks) {
-    const dim3 g(blocks), t(kTraceBlock);
-    const bool a = S.alphaTris != 0;
-    if (kind == 0) {
-        if (a) {
-            if (countVisits) hipLaunchKernelGGL((kClosestPA<true, 0>), g, t, 0, s, S, q, cnt, qsel);
-            else hipLaunchKernelGGL((kClosestPA<false, 0>), g, t, 0, s, S, q, cnt, qsel);
-        } else {
-            if (countVisits) hipLaunchKernelGGL((kClosestP<true, 0>), g, t, 0, s, S, q, cnt, qsel);
-            else hipLaunchKernelGGL((kClosestP<false, 0>), g, t, 0, s, S, q, cnt, qsel);
-        }
-    } else {
-        if (a) {
-            if (countVisits) hipLaunchKernelGGL((kClosestPA<true, 2>), g, t, 0, s, S, q, cnt, qsel);
-            else hipLaunchKernelGGL((kClosestPA<false, 2>), g, t, 0, s, S, q, cnt, qsel);
-        } else {
-            if (countVisits) hipLaunchKernelGGL((kClosestP<true, 2>), g, t, 0, s, S, q, cnt, qsel);
-            else hipLaunchKernelGGL((kClosestP<false, 2>), g, t, 0, s, S, q, cnt, qsel);
-        }
-    }
-}
-void launchShadowP(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
-                   int blocks) {
+void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
+                  Counters *cnt, bool countVisits, int blocks) {
     const dim3 g(blocks), t(kTraceBlock);
     if (S.alphaTris != 0) {
-        if (countVisits) hipLaunchKernelGGL(kShadowPA<true>, g, t, 0, s, S, q, pb, cnt);
-        else hipLaunchKernelGGL(kShadowPA<false>, g, t, 0, s, S, q, pb, cnt);
+        if (countVisits) hipLaunchKernelGGL(kTracePA<true>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt);
+        else hipLaunchKernelGGL(kTracePA<false>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt);
     } else {
-        if (countVisits) hipLaunchKernelGGL(kShadowP<true>, g, t, 0, s, S, q, pb, cnt);
-        else hipLaunchKernelGGL(kShadowP<false>, g, t, 0, s, S, q, pb, cnt);
+        if (countVisits) hipLaunchKernelGGL(kTraceP<true>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt);
+        else hipLaunchKernelGGL(kTraceP<false>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt);
     }
 }
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
-                        Counters *cnt, int blocks) {
-    hipLaunchKernelGGL(kProbeResolve, dim3(blocks), dim3(256), 0, s, S, fc, q, pb, cnt);
+                        Counters *cnt, int qsel, int blocks) {
+    hipLaunchKernelGGL(kProbeResolve, dim3(blocks), dim3(256), 0, s, S, fc, q, pb, cnt, qsel);
 }
 void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const DevGuiding &g, const FrameConst *fc,
                  BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt,
