@@ -307,6 +307,11 @@ typedef struct rtx_scene_desc {
 #define RTX_FRAME_TIME_KERNELS 0x8u /* time every kernel with HIP events      */
 #define RTX_FRAME_NO_GUIDING_UPDATE 0x10u /* record (pc.updateGuiding) but skip the
                                            * guiding update after the frame     */
+#define RTX_FRAME_TIME_TRACE 0x20u  /* time only the traversal launches with HIP
+                                     * events (msExtend, msShadow; the other
+                                     * per-kind times stay 0): each event pair
+                                     * costs device time, so this keeps the
+                                     * timing of the dominant kernel cheap    */
 
 /* One call to rtx_render renders `spp` frames and adds each frame's result
  * (per pixel: the mean of pc.samplesPerPixel camera samples and their

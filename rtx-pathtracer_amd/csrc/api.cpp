@@ -34,6 +34,10 @@ struct EventPair {
 };
 
 enum { kClsExtend = 0, kClsShadow = 1, kClsProbe = 2, kClsShade = 3 };
+// Bounces the device may run ahead of the host's loop-end check, and the
+// poll ring's slots (more than kPollLead, so that a slot is read before the
+// bounce that reuses it is launched).
+constexpr int kPollLead = 3, kPollSlots = 8;
 
 } // namespace
 
@@ -53,6 +57,11 @@ struct rtx_ctx {
     int32_t *done = nullptr;
     Counters *cnt = nullptr;
     FrameConst *fc = nullptr;
+    // bounce-loop end detection: each bounce's trace kernel writes the queue
+    // lengths it starts from into a slot of this pinned ring (kPollSlots
+    // slots of 4 words), and an event after the trace marks the slot readable
+    uint32_t *pollRing = nullptr;
+    hipEvent_t pollEv[kPollSlots] = {};
     std::vector<void *> batchAllocs;
     // per-path results of the last render
     float4 *keepRgb = nullptr;
@@ -443,9 +452,16 @@ int rtx_ctx_create(int device, rtx_ctx **out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&ctx->cnt, sizeof(Counters)) != hipSuccess || hipMalloc(&ctx->fc, sizeof(FrameConst)) != hipSuccess ||
-        hipEventCreate(&ctx->tStart) != hipSuccess || hipEventCreate(&ctx->tStop) != hipSuccess) {
+        hipEventCreate(&ctx->tStart) != hipSuccess || hipEventCreate(&ctx->tStop) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->pollRing, kPollSlots * 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         rtx_ctx_destroy(ctx);
         return RTX_E_DEVICE;
+    }
+    for (hipEvent_t &e : ctx->pollEv) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            rtx_ctx_destroy(ctx);
+            return RTX_E_DEVICE;
+        }
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -482,6 +498,9 @@ int rtx_ctx_destroy(rtx_ctx *ctx) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
+    for (hipEvent_t e : ctx->pollEv)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->pollRing) (void)hipHostFree(ctx->pollRing);
     if (ctx->tStart) (void)hipEventDestroy(ctx->tStart);
     if (ctx->tStop) (void)hipEventDestroy(ctx->tStop);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -921,6 +940,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
 
     const bool countVisits = fc.countVisits != 0;
     const bool timeKernels = (p->flags & RTX_FRAME_TIME_KERNELS) != 0;
+    const bool timeTrace = timeKernels || (p->flags & RTX_FRAME_TIME_TRACE) != 0;
     ctx->eventsUsed = 0;
     const int traceBlocks = ctx->cuCount * ctx->tuning.traceBlocksPerCU;
     const int shadeBlocks = ctx->cuCount * ctx->tuning.shadeBlocksPerCU;
@@ -935,7 +955,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     uint32_t launches[4] = {0, 0, 0, 0};
     auto timed = [&](int cls, auto &&launch) {
         launches[cls]++;
-        EventPair *ev = timeKernels ? nextEvents(ctx, cls) : nullptr;
+        const bool trace = cls == kClsExtend || cls == kClsShadow;
+        EventPair *ev = timeKernels || (timeTrace && trace) ? nextEvents(ctx, cls) : nullptr;
         if (ev) (void)hipEventRecord(ev->a, s);
         launch();
         if (ev) (void)hipEventRecord(ev->b, s);
@@ -948,10 +969,11 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     // A path ends within `iters` iterations of its start, and one more
     // bounce drains its last secondary rays.  With segments a path slot runs
     // a pixel's whole frame (several camera samples, queued splits), so the
-    // loop goes on until every queue is empty.  Every kPollEvery iterations
-    // the queue lengths are read back and the loop ends once they are all
-    // empty (later iterations would only launch kernels over empty queues).
-    constexpr int kPollEvery = 4;
+    // loop goes on until every queue is empty.  The host learns the queue
+    // lengths each bounce started from through the poll ring, kPollLead
+    // bounces late, so the stream never waits for the host: once a bounce
+    // is known to be followed only by empty ones, the loop ends (the bounces
+    // launched meanwhile run over empty queues).
     int itersRun = 0;
     // where the any-hit alpha test finds each ray's frame seed: extend entries
     // are path ids, shadow and probe entries path * numNEE + j
@@ -966,26 +988,31 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         const int cap = multi ? (int)(samplesPerFrame + kMaxSplits + 1) * (iters + 1) : iters + 1;
         for (int it = 0;; ++it) {
             if (it == cap && !multi) break;
-            if (it > 0 && it % kPollEvery == 0) {
-                uint32_t c[6] = {0, 0, 0, 0, 0, 0}; // extend[2], shadow, probe, done[2]
-                static_assert(offsetof(Counters, done) == 4 * sizeof(uint32_t), "counter layout");
-                CHECK_HIP(ctx, hipMemcpyAsync(c, ctx->cnt, sizeof(c), hipMemcpyDeviceToHost, s));
-                CHECK_HIP(ctx, hipStreamSynchronize(s));
-                if (c[qsel] == 0 && c[2] == 0 && c[3] == 0 && c[4 + (qsel ^ 1)] == 0) break;
+            if (it >= kPollLead) {
+                const int j = (it - kPollLead) % kPollSlots;
+                CHECK_HIP(ctx, hipEventSynchronize(ctx->pollEv[j]));
+                const uint32_t *c = ctx->pollRing + 4 * j;
+                // c = lengths bounce j started from: extend, probe, shadow, done.
+                // With no extend rays, bounce j shades nothing, so bounce j + 1
+                // starts empty unless kFinish restarts segments of j's done paths.
+                if (c[0] == 0 && (!multi || c[3] == 0)) break;
                 if (it >= cap) return fail(ctx, RTX_E_DEVICE, "paths did not finish within the iteration bound");
             }
+            uint32_t *poll = ctx->pollRing + 4 * (it % kPollSlots);
             itersRun = std::max(itersRun, it + 1);
             if (it == 0) timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
             if (persistent) {
                 timed(kClsExtend, [&] {
-                    launchTraceP(s, S, ctx->q[qsel], qsel, ctx->qs, ctx->qp, ctx->pb, ctx->cnt, countVisits,
+                    launchTraceP(s, S, ctx->q[qsel], qsel, ctx->qs, ctx->qp, ctx->pb, ctx->cnt, poll, countVisits,
                                  traceBlocks);
                 });
             } else {
-                timed(kClsExtend, [&] { launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, countVisits, traceBlocks); });
+                timed(kClsExtend,
+                      [&] { launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, poll, countVisits, traceBlocks); });
                 timed(kClsShadow, [&] { launchShadow(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
                 timed(kClsShadow, [&] { launchProbe(s, S, ctx->qp, ctx->cnt, countVisits, traceBlocks); });
             }
+            CHECK_HIP(ctx, hipEventRecord(ctx->pollEv[it % kPollSlots], s));
             timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, qsel, shadeBlocks); });
             timed(kClsShade, [&] {
                 launchFinish(s, fcd, b, ctx->pb, ctx->done, ctx->cnt, ctx->q[qsel ^ 1], qsel, shadeBlocks);

@@ -270,12 +270,14 @@ constexpr uint32_t kSplitVertex = 1u << 31;
 
 void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue q0,
                     Counters *cnt, int qsel, uint32_t *electCounter);
-void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits,
+// poll (may be NULL): host-visible slot for the bounce's starting queue
+// lengths (extend, probe, shadow, done), written by the first thread
+void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, uint32_t *poll, bool countVisits,
                   int blocks);
 // persistent lane-refill traversal of one bounce: extend queue qe (count
 // extend[esel]), then the probe and shadow queues; fetch[1] must be zero
 void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
-                  Counters *cnt, bool countVisits, int blocks);
+                  Counters *cnt, uint32_t *poll, bool countVisits, int blocks);
 // persistent variant of the two queries; *fetch must be zero
 void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_t n, void *out, bool closest,
                   uint32_t *fetch, int blocks);

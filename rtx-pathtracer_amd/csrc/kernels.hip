@@ -178,12 +178,27 @@ RTX_DI uint32_t raySeed(const DevScene &S, const Queue &q, uint32_t i) {
     return S.alphaTris ? q.pathSeed[((uint32_t)q.id[i] & 0x7fffffffu) / q.seedDiv] : 0u;
 }
 
+// The work a bounce starts with (its extend queue, the previous bounce's
+// probe, shadow and done lists), written to the host-visible slot `poll`
+// (may be NULL) so that the host can end the bounce loop without stopping
+// the stream.
+RTX_DI void reportBounce(uint32_t *poll, uint32_t ne, uint32_t np, uint32_t ns, uint32_t nd) {
+    if (!poll) return;
+    poll[0] = ne;
+    poll[1] = np;
+    poll[2] = ns;
+    poll[3] = nd;
+}
+
 // --------------------------------------------------------------- extend
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kExtend(DevScene S, Queue q, Counters *cnt, int qsel) {
+__global__ __launch_bounds__(kTraceBlock) void kExtend(DevScene S, Queue q, Counters *cnt, int qsel, uint32_t *poll) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
     const uint32_t n = cnt->extend[qsel];
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->extendRays, (unsigned long long)n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&cnt->extendRays, (unsigned long long)n);
+        reportBounce(poll, n, cnt->probe, cnt->shadow, cnt->done[qsel ^ 1]);
+    }
     HybridStack<kTraceBlock, kLdsStack> st;
     st.lds = stackMem + threadIdx.x;
     VisitCount vc;
@@ -277,13 +292,14 @@ __global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, Queue q, Count
 // with it (kTracePA), so that the test's code stays out of the common loop.
 template <bool COUNT, bool ALPHA>
 RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue &qs, const Queue &qp,
-                      const PathBuffers &pb, Counters *cnt, int *stackMem) {
+                      const PathBuffers &pb, Counters *cnt, uint32_t *poll, int *stackMem) {
     const uint32_t ne = cnt->extend[esel], np = cnt->probe, ns = cnt->shadow;
     const uint32_t nep = ne + np;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&cnt->extendRays, (unsigned long long)ne);
         atomicAdd(&cnt->probeRays, (unsigned long long)np);
         atomicAdd(&cnt->shadowRays, (unsigned long long)ns);
+        reportBounce(poll, ne, np, ns, cnt->done[esel ^ 1]);
     }
     HybridStack<kTraceBlock, kLdsStack> st;
     st.lds = stackMem + threadIdx.x;
@@ -351,15 +367,15 @@ RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue 
 }
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kTraceP(DevScene S, Queue qe, int esel, Queue qs, Queue qp,
-                                                      PathBuffers pb, Counters *cnt) {
+                                                      PathBuffers pb, Counters *cnt, uint32_t *poll) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
-    traceBody<COUNT, false>(S, qe, esel, qs, qp, pb, cnt, stackMem);
+    traceBody<COUNT, false>(S, qe, esel, qs, qp, pb, cnt, poll, stackMem);
 }
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kTracePA(DevScene S, Queue qe, int esel, Queue qs, Queue qp,
-                                                       PathBuffers pb, Counters *cnt) {
+                                                       PathBuffers pb, Counters *cnt, uint32_t *poll) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
-    traceBody<COUNT, true>(S, qe, esel, qs, qp, pb, cnt, stackMem);
+    traceBody<COUNT, true>(S, qe, esel, qs, qp, pb, cnt, poll, stackMem);
 }
 
 // The BSDF-side half of the MIS probe (raytrace.rgen:684-726) for probe rays
@@ -1027,19 +1043,20 @@ void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t 
     hipLaunchKernelGGL(kGenerateIC, dim3(gridFor(numPaths, 256)), dim3(256), 0, s, ic, frameSeed, firstJob, numPaths,
                        pb, q0, cnt, qsel);
 }
-void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, bool countVisits, int blocks) {
-    if (countVisits) hipLaunchKernelGGL(kExtend<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
-    else hipLaunchKernelGGL(kExtend<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel);
+void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, uint32_t *poll, bool countVisits,
+                  int blocks) {
+    if (countVisits) hipLaunchKernelGGL(kExtend<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel, poll);
+    else hipLaunchKernelGGL(kExtend<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel, poll);
 }
 void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
-                  Counters *cnt, bool countVisits, int blocks) {
+                  Counters *cnt, uint32_t *poll, bool countVisits, int blocks) {
     const dim3 g(blocks), t(kTraceBlock);
     if (S.alphaTris != 0) {
-        if (countVisits) hipLaunchKernelGGL(kTracePA<true>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt);
-        else hipLaunchKernelGGL(kTracePA<false>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt);
+        if (countVisits) hipLaunchKernelGGL(kTracePA<true>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
+        else hipLaunchKernelGGL(kTracePA<false>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
     } else {
-        if (countVisits) hipLaunchKernelGGL(kTraceP<true>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt);
-        else hipLaunchKernelGGL(kTraceP<false>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt);
+        if (countVisits) hipLaunchKernelGGL(kTraceP<true>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
+        else hipLaunchKernelGGL(kTraceP<false>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
     }
 }
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,

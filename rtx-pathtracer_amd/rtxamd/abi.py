@@ -19,6 +19,7 @@ RTX_FRAME_COUNT_VISITS = 0x2
 RTX_FRAME_RESET_ACCUM = 0x4
 RTX_FRAME_TIME_KERNELS = 0x8
 RTX_FRAME_NO_GUIDING_UPDATE = 0x10
+RTX_FRAME_TIME_TRACE = 0x20
 
 MAT_DIFFUSE, MAT_SPECULAR, MAT_DIELECTRIC, MAT_LIGHT, MAT_PHONG, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR = range(7)
 LIGHT_AREA, LIGHT_POINT, LIGHT_SPHERE, LIGHT_ENV_MAP = range(4)
