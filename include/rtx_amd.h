@@ -387,9 +387,10 @@ typedef struct rtx_scene_info {
  * keeps the context's current value. */
 typedef struct rtx_tuning {
     int32_t traversalKernel;  /* 1 = persistent lane-refill while-while (default), 2 = grid-stride stack loop */
-    int32_t traceBlocksPerCU; /* 128-thread traversal blocks per CU (default 12:
+    int32_t traceBlocksPerCU; /* 128-thread traversal blocks per CU (default 14:
                                  every block of the persistent kernel is
-                                 resident at once, 12 KB of LDS stack each) */
+                                 resident at once, 10 KB of LDS stack each,
+                                 7 waves per SIMD) */
     int32_t shadeBlocksPerCU; /* 256-thread shading blocks per CU (default 8) */
     uint32_t batchPaths;      /* paths in flight per batch when rtx_frame_params.batchPaths is 0 (2^22) */
     int32_t _reserved[4];

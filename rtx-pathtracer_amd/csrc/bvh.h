@@ -32,7 +32,7 @@ constexpr uint32_t kBvh4NodeBytesRead = 112; // the seven slots the traversal lo
 
 constexpr int kLeafMaxPrims = 16;
 constexpr int kMaxBvhDepth = 48;   // depth limit of the binary build
-constexpr int kStackCapacity = 64; // traversal stack entries (24 in LDS, the rest in registers/scratch)
+constexpr int kStackCapacity = 64; // traversal stack entries (20 in LDS, the rest in registers/scratch)
 
 inline int32_t makeLeaf(uint32_t start, uint32_t count) {
     return ~(int32_t)((start << 4) | (count - 1));

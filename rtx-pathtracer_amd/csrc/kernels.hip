@@ -23,7 +23,7 @@
 namespace rtx {
 
 constexpr int kTraceBlock = 128;
-constexpr int kLdsStack = 24;
+constexpr int kLdsStack = 20;
 constexpr int kShadeBlock = 256;
 
 // --------------------------------------------------------- wave helpers
