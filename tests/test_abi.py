@@ -129,3 +129,16 @@ def test_context_creation_without_device_fails_cleanly():
     h = C.c_void_p()
     rc = rtxamd.device_lib().rtx_ctx_create(0, C.byref(h))
     assert rc == abi.RTX_E_DEVICE and not h.value
+
+
+def test_header_constants_match_python_mirror():
+    """Every integer #define of the C headers that the Python mirror also
+    names has the same value there (frame flags, error codes, limits)."""
+    checked = 0
+    for h in sorted(INC.glob("*.h")):
+        for name, val in re.findall(r"^#define\s+(RTX\w*)\s+\(?(-?(?:0x[0-9A-Fa-f]+|\d+))[uU]?\)?\s*(?:/\*.*)?$",
+                                    h.read_text(), re.M):
+            if hasattr(abi, name):
+                assert getattr(abi, name) == int(val, 0), (h.name, name)
+                checked += 1
+    assert checked >= 10 and abi.RTX_FRAME_TIME_TRACE == 0x20
