@@ -159,52 +159,67 @@ RTX_DI V3 sphericalToCartesian(float theta, float phi) { // shaders/transform.gl
 // sub-batch, in the reference's loop order.
 __global__ __launch_bounds__(64) void kIcReduce(IcBuffers ic, uint32_t frameSeed, uint32_t firstJob,
                                                 uint32_t numJobs, PathBuffers pb) {
-    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
-    if (t >= numJobs) return;
+    // One 64-lane block per job: the lanes compute every stratum's terms
+    // (trigonometry, neighbour differences) in parallel, then lane 0 adds
+    // them in the loop order of calculateCacheData, so the sums are the
+    // sequential loop's.
+    __shared__ float4 smp[kIcStrata];                            // sample radiance, w = hit distance
+    __shared__ float rotT[3 * kIcStrata], ta[3 * kIcStrata], tb[3 * kIcStrata];
+    const uint32_t t = blockIdx.x;
+    if (t >= numJobs) return; // uniform per block
     const uint32_t job = firstJob + t;
     const V3 normal = v3(ic.jobs[2 * job + 1]);
     const float4 *sub = pb.out + (size_t)t * kIcStrata;
-    float invDistanceSum = 0.0f;
-    int numDistances = 0;
-    V3 color = v3(0.0f), rot = v3(0.0f), trans = v3(0.0f);
-    float prevKL[kIcStrataTheta], prevKR[kIcStrataTheta];
-    for (int k = 0; k < kIcStrataPhi; ++k) {
+    for (int q = (int)threadIdx.x; q < kIcStrata; q += 64) smp[q] = sub[q];
+    __syncthreads();
+    for (int q = (int)threadIdx.x; q < kIcStrata; q += 64) {
+        const int k = q / kIcStrataTheta, j = q % kIcStrataTheta;
         Rng col{tea(job * kIcStrataPhi + (uint32_t)k, frameSeed ^ kIcPhiKey)};
         float phi = 2.0f * kPi * ((float)k + col.next()) / (float)kIcStrataPhi;
         V3 uk = toWorld(sphericalToCartesian(kHalfPi, phi), normal);
         V3 vk = toWorld(sphericalToCartesian(kHalfPi, phi + kHalfPi), normal);
         V3 prevVk = toWorld(sphericalToCartesian(kHalfPi, 2.0f * kPi * (float)k / (float)kIcStrataPhi + kHalfPi),
                             normal);
-        float prevJR = 0.0f, prevJL = 0.0f;
-        for (int j = 0; j < kIcStrataTheta; ++j) {
-            Rng rs{tea((job * kIcStrataPhi + (uint32_t)k) * kIcStrataTheta + (uint32_t)j, frameSeed ^ kIcStratumKey)};
-            float theta = asinf(sqrtf(((float)j + rs.next()) / (float)kIcStrataTheta));
-            float4 o = sub[k * kIcStrataTheta + j];
-            V3 sample = v3(o);
-            float r = o.w;
-            color = color + sample;
-            if (r < kTMax) {
-                invDistanceSum += 1.0f / r;
-                numDistances++;
-            }
-            float L = length(sample);
-            float prevJTheta = asinf(sqrtf((float)j / (float)kIcStrataTheta));
-            float nextJTheta = asinf(sqrtf((float)(j + 1) / (float)kIcStrataTheta));
-            float tanTheta = tanf(theta);
-            if (isinf(tanTheta) || isnan(tanTheta)) tanTheta = 0.0f;
-            rot = rot - tanTheta * L * vk;
-            if (j > 0) {
-                float cosPrev = cosf(prevJTheta);
-                trans = trans + uk * 2.0f * kPi / (float)kIcStrataPhi * sinf(prevJTheta) * cosPrev * cosPrev /
-                                    fminf(r, prevJR) * (L - prevJL);
-            }
-            if (k > 0)
-                trans = trans + prevVk * (sinf(nextJTheta) - sinf(prevJTheta)) / fminf(r, prevKR[j]) * (L - prevKL[j]);
-            prevKL[j] = L;
-            prevKR[j] = r;
-            prevJL = L;
-            prevJR = r;
+        Rng rs{tea((job * kIcStrataPhi + (uint32_t)k) * kIcStrataTheta + (uint32_t)j, frameSeed ^ kIcStratumKey)};
+        float theta = asinf(sqrtf(((float)j + rs.next()) / (float)kIcStrataTheta));
+        const float4 o = smp[q];
+        const float r = o.w;
+        const float L = length(v3(o));
+        float prevJTheta = asinf(sqrtf((float)j / (float)kIcStrataTheta));
+        float nextJTheta = asinf(sqrtf((float)(j + 1) / (float)kIcStrataTheta));
+        float tanTheta = tanf(theta);
+        if (isinf(tanTheta) || isnan(tanTheta)) tanTheta = 0.0f;
+        V3 rt = tanTheta * L * vk, a = v3(0.0f), c = v3(0.0f);
+        if (j > 0) {
+            const float4 pj = smp[q - 1]; // stratum (k, j - 1)
+            float cosPrev = cosf(prevJTheta);
+            a = uk * 2.0f * kPi / (float)kIcStrataPhi * sinf(prevJTheta) * cosPrev * cosPrev / fminf(r, pj.w) *
+                (L - length(v3(pj)));
         }
+        if (k > 0) {
+            const float4 pk = smp[q - kIcStrataTheta]; // stratum (k - 1, j)
+            c = prevVk * (sinf(nextJTheta) - sinf(prevJTheta)) / fminf(r, pk.w) * (L - length(v3(pk)));
+        }
+        rotT[3 * q] = rt.x, rotT[3 * q + 1] = rt.y, rotT[3 * q + 2] = rt.z;
+        ta[3 * q] = a.x, ta[3 * q + 1] = a.y, ta[3 * q + 2] = a.z;
+        tb[3 * q] = c.x, tb[3 * q + 1] = c.y, tb[3 * q + 2] = c.z;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    float invDistanceSum = 0.0f;
+    int numDistances = 0;
+    V3 color = v3(0.0f), rot = v3(0.0f), trans = v3(0.0f);
+    for (int q = 0; q < kIcStrata; ++q) {
+        const int k = q / kIcStrataTheta, j = q % kIcStrataTheta;
+        const float4 o = smp[q];
+        color = color + v3(o);
+        if (o.w < kTMax) {
+            invDistanceSum += 1.0f / o.w;
+            numDistances++;
+        }
+        rot = rot - V3{rotT[3 * q], rotT[3 * q + 1], rotT[3 * q + 2]};
+        if (j > 0) trans = trans + V3{ta[3 * q], ta[3 * q + 1], ta[3 * q + 2]};
+        if (k > 0) trans = trans + V3{tb[3 * q], tb[3 * q + 1], tb[3 * q + 2]};
     }
     const float normFactor = kPi / (float)(kIcStrataTheta * kIcStrataPhi);
     color = normFactor * color;
@@ -412,7 +427,7 @@ void launchIcPlan(hipStream_t s, IcBuffers ic) {
 void launchIcReduce(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numJobs,
                     PathBuffers pb) {
     if (numJobs == 0) return;
-    hipLaunchKernelGGL(kIcReduce, dim3(blocksFor(numJobs, 64)), dim3(64), 0, s, ic, frameSeed, firstJob, numJobs, pb);
+    hipLaunchKernelGGL(kIcReduce, dim3(numJobs), dim3(64), 0, s, ic, frameSeed, firstJob, numJobs, pb);
 }
 void launchIcCommit(hipStream_t s, IcBuffers ic, const FrameConst *fc) {
     hipLaunchKernelGGL(kIcCommit, dim3(1), dim3(64), 0, s, ic, fc);
