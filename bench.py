@@ -58,7 +58,6 @@ def main():
     ap.add_argument("--prepare-frames", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traversal", type=int, default=0, help="1 persistent lane-refill, 2 grid-stride (0: default)")
     ap.add_argument("--trace-blocks", type=int, default=0, help="traversal blocks per CU (0: default)")
     ap.add_argument("--shade-blocks", type=int, default=0, help="shading blocks per CU (0: default)")
     ap.add_argument("--no-count", action="store_true", help="skip the node/triangle counting pass")
@@ -87,11 +86,9 @@ def main():
     W, H, spp = a.width, a.height, a.spp
     hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 1, a.triangles)
     r = rtxamd.Renderer(0)
-    r.set_tuning(traversalKernel=a.traversal, traceBlocksPerCU=a.trace_blocks, shadeBlocksPerCU=a.shade_blocks)
+    r.set_tuning(traceBlocksPerCU=a.trace_blocks, shadeBlocksPerCU=a.shade_blocks)
     tun = r.tuning()
-    persistent = tun.traversalKernel == 1
-    trace_kernel = "kTraceP<false> (persistent BVH traversal per bounce: extend, probe and shadow rays)" \
-        if persistent else "kExtend<false> (grid-stride closest-hit BVH traversal of extend rays)"
+    trace_kernel = "kTraceP<false> (persistent BVH traversal per bounce: extend, probe and shadow rays)"
     t0 = time.time()
     ds = r.upload(hs)
     info = ds.info()
@@ -155,16 +152,16 @@ def main():
     # the dominant kernel: the persistent per-bounce trace covers every ray;
     # the grid-stride build's extend kernel covers the extend rays
     launches_tr = launches[0] // a.steps
-    tr_bytes = all_bytes if persistent else ext_bytes
-    tr_rays = all_rays if persistent else ext_rays
-    tr_nodes = sum(nodes) if persistent else nodes[0]
-    tr_prims = sum(prims) if persistent else prims[0]
+    tr_bytes = all_bytes
+    tr_rays = all_rays
+    tr_nodes = sum(nodes)
+    tr_prims = sum(prims)
     avg_launch_ms = totals["msExtend"] / a.steps / max(1, launches_tr)
     achieved = tr_bytes / max(1, launches_tr) / (avg_launch_ms * 1e-3) / 1e9
     trav_ms = (totals["msExtend"] + totals["msShadow"]) / a.steps
     traffic = None
     tfile = ROOT / "profiles" / f"trace_traffic_room262k_1080p_{a.method}.json"
-    if persistent and tfile.exists():
+    if tfile.exists():
         try:
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except Exception:

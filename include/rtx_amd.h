@@ -55,6 +55,15 @@ extern "C" {
 #define RTX_MAT_CONDUCTOR 5
 #define RTX_MAT_ROUGH_CONDUCTOR 6
 
+/* rtx_material.distribution of a rough conductor.  BECKMANN is the
+ * reference's (shaders/raytrace.rgen:179-230, random.glsl:127-136, with its
+ * G1 as written); GGX is new (Walter et al. 2007: GGX D, separable Smith G,
+ * sampling of D(m)cos(theta_m)), selected by the Mitsuba roughconductor's
+ * "distribution" string, which the reference's loader ignores
+ * (src/SceneLoader.cpp:601-605). */
+#define RTX_DISTRIBUTION_BECKMANN 0
+#define RTX_DISTRIBUTION_GGX 1
+
 /* shaders/raytrace.rgen:29-32; src/SceneLoader.h:41-46 */
 #define RTX_LIGHT_AREA 0
 #define RTX_LIGHT_POINT 1
@@ -90,7 +99,11 @@ typedef struct rtx_material {
     int32_t textureIdDiffuse;
     int32_t textureIdSpecular;
     int32_t type;
-    int32_t _pad2[3];
+    /* Microfacet distribution of RTX_MAT_ROUGH_CONDUCTOR: RTX_DISTRIBUTION_*.
+     * New: the reference has Beckmann only and its std430 padding here is
+     * zero, so its bytes select Beckmann. */
+    int32_t distribution;
+    int32_t _pad2[2];
 } rtx_material;
 
 /* InstanceInfo, std140, 144 B: shaders/wavefront.glsl:25-30; src/SceneLoader.h:64-69.
@@ -348,12 +361,11 @@ typedef struct rtx_frame_stats {
     uint64_t nodeVisits;    /* BVH node fetches (RTX_FRAME_COUNT_VISITS)  */
     uint64_t primVisits;    /* primitive fetches (RTX_FRAME_COUNT_VISITS) */
     double msTotal;         /* device time of the whole render (HIP events) */
-    double msExtend;        /* time in traversal: the persistent build's
-                               one launch per bounce (extend rays with the
-                               previous bounce's probe and shadow rays), or
-                               the grid-stride build's extend kernel      */
-    double msShadow;        /* grid-stride build: shadow and probe ray
-                               traversal (0 in the persistent build)      */
+    double msExtend;        /* time in traversal: one launch per bounce
+                               (extend rays with the previous bounce's
+                               probe and shadow rays)                     */
+    double msShadow;        /* unused (0): shadow and probe rays are traced
+                               in the extend launch                       */
     double msProbe;         /* time in the probe's BSDF-side resolve        */
     double msShade;         /* time in generate/shade/finish/accumulate    */
     uint32_t batches;
@@ -386,7 +398,7 @@ typedef struct rtx_scene_info {
  * whose traversal is the driver's).  In rtx_ctx_set_tuning a zero field
  * keeps the context's current value. */
 typedef struct rtx_tuning {
-    int32_t traversalKernel;  /* 1 = persistent lane-refill while-while (default), 2 = grid-stride stack loop */
+    int32_t traversalKernel;  /* 1 = persistent lane-refill while-while (the only traversal) */
     int32_t traceBlocksPerCU; /* 128-thread traversal blocks per CU (default 14:
                                  every block of the persistent kernel is
                                  resident at once, 10 KB of LDS stack each,

@@ -58,6 +58,10 @@ def lib(abi):
         L.orc_fresnel_conductor.restype = C.c_double
         L.orc_beckmann_d.argtypes = [C.POINTER(abi.Material), d3, d3]
         L.orc_beckmann_d.restype = C.c_double
+        L.orc_microfacet_d.argtypes = [C.POINTER(abi.Material), d3, d3]
+        L.orc_microfacet_d.restype = C.c_double
+        L.orc_microfacet_g1.argtypes = [C.POINTER(abi.Material), d3, d3, d3]
+        L.orc_microfacet_g1.restype = C.c_double
         L.orc_pdf_bsdf.argtypes = [C.POINTER(abi.Material), d3, d3, d3]
         L.orc_pdf_bsdf.restype = C.c_double
         L.orc_eval_bsdf.argtypes = [C.POINTER(abi.Material), d3, d3, d3, C.c_int, d3]

@@ -158,6 +158,9 @@ void orc_camera_ray(const rtx_camera *cam, uint32_t width, uint32_t height, uint
 double orc_fresnel(double eta, double cosThetaI);
 double orc_fresnel_conductor(double cosThetaI, double eta, double k);
 double orc_beckmann_d(const rtx_material *mat, const double n[3], const double m[3]);
+/* D and G1(v) of the material's microfacet distribution (rtx_material.distribution) */
+double orc_microfacet_d(const rtx_material *mat, const double n[3], const double m[3]);
+double orc_microfacet_g1(const rtx_material *mat, const double n[3], const double m[3], const double v[3]);
 double orc_pdf_bsdf(const rtx_material *mat, const double n[3], const double wi[3], const double wo[3]);
 void orc_eval_bsdf(const rtx_material *mat, const double n[3], const double wi[3], const double wo[3],
                    int frontFace, double out[3]);

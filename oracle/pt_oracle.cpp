@@ -668,6 +668,20 @@ template <class R> struct Tracer {
         normal = normalize(normal);
         return f3<R>(s.center) + normal * (R)s.radius;
     }
+    // GGX normal with density D(m) cos(theta_m) (Walter et al. 2007, eq. 35-36:
+    // theta_m = atan(a sqrt(u1) / sqrt(1 - u1)), phi_m = 2 pi u2); new, drawn
+    // where the reference draws its Beckmann normal (two draws, same order)
+    V3<R> randomGGXNormal(const rtx_material &m, V3<R> n) {
+        R a = (R)m.roughness;
+        R u1 = rnd();
+        R thetaM = std::atan(a * std::sqrt(u1) / std::sqrt(R(1) - u1));
+        R phiM = R(2) * C::PI * rnd();
+        V3<R> localM(std::sin(thetaM) * std::cos(phiM), std::sin(thetaM) * std::sin(phiM), std::cos(thetaM));
+        return toWorld(localM, n);
+    }
+    V3<R> randomMicrofacetNormal(const rtx_material &m, V3<R> n) {
+        return m.distribution == RTX_DISTRIBUTION_GGX ? randomGGXNormal(m, n) : randomBeckmannNormal(m, n);
+    }
     // ref: shaders/random.glsl:127-136
     V3<R> randomBeckmannNormal(const rtx_material &m, V3<R> n) {
         R a = (R)m.roughness;
@@ -697,8 +711,32 @@ template <class R> struct Tracer {
         R Rp2 = (ek - R(2) * eta * cosThetaI + c2) / (ek + R(2) * eta * cosThetaI + c2);
         return (Rs2 + Rp2) / R(2);
     }
-    // Beckmann D, ref: shaders/raytrace.rgen:179-202 (pow(E, x) == exp(x))
+    // GGX D (Walter et al. 2007, eq. 33): a^2 / (pi cos^4 (a^2 + tan^2)^2)
+    // = a^2 / (pi ((a^2 - 1) cos^2 + 1)^2) for cos > 0; new (the reference has
+    // Beckmann only)
+    static R ggxD(const rtx_material &m, V3<R> n, V3<R> h) {
+        R c = dot(n, h);
+        if (c <= R(0)) return R(0);
+        R a2 = (R)m.roughness * (R)m.roughness;
+        R t = (a2 - R(1)) * c * c + R(1);
+        return a2 / (C::PI * t * t);
+    }
+    // Smith G1 of GGX (Walter et al. 2007, eq. 34):
+    // chi+(v.h / v.n) 2 / (1 + sqrt(1 + a^2 tan^2 theta_v))
+    static R ggxG1(const rtx_material &m, V3<R> n, V3<R> h, V3<R> v) {
+        R cv = dot(v, n);
+        if (dot(v, h) / cv <= R(0)) return R(0);
+        R tan2 = (R(1) - cv * cv) / (cv * cv);
+        if (tan2 < R(0)) tan2 = R(0);
+        R a = (R)m.roughness;
+        return R(2) / (R(1) + std::sqrt(R(1) + a * a * tan2));
+    }
+    // D of the material's microfacet distribution
     static R D(const rtx_material &m, V3<R> n, V3<R> h) {
+        return m.distribution == RTX_DISTRIBUTION_GGX ? ggxD(m, n, h) : beckmannD(m, n, h);
+    }
+    // Beckmann D, ref: shaders/raytrace.rgen:179-202 (pow(E, x) == exp(x))
+    static R beckmannD(const rtx_material &m, V3<R> n, V3<R> h) {
         R cosTheta = dot(n, h);
         if (cosTheta <= R(0)) return R(0);
         R theta = std::acos(cosTheta);
@@ -720,8 +758,10 @@ template <class R> struct Tracer {
         R a2 = a * a;
         return (R(3.535) * a + R(2.181) * a2) / (R(1) + R(2.276) * a + R(2.577) * a);
     }
-    // ref: shaders/raytrace.rgen:224-230
+    // ref: shaders/raytrace.rgen:224-230 (Beckmann); GGX: the separable Smith
+    // form G1(i) G1(o)
     static R G(const rtx_material &m, V3<R> i, V3<R> o, V3<R> n, V3<R> h) {
+        if (m.distribution == RTX_DISTRIBUTION_GGX) return ggxG1(m, n, h, i) * ggxG1(m, n, h, o);
         R result = G1(m, n, h, i) * G1(m, n, h, o);
         if (result < R(0)) return R(0);
         return result;
@@ -795,7 +835,7 @@ template <class R> struct Tracer {
     R sampleBSDF(const rtx_material &m, V3<R> wi, V3<R> n, bool frontFace, V3<R> &newDir) {
         switch (m.type) {
         case RTX_MAT_ROUGH_CONDUCTOR: {
-            V3<R> worldM = randomBeckmannNormal(m, n);
+            V3<R> worldM = randomMicrofacetNormal(m, n);
             newDir = reflect(-wi, worldM);
             return pdfBSDF(m, n, wi, newDir);
         }
@@ -2254,6 +2294,15 @@ double orc_fresnel_conductor(double c, double eta, double k) { return Tracer<dou
 double orc_beckmann_d(const rtx_material *m, const double n[3], const double h[3]) {
     return Tracer<double>::D(*m, vd<double>(n), vd<double>(h));
 }
+// D and G1 of the material's microfacet distribution (Beckmann with the
+// reference's G1, or GGX)
+double orc_microfacet_d(const rtx_material *m, const double n[3], const double h[3]) {
+    return Tracer<double>::D(*m, vd<double>(n), vd<double>(h));
+}
+double orc_microfacet_g1(const rtx_material *m, const double n[3], const double h[3], const double v[3]) {
+    if (m->distribution == RTX_DISTRIBUTION_GGX) return Tracer<double>::ggxG1(*m, vd<double>(n), vd<double>(h), vd<double>(v));
+    return Tracer<double>::G1(*m, vd<double>(n), vd<double>(h), vd<double>(v));
+}
 double orc_pdf_bsdf(const rtx_material *m, const double n[3], const double wi[3], const double wo[3]) {
     return Tracer<double>::pdfBSDF(*m, vd<double>(n), vd<double>(wi), vd<double>(wo));
 }
@@ -2307,7 +2356,7 @@ void orc_sample_beckmann_normal(const rtx_material *m, const double n[3], uint32
     rtx_frame_params fp = defaultParams();
     Tracer<double> t(emptyScene(), fp);
     t.seed = *state;
-    toD(t.randomBeckmannNormal(*m, vd<double>(n)), out);
+    toD(t.randomMicrofacetNormal(*m, vd<double>(n)), out);
     *state = t.seed;
 }
 void orc_sample_unit_sphere(uint32_t *state, double out[3]) {

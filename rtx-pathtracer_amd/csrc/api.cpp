@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "abi_guard.h"
 #include "dev_math.h"
 #include "internal.h"
 
@@ -373,6 +374,37 @@ EventPair *nextEvents(rtx_ctx *ctx, int cls) {
 } // namespace
 
 namespace rtx {
+int abiStatus() noexcept {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return RTX_E_MEMORY;
+    } catch (...) {
+        return RTX_E_DEVICE;
+    }
+}
+int abiFail(rtx_ctx *ctx) noexcept {
+    const int code = abiStatus();
+    if (ctx) {
+        try {
+            throw;
+        } catch (const std::exception &e) {
+            try {
+                ctx->err = std::string("internal error: ") + e.what();
+            } catch (...) {
+            }
+        } catch (...) {
+            try {
+                ctx->err = "internal error";
+            } catch (...) {
+            }
+        }
+    }
+    return code;
+}
+} // namespace rtx
+
+namespace rtx {
 std::vector<GuideNode> buildGuideNodes(const rtx_aabb *boxes, uint32_t n) {
     std::vector<GuideNode> out;
     if (n == 0) return out;
@@ -418,30 +450,34 @@ void rtx_push_constant_defaults(rtx_push_constant *pc) {
 
 const char *rtx_last_error(const rtx_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
-int rtx_ctx_set_tuning(rtx_ctx *ctx, const rtx_tuning *t) {
+int rtx_ctx_set_tuning(rtx_ctx *ctx, const rtx_tuning *t) try {
     if (!ctx || !t) return RTX_E_INVALID;
     rtx_tuning v = ctx->tuning;
     if (t->traversalKernel) v.traversalKernel = t->traversalKernel;
     if (t->traceBlocksPerCU) v.traceBlocksPerCU = t->traceBlocksPerCU;
     if (t->shadeBlocksPerCU) v.shadeBlocksPerCU = t->shadeBlocksPerCU;
     if (t->batchPaths) v.batchPaths = t->batchPaths;
-    if (v.traversalKernel < 1 || v.traversalKernel > 2)
-        return fail(ctx, RTX_E_INVALID, "rtx_tuning.traversalKernel must be 1 or 2");
+    if (v.traversalKernel != 1)
+        return fail(ctx, RTX_E_INVALID, "rtx_tuning.traversalKernel must be 1 (persistent lane-refill traversal)");
     if (v.traceBlocksPerCU < 1 || v.traceBlocksPerCU > 64 || v.shadeBlocksPerCU < 1 || v.shadeBlocksPerCU > 64)
         return fail(ctx, RTX_E_INVALID, "rtx_tuning blocks per CU must lie in [1, 64]");
     if (v.batchPaths < 1024 || v.batchPaths > (1u << 26))
         return fail(ctx, RTX_E_INVALID, "rtx_tuning.batchPaths must lie in [1024, 2^26]");
     ctx->tuning = v;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_ctx_get_tuning(const rtx_ctx *ctx, rtx_tuning *t) {
+int rtx_ctx_get_tuning(const rtx_ctx *ctx, rtx_tuning *t) try {
     if (!ctx || !t) return RTX_E_INVALID;
     *t = ctx->tuning;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(nullptr);
 }
 
-int rtx_ctx_create(int device, rtx_ctx **out) {
+int rtx_ctx_create(int device, rtx_ctx **out) try {
     if (!out) return RTX_E_INVALID;
     *out = nullptr;
     int n = 0;
@@ -468,9 +504,11 @@ int rtx_ctx_create(int device, rtx_ctx **out) {
         ctx->cuCount = prop.multiProcessorCount;
     *out = ctx;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(nullptr);
 }
 
-int rtx_ctx_destroy(rtx_ctx *ctx) {
+int rtx_ctx_destroy(rtx_ctx *ctx) try {
     if (!ctx) return RTX_E_INVALID;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
@@ -506,9 +544,11 @@ int rtx_ctx_destroy(rtx_ctx *ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
+int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) try {
     if (!ctx || !out) return RTX_E_INVALID;
     *out = nullptr;
     uint64_t numTris = 0;
@@ -698,9 +738,11 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) {
         }
     *out = sc;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_scene_destroy(rtx_ctx *ctx, rtx_scene *scene) {
+int rtx_scene_destroy(rtx_ctx *ctx, rtx_scene *scene) try {
     if (!scene) return RTX_E_INVALID;
     if (ctx) {
         (void)hipSetDevice(ctx->device);
@@ -709,17 +751,21 @@ int rtx_scene_destroy(rtx_ctx *ctx, rtx_scene *scene) {
     freeAll(scene->allocs);
     delete scene;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_scene_get_info(const rtx_scene *scene, rtx_scene_info *info) {
+int rtx_scene_get_info(const rtx_scene *scene, rtx_scene_info *info) try {
     if (!scene || !info) return RTX_E_INVALID;
     *info = scene->info;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(nullptr);
 }
 
 static int guideUpdateAsync(rtx_ctx *ctx);
 
-int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, rtx_frame_stats *stats) {
+int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, rtx_frame_stats *stats) try {
     if (!ctx) return RTX_E_INVALID;
     if (!scene || !p) return fail(ctx, RTX_E_INVALID, "NULL scene or params");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return fail(ctx, RTX_E_INVALID, "empty render");
@@ -944,7 +990,6 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     ctx->eventsUsed = 0;
     const int traceBlocks = ctx->cuCount * ctx->tuning.traceBlocksPerCU;
     const int shadeBlocks = ctx->cuCount * ctx->tuning.shadeBlocksPerCU;
-    const bool persistent = ctx->tuning.traversalKernel == 1;
     const int maxIter = p->pc.maxDepth + p->pc.maxFollowDiscrete + 3;
     const int maxIterIC = 1 + 10 + 3;
     const DevScene &S = scene->view;
@@ -1001,17 +1046,10 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             uint32_t *poll = ctx->pollRing + 4 * (it % kPollSlots);
             itersRun = std::max(itersRun, it + 1);
             if (it == 0) timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
-            if (persistent) {
-                timed(kClsExtend, [&] {
-                    launchTraceP(s, S, ctx->q[qsel], qsel, ctx->qs, ctx->qp, ctx->pb, ctx->cnt, poll, countVisits,
-                                 traceBlocks);
-                });
-            } else {
-                timed(kClsExtend,
-                      [&] { launchExtend(s, S, ctx->q[qsel], ctx->cnt, qsel, poll, countVisits, traceBlocks); });
-                timed(kClsShadow, [&] { launchShadow(s, S, ctx->qs, ctx->pb, ctx->cnt, countVisits, traceBlocks); });
-                timed(kClsShadow, [&] { launchProbe(s, S, ctx->qp, ctx->cnt, countVisits, traceBlocks); });
-            }
+            timed(kClsExtend, [&] {
+                launchTraceP(s, S, ctx->q[qsel], qsel, ctx->qs, ctx->qp, ctx->pb, ctx->cnt, poll, countVisits,
+                             traceBlocks);
+            });
             CHECK_HIP(ctx, hipEventRecord(ctx->pollEv[it % kPollSlots], s));
             timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, qsel, shadeBlocks); });
             timed(kClsShade, [&] {
@@ -1147,6 +1185,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         stats->primBytes = 48;
     }
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 // ---------------------------------------------------------- path guiding
@@ -1181,7 +1221,7 @@ static int guideUpload(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_thet
     return RTX_OK;
 }
 
-int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) {
+int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) try {
     if (!ctx) return RTX_E_INVALID;
     if (n && (!regions || !vmms)) return fail(ctx, RTX_E_INVALID, "NULL argument");
     if (n > (1u << 20)) return fail(ctx, RTX_E_INVALID, "more than 2^20 guiding regions");
@@ -1197,9 +1237,11 @@ int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *
     ctx->fit = GuideFitBuffers{};
     ctx->fitValid = false;
     return guideUpload(ctx, regions, vmms, n);
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32_t capacity, uint32_t *n) {
+int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32_t capacity, uint32_t *n) try {
     if (!ctx) return RTX_E_INVALID;
     const uint32_t count = (uint32_t)ctx->guideBoxes.size();
     if (n) *n = count;
@@ -1214,13 +1256,15 @@ int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32
     if (regions) std::copy(ctx->guideBoxes.begin(), ctx->guideBoxes.begin() + k, regions);
     if (vmms) std::copy(ctx->guideVmms.begin(), ctx->guideVmms.begin() + k, vmms);
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 // PathGuiding's constructor (src/PathGuiding.cpp:11-31, createRegions 82-106):
 // the scene box grown by Aabb::addEpsilon, split splitCount times along the
 // largest axis of every box; each region starts with lightpmm's initial
 // mixture (8 lobes, kappa 5, spherical-Fibonacci directions).
-int rtx_guiding_init(rtx_ctx *ctx, const rtx_aabb *sceneBox, uint32_t splitCount, int parallax) {
+int rtx_guiding_init(rtx_ctx *ctx, const rtx_aabb *sceneBox, uint32_t splitCount, int parallax) try {
     if (!ctx) return RTX_E_INVALID;
     if (!sceneBox) return fail(ctx, RTX_E_INVALID, "NULL scene box");
     if (splitCount > 16) return fail(ctx, RTX_E_INVALID, "splitCount must be at most 16");
@@ -1271,6 +1315,8 @@ int rtx_guiding_init(rtx_ctx *ctx, const rtx_aabb *sceneBox, uint32_t splitCount
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->guideVmmsStale = true;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 static int guideFitWork(rtx_ctx *ctx, size_t slots) {
@@ -1370,7 +1416,7 @@ static int guideUpdateAsync(rtx_ctx *ctx) {
     return RTX_OK;
 }
 
-int rtx_guiding_update(rtx_ctx *ctx) {
+int rtx_guiding_update(rtx_ctx *ctx) try {
     if (!ctx) return RTX_E_INVALID;
     if (!ctx->fitValid) return fail(ctx, RTX_E_STATE, "no guiding state to update (rtx_guiding_init)");
     if (ctx->dirCount == 0) return fail(ctx, RTX_E_STATE, "no render with updateGuiding yet");
@@ -1379,25 +1425,31 @@ int rtx_guiding_update(rtx_ctx *ctx) {
     if (rc != RTX_OK) return rc;
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_guiding_set_options(rtx_ctx *ctx, const rtx_guiding_options *options) {
+int rtx_guiding_set_options(rtx_ctx *ctx, const rtx_guiding_options *options) try {
     if (!ctx) return RTX_E_INVALID;
     if (!options) return fail(ctx, RTX_E_INVALID, "NULL options");
     if (!(options->splitMinDivergence >= 0.0f) || !(options->mergeMaxDivergence >= 0.0f))
         return fail(ctx, RTX_E_INVALID, "divergence thresholds must be >= 0");
     ctx->guideOpts = *options;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_guiding_get_options(rtx_ctx *ctx, rtx_guiding_options *options) {
+int rtx_guiding_get_options(rtx_ctx *ctx, rtx_guiding_options *options) try {
     if (!ctx) return RTX_E_INVALID;
     if (!options) return fail(ctx, RTX_E_INVALID, "NULL options");
     *options = ctx->guideOpts;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_guiding_read_samples(rtx_ctx *ctx, rtx_directional_data *out, size_t count) {
+int rtx_guiding_read_samples(rtx_ctx *ctx, rtx_directional_data *out, size_t count) try {
     if (!ctx) return RTX_E_INVALID;
     if (ctx->dirCount == 0) return fail(ctx, RTX_E_STATE, "no render with updateGuiding yet");
     if (count > ctx->dirCount) return fail(ctx, RTX_E_INVALID, "more samples asked for than recorded");
@@ -1407,6 +1459,8 @@ int rtx_guiding_read_samples(rtx_ctx *ctx, rtx_directional_data *out, size_t cou
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     CHECK_HIP(ctx, hipMemcpy(out, ctx->dirData, count * sizeof(rtx_directional_data), hipMemcpyDeviceToHost));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 // ------------------------------------------------------- irradiance cache
@@ -1416,7 +1470,7 @@ static void icFree(rtx_ctx *ctx) {
     ctx->icValid = false;
 }
 
-int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) {
+int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) try {
     if (!ctx) return RTX_E_INVALID;
     if (maxEntries == 0 || maxEntries > (1u << 24))
         return fail(ctx, RTX_E_INVALID, "maxEntries must be in [1, 2^24]");
@@ -1457,9 +1511,11 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) {
         return fail(ctx, RTX_E_DEVICE, std::string("irradiance cache allocation: ") + hipGetErrorString(e));
     }
     return rtx_ic_reset(ctx);
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_ic_reset(rtx_ctx *ctx) {
+int rtx_ic_reset(rtx_ctx *ctx) try {
     if (!ctx) return RTX_E_INVALID;
     if (ctx->ic.maxEntries == 0) return fail(ctx, RTX_E_STATE, "no irradiance cache (rtx_ic_create)");
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
@@ -1469,9 +1525,11 @@ int rtx_ic_reset(rtx_ctx *ctx) {
     ctx->icValid = true;
     ctx->keptJobs = ctx->keptUpdates = 0;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_estimate_write(rtx_ctx *ctx, const float *rgba, uint32_t width, uint32_t height) {
+int rtx_estimate_write(rtx_ctx *ctx, const float *rgba, uint32_t width, uint32_t height) try {
     if (!ctx) return RTX_E_INVALID;
     if (!rgba || width == 0 || height == 0) return fail(ctx, RTX_E_INVALID, "NULL or empty estimate");
     const size_t npix = (size_t)width * height;
@@ -1487,9 +1545,11 @@ int rtx_estimate_write(rtx_ctx *ctx, const float *rgba, uint32_t width, uint32_t
     }
     CHECK_HIP(ctx, hipMemcpy(ctx->estimate, rgba, npix * sizeof(float4), hipMemcpyHostToDevice));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_estimate_read(rtx_ctx *ctx, float *rgba, uint32_t width, uint32_t height) {
+int rtx_estimate_read(rtx_ctx *ctx, float *rgba, uint32_t width, uint32_t height) try {
     if (!ctx) return RTX_E_INVALID;
     if (!rgba) return fail(ctx, RTX_E_INVALID, "NULL argument");
     if (!ctx->estimate || ctx->estW != width || ctx->estH != height)
@@ -1498,10 +1558,12 @@ int rtx_estimate_read(rtx_ctx *ctx, float *rgba, uint32_t width, uint32_t height
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     CHECK_HIP(ctx, hipMemcpy(rgba, ctx->estimate, (size_t)width * height * sizeof(float4), hipMemcpyDeviceToHost));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 int rtx_ic_read(rtx_ctx *ctx, rtx_cache_header *hdr, rtx_cache_data *entries, rtx_sphere *spheres,
-                uint32_t capacity) {
+                uint32_t capacity) try {
     if (!ctx) return RTX_E_INVALID;
     if (!ctx->icValid) return fail(ctx, RTX_E_STATE, "no irradiance cache (rtx_ic_create)");
     const uint32_t n = std::min(capacity, ctx->ic.maxEntries);
@@ -1513,10 +1575,12 @@ int rtx_ic_read(rtx_ctx *ctx, rtx_cache_header *hdr, rtx_cache_data *entries, rt
     if (spheres && n) CHECK_HIP(ctx, hipMemcpy(spheres, ctx->ic.spheres, n * sizeof(rtx_sphere),
                                                hipMemcpyDeviceToHost));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 int rtx_ic_write(rtx_ctx *ctx, const rtx_cache_header *hdr, const rtx_cache_data *entries,
-                 const rtx_sphere *spheres) {
+                 const rtx_sphere *spheres) try {
     if (!ctx) return RTX_E_INVALID;
     if (!ctx->icValid) return fail(ctx, RTX_E_STATE, "no irradiance cache (rtx_ic_create)");
     if (!hdr || !entries || !spheres) return fail(ctx, RTX_E_INVALID, "NULL argument");
@@ -1529,10 +1593,12 @@ int rtx_ic_write(rtx_ctx *ctx, const rtx_cache_header *hdr, const rtx_cache_data
     CHECK_HIP(ctx, hipMemcpy(ctx->ic.entries, entries, n * sizeof(rtx_cache_data), hipMemcpyHostToDevice));
     CHECK_HIP(ctx, hipMemcpy(ctx->ic.spheres, spheres, n * sizeof(rtx_sphere), hipMemcpyHostToDevice));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 int rtx_ic_read_jobs(rtx_ctx *ctx, rtx_ic_job *jobs, float *sub, uint32_t *subSig, uint32_t capacity,
-                     uint32_t *numJobs) {
+                     uint32_t *numJobs) try {
     if (!ctx) return RTX_E_INVALID;
     if (!ctx->icValid) return fail(ctx, RTX_E_STATE, "no irradiance cache (rtx_ic_create)");
     if (numJobs) *numJobs = ctx->keptJobs;
@@ -1563,9 +1629,11 @@ int rtx_ic_read_jobs(rtx_ctx *ctx, rtx_ic_job *jobs, float *sub, uint32_t *subSi
     if (sub) CHECK_HIP(ctx, hipMemcpy(sub, ctx->keepSub, ns * sizeof(float4), hipMemcpyDeviceToHost));
     if (subSig) CHECK_HIP(ctx, hipMemcpy(subSig, ctx->keepSubSig, ns * sizeof(uint2), hipMemcpyDeviceToHost));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_read_image(rtx_ctx *ctx, float *rgba, size_t floatCount) {
+int rtx_read_image(rtx_ctx *ctx, float *rgba, size_t floatCount) try {
     if (!ctx) return RTX_E_INVALID;
     if (!rgba) return fail(ctx, RTX_E_INVALID, "rgba is NULL");
     if (!ctx->accum || ctx->accSamples == 0) return fail(ctx, RTX_E_STATE, "nothing rendered yet");
@@ -1577,15 +1645,19 @@ int rtx_read_image(rtx_ctx *ctx, float *rgba, size_t floatCount) {
     CHECK_HIP(ctx, hipMemcpyAsync(rgba, ctx->resolved, npix * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_accumulated_samples(const rtx_ctx *ctx, uint32_t *samples) {
+int rtx_accumulated_samples(const rtx_ctx *ctx, uint32_t *samples) try {
     if (!ctx || !samples) return RTX_E_INVALID;
     *samples = ctx->accSamples;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(nullptr);
 }
 
-int rtx_read_paths(rtx_ctx *ctx, float *rgb, uint32_t *sig, size_t n) {
+int rtx_read_paths(rtx_ctx *ctx, float *rgb, uint32_t *sig, size_t n) try {
     if (!ctx) return RTX_E_INVALID;
     if (!ctx->keepRgb || ctx->keptPaths == 0) return fail(ctx, RTX_E_STATE, "no render with RTX_FRAME_KEEP_PATHS");
     if (n > ctx->keptPaths) return fail(ctx, RTX_E_INVALID, "more paths requested than rendered");
@@ -1601,6 +1673,8 @@ int rtx_read_paths(rtx_ctx *ctx, float *rgb, uint32_t *sig, size_t n) {
     }
     if (sig) CHECK_HIP(ctx, hipMemcpy(sig, ctx->keepSig, n * sizeof(uint2), hipMemcpyDeviceToHost));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 static int queryCommon(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, void *out,
@@ -1620,11 +1694,8 @@ static int queryCommon(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays
     if (e == hipSuccess) e = hipMemsetAsync(fetch, 0, sizeof(uint32_t), ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(rD, rays, n * sizeof(rtx_ray), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
-        if (ctx->tuning.traversalKernel == 1)
-            launchQueryP(ctx->stream, scene->view, rD, (uint32_t)n, oD, closest, fetch,
-                         ctx->cuCount * ctx->tuning.traceBlocksPerCU);
-        else if (closest) launchQueryClosest(ctx->stream, scene->view, rD, n, (rtx_hit *)oD);
-        else launchQueryOccluded(ctx->stream, scene->view, rD, n, (uint8_t *)oD);
+        launchQueryP(ctx->stream, scene->view, rD, (uint32_t)n, oD, closest, fetch,
+                     ctx->cuCount * ctx->tuning.traceBlocksPerCU);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, oD, n * outBytes, hipMemcpyDeviceToHost, ctx->stream);
@@ -1634,12 +1705,16 @@ static int queryCommon(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays
     return RTX_OK;
 }
 
-int rtx_trace_closest(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, rtx_hit *hits) {
+int rtx_trace_closest(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, rtx_hit *hits) try {
     return queryCommon(ctx, scene, rays, n, hits, sizeof(rtx_hit), true);
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_trace_occluded(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, uint8_t *occluded) {
+int rtx_trace_occluded(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, uint8_t *occluded) try {
     return queryCommon(ctx, scene, rays, n, occluded, 1, false);
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 } // extern "C"

@@ -130,6 +130,41 @@ RTX_DI float Gterm(float alpha, V3 i, V3 o, V3 n, V3 m) { // shaders/raytrace.rg
     return r < 0.0f ? 0.0f : r;
 }
 
+// GGX (Walter, Marschner, Li and Torrance 2007), the rough conductor's
+// distribution when rtx_material.distribution is RTX_DISTRIBUTION_GGX (new;
+// the reference has Beckmann only).  D(m) = a^2 / (pi cos^4 (a^2 + tan^2)^2)
+// = a^2 / (pi (a^2 cos^2 + sin^2)^2), with sin^2 = |n x m|^2 (no cancellation
+// at small a near the pole).
+RTX_DI float ggxD(float alpha, V3 n, V3 m) {
+    float c = dot(n, m);
+    if (c <= 0.0f) return 0.0f;
+    V3 x = cross(n, m);
+    float a2 = alpha * alpha;
+    float den = a2 * c * c + dot(x, x);
+    return a2 / (kPi * den * den);
+}
+// Smith G1 of GGX: chi+(v.m / v.n) 2 / (1 + sqrt(1 + a^2 tan^2 theta_v)).
+RTX_DI float ggxG1(float alpha, V3 n, V3 m, V3 v) {
+    float cv = dot(v, n);
+    if (dot(v, m) / cv <= 0.0f) return 0.0f;
+    V3 x = cross(v, n);
+    float tan2 = dot(x, x) / (cv * cv);
+    return 2.0f / (1.0f + sqrtf(1.0f + alpha * alpha * tan2));
+}
+// Microfacet normal with density D(m) cos(theta_m): tan(theta_m) =
+// a sqrt(u1 / (1 - u1)), phi = 2 pi u2 (the same two draws as the Beckmann
+// sampler, in the same order).
+RTX_DI V3 randomGGXNormal(Rng &r, float alpha, V3 n) {
+    float u1 = r.next();
+    float t = alpha * sqrtf(u1 / (1.0f - u1));
+    float phiM = 2.0f * kPi * r.next();
+    float ct = 1.0f / sqrtf(1.0f + t * t);
+    float st = t * ct;
+    float sp, cp;
+    sincosf(phiM, &sp, &cp);
+    return toWorld(v3(st * cp, st * sp, ct), n);
+}
+
 // Material fields the shading reads, gathered once per vertex.  kd and ks
 // are the diffuse and specular colours with the vertex's textures applied
 // (diffuse() and phong(), raytrace.rgen:232-267); `diffuse` and `specular`
@@ -138,6 +173,7 @@ struct Mat {
     V3 lightColor, diffuse, specular, kd, ks;
     float highlight, ior, iorInv, eta, k, roughness;
     int type, texD, texS;
+    bool ggx; // rough conductor with the GGX distribution
 };
 RTX_DI Mat loadMat(const rtx_material *m) {
     const float4 *p = reinterpret_cast<const float4 *>(m);
@@ -157,6 +193,7 @@ RTX_DI Mat loadMat(const rtx_material *m) {
     r.texD = __float_as_int(e.z);
     r.texS = __float_as_int(e.w);
     r.type = __float_as_int(f.x);
+    r.ggx = __float_as_int(f.y) == RTX_DISTRIBUTION_GGX;
     r.kd = r.diffuse;
     r.ks = r.specular;
     return r;
@@ -198,14 +235,15 @@ RTX_DI V3 roughConductor(const Mat &m, V3 n, V3 wi, V3 wo) { // shaders/raytrace
     float cosThetaI = dot(wi, n);
     float cosThetaO = dot(wo, n);
     if (cosThetaI <= 0.0f) return v3(0.0f);
-    float val = fresnelConductor(cosThetaIHr, m.eta, m.k) * Gterm(m.roughness, wi, wo, n, hr) *
-                beckmannD(m.roughness, n, hr) / (4.0f * cosThetaI * cosThetaO);
+    float GD = m.ggx ? ggxG1(m.roughness, n, hr, wi) * ggxG1(m.roughness, n, hr, wo) * ggxD(m.roughness, n, hr)
+                     : Gterm(m.roughness, wi, wo, n, hr) * beckmannD(m.roughness, n, hr);
+    float val = fresnelConductor(cosThetaIHr, m.eta, m.k) * GD / (4.0f * cosThetaI * cosThetaO);
     return v3(cosThetaO * val);
 }
 RTX_DI float pdfBSDF(const Mat &m, V3 n, V3 wi, V3 wo) { // shaders/raytrace.rgen:284-332
     if (m.type == RTX_MAT_ROUGH_CONDUCTOR) {
         V3 hr = normalize(wi + wo);
-        float pm = beckmannD(m.roughness, n, hr) * fabsf(dot(hr, n));
+        float pm = (m.ggx ? ggxD(m.roughness, n, hr) : beckmannD(m.roughness, n, hr)) * fabsf(dot(hr, n));
         if (pm <= 0.0f || dot(wo, hr) <= 0.0f) return 0.0f;
         return pm / (4.0f * fabsf(dot(wo, hr)));
     }
@@ -228,7 +266,7 @@ RTX_DI float pdfBSDF(const Mat &m, V3 n, V3 wi, V3 wo) { // shaders/raytrace.rge
 RTX_DI float sampleBSDF(Rng &r, const Mat &m, V3 wi, V3 n, bool frontFace, V3 &newDir) { // raytrace.rgen:483-552
     switch (m.type) {
     case RTX_MAT_ROUGH_CONDUCTOR: {
-        V3 worldM = randomBeckmannNormal(r, m.roughness, n);
+        V3 worldM = m.ggx ? randomGGXNormal(r, m.roughness, n) : randomBeckmannNormal(r, m.roughness, n);
         newDir = reflect(-wi, worldM);
         return pdfBSDF(m, n, wi, newDir);
     }

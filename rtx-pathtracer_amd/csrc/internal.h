@@ -270,15 +270,13 @@ constexpr uint32_t kSplitVertex = 1u << 31;
 
 void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue q0,
                     Counters *cnt, int qsel, uint32_t *electCounter);
+// persistent lane-refill traversal of one bounce: extend queue qe (count
+// extend[esel]), then the probe and shadow queues; fetch[1] must be zero.
 // poll (may be NULL): host-visible slot for the bounce's starting queue
 // lengths (extend, probe, shadow, done), written by the first thread
-void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, uint32_t *poll, bool countVisits,
-                  int blocks);
-// persistent lane-refill traversal of one bounce: extend queue qe (count
-// extend[esel]), then the probe and shadow queues; fetch[1] must be zero
 void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
                   Counters *cnt, uint32_t *poll, bool countVisits, int blocks);
-// persistent variant of the two queries; *fetch must be zero
+// the two ray queries (closest hit / occlusion); *fetch must be zero
 void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_t n, void *out, bool closest,
                   uint32_t *fetch, int blocks);
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
@@ -299,9 +297,6 @@ void launchIcIndex(hipStream_t s, IcBuffers ic, float c0);
 // Device temp bytes the index build's radix sort needs for `items` slots.
 size_t icSortTempBytes(uint32_t items, uint32_t bucketBits);
 void launchIcInit(hipStream_t s, IcBuffers ic);
-void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
-                  int blocks);
-void launchProbe(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, bool countVisits, int blocks);
 // Ends the segments of the previous bounce's done list (done[qsel ^ 1]);
 // with FrameConst::segments a path with camera samples left or splits queued
 // goes on in qnext (count extend[qsel ^ 1]) instead.
@@ -312,8 +307,6 @@ void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers 
 void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,
                       bool keep, float4 *estimate);
 void launchResetCounters(hipStream_t s, Counters *cnt, int which);
-void launchQueryClosest(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, rtx_hit *hits);
-void launchQueryOccluded(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, uint8_t *occ);
 void launchResolveImage(hipStream_t s, const float4 *accum, uint32_t npix, float invSamples, float4 *out);
 
 } // namespace rtx

@@ -190,47 +190,6 @@ RTX_DI void reportBounce(uint32_t *poll, uint32_t ne, uint32_t np, uint32_t ns, 
     poll[3] = nd;
 }
 
-// --------------------------------------------------------------- extend
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kExtend(DevScene S, Queue q, Counters *cnt, int qsel, uint32_t *poll) {
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
-    const uint32_t n = cnt->extend[qsel];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(&cnt->extendRays, (unsigned long long)n);
-        reportBounce(poll, n, cnt->probe, cnt->shadow, cnt->done[qsel ^ 1]);
-    }
-    HybridStack<kTraceBlock, kLdsStack> st;
-    st.lds = stackMem + threadIdx.x;
-    VisitCount vc;
-    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
-        float4 o = q.org[i], d = q.dir[i];
-        TravRay r = makeTravRay(v3(o), v3(d));
-        float tmax = d.w, bu = 0.0f, bv = 0.0f;
-        int prim = traverse<false, COUNT>(S, r, o.w, tmax, bu, bv, st, vc, raySeed(S, q, i));
-        q.hit[i] = make_float4(tmax, __int_as_float(prim), bu, bv);
-    }
-    flushVisits<COUNT>(vc, cnt, 0);
-}
-
-// --------------------------------------------------------------- shadow
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kShadow(DevScene S, Queue q, PathBuffers pb, Counters *cnt) {
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
-    const uint32_t n = cnt->shadow;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->shadowRays, (unsigned long long)n);
-    HybridStack<kTraceBlock, kLdsStack> st;
-    st.lds = stackMem + threadIdx.x;
-    VisitCount vc;
-    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
-        float4 o = q.org[i], d = q.dir[i];
-        TravRay r = makeTravRay(v3(o), v3(d));
-        float tmax = d.w, bu, bv;
-        int prim = traverse<true, COUNT>(S, r, o.w, tmax, bu, bv, st, vc, raySeed(S, q, i));
-        if (prim < 0) pb.pendShadow[q.id[i]].w = 1.0f; // unoccluded: raytrace.shadow.rmiss:7-10
-    }
-    flushVisits<COUNT>(vc, cnt, 1);
-}
-
 // MIS probe result (raytrace.rgen:684-726): on a miss the environment colour
 // (black without a map) with pdf 1/(2 pi lightCount); on a light the
 // light-side pdf (pdfLight, raytrace.rgen:334-339) and the BSDF-side weight.
@@ -254,28 +213,6 @@ RTX_DI V3 probeContribution(const DevScene &S, bool power, V3 o, V3 d, int prim,
     float h = misWeight(power, pdfMat, pdfL);
     if (isnan(h)) return v3(0.0f);
     return v3(base) * v3(ms.lightColor) * h;
-}
-
-// ---------------------------------------------------------------- probe
-// The MIS probe rays of nextEventEstimation (raytrace.rgen:665-729), grid-
-// stride traversal: closest hits into q.hit; kProbeResolve adds the
-// BSDF-side MIS weight against the light hit.
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kProbe(DevScene S, Queue q, Counters *cnt) {
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
-    const uint32_t n = cnt->probe;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt->probeRays, (unsigned long long)n);
-    HybridStack<kTraceBlock, kLdsStack> st;
-    st.lds = stackMem + threadIdx.x;
-    VisitCount vc;
-    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += gridDim.x * kTraceBlock) {
-        float4 o4 = q.org[i], d4 = q.dir[i];
-        TravRay r = makeTravRay(v3(o4), v3(d4));
-        float tmax = d4.w, bu = 0.0f, bv = 0.0f;
-        int prim = traverse<false, COUNT>(S, r, o4.w, tmax, bu, bv, st, vc, raySeed(S, q, i));
-        q.hit[i] = make_float4(tmax, __int_as_float(prim), bu, bv);
-    }
-    flushVisits<COUNT>(vc, cnt, 2);
 }
 
 // --------------------------------------------------- persistent traversal
@@ -964,41 +901,8 @@ __global__ __launch_bounds__(256) void kResolveImage(const float4 *accum, uint32
 }
 
 // ---------------------------------------------------------- ray queries
-__global__ __launch_bounds__(kTraceBlock) void kQueryClosest(DevScene S, const rtx_ray *rays, size_t n, rtx_hit *hits) {
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
-    HybridStack<kTraceBlock, kLdsStack> st;
-    st.lds = stackMem + threadIdx.x;
-    VisitCount vc;
-    size_t i = (size_t)blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= n) return;
-    rtx_ray ry = rays[i];
-    TravRay r = makeTravRay(v3(ry.org), v3(ry.dir));
-    float tmax = ry.tmax, bu = 0.0f, bv = 0.0f;
-    int prim = traverse<false, false>(S, r, ry.tmin, tmax, bu, bv, st, vc);
-    rtx_hit h;
-    h.t = prim >= 0 ? tmax : ry.tmax;
-    h.prim = prim;
-    h.u = bu;
-    h.v = bv;
-    hits[i] = h;
-}
-
-__global__ __launch_bounds__(kTraceBlock) void kQueryOccluded(DevScene S, const rtx_ray *rays, size_t n,
-                                                              uint8_t *occ) {
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
-    HybridStack<kTraceBlock, kLdsStack> st;
-    st.lds = stackMem + threadIdx.x;
-    VisitCount vc;
-    size_t i = (size_t)blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= n) return;
-    rtx_ray ry = rays[i];
-    TravRay r = makeTravRay(v3(ry.org), v3(ry.dir));
-    float tmax = ry.tmax, bu, bv;
-    occ[i] = traverse<true, false>(S, r, ry.tmin, tmax, bu, bv, st, vc) >= 0 ? 1 : 0;
-}
-
-// Persistent lane-refill variants of the two queries (the traversal the
-// wavefront uses by default).
+// The two ray queries (closest hit, occlusion) with the persistent lane-refill
+// traversal of the wavefront.
 template <bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void kQueryP(DevScene S, const rtx_ray *rays, uint32_t n, void *out,
                                                       uint32_t *fetch) {
@@ -1043,11 +947,6 @@ void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t 
     hipLaunchKernelGGL(kGenerateIC, dim3(gridFor(numPaths, 256)), dim3(256), 0, s, ic, frameSeed, firstJob, numPaths,
                        pb, q0, cnt, qsel);
 }
-void launchExtend(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, int qsel, uint32_t *poll, bool countVisits,
-                  int blocks) {
-    if (countVisits) hipLaunchKernelGGL(kExtend<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel, poll);
-    else hipLaunchKernelGGL(kExtend<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt, qsel, poll);
-}
 void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
                   Counters *cnt, uint32_t *poll, bool countVisits, int blocks) {
     const dim3 g(blocks), t(kTraceBlock);
@@ -1069,15 +968,6 @@ void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const DevGui
     hipLaunchKernelGGL(kShade, dim3(blocks), dim3(kShadeBlock), 0, s, S, ic, g, fc, b, pb, qin, qout, qs, qp, done,
                        cnt, qsel);
 }
-void launchShadow(hipStream_t s, const DevScene &S, Queue q, PathBuffers pb, Counters *cnt, bool countVisits,
-                  int blocks) {
-    if (countVisits) hipLaunchKernelGGL(kShadow<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
-    else hipLaunchKernelGGL(kShadow<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, pb, cnt);
-}
-void launchProbe(hipStream_t s, const DevScene &S, Queue q, Counters *cnt, bool countVisits, int blocks) {
-    if (countVisits) hipLaunchKernelGGL(kProbe<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt);
-    else hipLaunchKernelGGL(kProbe<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, q, cnt);
-}
 void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, const int32_t *done,
                   Counters *cnt, Queue qnext, int qsel, int blocks) {
     hipLaunchKernelGGL(kFinish, dim3(blocks), dim3(256), 0, s, fc, b, pb, done, cnt, qnext, qsel);
@@ -1090,14 +980,6 @@ void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum,
 }
 void launchResetCounters(hipStream_t s, Counters *cnt, int which) {
     hipLaunchKernelGGL(kResetCounters, dim3(1), dim3(64), 0, s, cnt, which);
-}
-void launchQueryClosest(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, rtx_hit *hits) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(kQueryClosest, dim3(gridFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, S, rays, n, hits);
-}
-void launchQueryOccluded(hipStream_t s, const DevScene &S, const rtx_ray *rays, size_t n, uint8_t *occ) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(kQueryOccluded, dim3(gridFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, S, rays, n, occ);
 }
 void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_t n, void *out, bool closest,
                   uint32_t *fetch, int blocks) {

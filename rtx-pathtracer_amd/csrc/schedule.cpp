@@ -6,6 +6,7 @@
 #include <cstring>
 #include <new>
 
+#include "abi_guard.h"
 #include "rtx_amd.h"
 
 using Clock = std::chrono::steady_clock;
@@ -71,7 +72,7 @@ void rtx_schedule_config_defaults(rtx_schedule_config *c) {
     c->sppToOneAfterEval = 0;
 }
 
-int rtx_schedule_create(const rtx_schedule_config *cfg, const rtx_push_constant *pc, rtx_schedule **out) {
+int rtx_schedule_create(const rtx_schedule_config *cfg, const rtx_push_constant *pc, rtx_schedule **out) try {
     if (!cfg || !pc || !out) return RTX_E_INVALID;
     *out = nullptr;
     if (cfg->prepareFrames < 0 || cfg->guidingOptimizations < -1 || cfg->evalSamples < 0 || cfg->evalSeconds < 0)
@@ -86,27 +87,35 @@ int rtx_schedule_create(const rtx_schedule_config *cfg, const rtx_push_constant 
     s->evalXSeconds = cfg->evalSeconds > 0;
     *out = s;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiStatus();
 }
 
-int rtx_schedule_destroy(rtx_schedule *s) {
+int rtx_schedule_destroy(rtx_schedule *s) try {
     delete s;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiStatus();
 }
 
-int rtx_schedule_set_push_constants(rtx_schedule *s, const rtx_push_constant *pc) {
+int rtx_schedule_set_push_constants(rtx_schedule *s, const rtx_push_constant *pc) try {
     if (!s || !pc) return RTX_E_INVALID;
     s->pc = *pc;
     s->hasInputChanged = true;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiStatus();
 }
 
-int rtx_schedule_input_changed(rtx_schedule *s) {
+int rtx_schedule_input_changed(rtx_schedule *s) try {
     if (!s) return RTX_E_INVALID;
     s->hasInputChanged = true;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiStatus();
 }
 
-int rtx_schedule_begin_frame(rtx_schedule *s, const rtx_frame_params *base, rtx_frame_params *frame) {
+int rtx_schedule_begin_frame(rtx_schedule *s, const rtx_frame_params *base, rtx_frame_params *frame) try {
     if (!s || !base || !frame) return RTX_E_INVALID;
     if (s->planned) return RTX_E_STATE;
     // drawFrame: the eval modes switch accumulation on (src/RayTracingApp.cpp:91-100)
@@ -183,9 +192,11 @@ int rtx_schedule_begin_frame(rtx_schedule *s, const rtx_frame_params *base, rtx_
     s->hasInputChanged = false;
     s->planned = true;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiStatus();
 }
 
-int rtx_schedule_end_frame(rtx_schedule *s) {
+int rtx_schedule_end_frame(rtx_schedule *s) try {
     if (!s) return RTX_E_INVALID;
     if (!s->planned) return RTX_E_STATE;
     s->planned = false;
@@ -228,10 +239,12 @@ int rtx_schedule_end_frame(rtx_schedule *s) {
         }
     }
     return RTX_OK;
+} catch (...) {
+    return rtx::abiStatus();
 }
 
 int rtx_schedule_frame(rtx_ctx *ctx, const rtx_scene *scene, rtx_schedule *s, const rtx_frame_params *base,
-                       rtx_frame_stats *stats) {
+                       rtx_frame_stats *stats) try {
     if (!ctx || !scene || !s || !base) return RTX_E_INVALID;
     rtx_frame_params frame;
     const rtx_schedule before = *s;
@@ -243,9 +256,11 @@ int rtx_schedule_frame(rtx_ctx *ctx, const rtx_scene *scene, rtx_schedule *s, co
         return rc;
     }
     return rtx_schedule_end_frame(s);
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
-int rtx_schedule_get_state(const rtx_schedule *s, rtx_schedule_state *st) {
+int rtx_schedule_get_state(const rtx_schedule *s, rtx_schedule_state *st) try {
     if (!s || !st) return RTX_E_INVALID;
     std::memset(st, 0, sizeof(*st));
     st->frames = s->frames;
@@ -261,6 +276,8 @@ int rtx_schedule_get_state(const rtx_schedule *s, rtx_schedule_state *st) {
     st->guidingUpdateMs = s->guidingUpdateMs;
     st->pc = s->pc;
     return RTX_OK;
+} catch (...) {
+    return rtx::abiStatus();
 }
 
 } // extern "C"

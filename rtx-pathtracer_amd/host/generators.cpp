@@ -48,7 +48,7 @@ bool generateCornell(uint64_t /*seed*/, Scene &s, std::string &err) {
 // 12 for the shell, 8 for four ceiling area lights, the rest random
 // boxes (12 triangles) and 24-sided columns (48 side + 2*22 cap triangles)
 // at uniform positions.  Materials per object: 50% Lambert (albedo
-// U(0.1,0.8)), 30% rough conductor (alpha U(0.05,0.5)), 20% dielectric (1.5).
+// U(0.1,0.8)), 30% GGX rough conductor (alpha U(0.05,0.5)), 20% dielectric (1.5).
 bool generateRoom(uint64_t seed, uint32_t triangles, Scene &s, std::string &err) {
     if (triangles == 0) triangles = 262144;
     const uint32_t fixedTris = 12 + 8;
@@ -82,7 +82,7 @@ bool generateRoom(uint64_t seed, uint32_t triangles, Scene &s, std::string &err)
             float alpha = rng.uniform(0.05f, 0.5f);
             float eta = rng.uniform(0.2f, 1.5f);
             float k = rng.uniform(2.0f, 7.0f);
-            return s.addMaterial(makeRoughConductor(alpha, eta, k));
+            return s.addMaterial(makeRoughConductor(alpha, eta, k, RTX_DISTRIBUTION_GGX));
         }
         return s.addMaterial(makeDielectric(1.5f));
     };
@@ -120,7 +120,7 @@ bool generateRoom(uint64_t seed, uint32_t triangles, Scene &s, std::string &err)
 }
 
 // Plates-and-lights test after scenes/veachMIS/veachMIS.xml:11-117: four
-// glossy plates (alpha 0.005, 0.02, 0.05, 0.2) lit by four spherical lights
+// glossy GGX plates (alpha 0.005, 0.02, 0.05, 0.2) lit by four spherical lights
 // (r = 0.03, 0.1, 0.3, 0.9) of equal power, over a diffuse floor.
 bool generatePlates(uint64_t /*seed*/, Scene &s, std::string &err) {
     (void)err;
@@ -143,7 +143,7 @@ bool generatePlates(uint64_t /*seed*/, Scene &s, std::string &err) {
         s.addSphere({cx[i], 0.0f, 0.0f}, radii[i], m);
     }
     for (int i = 0; i < 4; ++i) {
-        int m = s.addMaterial(makeRoughConductor(alphas[i], 0.370f, 2.820f));
+        int m = s.addMaterial(makeRoughConductor(alphas[i], 0.370f, 2.820f, RTX_DISTRIBUTION_GGX));
         Model plate;
         addQuad(plate, {4, py0[i], pz0[i]}, {4, py1[i], pz1[i]}, {-4, py1[i], pz1[i]},
                 {-4, py0[i], pz0[i]}, m);

@@ -301,9 +301,10 @@ rtx_material makeDielectric(float ior) {
     return m;
 }
 
-rtx_material makeRoughConductor(float alpha, float eta, float k) {
+rtx_material makeRoughConductor(float alpha, float eta, float k, int distribution) {
     rtx_material m = blankMaterial();
     m.type = RTX_MAT_ROUGH_CONDUCTOR;
+    m.distribution = distribution;
     m.roughness = alpha;
     m.eta = eta;
     m.k = k;

@@ -103,7 +103,7 @@ struct Scene {
 rtx_material makeDiffuse(Vec3 albedo);
 rtx_material makeLight(Vec3 albedo, Vec3 radiance);
 rtx_material makeDielectric(float ior);
-rtx_material makeRoughConductor(float alpha, float eta, float k);
+rtx_material makeRoughConductor(float alpha, float eta, float k, int distribution = RTX_DISTRIBUTION_BECKMANN);
 rtx_material makePhong(Vec3 kd, Vec3 ks, float exponent);
 
 // Geometry helpers (flat-shaded faces; one vertex set per face).

@@ -237,6 +237,12 @@ rtx_material parseBsdf(const XmlNode *b, const std::map<std::string, int> &textu
         }
     } else if (type == "roughconductor") {
         m.type = RTX_MAT_ROUGH_CONDUCTOR;
+        // Mitsuba's "distribution" (default beckmann); the reference reads no
+        // distribution and always renders Beckmann (src/SceneLoader.cpp:601-605)
+        const XmlNode *dist = namedChild(b, "distribution", "string");
+        const char *dv = dist ? dist->attr("value") : nullptr;
+        if (dv && std::strcmp(dv, "ggx") == 0) m.distribution = RTX_DISTRIBUTION_GGX;
+        else if (dv && std::strcmp(dv, "beckmann") != 0) fail(std::string("unsupported microfacet distribution '") + dv + "'");
         m.roughness = childFloat(b, "alpha");
         m.eta = childSpectrum(b, "eta");
         m.k = childSpectrum(b, "k");

@@ -345,8 +345,10 @@ def dielectric(ior):
     return _material(abi.MAT_DIELECTRIC, specular=(1, 1, 1), refractionIndex=ior, refractionIndexInv=1.0 / ior)
 
 
-def rough_conductor(alpha, eta, k):
-    return _material(abi.MAT_ROUGH_CONDUCTOR, roughness=alpha, eta=eta, k=k)
+def rough_conductor(alpha, eta, k, distribution=abi.DISTRIBUTION_BECKMANN):
+    """A rough conductor; distribution abi.DISTRIBUTION_BECKMANN (the
+    reference's) or abi.DISTRIBUTION_GGX."""
+    return _material(abi.MAT_ROUGH_CONDUCTOR, roughness=alpha, eta=eta, k=k, distribution=distribution)
 
 
 def conductor(eta, k):
@@ -538,7 +540,7 @@ class Renderer:
         return t
 
     def set_tuning(self, **kw):
-        """Launch-shape knobs (traversalKernel, traceBlocksPerCU, shadeBlocksPerCU,
+        """Launch-shape knobs (traceBlocksPerCU, shadeBlocksPerCU,
         batchPaths); fields left out keep their current values."""
         t = abi.Tuning()
         for k, v in kw.items():

@@ -23,6 +23,7 @@ RTX_FRAME_TIME_TRACE = 0x20
 
 MAT_DIFFUSE, MAT_SPECULAR, MAT_DIELECTRIC, MAT_LIGHT, MAT_PHONG, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR = range(7)
 LIGHT_AREA, LIGHT_POINT, LIGHT_SPHERE, LIGHT_ENV_MAP = range(4)
+DISTRIBUTION_BECKMANN, DISTRIBUTION_GGX = 0, 1
 
 SCENE_CORNELL = 1
 SCENE_ROOM = 2
@@ -42,7 +43,8 @@ class Material(C.Structure):
                 ("specular", F3), ("specularHighlight", C.c_float), ("transparency", C.c_float),
                 ("refractionIndex", C.c_float), ("refractionIndexInv", C.c_float), ("eta", C.c_float),
                 ("k", C.c_float), ("roughness", C.c_float), ("textureIdDiffuse", C.c_int32),
-                ("textureIdSpecular", C.c_int32), ("type", C.c_int32), ("_pad2", C.c_int32 * 3)]
+                ("textureIdSpecular", C.c_int32), ("type", C.c_int32), ("distribution", C.c_int32),
+                ("_pad2", C.c_int32 * 2)]
 
 
 class Instance(C.Structure):

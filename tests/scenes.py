@@ -22,12 +22,13 @@ def box(lo, hi):
     return pos, idx.reshape(-1, 3)
 
 
-def material_zoo(point_light=True):
+def material_zoo(point_light=True, distribution=0):
     """Cornell-like box exercising every material type and light type:
     Lambert, Phong, smooth conductor, mirror, rough conductor, dielectric and
     light materials; area, sphere and point lights (area lights first).
     Point lights are reachable by NEE only, so estimator-equivalence tests
-    leave them out (point_light=False)."""
+    leave them out (point_light=False).  distribution: the rough conductors'
+    microfacet distribution (abi.DISTRIBUTION_BECKMANN or abi.DISTRIBUTION_GGX)."""
     s = rtxamd.HostScene.new()
     white = s.add_material(rtxamd.diffuse((0.7, 0.7, 0.7)))
     red = s.add_material(rtxamd.diffuse((0.6, 0.1, 0.1)))
@@ -35,8 +36,8 @@ def material_zoo(point_light=True):
     ph_sharp = s.add_material(rtxamd.phong((0.1, 0.1, 0.1), (0.8, 0.8, 0.8), 300.0))
     cond = s.add_material(rtxamd.conductor(0.2, 3.0))
     mir = s.add_material(rtxamd.mirror())
-    rough = s.add_material(rtxamd.rough_conductor(0.25, 0.37, 2.82))
-    rough_glossy = s.add_material(rtxamd.rough_conductor(0.08, 1.2, 5.0))
+    rough = s.add_material(rtxamd.rough_conductor(0.25, 0.37, 2.82, distribution))
+    rough_glossy = s.add_material(rtxamd.rough_conductor(0.08, 1.2, 5.0, distribution))
     glass = s.add_material(rtxamd.dielectric(1.5))
     lamp = s.add_material(rtxamd.light((8.0, 8.0, 8.0), (0.5, 0.5, 0.5)))
     ball = s.add_material(rtxamd.light((20.0, 10.0, 5.0)))

@@ -93,29 +93,31 @@ def test_traversal_matches_oracle(renderer, scene):
     ds.close()
 
 
-def test_traversal_kernels_agree(renderer):
-    """The persistent lane-refill traversal (the default) and the grid-stride
-    stack traversal find the same closest hit, bit for bit (ties go to the
-    smaller primitive id in both), so whole renders agree bitwise."""
+def test_launch_shape_does_not_change_results(renderer):
+    """The persistent traversal hands rays to lanes in whatever order the
+    work counter gives them out; closest hits are decided per triangle (ties
+    go to the smaller primitive id), so queries and whole renders are
+    bit-identical whatever the number of resident blocks and shading blocks."""
     hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 1)
     ds = renderer.upload(hs)
     rays = random_rays(100000, [-10, 0, -10], [10, 10, 10], seed=11)
     fp = scenes.params(hs, 320, 180, 2, maxDepth=12, enableMIS=1,
                        flags=abi.RTX_FRAME_KEEP_PATHS | abi.RTX_FRAME_RESET_ACCUM)
     out = {}
+    base = renderer.tuning()
     try:
-        for k in (1, 2):
-            renderer.set_tuning(traversalKernel=k)
+        for k, (tb, sb) in enumerate(((base.traceBlocksPerCU, base.shadeBlocksPerCU), (3, 2))):
+            renderer.set_tuning(traceBlocksPerCU=tb, shadeBlocksPerCU=sb)
             hits = renderer.trace_closest(ds, rays)
             occ = renderer.trace_occluded(ds, rays)
             st = renderer.render(ds, fp)
             rgb, sig = renderer.read_paths(320 * 180 * 2)
             out[k] = (hits, occ, rgb, sig, (st.extendRays, st.shadowRays, st.probeRays))
     finally:
-        renderer.set_tuning(traversalKernel=1)
+        renderer.set_tuning(traceBlocksPerCU=base.traceBlocksPerCU, shadeBlocksPerCU=base.shadeBlocksPerCU)
         ds.close()
-    (h1, o1, r1, s1, n1), (h2, o2, r2, s2, n2) = out[1], out[2]
-    log(f"traversal kernels: rays {n1} vs {n2}")
+    (h1, o1, r1, s1, n1), (h2, o2, r2, s2, n2) = out[0], out[1]
+    log(f"launch shapes: rays {n1} vs {n2}")
     for a, b in zip(h1, h2):
         assert np.array_equal(a, b)
     assert np.array_equal(o1, o2)
@@ -137,14 +139,15 @@ def render_both(renderer, hs, fp, n_paths=None, ids=None):
     return rgb[ids.astype(np.int64)], sig[ids.astype(np.int64)], orgb, osig, img, st
 
 
-@pytest.mark.parametrize("case", ["cornell", "cornell_rr", "zoo", "zoo_balance_numNEE2", "plates", "nee_off"])
+@pytest.mark.parametrize("case", ["cornell", "cornell_rr", "zoo", "zoo_ggx", "zoo_balance_numNEE2", "plates",
+                                  "nee_off"])
 def test_per_path_radiance_matches_oracle(renderer, case):
     log(f"per-path parity: {case}")
     if case.startswith("cornell"):
         hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
         fp = scenes.params(hs, 256, 256, 2, maxDepth=8, enableMIS=1, enableRR=int(case.endswith("rr")))
-    elif case == "zoo":
-        hs = scenes.material_zoo()
+    elif case in ("zoo", "zoo_ggx"):
+        hs = scenes.material_zoo(distribution=abi.DISTRIBUTION_GGX if case == "zoo_ggx" else abi.DISTRIBUTION_BECKMANN)
         fp = scenes.params(hs, 128, 128, 4, maxDepth=12, enableMIS=1)
     elif case == "zoo_balance_numNEE2":
         hs = scenes.material_zoo()

@@ -98,3 +98,36 @@ def test_guiding_cadence_skips_the_last_update(renderer, cornell):
     for x, y in zip(a_vmms, b_vmms):
         assert bytes(x) == bytes(y)
     ds.close()
+
+
+def test_command_line_renders_with_cache_and_guiding(tmp_path):
+    """bin/rtx_pathtracer WIDTH HEIGHT IC_SIZE GUIDING_SPLITS scene: the cache
+    is created from IC_SIZE and the guiding regions from GUIDING_SPLITS
+    (src/RayTracingApp.cpp:44-46, 346-351), the schedule runs the prepare
+    frames, the guiding cadence and eval-by-samples, and the EXR holds the
+    accumulated image: 4 samples per pixel, finite, lit like a plain render
+    of the same scene (the cache's interpolation bias is small next to a 4-spp
+    image's noise at this scale)."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(rtxamd.LIB_DIR).parent / "bin" / "rtx_pathtracer"
+    args = [str(exe), "64", "48", "400", "3", "gen:cornell", "--ic", "1", "--guiding", "1", "--spp", "4",
+            "--prepare", "3", "--guiding-updates", "2", "--depth", "6", "--out", str(tmp_path)]
+    log("cli: " + " ".join(args[1:]))
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    log(r.stdout.strip())
+    assert r.returncode == 0, r.stderr
+    files = sorted(tmp_path.glob("gen_cornell_4spp_*ms.exr"))
+    assert len(files) == 1, list(tmp_path.iterdir())
+    img = rtxamd.read_exr(files[0])
+    assert img.shape == (48, 64, 4) and np.isfinite(img).all() and (img[..., :3] >= 0).all()
+    with rtxamd.Renderer(0) as rr:
+        hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL, 1)
+        ds = rr.upload(hs)
+        fp = scenes.params(hs, 64, 48, 64, maxDepth=6, enableMIS=1)
+        rr.render(ds, fp)
+        ref = rr.read_image(64, 48)
+        ds.close()
+    m, mr = img[..., :3].mean(), ref[..., :3].mean()
+    log(f"cli image mean {m:.4f}, plain 64 spp {mr:.4f}")
+    assert abs(m - mr) < 0.15 * mr

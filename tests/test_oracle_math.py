@@ -14,6 +14,8 @@ L = lib(abi)
 
 
 def mat(kind, **kw):
+    if kind == "ggx":
+        return rtxamd.rough_conductor(distribution=abi.DISTRIBUTION_GGX, **kw)
     return {"diffuse": rtxamd.diffuse, "phong": rtxamd.phong, "rough": rtxamd.rough_conductor}[kind](**kw)
 
 
@@ -69,6 +71,110 @@ def test_beckmann_closed_form():
     assert L.orc_beckmann_d(C.byref(m), d3((0, 0, 1)), d3((1, 0, -0.1))) == 0.0
 
 
+# ------------------------------------------------------------------- GGX
+def _sph(t, p=0.0):
+    return (np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t))
+
+
+def _hemisphere_integral(f, alpha):
+    """int over the upper hemisphere of f(theta, phi) dw, with the theta
+    quadrature concentrated where a lobe of width alpha lives."""
+    from scipy import integrate
+    pts = [min(x, np.pi / 2) for x in (alpha, 3 * alpha, 10 * alpha)]
+
+    def inner(t):
+        return integrate.quad(lambda p: f(t, p), 0, 2 * np.pi, limit=200)[0] * np.sin(t)
+    return integrate.quad(inner, 0, np.pi / 2, points=pts, limit=400, epsabs=1e-10)[0]
+
+
+@pytest.mark.parametrize("alpha", [0.005, 0.05, 0.2, 0.5])
+def test_ggx_projected_area_normalisation(alpha):
+    """int D(m) (m.n) dw_m = 1 (Walter et al. 2007, eq. 4) for the GGX D."""
+    from scipy import integrate
+    m = mat("ggx", alpha=alpha, eta=1.0, k=1.0)
+    pts = [min(x, np.pi / 2) for x in (alpha, 3 * alpha, 10 * alpha)]
+    val = integrate.quad(lambda t: 2 * np.pi * L.orc_microfacet_d(C.byref(m), d3((0, 0, 1)), d3(_sph(t)))
+                         * np.cos(t) * np.sin(t), 0, np.pi / 2, points=pts, limit=400, epsabs=1e-12)[0]
+    assert val == pytest.approx(1.0, rel=1e-6)
+
+
+def test_ggx_closed_form():
+    """D = a^2 / (pi cos^4 (a^2 + tan^2)^2) (Walter et al. 2007, eq. 33)."""
+    m = mat("ggx", alpha=0.3, eta=1.0, k=1.0)
+    a2 = float(np.float32(0.3)) ** 2  # roughness is stored in FP32
+    for t in [0.0, 0.3, 0.9, 1.4]:
+        want = a2 / (np.pi * np.cos(t) ** 4 * (a2 + np.tan(t) ** 2) ** 2)
+        got = L.orc_microfacet_d(C.byref(m), d3((0, 0, 1)), d3(_sph(t)))
+        assert got == pytest.approx(want, rel=1e-10)
+    assert L.orc_microfacet_d(C.byref(m), d3((0, 0, 1)), d3((1, 0, -0.1))) == 0.0
+
+
+@pytest.mark.parametrize("alpha,theta_v", [(0.1, 0.3), (0.3, 1.0), (0.5, 1.35), (0.8, 0.7)])
+def test_ggx_smith_g1_projected_area(alpha, theta_v):
+    """The visible microsurface projects onto the geometric one:
+    int G1(v, m) max(0, v.m) D(m) dw_m = v.n (Heitz 2014, eq. 7), which fixes
+    the Smith G1 of a given D.  G1 is constant over the facets that face v,
+    so G1(v) = (v.n) / int max(0, v.m) D(m) dw_m."""
+    m = mat("ggx", alpha=alpha, eta=1.0, k=1.0)
+    v = np.array(_sph(theta_v))
+    n = d3((0, 0, 1))
+
+    def f(t, p):
+        h = _sph(t, p)
+        vm = float(np.dot(v, h))
+        if vm <= 0:
+            return 0.0
+        return L.orc_microfacet_g1(C.byref(m), n, d3(h), d3(v)) * vm * L.orc_microfacet_d(C.byref(m), n, d3(h))
+    assert _hemisphere_integral(f, alpha) == pytest.approx(np.cos(theta_v), rel=2e-5)
+    # facets facing away from v are invisible
+    assert L.orc_microfacet_g1(C.byref(m), n, d3(_sph(0.2, np.pi)), d3(_sph(1.45, 0.0))) == 0.0
+
+
+def test_ggx_bsdf_is_reciprocal_and_below_horizon_zero():
+    """f(wi, wo) = F G D / (4 |wi.n| |wo.n|) is symmetric in wi and wo; the
+    evaluated value is f cos(theta_o).  A direction below the horizon gets 0
+    (its G1 vanishes), unlike the reference's Beckmann G1."""
+    m = mat("ggx", alpha=0.35, eta=0.4, k=3.0)
+    n = (0, 0, 1)
+    rng = np.random.default_rng(1)
+    out1, out2 = (C.c_double * 3)(), (C.c_double * 3)()
+    for _ in range(50):
+        a = np.array(_sph(rng.uniform(0, 1.4), rng.uniform(0, 2 * np.pi)))
+        b = np.array(_sph(rng.uniform(0, 1.4), rng.uniform(0, 2 * np.pi)))
+        L.orc_eval_bsdf(C.byref(m), d3(n), d3(a), d3(b), 1, out1)
+        L.orc_eval_bsdf(C.byref(m), d3(n), d3(b), d3(a), 1, out2)
+        assert out1[0] / b[2] == pytest.approx(out2[0] / a[2], rel=1e-10)
+        assert out1[0] > 0
+    L.orc_eval_bsdf(C.byref(m), d3(n), d3(_sph(0.3)), d3((0.8, 0.0, -0.6)), 1, out1)
+    assert out1[0] == 0.0
+
+
+def test_ggx_normal_sampler_matches_D_cos():
+    """The GGX normal sampler draws m with density D(m) (m.n)."""
+    m = mat("ggx", alpha=0.2, eta=1.0, k=1.0)
+    st = C.c_uint32(3)
+    out = (C.c_double * 3)()
+    ms = np.empty((30000, 3))
+    for k in range(len(ms)):
+        L.orc_sample_beckmann_normal(C.byref(m), d3((0, 0, 1)), C.byref(st), out)
+        ms[k] = out[:]
+    assert st.value == _lcg_n(3, 2 * len(ms))  # two draws per normal
+
+    def pdf(w):
+        return L.orc_microfacet_d(C.byref(m), d3((0, 0, 1)), d3(w)) * max(w[2], 0)
+
+    ct, ph = sphere_grid(24, 8, hemisphere=True)
+    ct = 1 - (1 - ct) ** 3  # concentrate bins near the pole
+    exp = expected_counts(pdf, ct, ph, len(ms))
+    assert chi2_pvalue(ms, exp, ct, ph) > 1e-3
+
+
+def _lcg_n(state, n):
+    for _ in range(n):
+        state = L.orc_lcg_step(state)
+    return state
+
+
 # ------------------------------------------------------- sampling vs pdfs
 def sphere_grid(nt=24, np_=24, hemisphere=False):
     ct = np.linspace(0 if hemisphere else -1, 1, nt + 1)
@@ -122,6 +228,8 @@ def draw(m, wi, n, count, seed=7):
     ("diffuse", dict(albedo=(0.5, 0.5, 0.5))),
     ("phong", dict(kd=(0.3, 0.3, 0.3), ks=(0.5, 0.5, 0.5), exponent=20.0)),
     ("rough", dict(alpha=0.3, eta=0.4, k=3.0)),
+    ("ggx", dict(alpha=0.3, eta=0.4, k=3.0)),
+    ("ggx", dict(alpha=0.08, eta=0.4, k=3.0)),
 ])
 def test_sampleBSDF_distribution_matches_pdfBSDF(kind, kw):
     m = mat(kind, **kw)
@@ -194,7 +302,8 @@ def test_pdf_normalisation():
     ct = np.linspace(-1, 1, 801)
     ph = np.linspace(0, 2 * np.pi, 401)
     for m, want, hemi in [(mat("diffuse", albedo=(1, 1, 1)), 1.0, True),
-                          (mat("rough", alpha=0.3, eta=1.0, k=1.0), 1.0, False)]:
+                          (mat("rough", alpha=0.3, eta=1.0, k=1.0), 1.0, False),
+                          (mat("ggx", alpha=0.3, eta=1.0, k=1.0), 1.0, False)]:
         tot = 0.0
         for c in 0.5 * (ct[1:] + ct[:-1]):
             if hemi and c < 0:

@@ -439,6 +439,27 @@ def test_xml_materials_refs_emitters_and_camera(tmp_path):
     assert tex[1].shape == (2, 5, 4) and (tex[1][..., 3] == 255).all()
 
 
+def test_xml_rough_conductor_distribution(tmp_path):
+    """The roughconductor's Mitsuba "distribution" picks the microfacet
+    distribution: beckmann (Mitsuba's default, and the only one the reference
+    renders) or ggx; others are refused."""
+    write(tmp_path / "tri.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+
+    def rough(idname, dist):
+        d = f'<string name="distribution" value="{dist}"/>' if dist else ""
+        return (f'<bsdf type="roughconductor" id="{idname}">{d}<float name="alpha" value="0.1"/>'
+                '<spectrum name="eta" value="0.5"/><spectrum name="k" value="2"/></bsdf>')
+    body = rough("a", None) + rough("b", "beckmann") + rough("c", "ggx") + "".join(
+        obj_shape("tri.obj", f'<ref id="{i}"/>') for i in "abc")
+    ms = mats(rtxamd.HostScene.load(xml_scene(tmp_path, body)))
+    assert [m.type for m in ms] == [abi.MAT_ROUGH_CONDUCTOR] * 3
+    assert [m.distribution for m in ms] == [abi.DISTRIBUTION_BECKMANN, abi.DISTRIBUTION_BECKMANN,
+                                            abi.DISTRIBUTION_GGX]
+    assert ms[2].roughness == pytest.approx(0.1) and ms[2].eta == pytest.approx(0.5)
+    with pytest.raises(rtxamd.RtxError, match="distribution"):
+        rtxamd.HostScene.load(xml_scene(tmp_path, rough("d", "phong") + obj_shape("tri.obj", '<ref id="d"/>')))
+
+
 def test_xml_envmap_is_the_last_light(tmp_path):
     env = np.random.default_rng(3).random((8, 16, 4)).astype(np.float32)
     rtxamd.write_exr(tmp_path / "sky.exr", env)
