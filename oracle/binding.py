@@ -120,6 +120,17 @@ def lib(abi):
         L.orc_eigen2x2.restype = None
         L.orc_fastexp.argtypes = [C.c_float]
         L.orc_fastexp.restype = C.c_float
+        dp = C.POINTER(C.c_double)
+        L.orc_em_mix_doubles.argtypes = []
+        L.orc_em_mix_doubles.restype = C.c_int
+        L.orc_em_estep.argtypes = [dp, dp, dp, C.c_size_t, dp, dp, dp, dp]
+        L.orc_em_estep.restype = C.c_int
+        L.orc_em_mstep.argtypes = [dp, dp, dp, dp]
+        L.orc_em_mstep.restype = C.c_int
+        L.orc_em_fit.argtypes = [dp, dp, dp, dp, C.c_size_t, C.c_int]
+        L.orc_em_fit.restype = C.c_int
+        L.orc_partition_order.argtypes = [dp, C.c_size_t, C.c_double, C.POINTER(C.c_uint32)]
+        L.orc_partition_order.restype = C.c_size_t
         L.orc_ic_calculate.argtypes = [C.c_void_p, C.POINTER(abi.FrameParams), C.c_int, vp, vp, vp, C.c_uint32, d3,
                                        d3, d3, d3, d3, C.POINTER(C.c_double)]
         L.orc_estimate_set.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]

@@ -1090,6 +1090,100 @@ double orc_merge_metric(const double a[8], const double b[8]) {
     }
     return mergeMetric(m, 0, 1);
 }
+// ---- the MAP-EM's steps on one FP64 mixture (VMMFactory.h:157-216,
+// 305-362, 385-486), for tests that pin them by closed forms
+static Props propsFrom(const double p[7]) {
+    Props P;
+    P.minItr = (uint32_t)p[0];
+    P.maxItr = (uint32_t)p[1];
+    P.relLogLikelihoodThreshold = (float)p[2];
+    P.maxKappa = (float)p[3];
+    P.vPrior = (float)p[4];
+    P.rPrior = (float)p[5];
+    P.rPriorWeight = (float)p[6];
+    return P;
+}
+// m[]: K, then per component (w, mu xyz, kappa, r), then sampleWeight,
+// numSamples, totalNumSamples: 1 + 6 * 16 + 3 doubles
+constexpr int kMixDoubles = 1 + 6 * kMaxK + 3;
+static bool mixFrom(const double *v, Pmm<double> &m) {
+    m = Pmm<double>{};
+    m.K = (int)v[0];
+    if (m.K < 1 || m.K > kMaxK) return false;
+    for (int k = 0; k < kMaxK; ++k) {
+        const double *c = v + 1 + 6 * k;
+        m.w[k] = c[0];
+        m.mu[k][0] = c[1], m.mu[k][1] = c[2], m.mu[k][2] = c[3];
+        m.kappa[k] = c[4];
+        m.r[k] = c[5];
+        m.calNorm(k);
+    }
+    m.sampleWeight = v[1 + 6 * kMaxK];
+    m.numSamples = v[2 + 6 * kMaxK];
+    m.totalNumSamples = (uint64_t)v[3 + 6 * kMaxK];
+    return true;
+}
+static void mixTo(const Pmm<double> &m, double *v) {
+    v[0] = m.K;
+    for (int k = 0; k < kMaxK; ++k) {
+        double *c = v + 1 + 6 * k;
+        c[0] = m.w[k], c[1] = m.mu[k][0], c[2] = m.mu[k][1], c[3] = m.mu[k][2], c[4] = m.kappa[k], c[5] = m.r[k];
+    }
+    v[1 + 6 * kMaxK] = m.sampleWeight;
+    v[2 + 6 * kMaxK] = m.numSamples;
+    v[3 + 6 * kMaxK] = (double)m.totalNumSamples;
+}
+static std::vector<Sample> samplesFrom(const double *dirs, const double *weights, size_t n) {
+    std::vector<Sample> s(n);
+    for (size_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) s[i].dir[c] = dirs[3 * i + c], s[i].pos[c] = 0;
+        s[i].weight = weights[i];
+        s[i].distance = 0;
+        s[i].pdf = 1;
+    }
+    return s;
+}
+// E-step statistics: per component the responsibility-weighted resultant
+// rW (3 doubles) and weight W; the samples' weight sum and log-likelihood
+int orc_em_estep(const double *mix, const double *dirs, const double *weights, size_t n, double *rW, double *W,
+                 double *sumWeight, double *logLik) {
+    Pmm<double> m;
+    if (!mixFrom(mix, m)) return RTX_E_INVALID;
+    const Stats<double> st = statsFromSamples(m, samplesFrom(dirs, weights, n), sumWeight, logLik);
+    for (int k = 0; k < m.K; ++k) {
+        for (int c = 0; c < 3; ++c) rW[3 * k + c] = st.rW[k][c];
+        W[k] = st.W[k];
+    }
+    return RTX_OK;
+}
+// M-step of every component from given statistics (parameterUpdate)
+int orc_em_mstep(const double props[7], double *mix, const double *rW, const double *W) {
+    Pmm<double> m;
+    if (!mixFrom(mix, m)) return RTX_E_INVALID;
+    const Props P = propsFrom(props);
+    for (int k = 0; k < m.K; ++k) parameterUpdate(P, m, k, rW + 3 * k, W[k]);
+    mixTo(m, mix);
+    return RTX_OK;
+}
+// fit (update = 0) or updateFit (update = 1) of a mixture to weighted directions
+int orc_em_fit(const double props[7], double *mix, const double *dirs, const double *weights, size_t n, int update) {
+    Pmm<double> m;
+    if (!mixFrom(mix, m)) return RTX_E_INVALID;
+    emFit(propsFrom(props), m, samplesFrom(dirs, weights, n), update != 0);
+    mixTo(m, mix);
+    return RTX_OK;
+}
+int orc_em_mix_doubles(void) { return kMixDoubles; }
+// The element order partitionLikeLibstdcxx leaves for keys[] and "key <= pivot"
+// (out: the original indices in their new order); returns the partition point.
+size_t orc_partition_order(const double *keys, size_t n, double pivot, uint32_t *out) {
+    std::vector<std::pair<double, uint32_t>> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = {keys[i], (uint32_t)i};
+    const size_t mid = partitionLikeLibstdcxx(v, [&](const std::pair<double, uint32_t> &x) { return x.first <= pivot; });
+    for (size_t i = 0; i < n; ++i) out[i] = v[i].second;
+    return mid;
+}
+
 void orc_eigen2x2(const double a[4], double lam[2], double vec[4]) {
     double V[2][2];
     eigen2x2(a[0], a[1], a[2], a[3], lam, V);

@@ -144,6 +144,18 @@ void orc_vmf_combine(const double a[8], const double b[8], int divide, double ou
 void orc_vmf_merge(const double a[8], const double b[8], double out[8]);
 double orc_merge_metric(const double a[8], const double b[8]);
 void orc_eigen2x2(const double a[4], double lam[2], double vec[4]);
+/* The MAP-EM's steps on one FP64 mixture, for closed-form tests.  A mixture
+ * is orc_em_mix_doubles() doubles: K, then 16 x (w, mu xyz, kappa, r), then
+ * sampleWeight, numSamples, totalNumSamples.  props: minItr, maxItr,
+ * relLogLikelihoodThreshold, maxKappa, vPrior, rPrior, rPriorWeight. */
+int orc_em_mix_doubles(void);
+/* The order std::partition (libstdc++, bidirectional form) leaves for
+ * "keys[i] <= pivot"; out receives the original indices. */
+size_t orc_partition_order(const double *keys, size_t n, double pivot, uint32_t *out);
+int orc_em_estep(const double *mix, const double *dirs, const double *weights, size_t n, double *rW, double *W,
+                 double *sumWeight, double *logLik);
+int orc_em_mstep(const double props[7], double *mix, const double *rW, const double *W);
+int orc_em_fit(const double props[7], double *mix, const double *dirs, const double *weights, size_t n, int update);
 
 /* Building blocks, exposed for known-answer and distribution tests. */
 uint32_t orc_tea(uint32_t v0, uint32_t v1);
