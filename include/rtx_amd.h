@@ -320,6 +320,10 @@ typedef struct rtx_scene_desc {
 #define RTX_FRAME_TIME_KERNELS 0x8u /* time every kernel with HIP events      */
 #define RTX_FRAME_NO_GUIDING_UPDATE 0x10u /* record (pc.updateGuiding) but skip the
                                            * guiding update after the frame     */
+#define RTX_FRAME_NRC 0x40u         /* learned radiance cache (rtx_nrc_create): camera
+                                     * paths end into the cache after
+                                     * terminateBounces bounces, a sparse set of
+                                     * training paths continues and trains it */
 #define RTX_FRAME_TIME_TRACE 0x20u  /* time only the traversal launches with HIP
                                      * events (msExtend, msShadow; the other
                                      * per-kind times stay 0): each event pair
@@ -581,6 +585,91 @@ int rtx_trace_closest(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
  * primitive is hit in [tmin, tmax]. */
 int rtx_trace_occluded(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
                        size_t n, uint8_t *occluded);
+
+/* ------------------------------------------------ learned radiance cache */
+/* A neural radiance cache (Mueller, Rousselle, Novak and Keller 2021) of the
+ * reflected radiance leaving a surface point: a multiresolution hash-grid
+ * encoding of the position (Mueller, Evans, Schied and Keller 2022; 2 features
+ * per level) plus the direction, normal, roughness and albedo, into a 5 x 64
+ * ReLU MLP with an RGB output.  BF16 MFMA kernels with FP32 accumulation run
+ * inference and training; FP32 master parameters are updated by Adam on the
+ * device.  No counterpart in the reference, whose nearest behaviour is the
+ * irradiance cache ending a path at a cache hit
+ * (shaders/raytrace.rgen:1117-1123).  One cache per context.
+ *
+ * Renders with RTX_FRAME_NRC: at the first non-specular vertex reached after
+ * terminateBounces bounces a camera path adds throughput x max(cache, 0) and
+ * ends (no NEE there; the cache holds everything the tracer would add from
+ * that vertex on, emission at the vertex excluded).  About trainingPaths
+ * paths per frame (a regular pixel pattern that moves every frame) continue
+ * unchanged instead; what they add from that vertex on, divided by their
+ * throughput there, is the training target of the vertex.  After each frame
+ * the cache takes trainSteps Adam steps on trainBatch of these targets each
+ * (relative L2 loss, Mueller et al. 2021), so a frame reads the cache as the
+ * previous frames left it.  Not combined with the irradiance cache, ADRRS,
+ * guiding, splitting or several samples per frame (RTX_E_UNSUPPORTED). */
+typedef struct rtx_nrc_config {
+    uint32_t levels;           /* hash-grid levels (16; at most 16)             */
+    uint32_t log2TableSize;    /* entries per level, log2 (19)                  */
+    float baseResolution;      /* cells per axis of the coarsest level (16)     */
+    float maxResolution;       /* of the finest (2048)                          */
+    float learningRate;        /* Adam (1e-2)                                   */
+    float beta1, beta2, epsilon; /* Adam (0.9, 0.99, 1e-15)                     */
+    uint32_t trainBatch;       /* rows per Adam step (16384; at most 65536)     */
+    uint32_t trainSteps;       /* Adam steps per frame (4)                      */
+    uint32_t terminateBounces; /* bounces before a path ends into the cache (2) */
+    uint32_t trainingPaths;    /* training paths per frame (65536)              */
+    float sceneMin[3];         /* box the positions are encoded in              */
+    float sceneMax[3];
+    uint64_t seed;             /* parameter initialisation                      */
+} rtx_nrc_config;
+
+/* One cache query: the vertex's position, the direction towards the previous
+ * vertex (unit), the shading normal, a roughness (1 Lambert, alpha for rough
+ * conductors, sqrt(2 / (exponent + 2)) for Phong, 0 for specular) and an
+ * albedo. 64 B. */
+typedef struct rtx_nrc_input {
+    float pos[3];
+    float roughness;
+    float dir[3];
+    float _pad0;
+    float normal[3];
+    float _pad1;
+    float albedo[3];
+    float _pad2;
+} rtx_nrc_input;
+
+typedef struct rtx_nrc_info {
+    uint32_t levels;
+    uint32_t tableSize;         /* entries per level */
+    float resolution[16];       /* cells per axis of each level */
+    uint32_t dense[16];         /* 1: (resolution + 1)^3 <= tableSize, indexed densely */
+    float lo[3];                /* encoded position = clamp((x - lo) * inv, 0, 1) */
+    float inv[3];
+    uint64_t gridParams;        /* levels * tableSize * 2 */
+    uint32_t mlpParams;         /* 5 * 64 * 64 + 16 * 64: W[layer][out][in], output layer 16 x 64
+                                   (rows 3..15 stay zero) */
+    uint32_t steps;             /* Adam steps taken */
+    float lastLoss;             /* mean loss of the last frame's (or call's) steps */
+} rtx_nrc_info;
+
+void rtx_nrc_config_defaults(rtx_nrc_config *config);
+int rtx_nrc_create(rtx_ctx *ctx, const rtx_nrc_config *config);
+int rtx_nrc_destroy(rtx_ctx *ctx);
+int rtx_nrc_get_info(rtx_ctx *ctx, rtx_nrc_info *info);
+/* FP32 master parameters: the table [level][entry][feature], then the MLP.
+ * set_params restarts Adam. */
+int rtx_nrc_get_params(rtx_ctx *ctx, float *grid, size_t gridCount, float *mlp, size_t mlpCount);
+int rtx_nrc_set_params(rtx_ctx *ctx, const float *grid, size_t gridCount, const float *mlp, size_t mlpCount);
+/* The cache's RGB prediction for n queries (rgb: 3 floats each). */
+int rtx_nrc_infer(rtx_ctx *ctx, const rtx_nrc_input *in, size_t n, float *rgb);
+/* The loss gradients of n <= trainBatch rows against targets (3 floats each)
+ * without changing the parameters; grid/mlp as in get_params, either may be
+ * NULL.  loss: the relative L2 loss, mean over rows and channels. */
+int rtx_nrc_gradients(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target, size_t n, float *gridGrad,
+                      float *mlpGrad, float *loss);
+/* One Adam step on n <= trainBatch rows. */
+int rtx_nrc_train(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target, size_t n, float *loss);
 
 /* ------------------------------------------------------ frame schedule */
 /* The host loop around the render of the reference's application,

@@ -17,6 +17,7 @@
 #include "abi_guard.h"
 #include "dev_math.h"
 #include "internal.h"
+#include "nrc.h"
 
 using namespace rtx;
 
@@ -108,6 +109,15 @@ struct rtx_ctx {
     // ADRRS pixel estimate (estimateImage)
     float4 *estimate = nullptr;
     uint32_t estW = 0, estH = 0;
+    // learned radiance cache (rtx_nrc_create): parameters and training work,
+    // the render's queries and training vertices
+    bool nrcValid = false;
+    rtx_nrc_config nrcCfg{};
+    NrcBuffers nrc{};
+    std::vector<void *> nrcAllocs, nrcPathAllocs, nrcSlotAllocs;
+    float *nrcLoss = nullptr;   // device: summed loss of the last steps
+    uint32_t nrcLossSteps = 0;
+    size_t nrcPathCap = 0, nrcSlotCap = 0;
 };
 
 namespace {
@@ -520,6 +530,9 @@ int rtx_ctx_destroy(rtx_ctx *ctx) try {
     freeAll(ctx->recAllocs);
     freeAll(ctx->segAllocs);
     freeAll(ctx->splitAllocs);
+    freeAll(ctx->nrcAllocs);
+    freeAll(ctx->nrcPathAllocs);
+    freeAll(ctx->nrcSlotAllocs);
     if (ctx->estimate) (void)hipFree(ctx->estimate);
     if (ctx->dirData) (void)hipFree(ctx->dirData);
     if (ctx->fcIC) (void)hipFree(ctx->fcIC);
@@ -764,6 +777,7 @@ int rtx_scene_get_info(const rtx_scene *scene, rtx_scene_info *info) try {
 }
 
 static int guideUpdateAsync(rtx_ctx *ctx);
+static int nrcStep(rtx_ctx *ctx, const float4 *in, const float4 *target, uint32_t n);
 
 int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, rtx_frame_stats *stats) try {
     if (!ctx) return RTX_E_INVALID;
@@ -806,6 +820,13 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         if (!(p->pc.irradianceA > 0.0f)) return fail(ctx, RTX_E_INVALID, "irradianceA must be > 0");
         neeCap = std::max(neeCap, (uint32_t)p->pc.irradianceNumNEE);
     }
+    const bool nrc = (p->flags & RTX_FRAME_NRC) != 0;
+    if (nrc) {
+        if (!ctx->nrcValid) return fail(ctx, RTX_E_STATE, "RTX_FRAME_NRC needs rtx_nrc_create first");
+        if (icFrames || splitting || p->pc.useGuiding || p->pc.updateGuiding || p->pc.samplesPerPixel > 1)
+            return fail(ctx, RTX_E_UNSUPPORTED, "the learned radiance cache with the irradiance cache, ADRRS, "
+                                                "splitting, guiding or several samples per frame");
+    }
     if ((p->pc.useGuiding || p->pc.updateGuiding) && ctx->guide.numRegions == 0)
         return fail(ctx, RTX_E_STATE, "useGuiding/updateGuiding need guiding regions (rtx_guiding_set)");
     if (p->pc.useGuiding && !(p->pc.guidingProb >= 0.0f && p->pc.guidingProb <= 1.0f))
@@ -842,7 +863,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     uint64_t batch = p->batchPaths ? p->batchPaths : ctx->tuning.batchPaths;
     batch = std::min<uint64_t>(batch, total);
     const bool record = p->pc.updateGuiding != 0;
-    if (icFrames || record) batch = std::min<uint64_t>(batch, npix);
+    if (icFrames || record || nrc) batch = std::min<uint64_t>(batch, npix);
     if (batch >= npix) batch -= batch % npix;
     size_t bufPaths = (size_t)batch;
     if (icFrames) bufPaths = std::max<size_t>(bufPaths, kIcStrata); // at least one job of sub-paths
@@ -880,6 +901,30 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         ctx->candCap = bufPaths;
     }
     ctx->pb.cand = ctx->candBuf;
+    // learned radiance cache: one query and one training record per path of a
+    // batch, training slots for a frame
+    uint32_t nrcEvery = 1, nrcSlots = 0;
+    if (nrc) {
+        if (ctx->nrcPathCap < bufPaths) {
+            freeAll(ctx->nrcPathAllocs);
+            ctx->nrcPathCap = 0;
+            auto &L = ctx->nrcPathAllocs;
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.nrcQuery, 4 * bufPaths, L));
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.nrcQueryAux, bufPaths, L));
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.nrcPath, 2 * bufPaths, L));
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.nrcQueryCount, 1, L));
+            ctx->nrcPathCap = bufPaths;
+        }
+        nrcEvery = (uint32_t)std::max<uint64_t>(1, npix / ctx->nrcCfg.trainingPaths);
+        nrcSlots = (uint32_t)((npix + nrcEvery - 1) / nrcEvery + 1);
+        if (ctx->nrcSlotCap < nrcSlots) {
+            freeAll(ctx->nrcSlotAllocs);
+            ctx->nrcSlotCap = 0;
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.nrcTrainIn, 4 * (size_t)nrcSlots, ctx->nrcSlotAllocs));
+            CHECK_HIP(ctx, devAlloc(&ctx->pb.nrcTrainTarget, nrcSlots, ctx->nrcSlotAllocs));
+            ctx->nrcSlotCap = nrcSlots;
+        }
+    }
     if (record) {
         if (ctx->recCap < bufPaths) {
             freeAll(ctx->recAllocs);
@@ -963,6 +1008,10 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.segments = segments ? 1u : 0u;
     fc.useADRRS = useADRRS ? 1u : 0u;
     fc.splitOnFirst = p->pc.splitOnFirst ? 1u : 0u;
+    fc.nrc = nrc ? 1u : 0u;
+    fc.nrcDepth = nrc ? ctx->nrcCfg.terminateBounces + 1u : 0u;
+    fc.nrcTrainEvery = nrcEvery;
+    fc.nrcTrainSlots = nrcSlots;
     CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));
     // the cache's stratum sub-paths: raytrace(..., 1, 10, false, false, true, irradianceNumNEE,
     // true, false, ...) (raytrace.rgen:1260)
@@ -979,6 +1028,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fcIC.segments = 0;
     fcIC.useADRRS = 0;
     fcIC.splitOnFirst = 0;
+    fcIC.nrc = 0;
     fcIC.numNEE = icFrames ? (uint32_t)p->pc.irradianceNumNEE : 1u;
     fcIC.debugPath = ~0ull;
     if (icFrames) CHECK_HIP(ctx, hipMemcpyAsync(ctx->fcIC, &fcIC, sizeof(fcIC), hipMemcpyHostToDevice, ctx->stream));
@@ -1067,7 +1117,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     CHECK_HIP(ctx, hipEventRecord(ctx->tStart, s));
     uint32_t batches = 0;
     if (icFrames) timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
-    const uint64_t frameLen = (icFrames || record) ? npix : total; // paths per frame with per-frame state
+    const uint64_t frameLen = (icFrames || record || nrc) ? npix : total; // paths per frame with per-frame state
     for (uint64_t f0 = 0; f0 < total; f0 += frameLen) {
         const uint64_t fEnd = std::min(total, f0 + frameLen);
         if (icFrames) timed(kClsShade, [&] { launchIcFrameBegin(s, ctx->ic); });
@@ -1081,6 +1131,11 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             setRaySeeds(fc.numNEE);
             int rcb = bounces(ctx->fc, b, maxIter, segments);
             if (rcb != RTX_OK) return rcb;
+            if (nrc) // the paths that ended into the learned cache: throughput x cache
+                timed(kClsShade, [&] {
+                    launchNrcInfer(s, ctx->nrc, ctx->pb.nrcQuery, b.numPaths, ctx->pb.nrcQueryCount, nullptr,
+                                   ctx->pb.nrcQueryAux, ctx->pb.out, 2 * ctx->cuCount);
+                });
             timed(kClsShade, [&] {
                 launchAccumulate(s, b, ctx->pb, accumulate ? ctx->accum : nullptr, ctx->keepRgb, ctx->keepSig, keep,
                                  storeEstimate ? ctx->estimate : nullptr);
@@ -1088,6 +1143,21 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             if (icFrames) timed(kClsShade, [&] { launchIcCollect(s, ctx->ic, ctx->pb, b.numPaths); });
             CHECK_HIP(ctx, hipGetLastError());
             batches++;
+        }
+        if (nrc) { // the learned cache trains on the frame's training vertices
+            const uint32_t B = ctx->nrcCfg.trainBatch;
+            const uint32_t rows = std::min(nrcSlots, B * ctx->nrcCfg.trainSteps);
+            launchNrcZero(s, ctx->nrcLoss, 1);
+            ctx->nrcLossSteps = 0;
+            for (uint32_t r0 = 0; r0 < rows; r0 += B) {
+                const uint32_t nr = std::min(B, rows - r0);
+                int rcn = RTX_OK;
+                timed(kClsShade, [&] {
+                    rcn = nrcStep(ctx, ctx->pb.nrcTrainIn + 4 * (size_t)r0, ctx->pb.nrcTrainTarget + r0, nr);
+                });
+                if (rcn != RTX_OK) return rcn;
+                ctx->nrcLossSteps++;
+            }
         }
         // PathGuiding::update after every recording frame when the context owns the mixtures
         // (src/RayTracingApp.cpp:143-151)
@@ -1673,6 +1743,341 @@ int rtx_read_paths(rtx_ctx *ctx, float *rgb, uint32_t *sig, size_t n) try {
     }
     if (sig) CHECK_HIP(ctx, hipMemcpy(sig, ctx->keepSig, n * sizeof(uint2), hipMemcpyDeviceToHost));
     return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+// ------------------------------------------------ learned radiance cache
+// Deterministic parameter initialisation stream (splitmix64).
+static uint64_t splitmix64(uint64_t &x) {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+static float uniform01(uint64_t &x) { return (float)(splitmix64(x) >> 40) * (1.0f / 16777216.0f); }
+
+static void nrcFree(rtx_ctx *ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    freeAll(ctx->nrcAllocs);
+    freeAll(ctx->nrcPathAllocs);
+    freeAll(ctx->nrcSlotAllocs);
+    ctx->nrc = NrcBuffers{};
+    ctx->nrcLoss = nullptr;
+    ctx->nrcValid = false;
+    ctx->nrcPathCap = ctx->nrcSlotCap = 0;
+    ctx->pb.nrcQuery = ctx->pb.nrcQueryAux = ctx->pb.nrcPath = nullptr;
+    ctx->pb.nrcQueryCount = nullptr;
+    ctx->pb.nrcTrainIn = ctx->pb.nrcTrainTarget = nullptr;
+}
+
+// Adam restarts (moments and step count), gradients cleared.
+static int nrcResetOptimizer(rtx_ctx *ctx) {
+    const size_t total = ctx->nrc.gridParams + kNrcMlpParams;
+    CHECK_HIP(ctx, hipMemsetAsync(ctx->nrc.grads, 0, total * sizeof(float), ctx->stream));
+    CHECK_HIP(ctx, hipMemsetAsync(ctx->nrc.adamM, 0, total * sizeof(float), ctx->stream));
+    CHECK_HIP(ctx, hipMemsetAsync(ctx->nrc.adamV, 0, total * sizeof(float), ctx->stream));
+    ctx->nrc.step = 0;
+    return RTX_OK;
+}
+
+// One Adam step on rows [0, n) of device records `in` / `target`; the loss
+// adds to ctx->nrcLoss.
+static int nrcStep(rtx_ctx *ctx, const float4 *in, const float4 *target, uint32_t n) {
+    launchNrcGradients(ctx->stream, ctx->nrc, in, target, n, 1.0f / (3.0f * (float)n), ctx->nrcLoss, ctx->cuCount);
+    ctx->nrc.step++;
+    launchNrcAdam(ctx->stream, ctx->nrc);
+    CHECK_HIP(ctx, hipGetLastError());
+    return RTX_OK;
+}
+
+// Host rows (rtx_nrc_input, 3 floats of target each) into temporary device
+// buffers (4 float4 per input row, one float4 per target).
+static int nrcUpload(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target, size_t n, std::vector<void *> &tmp,
+                     float4 **inD, float4 **tD) {
+    CHECK_HIP(ctx, devAlloc(inD, 4 * n, tmp));
+    CHECK_HIP(ctx, hipMemcpyAsync(*inD, in, n * sizeof(rtx_nrc_input), hipMemcpyHostToDevice, ctx->stream));
+    if (target && tD) {
+        std::vector<float4> t(n);
+        for (size_t i = 0; i < n; ++i) t[i] = make_float4(target[3 * i], target[3 * i + 1], target[3 * i + 2], 1.0f);
+        CHECK_HIP(ctx, devAlloc(tD, n, tmp));
+        CHECK_HIP(ctx, hipMemcpy(*tD, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    return RTX_OK;
+}
+
+void rtx_nrc_config_defaults(rtx_nrc_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->levels = 16;
+    c->log2TableSize = 19;
+    c->baseResolution = 16.0f;
+    c->maxResolution = 2048.0f;
+    c->learningRate = 1e-2f;
+    c->beta1 = 0.9f;
+    c->beta2 = 0.99f;
+    c->epsilon = 1e-15f;
+    c->trainBatch = 16384;
+    c->trainSteps = 4;
+    c->terminateBounces = 2;
+    c->trainingPaths = 65536;
+    for (int a = 0; a < 3; ++a) {
+        c->sceneMin[a] = -1.0f;
+        c->sceneMax[a] = 1.0f;
+    }
+    c->seed = 1;
+}
+
+int rtx_nrc_create(rtx_ctx *ctx, const rtx_nrc_config *c) try {
+    if (!ctx) return RTX_E_INVALID;
+    if (!c) return fail(ctx, RTX_E_INVALID, "NULL config");
+    if (c->levels < 1 || c->levels > (uint32_t)kNrcMaxLevels || c->log2TableSize < 8 || c->log2TableSize > 24)
+        return fail(ctx, RTX_E_INVALID, "levels must be in [1, 16] and log2TableSize in [8, 24]");
+    if (!(c->baseResolution >= 2.0f) || !(c->maxResolution >= c->baseResolution) || !(c->maxResolution <= 1e6f))
+        return fail(ctx, RTX_E_INVALID, "need 2 <= baseResolution <= maxResolution <= 1e6");
+    if (c->trainBatch < 256 || c->trainBatch > 65536 || c->trainSteps < 1 || c->trainSteps > 64 ||
+        c->trainingPaths < 1 || c->terminateBounces > 64)
+        return fail(ctx, RTX_E_INVALID, "trainBatch in [256, 65536], trainSteps in [1, 64], trainingPaths >= 1, "
+                                        "terminateBounces <= 64");
+    if (!(c->learningRate > 0.0f) || !(c->beta1 >= 0.0f && c->beta1 < 1.0f) || !(c->beta2 >= 0.0f && c->beta2 < 1.0f) ||
+        !(c->epsilon >= 0.0f))
+        return fail(ctx, RTX_E_INVALID, "Adam: learningRate > 0, beta1 and beta2 in [0, 1), epsilon >= 0");
+    for (int a = 0; a < 3; ++a)
+        if (!(c->sceneMax[a] > c->sceneMin[a])) return fail(ctx, RTX_E_INVALID, "sceneMax must exceed sceneMin");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    nrcFree(ctx);
+    NrcBuffers nb{};
+    const uint32_t T = 1u << c->log2TableSize;
+    nb.grid.levels = c->levels;
+    nb.grid.tableMask = T - 1u;
+    const double growth =
+        c->levels > 1 ? std::exp((std::log((double)c->maxResolution) - std::log((double)c->baseResolution)) /
+                                 (double)(c->levels - 1))
+                      : 1.0;
+    for (uint32_t l = 0; l < (uint32_t)kNrcMaxLevels; ++l) {
+        const double r = std::floor((double)c->baseResolution * std::pow(growth, (double)l));
+        nb.grid.res[l] = (float)r;
+        nb.grid.dense[l] = (r + 1.0) * (r + 1.0) * (r + 1.0) <= (double)T ? 1u : 0u;
+    }
+    for (int a = 0; a < 3; ++a) {
+        nb.grid.lo[a] = c->sceneMin[a];
+        nb.grid.inv[a] = 1.0f / (c->sceneMax[a] - c->sceneMin[a]);
+    }
+    nb.gridParams = (size_t)c->levels * T * 2;
+    const size_t total = nb.gridParams + kNrcMlpParams;
+    nb.rowsCap = (c->trainBatch + 255u) & ~255u;
+    auto &L = ctx->nrcAllocs;
+    CHECK_HIP(ctx, devAlloc(&nb.params, total, L));
+    CHECK_HIP(ctx, devAlloc(&nb.grads, total, L));
+    CHECK_HIP(ctx, devAlloc(&nb.adamM, total, L));
+    CHECK_HIP(ctx, devAlloc(&nb.adamV, total, L));
+    CHECK_HIP(ctx, devAlloc(&nb.gridBf16, (size_t)c->levels * T, L));
+    CHECK_HIP(ctx, devAlloc(&nb.wFwd, (size_t)kNrcLayers * kNrcWidth * kNrcWidth, L));
+    CHECK_HIP(ctx, devAlloc(&nb.wBwd, (size_t)kNrcLayers * kNrcWidth * kNrcWidth, L));
+    CHECK_HIP(ctx, devAlloc(&nb.acts, (size_t)kNrcLayers * nb.rowsCap * 64, L));
+    CHECK_HIP(ctx, devAlloc(&nb.dz, (size_t)kNrcLayers * nb.rowsCap * 64, L));
+    CHECK_HIP(ctx, devAlloc(&nb.pred, (size_t)nb.rowsCap * 4, L));
+    CHECK_HIP(ctx, devAlloc(&nb.dFeat, (size_t)nb.rowsCap * 32, L));
+    CHECK_HIP(ctx, devAlloc(&nb.partials, (size_t)(nb.rowsCap / 256) * kNrcLayers * kNrcWidth * kNrcWidth, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->nrcLoss, 1, L));
+    hipStream_t s = ctx->stream;
+    CHECK_HIP(ctx, hipMemsetAsync(nb.wFwd, 0, (size_t)kNrcLayers * kNrcWidth * kNrcWidth * 2, s));
+    CHECK_HIP(ctx, hipMemsetAsync(nb.wBwd, 0, (size_t)kNrcLayers * kNrcWidth * kNrcWidth * 2, s));
+    CHECK_HIP(ctx, hipMemsetAsync(nb.acts, 0, (size_t)kNrcLayers * nb.rowsCap * 64 * 2, s));
+    CHECK_HIP(ctx, hipMemsetAsync(nb.dz, 0, (size_t)kNrcLayers * nb.rowsCap * 64 * 2, s));
+    CHECK_HIP(ctx, hipMemsetAsync(ctx->nrcLoss, 0, sizeof(float), s));
+    // the table U(-1e-4, 1e-4) (instant-ngp), the MLP Glorot-uniform, the
+    // output layer's padding rows zero
+    std::vector<float> h(total, 0.0f);
+    uint64_t st = c->seed;
+    for (size_t i = 0; i < nb.gridParams; ++i) h[i] = (uniform01(st) * 2.0f - 1.0f) * 1e-4f;
+    for (int layer = 0; layer < kNrcLayers; ++layer) {
+        const int outs = layer < kNrcHidden ? kNrcWidth : 3;
+        const float bound = std::sqrt(6.0f / (float)(kNrcWidth + outs));
+        float *w = h.data() + nb.gridParams + (size_t)layer * kNrcWidth * kNrcWidth;
+        for (int o = 0; o < outs; ++o)
+            for (int k = 0; k < kNrcWidth; ++k) w[o * kNrcWidth + k] = (uniform01(st) * 2.0f - 1.0f) * bound;
+    }
+    CHECK_HIP(ctx, hipMemcpyAsync(nb.params, h.data(), total * sizeof(float), hipMemcpyHostToDevice, s));
+    nb.lr = c->learningRate;
+    nb.beta1 = c->beta1;
+    nb.beta2 = c->beta2;
+    nb.eps = c->epsilon;
+    ctx->nrc = nb;
+    int rc = nrcResetOptimizer(ctx);
+    if (rc != RTX_OK) return rc;
+    launchNrcPackWeights(s, ctx->nrc);
+    CHECK_HIP(ctx, hipGetLastError());
+    CHECK_HIP(ctx, hipStreamSynchronize(s));
+    ctx->nrcCfg = *c;
+    ctx->nrcValid = true;
+    return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+int rtx_nrc_destroy(rtx_ctx *ctx) try {
+    if (!ctx) return RTX_E_INVALID;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    nrcFree(ctx);
+    return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+int rtx_nrc_get_info(rtx_ctx *ctx, rtx_nrc_info *info) try {
+    if (!ctx || !info) return RTX_E_INVALID;
+    if (!ctx->nrcValid) return fail(ctx, RTX_E_STATE, "no learned radiance cache (rtx_nrc_create)");
+    std::memset(info, 0, sizeof(*info));
+    const NrcGrid &g = ctx->nrc.grid;
+    info->levels = g.levels;
+    info->tableSize = g.tableMask + 1u;
+    for (int l = 0; l < kNrcMaxLevels; ++l) {
+        info->resolution[l] = g.res[l];
+        info->dense[l] = g.dense[l];
+    }
+    for (int a = 0; a < 3; ++a) {
+        info->lo[a] = g.lo[a];
+        info->inv[a] = g.inv[a];
+    }
+    info->gridParams = ctx->nrc.gridParams;
+    info->mlpParams = (uint32_t)kNrcMlpParams;
+    info->steps = ctx->nrc.step;
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    float loss = 0.0f;
+    CHECK_HIP(ctx, hipMemcpyAsync(&loss, ctx->nrcLoss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    info->lastLoss = ctx->nrcLossSteps ? loss / (float)ctx->nrcLossSteps : 0.0f;
+    return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+int rtx_nrc_get_params(rtx_ctx *ctx, float *grid, size_t gridCount, float *mlp, size_t mlpCount) try {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->nrcValid) return fail(ctx, RTX_E_STATE, "no learned radiance cache (rtx_nrc_create)");
+    if ((grid && gridCount != ctx->nrc.gridParams) || (mlp && mlpCount != (size_t)kNrcMlpParams))
+        return fail(ctx, RTX_E_INVALID, "parameter counts differ from rtx_nrc_info");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (grid) CHECK_HIP(ctx, hipMemcpy(grid, ctx->nrc.params, gridCount * sizeof(float), hipMemcpyDeviceToHost));
+    if (mlp)
+        CHECK_HIP(ctx, hipMemcpy(mlp, ctx->nrc.params + ctx->nrc.gridParams, mlpCount * sizeof(float),
+                                 hipMemcpyDeviceToHost));
+    return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+int rtx_nrc_set_params(rtx_ctx *ctx, const float *grid, size_t gridCount, const float *mlp, size_t mlpCount) try {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->nrcValid) return fail(ctx, RTX_E_STATE, "no learned radiance cache (rtx_nrc_create)");
+    if ((grid && gridCount != ctx->nrc.gridParams) || (mlp && mlpCount != (size_t)kNrcMlpParams))
+        return fail(ctx, RTX_E_INVALID, "parameter counts differ from rtx_nrc_info");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (grid) CHECK_HIP(ctx, hipMemcpy(ctx->nrc.params, grid, gridCount * sizeof(float), hipMemcpyHostToDevice));
+    if (mlp) {
+        std::vector<float> w(mlp, mlp + mlpCount);
+        // the output layer's padding rows stay zero
+        for (size_t i = (size_t)kNrcHidden * kNrcWidth * kNrcWidth + 3 * kNrcWidth; i < w.size(); ++i) w[i] = 0.0f;
+        CHECK_HIP(ctx, hipMemcpy(ctx->nrc.params + ctx->nrc.gridParams, w.data(), mlpCount * sizeof(float),
+                                 hipMemcpyHostToDevice));
+    }
+    int rc = nrcResetOptimizer(ctx);
+    if (rc != RTX_OK) return rc;
+    launchNrcPackWeights(ctx->stream, ctx->nrc);
+    CHECK_HIP(ctx, hipGetLastError());
+    CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+int rtx_nrc_infer(rtx_ctx *ctx, const rtx_nrc_input *in, size_t n, float *rgb) try {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->nrcValid) return fail(ctx, RTX_E_STATE, "no learned radiance cache (rtx_nrc_create)");
+    if (n && (!in || !rgb)) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    if (n == 0) return RTX_OK;
+    if (n > (size_t)INT32_MAX) return fail(ctx, RTX_E_INVALID, "more than 2^31-1 queries");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<void *> tmp;
+    float4 *inD = nullptr, *outD = nullptr;
+    int rc = nrcUpload(ctx, in, nullptr, n, tmp, &inD, nullptr);
+    if (rc == RTX_OK) {
+        hipError_t e = devAlloc(&outD, n, tmp);
+        if (e == hipSuccess) {
+            launchNrcInfer(ctx->stream, ctx->nrc, inD, (uint32_t)n, nullptr, outD, nullptr, nullptr, 2 * ctx->cuCount);
+            e = hipGetLastError();
+        }
+        std::vector<float4> out(n);
+        if (e == hipSuccess) e = hipMemcpyAsync(out.data(), outD, n * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, RTX_E_DEVICE, std::string("rtx_nrc_infer: ") + hipGetErrorString(e));
+        else
+            for (size_t i = 0; i < n; ++i) {
+                rgb[3 * i] = out[i].x;
+                rgb[3 * i + 1] = out[i].y;
+                rgb[3 * i + 2] = out[i].z;
+            }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    freeAll(tmp);
+    return rc;
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+static int nrcTrainCall(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target, size_t n, float *gridGrad,
+                        float *mlpGrad, float *loss, bool step) {
+    if (!ctx) return RTX_E_INVALID;
+    if (!ctx->nrcValid) return fail(ctx, RTX_E_STATE, "no learned radiance cache (rtx_nrc_create)");
+    if (!in || !target || n == 0) return fail(ctx, RTX_E_INVALID, "no rows");
+    if (n > ctx->nrcCfg.trainBatch) return fail(ctx, RTX_E_INVALID, "more rows than trainBatch");
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<void *> tmp;
+    float4 *inD = nullptr, *tD = nullptr;
+    int rc = nrcUpload(ctx, in, target, n, tmp, &inD, &tD);
+    hipStream_t s = ctx->stream;
+    if (rc == RTX_OK) {
+        launchNrcZero(s, ctx->nrcLoss, 1);
+        ctx->nrcLossSteps = 1;
+        if (step) {
+            rc = nrcStep(ctx, inD, tD, (uint32_t)n);
+        } else {
+            launchNrcGradients(s, ctx->nrc, inD, tD, (uint32_t)n, 1.0f / (3.0f * (float)n), ctx->nrcLoss, ctx->cuCount);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess && gridGrad)
+                e = hipMemcpyAsync(gridGrad, ctx->nrc.grads, ctx->nrc.gridParams * sizeof(float), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess && mlpGrad)
+                e = hipMemcpyAsync(mlpGrad, ctx->nrc.grads + ctx->nrc.gridParams, kNrcMlpParams * sizeof(float),
+                                   hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) launchNrcZeroGrads(s, ctx->nrc); // gradients are zero between calls
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e != hipSuccess) rc = fail(ctx, RTX_E_DEVICE, std::string("rtx_nrc_gradients: ") + hipGetErrorString(e));
+        }
+        float l = 0.0f;
+        if (rc == RTX_OK) {
+            hipError_t e = hipMemcpyAsync(&l, ctx->nrcLoss, sizeof(float), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) rc = fail(ctx, RTX_E_DEVICE, std::string("rtx_nrc_train: ") + hipGetErrorString(e));
+        }
+        if (loss) *loss = l;
+    }
+    (void)hipStreamSynchronize(s);
+    freeAll(tmp);
+    return rc;
+}
+
+int rtx_nrc_gradients(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target, size_t n, float *gridGrad,
+                      float *mlpGrad, float *loss) try {
+    return nrcTrainCall(ctx, in, target, n, gridGrad, mlpGrad, loss, false);
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+int rtx_nrc_train(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target, size_t n, float *loss) try {
+    return nrcTrainCall(ctx, in, target, n, nullptr, nullptr, loss, true);
 } catch (...) {
     return rtx::abiFail(ctx);
 }

@@ -95,6 +95,10 @@ struct FrameConst {
     // frame's next camera sample or queued split (segments)
     uint32_t samplesPerFrame, segments;
     uint32_t useADRRS, splitOnFirst; // pc's, for camera paths only
+    // learned radiance cache (RTX_FRAME_NRC): paths end into the cache at the
+    // first non-specular vertex of depth >= nrcDepth; pixels with
+    // (pixel + frame offset) % nrcTrainEvery == 0 are training paths
+    uint32_t nrc, nrcDepth, nrcTrainEvery, nrcTrainSlots;
 };
 
 // ------------------------------------------------------------ path guiding
@@ -248,6 +252,14 @@ struct PathBuffers {
                       // (wi, texture u), (throughput, texture v)  (splitInfo, raytrace.rgen:70-80)
     const float4 *estimate; // per pixel: the ADRRS estimate (estimateImage)
     uint32_t *frameSeed;    // per path: the frame's randomUInt (any-hit alpha test seeds)
+    // learned radiance cache: the queries of paths that end into it (4 float4
+    // of rtx_nrc_input each; aux = (throughput, path id bits)), training
+    // vertices by slot (input, and target rgb with w = 1 once the path ended),
+    // and per path (throughput at the training vertex, slot bits), (radiance so far)
+    float4 *nrcQuery, *nrcQueryAux;
+    uint32_t *nrcQueryCount;
+    float4 *nrcTrainIn, *nrcTrainTarget;
+    float4 *nrcPath;
 };
 constexpr int kMaxSplits = 10; // MAX_SPLITS, raytrace.rgen:82
 constexpr int kMaxRecords = RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL;
@@ -262,6 +274,7 @@ constexpr uint32_t kFlagPending = 1u << 18;       // NEE contributions waiting t
 constexpr uint32_t kFlagCandShift = 19;           // bits 19-21 irradiance-cache candidates
 constexpr uint32_t kFlagCandMask = 7u << kFlagCandShift;
 constexpr uint32_t kFlagPendTask = 1u << 22;      // the pending NEE terms belong to main()'s result (a split vertex)
+constexpr uint32_t kFlagNrcTrain = 1u << 23;      // a training path past its training vertex
 // Extend-queue id bit: the entry is a queued split to resume (its ray is
 // empty); kShade takes the vertex from the path's split record.
 constexpr uint32_t kSplitVertex = 1u << 31;

@@ -93,6 +93,44 @@ RTX_DI void cameraRay(const FrameConst &fc, uint32_t pixel, Rng &rng, V3 &o, V3 
     d = normalize(xform(fc.cam.viewInverse, normalize(target), 0.0f));
 }
 
+// ------------------------------------------------ learned radiance cache
+// The training slot of path pid, or -1 for a path that ends into the cache:
+// pixels with (pixel + offset) % every == 0 train, the offset moving with
+// the sample so that every pixel trains now and then.
+RTX_DI int nrcSlot(const FrameConst &fc, BatchDesc b, uint32_t pid) {
+    const uint64_t g = b.firstPath + pid;
+    const uint32_t pixel = (uint32_t)(g % b.npix);
+    const uint64_t sample = g / b.npix + fc.sampleOffset;
+    const uint32_t every = fc.nrcTrainEvery;
+    const uint32_t k = pixel + (uint32_t)((sample * 7919u) % every);
+    if (k % every != 0u) return -1;
+    const uint32_t slot = k / every;
+    return slot < fc.nrcTrainSlots ? (int)slot : -1;
+}
+// The cache's surface inputs: roughness 1 for Lambert, alpha for rough
+// conductors, sqrt(2 / (n + 2)) for Phong lobes (the Beckmann width of a
+// Phong exponent), 0 for specular; albedo the reflectance colour.
+RTX_DI void nrcSurface(const Mat &m, float &rough, V3 &albedo) {
+    switch (m.type) {
+    case RTX_MAT_DIFFUSE:
+    case RTX_MAT_LIGHT:
+        rough = 1.0f;
+        albedo = m.kd;
+        break;
+    case RTX_MAT_PHONG:
+        rough = sqrtf(2.0f / (m.highlight + 2.0f));
+        albedo = m.kd + m.ks;
+        break;
+    case RTX_MAT_ROUGH_CONDUCTOR:
+        rough = m.roughness;
+        albedo = v3(fresnelConductor(1.0f, m.eta, m.k));
+        break;
+    default:
+        rough = 0.0f;
+        albedo = m.specular;
+    }
+}
+
 // State of a path at the start of a raytrace() call from a camera sample:
 // depth 0, followCount 0, follow = true, addNextDirectLights = addFirstHitLight = true.
 constexpr uint32_t kFlagsCameraStart = kFlagFollow | kFlagAddNext;
@@ -105,7 +143,10 @@ __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ 
                                                  Queue q0, Counters *cnt, int qsel, uint32_t *electCounter) {
     const FrameConst &fc = *fcp;
     uint32_t pid = blockIdx.x * 256 + threadIdx.x;
-    if (pid == 0) cnt->extend[qsel] = b.numPaths;
+    if (pid == 0) {
+        cnt->extend[qsel] = b.numPaths;
+        if (fc.nrc) *pb.nrcQueryCount = 0u;
+    }
     const bool active = pid < b.numPaths;
     uint64_t g = b.firstPath + pid;
     uint32_t pixel = (uint32_t)(g % b.npix);
@@ -139,6 +180,10 @@ __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ 
     }
     pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagsCameraStart));
     pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
+    if (fc.nrc) {
+        const int slot = nrcSlot(fc, b, pid);
+        if (slot >= 0) pb.nrcTrainTarget[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); // no target until the path ends
+    }
 }
 
 // The sub-paths of calculateCacheData (raytrace.rgen:1246-1263) for jobs
@@ -432,7 +477,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
         m.type = -1;
         int depth = 0, followCount = 0;
         bool follow = false, addNext = false, icHit = false;
-        bool splitV = false, pendTask = false, dead = false, taskDirty = false;
+        bool splitV = false, pendTask = false, dead = false, taskDirty = false, nrcQ = false;
         int4 task = make_int4(0, 0, 0, 0); // (camera samples left, splits queued, next split, record sampleOffset)
         uint32_t ncand = 0;
         int4 grec = make_int4(0, -1, 0, 0); // record state: slot, distance slot, distanceFactor bits, NEE base
@@ -535,6 +580,26 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                     addNext = false;
                     follow = false;
                     bool nee = useNEE;
+                    if (fc.nrc && depth >= (int)fc.nrcDepth && !(flags & kFlagNrcTrain)) {
+                        const int slot = nrcSlot(fc, b, pid);
+                        if (slot < 0) { // the path ends into the learned cache here, without NEE
+                            nrcQ = true;
+                            isDone = true;
+                            nee = false;
+                        } else { // a training path: what it adds from here on is this vertex's target
+                            float rough;
+                            V3 alb;
+                            nrcSurface(m, rough, alb);
+                            float4 *ti = pb.nrcTrainIn + 4 * (size_t)slot;
+                            ti[0] = f4(h.worldPos, rough);
+                            ti[1] = f4(-d, 0.0f);
+                            ti[2] = f4(h.normal, 0.0f);
+                            ti[3] = f4(alb, 0.0f);
+                            pb.nrcPath[2 * (size_t)pid] = f4(thr, __int_as_float(slot));
+                            pb.nrcPath[2 * (size_t)pid + 1] = f4(res, 0.0f);
+                            flags |= kFlagNrcTrain;
+                        }
+                    }
                     if ((useIC || (useADRRS && depth > 1)) && isICCapable(m.type, icOnGlossy)) { // raytrace.rgen:1080-1139
                         V3 irr;
                         if (icQuery(IC, icp, h.worldPos, h.normal, irr)) {
@@ -665,6 +730,20 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
                 qp.aux[pi] = f4(probeBase, pdfMatProbe);
             }
         }
+        if (fc.nrc) { // the learned cache's queries, in the order the waves reach them
+            const int qn = waveAppend(nrcQ, pb.nrcQueryCount);
+            if (nrcQ) {
+                float rough;
+                V3 alb;
+                nrcSurface(m, rough, alb);
+                float4 *q = pb.nrcQuery + 4 * (size_t)qn;
+                q[0] = f4(h.worldPos, rough);
+                q[1] = f4(-d, 0.0f);
+                q[2] = f4(h.normal, 0.0f);
+                q[3] = f4(alb, 0.0f);
+                pb.nrcQueryAux[qn] = f4(thr, __uint_as_float(pid));
+            }
+        }
         // ---- continuation, raytrace.rgen:1169-1215
         V3 newDir = v3(0.0f);
         if (icHit) isDone = true;
@@ -734,7 +813,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevG
             uint32_t nf = (uint32_t)depth | ((uint32_t)followCount << kFlagFollowCountShift) |
                           (follow ? kFlagFollow : 0u) | (addNext ? kFlagAddNext : 0u) |
                           (doNee ? kFlagPending : 0u) | (doNee && pendTask ? kFlagPendTask : 0u) |
-                          (ncand << kFlagCandShift);
+                          (ncand << kFlagCandShift) | (flags & kFlagNrcTrain);
             if (taskDirty) pb.task[pid] = task;
             pb.rng[pid] = rng.s;
             pb.hash[pid] = hash;
@@ -804,6 +883,13 @@ __global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fc
                 } else {
                     res = res + nee;
                 }
+            }
+            if (flags & kFlagNrcTrain) { // the training vertex's target: what the path added after it
+                const float4 tp = pb.nrcPath[2 * (size_t)pid], r0 = pb.nrcPath[2 * (size_t)pid + 1];
+                const V3 add = res - v3(r0);
+                const V3 t = v3(tp.x != 0.0f ? add.x / tp.x : 0.0f, tp.y != 0.0f ? add.y / tp.y : 0.0f,
+                                tp.z != 0.0f ? add.z / tp.z : 0.0f);
+                pb.nrcTrainTarget[__float_as_int(tp.w)] = f4(t, 1.0f);
             }
             if (record) { // commitSamples
                 rtx_directional_data *dd = recordSlots(pb, pid);

@@ -124,6 +124,17 @@ def device_lib():
         L.rtx_schedule_get_state.argtypes = [C.c_void_p, C.POINTER(abi.ScheduleState)]
         L.rtx_estimate_write.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
         L.rtx_estimate_read.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]
+        fp = C.POINTER(C.c_float)
+        L.rtx_nrc_config_defaults.argtypes = [C.POINTER(abi.NrcConfig)]
+        L.rtx_nrc_config_defaults.restype = None
+        L.rtx_nrc_create.argtypes = [C.c_void_p, C.POINTER(abi.NrcConfig)]
+        L.rtx_nrc_destroy.argtypes = [C.c_void_p]
+        L.rtx_nrc_get_info.argtypes = [C.c_void_p, C.POINTER(abi.NrcInfo)]
+        L.rtx_nrc_get_params.argtypes = [C.c_void_p, fp, C.c_size_t, fp, C.c_size_t]
+        L.rtx_nrc_set_params.argtypes = [C.c_void_p, fp, C.c_size_t, fp, C.c_size_t]
+        L.rtx_nrc_infer.argtypes = [C.c_void_p, C.POINTER(abi.NrcInput), C.c_size_t, fp]
+        L.rtx_nrc_gradients.argtypes = [C.c_void_p, C.POINTER(abi.NrcInput), fp, C.c_size_t, fp, fp, fp]
+        L.rtx_nrc_train.argtypes = [C.c_void_p, C.POINTER(abi.NrcInput), fp, C.c_size_t, fp]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         L.rtx_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         _dev = L
@@ -546,6 +557,77 @@ class Renderer:
         for k, v in kw.items():
             setattr(t, k, v)
         self._check(device_lib().rtx_ctx_set_tuning(self._h, C.byref(t)))
+
+    # --------------------------------------------- learned radiance cache
+    def nrc_create(self, scene_min, scene_max, **kw):
+        """rtx_nrc_create with the defaults of rtx_nrc_config_defaults, the
+        encoded box and any config fields given as keywords."""
+        c = abi.NrcConfig()
+        device_lib().rtx_nrc_config_defaults(C.byref(c))
+        c.sceneMin[:] = [float(x) for x in scene_min]
+        c.sceneMax[:] = [float(x) for x in scene_max]
+        for k, v in kw.items():
+            setattr(c, k, v)
+        self._check(device_lib().rtx_nrc_create(self._h, C.byref(c)))
+        return c
+
+    def nrc_destroy(self):
+        self._check(device_lib().rtx_nrc_destroy(self._h))
+
+    def nrc_info(self):
+        info = abi.NrcInfo()
+        self._check(device_lib().rtx_nrc_get_info(self._h, C.byref(info)))
+        return info
+
+    def nrc_params(self):
+        """(table (levels, tableSize, 2), mlp (kNrcMlpParams,)) FP32 masters."""
+        info = self.nrc_info()
+        grid = np.empty(int(info.gridParams), np.float32)
+        mlp = np.empty(int(info.mlpParams), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(device_lib().rtx_nrc_get_params(self._h, grid.ctypes.data_as(fp), grid.size,
+                                                     mlp.ctypes.data_as(fp), mlp.size))
+        return grid.reshape(info.levels, info.tableSize, 2), mlp
+
+    def nrc_set_params(self, grid=None, mlp=None):
+        fp = C.POINTER(C.c_float)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32).reshape(-1)
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32).reshape(-1)
+        self._check(device_lib().rtx_nrc_set_params(self._h, None if g is None else g.ctypes.data_as(fp),
+                                                     0 if g is None else g.size,
+                                                     None if m is None else m.ctypes.data_as(fp),
+                                                     0 if m is None else m.size))
+
+    def nrc_infer(self, inputs):
+        """inputs: (n, 16) float32 rows laid out as rtx_nrc_input; returns (n, 3)."""
+        x = np.ascontiguousarray(inputs, np.float32)
+        out = np.empty((len(x), 3), np.float32)
+        self._check(device_lib().rtx_nrc_infer(self._h, x.ctypes.data_as(C.POINTER(abi.NrcInput)), len(x),
+                                                out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def nrc_gradients(self, inputs, targets):
+        """(loss, table gradient, mlp gradient) without a parameter change."""
+        x = np.ascontiguousarray(inputs, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        info = self.nrc_info()
+        gg = np.empty(int(info.gridParams), np.float32)
+        mg = np.empty(int(info.mlpParams), np.float32)
+        loss = C.c_float()
+        fp = C.POINTER(C.c_float)
+        self._check(device_lib().rtx_nrc_gradients(self._h, x.ctypes.data_as(C.POINTER(abi.NrcInput)),
+                                                    t.ctypes.data_as(fp), len(x), gg.ctypes.data_as(fp),
+                                                    mg.ctypes.data_as(fp), C.byref(loss)))
+        return loss.value, gg.reshape(info.levels, info.tableSize, 2), mg
+
+    def nrc_train(self, inputs, targets):
+        """One Adam step; returns the loss before it."""
+        x = np.ascontiguousarray(inputs, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        loss = C.c_float()
+        self._check(device_lib().rtx_nrc_train(self._h, x.ctypes.data_as(C.POINTER(abi.NrcInput)),
+                                                t.ctypes.data_as(C.POINTER(C.c_float)), len(x), C.byref(loss)))
+        return loss.value
 
     # ------------------------------------------------------ guiding
     def guiding_set(self, regions, vmms):

@@ -20,6 +20,7 @@ RTX_FRAME_RESET_ACCUM = 0x4
 RTX_FRAME_TIME_KERNELS = 0x8
 RTX_FRAME_NO_GUIDING_UPDATE = 0x10
 RTX_FRAME_TIME_TRACE = 0x20
+RTX_FRAME_NRC = 0x40
 
 MAT_DIFFUSE, MAT_SPECULAR, MAT_DIELECTRIC, MAT_LIGHT, MAT_PHONG, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR = range(7)
 LIGHT_AREA, LIGHT_POINT, LIGHT_SPHERE, LIGHT_ENV_MAP = range(4)
@@ -194,6 +195,25 @@ class ScheduleState(C.Structure):
                 ("evalCurrentSamples", C.c_int32), ("evalSecondsTotalSamples", C.c_int64),
                 ("evalSecondsResult", C.c_int64), ("evalSamplesMs", C.c_double), ("guidingUpdateMs", C.c_double),
                 ("pc", PushConstant)]
+
+
+class NrcConfig(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("log2TableSize", C.c_uint32), ("baseResolution", C.c_float),
+                ("maxResolution", C.c_float), ("learningRate", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("epsilon", C.c_float), ("trainBatch", C.c_uint32), ("trainSteps", C.c_uint32),
+                ("terminateBounces", C.c_uint32), ("trainingPaths", C.c_uint32), ("sceneMin", F3),
+                ("sceneMax", F3), ("seed", C.c_uint64)]
+
+
+class NrcInput(C.Structure):
+    _fields_ = [("pos", F3), ("roughness", C.c_float), ("dir", F3), ("_pad0", C.c_float), ("normal", F3),
+                ("_pad1", C.c_float), ("albedo", F3), ("_pad2", C.c_float)]
+
+
+class NrcInfo(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("tableSize", C.c_uint32), ("resolution", C.c_float * 16),
+                ("dense", C.c_uint32 * 16), ("lo", F3), ("inv", F3), ("gridParams", C.c_uint64),
+                ("mlpParams", C.c_uint32), ("steps", C.c_uint32), ("lastLoss", C.c_float)]
 
 
 class GuidingOptions(C.Structure):

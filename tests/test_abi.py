@@ -19,7 +19,8 @@ C_NAMES = {
     abi.FrameParams: "rtx_frame_params", abi.FrameStats: "rtx_frame_stats", abi.SceneInfo: "rtx_scene_info",
     abi.Ray: "rtx_ray", abi.Hit: "rtx_hit", abi.Tuning: "rtx_tuning", abi.CacheHeader: "rtx_cache_header",
     abi.CacheData: "rtx_cache_data", abi.IcJob: "rtx_ic_job", abi.Aabb: "rtx_aabb", abi.VmfTheta: "rtx_vmf_theta",
-    abi.VmmTheta: "rtx_vmm_theta",
+    abi.VmmTheta: "rtx_vmm_theta", abi.NrcConfig: "rtx_nrc_config", abi.NrcInput: "rtx_nrc_input",
+    abi.NrcInfo: "rtx_nrc_info",
 }
 
 

@@ -1,0 +1,214 @@
+"""The learned radiance cache's BF16 MFMA kernels against a plain PyTorch
+FP32 restatement (tests/nrc_reference.py), through the C ABI:
+
+* the hash-grid encoding exactly (an MLP of identity layers copies chosen
+  features to the output, so every level's indexing and trilinear weights
+  are compared bit for bit);
+* the forward pass of random parameters;
+* the loss and every parameter gradient (the MLP's six matrices and the
+  table) against autograd;
+* one Adam step against its formula;
+* training on a smooth function lowers the loss tenfold;
+* in a render, training paths are the plain tracer's own paths (a frame
+  whose every path trains equals the plain render bit for bit), and a
+  trained cache renders the Cornell box close to the plain image."""
+import numpy as np
+import pytest
+import torch
+
+import rtxamd
+import scenes
+from nrc_reference import HIDDEN, OUT_PAD, WIDTH, Reference, bf16, random_inputs
+from parity import log, relmse
+from rtxamd import abi
+
+pytestmark = pytest.mark.gpu
+
+LO, HI = (-1.0, -2.0, 0.0), (3.0, 2.0, 1.5)
+
+
+@pytest.fixture(scope="module")
+def cache():
+    r = rtxamd.Renderer(0)
+    r.nrc_create(LO, HI, seed=5)
+    yield r
+    r.close()
+
+
+def _random_params(r, rng, grid_scale=0.5):
+    info = r.nrc_info()
+    grid = rng.uniform(-grid_scale, grid_scale, (info.levels, info.tableSize, 2)).astype(np.float32)
+    mlp = np.zeros(info.mlpParams, np.float32)
+    for l in range(HIDDEN):
+        b = np.sqrt(6.0 / (2 * WIDTH))
+        mlp[l * WIDTH * WIDTH:(l + 1) * WIDTH * WIDTH] = rng.uniform(-b, b, WIDTH * WIDTH)
+    mlp[HIDDEN * WIDTH * WIDTH:HIDDEN * WIDTH * WIDTH + 3 * WIDTH] = rng.uniform(-0.3, 0.3, 3 * WIDTH)
+    r.nrc_set_params(grid, mlp)
+    return grid, mlp
+
+
+def test_encoding_is_exact(cache):
+    """Identity hidden layers and an output layer that picks three input
+    columns give back BF16(feature) of those columns exactly."""
+    info = cache.nrc_info()
+    log(f"nrc: {info.levels} levels x {info.tableSize} entries, resolutions "
+        f"{[int(x) for x in info.resolution[:info.levels]]}, dense {list(info.dense[:info.levels])}")
+    rng = np.random.default_rng(1)
+    grid = rng.uniform(0.0, 1.0, (info.levels, info.tableSize, 2)).astype(np.float32)  # features >= 0 pass the ReLUs
+    x = random_inputs(4000, LO, HI, rng)
+    # positions exactly on the box's faces and corners
+    x[:8, 0:3] = np.array([[HI[a] if (c >> a) & 1 else LO[a] for a in range(3)] for c in range(8)])
+    ref = Reference(info)
+    enc = ref.encode(torch.from_numpy(x), torch.from_numpy(grid)).detach().numpy()
+    for cols in ([0, 1, 2], [3, 8, 13], [17, 22, 27], [29, 30, 31], [32, 35, 38], [39, 41, 42]):
+        mlp = np.zeros(info.mlpParams, np.float32)
+        for l in range(HIDDEN):
+            mlp[l * WIDTH * WIDTH:(l + 1) * WIDTH * WIDTH] = np.eye(WIDTH, dtype=np.float32).reshape(-1)
+        out = mlp[HIDDEN * WIDTH * WIDTH:].reshape(OUT_PAD, WIDTH)
+        for o, c in enumerate(cols):
+            out[o, c] = 1.0
+        cache.nrc_set_params(grid, mlp)
+        got = cache.nrc_infer(x)
+        want = np.maximum(enc[:, cols], 0.0)  # the ReLUs of the identity layers
+        bad = np.flatnonzero((got != want).any(1))
+        assert len(bad) == 0, (cols, bad[:5], got[bad[:3]], want[bad[:3]])
+
+
+def test_forward_matches_torch(cache):
+    rng = np.random.default_rng(2)
+    grid, mlp = _random_params(cache, rng)
+    x = random_inputs(5000, LO, HI, rng)
+    got = cache.nrc_infer(x)
+    ref = Reference(cache.nrc_info())
+    want = ref.forward(torch.from_numpy(x), torch.from_numpy(grid), torch.from_numpy(mlp)).detach().numpy()
+    err = np.abs(got - want).max() / np.abs(want).max()
+    log(f"nrc forward: max |gpu - torch| / max |torch| = {err:.2e} (output scale {np.abs(want).max():.3f})")
+    assert err < 1e-2
+    assert np.corrcoef(got.reshape(-1), want.reshape(-1))[0, 1] > 0.9999
+
+
+def test_gradients_match_torch_autograd(cache):
+    rng = np.random.default_rng(3)
+    grid, mlp = _random_params(cache, rng)
+    n = 4096
+    x = random_inputs(n, LO, HI, rng)
+    target = rng.uniform(0.0, 2.0, (n, 3)).astype(np.float32)
+    loss, ggrid, gmlp = cache.nrc_gradients(x, target)
+    ref = Reference(cache.nrc_info())
+    tg = torch.from_numpy(grid).requires_grad_(True)
+    tm = torch.from_numpy(mlp).requires_grad_(True)
+    lt = Reference.loss(ref.forward(torch.from_numpy(x), tg, tm), torch.from_numpy(target))
+    lt.backward()
+    log(f"nrc loss gpu {loss:.6f} torch {lt.item():.6f}")
+    assert loss == pytest.approx(lt.item(), rel=2e-3)
+    want = tm.grad.numpy()
+    for l, (a, b) in enumerate(zip(Reference.weights(gmlp), Reference.weights(want))):
+        e = np.linalg.norm(a - b) / np.linalg.norm(b)
+        log(f"  layer {l}: relative gradient error {e:.2e}")
+        assert e < 5e-2, l
+    wg = tg.grad.numpy()
+    e = np.linalg.norm(ggrid - wg) / np.linalg.norm(wg)
+    log(f"  table: relative gradient error {e:.2e} ({int((wg != 0).sum())} entries touched)")
+    assert e < 5e-2
+    # entries no corner touched get no gradient
+    assert np.all(ggrid[wg == 0] == 0)
+    # the padding rows of the output layer stay out of training
+    assert np.all(gmlp[HIDDEN * WIDTH * WIDTH + 3 * WIDTH:] == 0)
+
+
+def test_adam_step_matches_formula(cache):
+    rng = np.random.default_rng(4)
+    grid, mlp = _random_params(cache, rng)
+    n = 2048
+    x = random_inputs(n, LO, HI, rng)
+    target = rng.uniform(0.0, 2.0, (n, 3)).astype(np.float32)
+    _, ggrid, gmlp = cache.nrc_gradients(x, target)
+    cache.nrc_train(x, target)
+    g1, m1 = cache.nrc_params()
+    assert cache.nrc_info().steps == 1
+    lr, b1, b2, eps = 1e-2, 0.9, 0.99, 1e-15
+    for p0, g, p1 in ((grid.reshape(-1), ggrid.reshape(-1), g1.reshape(-1)), (mlp, gmlp, m1)):
+        m = (1 - b1) * g.astype(np.float64)
+        v = (1 - b2) * g.astype(np.float64) ** 2
+        want = p0 - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+        big = np.abs(g) > 1e-6 * np.abs(g).max()
+        assert np.allclose(p1[big], want[big], rtol=0, atol=2e-6)
+        assert np.array_equal(p1[g == 0], p0[g == 0])
+
+
+def test_training_lowers_the_loss(cache):
+    """200 Adam steps on a smooth radiance-like function of position, normal
+    and albedo lower the loss at least tenfold."""
+    cache.nrc_create(LO, HI, seed=7)
+    rng = np.random.default_rng(5)
+
+    def f(x):
+        p = x[:, 0:3]
+        base = 0.6 + 0.4 * np.sin(2.0 * p[:, 0:1]) * np.cos(1.5 * p[:, 1:2]) + 0.2 * x[:, 10:11]
+        return (base * (0.3 + x[:, 12:15])).astype(np.float32)
+
+    losses = []
+    for it in range(200):
+        x = random_inputs(4096, LO, HI, rng, margin=0.0)
+        losses.append(cache.nrc_train(x, f(x)))
+    first, last = np.mean(losses[:5]), np.mean(losses[-10:])
+    log(f"nrc training: loss {first:.4f} -> {last:.5f}")
+    assert last < first / 10
+    x = random_inputs(4096, LO, HI, rng, margin=0.0)
+    pred = cache.nrc_infer(x)
+    assert relmse(pred, f(x)) < 0.05
+
+
+def test_training_paths_are_the_plain_paths():
+    """With one training path per pixel no path ends into the cache: the frame
+    is the plain tracer's, path for path."""
+    hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL, 1)
+    W = H = 64
+    with rtxamd.Renderer(0) as r:
+        ds = r.upload(hs)
+        info = ds.info()
+        r.nrc_create(list(info.boundsMin), list(info.boundsMax), trainingPaths=W * H, trainBatch=4096, trainSteps=1)
+        out = {}
+        for flags in (0, abi.RTX_FRAME_NRC):
+            fp = scenes.params(hs, W, H, 2, maxDepth=8, enableMIS=1,
+                               flags=flags | abi.RTX_FRAME_KEEP_PATHS | abi.RTX_FRAME_RESET_ACCUM)
+            r.render(ds, fp)
+            out[flags] = r.read_paths(W * H * 2)
+        assert r.nrc_info().steps == 2  # one step per frame
+        ds.close()
+    assert np.array_equal(out[0][0], out[abi.RTX_FRAME_NRC][0])
+    assert np.array_equal(out[0][1], out[abi.RTX_FRAME_NRC][1])
+
+
+def test_trained_cache_renders_close_to_plain():
+    """Cornell box, paths ending into the cache after 2 bounces: after 256
+    training frames (4 steps each) the cache's 16-spp image has the plain
+    image's mean within 10%, and its per-pixel error against a 1024-spp plain
+    image is of the order of the plain tracer's at the same sample count
+    (the equal-time comparison on the room is in the benchmark)."""
+    hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL, 1)
+    W = H = 96
+    with rtxamd.Renderer(0) as r:
+        ds = r.upload(hs)
+        info = ds.info()
+        pc = dict(maxDepth=8, enableMIS=1)
+        r.render(ds, scenes.params(hs, W, H, 1024, **pc))
+        ref = r.read_image(W, H)[..., :3]
+        r.render(ds, scenes.params(hs, W, H, 16, sample_offset=5000, **pc))
+        plain16 = r.read_image(W, H)[..., :3]
+        r.nrc_create(list(info.boundsMin), list(info.boundsMax), trainingPaths=4096, trainBatch=1024,
+                     trainSteps=4, seed=3)
+        r.render(ds, scenes.params(hs, W, H, 256, sample_offset=10000, flags=abi.RTX_FRAME_NRC, **pc))
+        ni = r.nrc_info()
+        r.render(ds, scenes.params(hs, W, H, 16, sample_offset=20000, flags=abi.RTX_FRAME_NRC | abi.RTX_FRAME_RESET_ACCUM,
+                                   **pc))
+        nrc16 = r.read_image(W, H)[..., :3]
+        ds.close()
+    e_plain, e_nrc = relmse(plain16, ref), relmse(nrc16, ref)
+    m_ref, m_nrc = ref.mean(), nrc16.mean()
+    log(f"nrc render: {ni.steps} steps, last loss {ni.lastLoss:.4f}; 16 spp relmse plain {e_plain:.4f} "
+        f"cache {e_nrc:.4f}; mean {m_nrc:.4f} vs {m_ref:.4f}")
+    assert np.isfinite(nrc16).all()
+    assert ni.steps == 256 * 4
+    assert abs(m_nrc - m_ref) < 0.1 * m_ref
+    assert e_nrc < 2 * e_plain
