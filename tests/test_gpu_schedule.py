@@ -105,9 +105,13 @@ def test_command_line_renders_with_cache_and_guiding(tmp_path):
     is created from IC_SIZE and the guiding regions from GUIDING_SPLITS
     (src/RayTracingApp.cpp:44-46, 346-351), the schedule runs the prepare
     frames, the guiding cadence and eval-by-samples, and the EXR holds the
-    accumulated image: 4 samples per pixel, finite, lit like a plain render
-    of the same scene (the cache's interpolation bias is small next to a 4-spp
-    image's noise at this scale)."""
+    accumulated image, finite, lit like a plain render of the same scene (the
+    cache's interpolation bias is small next to a 4-spp image's noise at this
+    scale).  --spp 4 accumulates 3 samples per pixel: the reference's eval
+    counter checks currentPrepareFrames after raytrace() has already counted
+    the last prepare frame (src/RayTracingApp.cpp:1131-1142, 160-163), so
+    that frame's sample is counted though it is not accumulated, and six
+    frames run: three prepare frames, three accumulated ones."""
     import subprocess
     from pathlib import Path
     exe = Path(rtxamd.LIB_DIR).parent / "bin" / "rtx_pathtracer"
@@ -117,7 +121,8 @@ def test_command_line_renders_with_cache_and_guiding(tmp_path):
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     log(r.stdout.strip())
     assert r.returncode == 0, r.stderr
-    files = sorted(tmp_path.glob("gen_cornell_4spp_*ms.exr"))
+    assert ", 6 frames, 3 spp accumulated" in r.stdout
+    files = sorted(tmp_path.glob("gen_cornell_3spp_*ms.exr"))
     assert len(files) == 1, list(tmp_path.iterdir())
     img = rtxamd.read_exr(files[0])
     assert img.shape == (48, 64, 4) and np.isfinite(img).all() and (img[..., :3] >= 0).all()
