@@ -430,11 +430,15 @@ RTX_DI void pushSplit(const PathBuffers &pb, uint32_t pid, int4 &task, const Hit
 }
 
 // ----------------------------------------------------------------- shade
+// Register budget: two waves per SIMD (at most 256 VGPRs and AGPRs
+// together).  Without the bound the compiler reaches 254 VGPRs plus 4 AGPRs
+// and the kernel drops to one wave per SIMD, which costs the 1080p cache
+// workload about 16% (2.35 s per image instead of 2.74 s).
 // One iteration of the bounce loop, raytrace.rgen:1008-1215, for the paths of
 // the extend queue; an entry marked kSplitVertex resumes a queued split
 // instead (raytrace.rgen:1684-1703: NEE at its vertex, a new direction, then
 // the resumed path at the split's depth).
-__global__ __launch_bounds__(kShadeBlock) void kShade(DevScene S, DevIC IC, DevGuiding G,
+__global__ __launch_bounds__(kShadeBlock, 2) void kShade(DevScene S, DevIC IC, DevGuiding G,
                                                       const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
                                                       Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done,
                                                       Counters *cnt, int qsel) {
