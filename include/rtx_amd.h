@@ -586,6 +586,19 @@ int rtx_trace_closest(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
 int rtx_trace_occluded(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
                        size_t n, uint8_t *occluded);
 
+/* Allocation-free closest-hit query: rtx_trace_closest for at most
+ * RTX_NOALLOC_MAX_RAYS rays, staged through a fixed device array of the
+ * library (3 MiB + 256 B per device, held until the process ends) instead of
+ * per-call device allocations; it calls no hipMalloc or hipFree.  Rays past
+ * the limit are refused with RTX_E_INVALID.  Calls that overlap on one device
+ * share that array and race; each call finishes (stream synchronised) before
+ * it returns.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+#define RTX_NOALLOC_MAX_RAYS 65536
+int rtx_trace_closest_noalloc(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
+                              size_t n, rtx_hit *hits);
+
 /* ------------------------------------------------ learned radiance cache */
 /* A neural radiance cache (Mueller, Rousselle, Novak and Keller 2021) of the
  * reflected radiance leaving a surface point: a multiresolution hash-grid

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -118,6 +119,9 @@ struct rtx_ctx {
     float *nrcLoss = nullptr;   // device: summed loss of the last steps
     uint32_t nrcLossSteps = 0;
     size_t nrcPathCap = 0, nrcSlotCap = 0;
+    // device address of the allocation-free query's scratch array (rtx_trace_closest_noalloc)
+    void *scratch = nullptr;
+    std::string scratchErr;
 };
 
 namespace {
@@ -383,6 +387,8 @@ EventPair *nextEvents(rtx_ctx *ctx, int cls) {
 
 } // namespace
 
+static void *noAllocScratch(int device, hipError_t &err);
+
 namespace rtx {
 int abiStatus() noexcept {
     try {
@@ -512,6 +518,10 @@ int rtx_ctx_create(int device, rtx_ctx **out) try {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->cuCount = prop.multiProcessorCount;
+    // the allocation-free query's scratch on this device (looked up once per device)
+    hipError_t se = hipSuccess;
+    ctx->scratch = noAllocScratch(device, se);
+    if (!ctx->scratch) ctx->scratchErr = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
     *out = ctx;
     return RTX_OK;
 } catch (...) {
@@ -2082,6 +2092,43 @@ int rtx_nrc_train(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target, si
     return rtx::abiFail(ctx);
 }
 
+// ------------------------------------------------ allocation-free query
+// Device memory of rtx_trace_closest_noalloc: one call's rays, hits and the
+// traversal's work counter, in a file-scope array that each device holds
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel or a copy: its name is a host-side stand-in, and the device address
+// comes from hipGetSymbolAddress, once per device (noAllocScratch).
+constexpr size_t kNoAllocRayBytes = (size_t)RTX_NOALLOC_MAX_RAYS * sizeof(rtx_ray);
+constexpr size_t kNoAllocHitBytes = (size_t)RTX_NOALLOC_MAX_RAYS * sizeof(rtx_hit);
+constexpr size_t kNoAllocBytes = kNoAllocRayBytes + kNoAllocHitBytes + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex gScratchMutex;
+void *gScratch[kNoAllocDevices] = {};
+hipError_t gScratchErr[kNoAllocDevices] = {};
+bool gScratchDone[kNoAllocDevices] = {};
+} // namespace
+
+// The device address of `scratch` on `device` (the current device), looked
+// up on the first call for that device.
+static void *noAllocScratch(int device, hipError_t &err) {
+    if (device < 0 || device >= kNoAllocDevices) {
+        err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchDone[device]) {
+        void *ptr = nullptr;
+        gScratchErr[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        gScratch[device] = gScratchErr[device] == hipSuccess ? ptr : nullptr;
+        gScratchDone[device] = true;
+    }
+    err = gScratchErr[device];
+    return gScratch[device];
+}
+
 static int queryCommon(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, void *out,
                        size_t outBytes, bool closest) {
     if (!ctx) return RTX_E_INVALID;
@@ -2118,6 +2165,31 @@ int rtx_trace_closest(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays,
 
 int rtx_trace_occluded(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n, uint8_t *occluded) try {
     return queryCommon(ctx, scene, rays, n, occluded, 1, false);
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+int rtx_trace_closest_noalloc(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ray *rays, size_t n,
+                              rtx_hit *hits) try {
+    if (!ctx) return RTX_E_INVALID;
+    if (!scene || (n && (!rays || !hits))) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    if (n > RTX_NOALLOC_MAX_RAYS)
+        return fail(ctx, RTX_E_INVALID, "rtx_trace_closest_noalloc: at most RTX_NOALLOC_MAX_RAYS (65536) rays per call");
+    if (n == 0) return RTX_OK;
+    if (!ctx->scratch) return fail(ctx, RTX_E_DEVICE, "rtx_trace_closest_noalloc: " + ctx->scratchErr);
+    CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    char *base = static_cast<char *>(ctx->scratch);
+    rtx_ray *rD = reinterpret_cast<rtx_ray *>(base);
+    rtx_hit *hD = reinterpret_cast<rtx_hit *>(base + kNoAllocRayBytes);
+    uint32_t *fetch = reinterpret_cast<uint32_t *>(base + kNoAllocRayBytes + kNoAllocHitBytes);
+    hipStream_t s = ctx->stream;
+    launchFillU32(s, fetch, 1, 0u); // a kernel, not a memset node: callers may capture the stream
+    CHECK_HIP(ctx, hipMemcpyAsync(rD, rays, n * sizeof(rtx_ray), hipMemcpyHostToDevice, s));
+    launchQueryP(s, scene->view, rD, (uint32_t)n, hD, true, fetch, ctx->cuCount * ctx->tuning.traceBlocksPerCU);
+    CHECK_HIP(ctx, hipGetLastError());
+    CHECK_HIP(ctx, hipMemcpyAsync(hits, hD, n * sizeof(rtx_hit), hipMemcpyDeviceToHost, s));
+    CHECK_HIP(ctx, hipStreamSynchronize(s));
+    return RTX_OK;
 } catch (...) {
     return rtx::abiFail(ctx);
 }

@@ -320,6 +320,8 @@ void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers 
 void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,
                       bool keep, float4 *estimate);
 void launchResetCounters(hipStream_t s, Counters *cnt, int which);
+// p[0..n) = v (a kernel, so that captured streams hold no memset nodes)
+void launchFillU32(hipStream_t s, uint32_t *p, uint32_t n, uint32_t v);
 void launchResolveImage(hipStream_t s, const float4 *accum, uint32_t npix, float invSamples, float4 *out);
 
 } // namespace rtx

@@ -983,6 +983,11 @@ __global__ void kResetCounters(Counters *cnt, int which) {
     }
 }
 
+__global__ __launch_bounds__(256) void kFillU32(uint32_t *p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ __launch_bounds__(256) void kResolveImage(const float4 *accum, uint32_t npix, float inv, float4 *out) {
     uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= npix) return;
@@ -1076,6 +1081,10 @@ void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_
     if (n == 0) return;
     if (closest) hipLaunchKernelGGL(kQueryP<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, rays, n, out, fetch);
     else hipLaunchKernelGGL(kQueryP<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, rays, n, out, fetch);
+}
+void launchFillU32(hipStream_t s, uint32_t *p, uint32_t n, uint32_t v) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(kFillU32, dim3(gridFor(n, 256)), dim3(256), 0, s, p, n, v);
 }
 void launchResolveImage(hipStream_t s, const float4 *accum, uint32_t npix, float inv, float4 *out) {
     hipLaunchKernelGGL(kResolveImage, dim3(gridFor(npix, 256)), dim3(256), 0, s, accum, npix, inv, out);

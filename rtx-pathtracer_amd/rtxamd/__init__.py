@@ -96,6 +96,8 @@ def device_lib():
         L.rtx_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Ray), C.c_size_t, C.POINTER(Hit)]
         L.rtx_trace_occluded.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Ray), C.c_size_t,
                                          C.POINTER(C.c_uint8)]
+        L.rtx_trace_closest_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Ray), C.c_size_t,
+                                                C.POINTER(Hit)]
         L.rtx_ic_create.argtypes = [C.c_void_p, C.c_uint32]
         L.rtx_ic_reset.argtypes = [C.c_void_p]
         L.rtx_ic_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
@@ -743,6 +745,18 @@ class Renderer:
         hits = np.empty((n, 4), np.float32)
         self._check(device_lib().rtx_trace_closest(self._h, scene._h, rays.ctypes.data_as(C.POINTER(Ray)), n,
                                                    hits.ctypes.data_as(C.POINTER(Hit))))
+        t = hits[:, 0].copy()
+        prim = hits[:, 1].view(np.int32).copy()
+        return t, prim, hits[:, 2].copy(), hits[:, 3].copy()
+
+    def trace_closest_noalloc(self, scene, rays):
+        """rtx_trace_closest_noalloc: the closest-hit query through the
+        library's fixed device array (at most RTX_NOALLOC_MAX_RAYS rays)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.empty((n, 4), np.float32)
+        self._check(device_lib().rtx_trace_closest_noalloc(self._h, scene._h, rays.ctypes.data_as(C.POINTER(Ray)),
+                                                           n, hits.ctypes.data_as(C.POINTER(Hit))))
         t = hits[:, 0].copy()
         prim = hits[:, 1].view(np.int32).copy()
         return t, prim, hits[:, 2].copy(), hits[:, 3].copy()

@@ -22,6 +22,8 @@ RTX_FRAME_NO_GUIDING_UPDATE = 0x10
 RTX_FRAME_TIME_TRACE = 0x20
 RTX_FRAME_NRC = 0x40
 
+RTX_NOALLOC_MAX_RAYS = 65536  # rays per rtx_trace_closest_noalloc call
+
 MAT_DIFFUSE, MAT_SPECULAR, MAT_DIELECTRIC, MAT_LIGHT, MAT_PHONG, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR = range(7)
 LIGHT_AREA, LIGHT_POINT, LIGHT_SPHERE, LIGHT_ENV_MAP = range(4)
 DISTRIBUTION_BECKMANN, DISTRIBUTION_GGX = 0, 1
