@@ -44,6 +44,239 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def room_triangles(t):
+    """The room generator's nearest valid count: 20 + a multiple of 4, >= 124."""
+    t = max(124, int(t))
+    return 20 + 4 * ((t - 20) // 4)
+
+
+def serving_leg(rtxamd, abi, r, a):
+    """Serving workload: Poisson arrivals at `serve_rate` jobs/s (seeded) over
+    16 resident seeded procedural rooms of 10k to 2M triangles (log-uniform);
+    per job a resolution from {256^2, 512^2, 1024^2, 1920x1080}, spp from
+    {4, 16, 64, 256}, depth 8-12, irradiance-cache lookups on or off and
+    guided sampling on or off (each with probability 1/2).  Each scene's cache
+    (10 prepare frames at 256^2) and guiding regions (2^4 regions, 3 training
+    frames at 128^2) are built before the stream starts and stay resident
+    with the BVHs.  The first `serve_jobs` arrivals of the stream are
+    submitted at their arrival times; p50/p99 are over their latencies
+    (submission to image complete on the device)."""
+    rng = np.random.default_rng(40)
+    tris = [room_triangles(np.exp(rng.uniform(np.log(1e4), np.log(2e6)))) for _ in range(16)]
+    t0 = time.perf_counter()
+    hosts, devs, states = [], [], []
+    pc = dict(maxDepth=12, enableMIS=1, enableNEE=1, usePowerHeuristic=1)
+    for k, t in enumerate(tris):
+        hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 100 + k, t)
+        ds = r.upload(hs)
+        st = rtxamd.Renderer(0)
+        sds = st.upload(hs)
+        st.ic_create(10000)
+        cam = hs.camera(256, 256)
+        st.render(sds, rtxamd.frame_params(cam, 256, 256, 10, seed=7, flags=abi.RTX_FRAME_RESET_ACCUM,
+                                           useIrradianceCache=1, isIrradiancePrepareFrame=1, **pc))
+        st.guiding_init(sds.bounds(), 4)
+        gcam = hs.camera(128, 128)
+        st.render(sds, rtxamd.frame_params(gcam, 128, 128, 3, seed=9, flags=abi.RTX_FRAME_RESET_ACCUM,
+                                           updateGuiding=1, useGuiding=1, **pc))
+        hosts.append(hs)
+        devs.append(ds)
+        states.append((st, sds))
+    setup_s = time.perf_counter() - t0
+    log(f"serving: 16 scenes ({min(tris)}..{max(tris)} triangles, {sum(tris)} total) resident, caches and "
+        f"guides built in {setup_s:.1f} s")
+    res = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+    spps = [4, 16, 64, 256]
+    jobs = []
+    t_arr = 0.0
+    for _ in range(a.serve_jobs):
+        t_arr += rng.exponential(1.0 / a.serve_rate)
+        k = int(rng.integers(16))
+        w, h = res[int(rng.integers(4))]
+        spp = spps[int(rng.integers(4))]
+        depth = int(rng.integers(8, 13))
+        ic, guide = bool(rng.integers(2)), bool(rng.integers(2))
+        fp = rtxamd.frame_params(hosts[k].camera(w, h), w, h, spp, seed=int(rng.integers(1 << 30)), maxDepth=depth,
+                                 enableMIS=1, enableNEE=1, usePowerHeuristic=1, useIrradianceCache=int(ic),
+                                 useGuiding=int(guide), guidingProb=0.5)
+        jobs.append((t_arr, k, fp, ic or guide))
+    with rtxamd.Server(r, slotPaths=1 << 18, slots=16, framebufferBytes=1 << 30) as sv:
+        ids = []
+        t_start = time.perf_counter()
+        for t_arr, k, fp, stateful in jobs:
+            dt = t_start + t_arr - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(sv.submit(devs[k], fp, states[k][0] if stateful else None))
+        for jid in ids:
+            sv.wait(jid)
+            sv.read_image(jid)
+        wall = time.perf_counter() - t_start
+        st = sv.stats()
+    paths = sum(fp.width * fp.height * fp.spp for _, _, fp, _ in jobs)
+    out = {
+        "workload": (f"{a.serve_jobs} jobs of a Poisson stream at {a.serve_rate:g} jobs/s (seed 40) over 16 resident "
+                     f"rooms of {min(tris)}-{max(tris)} triangles; resolutions 256^2..1920x1080, spp 4..256, depth "
+                     f"8-12, cache and guiding lookups per job"),
+        "jobs": a.serve_jobs, "paths": int(paths),
+        "p50_ms": round(st.latencyP50Ms, 1), "p99_ms": round(st.latencyP99Ms, 1),
+        "max_ms": round(st.latencyMaxMs, 1), "mean_ms": round(st.latencyMeanMs, 1),
+        "msamples_per_s": round(st.msamplesPerSec, 2), "wall_s": round(wall, 2),
+        "offered_msamples_per_s": round(paths / (jobs[-1][0] * 1e6), 1),
+        "peak_slots_busy": st.peakSlotsBusy, "peak_jobs_running": st.peakJobsRunning,
+        "bounces": st.bounces, "mrays_per_s": round((st.extendRays + st.shadowRays + st.probeRays) / (wall * 1e6), 1),
+        "pool": "16 slots x 2^18 paths", "setup_s": round(setup_s, 1),
+    }
+    log(f"serving: {a.serve_jobs} jobs, p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, "
+        f"{out['msamples_per_s']} Msamples/s")
+    for (st_, sds) in states:
+        sds.close()
+        st_.close()
+    for ds in devs:
+        ds.close()
+    return out
+
+
+def relmse(img, ref, eps=1e-2):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + eps)."""
+    d = img[..., :3].astype(np.float64) - ref[..., :3]
+    return float(np.mean(d * d / (ref[..., :3] ** 2 + eps)))
+
+
+def equal_time(render_call, budget_s):
+    """Calls render_call(first) until budget_s of wall time have passed;
+    returns the number of calls and the seconds spent (the last call may end
+    past the budget, and its time is counted)."""
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        render_call(n == 0)
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    return n, time.perf_counter() - t0
+
+
+def relmse_leg(rtxamd, abi, r, a):
+    """Equal-time RelMSE against a plain reference, RelMSE = mean over pixels
+    and channels of (I - R)^2 / (R^2 + 1e-2).  Room: the benchmark scene
+    (262,144 triangles, depth 12, NEE + MIS) at 480x270 (the 1080p camera at a
+    quarter of the resolution): plain, irradiance cache (10,000 records, 50
+    prepare frames inside the budget, per-frame upkeep), learned cache (hash
+    grid 16 x 2 x 2^19, 5 x 64 MLP, 2^16 training suffixes per frame, trained
+    from scratch inside the budget).  Plates (veachMIS-like, 4 sphere lights,
+    glossy plates) at 320x180: plain vs guided sampling (2^8 regions of
+    vMF mixtures, 6 recording frames with a GPU EM update each inside the
+    budget, then guided frames).  References: `relmse_ref_spp` plain samples
+    per pixel with another seed, 64-spp chunks accumulated in FP64 on the host."""
+    pc = dict(maxDepth=12, enableMIS=1, enableNEE=1, usePowerHeuristic=1)
+    RESET = abi.RTX_FRAME_RESET_ACCUM
+
+    def reference(ds, cam, W, H):
+        ref = np.zeros((H, W, 4))
+        for k in range(0, a.relmse_ref_spp, 64):
+            r.render(ds, rtxamd.frame_params(cam, W, H, 64, seed=1001, sample_offset=k, flags=RESET, **pc))
+            ref += r.read_image(W, H).astype(np.float64) * 64
+        return ref / a.relmse_ref_spp
+
+    out = {}
+    T = a.relmse_seconds
+    # ---- room: plain, irradiance cache, learned cache
+    W, H = 480, 270
+    hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 1)
+    ds = r.upload(hs)
+    cam = hs.camera(W, H)
+    t0 = time.perf_counter()
+    ref = reference(ds, cam, W, H)
+    ref_s = time.perf_counter() - t0
+    room = {"scene": f"room 262144 tris, {W}x{H}, depth 12, NEE+MIS",
+            "reference": f"{a.relmse_ref_spp} spp plain, FP64 accumulation ({ref_s:.1f} s)", "budget_s": T}
+    chunk = 4
+    state = {"spp": 0}
+
+    def plain_call(first):
+        r.render(ds, rtxamd.frame_params(cam, W, H, chunk, seed=1, sample_offset=state["spp"],
+                                         flags=RESET if first else 0, **pc))
+        state["spp"] += chunk
+
+    state["spp"] = 0
+    n, sec = equal_time(plain_call, T)
+    e_plain = relmse(r.read_image(W, H), ref)
+    room["plain"] = {"spp": state["spp"], "seconds": round(sec, 3), "relmse": e_plain}
+
+    r.ic_create(10000)
+    icpc = dict(pc, useIrradianceCache=1)
+
+    def ic_call(first):
+        if first:
+            r.ic_reset()
+            r.render(ds, rtxamd.frame_params(cam, W, H, 50, seed=1, flags=RESET, isIrradiancePrepareFrame=1, **icpc))
+            state["spp"] = 50
+        r.render(ds, rtxamd.frame_params(cam, W, H, chunk, seed=1, sample_offset=state["spp"],
+                                         flags=RESET if first else 0, **icpc))
+        state["spp"] += chunk
+
+    n, sec = equal_time(ic_call, T)
+    e_ic = relmse(r.read_image(W, H), ref)
+    room["irradiance_cache"] = {"spp": state["spp"] - 50, "seconds": round(sec, 3), "relmse": e_ic,
+                                "improvement": round(e_plain / e_ic, 3)}
+
+    info = ds.info()
+
+    def nrc_call(first):
+        if first:
+            r.nrc_create(list(info.boundsMin), list(info.boundsMax), seed=5)
+            state["spp"] = 0
+        r.render(ds, rtxamd.frame_params(cam, W, H, chunk, seed=1, sample_offset=state["spp"],
+                                         flags=abi.RTX_FRAME_NRC | (RESET if first else 0), **pc))
+        state["spp"] += chunk
+
+    n, sec = equal_time(nrc_call, T)
+    e_nrc = relmse(r.read_image(W, H), ref)
+    ni = r.nrc_info()
+    room["learned_cache"] = {"spp": state["spp"], "seconds": round(sec, 3), "relmse": e_nrc,
+                             "improvement": round(e_plain / e_nrc, 3), "adam_steps": ni.steps,
+                             "last_loss": round(ni.lastLoss, 5)}
+    r.nrc_destroy()
+    ds.close()
+    out["room"] = room
+    log(f"relmse room: plain {e_plain:.4g}, cache {e_ic:.4g}, learned {e_nrc:.4g}")
+
+    # ---- plates: plain vs guided
+    W, H = 320, 180
+    hs = rtxamd.HostScene.generate(abi.SCENE_PLATES, 1)
+    ds = r.upload(hs)
+    cam = hs.camera(W, H)
+    t0 = time.perf_counter()
+    ref = reference(ds, cam, W, H)
+    ref_s = time.perf_counter() - t0
+    plates = {"scene": f"plates and sphere lights, {W}x{H}, depth 12, NEE+MIS",
+              "reference": f"{a.relmse_ref_spp} spp plain, FP64 accumulation ({ref_s:.1f} s)", "budget_s": T}
+    state["spp"] = 0
+    n, sec = equal_time(plain_call, T)
+    e_plain = relmse(r.read_image(W, H), ref)
+    plates["plain"] = {"spp": state["spp"], "seconds": round(sec, 3), "relmse": e_plain}
+    bounds = ds.bounds()
+
+    def guided_call(first):
+        if first:
+            r.guiding_init(bounds, 8)
+            r.render(ds, rtxamd.frame_params(cam, W, H, 6, seed=1, flags=RESET, updateGuiding=1, useGuiding=1, **pc))
+            state["spp"] = 6
+        r.render(ds, rtxamd.frame_params(cam, W, H, chunk, seed=1, sample_offset=state["spp"],
+                                         flags=RESET if first else 0, useGuiding=1, **pc))
+        state["spp"] += chunk
+
+    n, sec = equal_time(guided_call, T)
+    e_g = relmse(r.read_image(W, H), ref)
+    plates["guided"] = {"spp": state["spp"] - 6, "seconds": round(sec, 3), "relmse": e_g,
+                        "improvement": round(e_plain / e_g, 3)}
+    ds.close()
+    out["plates"] = plates
+    log(f"relmse plates: plain {e_plain:.4g}, guided {e_g:.4g}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +295,13 @@ def main():
     ap.add_argument("--shade-blocks", type=int, default=0, help="shading blocks per CU (0: default)")
     ap.add_argument("--no-count", action="store_true", help="skip the node/triangle counting pass")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-method comparison step")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotune)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-jobs", type=int, default=24, help="jobs of the serving leg's arrival stream")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate of the serving leg, jobs/s")
+    ap.add_argument("--no-relmse", action="store_true", help="skip the equal-time RelMSE leg")
+    ap.add_argument("--relmse-seconds", type=float, default=1.0, help="equal-time budget per method")
+    ap.add_argument("--relmse-ref-spp", type=int, default=4096, help="samples per pixel of the reference image")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 byte counters)")
     a = ap.parse_args()
@@ -116,6 +356,23 @@ def main():
     for _ in range(a.warmup):
         step(0, a.method)
     log("warmup done")
+    # launch shapes: rtx_autotune on two frames of the workload (outside the timed region)
+    tuning = None
+    if not a.no_tune and a.trace_blocks == 0 and a.shade_blocks == 0:
+        probe_pc = icpc if a.method == "ic" else pc
+        probe = rtxamd.frame_params(cam, W, H, 2, seed=1, sample_offset=a.prepare_frames,
+                                    flags=abi.RTX_FRAME_RESET_ACCUM, **probe_pc)
+        res = r.autotune(ds, probe, repeats=2)
+        b = res.best
+        tuning = {"traceBlocksPerCU": b.traceBlocksPerCU, "ldsStackEntries": b.ldsStackEntries,
+                  "shadeBlocksPerCU": b.shadeBlocksPerCU, "batchPaths": b.batchPaths,
+                  "probe": f"2 frames of the workload ({a.method}), {res.trials} trials",
+                  "probe_ms_default": round(res.startMs, 2), "probe_ms_tuned": round(res.bestMs, 2)}
+        log(f"autotune: {tuning}")
+    else:
+        t = r.tuning()
+        tuning = {"traceBlocksPerCU": t.traceBlocksPerCU, "ldsStackEntries": t.ldsStackEntries,
+                  "shadeBlocksPerCU": t.shadeBlocksPerCU, "batchPaths": t.batchPaths, "probe": None}
     # timed region: K full images; each render ends with a stream synchronize
     keys = ("msTotal", "msExtend", "msShadow", "msProbe", "msShade")
     totals = dict.fromkeys(keys, 0.0)
@@ -228,6 +485,9 @@ def main():
                   "note": "FP32 shading normals on 2-8 cm smooth columns at ~10 m coordinates make ~20% of "
                           "13-bounce paths branch differently; the CPU FP32 run of the same logic does the same"}
 
+    serving = None if a.no_serve else serving_leg(rtxamd, abi, r, a)
+    eq = None if a.no_relmse else relmse_leg(rtxamd, abi, r, a)
+
     out = {
         "metric": "Msamples/s at 1080p depth 12; equal-time RelMSE vs plain; p99 job latency",
         "value": round(value, 3),
@@ -257,9 +517,12 @@ def main():
         "ms_device_per_step": {"total": round(totals["msTotal"] / a.steps, 1),
                                "trace": round(trav_ms, 1),
                                "other": round(totals["msTotal"] / a.steps - trav_ms, 1)},
+        "tuning": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32_vs_fp64": parity,
+        "serving": serving,
+        "equal_time_relmse": eq,
     }
     print(json.dumps(out), flush=True)
     ds.close()

@@ -409,8 +409,27 @@ typedef struct rtx_tuning {
                                  7 waves per SIMD) */
     int32_t shadeBlocksPerCU; /* 256-thread shading blocks per CU (default 8) */
     uint32_t batchPaths;      /* paths in flight per batch when rtx_frame_params.batchPaths is 0 (2^22) */
-    int32_t _reserved[4];
+    int32_t ldsStackEntries;  /* traversal stack entries per thread held in LDS: 12, 16, 20 or 24
+                                 (20); deeper entries spill to scratch */
+    int32_t _reserved[3];
 } rtx_tuning;
+
+/* Autotuner (no counterpart in the reference).  Renders `probe` (as given,
+ * with the context's cache and guiding state; its side effects - the
+ * accumulation image, cache upkeep - happen once per trial) under launch
+ * shapes chosen by coordinate descent, one knob at a time from the current
+ * tuning: traceBlocksPerCU in {8, 10, 12, 14, 16, 20}, ldsStackEntries in
+ * {12, 16, 20, 24}, shadeBlocksPerCU in {4, 6, 8, 12, 16}, batchPaths in
+ * {2^20, 2^21, 2^22, 2^23}; every candidate is timed `repeats` times (device
+ * time of the render, HIP events) and its fastest time kept.  The best shape
+ * is set on the context (rtx_ctx_set_tuning) and returned. */
+typedef struct rtx_autotune_result {
+    rtx_tuning best;
+    double bestMs;            /* fastest probe render with `best` */
+    double startMs;           /* fastest probe render with the tuning in place at the call */
+    uint32_t trials;          /* probe renders run */
+    uint32_t reserved;
+} rtx_autotune_result;
 
 typedef struct rtx_ctx rtx_ctx;
 typedef struct rtx_scene rtx_scene;
@@ -430,6 +449,8 @@ const char *rtx_last_error(const rtx_ctx *ctx);
  * the reference). */
 int rtx_ctx_set_tuning(rtx_ctx *ctx, const rtx_tuning *tuning);
 int rtx_ctx_get_tuning(const rtx_ctx *ctx, rtx_tuning *tuning);
+int rtx_autotune(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *probe, uint32_t repeats,
+                 rtx_autotune_result *result);
 
 /* Upload a scene and build its BVH (replaces SceneLoader::createVulkanObjects,
  * src/SceneLoader.cpp:67-87, incl. createBottomLevelAS/createTopLevelAS). */
@@ -683,6 +704,101 @@ int rtx_nrc_gradients(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target
                       float *mlpGrad, float *loss);
 /* One Adam step on n <= trainBatch rows. */
 int rtx_nrc_train(rtx_ctx *ctx, const rtx_nrc_input *in, const float *target, size_t n, float *loss);
+
+/* ------------------------------------------------------------- serving */
+/* A persistent request loop on one device (no counterpart in the reference,
+ * which renders one scene per process through RayTracingApp::drawFrame,
+ * src/RayTracingApp.cpp:74-215).  Render jobs of any resident scene,
+ * resolution and sample count are submitted as they arrive; a server thread
+ * admits them in arrival order while their images fit the framebuffer
+ * budget, cuts each into batches of whole sample planes (or pixel ranges of
+ * one plane) of at most slotPaths paths, and runs the batches of all
+ * admitted jobs together: the pool's slots hold one batch each, every bounce
+ * is one launch of each wavefront kernel over the paths of all slots (rays
+ * of different scenes in one persistent traversal launch), and a slot whose
+ * paths have all ended takes the next batch at the next bounce (continuous
+ * batching).  Each job's batches are added to its image in batch order, and
+ * every path draws from its own (pixel, sample) stream, so a job's image is
+ * the same bit for bit whatever else runs beside it, and equal to rtx_render
+ * of the same frame with batchPaths equal to the job's batch size.
+ * Jobs render the plain method with one camera sample per path (NEE, MIS,
+ * Russian roulette, textures, environment map); with pc.useIrradianceCache
+ * or pc.useGuiding they read the cache or the guiding regions of `state`, a
+ * context whose own frames built them (read-only while jobs use them).
+ * Splitting, ADRRS, guiding records, samplesPerPixel > 1, cache upkeep and
+ * the learned cache are RTX_E_UNSUPPORTED here (rtx_render has them). */
+typedef struct rtx_server rtx_server;
+
+typedef struct rtx_server_config {
+    uint64_t framebufferBytes; /* budget of the admitted jobs' images: 32 B per pixel (1 GiB) */
+    uint32_t slotPaths;        /* paths per slot, a power of two in [2^10, 2^22] (2^18) */
+    uint32_t slots;            /* slots in the pool, 1..64 (16) */
+    uint32_t maxNumNEE;        /* largest pc.numNEE a job may use, 1..8 (1) */
+    uint32_t maxJobs;          /* jobs queued, running or unread at once (4096) */
+} rtx_server_config;
+
+typedef struct rtx_job_desc {
+    const rtx_scene *scene;
+    const rtx_ctx *state;      /* cache / guiding regions read with pc.useIrradianceCache /
+                                  pc.useGuiding (NULL: none) */
+    rtx_push_constant pc;
+    rtx_camera cam;
+    uint32_t width, height;
+    uint32_t spp;              /* sample planes (one camera path per pixel each) */
+    int32_t rrStartDepth;
+    uint64_t seed;
+    uint32_t sampleOffset;
+    uint32_t reserved;
+} rtx_job_desc;
+
+#define RTX_JOB_QUEUED 0   /* waiting for framebuffer budget */
+#define RTX_JOB_RUNNING 1  /* admitted: batches in the pool */
+#define RTX_JOB_DONE 2     /* image complete, not yet read */
+#define RTX_JOB_RELEASED 3 /* read (rtx_server_read_image) */
+
+typedef struct rtx_job_status {
+    int32_t state;
+    uint32_t batches;          /* batches of the job */
+    uint32_t batchesDone;      /* added to its image */
+    uint32_t reserved;
+    uint64_t paths;            /* width * height * spp */
+    double queuedMs;           /* submission to admission */
+    double latencyMs;          /* submission to the image complete on the device, as the
+                                  server loop observed it (resolution: one bounce) */
+} rtx_job_status;
+
+typedef struct rtx_server_stats {
+    uint64_t jobsDone;
+    uint64_t pathsDone;
+    double latencyP50Ms, latencyP99Ms, latencyMaxMs, latencyMeanMs; /* jobs done */
+    double msamplesPerSec;     /* paths of the jobs done / (last completion - first submission) */
+    uint64_t bounces;          /* pool bounces launched */
+    uint64_t extendRays, shadowRays, probeRays;
+    uint32_t peakSlotsBusy;
+    uint32_t peakJobsRunning;
+    uint64_t framebufferPeakBytes;
+    uint64_t poolBytes;        /* device memory of the path pool */
+} rtx_server_stats;
+
+void rtx_server_config_defaults(rtx_server_config *config);
+/* The server runs on ctx's device with its own stream and thread; ctx must
+ * outlive it. */
+int rtx_server_create(rtx_ctx *ctx, const rtx_server_config *config, rtx_server **out);
+/* Finishes the jobs already admitted (queued ones are dropped), then stops
+ * the thread and frees the pool. */
+int rtx_server_destroy(rtx_server *server);
+/* Checks the job (RTX_E_INVALID, RTX_E_UNSUPPORTED, RTX_E_MEMORY when its
+ * image alone exceeds the budget) and queues it; the scene (and state) must
+ * stay alive until the job is done. */
+int rtx_server_submit(rtx_server *server, const rtx_job_desc *job, uint64_t *jobId);
+int rtx_server_poll(rtx_server *server, uint64_t jobId, rtx_job_status *status);
+/* Blocks until the job is done. */
+int rtx_server_wait(rtx_server *server, uint64_t jobId, rtx_job_status *status);
+/* Copies a done job's image (RGBA32F, width * height * 4 floats, the mean of
+ * its samples) and releases its framebuffer. */
+int rtx_server_read_image(rtx_server *server, uint64_t jobId, float *rgba, size_t floatCount);
+int rtx_server_get_stats(rtx_server *server, rtx_server_stats *stats);
+const char *rtx_server_last_error(const rtx_server *server);
 
 /* ------------------------------------------------------ frame schedule */
 /* The host loop around the render of the reference's application,

@@ -16,113 +16,12 @@
 #include <vector>
 
 #include "abi_guard.h"
+#include "context.h"
 #include "dev_math.h"
 #include "internal.h"
 #include "nrc.h"
 
 using namespace rtx;
-
-struct rtx_scene {
-    DevScene view{};
-    std::vector<void *> allocs;
-    rtx_scene_info info{};
-    int device = 0;
-};
-
-namespace {
-
-struct EventPair {
-    hipEvent_t a = nullptr, b = nullptr;
-    int cls = 0;
-};
-
-enum { kClsExtend = 0, kClsShadow = 1, kClsProbe = 2, kClsShade = 3 };
-// Bounces the device may run ahead of the host's loop-end check, and the
-// poll ring's slots (more than kPollLead, so that a slot is read before the
-// bounce that reuses it is launched).
-constexpr int kPollLead = 3, kPollSlots = 8;
-
-} // namespace
-
-struct rtx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // accumulation image
-    float4 *accum = nullptr;
-    float4 *resolved = nullptr;
-    uint32_t accW = 0, accH = 0, accSamples = 0;
-    // wavefront buffers
-    size_t cap = 0;
-    uint32_t capNEE = 0;
-    PathBuffers pb{};
-    Queue q[2]{}, qs{}, qp{};
-    int32_t *done = nullptr;
-    Counters *cnt = nullptr;
-    FrameConst *fc = nullptr;
-    // bounce-loop end detection: each bounce's trace kernel writes the queue
-    // lengths it starts from into a slot of this pinned ring (kPollSlots
-    // slots of 4 words), and an event after the trace marks the slot readable
-    uint32_t *pollRing = nullptr;
-    hipEvent_t pollEv[kPollSlots] = {};
-    std::vector<void *> batchAllocs;
-    // per-path results of the last render
-    float4 *keepRgb = nullptr;
-    uint2 *keepSig = nullptr;
-    size_t keepCap = 0, keptPaths = 0;
-    // timing
-    std::vector<EventPair> events;
-    size_t eventsUsed = 0;
-    hipEvent_t tStart = nullptr, tStop = nullptr;
-    int cuCount = 256;
-    rtx_tuning tuning{1, 14, 8, 1u << 22, {0, 0, 0, 0}};
-    // irradiance cache (rtx_ic_create)
-    IcBuffers ic{};
-    std::vector<void *> icAllocs;
-    bool icValid = false;
-    FrameConst *fcIC = nullptr;      // constants of the cache's stratum sub-paths
-    float4 *candBuf = nullptr;       // per-path creation candidates of a batch
-    size_t candCap = 0;
-    // sub-path results of the last cache frame of a render with RTX_FRAME_KEEP_PATHS
-    float4 *keepSub = nullptr;
-    uint2 *keepSubSig = nullptr;
-    size_t keepSubCap = 0;
-    uint32_t keptJobs = 0, keptUpdates = 0;
-    // guiding sample records (pc.updateGuiding)
-    std::vector<void *> recAllocs;    // per-path record state, recCap paths
-    size_t recCap = 0;
-    rtx_directional_data *dirData = nullptr;
-    size_t dirCap = 0, dirCount = 0;  // slots allocated / slots of the last recording render
-    // path guiding (rtx_guiding_set / rtx_guiding_init)
-    std::vector<void *> guideAllocs;
-    DevGuiding guide{};
-    std::vector<rtx_aabb> guideBoxes;
-    std::vector<rtx_vmm_theta> guideVmms;
-    bool guideVmmsStale = false;      // the device mixtures moved on (updates)
-    // the update (rtx_guiding_init): per-region mixtures and work buffers
-    std::vector<void *> fitAllocs, fitWorkAllocs;
-    GuideFitBuffers fit{};
-    bool fitValid = false, fitParallax = true, fitFirst = true;
-    rtx_guiding_options guideOpts{1, 8192, 4096, 4096, 0.5f, 0.025f, 0, 10000}; // src/PathGuiding.h:118-129
-    // frames of several segments: per-path frame state, split records
-    std::vector<void *> segAllocs, splitAllocs;
-    size_t segCap = 0, splitCap = 0;
-    // ADRRS pixel estimate (estimateImage)
-    float4 *estimate = nullptr;
-    uint32_t estW = 0, estH = 0;
-    // learned radiance cache (rtx_nrc_create): parameters and training work,
-    // the render's queries and training vertices
-    bool nrcValid = false;
-    rtx_nrc_config nrcCfg{};
-    NrcBuffers nrc{};
-    std::vector<void *> nrcAllocs, nrcPathAllocs, nrcSlotAllocs;
-    float *nrcLoss = nullptr;   // device: summed loss of the last steps
-    uint32_t nrcLossSteps = 0;
-    size_t nrcPathCap = 0, nrcSlotCap = 0;
-    // device address of the allocation-free query's scratch array (rtx_trace_closest_noalloc)
-    void *scratch = nullptr;
-    std::string scratchErr;
-};
 
 namespace {
 
@@ -347,17 +246,6 @@ static int buildGuideRec(std::vector<GuideNode> &out, const rtx_aabb *boxes, std
     return me;
 }
 
-DevIC icView(const IcBuffers &ic) {
-    DevIC v;
-    v.entries = ic.entries;
-    v.spheres = ic.spheres;
-    v.bucketStart = ic.bucketStart;
-    v.items = ic.items;
-    v.meta = ic.meta;
-    v.nb = ic.nb;
-    return v;
-}
-
 // Level-0 cell of the lookup index: the diameter of the smallest record
 // (irradianceA * irradianceCacheMinRadius), or 1e-3 when that is 0.
 float icBaseCell(const rtx_push_constant &pc) {
@@ -473,12 +361,15 @@ int rtx_ctx_set_tuning(rtx_ctx *ctx, const rtx_tuning *t) try {
     if (t->traceBlocksPerCU) v.traceBlocksPerCU = t->traceBlocksPerCU;
     if (t->shadeBlocksPerCU) v.shadeBlocksPerCU = t->shadeBlocksPerCU;
     if (t->batchPaths) v.batchPaths = t->batchPaths;
+    if (t->ldsStackEntries) v.ldsStackEntries = t->ldsStackEntries;
     if (v.traversalKernel != 1)
         return fail(ctx, RTX_E_INVALID, "rtx_tuning.traversalKernel must be 1 (persistent lane-refill traversal)");
     if (v.traceBlocksPerCU < 1 || v.traceBlocksPerCU > 64 || v.shadeBlocksPerCU < 1 || v.shadeBlocksPerCU > 64)
         return fail(ctx, RTX_E_INVALID, "rtx_tuning blocks per CU must lie in [1, 64]");
     if (v.batchPaths < 1024 || v.batchPaths > (1u << 26))
         return fail(ctx, RTX_E_INVALID, "rtx_tuning.batchPaths must lie in [1024, 2^26]");
+    if (v.ldsStackEntries != 12 && v.ldsStackEntries != 16 && v.ldsStackEntries != 20 && v.ldsStackEntries != 24)
+        return fail(ctx, RTX_E_INVALID, "rtx_tuning.ldsStackEntries must be 12, 16, 20 or 24");
     ctx->tuning = v;
     return RTX_OK;
 } catch (...) {
@@ -491,6 +382,71 @@ int rtx_ctx_get_tuning(const rtx_ctx *ctx, rtx_tuning *t) try {
     return RTX_OK;
 } catch (...) {
     return rtx::abiFail(nullptr);
+}
+
+int rtx_autotune(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *probe, uint32_t repeats,
+                 rtx_autotune_result *result) try {
+    if (!ctx) return RTX_E_INVALID;
+    if (!scene || !probe || !result) return fail(ctx, RTX_E_INVALID, "NULL argument");
+    if (repeats < 1 || repeats > 16) return fail(ctx, RTX_E_INVALID, "repeats must be in [1, 16]");
+    std::memset(result, 0, sizeof(*result));
+    const rtx_tuning start = ctx->tuning;
+    rtx_tuning cur = start;
+    uint32_t trials = 0;
+    // fastest of `repeats` renders of the probe under tuning t (device time)
+    auto timeWith = [&](const rtx_tuning &t, double &ms) -> int {
+        ctx->tuning = t;
+        ms = 1e300;
+        for (uint32_t k = 0; k < repeats; ++k) {
+            rtx_frame_stats st;
+            const int rc = rtx_render(ctx, scene, probe, &st);
+            if (rc != RTX_OK) return rc;
+            ms = std::min(ms, st.msTotal);
+            trials++;
+        }
+        return RTX_OK;
+    };
+    double best = 0;
+    int rc = timeWith(cur, best);
+    if (rc != RTX_OK) {
+        ctx->tuning = start;
+        return rc;
+    }
+    result->startMs = best;
+    const int32_t traceOpts[] = {8, 10, 12, 14, 16, 20};
+    const int32_t ldsOpts[] = {12, 16, 20, 24};
+    const int32_t shadeOpts[] = {4, 6, 8, 12, 16};
+    const uint32_t batchOpts[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    auto descend = [&](auto field, const auto *opts, size_t n) -> int {
+        for (size_t i = 0; i < n; ++i) {
+            if (*field(cur) == opts[i]) continue;
+            rtx_tuning t = cur;
+            *field(t) = opts[i];
+            double ms = 0;
+            const int r = timeWith(t, ms);
+            if (r != RTX_OK) return r;
+            if (ms < best) {
+                best = ms;
+                cur = t;
+            }
+        }
+        return RTX_OK;
+    };
+    rc = descend([](rtx_tuning &t) { return &t.traceBlocksPerCU; }, traceOpts, 6);
+    if (rc == RTX_OK) rc = descend([](rtx_tuning &t) { return &t.ldsStackEntries; }, ldsOpts, 4);
+    if (rc == RTX_OK) rc = descend([](rtx_tuning &t) { return &t.shadeBlocksPerCU; }, shadeOpts, 5);
+    if (rc == RTX_OK) rc = descend([](rtx_tuning &t) { return &t.batchPaths; }, batchOpts, 4);
+    if (rc != RTX_OK) {
+        ctx->tuning = start;
+        return rc;
+    }
+    ctx->tuning = cur;
+    result->best = cur;
+    result->bestMs = best;
+    result->trials = trials;
+    return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
 }
 
 int rtx_ctx_create(int device, rtx_ctx **out) try {
@@ -1000,6 +956,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.sampleOffset = p->sampleOffset;
     fc.rrStartDepth = p->rrStartDepth;
     fc.numNEE = numNEE;
+    fc.neeStride = numNEE;
     fc.countVisits = (p->flags & RTX_FRAME_COUNT_VISITS) ? 1 : 0;
     // developer aid: RTX_DEBUG_PATH=<path index within this call> prints that path's bounces
     fc.debugPath = ~0ull;
@@ -1040,6 +997,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fcIC.splitOnFirst = 0;
     fcIC.nrc = 0;
     fcIC.numNEE = icFrames ? (uint32_t)p->pc.irradianceNumNEE : 1u;
+    fcIC.neeStride = fcIC.numNEE;
     fcIC.debugPath = ~0ull;
     if (icFrames) CHECK_HIP(ctx, hipMemcpyAsync(ctx->fcIC, &fcIC, sizeof(fcIC), hipMemcpyHostToDevice, ctx->stream));
     CHECK_HIP(ctx, hipMemsetAsync(ctx->cnt, 0, sizeof(Counters), ctx->stream));
@@ -1108,7 +1066,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             if (it == 0) timed(kClsShade, [&] { launchResetCounters(s, ctx->cnt, qsel ^ 1); });
             timed(kClsExtend, [&] {
                 launchTraceP(s, S, ctx->q[qsel], qsel, ctx->qs, ctx->qp, ctx->pb, ctx->cnt, poll, countVisits,
-                             traceBlocks);
+                             traceBlocks, ctx->tuning.ldsStackEntries);
             });
             CHECK_HIP(ctx, hipEventRecord(ctx->pollEv[it % kPollSlots], s));
             timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, qsel, shadeBlocks); });
@@ -1132,7 +1090,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         const uint64_t fEnd = std::min(total, f0 + frameLen);
         if (icFrames) timed(kClsShade, [&] { launchIcFrameBegin(s, ctx->ic); });
         for (uint64_t first = f0; first < fEnd; first += batch) {
-            BatchDesc b;
+            BatchDesc b{};
             b.firstPath = first;
             b.numPaths = (uint32_t)std::min<uint64_t>(batch, fEnd - first);
             b.npix = (uint32_t)npix;
@@ -1205,7 +1163,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         for (uint32_t j0 = 0; j0 < numJobs; j0 += jobsPerBatch) {
             const uint32_t nj = std::min(jobsPerBatch, numJobs - j0);
             const uint32_t np = nj * kIcStrata;
-            BatchDesc b;
+            BatchDesc b{};
             b.firstPath = 0;
             b.numPaths = np;
             b.npix = np;

@@ -154,8 +154,8 @@ template <class Stack> RTX_DI int popOrDone(Stack &st) { return st.sp == 0 ? kLi
 // One four-wide node visit: of the children whose boxes the ray enters within
 // [tmin, tmax], the nearest is returned and the others are pushed, farthest
 // first; with none entered, the next stack entry (kLinkDone when empty).
-template <class Stack>
-RTX_DI int nodeStep4(const DevScene &S, const TravRay &r, float tmin, float tmax, int link, Stack &st) {
+template <class SceneT, class Stack>
+RTX_DI int nodeStep4(const SceneT &S, const TravRay &r, float tmin, float tmax, int link, Stack &st) {
     const float4 *np = S.nodes + 8 * (size_t)link;
     const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
     const int4 ln = *reinterpret_cast<const int4 *>(np + 6);
@@ -246,6 +246,28 @@ RTX_DI int traverse(const DevScene &S, const TravRay &r, float tmin, float &tmax
     return best;
 }
 
+// The part of a scene that traversal reads, per lane: launches that trace
+// rays of several scenes (jobs of the serving pool) point each lane at its
+// ray's scene when the ray is fetched.
+struct LaneScene {
+    const float4 *nodes;
+    const float4 *tris;
+    const rtx_sphere *spheres;
+    uint32_t numTris, numSpheres;
+    const DevScene *full; // for the any-hit alpha test
+};
+RTX_DI void pointAt(LaneScene &ls, const DevScene &S, const DevScene *full) {
+    ls.nodes = S.nodes;
+    ls.tris = S.tris;
+    ls.spheres = S.spheres;
+    ls.numTris = S.numTris;
+    ls.numSpheres = S.numSpheres;
+    ls.full = full;
+}
+RTX_DI bool alphaAccept(const LaneScene &S, int prim, float ox, float t, float bu, float bv, uint32_t randomUInt) {
+    return alphaAccept(*S.full, prim, ox, t, bu, bv, randomUInt);
+}
+
 // ------------------------------------------------------------------
 // Persistent traversal with lane refill (Aila & Laine 2009, sections 4-5):
 // each wave loops until the queue is drained; lanes whose ray finished
@@ -270,16 +292,16 @@ constexpr int kHitClosest = 0, kHitAny = 1, kHitLane = 2;
 
 // One inner-node visit: returns the next link (pushing the farther entered
 // children), or the next stack entry / kLinkDone when none is entered.
-template <bool COUNT, class Stack>
-RTX_DI int innerStep(const DevScene &S, LaneRay &L, Stack &st, VisitCount &vc) {
+template <bool COUNT, class SceneT, class Stack>
+RTX_DI int innerStep(const SceneT &S, LaneRay &L, Stack &st, VisitCount &vc) {
     if (COUNT) vc.nodes++;
     return nodeStep4(S, L.r, L.tmin, L.tmax, L.link, st);
 }
 
 // Intersects the primitives of leaf link `leaf`; returns true when an any-hit
 // ray is done.
-template <int HIT, bool COUNT, bool ALPHA>
-RTX_DI bool leafStep(const DevScene &S, LaneRay &L, int leaf, VisitCount &vc) {
+template <int HIT, bool COUNT, bool ALPHA, class SceneT>
+RTX_DI bool leafStep(const SceneT &S, LaneRay &L, int leaf, VisitCount &vc) {
     const bool ANY = HIT == kHitAny || (HIT == kHitLane && L.any);
     uint32_t data = ~(uint32_t)leaf;
     uint32_t start = data >> 4, cnt = (data & 15u) + 1u;
@@ -310,7 +332,7 @@ RTX_DI bool leafStep(const DevScene &S, LaneRay &L, int leaf, VisitCount &vc) {
 }
 
 // Starts a ray: spheres are tested up front (all workloads have few).
-template <int HIT> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
+template <int HIT, class SceneT> RTX_DI void beginRay(const SceneT &S, LaneRay &L) {
     const bool ANY = HIT == kHitAny || (HIT == kHitLane && L.any);
     L.best = -1;
     L.bu = L.bv = 0.0f;
@@ -333,7 +355,8 @@ template <int HIT> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
 }
 
 // Drives the persistent loop.  load(idx, L) fills L.r/L.tmin/L.tmax (and
-// L.any under kHitLane);
+// L.any under kHitLane); S is the scene (a DevScene for launches over one
+// scene, or a per-lane LaneScene that load() points at the ray's scene);
 // store(L) consumes a finished ray.  `fetch` must be zero at launch.  Each
 // wave first takes its own 64 rays of the grid's static span (thread i of
 // the grid: ray i) without an atomic, so a launch over a short or empty queue
@@ -343,8 +366,8 @@ template <int HIT> RTX_DI void beginRay(const DevScene &S, LaneRay &L) {
 // fetching, so `fetch` ends below n + 64 * waves.
 constexpr int kRefillLanes = 16;
 
-template <int HIT, bool COUNT, bool ALPHA, class Stack, class Load, class Store>
-RTX_DI void persistentTrace(const DevScene &S, uint32_t n, uint32_t *fetch, Stack &st, VisitCount &vc, Load load,
+template <int HIT, bool COUNT, bool ALPHA, class SceneT, class Stack, class Load, class Store>
+RTX_DI void persistentTrace(const SceneT &S, uint32_t n, uint32_t *fetch, Stack &st, VisitCount &vc, Load load,
                             Store store) {
     LaneRay L;
     L.link = L.leaf = kLinkDone;

@@ -82,6 +82,7 @@ struct FrameConst {
     uint32_t sampleOffset;   // first sample of this render call
     int32_t rrStartDepth;
     uint32_t numNEE;         // NEE samples per vertex (>= 1)
+    uint32_t neeStride;      // NEE slot j of path pid is pid * neeStride + j (>= numNEE)
     uint32_t countVisits;
     uint64_t debugPath;      // path whose bounces are printed (RTX_DEBUG_PATH), ~0 = none
     // The arguments of raytrace() (raytrace.rgen:992) for this class of paths:
@@ -225,6 +226,8 @@ struct BatchDesc {
     uint64_t firstPath;   // global path index (within the call) of local path 0
     uint32_t numPaths;    // paths in this batch
     uint32_t npix;        // width * height
+    uint32_t pathBase;    // buffer index of local path 0 (0, or the serving slot's first path)
+    uint32_t pad;
 };
 
 struct PathBuffers {
@@ -279,6 +282,27 @@ constexpr uint32_t kFlagNrcTrain = 1u << 23;      // a training path past its tr
 // empty); kShade takes the vertex from the path's split record.
 constexpr uint32_t kSplitVertex = 1u << 31;
 
+// ------------------------------------------------------ serving pool
+// One slot of the serving pool: a batch of one job.  Paths of slot k have
+// buffer ids k * 2^slotShift + local; the bounce kernels' multi-job builds
+// read each path's scene, cache, guide, frame constants and batch here.
+struct SlotDesc {
+    DevScene S;
+    DevIC ic;
+    DevGuiding g;
+    FrameConst fc;
+    BatchDesc b;
+};
+static_assert(sizeof(SlotDesc) % 4 == 0, "slot descriptors are copied as words");
+
+struct MultiArgs {
+    const SlotDesc *slots;
+    uint32_t *slotAlive;   // per slot: paths not yet ended
+    uint32_t slotShift;    // log2 of paths per slot
+    uint32_t neeStride;    // NEE slots per path in the pool
+    uint32_t numSlots;
+};
+
 // ----------------------------------------------------------- launchers
 
 void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, Queue q0,
@@ -287,8 +311,9 @@ void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffer
 // extend[esel]), then the probe and shadow queues; fetch[1] must be zero.
 // poll (may be NULL): host-visible slot for the bounce's starting queue
 // lengths (extend, probe, shadow, done), written by the first thread
+// ldsEntries: stack entries per thread in LDS, 12, 16, 20 or 24
 void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
-                  Counters *cnt, uint32_t *poll, bool countVisits, int blocks);
+                  Counters *cnt, uint32_t *poll, bool countVisits, int blocks, int ldsEntries);
 // the two ray queries (closest hit / occlusion); *fetch must be zero
 void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_t n, void *out, bool closest,
                   uint32_t *fetch, int blocks);
@@ -322,6 +347,18 @@ void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum,
 void launchResetCounters(hipStream_t s, Counters *cnt, int which);
 // p[0..n) = v (a kernel, so that captured streams hold no memset nodes)
 void launchFillU32(hipStream_t s, uint32_t *p, uint32_t n, uint32_t v);
+// serving pool (multi-job builds of the bounce kernels)
+void launchInstallSlot(hipStream_t s, const SlotDesc &sd, SlotDesc *slots, uint32_t slot, uint32_t *alive);
+void launchGenerateM(hipStream_t s, const SlotDesc *slots, uint32_t slot, uint32_t numPaths, PathBuffers pb, Queue q0,
+                     Counters *cnt, int qsel);
+void launchTraceM(hipStream_t s, const MultiArgs &ma, bool alpha, Queue qe, int esel, Queue qs, Queue qp,
+                  PathBuffers pb, Counters *cnt, uint32_t *poll, int blocks);
+void launchProbeResolveM(hipStream_t s, const MultiArgs &ma, Queue q, PathBuffers pb, Counters *cnt, int qsel,
+                         int blocks);
+void launchFinishM(hipStream_t s, const MultiArgs &ma, PathBuffers pb, const int32_t *done, Counters *cnt,
+                   Queue qnext, int qsel, int blocks);
+void launchShadeM(hipStream_t s, const MultiArgs &ma, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp,
+                  int32_t *done, Counters *cnt, int qsel, int blocks);
 void launchResolveImage(hipStream_t s, const float4 *accum, uint32_t npix, float invSamples, float4 *out);
 
 } // namespace rtx

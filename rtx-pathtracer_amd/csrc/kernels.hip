@@ -42,6 +42,23 @@ RTX_DI int waveAppend(bool pred, uint32_t *counter) {
     return pred ? (int)(base + (uint32_t)__popcll(below)) : -1;
 }
 
+// Runs body(k) once for every distinct value k of `key` among the lanes
+// with `active`, with exactly the lanes holding that key active and k
+// wave-uniform (read with readlane, so per-key data are scalar loads).  The
+// lanes of a pass are picked by a ballot mask rather than by comparing key
+// with k inside the branch: the compiler would otherwise substitute the
+// lane's own key for k there and drop the loop.
+template <class F> RTX_DI void forEachKey(bool active, uint32_t key, F &&body) {
+    uint64_t todo = __ballot(active);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const uint64_t m = __ballot(active && key == k);
+        if ((m >> __lane_id()) & 1ull) body(k);
+        todo &= ~m;
+    }
+}
+
 RTX_DI uint32_t waveSum(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -98,7 +115,7 @@ RTX_DI void cameraRay(const FrameConst &fc, uint32_t pixel, Rng &rng, V3 &o, V3 
 // pixels with (pixel + offset) % every == 0 train, the offset moving with
 // the sample so that every pixel trains now and then.
 RTX_DI int nrcSlot(const FrameConst &fc, BatchDesc b, uint32_t pid) {
-    const uint64_t g = b.firstPath + pid;
+    const uint64_t g = b.firstPath + (pid - b.pathBase);
     const uint32_t pixel = (uint32_t)(g % b.npix);
     const uint64_t sample = g / b.npix + fc.sampleOffset;
     const uint32_t every = fc.nrcTrainEvery;
@@ -184,6 +201,52 @@ __global__ __launch_bounds__(256) void kGenerate(const FrameConst *__restrict__ 
         const int slot = nrcSlot(fc, b, pid);
         if (slot >= 0) pb.nrcTrainTarget[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); // no target until the path ends
     }
+}
+
+// The paths of one serving slot: main() up to the camera ray, as kGenerate
+// (one camera sample per path, no records), at pool ids pathBase + local,
+// appended to the extend queue qsel (the next bounce's).  With the cache on
+// the stream's first draw is the update election, which the serving pool
+// does not act on (its caches are read-only) but draws to keep the stream
+// the reference's.
+__global__ __launch_bounds__(256) void kGenerateM(const SlotDesc *__restrict__ slots, uint32_t slot, PathBuffers pb,
+                                                  Queue q0, Counters *cnt, int qsel) {
+    const SlotDesc &sd = slots[slot];
+    const FrameConst &fc = sd.fc;
+    const BatchDesc b = sd.b;
+    const uint32_t local = blockIdx.x * 256 + threadIdx.x;
+    const bool active = local < b.numPaths;
+    const uint32_t pid = b.pathBase + local;
+    V3 o = v3(0.0f), d = v3(0.0f, 0.0f, 1.0f);
+    if (active) {
+        const uint64_t g = b.firstPath + local;
+        const uint32_t pixel = (uint32_t)(g % b.npix);
+        const uint64_t sample = g / b.npix + fc.sampleOffset;
+        const uint32_t frameSeed = pcg32First(fc.seed, sample);
+        Rng rng{tea(pixel, frameSeed)};
+        if (fc.pc.useIrradianceCache) (void)rng.next();
+        cameraRay(fc, pixel, rng, o, d);
+        pb.rng[pid] = rng.s;
+        pb.hash[pid] = 2166136261u;
+        pb.frameSeed[pid] = frameSeed;
+        pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagsCameraStart));
+        pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
+    }
+    const int qi = waveAppend(active, &cnt->extend[qsel]);
+    if (active) {
+        q0.id[qi] = (int)pid;
+        q0.org[qi] = f4(o, kTMin);
+        q0.dir[qi] = f4(d, kTMax);
+    }
+}
+
+// Writes slot `slot`'s descriptor and its live-path count (the batch's
+// paths); passed by value so that the host's copy needs no lifetime.
+__global__ __launch_bounds__(256) void kInstallSlot(SlotDesc sd, SlotDesc *slots, uint32_t slot, uint32_t *alive) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&sd);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(slots + slot);
+    for (uint32_t k = threadIdx.x; k < sizeof(SlotDesc) / 4; k += 256) dst[k] = src[k];
+    if (threadIdx.x == 0) alive[slot] = sd.b.numPaths;
 }
 
 // The sub-paths of calculateCacheData (raytrace.rgen:1246-1263) for jobs
@@ -272,7 +335,7 @@ RTX_DI V3 probeContribution(const DevScene &S, bool power, V3 o, V3 d, int prim,
 // extend, 1 shadow, 2 probe).  The kernels come in two builds: without the
 // any-hit alpha test (scenes with no textured triangles, the hot path) and
 // with it (kTracePA), so that the test's code stays out of the common loop.
-template <bool COUNT, bool ALPHA>
+template <bool COUNT, bool ALPHA, int LDS>
 RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue &qs, const Queue &qp,
                       const PathBuffers &pb, Counters *cnt, uint32_t *poll, int *stackMem) {
     const uint32_t ne = cnt->extend[esel], np = cnt->probe, ns = cnt->shadow;
@@ -283,7 +346,7 @@ RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue 
         atomicAdd(&cnt->shadowRays, (unsigned long long)ns);
         reportBounce(poll, ne, np, ns, cnt->done[esel ^ 1]);
     }
-    HybridStack<kTraceBlock, kLdsStack> st;
+    HybridStack<kTraceBlock, LDS> st;
     st.lds = stackMem + threadIdx.x;
     VisitCount vc, vk[3];
     // the queues' arrays as plain pointers, read in per-queue branches (a
@@ -347,17 +410,89 @@ RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue 
     flushVisits<COUNT>(vk[1], cnt, 1);
     flushVisits<COUNT>(vk[2], cnt, 2);
 }
-template <bool COUNT>
+// LDS: stack entries per thread held in LDS (the rest in scratch), a
+// launch-shape choice (rtx_tuning.ldsStackEntries): more entries cost LDS and
+// so resident blocks.
+template <bool COUNT, int LDS = kLdsStack>
 __global__ __launch_bounds__(kTraceBlock) void kTraceP(DevScene S, Queue qe, int esel, Queue qs, Queue qp,
                                                       PathBuffers pb, Counters *cnt, uint32_t *poll) {
-    __shared__ int stackMem[kLdsStack * kTraceBlock];
-    traceBody<COUNT, false>(S, qe, esel, qs, qp, pb, cnt, poll, stackMem);
+    __shared__ int stackMem[LDS * kTraceBlock];
+    traceBody<COUNT, false, LDS>(S, qe, esel, qs, qp, pb, cnt, poll, stackMem);
 }
-template <bool COUNT>
+template <bool COUNT, int LDS = kLdsStack>
 __global__ __launch_bounds__(kTraceBlock) void kTracePA(DevScene S, Queue qe, int esel, Queue qs, Queue qp,
                                                        PathBuffers pb, Counters *cnt, uint32_t *poll) {
+    __shared__ int stackMem[LDS * kTraceBlock];
+    traceBody<COUNT, true, LDS>(S, qe, esel, qs, qp, pb, cnt, poll, stackMem);
+}
+
+// One bounce of the serving pool: the three queues of every slot's paths in
+// one persistent launch.  Each lane points its traversal at the scene of the
+// slot its ray belongs to when it fetches the ray (pid >> slotShift; probe
+// and shadow entries hold NEE slots, pid * neeStride + j).  Block 0 also
+// copies every slot's live-path count after the bounce's queue lengths into
+// the host-visible slot `poll` (4 + numSlots words).
+template <bool ALPHA>
+__global__ __launch_bounds__(kTraceBlock) void kTraceM(MultiArgs ma, Queue qe, int esel, Queue qs, Queue qp,
+                                                      PathBuffers pb, Counters *cnt, uint32_t *poll) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
-    traceBody<COUNT, true>(S, qe, esel, qs, qp, pb, cnt, poll, stackMem);
+    const uint32_t ne = cnt->extend[esel], np = cnt->probe, ns = cnt->shadow;
+    const uint32_t nep = ne + np;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&cnt->extendRays, (unsigned long long)ne);
+        atomicAdd(&cnt->probeRays, (unsigned long long)np);
+        atomicAdd(&cnt->shadowRays, (unsigned long long)ns);
+        reportBounce(poll, ne, np, ns, cnt->done[esel ^ 1]);
+    }
+    if (blockIdx.x == 0 && poll)
+        for (uint32_t k = threadIdx.x; k < ma.numSlots; k += kTraceBlock) poll[4 + k] = ma.slotAlive[k];
+    HybridStack<kTraceBlock, kLdsStack> st;
+    st.lds = stackMem + threadIdx.x;
+    VisitCount vc;
+    LaneScene ls{};
+    const float4 *eOrg = qe.org, *eDir = qe.dir, *pOrg = qp.org, *pDir = qp.dir, *sOrg = qs.org, *sDir = qs.dir;
+    const int *eId = qe.id, *pId = qp.id, *sId = qs.id;
+    float4 *eHit = qe.hit, *pHit = qp.hit;
+    float4 *pendShadow = pb.pendShadow;
+    const uint32_t *frameSeed = pb.frameSeed;
+    persistentTrace<kHitLane, false, ALPHA>(
+        ls, nep + ns, &cnt->fetch[1], st, vc,
+        [&](uint32_t i, LaneRay &L) {
+            const bool ext = i < ne, shadow = i >= nep;
+            const uint32_t j = ext ? i : shadow ? i - nep : i - ne;
+            float4 o, d;
+            uint32_t id;
+            if (ext) {
+                o = eOrg[j];
+                d = eDir[j];
+                id = (uint32_t)eId[j];
+            } else if (shadow) {
+                o = sOrg[j];
+                d = sDir[j];
+                id = (uint32_t)sId[j];
+            } else {
+                o = pOrg[j];
+                d = pDir[j];
+                id = (uint32_t)pId[j];
+            }
+            const uint32_t pid = ext ? (id & ~kSplitVertex) : id / ma.neeStride;
+            const DevScene &sc = ma.slots[pid >> ma.slotShift].S;
+            pointAt(ls, sc, &sc);
+            L.r = makeTravRay(v3(o), v3(d));
+            L.tmin = o.w;
+            L.tmax = d.w;
+            L.any = shadow;
+            if (ALPHA) L.randomUInt = frameSeed[pid];
+        },
+        [&](const LaneRay &L) {
+            if (L.any) {
+                if (L.best < 0) pendShadow[sId[L.idx - nep]].w = 1.0f; // raytrace.shadow.rmiss:7-10
+            } else if (L.idx < ne) {
+                eHit[L.idx] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv);
+            } else {
+                pHit[L.idx - ne] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv);
+            }
+        });
 }
 
 // The BSDF-side half of the MIS probe (raytrace.rgen:684-726) for probe rays
@@ -381,14 +516,39 @@ __global__ __launch_bounds__(256) void kProbeResolve(DevScene S, const FrameCons
     }
 }
 
+// The same over the serving pool: probe entry ids are NEE slots, path * neeStride + j.
+__global__ __launch_bounds__(256) void kProbeResolveM(MultiArgs ma, Queue q, PathBuffers pb, Counters *cnt,
+                                                      int qsel) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt->extend[qsel ^ 1] = 0;
+        cnt->done[qsel] = 0;
+    }
+    const uint32_t n = cnt->probe;
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        const uint32_t id = active ? (uint32_t)q.id[i] : 0u;
+        const uint32_t slot = (id / ma.neeStride) >> ma.slotShift;
+        forEachKey(active, slot, [&](uint32_t k) {
+            const SlotDesc &sd = ma.slots[k];
+            float4 o4 = q.org[i], d4 = q.dir[i], base4 = q.aux[i], hit = q.hit[i];
+            int prim = __float_as_int(hit.y);
+            pb.pendProbe[id] = f4(probeContribution(sd.S, sd.fc.pc.usePowerHeuristic != 0, v3(o4), v3(d4), prim,
+                                                    hit.x, hit.z, hit.w, base4),
+                                  __int_as_float(prim));
+        });
+    }
+}
+
 // Pending NEE terms of the previous vertex, in NEE-sample order.
 // pendShadow.w: 0 no shadow ray, -1 traced and occluded, +1 traced and visible.
 // pendProbe.w (int bits): -2 no probe ray, else the probe's hit id (-1 = miss).
-RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint32_t &hash) {
+// NEE slot j of path pid is pid * stride + j (stride >= numNEE: FrameConst::neeStride).
+RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint32_t stride, uint32_t &hash) {
     V3 s = v3(0.0f);
     for (uint32_t j = 0; j < numNEE; ++j) {
-        float4 sh = pb.pendShadow[pid * numNEE + j];
-        float4 pr = pb.pendProbe[pid * numNEE + j];
+        float4 sh = pb.pendShadow[pid * stride + j];
+        float4 pr = pb.pendProbe[pid * stride + j];
         if (sh.w != 0.0f) hash = sigFold(hash, sh.w > 0.0f ? kSigVisible : kSigOccluded);
         int probePrim = __float_as_int(pr.w);
         if (probePrim != -2) hash = sigFold(hash, kSigProbe ^ (uint32_t)probePrim);
@@ -438,12 +598,9 @@ RTX_DI void pushSplit(const PathBuffers &pb, uint32_t pid, int4 &task, const Hit
 // the extend queue; an entry marked kSplitVertex resumes a queued split
 // instead (raytrace.rgen:1684-1703: NEE at its vertex, a new direction, then
 // the resumed path at the split's depth).
-__global__ __launch_bounds__(kShadeBlock, 2) void kShade(DevScene S, DevIC IC, DevGuiding G,
-                                                      const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
-                                                      Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done,
-                                                      Counters *cnt, int qsel) {
-    const FrameConst &fc = *fcp;
-    const uint32_t n = cnt->extend[qsel];
+RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, const FrameConst &fc,
+                      const BatchDesc &b, const PathBuffers &pb, const Queue &qin, const Queue &qout, const Queue &qs,
+                      const Queue &qp, int32_t *done, Counters *cnt, int qsel, uint32_t i, bool active) {
     const uint32_t numNEE = fc.numNEE;
     const float invNEE = 1.0f / (float)numNEE;
     const bool useNEE = fc.useNEE != 0;
@@ -466,376 +623,401 @@ __global__ __launch_bounds__(kShadeBlock, 2) void kShade(DevScene S, DevIC IC, D
     gp.on = fc.pc.useGuiding != 0;
     gp.parallax = fc.pc.useParallaxCompensation != 0;
     gp.prob = fc.pc.guidingProb;
-    const uint32_t stride = gridDim.x * kShadeBlock;
     uint32_t *nextCount = &cnt->extend[qsel ^ 1];
-    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += stride) {
-        const uint32_t i = base + threadIdx.x;
-        const bool active = i < n;
-        bool cont = false, isDone = false, doNee = false;
-        uint32_t pid = 0;
-        Rng rng{0};
-        V3 thr = v3(0.0f), res = v3(0.0f), d = v3(0.0f);
-        uint32_t flags = 0, hash = 0;
-        HitInfo h;
-        Mat m;
-        m.type = -1;
-        int depth = 0, followCount = 0;
-        bool follow = false, addNext = false, icHit = false;
-        bool splitV = false, pendTask = false, dead = false, taskDirty = false, nrcQ = false;
-        int4 task = make_int4(0, 0, 0, 0); // (camera samples left, splits queued, next split, record sampleOffset)
-        uint32_t ncand = 0;
-        int4 grec = make_int4(0, -1, 0, 0); // record state: slot, distance slot, distanceFactor bits, NEE base
-        float firstT = kTMax;
-        if (active) {
-            const uint32_t raw = (uint32_t)qin.id[i];
-            splitV = (raw & kSplitVertex) != 0;
-            pid = raw & ~kSplitVertex;
-            float4 hit = qin.hit[i];
-            d = v3(qin.dir[i]);
-            V3 o = v3(qin.org[i]);
-            rng.s = pb.rng[pid];
-            float4 t4 = pb.thr[pid];
-            thr = v3(t4);
-            flags = __float_as_uint(t4.w);
-            float4 r4 = pb.result[pid];
-            res = v3(r4);
-            firstT = r4.w;
-            ncand = (flags & kFlagCandMask) >> kFlagCandShift;
-            hash = pb.hash[pid];
-            if (record) grec = pb.grec[pid];
-            if (segments) task = pb.task[pid];
-            if (flags & kFlagPending) {
-                V3 nee = pendingSum(pb, pid, numNEE, hash);
-                if (flags & kFlagPendTask) { // a split vertex's NEE joins main()'s result
-                    float4 tr = pb.taskRes[pid];
-                    pb.taskRes[pid] = f4(v3(tr) + nee, tr.w);
-                } else if (record) { // recorded NEE terms are kept without the throughput
-                    res = res + v3(pb.grecThr[pid]) * nee;
-                    updateSamples(pb, pid, task.w, grec.w, nee);
-                } else {
-                    res = res + nee;
+    bool cont = false, isDone = false, doNee = false;
+    uint32_t pid = 0;
+    Rng rng{0};
+    V3 thr = v3(0.0f), res = v3(0.0f), d = v3(0.0f);
+    uint32_t flags = 0, hash = 0;
+    HitInfo h;
+    Mat m;
+    m.type = -1;
+    int depth = 0, followCount = 0;
+    bool follow = false, addNext = false, icHit = false;
+    bool splitV = false, pendTask = false, dead = false, taskDirty = false, nrcQ = false;
+    int4 task = make_int4(0, 0, 0, 0); // (camera samples left, splits queued, next split, record sampleOffset)
+    uint32_t ncand = 0;
+    int4 grec = make_int4(0, -1, 0, 0); // record state: slot, distance slot, distanceFactor bits, NEE base
+    float firstT = kTMax;
+    if (active) {
+        const uint32_t raw = (uint32_t)qin.id[i];
+        splitV = (raw & kSplitVertex) != 0;
+        pid = raw & ~kSplitVertex;
+        float4 hit = qin.hit[i];
+        d = v3(qin.dir[i]);
+        V3 o = v3(qin.org[i]);
+        rng.s = pb.rng[pid];
+        float4 t4 = pb.thr[pid];
+        thr = v3(t4);
+        flags = __float_as_uint(t4.w);
+        float4 r4 = pb.result[pid];
+        res = v3(r4);
+        firstT = r4.w;
+        ncand = (flags & kFlagCandMask) >> kFlagCandShift;
+        hash = pb.hash[pid];
+        if (record) grec = pb.grec[pid];
+        if (segments) task = pb.task[pid];
+        if (flags & kFlagPending) {
+            V3 nee = pendingSum(pb, pid, numNEE, fc.neeStride, hash);
+            if (flags & kFlagPendTask) { // a split vertex's NEE joins main()'s result
+                float4 tr = pb.taskRes[pid];
+                pb.taskRes[pid] = f4(v3(tr) + nee, tr.w);
+            } else if (record) { // recorded NEE terms are kept without the throughput
+                res = res + v3(pb.grecThr[pid]) * nee;
+                updateSamples(pb, pid, task.w, grec.w, nee);
+            } else {
+                res = res + nee;
+            }
+        }
+        int prim = __float_as_int(hit.y);
+        if (splitV) {
+            // the next queued split: its vertex, throughput and depth; raytrace(...,
+            // addFirstHitLight = false, ...) from there (raytrace.rgen:1684-1703)
+            const float4 *sr = pb.splitRec + ((size_t)pid * kMaxSplits + (uint32_t)task.z) * 4;
+            const float4 a = sr[0], nb = sr[1], c = sr[2], t = sr[3];
+            task.z++;
+            taskDirty = true;
+            const uint32_t mf = __float_as_uint(nb.w);
+            h.worldPos = v3(a);
+            h.normal = v3(nb);
+            h.matIndex = (int)(mf & 0x7FFFFFFFu);
+            h.frontFace = (mf >> 31) != 0;
+            h.isSphere = false;
+            h.instanceIndex = 0;
+            h.u = c.w;
+            h.v = t.w;
+            d = -v3(c);
+            thr = v3(t);
+            m = loadMat(&S.materials[h.matIndex]);
+            applyTextures(S, m, h.u, h.v);
+            depth = __float_as_int(a.w);
+            followCount = 0;
+            follow = true;
+            addNext = false;
+            res = v3(0.0f);
+            firstT = kTMax;
+            doNee = useNEE && neeCapable(m.type) && S.lightCount > 0;
+            pendTask = doNee;
+        } else {
+        hash = sigFold(hash, (uint32_t)prim);
+        depth = (int)(flags & kFlagDepthMask) + 1;
+        followCount = (int)((flags >> kFlagFollowCountShift) & 0xFFu);
+        follow = (flags & kFlagFollow) != 0;
+        addNext = (flags & kFlagAddNext) != 0;
+        if (prim < 0) {
+            // raytrace.rmiss: the environment map (black when the scene has none)
+            if (addDirect || addNext) {
+                const V3 missColor = envColor(S, normalize(d));
+                res = res + thr * missColor;
+                if (record) {
+                    updateSamples(pb, pid, task.w, grec.x, missColor);
+                    if (grec.y != -1) recordSlots(pb, pid)[grec.y].distance = 0.0f; // infinitely far
                 }
             }
-            int prim = __float_as_int(hit.y);
-            if (splitV) {
-                // the next queued split: its vertex, throughput and depth; raytrace(...,
-                // addFirstHitLight = false, ...) from there (raytrace.rgen:1684-1703)
-                const float4 *sr = pb.splitRec + ((size_t)pid * kMaxSplits + (uint32_t)task.z) * 4;
-                const float4 a = sr[0], nb = sr[1], c = sr[2], t = sr[3];
-                task.z++;
-                taskDirty = true;
-                const uint32_t mf = __float_as_uint(nb.w);
-                h.worldPos = v3(a);
-                h.normal = v3(nb);
-                h.matIndex = (int)(mf & 0x7FFFFFFFu);
-                h.frontFace = (mf >> 31) != 0;
-                h.isSphere = false;
-                h.instanceIndex = 0;
-                h.u = c.w;
-                h.v = t.w;
-                d = -v3(c);
-                thr = v3(t);
-                m = loadMat(&S.materials[h.matIndex]);
-                applyTextures(S, m, h.u, h.v);
-                depth = __float_as_int(a.w);
-                followCount = 0;
+            isDone = true;
+        } else {
+            h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
+            m = loadMat(&S.materials[h.matIndex]);
+            applyTextures(S, m, h.u, h.v);
+            if (depth == 1) firstT = hit.x;
+            if (b.firstPath + (pid - b.pathBase) == fc.debugPath)
+                printf("[gpu] depth %d prim %d t %.9g mat %d seed %u thr %.6g %.6g %.6g d %.6g %.6g %.6g n %.6g %.6g %.6g ff %d res %.6g\n",
+                       depth, prim, hit.x, m.type, rng.s, thr.x, thr.y, thr.z, d.x, d.y, d.z, h.normal.x,
+                       h.normal.y, h.normal.z, (int)h.frontFace, res.x);
+            if (m.type == RTX_MAT_LIGHT && (addDirect || addNext)) {
+                res = res + thr * m.lightColor;
+                if (record) updateSamples(pb, pid, task.w, grec.x, m.lightColor);
+            }
+            if (hasDiscreteDirection(m.type)) {
+                addNext = true;
                 follow = true;
+                if (depth >= maxDepth) followCount++;
+                if (record && grec.y != -1) recordSlots(pb, pid)[grec.y].distance += hit.x * __int_as_float(grec.z);
+            } else {
                 addNext = false;
-                res = v3(0.0f);
-                firstT = kTMax;
-                doNee = useNEE && neeCapable(m.type) && S.lightCount > 0;
-                pendTask = doNee;
-            } else {
-            hash = sigFold(hash, (uint32_t)prim);
-            depth = (int)(flags & kFlagDepthMask) + 1;
-            followCount = (int)((flags >> kFlagFollowCountShift) & 0xFFu);
-            follow = (flags & kFlagFollow) != 0;
-            addNext = (flags & kFlagAddNext) != 0;
-            if (prim < 0) {
-                // raytrace.rmiss: the environment map (black when the scene has none)
-                if (addDirect || addNext) {
-                    const V3 missColor = envColor(S, normalize(d));
-                    res = res + thr * missColor;
-                    if (record) {
-                        updateSamples(pb, pid, task.w, grec.x, missColor);
-                        if (grec.y != -1) recordSlots(pb, pid)[grec.y].distance = 0.0f; // infinitely far
-                    }
-                }
-                isDone = true;
-            } else {
-                h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
-                m = loadMat(&S.materials[h.matIndex]);
-                applyTextures(S, m, h.u, h.v);
-                if (depth == 1) firstT = hit.x;
-                if (b.firstPath + pid == fc.debugPath)
-                    printf("[gpu] depth %d prim %d t %.9g mat %d seed %u thr %.6g %.6g %.6g d %.6g %.6g %.6g n %.6g %.6g %.6g ff %d res %.6g\n",
-                           depth, prim, hit.x, m.type, rng.s, thr.x, thr.y, thr.z, d.x, d.y, d.z, h.normal.x,
-                           h.normal.y, h.normal.z, (int)h.frontFace, res.x);
-                if (m.type == RTX_MAT_LIGHT && (addDirect || addNext)) {
-                    res = res + thr * m.lightColor;
-                    if (record) updateSamples(pb, pid, task.w, grec.x, m.lightColor);
-                }
-                if (hasDiscreteDirection(m.type)) {
-                    addNext = true;
-                    follow = true;
-                    if (depth >= maxDepth) followCount++;
-                    if (record && grec.y != -1) recordSlots(pb, pid)[grec.y].distance += hit.x * __int_as_float(grec.z);
-                } else {
-                    addNext = false;
-                    follow = false;
-                    bool nee = useNEE;
-                    if (fc.nrc && depth >= (int)fc.nrcDepth && !(flags & kFlagNrcTrain)) {
-                        const int slot = nrcSlot(fc, b, pid);
-                        if (slot < 0) { // the path ends into the learned cache here, without NEE
-                            nrcQ = true;
-                            isDone = true;
-                            nee = false;
-                        } else { // a training path: what it adds from here on is this vertex's target
-                            float rough;
-                            V3 alb;
-                            nrcSurface(m, rough, alb);
-                            float4 *ti = pb.nrcTrainIn + 4 * (size_t)slot;
-                            ti[0] = f4(h.worldPos, rough);
-                            ti[1] = f4(-d, 0.0f);
-                            ti[2] = f4(h.normal, 0.0f);
-                            ti[3] = f4(alb, 0.0f);
-                            pb.nrcPath[2 * (size_t)pid] = f4(thr, __int_as_float(slot));
-                            pb.nrcPath[2 * (size_t)pid + 1] = f4(res, 0.0f);
-                            flags |= kFlagNrcTrain;
-                        }
-                    }
-                    if ((useIC || (useADRRS && depth > 1)) && isICCapable(m.type, icOnGlossy)) { // raytrace.rgen:1080-1139
-                        V3 irr;
-                        if (icQuery(IC, icp, h.worldPos, h.normal, irr)) {
-                            if (useADRRS) { // weight window against the cache's irradiance (1087-1116)
-                                const uint32_t pixel = (uint32_t)((b.firstPath + pid) % b.npix);
-                                int nw;
-                                float q = weightWindow(rng, v3(pb.estimate[pixel]), irr, thr, adrrsS, nw);
-                                if (nw == 1) {
-                                    if (rng.next() > q) { // the path dies here, before its NEE
-                                        hash = sigFold(hash, kSigRRKill);
-                                        dead = true;
-                                    } else {
-                                        thr = thr / q;
-                                    }
-                                } else if (adrrsSplit) {
-                                    const int room = kMaxSplits - task.y;
-                                    if (nw - 1 > room) {
-                                        nw = room + 1;
-                                        q = (float)nw;
-                                    }
-                                    thr = thr / q;
-                                    for (int k = 0; k < nw - 1; ++k) pushSplit(pb, pid, task, h, -d, thr, depth);
-                                    taskDirty = true;
-                                    hash = sigFold(hash, kSigSplit ^ (uint32_t)(nw - 1));
-                                }
-                            } else {
-                                // a cache record ends the path after its NEE (1114-1122)
-                                hash = sigFold(hash, kSigCacheHit);
-                                res = res + thr * approxDiffuse(m, h.normal, -d) * irr;
-                                icHit = true;
-                                nee = true;
-                            }
-                        } else if (createIC && rng.next() < fc.pc.irradianceCreateProb) {
-                            if (ncand < (uint32_t)kMaxNewCacheEntries) {
-                                float4 *c = pb.cand + 2 * ((size_t)pid * kMaxNewCacheEntries + ncand);
-                                c[0] = f4(h.worldPos, 0.0f);
-                                c[1] = f4(h.normal, 0.0f);
-                                ncand++;
-                            }
-                        }
-                    }
-                    if (!dead && splitOnFirst && depth == 1 && task.y < kMaxSplits) { // raytrace.rgen:1142-1146
-                        pushSplit(pb, pid, task, h, -d, thr * 0.5f, depth);
-                        thr = thr * 0.5f;
-                        taskDirty = true;
-                        hash = sigFold(hash, kSigSplit ^ 1u);
-                    }
-                    if (record && !icHit && !dead && grec.y != -1) { // raytrace.rgen:1153-1160
-                        recordSlots(pb, pid)[grec.y].distance += hit.x * __int_as_float(grec.z);
-                        if (!isMatAlmostDiscrete(m)) grec.y = -1;
-                    }
-                    if (dead) {
+                follow = false;
+                bool nee = useNEE;
+                if (fc.nrc && depth >= (int)fc.nrcDepth && !(flags & kFlagNrcTrain)) {
+                    const int slot = nrcSlot(fc, b, pid);
+                    if (slot < 0) { // the path ends into the learned cache here, without NEE
+                        nrcQ = true;
                         isDone = true;
                         nee = false;
+                    } else { // a training path: what it adds from here on is this vertex's target
+                        float rough;
+                        V3 alb;
+                        nrcSurface(m, rough, alb);
+                        float4 *ti = pb.nrcTrainIn + 4 * (size_t)slot;
+                        ti[0] = f4(h.worldPos, rough);
+                        ti[1] = f4(-d, 0.0f);
+                        ti[2] = f4(h.normal, 0.0f);
+                        ti[3] = f4(alb, 0.0f);
+                        pb.nrcPath[2 * (size_t)pid] = f4(thr, __int_as_float(slot));
+                        pb.nrcPath[2 * (size_t)pid + 1] = f4(res, 0.0f);
+                        flags |= kFlagNrcTrain;
                     }
-                    doNee = nee && neeCapable(m.type) && S.lightCount > 0;
-                    if (record) grec.w = grec.x; // the NEE light reaches the records made so far
                 }
-            }
-            } // not a split vertex
-        }
-        // ---- next event estimation, raytrace.rgen:601-731 (numNEE samples, multipleNEE 871-877)
-        const V3 wi = -d;
-        const V3 thrNee = thr; // the throughput NEE terms of this vertex carry
-        for (uint32_t j = 0; j < numNEE; ++j) {
-            bool wantShadow = false, wantProbe = false;
-            V3 lightDir = v3(0.0f), bsdfDir = v3(0.0f), probeBase = v3(0.0f);
-            float lightDist = 0.0f, pdfMatProbe = 0.0f;
-            const uint32_t slot = pid * numNEE + j;
-            if (doNee) {
-                V3 lightColor;
-                float pdfL = sampleLights(S, rng, h.worldPos, h.normal, lightDir, lightColor, lightDist);
-                float cosS = dot(h.normal, lightDir);
-                bool traced = cosS > 0.0f && pdfL > 0.0f;
-                bool probeAllowed = true;
-                float shadowState = traced ? -1.0f : 0.0f; // occluded until the shadow kernel says otherwise
-                V3 sh = v3(0.0f);
-                if (traced) {
-                    if (mis) {
-                        float pdfMat = pdfBSDF(m, h.normal, wi, lightDir);
-                        float hw = misWeight(power, pdfL, pdfMat);
-                        if (isnan(hw)) {
-                            // The reference returns from NEE here if the shadow ray is
-                            // unoccluded (raytrace.rgen:653-655), skipping the probe's draws:
-                            // resolve visibility now to keep the random stream in step.
-                            PrivateStack ps;
-                            VisitCount vc0;
-                            TravRay sr = makeTravRay(h.worldPos, lightDir);
-                            float tm = lightDist * (1.0f - 0.0001f), bu, bv;
-                            bool occl = traverse<true, false>(S, sr, kTMin, tm, bu, bv, ps, vc0, pb.frameSeed[pid]) >= 0;
-                            traced = false;
-                            shadowState = occl ? -1.0f : 1.0f; // resolved here; contributes nothing
-                            if (!occl) probeAllowed = false;
+                if ((useIC || (useADRRS && depth > 1)) && isICCapable(m.type, icOnGlossy)) { // raytrace.rgen:1080-1139
+                    V3 irr;
+                    if (icQuery(IC, icp, h.worldPos, h.normal, irr)) {
+                        if (useADRRS) { // weight window against the cache's irradiance (1087-1116)
+                            const uint32_t pixel = (uint32_t)((b.firstPath + (pid - b.pathBase)) % b.npix);
+                            int nw;
+                            float q = weightWindow(rng, v3(pb.estimate[pixel]), irr, thr, adrrsS, nw);
+                            if (nw == 1) {
+                                if (rng.next() > q) { // the path dies here, before its NEE
+                                    hash = sigFold(hash, kSigRRKill);
+                                    dead = true;
+                                } else {
+                                    thr = thr / q;
+                                }
+                            } else if (adrrsSplit) {
+                                const int room = kMaxSplits - task.y;
+                                if (nw - 1 > room) {
+                                    nw = room + 1;
+                                    q = (float)nw;
+                                }
+                                thr = thr / q;
+                                for (int k = 0; k < nw - 1; ++k) pushSplit(pb, pid, task, h, -d, thr, depth);
+                                taskDirty = true;
+                                hash = sigFold(hash, kSigSplit ^ (uint32_t)(nw - 1));
+                            }
                         } else {
-                            sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor * hw / pdfL;
+                            // a cache record ends the path after its NEE (1114-1122)
+                            hash = sigFold(hash, kSigCacheHit);
+                            res = res + thr * approxDiffuse(m, h.normal, -d) * irr;
+                            icHit = true;
+                            nee = true;
                         }
-                    } else {
-                        sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor / pdfL;
+                    } else if (createIC && rng.next() < fc.pc.irradianceCreateProb) {
+                        if (ncand < (uint32_t)kMaxNewCacheEntries) {
+                            float4 *c = pb.cand + 2 * ((size_t)pid * kMaxNewCacheEntries + ncand);
+                            c[0] = f4(h.worldPos, 0.0f);
+                            c[1] = f4(h.normal, 0.0f);
+                            ncand++;
+                        }
                     }
                 }
-                const V3 tw = record ? v3(1.0f) : thr;
-                pb.pendShadow[slot] = f4(tw * sh * invNEE, shadowState);
-                wantShadow = traced;
-                bool probeZero = true;
-                if (mis && probeAllowed) {
-                    float pdfMat = sampleBSDF(rng, m, wi, h.normal, h.frontFace, bsdfDir);
-                    if (pdfMat > 0.0f) {
-                        wantProbe = true;
-                        probeZero = false;
-                        pdfMatProbe = pdfMat;
-                        probeBase = (record ? v3(1.0f) : thr) * evalBsdf(m, h.normal, wi, bsdfDir, true) * invNEE /
-                                    pdfMat;
-                    }
-                }
-                if (probeZero) pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-2));
-            }
-            int si = waveAppend(wantShadow, &cnt->shadow);
-            if (wantShadow) {
-                qs.id[si] = (int)slot;
-                qs.org[si] = f4(h.worldPos, kTMin);
-                qs.dir[si] = f4(lightDir, lightDist * (1.0f - 0.0001f));
-            }
-            int pi = waveAppend(wantProbe, &cnt->probe);
-            if (wantProbe) {
-                qp.id[pi] = (int)slot;
-                qp.org[pi] = f4(h.worldPos, kTMin);
-                qp.dir[pi] = f4(bsdfDir, kTMax);
-                qp.aux[pi] = f4(probeBase, pdfMatProbe);
-            }
-        }
-        if (fc.nrc) { // the learned cache's queries, in the order the waves reach them
-            const int qn = waveAppend(nrcQ, pb.nrcQueryCount);
-            if (nrcQ) {
-                float rough;
-                V3 alb;
-                nrcSurface(m, rough, alb);
-                float4 *q = pb.nrcQuery + 4 * (size_t)qn;
-                q[0] = f4(h.worldPos, rough);
-                q[1] = f4(-d, 0.0f);
-                q[2] = f4(h.normal, 0.0f);
-                q[3] = f4(alb, 0.0f);
-                pb.nrcQueryAux[qn] = f4(thr, __uint_as_float(pid));
-            }
-        }
-        // ---- continuation, raytrace.rgen:1169-1215
-        V3 newDir = v3(0.0f);
-        if (icHit) isDone = true;
-        if (active && !isDone) {
-            float pdf = getNewDirection(rng, G, gp, m, h.worldPos, h.normal, wi, h.frontFace, newDir);
-            if (b.firstPath + pid == fc.debugPath) {
-                V3 ev = evalBsdf(m, h.normal, wi, newDir, h.frontFace);
-                printf("[gpu]   continue pdf %.9g eval %.9g new %.6g %.6g %.6g\n", pdf, ev.x, newDir.x, newDir.y, newDir.z);
-            }
-            // NaN guard beyond the reference (which stops on pdf <= 0 only): a NaN
-            // pdf or BSDF ratio ends the path instead of poisoning the pixel.
-            V3 change = v3(0.0f);
-            if (!(pdf <= 0.0f) && !isnan(pdf)) change = evalBsdf(m, h.normal, wi, newDir, h.frontFace) / pdf;
-            if (pdf <= 0.0f || isnan(pdf) || isnan(change.x) || isnan(change.y) || isnan(change.z)) {
-                isDone = true;
-                if (splitV) { // a split without a direction ends the frame's split loop (raytrace.rgen:1698-1700)
-                    task.z = task.y;
+                if (!dead && splitOnFirst && depth == 1 && task.y < kMaxSplits) { // raytrace.rgen:1142-1146
+                    pushSplit(pb, pid, task, h, -d, thr * 0.5f, depth);
+                    thr = thr * 0.5f;
                     taskDirty = true;
+                    hash = sigFold(hash, kSigSplit ^ 1u);
                 }
-            } else if (splitV) {
-                thr = thr * change;
-                cont = true; // raytrace() traces at least once
-            } else {
-                thr = thr * change;
-                if (record && grec.x < kMaxRecords) { // raytrace.rgen:1179-1199
-                    rtx_directional_data &dd = recordSlots(pb, pid)[grec.x];
-                    if (hasDiscreteDirection(m.type) || isMatAlmostDiscrete(m)) {
-                        dd.flags = kDirInvalid;
-                    } else {
-                        dd.position[0] = h.worldPos.x;
-                        dd.position[1] = h.worldPos.y;
-                        dd.position[2] = h.worldPos.z;
-                        dd.direction[0] = newDir.x;
-                        dd.direction[1] = newDir.y;
-                        dd.direction[2] = newDir.z;
-                        dd.pdf = pdf;
-                        dd.weight = 0.0f;
-                        dd.distance = 0.0f;
-                        dd.flags = (uint32_t)guidingRegion(G, h.worldPos);
-                        grec.y = grec.x;
-                        grec.z = __float_as_int(1.0f);
-                    }
-                    pb.lightSum[(size_t)pid * kMaxRecords + grec.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    pb.sampleThr[(size_t)pid * kMaxRecords + grec.x] = f4(change, 0.0f);
-                    grec.x++;
-                } else if (record && grec.y != -1 && m.type == RTX_MAT_DIELECTRIC && dot(newDir, h.normal) < 0.0f) {
-                    // refraction: the next distance is the apparent one (raytrace.rgen:1200-1210)
-                    float eta = h.frontFace ? m.ior : m.iorInv;
-                    grec.z = __float_as_int(fabsf(dot(h.normal, d) / dot(h.normal, newDir)) * eta);
+                if (record && !icHit && !dead && grec.y != -1) { // raytrace.rgen:1153-1160
+                    recordSlots(pb, pid)[grec.y].distance += hit.x * __int_as_float(grec.z);
+                    if (!isMatAlmostDiscrete(m)) grec.y = -1;
                 }
-                bool killed = false;
-                if (fc.allowRR && fc.pc.enableRR && depth >= fc.rrStartDepth) { // Russian roulette (not in the reference)
-                    float q = fmaxf(fabsf(thr.x), fmaxf(fabsf(thr.y), fabsf(thr.z)));
-                    q = fminf(q, 0.95f);
-                    if (rng.next() >= q) killed = true;
-                    else thr = thr / q;
-                }
-                if (killed) {
+                if (dead) {
                     isDone = true;
+                    nee = false;
+                }
+                doNee = nee && neeCapable(m.type) && S.lightCount > 0;
+                if (record) grec.w = grec.x; // the NEE light reaches the records made so far
+            }
+        }
+        } // not a split vertex
+    }
+    // ---- next event estimation, raytrace.rgen:601-731 (numNEE samples, multipleNEE 871-877)
+    const V3 wi = -d;
+    const V3 thrNee = thr; // the throughput NEE terms of this vertex carry
+    for (uint32_t j = 0; j < numNEE; ++j) {
+        bool wantShadow = false, wantProbe = false;
+        V3 lightDir = v3(0.0f), bsdfDir = v3(0.0f), probeBase = v3(0.0f);
+        float lightDist = 0.0f, pdfMatProbe = 0.0f;
+        const uint32_t slot = pid * fc.neeStride + j;
+        if (doNee) {
+            V3 lightColor;
+            float pdfL = sampleLights(S, rng, h.worldPos, h.normal, lightDir, lightColor, lightDist);
+            float cosS = dot(h.normal, lightDir);
+            bool traced = cosS > 0.0f && pdfL > 0.0f;
+            bool probeAllowed = true;
+            float shadowState = traced ? -1.0f : 0.0f; // occluded until the shadow kernel says otherwise
+            V3 sh = v3(0.0f);
+            if (traced) {
+                if (mis) {
+                    float pdfMat = pdfBSDF(m, h.normal, wi, lightDir);
+                    float hw = misWeight(power, pdfL, pdfMat);
+                    if (isnan(hw)) {
+                        // The reference returns from NEE here if the shadow ray is
+                        // unoccluded (raytrace.rgen:653-655), skipping the probe's draws:
+                        // resolve visibility now to keep the random stream in step.
+                        PrivateStack ps;
+                        VisitCount vc0;
+                        TravRay sr = makeTravRay(h.worldPos, lightDir);
+                        float tm = lightDist * (1.0f - 0.0001f), bu, bv;
+                        bool occl = traverse<true, false>(S, sr, kTMin, tm, bu, bv, ps, vc0, pb.frameSeed[pid]) >= 0;
+                        traced = false;
+                        shadowState = occl ? -1.0f : 1.0f; // resolved here; contributes nothing
+                        if (!occl) probeAllowed = false;
+                    } else {
+                        sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor * hw / pdfL;
+                    }
                 } else {
-                    cont = depth <= maxDepth || (follow && followCount <= maxFollow);
-                    isDone = !cont;
+                    sh = evalBsdf(m, h.normal, wi, lightDir, true) * lightColor / pdfL;
                 }
             }
+            const V3 tw = record ? v3(1.0f) : thr;
+            pb.pendShadow[slot] = f4(tw * sh * invNEE, shadowState);
+            wantShadow = traced;
+            bool probeZero = true;
+            if (mis && probeAllowed) {
+                float pdfMat = sampleBSDF(rng, m, wi, h.normal, h.frontFace, bsdfDir);
+                if (pdfMat > 0.0f) {
+                    wantProbe = true;
+                    probeZero = false;
+                    pdfMatProbe = pdfMat;
+                    probeBase = (record ? v3(1.0f) : thr) * evalBsdf(m, h.normal, wi, bsdfDir, true) * invNEE /
+                                pdfMat;
+                }
+            }
+            if (probeZero) pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-2));
         }
-        if (active) {
-            uint32_t nf = (uint32_t)depth | ((uint32_t)followCount << kFlagFollowCountShift) |
-                          (follow ? kFlagFollow : 0u) | (addNext ? kFlagAddNext : 0u) |
-                          (doNee ? kFlagPending : 0u) | (doNee && pendTask ? kFlagPendTask : 0u) |
-                          (ncand << kFlagCandShift) | (flags & kFlagNrcTrain);
-            if (taskDirty) pb.task[pid] = task;
-            pb.rng[pid] = rng.s;
-            pb.hash[pid] = hash;
-            pb.thr[pid] = f4(thr, __uint_as_float(nf));
-            pb.result[pid] = f4(res, firstT);
-            if (record) {
-                if (doNee) pb.grecThr[pid] = f4(thrNee, 0.0f); // the throughput the pending NEE terms carry
-                pb.grec[pid] = grec;
+        int si = waveAppend(wantShadow, &cnt->shadow);
+        if (wantShadow) {
+            qs.id[si] = (int)slot;
+            qs.org[si] = f4(h.worldPos, kTMin);
+            qs.dir[si] = f4(lightDir, lightDist * (1.0f - 0.0001f));
+        }
+        int pi = waveAppend(wantProbe, &cnt->probe);
+        if (wantProbe) {
+            qp.id[pi] = (int)slot;
+            qp.org[pi] = f4(h.worldPos, kTMin);
+            qp.dir[pi] = f4(bsdfDir, kTMax);
+            qp.aux[pi] = f4(probeBase, pdfMatProbe);
+        }
+    }
+    if (fc.nrc) { // the learned cache's queries, in the order the waves reach them
+        const int qn = waveAppend(nrcQ, pb.nrcQueryCount);
+        if (nrcQ) {
+            float rough;
+            V3 alb;
+            nrcSurface(m, rough, alb);
+            float4 *q = pb.nrcQuery + 4 * (size_t)qn;
+            q[0] = f4(h.worldPos, rough);
+            q[1] = f4(-d, 0.0f);
+            q[2] = f4(h.normal, 0.0f);
+            q[3] = f4(alb, 0.0f);
+            pb.nrcQueryAux[qn] = f4(thr, __uint_as_float(pid));
+        }
+    }
+    // ---- continuation, raytrace.rgen:1169-1215
+    V3 newDir = v3(0.0f);
+    if (icHit) isDone = true;
+    if (active && !isDone) {
+        float pdf = getNewDirection(rng, G, gp, m, h.worldPos, h.normal, wi, h.frontFace, newDir);
+        if (b.firstPath + (pid - b.pathBase) == fc.debugPath) {
+            V3 ev = evalBsdf(m, h.normal, wi, newDir, h.frontFace);
+            printf("[gpu]   continue pdf %.9g eval %.9g new %.6g %.6g %.6g\n", pdf, ev.x, newDir.x, newDir.y, newDir.z);
+        }
+        // NaN guard beyond the reference (which stops on pdf <= 0 only): a NaN
+        // pdf or BSDF ratio ends the path instead of poisoning the pixel.
+        V3 change = v3(0.0f);
+        if (!(pdf <= 0.0f) && !isnan(pdf)) change = evalBsdf(m, h.normal, wi, newDir, h.frontFace) / pdf;
+        if (pdf <= 0.0f || isnan(pdf) || isnan(change.x) || isnan(change.y) || isnan(change.z)) {
+            isDone = true;
+            if (splitV) { // a split without a direction ends the frame's split loop (raytrace.rgen:1698-1700)
+                task.z = task.y;
+                taskDirty = true;
+            }
+        } else if (splitV) {
+            thr = thr * change;
+            cont = true; // raytrace() traces at least once
+        } else {
+            thr = thr * change;
+            if (record && grec.x < kMaxRecords) { // raytrace.rgen:1179-1199
+                rtx_directional_data &dd = recordSlots(pb, pid)[grec.x];
+                if (hasDiscreteDirection(m.type) || isMatAlmostDiscrete(m)) {
+                    dd.flags = kDirInvalid;
+                } else {
+                    dd.position[0] = h.worldPos.x;
+                    dd.position[1] = h.worldPos.y;
+                    dd.position[2] = h.worldPos.z;
+                    dd.direction[0] = newDir.x;
+                    dd.direction[1] = newDir.y;
+                    dd.direction[2] = newDir.z;
+                    dd.pdf = pdf;
+                    dd.weight = 0.0f;
+                    dd.distance = 0.0f;
+                    dd.flags = (uint32_t)guidingRegion(G, h.worldPos);
+                    grec.y = grec.x;
+                    grec.z = __float_as_int(1.0f);
+                }
+                pb.lightSum[(size_t)pid * kMaxRecords + grec.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                pb.sampleThr[(size_t)pid * kMaxRecords + grec.x] = f4(change, 0.0f);
+                grec.x++;
+            } else if (record && grec.y != -1 && m.type == RTX_MAT_DIELECTRIC && dot(newDir, h.normal) < 0.0f) {
+                // refraction: the next distance is the apparent one (raytrace.rgen:1200-1210)
+                float eta = h.frontFace ? m.ior : m.iorInv;
+                grec.z = __float_as_int(fabsf(dot(h.normal, d) / dot(h.normal, newDir)) * eta);
+            }
+            bool killed = false;
+            if (fc.allowRR && fc.pc.enableRR && depth >= fc.rrStartDepth) { // Russian roulette (not in the reference)
+                float q = fmaxf(fabsf(thr.x), fmaxf(fabsf(thr.y), fabsf(thr.z)));
+                q = fminf(q, 0.95f);
+                if (rng.next() >= q) killed = true;
+                else thr = thr / q;
+            }
+            if (killed) {
+                isDone = true;
+            } else {
+                cont = depth <= maxDepth || (follow && followCount <= maxFollow);
+                isDone = !cont;
             }
         }
-        int qi = waveAppend(cont, nextCount);
-        if (cont) {
-            qout.id[qi] = (int)pid;
-            qout.org[qi] = f4(h.worldPos, kTMin);
-            qout.dir[qi] = f4(newDir, kTMax);
+    }
+    if (active) {
+        uint32_t nf = (uint32_t)depth | ((uint32_t)followCount << kFlagFollowCountShift) |
+                      (follow ? kFlagFollow : 0u) | (addNext ? kFlagAddNext : 0u) |
+                      (doNee ? kFlagPending : 0u) | (doNee && pendTask ? kFlagPendTask : 0u) |
+                      (ncand << kFlagCandShift) | (flags & kFlagNrcTrain);
+        if (taskDirty) pb.task[pid] = task;
+        pb.rng[pid] = rng.s;
+        pb.hash[pid] = hash;
+        pb.thr[pid] = f4(thr, __uint_as_float(nf));
+        pb.result[pid] = f4(res, firstT);
+        if (record) {
+            if (doNee) pb.grecThr[pid] = f4(thrNee, 0.0f); // the throughput the pending NEE terms carry
+            pb.grec[pid] = grec;
         }
-        int di = waveAppend(isDone, &cnt->done[qsel]);
-        if (isDone) done[di] = (int)pid;
+    }
+    int qi = waveAppend(cont, nextCount);
+    if (cont) {
+        qout.id[qi] = (int)pid;
+        qout.org[qi] = f4(h.worldPos, kTMin);
+        qout.dir[qi] = f4(newDir, kTMax);
+    }
+    int di = waveAppend(isDone, &cnt->done[qsel]);
+    if (isDone) done[di] = (int)pid;
+}
+
+__global__ __launch_bounds__(kShadeBlock, 2) void kShade(DevScene S, DevIC IC, DevGuiding G,
+                                                      const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
+                                                      Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done,
+                                                      Counters *cnt, int qsel) {
+    const uint32_t n = cnt->extend[qsel];
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+        const uint32_t i = base + threadIdx.x;
+        shadeLane(S, IC, G, *fcp, b, pb, qin, qout, qs, qp, done, cnt, qsel, i, i < n);
+    }
+}
+
+// The same over the serving pool: each queue entry's path belongs to the
+// slot pid >> slotShift, whose scene, cache, guide, frame constants and batch
+// the lane shades with.  A wave takes one pass per slot among its lanes
+// (forEachKey), so the slot's data are read with scalar loads as in the
+// single-job kernel.
+__global__ __launch_bounds__(kShadeBlock, 2) void kShadeM(MultiArgs ma, PathBuffers pb, Queue qin, Queue qout,
+                                                       Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel) {
+    const uint32_t n = cnt->extend[qsel];
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        const uint32_t slot = active ? ((uint32_t)qin.id[i] & ~kSplitVertex) >> ma.slotShift : 0u;
+        forEachKey(active, slot, [&](uint32_t k) {
+            const SlotDesc &sd = ma.slots[k];
+            shadeLane(sd.S, sd.ic, sd.g, sd.fc, sd.b, pb, qin, qout, qs, qp, done, cnt, qsel, i, true);
+        });
     }
 }
 
@@ -849,101 +1031,132 @@ __global__ __launch_bounds__(kShadeBlock, 2) void kShade(DevScene S, DevIC IC, D
 // qsel, before its kShade, on the paths the previous bounce's kShade ended
 // (done list qsel ^ 1, whose NEE rays this bounce traced), appending to the
 // next bounce's extend queue (qn, count extend[qsel ^ 1]).
-__global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
-                                               const int32_t *done, Counters *cnt, Queue qn, int qsel) {
-    const FrameConst &fc = *fcp;
-    const uint32_t n = cnt->done[qsel ^ 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // the secondary queues were traced and resolved; kShade refills them
-        cnt->shadow = 0;
-        cnt->probe = 0;
-        cnt->fetch[0] = cnt->fetch[1] = cnt->fetch[2] = 0;
-    }
+// Returns whether the lane's path ended for good (its result is final).
+RTX_DI bool finishLane(const FrameConst &fc, const BatchDesc &b, const PathBuffers &pb, const int32_t *done,
+                       Counters *cnt, const Queue &qn, int qsel, uint32_t i, bool active) {
     const uint32_t numNEE = fc.numNEE;
     const bool record = fc.record != 0;
     const bool segments = fc.segments != 0;
     uint32_t *nextCount = &cnt->extend[qsel ^ 1];
+    bool again = false, splitNext = false;
+    uint32_t pid = 0;
+    V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
+    if (active) {
+        pid = (uint32_t)done[i];
+        float4 r4 = pb.result[pid];
+        V3 res = v3(r4);
+        uint32_t flags = __float_as_uint(pb.thr[pid].w);
+        uint32_t hash = pb.hash[pid];
+        int4 grec = record ? pb.grec[pid] : make_int4(0, -1, 0, 0);
+        int4 task = segments ? pb.task[pid] : make_int4(0, 0, 0, 0);
+        V3 tr = segments ? v3(pb.taskRes[pid]) : v3(0.0f);
+        if (flags & kFlagPending) {
+            V3 nee = pendingSum(pb, pid, numNEE, fc.neeStride, hash);
+            if (flags & kFlagPendTask) {
+                tr = tr + nee;
+            } else if (record) {
+                res = res + v3(pb.grecThr[pid]) * nee;
+                updateSamples(pb, pid, task.w, grec.w, nee);
+            } else {
+                res = res + nee;
+            }
+        }
+        if (flags & kFlagNrcTrain) { // the training vertex's target: what the path added after it
+            const float4 tp = pb.nrcPath[2 * (size_t)pid], r0 = pb.nrcPath[2 * (size_t)pid + 1];
+            const V3 add = res - v3(r0);
+            const V3 t = v3(tp.x != 0.0f ? add.x / tp.x : 0.0f, tp.y != 0.0f ? add.y / tp.y : 0.0f,
+                            tp.z != 0.0f ? add.z / tp.z : 0.0f);
+            pb.nrcTrainTarget[__float_as_int(tp.w)] = f4(t, 1.0f);
+        }
+        if (record) { // commitSamples
+            rtx_directional_data *dd = recordSlots(pb, pid);
+            const float4 *ls = pb.lightSum + (size_t)pid * kMaxRecords;
+            for (int k = task.w; k < grec.x; ++k) {
+                float w = length(v3(ls[k])) / dd[k].pdf;
+                if (w <= 0.0f) dd[k].flags = kDirInvalid;
+                else dd[k].weight = w;
+            }
+            // a path without light leaves its slots to the next one; otherwise
+            // the next starts at sampleOffset + currentSampleOffset
+            if (length(res) > 0.0f) task.w += grec.x;
+        }
+        if (!segments) {
+            pb.out[pid] = f4(res, r4.w);
+            pb.outSig[pid] = make_uint2(pb.rng[pid], hash);
+        } else {
+            tr = tr + res; // result += raytrace(...)
+            const uint32_t keepCand = flags & kFlagCandMask;
+            Rng rng{pb.rng[pid]};
+            if (task.x > 0) {
+                task.x--;
+                const uint32_t pixel = (uint32_t)((b.firstPath + (pid - b.pathBase)) % b.npix);
+                cameraRay(fc, pixel, rng, o, dir);
+                again = true;
+                pb.rng[pid] = rng.s;
+                pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagsCameraStart | keepCand));
+                if (record) pb.grec[pid] = make_int4(task.w, -1, __float_as_int(1.0f), task.w);
+            } else if (task.z < task.y) {
+                again = splitNext = true;
+                pb.thr[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(keepCand));
+            } else {
+                pb.out[pid] = f4(tr / (float)fc.samplesPerFrame, r4.w);
+                pb.outSig[pid] = make_uint2(rng.s, hash);
+            }
+            if (again) {
+                pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
+                pb.hash[pid] = hash;
+            }
+            pb.task[pid] = task;
+            pb.taskRes[pid] = f4(tr, 0.0f);
+        }
+    }
+    int qi = waveAppend(again, nextCount);
+    if (again) {
+        // a split's entry carries an empty ray (tmax < tmin): traversal finds nothing
+        qn.id[qi] = (int)(splitNext ? (pid | kSplitVertex) : pid);
+        qn.org[qi] = f4(o, splitNext ? 1.0f : kTMin);
+        qn.dir[qi] = f4(dir, splitNext ? -1.0f : kTMax);
+    }
+    return active && !again;
+}
+
+// Clears the secondary queues' counters: they were traced and resolved, and
+// kShade refills them.
+RTX_DI void finishClears(Counters *cnt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt->shadow = 0;
+        cnt->probe = 0;
+        cnt->fetch[0] = cnt->fetch[1] = cnt->fetch[2] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fcp, BatchDesc b, PathBuffers pb,
+                                               const int32_t *done, Counters *cnt, Queue qn, int qsel) {
+    const uint32_t n = cnt->done[qsel ^ 1];
+    finishClears(cnt);
     for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
         const uint32_t i = base + threadIdx.x;
-        bool again = false, splitNext = false;
-        uint32_t pid = 0;
-        V3 o = v3(0.0f), dir = v3(0.0f, 0.0f, 1.0f);
-        if (i < n) {
-            pid = (uint32_t)done[i];
-            float4 r4 = pb.result[pid];
-            V3 res = v3(r4);
-            uint32_t flags = __float_as_uint(pb.thr[pid].w);
-            uint32_t hash = pb.hash[pid];
-            int4 grec = record ? pb.grec[pid] : make_int4(0, -1, 0, 0);
-            int4 task = segments ? pb.task[pid] : make_int4(0, 0, 0, 0);
-            V3 tr = segments ? v3(pb.taskRes[pid]) : v3(0.0f);
-            if (flags & kFlagPending) {
-                V3 nee = pendingSum(pb, pid, numNEE, hash);
-                if (flags & kFlagPendTask) {
-                    tr = tr + nee;
-                } else if (record) {
-                    res = res + v3(pb.grecThr[pid]) * nee;
-                    updateSamples(pb, pid, task.w, grec.w, nee);
-                } else {
-                    res = res + nee;
-                }
-            }
-            if (flags & kFlagNrcTrain) { // the training vertex's target: what the path added after it
-                const float4 tp = pb.nrcPath[2 * (size_t)pid], r0 = pb.nrcPath[2 * (size_t)pid + 1];
-                const V3 add = res - v3(r0);
-                const V3 t = v3(tp.x != 0.0f ? add.x / tp.x : 0.0f, tp.y != 0.0f ? add.y / tp.y : 0.0f,
-                                tp.z != 0.0f ? add.z / tp.z : 0.0f);
-                pb.nrcTrainTarget[__float_as_int(tp.w)] = f4(t, 1.0f);
-            }
-            if (record) { // commitSamples
-                rtx_directional_data *dd = recordSlots(pb, pid);
-                const float4 *ls = pb.lightSum + (size_t)pid * kMaxRecords;
-                for (int k = task.w; k < grec.x; ++k) {
-                    float w = length(v3(ls[k])) / dd[k].pdf;
-                    if (w <= 0.0f) dd[k].flags = kDirInvalid;
-                    else dd[k].weight = w;
-                }
-                // a path without light leaves its slots to the next one; otherwise
-                // the next starts at sampleOffset + currentSampleOffset
-                if (length(res) > 0.0f) task.w += grec.x;
-            }
-            if (!segments) {
-                pb.out[pid] = f4(res, r4.w);
-                pb.outSig[pid] = make_uint2(pb.rng[pid], hash);
-            } else {
-                tr = tr + res; // result += raytrace(...)
-                const uint32_t keepCand = flags & kFlagCandMask;
-                Rng rng{pb.rng[pid]};
-                if (task.x > 0) {
-                    task.x--;
-                    const uint32_t pixel = (uint32_t)((b.firstPath + pid) % b.npix);
-                    cameraRay(fc, pixel, rng, o, dir);
-                    again = true;
-                    pb.rng[pid] = rng.s;
-                    pb.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(kFlagsCameraStart | keepCand));
-                    if (record) pb.grec[pid] = make_int4(task.w, -1, __float_as_int(1.0f), task.w);
-                } else if (task.z < task.y) {
-                    again = splitNext = true;
-                    pb.thr[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(keepCand));
-                } else {
-                    pb.out[pid] = f4(tr / (float)fc.samplesPerFrame, r4.w);
-                    pb.outSig[pid] = make_uint2(rng.s, hash);
-                }
-                if (again) {
-                    pb.result[pid] = make_float4(0.0f, 0.0f, 0.0f, kTMax);
-                    pb.hash[pid] = hash;
-                }
-                pb.task[pid] = task;
-                pb.taskRes[pid] = f4(tr, 0.0f);
-            }
-        }
-        int qi = waveAppend(again, nextCount);
-        if (again) {
-            // a split's entry carries an empty ray (tmax < tmin): traversal finds nothing
-            qn.id[qi] = (int)(splitNext ? (pid | kSplitVertex) : pid);
-            qn.org[qi] = f4(o, splitNext ? 1.0f : kTMin);
-            qn.dir[qi] = f4(dir, splitNext ? -1.0f : kTMax);
-        }
+        (void)finishLane(*fcp, b, pb, done, cnt, qn, qsel, i, i < n);
+    }
+}
+
+// The same over the serving pool, one pass per slot among a wave's lanes;
+// the paths that end leave their slot's live count (one atomic per pass).
+__global__ __launch_bounds__(256) void kFinishM(MultiArgs ma, PathBuffers pb, const int32_t *done, Counters *cnt,
+                                                Queue qn, int qsel) {
+    const uint32_t n = cnt->done[qsel ^ 1];
+    finishClears(cnt);
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        const uint32_t slot = active ? (uint32_t)done[i] >> ma.slotShift : 0u;
+        forEachKey(active, slot, [&](uint32_t k) {
+            const SlotDesc &sd = ma.slots[k];
+            const bool ended = finishLane(sd.fc, sd.b, pb, done, cnt, qn, qsel, i, true);
+            const uint64_t m = __ballot(ended); // the lanes of slot k only
+            if (m && (int)__lane_id() == __ffsll((unsigned long long)m) - 1)
+                atomicSub(&ma.slotAlive[k], (uint32_t)__popcll(m));
+        });
     }
 }
 
@@ -1042,15 +1255,25 @@ void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t 
     hipLaunchKernelGGL(kGenerateIC, dim3(gridFor(numPaths, 256)), dim3(256), 0, s, ic, frameSeed, firstJob, numPaths,
                        pb, q0, cnt, qsel);
 }
-void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
-                  Counters *cnt, uint32_t *poll, bool countVisits, int blocks) {
+template <int LDS>
+static void launchTracePL(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
+                          Counters *cnt, uint32_t *poll, bool countVisits, int blocks) {
     const dim3 g(blocks), t(kTraceBlock);
     if (S.alphaTris != 0) {
-        if (countVisits) hipLaunchKernelGGL(kTracePA<true>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
-        else hipLaunchKernelGGL(kTracePA<false>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
+        if (countVisits) hipLaunchKernelGGL((kTracePA<true, LDS>), g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
+        else hipLaunchKernelGGL((kTracePA<false, LDS>), g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
     } else {
-        if (countVisits) hipLaunchKernelGGL(kTraceP<true>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
-        else hipLaunchKernelGGL(kTraceP<false>, g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
+        if (countVisits) hipLaunchKernelGGL((kTraceP<true, LDS>), g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
+        else hipLaunchKernelGGL((kTraceP<false, LDS>), g, t, 0, s, S, qe, esel, qs, qp, pb, cnt, poll);
+    }
+}
+void launchTraceP(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
+                  Counters *cnt, uint32_t *poll, bool countVisits, int blocks, int ldsEntries) {
+    switch (ldsEntries) {
+    case 12: launchTracePL<12>(s, S, qe, esel, qs, qp, pb, cnt, poll, countVisits, blocks); break;
+    case 16: launchTracePL<16>(s, S, qe, esel, qs, qp, pb, cnt, poll, countVisits, blocks); break;
+    case 24: launchTracePL<24>(s, S, qe, esel, qs, qp, pb, cnt, poll, countVisits, blocks); break;
+    default: launchTracePL<kLdsStack>(s, S, qe, esel, qs, qp, pb, cnt, poll, countVisits, blocks); break;
     }
 }
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
@@ -1081,6 +1304,32 @@ void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_
     if (n == 0) return;
     if (closest) hipLaunchKernelGGL(kQueryP<false>, dim3(blocks), dim3(kTraceBlock), 0, s, S, rays, n, out, fetch);
     else hipLaunchKernelGGL(kQueryP<true>, dim3(blocks), dim3(kTraceBlock), 0, s, S, rays, n, out, fetch);
+}
+void launchInstallSlot(hipStream_t s, const SlotDesc &sd, SlotDesc *slots, uint32_t slot, uint32_t *alive) {
+    hipLaunchKernelGGL(kInstallSlot, dim3(1), dim3(256), 0, s, sd, slots, slot, alive);
+}
+void launchGenerateM(hipStream_t s, const SlotDesc *slots, uint32_t slot, uint32_t numPaths, PathBuffers pb, Queue q0,
+                     Counters *cnt, int qsel) {
+    if (numPaths == 0) return;
+    hipLaunchKernelGGL(kGenerateM, dim3(gridFor(numPaths, 256)), dim3(256), 0, s, slots, slot, pb, q0, cnt, qsel);
+}
+void launchTraceM(hipStream_t s, const MultiArgs &ma, bool alpha, Queue qe, int esel, Queue qs, Queue qp,
+                  PathBuffers pb, Counters *cnt, uint32_t *poll, int blocks) {
+    const dim3 g(blocks), t(kTraceBlock);
+    if (alpha) hipLaunchKernelGGL(kTraceM<true>, g, t, 0, s, ma, qe, esel, qs, qp, pb, cnt, poll);
+    else hipLaunchKernelGGL(kTraceM<false>, g, t, 0, s, ma, qe, esel, qs, qp, pb, cnt, poll);
+}
+void launchProbeResolveM(hipStream_t s, const MultiArgs &ma, Queue q, PathBuffers pb, Counters *cnt, int qsel,
+                         int blocks) {
+    hipLaunchKernelGGL(kProbeResolveM, dim3(blocks), dim3(256), 0, s, ma, q, pb, cnt, qsel);
+}
+void launchFinishM(hipStream_t s, const MultiArgs &ma, PathBuffers pb, const int32_t *done, Counters *cnt,
+                   Queue qnext, int qsel, int blocks) {
+    hipLaunchKernelGGL(kFinishM, dim3(blocks), dim3(256), 0, s, ma, pb, done, cnt, qnext, qsel);
+}
+void launchShadeM(hipStream_t s, const MultiArgs &ma, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp,
+                  int32_t *done, Counters *cnt, int qsel, int blocks) {
+    hipLaunchKernelGGL(kShadeM, dim3(blocks), dim3(kShadeBlock), 0, s, ma, pb, qin, qout, qs, qp, done, cnt, qsel);
 }
 void launchFillU32(hipStream_t s, uint32_t *p, uint32_t n, uint32_t v) {
     if (n == 0) return;

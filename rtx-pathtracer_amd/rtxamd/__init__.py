@@ -137,6 +137,19 @@ def device_lib():
         L.rtx_nrc_infer.argtypes = [C.c_void_p, C.POINTER(abi.NrcInput), C.c_size_t, fp]
         L.rtx_nrc_gradients.argtypes = [C.c_void_p, C.POINTER(abi.NrcInput), fp, C.c_size_t, fp, fp, fp]
         L.rtx_nrc_train.argtypes = [C.c_void_p, C.POINTER(abi.NrcInput), fp, C.c_size_t, fp]
+        L.rtx_server_config_defaults.argtypes = [C.POINTER(abi.ServerConfig)]
+        L.rtx_server_config_defaults.restype = None
+        L.rtx_server_create.argtypes = [C.c_void_p, C.POINTER(abi.ServerConfig), C.POINTER(C.c_void_p)]
+        L.rtx_server_destroy.argtypes = [C.c_void_p]
+        L.rtx_server_submit.argtypes = [C.c_void_p, C.POINTER(abi.JobDesc), C.POINTER(C.c_uint64)]
+        L.rtx_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(abi.JobStatus)]
+        L.rtx_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(abi.JobStatus)]
+        L.rtx_server_read_image.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_size_t]
+        L.rtx_server_get_stats.argtypes = [C.c_void_p, C.POINTER(abi.ServerStats)]
+        L.rtx_server_last_error.argtypes = [C.c_void_p]
+        L.rtx_server_last_error.restype = C.c_char_p
+        L.rtx_autotune.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(FrameParams), C.c_uint32,
+                                   C.POINTER(abi.AutotuneResult)]
         L.rtx_ctx_set_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         L.rtx_ctx_get_tuning.argtypes = [C.c_void_p, C.POINTER(abi.Tuning)]
         _dev = L
@@ -553,12 +566,19 @@ class Renderer:
         return t
 
     def set_tuning(self, **kw):
-        """Launch-shape knobs (traceBlocksPerCU, shadeBlocksPerCU,
-        batchPaths); fields left out keep their current values."""
+        """Launch-shape knobs (traceBlocksPerCU, shadeBlocksPerCU, batchPaths,
+        ldsStackEntries); fields left out keep their current values."""
         t = abi.Tuning()
         for k, v in kw.items():
             setattr(t, k, v)
         self._check(device_lib().rtx_ctx_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, scene, probe, repeats=2):
+        """rtx_autotune: times `probe` under candidate launch shapes, keeps
+        the fastest on the context and returns the rtx_autotune_result."""
+        res = abi.AutotuneResult()
+        self._check(device_lib().rtx_autotune(self._h, scene._h, C.byref(probe), int(repeats), C.byref(res)))
+        return res
 
     # --------------------------------------------- learned radiance cache
     def nrc_create(self, scene_min, scene_max, **kw):
@@ -768,6 +788,96 @@ class Renderer:
         self._check(device_lib().rtx_trace_occluded(self._h, scene._h, rays.ctypes.data_as(C.POINTER(Ray)), n,
                                                     occ.ctypes.data_as(C.POINTER(C.c_uint8))))
         return occ
+
+
+def job_desc(scene, params, state=None):
+    """rtx_job_desc of one frame's worth of work: the frame parameters'
+    push constants, camera, size, spp (sample planes), seed, sampleOffset and
+    rrStartDepth; state: a Renderer whose cache / guiding regions the job
+    reads (pc.useIrradianceCache / pc.useGuiding)."""
+    d = abi.JobDesc()
+    d.scene = scene._h
+    d.state = state._h if state is not None else None
+    d.pc = params.pc
+    d.cam = params.cam
+    d.width, d.height, d.spp = params.width, params.height, params.spp
+    d.rrStartDepth = params.rrStartDepth
+    d.seed = params.seed
+    d.sampleOffset = params.sampleOffset
+    return d
+
+
+class Server:
+    """rtx_server: the serving pool on a Renderer's device."""
+
+    def __init__(self, renderer, **cfg):
+        c = abi.ServerConfig()
+        device_lib().rtx_server_config_defaults(C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.config = c
+        self._h = C.c_void_p()
+        rc = device_lib().rtx_server_create(renderer._h, C.byref(c), C.byref(self._h))
+        if rc != abi.RTX_OK:
+            raise RtxError(rc, device_lib().rtx_last_error(renderer._h).decode())
+        self._sizes = {}
+
+    def _check(self, rc):
+        if rc != abi.RTX_OK:
+            raise RtxError(rc, device_lib().rtx_server_last_error(self._h).decode())
+
+    def submit(self, scene, params, state=None):
+        d = job_desc(scene, params, state)
+        jid = C.c_uint64()
+        self._check(device_lib().rtx_server_submit(self._h, C.byref(d), C.byref(jid)))
+        self._sizes[jid.value] = (params.width, params.height)
+        return jid.value
+
+    def poll(self, jid):
+        st = abi.JobStatus()
+        self._check(device_lib().rtx_server_poll(self._h, jid, C.byref(st)))
+        return st
+
+    def wait(self, jid, timeout=None):
+        """rtx_server_wait; with a timeout (seconds) the job is polled instead
+        and TimeoutError raised when it is not done in time."""
+        st = abi.JobStatus()
+        if timeout is None:
+            self._check(device_lib().rtx_server_wait(self._h, jid, C.byref(st)))
+            return st
+        import time
+        t_end = time.monotonic() + timeout
+        while True:
+            self._check(device_lib().rtx_server_poll(self._h, jid, C.byref(st)))
+            if st.state >= abi.RTX_JOB_DONE:
+                return st
+            if time.monotonic() > t_end:
+                raise TimeoutError(f"job {jid} not done after {timeout} s (state {st.state}, "
+                                   f"{st.batchesDone}/{st.batches} batches)")
+            time.sleep(0.0005)
+
+    def read_image(self, jid):
+        w, h = self._sizes.pop(jid)
+        out = np.empty((h, w, 4), np.float32)
+        self._check(device_lib().rtx_server_read_image(self._h, jid, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                        out.size))
+        return out
+
+    def stats(self):
+        st = abi.ServerStats()
+        self._check(device_lib().rtx_server_get_stats(self._h, C.byref(st)))
+        return st
+
+    def close(self):
+        if self._h:
+            device_lib().rtx_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def schedule_config(**kw):

@@ -182,7 +182,41 @@ class FrameStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("traversalKernel", C.c_int32), ("traceBlocksPerCU", C.c_int32),
-                ("shadeBlocksPerCU", C.c_int32), ("batchPaths", C.c_uint32), ("_reserved", C.c_int32 * 4)]
+                ("shadeBlocksPerCU", C.c_int32), ("batchPaths", C.c_uint32), ("ldsStackEntries", C.c_int32),
+                ("_reserved", C.c_int32 * 3)]
+
+
+class AutotuneResult(C.Structure):
+    _fields_ = [("best", Tuning), ("bestMs", C.c_double), ("startMs", C.c_double), ("trials", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("framebufferBytes", C.c_uint64), ("slotPaths", C.c_uint32), ("slots", C.c_uint32),
+                ("maxNumNEE", C.c_uint32), ("maxJobs", C.c_uint32)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("state", C.c_void_p), ("pc", PushConstant), ("cam", Camera),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("rrStartDepth", C.c_int32),
+                ("seed", C.c_uint64), ("sampleOffset", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("batches", C.c_uint32), ("batchesDone", C.c_uint32),
+                ("reserved", C.c_uint32), ("paths", C.c_uint64), ("queuedMs", C.c_double),
+                ("latencyMs", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("jobsDone", C.c_uint64), ("pathsDone", C.c_uint64), ("latencyP50Ms", C.c_double),
+                ("latencyP99Ms", C.c_double), ("latencyMaxMs", C.c_double), ("latencyMeanMs", C.c_double),
+                ("msamplesPerSec", C.c_double), ("bounces", C.c_uint64), ("extendRays", C.c_uint64),
+                ("shadowRays", C.c_uint64), ("probeRays", C.c_uint64), ("peakSlotsBusy", C.c_uint32),
+                ("peakJobsRunning", C.c_uint32), ("framebufferPeakBytes", C.c_uint64), ("poolBytes", C.c_uint64)]
+
+
+RTX_JOB_QUEUED, RTX_JOB_RUNNING, RTX_JOB_DONE, RTX_JOB_RELEASED = range(4)
 
 
 class ScheduleConfig(C.Structure):

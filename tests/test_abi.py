@@ -20,7 +20,8 @@ C_NAMES = {
     abi.Ray: "rtx_ray", abi.Hit: "rtx_hit", abi.Tuning: "rtx_tuning", abi.CacheHeader: "rtx_cache_header",
     abi.CacheData: "rtx_cache_data", abi.IcJob: "rtx_ic_job", abi.Aabb: "rtx_aabb", abi.VmfTheta: "rtx_vmf_theta",
     abi.VmmTheta: "rtx_vmm_theta", abi.NrcConfig: "rtx_nrc_config", abi.NrcInput: "rtx_nrc_input",
-    abi.NrcInfo: "rtx_nrc_info",
+    abi.NrcInfo: "rtx_nrc_info", abi.ServerConfig: "rtx_server_config", abi.JobDesc: "rtx_job_desc",
+    abi.JobStatus: "rtx_job_status", abi.ServerStats: "rtx_server_stats", abi.AutotuneResult: "rtx_autotune_result",
 }
 
 

@@ -106,22 +106,60 @@ def test_launch_shape_does_not_change_results(renderer):
     out = {}
     base = renderer.tuning()
     try:
-        for k, (tb, sb) in enumerate(((base.traceBlocksPerCU, base.shadeBlocksPerCU), (3, 2))):
-            renderer.set_tuning(traceBlocksPerCU=tb, shadeBlocksPerCU=sb)
+        shapes = ((base.traceBlocksPerCU, base.shadeBlocksPerCU, base.ldsStackEntries), (3, 2, 12), (16, 6, 24))
+        for k, (tb, sb, lds) in enumerate(shapes):
+            renderer.set_tuning(traceBlocksPerCU=tb, shadeBlocksPerCU=sb, ldsStackEntries=lds)
             hits = renderer.trace_closest(ds, rays)
             occ = renderer.trace_occluded(ds, rays)
             st = renderer.render(ds, fp)
             rgb, sig = renderer.read_paths(320 * 180 * 2)
             out[k] = (hits, occ, rgb, sig, (st.extendRays, st.shadowRays, st.probeRays))
     finally:
-        renderer.set_tuning(traceBlocksPerCU=base.traceBlocksPerCU, shadeBlocksPerCU=base.shadeBlocksPerCU)
+        renderer.set_tuning(traceBlocksPerCU=base.traceBlocksPerCU, shadeBlocksPerCU=base.shadeBlocksPerCU,
+                            ldsStackEntries=base.ldsStackEntries)
         ds.close()
-    (h1, o1, r1, s1, n1), (h2, o2, r2, s2, n2) = out[0], out[1]
-    log(f"launch shapes: rays {n1} vs {n2}")
-    for a, b in zip(h1, h2):
-        assert np.array_equal(a, b)
-    assert np.array_equal(o1, o2)
-    assert np.array_equal(r1, r2) and np.array_equal(s1, s2) and n1 == n2
+    h1, o1, r1, s1, n1 = out[0]
+    for k in range(1, len(out)):
+        h2, o2, r2, s2, n2 = out[k]
+        log(f"launch shapes: rays {n1} vs {n2}")
+        for a, b in zip(h1, h2):
+            assert np.array_equal(a, b)
+        assert np.array_equal(o1, o2)
+        assert np.array_equal(r1, r2) and np.array_equal(s1, s2) and n1 == n2
+
+
+def test_autotune_keeps_the_fastest_valid_shape(renderer):
+    """rtx_autotune over the probe render: the pick is one of the candidate
+    shapes, no slower than the starting shape, set on the context, and the
+    render under it is bit-identical to the default shape's."""
+    hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 1)
+    ds = renderer.upload(hs)
+    base = renderer.tuning()
+    fp = scenes.params(hs, 320, 180, 2, maxDepth=12, enableMIS=1,
+                       flags=abi.RTX_FRAME_KEEP_PATHS | abi.RTX_FRAME_RESET_ACCUM)
+    try:
+        renderer.render(ds, fp)
+        ref = renderer.read_paths(320 * 180 * 2)
+        res = renderer.autotune(ds, fp, repeats=1)
+        b = res.best
+        log(f"autotune: {res.trials} trials, {res.startMs:.3f} -> {res.bestMs:.3f} ms; trace {b.traceBlocksPerCU} "
+            f"lds {b.ldsStackEntries} shade {b.shadeBlocksPerCU} batch {b.batchPaths}")
+        assert b.traceBlocksPerCU in (8, 10, 12, 14, 16, 20) or b.traceBlocksPerCU == base.traceBlocksPerCU
+        assert b.ldsStackEntries in (12, 16, 20, 24) and b.shadeBlocksPerCU in (4, 6, 8, 12, 16)
+        assert b.batchPaths in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+        assert res.bestMs <= res.startMs and res.trials >= 16
+        t = renderer.tuning()
+        assert (t.traceBlocksPerCU, t.ldsStackEntries, t.shadeBlocksPerCU, t.batchPaths) == \
+            (b.traceBlocksPerCU, b.ldsStackEntries, b.shadeBlocksPerCU, b.batchPaths)
+        renderer.render(ds, fp)
+        got = renderer.read_paths(320 * 180 * 2)
+        assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+        with pytest.raises(rtxamd.RtxError):
+            renderer.set_tuning(ldsStackEntries=18)
+    finally:
+        renderer.set_tuning(traceBlocksPerCU=base.traceBlocksPerCU, shadeBlocksPerCU=base.shadeBlocksPerCU,
+                            ldsStackEntries=base.ldsStackEntries, batchPaths=base.batchPaths)
+        ds.close()
 
 
 # ----------------------------------------------------------------- per path
