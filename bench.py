@@ -137,10 +137,13 @@ def serving_leg(rtxamd, abi, r, a):
     return out
 
 
-def relmse(img, ref, eps=1e-2):
-    """mean over pixels and channels of (I - R)^2 / (R^2 + eps)."""
+def relmse(img, ref, eps=1e-2, ref_var=None):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + eps); with ref_var
+    (the per-pixel variance of the reference's own estimate) the reference's
+    noise is taken out: E[(I - R)^2] = MSE(I) + Var(R) for independent I, R."""
     d = img[..., :3].astype(np.float64) - ref[..., :3]
-    return float(np.mean(d * d / (ref[..., :3] ** 2 + eps)))
+    num = d * d if ref_var is None else d * d - ref_var[..., :3]
+    return float(np.mean(num / (ref[..., :3] ** 2 + eps)))
 
 
 def equal_time(render_call, budget_s):
@@ -159,40 +162,52 @@ def equal_time(render_call, budget_s):
 
 def relmse_leg(rtxamd, abi, r, a):
     """Equal-time RelMSE against a plain reference, RelMSE = mean over pixels
-    and channels of (I - R)^2 / (R^2 + 1e-2).  Room: the benchmark scene
-    (262,144 triangles, depth 12, NEE + MIS) at 480x270 (the 1080p camera at a
-    quarter of the resolution): plain, irradiance cache (10,000 records, 50
-    prepare frames inside the budget, per-frame upkeep), learned cache (hash
-    grid 16 x 2 x 2^19, 5 x 64 MLP, 2^16 training suffixes per frame, trained
-    from scratch inside the budget).  Plates (veachMIS-like, 4 sphere lights,
-    glossy plates) at 320x180: plain vs guided sampling (2^8 regions of
-    vMF mixtures, 6 recording frames with a GPU EM update each inside the
-    budget, then guided frames).  References: `relmse_ref_spp` plain samples
-    per pixel with another seed, 64-spp chunks accumulated in FP64 on the host."""
+    and channels of (I - R)^2 / (R^2 + 1e-2) with the reference's own noise
+    taken out (two independent half-references A, B: R = (A + B) / 2,
+    Var(R) = (A - B)^2 / 4 per pixel).
+    Room (the learned-cache configuration): the benchmark scene at 1920x1080,
+    depth 12, NEE + MIS; plain, irradiance cache (10,000 records, 50 prepare
+    frames inside the budget, per-frame upkeep) and learned cache (hash grid
+    16 x 2 x 2^19, 5 x 64 MLP, 2^16 training suffixes per frame, trained from
+    scratch inside the budget); reference `relmse_ref_spp` / 4 plain samples
+    per pixel per half (1080p keeps the reference short; its variance is
+    subtracted).  Plates (the guiding configuration's plates-and-lights scene)
+    at 640x360: plain vs guided sampling (2^8 regions of vMF mixtures, 6
+    recording frames with a GPU EM update each inside the budget, then guided
+    frames); reference `relmse_ref_spp` / 2 samples per half.  Each method gets
+    `relmse_seconds` of wall time (the plates a tenth of it)."""
     pc = dict(maxDepth=12, enableMIS=1, enableNEE=1, usePowerHeuristic=1)
     RESET = abi.RTX_FRAME_RESET_ACCUM
 
-    def reference(ds, cam, W, H):
-        ref = np.zeros((H, W, 4))
-        for k in range(0, a.relmse_ref_spp, 64):
-            r.render(ds, rtxamd.frame_params(cam, W, H, 64, seed=1001, sample_offset=k, flags=RESET, **pc))
-            ref += r.read_image(W, H).astype(np.float64) * 64
-        return ref / a.relmse_ref_spp
+    def reference(ds, cam, W, H, half_spp):
+        halves = []
+        for h in range(2):
+            acc = np.zeros((H, W, 4))
+            chunk = min(64, half_spp)
+            for k in range(0, half_spp, chunk):
+                r.render(ds, rtxamd.frame_params(cam, W, H, chunk, seed=1001 + h, sample_offset=k, flags=RESET, **pc))
+                acc += r.read_image(W, H).astype(np.float64) * chunk
+            halves.append(acc / half_spp)
+        A, B = halves
+        return (A + B) / 2, (A - B) ** 2 / 4
 
     out = {}
-    T = a.relmse_seconds
+    state = {"spp": 0}
     # ---- room: plain, irradiance cache, learned cache
-    W, H = 480, 270
+    T = a.relmse_seconds
+    W, H = 1920, 1080
     hs = rtxamd.HostScene.generate(abi.SCENE_ROOM, 1)
     ds = r.upload(hs)
     cam = hs.camera(W, H)
+    chunk = 1
     t0 = time.perf_counter()
-    ref = reference(ds, cam, W, H)
+    half = max(1, a.relmse_ref_spp // 4)
+    ref, var = reference(ds, cam, W, H, half)
     ref_s = time.perf_counter() - t0
     room = {"scene": f"room 262144 tris, {W}x{H}, depth 12, NEE+MIS",
-            "reference": f"{a.relmse_ref_spp} spp plain, FP64 accumulation ({ref_s:.1f} s)", "budget_s": T}
-    chunk = 4
-    state = {"spp": 0}
+            "reference": f"2 x {half} spp plain (independent seeds), FP64 accumulation, its variance subtracted "
+                         f"({ref_s:.1f} s)",
+            "reference_relmse": relmse(ref, ref, ref_var=-var), "budget_s": T}
 
     def plain_call(first):
         r.render(ds, rtxamd.frame_params(cam, W, H, chunk, seed=1, sample_offset=state["spp"],
@@ -201,7 +216,7 @@ def relmse_leg(rtxamd, abi, r, a):
 
     state["spp"] = 0
     n, sec = equal_time(plain_call, T)
-    e_plain = relmse(r.read_image(W, H), ref)
+    e_plain = relmse(r.read_image(W, H), ref, ref_var=var)
     room["plain"] = {"spp": state["spp"], "seconds": round(sec, 3), "relmse": e_plain}
 
     r.ic_create(10000)
@@ -217,7 +232,7 @@ def relmse_leg(rtxamd, abi, r, a):
         state["spp"] += chunk
 
     n, sec = equal_time(ic_call, T)
-    e_ic = relmse(r.read_image(W, H), ref)
+    e_ic = relmse(r.read_image(W, H), ref, ref_var=var)
     room["irradiance_cache"] = {"spp": state["spp"] - 50, "seconds": round(sec, 3), "relmse": e_ic,
                                 "improvement": round(e_plain / e_ic, 3)}
 
@@ -232,7 +247,7 @@ def relmse_leg(rtxamd, abi, r, a):
         state["spp"] += chunk
 
     n, sec = equal_time(nrc_call, T)
-    e_nrc = relmse(r.read_image(W, H), ref)
+    e_nrc = relmse(r.read_image(W, H), ref, ref_var=var)
     ni = r.nrc_info()
     room["learned_cache"] = {"spp": state["spp"], "seconds": round(sec, 3), "relmse": e_nrc,
                              "improvement": round(e_plain / e_nrc, 3), "adam_steps": ni.steps,
@@ -243,18 +258,22 @@ def relmse_leg(rtxamd, abi, r, a):
     log(f"relmse room: plain {e_plain:.4g}, cache {e_ic:.4g}, learned {e_nrc:.4g}")
 
     # ---- plates: plain vs guided
-    W, H = 320, 180
+    W, H = 640, 360
+    T = a.relmse_seconds / 10
+    chunk = 2
     hs = rtxamd.HostScene.generate(abi.SCENE_PLATES, 1)
     ds = r.upload(hs)
     cam = hs.camera(W, H)
     t0 = time.perf_counter()
-    ref = reference(ds, cam, W, H)
+    half = max(1, a.relmse_ref_spp // 2)
+    ref, var = reference(ds, cam, W, H, half)
     ref_s = time.perf_counter() - t0
     plates = {"scene": f"plates and sphere lights, {W}x{H}, depth 12, NEE+MIS",
-              "reference": f"{a.relmse_ref_spp} spp plain, FP64 accumulation ({ref_s:.1f} s)", "budget_s": T}
+              "reference": f"2 x {half} spp plain (independent seeds), FP64 accumulation, its variance "
+                           f"subtracted ({ref_s:.1f} s)", "budget_s": T}
     state["spp"] = 0
     n, sec = equal_time(plain_call, T)
-    e_plain = relmse(r.read_image(W, H), ref)
+    e_plain = relmse(r.read_image(W, H), ref, ref_var=var)
     plates["plain"] = {"spp": state["spp"], "seconds": round(sec, 3), "relmse": e_plain}
     bounds = ds.bounds()
 
@@ -268,13 +287,44 @@ def relmse_leg(rtxamd, abi, r, a):
         state["spp"] += chunk
 
     n, sec = equal_time(guided_call, T)
-    e_g = relmse(r.read_image(W, H), ref)
+    e_g = relmse(r.read_image(W, H), ref, ref_var=var)
     plates["guided"] = {"spp": state["spp"] - 6, "seconds": round(sec, 3), "relmse": e_g,
                         "improvement": round(e_plain / e_g, 3)}
     ds.close()
     out["plates"] = plates
     log(f"relmse plates: plain {e_plain:.4g}, guided {e_g:.4g}")
     return out
+
+
+def em_cpu_leg(rtxamd, abi, r):
+    """The guiding update (PathGuiding::update: preFit, MAP-EM, split and
+    merge, distances) on one recording frame of the plates-and-lights scene
+    at 640x360 (2^8 regions), timed on the host the reference's way - the FP32
+    oracle, serial over regions on one thread (src/PathGuiding.cpp:281-288) -
+    and on the device (rtx_guiding_update on the same samples)."""
+    from oracle.binding import OracleGuide
+    hs = rtxamd.HostScene.generate(abi.SCENE_PLATES, 1)
+    ds = r.upload(hs)
+    box = ds.bounds()
+    W, H = 640, 360
+    r.guiding_init(box, 8)
+    r.render(ds, rtxamd.frame_params(hs.camera(W, H), W, H, 1, seed=3, updateGuiding=1, maxDepth=12, enableMIS=1,
+                                     flags=abi.RTX_FRAME_RESET_ACCUM | abi.RTX_FRAME_NO_GUIDING_UPDATE))
+    dd = r.guiding_read_samples(W, H)
+    n = int((dd["flags"] != abi.DIR_INVALID).sum())
+    t1 = time.perf_counter()
+    r.guiding_update()
+    r.guiding_get()  # waits for the update
+    gpu_s = time.perf_counter() - t1
+    og = OracleGuide(abi, box, 8, parallax=True, prec=32)
+    t1 = time.perf_counter()
+    og.update(dd)
+    cpu_s = time.perf_counter() - t1
+    r.guiding_set([], [])
+    ds.close()
+    return {"samples": n, "regions": 256, "cpu_ms_serial": round(cpu_s * 1e3, 1), "gpu_ms": round(gpu_s * 1e3, 2),
+            "cpu_msamples_per_s": round(n / cpu_s / 1e6, 3), "gpu_msamples_per_s": round(n / gpu_s / 1e6, 2),
+            "sample": "one recording frame of the plates scene at 640x360, 2^8 regions, FP32, first update"}
 
 
 def main():
@@ -297,11 +347,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-method comparison step")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotune)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--serve-jobs", type=int, default=24, help="jobs of the serving leg's arrival stream")
+    ap.add_argument("--serve-jobs", type=int, default=16, help="jobs of the serving leg's arrival stream")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate of the serving leg, jobs/s")
     ap.add_argument("--no-relmse", action="store_true", help="skip the equal-time RelMSE leg")
-    ap.add_argument("--relmse-seconds", type=float, default=1.0, help="equal-time budget per method")
-    ap.add_argument("--relmse-ref-spp", type=int, default=4096, help="samples per pixel of the reference image")
+    ap.add_argument("--relmse-seconds", type=float, default=2.0, help="equal-time budget per method (room)")
+    ap.add_argument("--relmse-ref-spp", type=int, default=1024,
+                    help="reference samples per pixel (the room renders a quarter of them at 1080p)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 byte counters)")
     a = ap.parse_args()
@@ -479,7 +530,14 @@ def main():
                     "rel_err_p50": float(np.median(rel[same])), "rel_err_p99": float(np.percentile(rel[same], 99))}
 
         sub = ids[::8]
+        t1 = time.perf_counter()
         r32, s32 = orc.render_paths(fp1, sub, prec=32, threads=DEFAULT_THREADS)
+        cpu32_s = time.perf_counter() - t1
+        cpu["fp32_value"] = round(len(sub) / cpu32_s / 1e6, 5)
+        cpu["fp32_sample"] = f"{len(sub)} paths (every 8th of the FP64 sample), FP32 instantiation, {cpu32_s:.1f} s"
+        cpu["host_cpus"] = os.cpu_count()
+        cpu["threads_note"] = f"{DEFAULT_THREADS} threads: the host cores one GPU's share of the box grants"
+        cpu["em"] = em_cpu_leg(rtxamd, abi, r)
         parity = {"device_vs_fp64": stats(rgb[ids.astype(np.int64)], sig[ids.astype(np.int64)], orgb, osig),
                   "reference_logic_in_fp32_vs_fp64": stats(r32, s32, orgb[::8], osig[::8]),
                   "note": "FP32 shading normals on 2-8 cm smooth columns at ~10 m coordinates make ~20% of "

@@ -43,7 +43,8 @@ extern "C" {
 /* shaders/limits.glsl:1-7 */
 #define RTX_SIZE_LIGHT_RANDOM 10000
 #define RTX_SIZE_TRI_RANDOM 10000
-#define RTX_MAX_DISTRIBUTIONS 16
+#define RTX_MAX_DISTRIBUTIONS 16 /* lobes of a VMM_Theta (shaders/limits.glsl:6) */
+#define RTX_MAX_LOBES 32         /* lobes a device mixture may hold (rtx_guiding_options.maxComponents) */
 #define RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL 16
 
 /* shaders/raytrace.rgen:21-27; src/SceneLoader.h:31-39 */
@@ -569,9 +570,30 @@ typedef struct rtx_guiding_options {
      * sample counts decayed by 0.25, the new half appended as the last region */
     int32_t splitRegions;                   /* 0 */
     uint32_t samplesForRegionSplit;         /* 10000 */
+    /* Scale limits (new; the reference fixes MAX_DISTRIBUTIONS = 16 at compile
+     * time, shaders/limits.glsl:6, and has no region or sample caps):
+     * maxComponents: lobes a mixture may grow to by splits, 1..RTX_MAX_LOBES (16);
+     * maxRegions: region splitting stops once there are this many regions (0: 2^20);
+     * maxSamplesPerUpdate: an update fits the first this many valid samples in
+     * slot order (0: all). */
+    uint32_t maxComponents;
+    uint32_t maxRegions;
+    uint32_t maxSamplesPerUpdate;
+    uint32_t reserved;
 } rtx_guiding_options;
 int rtx_guiding_set_options(rtx_ctx *ctx, const rtx_guiding_options *options);
 int rtx_guiding_get_options(rtx_ctx *ctx, rtx_guiding_options *options);
+/* A mixture of up to RTX_MAX_LOBES lobes: VMM_Theta's layout with
+ * MAX_DISTRIBUTIONS = 32 (shaders/guiding.glsl:49-54).  rtx_guiding_get
+ * returns RTX_E_UNSUPPORTED when a region uses more than 16 lobes; this
+ * getter returns every mixture. */
+typedef struct rtx_vmm_theta_wide {
+    rtx_vmf_theta thetas[RTX_MAX_LOBES];
+    float pi[RTX_MAX_LOBES];
+    float meanPosition[3];
+    int32_t usedDistributions;
+} rtx_vmm_theta_wide;
+int rtx_guiding_get_wide(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta_wide *vmms, uint32_t capacity, uint32_t *n);
 /* The guiding samples of the last frame rendered with pc.updateGuiding: the
  * reference's DirectionalData buffer (binding 18, src/SampleCollector.cpp:
  * 29-51), RTX_MAX_DIRECTIONAL_DATA_PER_PIXEL slots per pixel in pixel order;

@@ -106,6 +106,8 @@ def lib(abi):
         L.orc_guide_destroy.restype = None
         L.orc_guide_regions.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32]
         L.orc_guide_regions.restype = C.c_uint32
+        L.orc_guide_regions_wide.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmThetaWide), C.c_uint32]
+        L.orc_guide_regions_wide.restype = C.c_uint32
         L.orc_guide_update.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.orc_guide_last_error.restype = C.c_char_p
         L.orc_guide_set_options.argtypes = [C.c_void_p, C.POINTER(abi.GuidingOptions)]
@@ -398,9 +400,13 @@ class OracleGuide:
         if rc != 0:
             raise RuntimeError(self.L.orc_guide_last_error().decode())
         if options is not None:
-            rc = self.L.orc_guide_set_options(self._h, C.byref(options))
-            if rc != 0:
-                raise RuntimeError(self.L.orc_guide_last_error().decode())
+            self.set_options(options)
+
+    def set_options(self, options):
+        """Replace the split/merge and scale options (abi.GuidingOptions)."""
+        rc = self.L.orc_guide_set_options(self._h, C.byref(options))
+        if rc != 0:
+            raise RuntimeError(self.L.orc_guide_last_error().decode())
 
     def __del__(self):
         try:
@@ -414,6 +420,15 @@ class OracleGuide:
         boxes = (self.abi.Aabb * n)()
         vmms = (self.abi.VmmTheta * n)()
         self.L.orc_guide_regions(self._h, boxes, vmms, n)
+        arr = np.array([[b.min[:], b.max[:]] for b in boxes], np.float32)
+        return arr, [vmms[i] for i in range(n)]
+
+    def regions_wide(self):
+        """(boxes, [abi.VmmThetaWide]): mixtures of up to 32 lobes."""
+        n = self.L.orc_guide_regions_wide(self._h, None, None, 0)
+        boxes = (self.abi.Aabb * n)()
+        vmms = (self.abi.VmmThetaWide * n)()
+        self.L.orc_guide_regions_wide(self._h, boxes, vmms, n)
         arr = np.array([[b.min[:], b.max[:]] for b in boxes], np.float32)
         return arr, [vmms[i] for i in range(n)]
 

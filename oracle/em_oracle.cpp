@@ -33,7 +33,7 @@
 
 namespace {
 
-constexpr int kMaxK = RTX_MAX_DISTRIBUTIONS;
+constexpr int kMaxK = RTX_MAX_LOBES; // array bound; SplitMergeProps::maxComponents caps the lobes in use
 constexpr double kPi = 3.1415926535897932384626433832795;
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
@@ -296,6 +296,10 @@ struct SplitMergeProps {
     float splitMinDivergence = 0.5f, mergeMaxDivergence = 0.025f;
     bool splitRegions = false; // src/PathGuiding.h:118-119
     uint32_t samplesForRegionSplit = 10000;
+    // scale limits (rtx_guiding_options): lobes per mixture (limits.glsl:6's
+    // 16 by default), regions (0: no cap), samples per update (0: all)
+    int maxComponents = RTX_MAX_DISTRIBUTIONS;
+    uint32_t maxRegions = 0, maxSamplesPerUpdate = 0;
 };
 
 // PMM::removeWeightPrior / applyWeightPrior (ParametricMixtureModel.h:339-375):
@@ -703,9 +707,9 @@ template <class R> void eigen2x2(R a00, R a01, R a10, R a11, R lam[2], R V[2][2]
 // of its directions' covariance), each with half the weight.  The reference
 // takes the principal axis from a ROW of Eigen's eigenvector matrix
 // (computeEigenValuesVectors, 453-486); kept.
-template <class R> void splitComponent(Region<R> &g, int comp, float maxKappa) {
+template <class R> void splitComponent(Region<R> &g, int comp, float maxKappa, int maxK) {
     Pmm<R> &m = g.pmm;
-    if (m.K == kMaxK) return;
+    if (m.K >= maxK) return;
     const int tgt = m.K;
     R fx[3], fy[3];
     tangentFrame(m.mu[comp], fx, fy);
@@ -799,7 +803,8 @@ template <class R> void maskedFit(const Props &P, Pmm<R> &m, const std::vector<S
 template <class R>
 uint32_t splitAll(const Props &P, const SplitMergeProps &SP, Region<R> &g, const std::vector<Sample> &s, bool fit) {
     Pmm<R> &m = g.pmm;
-    if (m.K >= kMaxK) return 0;
+    const int maxK = SP.maxComponents;
+    if (m.K >= maxK) return 0;
     const bool firstFit = s.size() == m.totalNumSamples;
     uint32_t total = 0;
     for (;;) {
@@ -812,7 +817,7 @@ uint32_t splitAll(const Props &P, const SplitMergeProps &SP, Region<R> &g, const
         const size_t nCand = partitionLikeLibstdcxx(c, [&](const std::pair<R, int> &x) {
             return x.first >= (R)SP.splitMinDivergence;
         });
-        const uint32_t numSplits = (uint32_t)std::min<size_t>(nCand, (size_t)(kMaxK - m.K));
+        const uint32_t numSplits = (uint32_t)std::min<size_t>(nCand, (size_t)(maxK - m.K));
         if (numSplits == 0) break;
         if (nCand > numSplits)
             std::partial_sort(c.begin(), c.begin() + numSplits, c.begin() + nCand,
@@ -821,7 +826,7 @@ uint32_t splitAll(const Props &P, const SplitMergeProps &SP, Region<R> &g, const
         for (uint32_t i = 0; i < numSplits; ++i) {
             mask[c[i].second] = true;
             mask[m.K] = true;
-            splitComponent(g, c[i].second, P.maxKappa);
+            splitComponent(g, c[i].second, P.maxKappa, maxK);
         }
         if (fit) {
             pmmApplyPrior(m, P.vPrior);
@@ -854,10 +859,12 @@ template <class R> void splitAndMerge(const Props &P, const SplitMergeProps &SP,
     pmmApplyPrior(m, P.vPrior);
 }
 
-template <class R> void toVmm(const Region<R> &g, bool parallax, rtx_vmm_theta &v) { // syncPMMsToVMM_Thetas
-    v = rtx_vmm_theta{};
+// V: rtx_vmm_theta (16 lobes, mixtures of at most 16) or rtx_vmm_theta_wide
+template <class R, class V> void toVmm(const Region<R> &g, bool parallax, V &v) { // syncPMMsToVMM_Thetas
+    constexpr int N = (int)(sizeof(v.pi) / sizeof(v.pi[0]));
+    v = V{};
     v.usedDistributions = g.pmm.K;
-    for (int k = 0; k < kMaxK; ++k) v.thetas[k].distance = -1.0f;
+    for (int k = 0; k < N; ++k) v.thetas[k].distance = -1.0f;
     for (int k = 0; k < g.pmm.K; ++k) {
         rtx_vmf_theta &t = v.thetas[k];
         v.pi[k] = (float)g.pmm.w[k];
@@ -871,7 +878,7 @@ template <class R> void toVmm(const Region<R> &g, bool parallax, rtx_vmm_theta &
     }
     if (parallax) {
         for (int c = 0; c < 3; ++c) v.meanPosition[c] = (float)g.mean[c];
-        for (int k = 0; k < kMaxK; ++k) {
+        for (int k = 0; k < N; ++k) {
             float d = k < g.pmm.K ? (float)g.dist[k] : std::numeric_limits<float>::infinity();
             d = std::isinf(d) ? -1.0f : d;
             v.thetas[k].distance = d;
@@ -904,9 +911,12 @@ template <> std::vector<Region<float>> &regionsOf<float>(orc_guide *g) { return 
 template <class R> void updateAll(orc_guide *g, const rtx_directional_data *dd, size_t n) {
     std::vector<Region<R>> &regs = regionsOf<R>(g);
     std::vector<std::vector<Sample>> per(regs.size());
+    size_t taken = 0; // maxSamplesPerUpdate: the first valid samples in slot order
     for (size_t i = 0; i < n; ++i) {
         const rtx_directional_data &d = dd[i];
         if (d.flags == kInvalid || d.flags >= regs.size()) continue;
+        if (g->sm.maxSamplesPerUpdate && taken == g->sm.maxSamplesPerUpdate) break;
+        taken++;
         Sample x;
         for (int c = 0; c < 3; ++c) {
             x.pos[c] = d.position[c];
@@ -958,8 +968,10 @@ template <class R> void updateAll(orc_guide *g, const rtx_directional_data *dd, 
     // region splitting (src/PathGuiding.cpp:295-301, splitRegion 328-348)
     if (g->sm.splitRegions) {
         const size_t count = regs.size();
+        const size_t cap = g->sm.maxRegions ? std::min<size_t>(g->sm.maxRegions, 1u << 20) : 1u << 20;
         for (size_t r = 0; r < count; ++r) {
             if (!(regs[r].pmm.numSamples > (R)g->sm.samplesForRegionSplit)) continue;
+            if (g->boxes.size() >= cap) break; // the cap: no further region is split
             const rtx_aabb b = g->boxes[r];
             const float size[3] = {b.max[0] - b.min[0], b.max[1] - b.min[1], b.max[2] - b.min[2]};
             const int axis = size[0] > size[1] ? (size[0] > size[2] ? 0 : 2) : (size[1] > size[2] ? 1 : 2);
@@ -1038,6 +1050,18 @@ uint32_t orc_guide_regions(const orc_guide *g, rtx_aabb *boxes, rtx_vmm_theta *v
     return n;
 }
 
+uint32_t orc_guide_regions_wide(const orc_guide *g, rtx_aabb *boxes, rtx_vmm_theta_wide *vmms, uint32_t capacity) {
+    const uint32_t n = (uint32_t)g->boxes.size();
+    for (uint32_t i = 0; i < std::min(n, capacity); ++i) {
+        if (boxes) boxes[i] = g->boxes[i];
+        if (vmms) {
+            if (g->prec == 64) toVmm(g->r64[i], g->parallax, vmms[i]);
+            else toVmm(g->r32[i], g->parallax, vmms[i]);
+        }
+    }
+    return n;
+}
+
 int orc_guide_update(orc_guide *g, const rtx_directional_data *samples, size_t n) {
     if (!g || (n && !samples)) {
         g_guideErr = "NULL argument";
@@ -1063,6 +1087,13 @@ int orc_guide_set_options(orc_guide *g, const rtx_guiding_options *o) {
     g->sm.mergeMaxDivergence = o->mergeMaxDivergence;
     g->sm.splitRegions = o->splitRegions != 0;
     g->sm.samplesForRegionSplit = o->samplesForRegionSplit;
+    if (o->maxComponents < 1 || o->maxComponents > RTX_MAX_LOBES) {
+        g_guideErr = "maxComponents must be in [1, RTX_MAX_LOBES]";
+        return RTX_E_INVALID;
+    }
+    g->sm.maxComponents = (int)o->maxComponents;
+    g->sm.maxRegions = o->maxRegions;
+    g->sm.maxSamplesPerUpdate = o->maxSamplesPerUpdate;
     return RTX_OK;
 }
 
@@ -1104,7 +1135,7 @@ static Props propsFrom(const double p[7]) {
     return P;
 }
 // m[]: K, then per component (w, mu xyz, kappa, r), then sampleWeight,
-// numSamples, totalNumSamples: 1 + 6 * 16 + 3 doubles
+// numSamples, totalNumSamples: 1 + 6 * kMaxK + 3 doubles
 constexpr int kMixDoubles = 1 + 6 * kMaxK + 3;
 static bool mixFrom(const double *v, Pmm<double> &m) {
     m = Pmm<double>{};

@@ -127,6 +127,7 @@ int orc_guide_create(const rtx_aabb *sceneBox, uint32_t splitCount, int parallax
 void orc_guide_destroy(orc_guide *g);
 /* Region boxes and VMM_Theta (as syncPMMsToVMM_Thetas fills them); returns the region count. */
 uint32_t orc_guide_regions(const orc_guide *g, rtx_aabb *boxes, rtx_vmm_theta *vmms, uint32_t capacity);
+uint32_t orc_guide_regions_wide(const orc_guide *g, rtx_aabb *boxes, rtx_vmm_theta_wide *vmms, uint32_t capacity);
 /* PathGuiding::update with these samples (flags = region, 0xFFFFFFFF = unused). */
 int orc_guide_update(orc_guide *g, const rtx_directional_data *samples, size_t n);
 const char *orc_guide_last_error(void);

@@ -17,6 +17,7 @@
 
 #include "abi_guard.h"
 #include "context.h"
+#include "dev_sort.h"
 #include "dev_math.h"
 #include "internal.h"
 #include "nrc.h"
@@ -832,7 +833,11 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     if (icFrames || record || nrc) batch = std::min<uint64_t>(batch, npix);
     if (batch >= npix) batch -= batch % npix;
     size_t bufPaths = (size_t)batch;
-    if (icFrames) bufPaths = std::max<size_t>(bufPaths, kIcStrata); // at least one job of sub-paths
+    // with the cache, room for every job a frame can make (2 x maxCaches, 200
+    // sub-paths each) when that stays within 2^26 paths: the frame's upkeep
+    // then runs in one batch sized on the device
+    if (icFrames)
+        bufPaths = std::max<size_t>(bufPaths, std::min<size_t>((size_t)ctx->ic.jobCap * kIcStrata, (size_t)1 << 26));
     int rc = ensureBatchBuffers(ctx, bufPaths, neeCap);
     if (rc != RTX_OK) return rc;
     if (segments && ctx->segCap < bufPaths) {
@@ -1139,11 +1144,30 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         const uint64_t sample = f0 / npix + p->sampleOffset;
         const uint32_t frameSeed = pcg32First(p->seed, sample);
         timed(kClsShade, [&] { launchIcPlan(s, ctx->ic); });
+        const bool keepSub = keep && fEnd == total;
+        const uint32_t jobsPerBatch = (uint32_t)std::max<size_t>(1, ctx->cap / kIcStrata);
+        if (!keepSub && ctx->ic.jobCap <= jobsPerBatch) {
+            // every job of the frame fits one batch: the launches take the job
+            // count from device memory, so no host round trip sizes them
+            const uint32_t np = ctx->ic.jobCap * kIcStrata;
+            BatchDesc b{};
+            b.numPaths = np;
+            b.npix = np;
+            timed(kClsShade,
+                  [&] { launchGenerateIC(s, ctx->ic, frameSeed, 0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, true); });
+            setRaySeeds(fcIC.numNEE);
+            int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
+            if (rcs != RTX_OK) return rcs;
+            timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, 0, ctx->ic.jobCap, ctx->pb, true); });
+            timed(kClsShade, [&] { launchIcCommit(s, ctx->ic, ctx->fcIC); });
+            timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
+            CHECK_HIP(ctx, hipGetLastError());
+            continue;
+        }
         uint32_t jobInfo[2];
         CHECK_HIP(ctx, hipMemcpyAsync(jobInfo, ctx->ic.frame + 3, sizeof(jobInfo), hipMemcpyDeviceToHost, s));
         CHECK_HIP(ctx, hipStreamSynchronize(s));
         const uint32_t numJobs = jobInfo[1];
-        const bool keepSub = keep && fEnd == total;
         if (keepSub) {
             size_t need = (size_t)numJobs * kIcStrata;
             if (need > ctx->keepSubCap) {
@@ -1159,7 +1183,6 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             ctx->keptJobs = numJobs;
             ctx->keptUpdates = jobInfo[0];
         }
-        const uint32_t jobsPerBatch = (uint32_t)std::max<size_t>(1, ctx->cap / kIcStrata);
         for (uint32_t j0 = 0; j0 < numJobs; j0 += jobsPerBatch) {
             const uint32_t nj = std::min(jobsPerBatch, numJobs - j0);
             const uint32_t np = nj * kIcStrata;
@@ -1168,11 +1191,11 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.numPaths = np;
             b.npix = np;
             timed(kClsShade,
-                  [&] { launchGenerateIC(s, ctx->ic, frameSeed, j0, np, ctx->pb, ctx->q[0], ctx->cnt, 0); });
+                  [&] { launchGenerateIC(s, ctx->ic, frameSeed, j0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, false); });
             setRaySeeds(fcIC.numNEE);
             int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
             if (rcs != RTX_OK) return rcs;
-            timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, j0, nj, ctx->pb); });
+            timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, j0, nj, ctx->pb, false); });
             if (keepSub) {
                 CHECK_HIP(ctx, hipMemcpyAsync(ctx->keepSub + (size_t)j0 * kIcStrata, ctx->pb.out, np * sizeof(float4),
                                               hipMemcpyDeviceToDevice, s));
@@ -1228,24 +1251,44 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
 }
 
 // ---------------------------------------------------------- path guiding
+// A 16-lobe VMM_Theta (the reference's layout) as a device mixture: the
+// lobes past usedDistributions zero, as kToVmm leaves them.
+static rtx_vmm_theta_wide widen(const rtx_vmm_theta &v) {
+    rtx_vmm_theta_wide w;
+    std::memset(&w, 0, sizeof(w));
+    for (int k = 0; k < RTX_MAX_LOBES; ++k) {
+        w.thetas[k].mu[2] = 1.0f;
+        w.thetas[k].distance = -1.0f;
+    }
+    for (int k = 0; k < RTX_MAX_DISTRIBUTIONS; ++k) {
+        w.thetas[k] = v.thetas[k];
+        w.pi[k] = v.pi[k];
+    }
+    for (int c = 0; c < 3; ++c) w.meanPosition[c] = v.meanPosition[c];
+    w.usedDistributions = v.usedDistributions;
+    return w;
+}
+
 static int guideUpload(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *vmms, uint32_t n) {
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     freeAll(ctx->guideAllocs);
     ctx->guide = DevGuiding{};
     ctx->guideBoxes.assign(regions, regions + n);
-    ctx->guideVmms.assign(vmms, vmms + n);
+    ctx->guideVmms.resize(n);
+    for (uint32_t i = 0; i < n; ++i) ctx->guideVmms[i] = widen(vmms[i]);
     ctx->guideVmmsStale = false;
     if (n == 0) return RTX_OK;
     std::vector<GuideNode> nodes = buildGuideNodes(regions, n);
     rtx_aabb *dBoxes = nullptr;
-    rtx_vmm_theta *dVmms = nullptr;
+    rtx_vmm_theta_wide *dVmms = nullptr;
     GuideNode *dNodes = nullptr;
     hipError_t e = devAlloc(&dBoxes, n, ctx->guideAllocs);
     if (e == hipSuccess) e = devAlloc(&dVmms, n, ctx->guideAllocs);
     if (e == hipSuccess) e = devAlloc(&dNodes, nodes.size(), ctx->guideAllocs);
     if (e == hipSuccess) e = hipMemcpy(dBoxes, regions, n * sizeof(rtx_aabb), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dVmms, vmms, n * sizeof(rtx_vmm_theta), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(dVmms, ctx->guideVmms.data(), n * sizeof(rtx_vmm_theta_wide), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dNodes, nodes.data(), nodes.size() * sizeof(GuideNode), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         freeAll(ctx->guideAllocs);
@@ -1279,17 +1322,55 @@ int rtx_guiding_set(rtx_ctx *ctx, const rtx_aabb *regions, const rtx_vmm_theta *
     return rtx::abiFail(ctx);
 }
 
+// The host copy of the device mixtures, brought up to date.
+static int guideSync(rtx_ctx *ctx) {
+    const uint32_t count = (uint32_t)ctx->guideBoxes.size();
+    if (ctx->guideVmmsStale && count) {
+        CHECK_HIP(ctx, hipSetDevice(ctx->device));
+        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->guideVmms.resize(count);
+        CHECK_HIP(ctx, hipMemcpy(ctx->guideVmms.data(), ctx->guide.vmms, count * sizeof(rtx_vmm_theta_wide),
+                                 hipMemcpyDeviceToHost));
+        ctx->guideVmmsStale = false;
+    }
+    return RTX_OK;
+}
+
 int rtx_guiding_get(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta *vmms, uint32_t capacity, uint32_t *n) try {
     if (!ctx) return RTX_E_INVALID;
     const uint32_t count = (uint32_t)ctx->guideBoxes.size();
     if (n) *n = count;
-    if (ctx->guideVmmsStale && count) {
-        CHECK_HIP(ctx, hipSetDevice(ctx->device));
-        CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        CHECK_HIP(ctx, hipMemcpy(ctx->guideVmms.data(), ctx->guide.vmms, count * sizeof(rtx_vmm_theta),
-                                 hipMemcpyDeviceToHost));
-        ctx->guideVmmsStale = false;
+    const int rc = guideSync(ctx);
+    if (rc != RTX_OK) return rc;
+    const uint32_t k = std::min(capacity, count);
+    if (regions) std::copy(ctx->guideBoxes.begin(), ctx->guideBoxes.begin() + k, regions);
+    if (vmms) {
+        for (uint32_t i = 0; i < k; ++i) {
+            const rtx_vmm_theta_wide &w = ctx->guideVmms[i];
+            if (w.usedDistributions > RTX_MAX_DISTRIBUTIONS)
+                return fail(ctx, RTX_E_UNSUPPORTED, "a mixture uses more than 16 lobes: rtx_guiding_get_wide");
+            rtx_vmm_theta v;
+            for (int c = 0; c < RTX_MAX_DISTRIBUTIONS; ++c) {
+                v.thetas[c] = w.thetas[c];
+                v.pi[c] = w.pi[c];
+            }
+            for (int c = 0; c < 3; ++c) v.meanPosition[c] = w.meanPosition[c];
+            v.usedDistributions = w.usedDistributions;
+            vmms[i] = v;
+        }
     }
+    return RTX_OK;
+} catch (...) {
+    return rtx::abiFail(ctx);
+}
+
+int rtx_guiding_get_wide(rtx_ctx *ctx, rtx_aabb *regions, rtx_vmm_theta_wide *vmms, uint32_t capacity,
+                         uint32_t *n) try {
+    if (!ctx) return RTX_E_INVALID;
+    const uint32_t count = (uint32_t)ctx->guideBoxes.size();
+    if (n) *n = count;
+    const int rc = guideSync(ctx);
+    if (rc != RTX_OK) return rc;
     const uint32_t k = std::min(capacity, count);
     if (regions) std::copy(ctx->guideBoxes.begin(), ctx->guideBoxes.begin() + k, regions);
     if (vmms) std::copy(ctx->guideVmms.begin(), ctx->guideVmms.begin() + k, vmms);
@@ -1348,7 +1429,7 @@ int rtx_guiding_init(rtx_ctx *ctx, const rtx_aabb *sceneBox, uint32_t splitCount
     ctx->fitFirst = true;
     ctx->fitValid = true;
     launchGuideInit(ctx->stream, g, 8, 5.0f);
-    launchGuideToVmm(ctx->stream, g, ctx->fitParallax, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
+    launchGuideToVmm(ctx->stream, g, ctx->fitParallax, const_cast<rtx_vmm_theta_wide *>(ctx->guide.vmms));
     CHECK_HIP(ctx, hipGetLastError());
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->guideVmmsStale = true;
@@ -1403,9 +1484,10 @@ static int guideSplitRegions(rtx_ctx *ctx) {
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<rtx_aabb> boxes = ctx->guideBoxes;
     bool any = false;
+    const size_t cap = ctx->guideOpts.maxRegions ? std::min<size_t>(ctx->guideOpts.maxRegions, 1u << 20) : 1u << 20;
     for (uint32_t r = 0; r < n; ++r) {
         if (!(regs[r].numSamples > (float)ctx->guideOpts.samplesForRegionSplit)) continue;
-        if (boxes.size() >= (1u << 20)) return fail(ctx, RTX_E_INVALID, "more than 2^20 guiding regions");
+        if (boxes.size() >= cap) break; // the cap: no further region is split
         const rtx_aabb b = boxes[r];
         const float size[3] = {b.max[0] - b.min[0], b.max[1] - b.min[1], b.max[2] - b.min[2]};
         const int axis = size[0] > size[1] ? (size[0] > size[2] ? 0 : 2) : (size[1] > size[2] ? 1 : 2);
@@ -1431,7 +1513,7 @@ static int guideSplitRegions(rtx_ctx *ctx) {
     CHECK_HIP(ctx, hipMemcpy(g.regions, regs.data(), m * sizeof(GuideRegion), hipMemcpyHostToDevice));
     g.numRegions = m;
     g.boxes = ctx->guide.boxes;
-    launchGuideToVmm(ctx->stream, g, ctx->fitParallax, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
+    launchGuideToVmm(ctx->stream, g, ctx->fitParallax, const_cast<rtx_vmm_theta_wide *>(ctx->guide.vmms));
     CHECK_HIP(ctx, hipGetLastError());
     return RTX_OK;
 }
@@ -1443,7 +1525,7 @@ static int guideUpdateAsync(rtx_ctx *ctx) {
     int rc = guideFitWork(ctx, ctx->dirCount);
     if (rc != RTX_OK) return rc;
     launchGuideUpdate(ctx->stream, ctx->fit, ctx->dirData, (uint32_t)ctx->dirCount, guideProps(), ctx->guideOpts,
-                      ctx->fitParallax, ctx->fitFirst, const_cast<rtx_vmm_theta *>(ctx->guide.vmms));
+                      ctx->fitParallax, ctx->fitFirst, const_cast<rtx_vmm_theta_wide *>(ctx->guide.vmms));
     CHECK_HIP(ctx, hipGetLastError());
     ctx->fitFirst = false;
     if (ctx->guideOpts.splitRegions) {
@@ -1472,6 +1554,8 @@ int rtx_guiding_set_options(rtx_ctx *ctx, const rtx_guiding_options *options) tr
     if (!options) return fail(ctx, RTX_E_INVALID, "NULL options");
     if (!(options->splitMinDivergence >= 0.0f) || !(options->mergeMaxDivergence >= 0.0f))
         return fail(ctx, RTX_E_INVALID, "divergence thresholds must be >= 0");
+    if (options->maxComponents < 1 || options->maxComponents > RTX_MAX_LOBES)
+        return fail(ctx, RTX_E_INVALID, "maxComponents must be in [1, RTX_MAX_LOBES]");
     ctx->guideOpts = *options;
     return RTX_OK;
 } catch (...) {
@@ -1523,7 +1607,6 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) try {
     ic.itemCap = 27u * maxEntries;
     ic.bucketBits = 0;
     while ((1u << ic.bucketBits) < ic.nb) ic.bucketBits++;
-    ic.sortTempBytes = icSortTempBytes(ic.itemCap, ic.bucketBits);
     ic.jobCap = 2u * maxEntries;
     ic.blockCap = kMaxBlockSums;
     hipError_t e = devAlloc(&ic.hdr, 1, L);
@@ -1535,9 +1618,11 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) try {
     if (e == hipSuccess) e = devAlloc(&ic.meta, 4, L);
     if (e == hipSuccess) e = devAlloc(&ic.items, ic.itemCap, L);
     if (e == hipSuccess) e = devAlloc(&ic.slotKeys, ic.itemCap, L);
-    if (e == hipSuccess) e = devAlloc(&ic.slotKeysSorted, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.slotKeysB, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.slotIdx, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.slotIdxB, ic.itemCap, L);
+    if (e == hipSuccess) e = devAlloc(&ic.sortHist, (size_t)((ic.itemCap + kSortBlock - 1) / kSortBlock) * kDigits, L);
     if (e == hipSuccess) e = devAlloc(&ic.slotItems, ic.itemCap, L);
-    if (e == hipSuccess) e = devAlloc(reinterpret_cast<uint8_t **>(&ic.sortTemp), std::max<size_t>(ic.sortTempBytes, 1), L);
     if (e == hipSuccess) e = devAlloc(&ic.frame, 8, L);
     if (e == hipSuccess) e = devAlloc(&ic.cand, 2 * (size_t)maxEntries, L);
     if (e == hipSuccess) e = devAlloc(&ic.jobs, 2 * (size_t)ic.jobCap, L);

@@ -235,10 +235,15 @@ struct Collapser {
     }
 };
 
-// Pad a coordinate bound outwards by ~2^-20 relative so that FP32 hits
-// accepted by the watertight test just outside an edge are never culled.
-inline float padLo(float x) { return x - (std::fabs(x) + 1.0f) * 9.5367431640625e-07f; }
-inline float padHi(float x) { return x + (std::fabs(x) + 1.0f) * 9.5367431640625e-07f; }
+// Pad a coordinate bound outwards by (|x| + S) 2^-20, S the scene's largest
+// coordinate magnitude (at least 1).  This covers FP32 hits that the
+// watertight test accepts just outside an edge, and the slab test's rounding:
+// it computes each plane distance as fma(l, 1/d, -o/d), whose point along the
+// axis lies within (|l| + 2|o|) 2^-24 (1 + 2^-23) of the plane, below
+// 16 (|l| + S) 2^-24 for every ray that starts within 7.5 S of the origin
+// (every secondary ray, and cameras up to 7 scene sizes away).
+inline float padLo(float x, float S) { return x - (std::fabs(x) + S) * 9.5367431640625e-07f; }
+inline float padHi(float x, float S) { return x + (std::fabs(x) + S) * 9.5367431640625e-07f; }
 
 } // namespace
 
@@ -248,13 +253,15 @@ void buildBvh(const float *tv, uint32_t numTris, BvhBuildResult &out) {
     b.primBox.resize(numTris);
     b.centroid.resize(3 * (size_t)numTris);
     b.order.resize(numTris);
+    float S = 1.0f;
+    for (size_t k = 0; k < 9 * (size_t)numTris; ++k) S = std::max(S, std::fabs(tv[k]));
     for (uint32_t i = 0; i < numTris; ++i) {
         Box &bx = b.primBox[i];
         bx.reset();
         for (int k = 0; k < 3; ++k) bx.grow(&tv[9 * (size_t)i + 3 * k]);
         for (int a = 0; a < 3; ++a) {
-            bx.lo[a] = padLo(bx.lo[a]);
-            bx.hi[a] = padHi(bx.hi[a]);
+            bx.lo[a] = padLo(bx.lo[a], S);
+            bx.hi[a] = padHi(bx.hi[a], S);
             b.centroid[3 * (size_t)i + a] = 0.5f * (bx.lo[a] + bx.hi[a]);
         }
         b.order[i] = i;

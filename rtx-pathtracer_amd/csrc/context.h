@@ -88,13 +88,14 @@ struct rtx_ctx {
     std::vector<void *> guideAllocs;
     DevGuiding guide{};
     std::vector<rtx_aabb> guideBoxes;
-    std::vector<rtx_vmm_theta> guideVmms;
+    std::vector<rtx_vmm_theta_wide> guideVmms;
     bool guideVmmsStale = false;      // the device mixtures moved on (updates)
     // the update (rtx_guiding_init): per-region mixtures and work buffers
     std::vector<void *> fitAllocs, fitWorkAllocs;
     GuideFitBuffers fit{};
     bool fitValid = false, fitParallax = true, fitFirst = true;
-    rtx_guiding_options guideOpts{1, 8192, 4096, 4096, 0.5f, 0.025f, 0, 10000}; // src/PathGuiding.h:118-129
+    // src/PathGuiding.h:118-129; 16 lobes at most (shaders/limits.glsl:6), no region or sample cap
+    rtx_guiding_options guideOpts{1, 8192, 4096, 4096, 0.5f, 0.025f, 0, 10000, RTX_MAX_DISTRIBUTIONS, 0, 0, 0};
     // frames of several segments: per-path frame state, split records
     std::vector<void *> segAllocs, splitAllocs;
     size_t segCap = 0, splitCap = 0;

@@ -51,7 +51,7 @@ RTX_DI float vmfPdf(V3 wo, const rtx_vmf_theta &t, V3 x, bool parallax) {
     return t.norm * expf(t.k * (dot(mu, wo) - 1.0f));
 }
 
-RTX_DI float vmmPdf(V3 wo, const rtx_vmm_theta &m, V3 x, bool parallax) { // guiding.glsl:57-65
+RTX_DI float vmmPdf(V3 wo, const rtx_vmm_theta_wide &m, V3 x, bool parallax) { // guiding.glsl:57-65
     float res = 0.0f;
     for (int i = 0; i < m.usedDistributions; ++i) res += m.pi[i] * vmfPdf(wo, m.thetas[i], x, parallax);
     return res;
@@ -71,7 +71,7 @@ RTX_DI V3 sampleVmf(Rng &r, const rtx_vmf_theta &t, V3 x, bool parallax) { // gu
     return randomOnUnitSphere(r);
 }
 
-RTX_DI V3 sampleVmm(Rng &r, const rtx_vmm_theta &m, V3 x, bool parallax) { // guiding.glsl:91-104
+RTX_DI V3 sampleVmm(Rng &r, const rtx_vmm_theta_wide &m, V3 x, bool parallax) { // guiding.glsl:91-104
     float u = r.next();
     int i = 0;
     const int last = m.usedDistributions - 1;
@@ -96,7 +96,7 @@ RTX_DI float getNewDirection(Rng &r, const DevGuiding &G, const GuidingParams &g
     const int region = guidingRegion(G, x);
     if (regionOut) *regionOut = region;
     if (region < 0) return 0.0f;
-    const rtx_vmm_theta &vm = G.vmms[region];
+    const rtx_vmm_theta_wide &vm = G.vmms[region];
     float pdfMat;
     if (r.next() < gp.prob) {
         newDir = sampleVmm(r, vm, x, gp.parallax);

@@ -18,6 +18,7 @@ namespace rtx {
 
 struct TravRay {
     V3 o, d, inv;
+    V3 oinv; // o * inv: slab plane distances are fma(l, inv, -oinv)
     int kx, ky, kz;
     float Sx, Sy, Sz;
 };
@@ -31,6 +32,7 @@ RTX_DI TravRay makeTravRay(V3 o, V3 d) {
     float dy = d.y != 0.0f ? d.y : copysignf(1e-30f, d.y);
     float dz = d.z != 0.0f ? d.z : copysignf(1e-30f, d.z);
     r.inv = v3(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    r.oinv = v3(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
     // watertight test set-up (Woop, Benthin, Wald, JCGT 2013)
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
@@ -51,13 +53,15 @@ RTX_DI TravRay makeTravRay(V3 o, V3 d) {
     return r;
 }
 
-// Slab test against [lo, hi]; conservative by the 1 + 2*gamma_3 factor of
-// Ize, "Robust BVH ray traversal" (JCGT 2013).
+// Slab test against [lo, hi]: plane distances as one FMA each,
+// fma(l, inv, -o * inv); the absolute part of its error (o * inv rounded) is
+// covered by the builder's box padding (bvh_build.cpp), the relative part by
+// the 1 + 2*gamma_3 factor of Ize, "Robust BVH ray traversal" (JCGT 2013).
 RTX_DI bool slab(float lx, float ly, float lz, float hx, float hy, float hz, const TravRay &r, float tmin,
                  float tmax, float &tnear) {
-    float tx0 = (lx - r.o.x) * r.inv.x, tx1 = (hx - r.o.x) * r.inv.x;
-    float ty0 = (ly - r.o.y) * r.inv.y, ty1 = (hy - r.o.y) * r.inv.y;
-    float tz0 = (lz - r.o.z) * r.inv.z, tz1 = (hz - r.o.z) * r.inv.z;
+    float tx0 = fmaf(lx, r.inv.x, -r.oinv.x), tx1 = fmaf(hx, r.inv.x, -r.oinv.x);
+    float ty0 = fmaf(ly, r.inv.y, -r.oinv.y), ty1 = fmaf(hy, r.inv.y, -r.oinv.y);
+    float tz0 = fmaf(lz, r.inv.z, -r.oinv.z), tz1 = fmaf(hz, r.inv.z, -r.oinv.z);
     float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.0000004f;
     tnear = tn;

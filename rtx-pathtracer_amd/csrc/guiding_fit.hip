@@ -22,14 +22,12 @@
 #include <hip/hip_runtime.h>
 
 #include "dev_math.h"
+#include "dev_sort.h"
 #include "internal.h"
 
 namespace rtx {
 
-constexpr int kK = RTX_MAX_DISTRIBUTIONS;
-constexpr int kSortBlock = 1024;
-constexpr int kDigitBits = 6;
-constexpr int kDigits = 1 << kDigitBits;
+constexpr int kK = RTX_MAX_LOBES; // array bound; rtx_guiding_options.maxComponents caps the lobes in use
 constexpr int kEmBlock = 256;
 
 // fastapprox fastexp (P. Mineiro, 2011), as lightpmm evaluates exp
@@ -54,48 +52,6 @@ RTX_DI float meanCosineToKappa(float r) { return (r * 3.0f - r * r * r) / (1.0f 
 RTX_DI void calNorm(GuideRegion &g, int k) { // VMFKernel.h:651-659
     g.e2k[k] = fastexpf(-2.0f * g.kappa[k]);
     g.norm[k] = g.kappa[k] > 0.0f ? g.kappa[k] / (2.0f * kPi * (1.0f - g.e2k[k])) : 1.0f / (4.0f * kPi);
-}
-
-// ------------------------------------------------------------ block scans
-template <int BLOCK> RTX_DI uint32_t scanExclusive(uint32_t v, uint32_t *sh, uint32_t &total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) sh[wave] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int w = 0; w < BLOCK / 64; ++w) {
-            uint32_t t = sh[w];
-            sh[w] = acc;
-            acc += t;
-        }
-        sh[BLOCK / 64] = acc;
-    }
-    __syncthreads();
-    const uint32_t r = sh[wave] + x - v;
-    total = sh[BLOCK / 64];
-    __syncthreads();
-    return r;
-}
-
-// One block: exclusive scan of n values in place, base added; returns nothing.
-__global__ __launch_bounds__(1024) void kScanInPlace(uint32_t *a, uint32_t n, uint32_t *totalOut) {
-    __shared__ uint32_t sh[1024 / 64 + 1];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += 1024) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < n ? a[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = scanExclusive<1024>(v, sh, total);
-        if (i < n) a[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0 && totalOut) *totalOut = carry;
 }
 
 // ---------------------------------------------------------- compaction
@@ -123,62 +79,6 @@ __global__ __launch_bounds__(kSortBlock) void kCompactScatter(const rtx_directio
     if (v) {
         keys[pos] = dd[i].flags;
         vals[pos] = i;
-    }
-}
-
-// ------------------------------------------------------------ radix sort
-__global__ __launch_bounds__(kSortBlock) void kRadixHist(const uint32_t *keys, const uint32_t *count, int shift,
-                                                        uint32_t nblk, uint32_t *hist) {
-    __shared__ uint32_t h[kDigits];
-    if (threadIdx.x < kDigits) h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t n = *count;
-    const uint32_t i = blockIdx.x * kSortBlock + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kDigits - 1)], 1u);
-    __syncthreads();
-    if (threadIdx.x < kDigits) hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
-}
-
-// Stable scatter: an item's place among the block's items of its digit is
-// the count in earlier waves plus the count in earlier lanes of its wave
-// (lanes of a digit matched with one ballot per digit bit).
-__global__ __launch_bounds__(kSortBlock) void kRadixScatter(const uint32_t *keysIn, const uint32_t *valsIn,
-                                                           uint32_t *keysOut, uint32_t *valsOut,
-                                                           const uint32_t *count, int shift, uint32_t nblk,
-                                                           const uint32_t *hist) {
-    constexpr int kWaves = kSortBlock / 64;
-    __shared__ uint32_t cnt[kWaves][kDigits];
-    for (int t = threadIdx.x; t < kWaves * kDigits; t += kSortBlock) (&cnt[0][0])[t] = 0;
-    __syncthreads();
-    const uint32_t n = *count;
-    const uint32_t i = blockIdx.x * kSortBlock + threadIdx.x;
-    const bool valid = i < n;
-    const uint32_t key = valid ? keysIn[i] : 0u;
-    const uint32_t d = (key >> shift) & (kDigits - 1);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint64_t m = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < kDigitBits; ++b) {
-        const uint64_t bb = __ballot((d >> b) & 1u);
-        m &= ((d >> b) & 1u) ? bb : ~bb;
-    }
-    const uint64_t below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    const uint32_t rankInWave = (uint32_t)__popcll(below);
-    if (valid && rankInWave == 0) cnt[wave][d] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x < kDigits) { // exclusive prefix over the waves, per digit
-        uint32_t acc = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            uint32_t t = cnt[w][threadIdx.x];
-            cnt[w][threadIdx.x] = acc;
-            acc += t;
-        }
-    }
-    __syncthreads();
-    if (valid) {
-        const uint32_t pos = hist[d * nblk + blockIdx.x] + cnt[wave][d] + rankInWave;
-        keysOut[pos] = key;
-        valsOut[pos] = valsIn[i];
     }
 }
 
@@ -647,8 +547,8 @@ RTX_DI void eigen2x2(float a00, float a01, float a10, float a11, float lam[2], f
 
 // splitComponentUsingPCA (src/PathGuiding.cpp:494-561); the principal axis
 // is a row of Eigen's eigenvector matrix, as the reference reads it.
-RTX_DI void splitComponent(GuideRegion &R, int comp, float maxKappa) {
-    if (R.K == kK) return;
+RTX_DI void splitComponent(GuideRegion &R, int comp, float maxKappa, int maxK) {
+    if (R.K >= maxK) return;
     const int tgt = R.K;
     float fx[3], fy[3];
     tangentFrame(R.mu[comp], fx, fy);
@@ -952,7 +852,8 @@ __global__ __launch_bounds__(kEmBlock) void kSplitMerge(GuideFitBuffers g, EmPro
     const bool fit = firstFit || n > SP.minSamplesForPostSplitFitting;
     if (threadIdx.x == 0) pmmRemovePrior(R, P.vPrior);
     __syncthreads();
-    if (R.K < kK) {
+    const int maxK = (int)SP.maxComponents;
+    if (R.K < maxK) {
         for (;;) {
             if (threadIdx.x == 0) {
                 const int K = R.K;
@@ -979,7 +880,7 @@ __global__ __launch_bounds__(kEmBlock) void kSplitMerge(GuideFitBuffers g, EmPro
                     ++first;
                 }
                 const int nCand = first;
-                const int numSplits = min(nCand, kK - K);
+                const int numSplits = min(nCand, maxK - K);
                 if (nCand > numSplits) // the numSplits largest, descending (std::partial_sort)
                     for (int q = 0; q < numSplits; ++q) {
                         int best = q;
@@ -994,7 +895,7 @@ __global__ __launch_bounds__(kEmBlock) void kSplitMerge(GuideFitBuffers g, EmPro
                 for (int q = 0; q < numSplits; ++q) {
                     S.mask[ci[q]] = 1;
                     S.mask[R.K] = 1;
-                    splitComponent(R, ci[q], P.maxKappa);
+                    splitComponent(R, ci[q], P.maxKappa, maxK);
                 }
                 S.numSplits = numSplits;
             }
@@ -1069,11 +970,11 @@ __global__ __launch_bounds__(kEmBlock) void kDistances(GuideFitBuffers g) {
 
 // syncPMMsToVMM_Thetas (src/PathGuiding.cpp:117-139) with VMF_Theta::setK
 // (src/PathGuiding.h:44-50)
-__global__ __launch_bounds__(64) void kToVmm(GuideFitBuffers g, bool parallax, rtx_vmm_theta *out) {
+__global__ __launch_bounds__(64) void kToVmm(GuideFitBuffers g, bool parallax, rtx_vmm_theta_wide *out) {
     const uint32_t r = blockIdx.x * 64 + threadIdx.x;
     if (r >= g.numRegions) return;
     const GuideRegion &R = g.regions[r];
-    rtx_vmm_theta v;
+    rtx_vmm_theta_wide v;
     v.usedDistributions = R.K;
     v.meanPosition[0] = v.meanPosition[1] = v.meanPosition[2] = 0.0f;
     for (int k = 0; k < kK; ++k) {
@@ -1144,6 +1045,10 @@ __global__ __launch_bounds__(64) void kGuideInit(GuideFitBuffers g, uint32_t K, 
     g.regions[r] = R;
 }
 
+__global__ void kClampCount(uint32_t *counters, uint32_t cap) {
+    if (threadIdx.x == 0 && counters[0] > cap) counters[0] = cap;
+}
+
 // ------------------------------------------------------------- launchers
 static uint32_t blocksOf(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
@@ -1152,27 +1057,29 @@ void launchGuideInit(hipStream_t s, GuideFitBuffers g, uint32_t numInitialCompon
                        initKappa);
 }
 
-void launchGuideToVmm(hipStream_t s, GuideFitBuffers g, bool parallax, rtx_vmm_theta *vmmsOut) {
+void launchGuideToVmm(hipStream_t s, GuideFitBuffers g, bool parallax, rtx_vmm_theta_wide *vmmsOut) {
     hipLaunchKernelGGL(kToVmm, dim3(blocksOf(g.numRegions, 64)), dim3(64), 0, s, g, parallax, vmmsOut);
 }
 
 void launchGuideUpdate(hipStream_t s, GuideFitBuffers g, const rtx_directional_data *dd, uint32_t slots,
                        EmProps props, const rtx_guiding_options &sm, bool parallax, bool firstFit,
-                       rtx_vmm_theta *vmmsOut) {
+                       rtx_vmm_theta_wide *vmmsOut) {
     const uint32_t nblk = blocksOf(slots, kSortBlock);
     // valid samples in slot order
     hipLaunchKernelGGL(kCompactCount, dim3(nblk), dim3(kSortBlock), 0, s, dd, slots, g.numRegions, g.blockSums);
-    hipLaunchKernelGGL(kScanInPlace, dim3(1), dim3(1024), 0, s, g.blockSums, nblk, g.counters);
+    hipLaunchKernelGGL(kScanInPlace<0>, dim3(1), dim3(1024), 0, s, g.blockSums, nblk, g.counters);
     hipLaunchKernelGGL(kCompactScatter, dim3(nblk), dim3(kSortBlock), 0, s, dd, slots, g.numRegions, g.blockSums,
                        g.keysA, g.valsA);
+    // at most maxSamplesPerUpdate of them: the first in slot order
+    if (sm.maxSamplesPerUpdate) hipLaunchKernelGGL(kClampCount, dim3(1), dim3(64), 0, s, g.counters, sm.maxSamplesPerUpdate);
     // stable sort by region
     int passes = 0;
     for (uint32_t span = 1; span < g.numRegions; span <<= kDigitBits) ++passes;
     for (int p = 0; p < passes; ++p) {
         const int shift = p * kDigitBits;
-        hipLaunchKernelGGL(kRadixHist, dim3(nblk), dim3(kSortBlock), 0, s, g.keysA, g.counters, shift, nblk, g.hist);
-        hipLaunchKernelGGL(kScanInPlace, dim3(1), dim3(1024), 0, s, g.hist, nblk * kDigits, (uint32_t *)nullptr);
-        hipLaunchKernelGGL(kRadixScatter, dim3(nblk), dim3(kSortBlock), 0, s, g.keysA, g.valsA, g.keysB, g.valsB,
+        hipLaunchKernelGGL(kRadixHist<0>, dim3(nblk), dim3(kSortBlock), 0, s, g.keysA, g.counters, shift, nblk, g.hist);
+        hipLaunchKernelGGL(kScanInPlace<0>, dim3(1), dim3(1024), 0, s, g.hist, nblk * kDigits, (uint32_t *)nullptr);
+        hipLaunchKernelGGL(kRadixScatter<0>, dim3(nblk), dim3(kSortBlock), 0, s, g.keysA, g.valsA, g.keysB, g.valsB,
                            g.counters, shift, nblk, g.hist);
         std::swap(g.keysA, g.keysB);
         std::swap(g.valsA, g.valsB);
@@ -1180,7 +1087,7 @@ void launchGuideUpdate(hipStream_t s, GuideFitBuffers g, const rtx_directional_d
     // region segments
     hipLaunchKernelGGL(kZero, dim3(blocksOf(g.numRegions + 1, 256)), dim3(256), 0, s, g.regionStart, g.numRegions + 1);
     hipLaunchKernelGGL(kSegmentCount, dim3(1024), dim3(256), 0, s, g.keysA, g.counters, g.regionStart);
-    hipLaunchKernelGGL(kScanInPlace, dim3(1), dim3(1024), 0, s, g.regionStart, g.numRegions + 1, (uint32_t *)nullptr);
+    hipLaunchKernelGGL(kScanInPlace<0>, dim3(1), dim3(1024), 0, s, g.regionStart, g.numRegions + 1, (uint32_t *)nullptr);
     // preFit, EM, distances
     hipLaunchKernelGGL(kPrefit, dim3(blocksOf(g.numRegions, 64)), dim3(64), 0, s, g, parallax, firstFit);
     hipLaunchKernelGGL(kGather, dim3(1024), dim3(256), 0, s, g, dd, parallax);

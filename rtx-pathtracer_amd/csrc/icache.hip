@@ -19,11 +19,11 @@
 //   kIcIndex*      - rebuild of the lookup index (dev_icache.h)
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
 
 #include "dev_icache.h"
 #include "dev_math.h"
 #include "dev_shading.h"
+#include "dev_sort.h"
 #include "internal.h"
 
 namespace rtx {
@@ -158,7 +158,12 @@ RTX_DI V3 sphericalToCartesian(float theta, float phi) { // shaders/transform.gl
 // sub-path results (radiance, first-hit distance in w) of each job of the
 // sub-batch, in the reference's loop order.
 __global__ __launch_bounds__(64) void kIcReduce(IcBuffers ic, uint32_t frameSeed, uint32_t firstJob,
-                                                uint32_t numJobs, PathBuffers pb) {
+                                                uint32_t numJobs, PathBuffers pb, int devCount) {
+    // devCount: the frame's job count in device memory (ic.frame[4]) bounds numJobs
+    if (devCount) {
+        const uint32_t all = ic.frame[4];
+        numJobs = min(numJobs, all > firstJob ? all - firstJob : 0u);
+    }
     // One 64-lane block per job: the lanes compute every stratum's terms
     // (trigonometry, neighbour differences) in parallel, then lane 0 adds
     // them in the loop order of calculateCacheData, so the sums are the
@@ -387,8 +392,16 @@ template <bool SCATTER> __global__ __launch_bounds__(256) void kIcIndexFill(IcBu
                     ++k;
                 }
     }
-    if (SCATTER)
+    if (SCATTER) {
         for (; k < 27; ++k) ic.slotKeys[27 * (size_t)i + k] = ic.nb;
+        for (uint32_t q = 0; q < 27; ++q) ic.slotIdx[27 * (size_t)i + q] = 27u * i + q;
+    }
+}
+
+// items[j] = the item of the j-th slot in bucket order, for the meta[1] items in use.
+__global__ __launch_bounds__(256) void kIcIndexGather(IcBuffers ic, const uint32_t *order) {
+    const uint32_t n = ic.meta[1];
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) ic.items[j] = ic.slotItems[order[j]];
 }
 
 __global__ __launch_bounds__(1024) void kIcIndexScan(IcBuffers ic) {
@@ -405,6 +418,7 @@ __global__ __launch_bounds__(1024) void kIcIndexScan(IcBuffers ic) {
     if (threadIdx.x == 0) {
         ic.bucketStart[ic.nb] = carry;
         ic.meta[1] = carry;
+        ic.meta[3] = 27u * min(ic.hdr->nextCacheSlot, ic.hdr->maxCaches); // slots of the records in use
     }
 }
 
@@ -425,9 +439,10 @@ void launchIcPlan(hipStream_t s, IcBuffers ic) {
     hipLaunchKernelGGL(kIcPlan, dim3(1), dim3(256), 0, s, ic);
 }
 void launchIcReduce(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numJobs,
-                    PathBuffers pb) {
+                    PathBuffers pb, bool devCount) {
     if (numJobs == 0) return;
-    hipLaunchKernelGGL(kIcReduce, dim3(numJobs), dim3(64), 0, s, ic, frameSeed, firstJob, numJobs, pb);
+    hipLaunchKernelGGL(kIcReduce, dim3(numJobs), dim3(64), 0, s, ic, frameSeed, firstJob, numJobs, pb,
+                       devCount ? 1 : 0);
 }
 void launchIcCommit(hipStream_t s, IcBuffers ic, const FrameConst *fc) {
     hipLaunchKernelGGL(kIcCommit, dim3(1), dim3(64), 0, s, ic, fc);
@@ -440,18 +455,12 @@ void launchIcIndex(hipStream_t s, IcBuffers ic, float c0) {
     hipLaunchKernelGGL(kIcIndexFill<false>, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
     hipLaunchKernelGGL(kIcIndexScan, dim3(1), dim3(1024), 0, s, ic);
     hipLaunchKernelGGL(kIcIndexFill<true>, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
-    // stable LSD radix sort by bucket (keys below 2^(bucketBits + 1): nb itself marks unused slots)
-    size_t bytes = ic.sortTempBytes;
-    (void)hipcub::DeviceRadixSort::SortPairs(ic.sortTemp, bytes, ic.slotKeys, ic.slotKeysSorted, ic.slotItems,
-                                             ic.items, (int)ic.itemCap, 0, (int)ic.bucketBits + 1, s);
-}
-
-size_t icSortTempBytes(uint32_t items, uint32_t bucketBits) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                             (const IcItem *)nullptr, (IcItem *)nullptr, (int)items, 0,
-                                             (int)bucketBits + 1, (hipStream_t)0);
-    return bytes;
+    // stable LSD radix sort of (bucket, slot) over the slots of the records in
+    // use (meta[3]; keys below 2^(bucketBits + 1): nb itself marks unused
+    // slots, which sort last), then the items gathered in that order
+    uint32_t *keysA = ic.slotKeys, *valsA = ic.slotIdx, *keysB = ic.slotKeysB, *valsB = ic.slotIdxB;
+    launchRadixSortPairs(s, keysA, valsA, keysB, valsB, ic.meta + 3, ic.itemCap, (int)ic.bucketBits + 1, ic.sortHist);
+    hipLaunchKernelGGL(kIcIndexGather, dim3(blocksFor(ic.itemCap, 256)), dim3(256), 0, s, ic, valsA);
 }
 
 } // namespace rtx

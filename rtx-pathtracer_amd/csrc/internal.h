@@ -114,7 +114,7 @@ struct GuideNode {
 
 struct DevGuiding {
     const rtx_aabb *boxes;
-    const rtx_vmm_theta *vmms;
+    const rtx_vmm_theta_wide *vmms;
     const GuideNode *nodes;
     uint32_t numRegions, numNodes;
 };
@@ -132,16 +132,16 @@ struct GuideRegion {
     uint32_t numEMIterations;
     float sampleWeight, numSamples;
     uint64_t totalNumSamples;
-    float w[RTX_MAX_DISTRIBUTIONS], kappa[RTX_MAX_DISTRIBUTIONS], r[RTX_MAX_DISTRIBUTIONS];
-    float norm[RTX_MAX_DISTRIBUTIONS], e2k[RTX_MAX_DISTRIBUTIONS];
-    float mu[RTX_MAX_DISTRIBUTIONS][3];
-    float dist[RTX_MAX_DISTRIBUTIONS], distW[RTX_MAX_DISTRIBUTIONS];
+    float w[RTX_MAX_LOBES], kappa[RTX_MAX_LOBES], r[RTX_MAX_LOBES];
+    float norm[RTX_MAX_LOBES], e2k[RTX_MAX_LOBES];
+    float mu[RTX_MAX_LOBES][3];
+    float dist[RTX_MAX_LOBES], distW[RTX_MAX_LOBES];
     float mean[3], lastMean[3];
     // postFit's statistics (external/guiding/incrementalpearsonchisquared.h,
     // incrementalcovariance2d.h): divergence + 1 times the squared integral and
     // its sample count; tangent-plane (var x, var y, cov xy) and weight sum
-    float chi[RTX_MAX_DISTRIBUTIONS], chiN[RTX_MAX_DISTRIBUTIONS];
-    float cov[RTX_MAX_DISTRIBUTIONS][3], covW[RTX_MAX_DISTRIBUTIONS];
+    float chi[RTX_MAX_LOBES], chiN[RTX_MAX_LOBES];
+    float cov[RTX_MAX_LOBES][3], covW[RTX_MAX_LOBES];
     uint32_t samplesSinceLastMerge;
 };
 
@@ -169,8 +169,8 @@ void launchGuideInit(hipStream_t s, GuideFitBuffers g, uint32_t numInitialCompon
 // PathGuiding::update on the recorded samples (slots = pixels * 16)
 void launchGuideUpdate(hipStream_t s, GuideFitBuffers g, const rtx_directional_data *dd, uint32_t slots,
                        EmProps props, const rtx_guiding_options &sm, bool parallax, bool firstFit,
-                       rtx_vmm_theta *vmmsOut);
-void launchGuideToVmm(hipStream_t s, GuideFitBuffers g, bool parallax, rtx_vmm_theta *vmmsOut);
+                       rtx_vmm_theta_wide *vmmsOut);
+void launchGuideToVmm(hipStream_t s, GuideFitBuffers g, bool parallax, rtx_vmm_theta_wide *vmmsOut);
 
 // ------------------------------------------------------ irradiance cache
 struct IcItem {              // one (record, grid cell) pair of the lookup index
@@ -199,11 +199,11 @@ struct IcBuffers {
     uint32_t *bucketCount, *bucketStart, *cursor, *meta;
     IcItem *items;
     // the index build's sort: 27 fixed slots per record (bucket keys and
-    // items, unused slots keyed nb), sorted stably by bucket into `items`
-    uint32_t *slotKeys, *slotKeysSorted;
+    // items, unused slots keyed nb); (key, slot) pairs are sorted stably by
+    // bucket (keys, slot ids and their ping-pong copies, digit histograms),
+    // then the items gathered into `items`
+    uint32_t *slotKeys, *slotKeysB, *slotIdx, *slotIdxB, *sortHist;
     IcItem *slotItems;
-    void *sortTemp;
-    size_t sortTempBytes;
     uint32_t bucketBits; // log2(nb)
     // frame: [0] update elections, [1] creation candidates, [2] free slots at
     // frame start, [3] update jobs, [4] jobs
@@ -322,18 +322,18 @@ void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, 
 void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const DevGuiding &g, const FrameConst *fc,
                  BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt,
                  int qsel, int blocks);
+// devCount: numPaths (numJobs) is an upper bound and the frame's job count in
+// device memory (ic.frame[4]) decides, so no host round trip sizes the launch
 void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numPaths,
-                      PathBuffers pb, Queue q0, Counters *cnt, int qsel);
+                      PathBuffers pb, Queue q0, Counters *cnt, int qsel, bool devCount);
 // irradiance-cache frame bookkeeping (icache.hip)
 void launchIcFrameBegin(hipStream_t s, IcBuffers ic);
 void launchIcCollect(hipStream_t s, IcBuffers ic, PathBuffers pb, uint32_t numPaths);
 void launchIcPlan(hipStream_t s, IcBuffers ic);
 void launchIcReduce(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numJobs,
-                    PathBuffers pb);
+                    PathBuffers pb, bool devCount);
 void launchIcCommit(hipStream_t s, IcBuffers ic, const FrameConst *fc);
 void launchIcIndex(hipStream_t s, IcBuffers ic, float c0);
-// Device temp bytes the index build's radix sort needs for `items` slots.
-size_t icSortTempBytes(uint32_t items, uint32_t bucketBits);
 void launchIcInit(hipStream_t s, IcBuffers ic);
 // Ends the segments of the previous bounce's done list (done[qsel ^ 1]);
 // with FrameConst::segments a path with camera samples left or splits queued

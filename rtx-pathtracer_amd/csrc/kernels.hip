@@ -257,7 +257,12 @@ __global__ __launch_bounds__(256) void kInstallSlot(SlotDesc sd, SlotDesc *slots
 // addNextDirectLights off.
 __global__ __launch_bounds__(256) void kGenerateIC(IcBuffers ic, uint32_t frameSeed, uint32_t firstJob,
                                                    uint32_t numPaths, PathBuffers pb, Queue q0, Counters *cnt,
-                                                   int qsel) {
+                                                   int qsel, int devCount) {
+    // devCount: the frame's job count in device memory (ic.frame[4]) bounds numPaths
+    if (devCount) {
+        const uint32_t all = ic.frame[4];
+        numPaths = min(numPaths, (all > firstJob ? all - firstJob : 0u) * (uint32_t)kIcStrata);
+    }
     uint32_t pid = blockIdx.x * 256 + threadIdx.x;
     if (pid == 0) cnt->extend[qsel] = numPaths;
     if (pid >= numPaths) return;
@@ -1250,10 +1255,10 @@ void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffer
                        electCounter);
 }
 void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numPaths,
-                      PathBuffers pb, Queue q0, Counters *cnt, int qsel) {
+                      PathBuffers pb, Queue q0, Counters *cnt, int qsel, bool devCount) {
     if (numPaths == 0) return;
     hipLaunchKernelGGL(kGenerateIC, dim3(gridFor(numPaths, 256)), dim3(256), 0, s, ic, frameSeed, firstJob, numPaths,
-                       pb, q0, cnt, qsel);
+                       pb, q0, cnt, qsel, devCount ? 1 : 0);
 }
 template <int LDS>
 static void launchTracePL(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,

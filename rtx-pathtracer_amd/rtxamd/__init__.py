@@ -108,6 +108,8 @@ def device_lib():
         L.rtx_guiding_get.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmTheta), C.c_uint32,
                                       C.POINTER(C.c_uint32)]
         L.rtx_guiding_read_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.rtx_guiding_get_wide.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.POINTER(abi.VmmThetaWide), C.c_uint32,
+                                           C.POINTER(C.c_uint32)]
         L.rtx_guiding_init.argtypes = [C.c_void_p, C.POINTER(abi.Aabb), C.c_uint32, C.c_int]
         L.rtx_guiding_update.argtypes = [C.c_void_p]
         L.rtx_guiding_set_options.argtypes = [C.c_void_p, C.POINTER(abi.GuidingOptions)]
@@ -665,6 +667,16 @@ class Renderer:
 
     def guiding_update(self):
         self._check(device_lib().rtx_guiding_update(self._h))
+
+    def guiding_get_wide(self):
+        """(boxes (n, 2, 3), [abi.VmmThetaWide]): mixtures of up to 32 lobes."""
+        n = C.c_uint32()
+        self._check(device_lib().rtx_guiding_get_wide(self._h, None, None, 0, C.byref(n)))
+        boxes = (abi.Aabb * max(1, n.value))()
+        vmms = (abi.VmmThetaWide * max(1, n.value))()
+        self._check(device_lib().rtx_guiding_get_wide(self._h, boxes, vmms, n.value, C.byref(n)))
+        arr = np.array([[boxes[i].min[:], boxes[i].max[:]] for i in range(n.value)], np.float32)
+        return arr, [vmms[i] for i in range(n.value)]
 
     def guiding_get(self):
         """(boxes (n, 2, 3), [abi.VmmTheta]) of the context."""

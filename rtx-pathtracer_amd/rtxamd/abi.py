@@ -73,6 +73,7 @@ class Aabb(C.Structure):
 
 
 MAX_DISTRIBUTIONS = 16
+RTX_MAX_LOBES = 32
 
 
 class VmfTheta(C.Structure):
@@ -82,6 +83,11 @@ class VmfTheta(C.Structure):
 
 class VmmTheta(C.Structure):
     _fields_ = [("thetas", VmfTheta * MAX_DISTRIBUTIONS), ("pi", C.c_float * MAX_DISTRIBUTIONS),
+                ("meanPosition", F3), ("usedDistributions", C.c_int32)]
+
+
+class VmmThetaWide(C.Structure):
+    _fields_ = [("thetas", VmfTheta * RTX_MAX_LOBES), ("pi", C.c_float * RTX_MAX_LOBES),
                 ("meanPosition", F3), ("usedDistributions", C.c_int32)]
 
 
@@ -256,12 +262,15 @@ class GuidingOptions(C.Structure):
     _fields_ = [("splitAndMerge", C.c_int32), ("minSamplesForMerging", C.c_uint32),
                 ("minSamplesForSplitting", C.c_uint32), ("minSamplesForPostSplitFitting", C.c_uint32),
                 ("splitMinDivergence", C.c_float), ("mergeMaxDivergence", C.c_float),
-                ("splitRegions", C.c_int32), ("samplesForRegionSplit", C.c_uint32)]
+                ("splitRegions", C.c_int32), ("samplesForRegionSplit", C.c_uint32),
+                ("maxComponents", C.c_uint32), ("maxRegions", C.c_uint32), ("maxSamplesPerUpdate", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 def guiding_options(**kw):
-    """PathGuiding's defaults (src/PathGuiding.h:118-129), overridden by kw."""
-    o = GuidingOptions(1, 8192, 4096, 4096, 0.5, 0.025, 0, 10000)
+    """PathGuiding's defaults (src/PathGuiding.h:118-129; 16 lobes, no region
+    or sample cap), overridden by kw."""
+    o = GuidingOptions(1, 8192, 4096, 4096, 0.5, 0.025, 0, 10000, MAX_DISTRIBUTIONS, 0, 0, 0)
     for k, v in kw.items():
         if not hasattr(o, k):
             raise KeyError(k)
@@ -290,7 +299,7 @@ class View(C.Structure):
 EXPECTED_SIZES = {
     Vertex: 48, Material: 96, Instance: 144, Light: 40, FaceSample: 12, Sphere: 24,
     PushConstant: 192, Camera: 256, Ray: 32, Hit: 16, Tuning: 32, CacheHeader: 12, CacheData: 56, IcJob: 32,
-    Aabb: 24, VmfTheta: 40, VmmTheta: 720, GuidingOptions: 32,
+    Aabb: 24, VmfTheta: 40, VmmTheta: 720, VmmThetaWide: 1424, GuidingOptions: 48,
 }
 for _t, _n in EXPECTED_SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)

@@ -22,6 +22,7 @@ C_NAMES = {
     abi.VmmTheta: "rtx_vmm_theta", abi.NrcConfig: "rtx_nrc_config", abi.NrcInput: "rtx_nrc_input",
     abi.NrcInfo: "rtx_nrc_info", abi.ServerConfig: "rtx_server_config", abi.JobDesc: "rtx_job_desc",
     abi.JobStatus: "rtx_job_status", abi.ServerStats: "rtx_server_stats", abi.AutotuneResult: "rtx_autotune_result",
+    abi.GuidingOptions: "rtx_guiding_options", abi.VmmThetaWide: "rtx_vmm_theta_wide",
 }
 
 

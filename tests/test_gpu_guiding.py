@@ -278,3 +278,133 @@ def test_trained_guiding_is_unbiased_and_reduces_variance(renderer):
     assert abs(la.mean() - lb.mean()) < 4 * se
     renderer.guiding_set([], [])
     ds.close()
+
+
+def mixture_faults(vmm):
+    """Broken invariants of a fitted mixture (an empty list when it is valid):
+    weights positive, kappa >= 0, unit means.  That the weights sum to 1 is
+    not one of them: the refit folds the previous fit back in through the
+    weight prior's removal, clamped at zero for lobes below the prior
+    (removeWeightPrior), so the sums drift; they are checked against the
+    oracle's instead."""
+    K = vmm.usedDistributions
+    w = np.array(vmm.pi[:K], np.float64)
+    mu = np.array([vmm.thetas[k].mu[:] for k in range(K)], np.float64)
+    kap = np.array([vmm.thetas[k].k for k in range(K)], np.float64)
+    faults = []
+    if not (w > 0).all():
+        faults.append(f"weights {w[~(w > 0)]}")
+    if not (kap >= 0).all():
+        faults.append(f"kappa {kap[~(kap >= 0)]}")
+    norm = np.linalg.norm(mu, axis=1)
+    off = ~(np.abs(norm - 1) <= 1e-4)
+    if off.any():
+        faults.append(f"|mu| {norm[off]} at kappa {kap[off]}, weight {w[off]}")
+    return faults
+
+
+def compare_guides(dev, orc, dirs):
+    """(regions whose lobe counts agree, density L1 of each such region
+    relative to the oracle's mean density, weight-sum difference of each)."""
+    (db, dv), (ob, ov) = dev, orc
+    assert len(db) == len(ob) and np.allclose(db, ob, atol=1e-5)
+    same, l1, dw = [], [], []
+    for r in range(len(dv)):
+        if dv[r].usedDistributions != ov[r].usedDistributions:
+            continue
+        same.append(r)
+        x = np.array(ov[r].meanPosition[:], np.float64)
+        pg, po = mixture_pdf_grid(dv[r], dirs, x), mixture_pdf_grid(ov[r], dirs, x)
+        l1.append(np.abs(pg - po).mean() / po.mean())
+        K = dv[r].usedDistributions
+        dw.append(sum(dv[r].pi[:K]) - sum(ov[r].pi[:K]))
+    return np.array(same), np.array(l1), np.array(dw)
+
+
+def test_guiding_at_configuration_3_scale(renderer):
+    """Configuration 3's guiding scale on its plates-and-lights scene at
+    1280x720: GUIDING_SPLITS = 8 (256 regions), region splitting on with a cap
+    of 4096 regions, mixtures of up to 32 lobes, eight recording frames each
+    followed by an EM update on at most 2^22 samples (the first valid ones in
+    slot order; the first update caps at 2^17, fewer than the frame records,
+    so that the cap is exercised).  The FP64 oracle (OracleGuide, same options)
+    refits from the same samples alongside:
+      * after the first update (whose region splitting already adds regions):
+        the same regions, the same lobe counts in every region, densities within 1e-3 (median) / 1e-2 (99th percentile)
+        L1;
+      * after eight updates: the same regions, grown past 256 and within 4096;
+        lobe counts equal in 95% of them, and there densities within 1e-2
+        (median) L1 and weight sums within 1e-3 of the oracle's; more than 16
+        lobes somewhere and never more than 32; every mixture has positive
+        weights, kappa >= 0 and unit means; rtx_guiding_get refuses the wide
+        mixtures, rtx_guiding_get_wide returns them; a guided frame renders
+        finite and keeps the plain mean."""
+    from oracle.binding import OracleGuide
+    hs = rtxamd.HostScene.generate(abi.SCENE_PLATES)
+    ds = renderer.upload(hs)
+    box = ds.bounds()
+    w, h = 1280, 720
+    first_cap, cap = 1 << 17, 1 << 22
+
+    def options(c):
+        return abi.guiding_options(splitRegions=1, maxRegions=4096, maxComponents=32, maxSamplesPerUpdate=c)
+
+    base = dict(maxDepth=12, enableMIS=1)
+    renderer.set_guiding_options(options(first_cap))
+    renderer.guiding_init(box, 8, parallax=True)
+    og = OracleGuide(abi, box, 8, parallax=True, prec=64, options=options(first_cap))
+    dirs = fib_sphere(4000)
+    try:
+        for f in range(8):
+            renderer.render(ds, scenes.params(hs, w, h, 1, updateGuiding=1, useGuiding=int(f > 0), guidingProb=0.5,
+                                              sample_offset=f, flags=abi.RTX_FRAME_NO_GUIDING_UPDATE, **base))
+            dd = renderer.guiding_read_samples(w, h).reshape(-1)
+            valid = dd[dd["flags"] != abi.DIR_INVALID]
+            renderer.guiding_update()
+            og.update(valid)
+            if f == 0:
+                log(f"configuration 3: {len(valid)} valid samples of {len(dd)} slots; the first update fits "
+                    f"{min(len(valid), first_cap)}")
+                assert len(valid) > first_cap
+                gb, gv = renderer.guiding_get_wide()
+                same, l1, dw = compare_guides((gb, gv), og.regions_wide(), dirs)
+                log(f"  first update: {len(same)} of {len(gb)} regions with equal lobe counts; density L1 median "
+                    f"{np.median(l1):.2e} p99 {np.quantile(l1, 0.99):.2e} max {l1.max():.2e}")
+                assert len(same) == len(gb)
+                assert np.median(l1) <= 1e-3 and np.quantile(l1, 0.99) <= 1e-2
+                renderer.set_guiding_options(options(cap))
+                og.set_options(options(cap))
+        gb, gv = renderer.guiding_get_wide()
+        ob, ov = og.regions_wide()
+        ks = np.array([v.usedDistributions for v in gv])
+        same, l1, dw = compare_guides((gb, gv), (ob, ov), dirs)
+        osum = np.array([sum(v.pi[:v.usedDistributions]) for v in ov])
+        log(f"  after 8 updates: {len(gb)} regions, lobes per mixture {ks.min()}..{ks.max()} "
+            f"(> 16 in {(ks > 16).sum()} regions); equal lobe counts in {len(same)}, density L1 median "
+            f"{np.median(l1):.2e} p95 {np.quantile(l1, 0.95):.2e}; weight sums: oracle's within "
+            f"{np.abs(osum - 1).max():.2e} of 1, device's within {np.abs(dw).max():.2e} of the oracle's")
+        assert 256 < len(gb) <= 4096
+        assert len(same) >= 0.95 * len(gb)
+        assert np.median(l1) <= 1e-2 and np.abs(dw).max() <= 1e-3
+        assert ks.max() <= 32 and (ks > 16).any()
+        faults = [(r, f) for r, v in enumerate(gv) if (f := mixture_faults(v))]
+        assert not faults, faults[:4]
+        with pytest.raises(rtxamd.RtxError) as e:
+            renderer.guiding_get()
+        assert e.value.code == abi.RTX_E_UNSUPPORTED
+        # a guided frame with the wide mixtures: finite, the plain image's mean
+        n = 320 * 180 * 8
+        keep = abi.RTX_FRAME_KEEP_PATHS | abi.RTX_FRAME_RESET_ACCUM
+        renderer.render(ds, scenes.params(hs, 320, 180, 8, seed=21, flags=keep, **base))
+        a, _ = renderer.read_paths(n)
+        renderer.render(ds, scenes.params(hs, 320, 180, 8, seed=22, flags=keep, useGuiding=1, guidingProb=0.5,
+                                          **base))
+        b, _ = renderer.read_paths(n)
+        la, lb = a.mean(1).astype(np.float64), b.mean(1).astype(np.float64)
+        se = np.sqrt(la.var() / n + lb.var() / n)
+        log(f"  guided 32-lobe frame: mean {lb.mean():.5f} vs plain {la.mean():.5f} (se {se:.1e})")
+        assert np.isfinite(b).all() and abs(la.mean() - lb.mean()) < 4 * se
+    finally:
+        renderer.set_guiding_options()
+        renderer.guiding_set([], [])
+        ds.close()

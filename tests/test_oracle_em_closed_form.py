@@ -29,7 +29,7 @@ from oracle.binding import lib
 
 L = lib(abi)
 ND = L.orc_em_mix_doubles()
-KMAX = 16
+KMAX = (ND - 4) // 6  # lobe slots of the oracle's mixture record
 DP = C.POINTER(C.c_double)
 
 
