@@ -79,42 +79,73 @@ struct IcQueryParams {
     bool visibility;     // irradianceCachePerformVisibilityCheck
 };
 
+// The bucket of x's cell at one level of the index.
+RTX_DI uint32_t icLevelBucket(const DevIC &ic, float c0, int level, V3 x, uint64_t &key) {
+    const float cs = icCellSize(c0, level);
+    key = icKey(level, icCellCoord(x.x, cs), icCellCoord(x.y, cs), icCellCoord(x.z, cs));
+    return icBucket(key, ic.nb);
+}
+
+// One record's term of the weighted mean (raytrace.irradiance.rint:17-20,
+// raytrace.irradiance.rahit:18-51), when its sphere contains x.
+RTX_DI void icAccept(const DevIC &ic, const IcQueryParams &qp, V3 x, V3 n, const IcItem &item, V3 &sum,
+                     float &total) {
+    V3 oc = x - v3(item.center);
+    if (!(length(oc) <= item.radius)) return;
+    const rtx_cache_data &e = ic.entries[item.entry];
+    V3 ni = v3(e.normal);
+    float w = 1.0f / (length(oc) / e.harmonicR + sqrtf(1.0f - dot(n, ni)));
+    if (isnan(w) || isinf(w)) w = 1000000.0f;
+    if (w <= qp.invA) return;
+    if (qp.visibility && !(-0.001f <= dot(oc, (n + ni) / 2.0f))) return;
+    V3 col = v3(e.color);
+    if (qp.gradients) {
+        float E = length(col);
+        V3 cn = E != 0.0f ? normalize(col) : v3(0.0f);
+        col = cn * (E + dot(cross(ni, n), v3(e.rotGrad)) + dot(oc, v3(e.transGrad)));
+    }
+    sum = sum + w * col;
+    total += w;
+}
+
 // Weighted cache colour at x (normal n); false when no record accepts x.
+// Records are weighed in (level, bucket order) order.  The loads are
+// arranged so that each level costs about one memory latency: the next
+// level's bucket range is fetched while this level's items are read, and the
+// next item while this one is weighed.
 RTX_DI bool icQuery(const DevIC &ic, const IcQueryParams &qp, V3 x, V3 n, V3 &color) {
-    const uint32_t mask = ic.meta[0];
-    if (mask == 0) return false;
+    uint32_t lm = ic.meta[0];
+    if (lm == 0) return false;
     const float c0 = __uint_as_float(ic.meta[2]);
     V3 sum = v3(0.0f);
     float total = 0.0f;
-    for (uint32_t lm = mask; lm != 0; lm &= lm - 1) {
-        const int level = __ffs(lm) - 1;
-        const float cs = icCellSize(c0, level);
-        const uint64_t key = icKey(level, icCellCoord(x.x, cs), icCellCoord(x.y, cs), icCellCoord(x.z, cs));
-        const uint32_t b = icBucket(key, ic.nb);
-        const uint32_t end = ic.bucketStart[b + 1];
-        for (uint32_t it = ic.bucketStart[b]; it < end; ++it) {
-            const IcItem item = ic.items[it];
-            if (item.key != key) continue;
-            const uint32_t i = item.entry;
-            const rtx_sphere &sp = ic.spheres[i];
-            V3 c = v3(sp.center);
-            V3 oc = x - c;
-            if (!(length(oc) <= sp.radius)) continue; // raytrace.irradiance.rint:17-20
-            const rtx_cache_data &e = ic.entries[i];
-            V3 ni = v3(e.normal);
-            float w = 1.0f / (length(oc) / e.harmonicR + sqrtf(1.0f - dot(n, ni)));
-            if (isnan(w) || isinf(w)) w = 1000000.0f;
-            if (w <= qp.invA) continue;
-            if (qp.visibility && !(-0.001f <= dot(oc, (n + ni) / 2.0f))) continue;
-            V3 col = v3(e.color);
-            if (qp.gradients) {
-                float E = length(col);
-                V3 cn = E != 0.0f ? normalize(col) : v3(0.0f);
-                col = cn * (E + dot(cross(ni, n), v3(e.rotGrad)) + dot(oc, v3(e.transGrad)));
-            }
-            sum = sum + w * col;
-            total += w;
+    uint64_t key;
+    uint32_t b = icLevelBucket(ic, c0, __ffs(lm) - 1, x, key);
+    uint32_t it = ic.bucketStart[b], end = ic.bucketStart[b + 1];
+    for (;;) {
+        lm &= lm - 1;
+        uint64_t nkey = 0;
+        uint32_t nit = 0, nend = 0;
+        if (lm != 0) {
+            const uint32_t nbk = icLevelBucket(ic, c0, __ffs(lm) - 1, x, nkey);
+            nit = ic.bucketStart[nbk];
+            nend = ic.bucketStart[nbk + 1];
         }
+        if (it < end) {
+            IcItem cur = ic.items[it];
+            for (;;) {
+                const bool more = ++it < end;
+                IcItem nxt;
+                if (more) nxt = ic.items[it];
+                if (cur.key == key) icAccept(ic, qp, x, n, cur, sum, total);
+                if (!more) break;
+                cur = nxt;
+            }
+        }
+        if (lm == 0) break;
+        key = nkey;
+        it = nit;
+        end = nend;
     }
     if (total > 0.0f) {
         color = sum / total;

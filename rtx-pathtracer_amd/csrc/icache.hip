@@ -383,6 +383,10 @@ template <bool SCATTER> __global__ __launch_bounds__(256) void kIcIndexFill(IcBu
                         atomicAdd(&ic.bucketCount[b], 1u);
                     } else {
                         IcItem it;
+                        it.center[0] = sp.center[0];
+                        it.center[1] = sp.center[1];
+                        it.center[2] = sp.center[2];
+                        it.radius = sp.radius;
                         it.entry = i;
                         it.pad = 0;
                         it.key = key;
