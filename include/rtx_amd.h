@@ -412,7 +412,11 @@ typedef struct rtx_tuning {
     uint32_t batchPaths;      /* paths in flight per batch when rtx_frame_params.batchPaths is 0 (2^22) */
     int32_t ldsStackEntries;  /* traversal stack entries per thread held in LDS: 12, 16, 20 or 24
                                  (20); deeper entries spill to scratch */
-    int32_t _reserved[3];
+    int32_t cacheUpkeep;      /* irradiance-cache frames: 1 = the update jobs' sub-paths run in
+                                 the frame's own bounces, beside its camera paths, where the
+                                 path buffers hold both (default); 2 = every job after the
+                                 frame.  Results are the same either way. */
+    int32_t _reserved[2];
 } rtx_tuning;
 
 /* Autotuner (no counterpart in the reference).  Renders `probe` (as given,

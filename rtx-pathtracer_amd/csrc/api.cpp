@@ -363,6 +363,7 @@ int rtx_ctx_set_tuning(rtx_ctx *ctx, const rtx_tuning *t) try {
     if (t->shadeBlocksPerCU) v.shadeBlocksPerCU = t->shadeBlocksPerCU;
     if (t->batchPaths) v.batchPaths = t->batchPaths;
     if (t->ldsStackEntries) v.ldsStackEntries = t->ldsStackEntries;
+    if (t->cacheUpkeep) v.cacheUpkeep = t->cacheUpkeep;
     if (v.traversalKernel != 1)
         return fail(ctx, RTX_E_INVALID, "rtx_tuning.traversalKernel must be 1 (persistent lane-refill traversal)");
     if (v.traceBlocksPerCU < 1 || v.traceBlocksPerCU > 64 || v.shadeBlocksPerCU < 1 || v.shadeBlocksPerCU > 64)
@@ -371,6 +372,7 @@ int rtx_ctx_set_tuning(rtx_ctx *ctx, const rtx_tuning *t) try {
         return fail(ctx, RTX_E_INVALID, "rtx_tuning.batchPaths must lie in [1024, 2^26]");
     if (v.ldsStackEntries != 12 && v.ldsStackEntries != 16 && v.ldsStackEntries != 20 && v.ldsStackEntries != 24)
         return fail(ctx, RTX_E_INVALID, "rtx_tuning.ldsStackEntries must be 12, 16, 20 or 24");
+    if (v.cacheUpkeep != 1 && v.cacheUpkeep != 2) return fail(ctx, RTX_E_INVALID, "rtx_tuning.cacheUpkeep must be 1 or 2");
     ctx->tuning = v;
     return RTX_OK;
 } catch (...) {
@@ -634,6 +636,38 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) try
     if (e == hipSuccess)
         e = upload(&ftD, d->randomTriIndex, d->randomTriIndex ? (size_t)d->randomTriRows * RTX_SIZE_TRI_RANDOM : 0);
     if (e == hipSuccess) e = upload(&spD, d->spheres, d->sphereCount);
+    // the area lights' face-table entries with their vertices and colour
+    // (validateScene checked rows, instances and face indices)
+    float4 *emitD = nullptr;
+    rtx_instance *linstD = nullptr;
+    const size_t emitRows = d->randomTriIndex ? std::min<size_t>(d->randomTriRows, d->lightCount) : 0;
+    if (e == hipSuccess && emitRows && emitRows * RTX_SIZE_TRI_RANDOM * 6 * sizeof(float4) <= kEmitTableBytes) {
+        std::vector<float4> emit(emitRows * RTX_SIZE_TRI_RANDOM * 6, make_float4(0, 0, 0, 0));
+        std::vector<rtx_instance> linst(emitRows);
+        std::memset(linst.data(), 0, linst.size() * sizeof(rtx_instance));
+        for (size_t r = 0; r < emitRows; ++r) {
+            const rtx_light &L = d->lights[r];
+            if (L.type != RTX_LIGHT_AREA) continue;
+            const rtx_instance &in = d->instances[L.instanceIndex];
+            linst[r] = in;
+            const rtx_model_desc &md = d->models[in.modelIndex];
+            for (size_t k = 0; k < RTX_SIZE_TRI_RANDOM; ++k) {
+                const size_t f = (size_t)d->randomTriIndex[r * RTX_SIZE_TRI_RANDOM + k].index;
+                const rtx_vertex &v0 = md.vertices[md.indices[3 * f]], &v1 = md.vertices[md.indices[3 * f + 1]],
+                                 &v2 = md.vertices[md.indices[3 * f + 2]];
+                const float *lc = d->materials[v0.materialIndex].lightColor;
+                float4 *o = &emit[6 * (r * RTX_SIZE_TRI_RANDOM + k)];
+                o[0] = make_float4(v0.pos[0], v0.pos[1], v0.pos[2], lc[0]);
+                o[1] = make_float4(v1.pos[0], v1.pos[1], v1.pos[2], lc[1]);
+                o[2] = make_float4(v2.pos[0], v2.pos[1], v2.pos[2], lc[2]);
+                o[3] = make_float4(v0.normal[0], v0.normal[1], v0.normal[2], 0.0f);
+                o[4] = make_float4(v1.normal[0], v1.normal[1], v1.normal[2], 0.0f);
+                o[5] = make_float4(v2.normal[0], v2.normal[1], v2.normal[2], 0.0f);
+            }
+        }
+        e = upload(&emitD, emit.data(), emit.size());
+        if (e == hipSuccess) e = upload(&linstD, linst.data(), linst.size());
+    }
     // textures: the environment map as float4 texels, material textures packed
     // one after another, the sRGB decoding table of the R8G8B8A8_SRGB images
     float4 *envD = nullptr;
@@ -673,6 +707,8 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) try
     S.lights = lightD;
     S.lightIndex = liD;
     S.triTable = ftD;
+    S.emitTris = emitD;
+    S.lightInst = linstD;
     S.spheres = spD;
     S.numTris = (uint32_t)numTris;
     S.numSpheres = d->sphereCount;
@@ -838,6 +874,20 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     // then runs in one batch sized on the device
     if (icFrames)
         bufPaths = std::max<size_t>(bufPaths, std::min<size_t>((size_t)ctx->ic.jobCap * kIcStrata, (size_t)1 << 26));
+    // With the cache on, a frame's update jobs are known once its camera
+    // paths are generated (the elections are the paths' first draws and the
+    // records they update are the frame-start cache's), so their sub-paths
+    // can run in the frame's own bounces, beside the camera paths, as a second
+    // class of paths with ids from `batch` on; only the creation jobs, which
+    // need the frame's candidates, run after it.  The reads are the
+    // frame-start cache's either way and every job keeps its index, streams
+    // and place in the commit, so the cache and the image are the same as
+    // with every job after the frame.
+    const bool keepFlag = (p->flags & RTX_FRAME_KEEP_PATHS) != 0;
+    const size_t mergedPaths = (size_t)batch + (size_t)ctx->ic.maxEntries * kIcStrata;
+    const bool mergeUpkeep = useIC && ctx->tuning.cacheUpkeep == 1 && batch == npix && !keepFlag && !segments &&
+                             !record && !nrc && mergedPaths <= ((size_t)1 << 26);
+    if (mergeUpkeep) bufPaths = std::max(bufPaths, mergedPaths);
     int rc = ensureBatchBuffers(ctx, bufPaths, neeCap);
     if (rc != RTX_OK) return rc;
     if (segments && ctx->segCap < bufPaths) {
@@ -962,8 +1012,10 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.rrStartDepth = p->rrStartDepth;
     fc.numNEE = numNEE;
     fc.neeStride = numNEE;
+    if (mergeUpkeep) fc.neeStride = neeCap; // both classes of paths in one set of NEE slots
     fc.countVisits = (p->flags & RTX_FRAME_COUNT_VISITS) ? 1 : 0;
-    // developer aid: RTX_DEBUG_PATH=<path index within this call> prints that path's bounces
+    // developer aid (a build with -DRTX_DEVICE_DEBUG): RTX_DEBUG_PATH=<path index
+    // within this call> prints that path's bounces
     fc.debugPath = ~0ull;
     if (const char *dbg = std::getenv("RTX_DEBUG_PATH")) fc.debugPath = std::strtoull(dbg, nullptr, 10);
     // camera paths: raytrace(..., maxDepth, maxFollowDiscrete, !enableNEE, true, enableNEE, numNEE,
@@ -1002,7 +1054,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fcIC.splitOnFirst = 0;
     fcIC.nrc = 0;
     fcIC.numNEE = icFrames ? (uint32_t)p->pc.irradianceNumNEE : 1u;
-    fcIC.neeStride = fcIC.numNEE;
+    fcIC.neeStride = mergeUpkeep ? neeCap : fcIC.numNEE;
     fcIC.debugPath = ~0ull;
     if (icFrames) CHECK_HIP(ctx, hipMemcpyAsync(ctx->fcIC, &fcIC, sizeof(fcIC), hipMemcpyHostToDevice, ctx->stream));
     CHECK_HIP(ctx, hipMemsetAsync(ctx->cnt, 0, sizeof(Counters), ctx->stream));
@@ -1051,7 +1103,10 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             q->seedDiv = q == &ctx->qs || q == &ctx->qp ? nee : 1u;
         }
     };
-    auto bounces = [&](const FrameConst *fcd, BatchDesc b, int iters, bool multi) -> int {
+    // developer aid: RTX_LOG_BOUNCES=1 prints the queue lengths of every bounce
+    static const bool logBounces = std::getenv("RTX_LOG_BOUNCES") != nullptr;
+    auto bounces = [&](const FrameConst *fcd, BatchDesc b, int iters, bool multi,
+                       const PathClass2 *second = nullptr) -> int {
         int qsel = 0;
         const int cap = multi ? (int)(samplesPerFrame + kMaxSplits + 1) * (iters + 1) : iters + 1;
         for (int it = 0;; ++it) {
@@ -1060,6 +1115,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
                 const int j = (it - kPollLead) % kPollSlots;
                 CHECK_HIP(ctx, hipEventSynchronize(ctx->pollEv[j]));
                 const uint32_t *c = ctx->pollRing + 4 * j;
+                if (logBounces) std::fprintf(stderr, "[rtx] bounce %d: extend %u probe %u shadow %u done %u\n",
+                                             it - kPollLead, c[0], c[1], c[2], c[3]);
                 // c = lengths bounce j started from: extend, probe, shadow, done.
                 // With no extend rays, bounce j shades nothing, so bounce j + 1
                 // starts empty unless kFinish restarts segments of j's done paths.
@@ -1076,11 +1133,11 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             CHECK_HIP(ctx, hipEventRecord(ctx->pollEv[it % kPollSlots], s));
             timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, qsel, shadeBlocks); });
             timed(kClsShade, [&] {
-                launchFinish(s, fcd, b, ctx->pb, ctx->done, ctx->cnt, ctx->q[qsel ^ 1], qsel, shadeBlocks);
+                launchFinish(s, fcd, b, ctx->pb, ctx->done, ctx->cnt, ctx->q[qsel ^ 1], qsel, shadeBlocks, second);
             });
             timed(kClsShade, [&] {
                 launchShade(s, S, icv, ctx->guide, fcd, b, ctx->pb, ctx->q[qsel], ctx->q[qsel ^ 1], ctx->qs, ctx->qp,
-                            ctx->done, ctx->cnt, qsel, shadeBlocks);
+                            ctx->done, ctx->cnt, qsel, shadeBlocks, second);
             });
             qsel ^= 1;
         }
@@ -1101,8 +1158,27 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.npix = (uint32_t)npix;
             ctx->pb.dirFirstPixel = (uint32_t)(first % npix);
             timed(kClsShade, [&] { launchGenerate(s, ctx->fc, b, ctx->pb, ctx->q[0], ctx->cnt, 0, ctx->ic.frame); });
-            setRaySeeds(fc.numNEE);
-            int rcb = bounces(ctx->fc, b, maxIter, segments);
+            int rcb = RTX_OK;
+            if (mergeUpkeep) {
+                // the update jobs, planned from the elections just drawn; their
+                // sub-paths join the camera paths' first extend queue
+                const uint32_t frameSeed = pcg32First(p->seed, f0 / npix + p->sampleOffset);
+                const uint32_t np = ctx->ic.maxEntries * kIcStrata;
+                timed(kClsShade, [&] { launchIcPlan(s, ctx->ic, kIcPlanUpdates); });
+                timed(kClsShade, [&] {
+                    launchGenerateIC(s, ctx->ic, frameSeed, 0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, kIcJobsUpdates,
+                                     (uint32_t)batch, b.numPaths);
+                });
+                PathClass2 sub{ctx->fcIC, BatchDesc{}, (uint32_t)batch};
+                sub.b.numPaths = np;
+                sub.b.npix = np;
+                sub.b.pathBase = (uint32_t)batch;
+                setRaySeeds(neeCap);
+                rcb = bounces(ctx->fc, b, std::max(maxIter, maxIterIC), false, &sub);
+            } else {
+                setRaySeeds(fc.numNEE);
+                rcb = bounces(ctx->fc, b, maxIter, segments);
+            }
             if (rcb != RTX_OK) return rcb;
             if (nrc) // the paths that ended into the learned cache: throughput x cache
                 timed(kClsShade, [&] {
@@ -1143,6 +1219,31 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         // the frame's cache upkeep
         const uint64_t sample = f0 / npix + p->sampleOffset;
         const uint32_t frameSeed = pcg32First(p->seed, sample);
+        if (mergeUpkeep) {
+            // the update jobs' records from their sub-paths (ids from batch on),
+            // then the creation jobs behind them, sized on the device
+            const uint32_t np = ctx->ic.maxEntries * kIcStrata;
+            timed(kClsShade, [&] {
+                launchIcReduce(s, ctx->ic, frameSeed, 0, ctx->ic.maxEntries, ctx->pb, kIcJobsUpdates, (uint32_t)batch);
+            });
+            timed(kClsShade, [&] { launchIcPlan(s, ctx->ic, kIcPlanCreations); });
+            BatchDesc b{};
+            b.numPaths = np;
+            b.npix = np;
+            timed(kClsShade, [&] {
+                launchGenerateIC(s, ctx->ic, frameSeed, 0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, kIcJobsCreations);
+            });
+            setRaySeeds(neeCap);
+            int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
+            if (rcs != RTX_OK) return rcs;
+            timed(kClsShade, [&] {
+                launchIcReduce(s, ctx->ic, frameSeed, 0, ctx->ic.maxEntries, ctx->pb, kIcJobsCreations);
+            });
+            timed(kClsShade, [&] { launchIcCommit(s, ctx->ic, ctx->fcIC); });
+            timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
+            CHECK_HIP(ctx, hipGetLastError());
+            continue;
+        }
         timed(kClsShade, [&] { launchIcPlan(s, ctx->ic); });
         const bool keepSub = keep && fEnd == total;
         const uint32_t jobsPerBatch = (uint32_t)std::max<size_t>(1, ctx->cap / kIcStrata);
@@ -1154,11 +1255,12 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.numPaths = np;
             b.npix = np;
             timed(kClsShade,
-                  [&] { launchGenerateIC(s, ctx->ic, frameSeed, 0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, true); });
+                  [&] { launchGenerateIC(s, ctx->ic, frameSeed, 0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, kIcJobsFromFirst); });
             setRaySeeds(fcIC.numNEE);
             int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
             if (rcs != RTX_OK) return rcs;
-            timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, 0, ctx->ic.jobCap, ctx->pb, true); });
+            timed(kClsShade,
+                  [&] { launchIcReduce(s, ctx->ic, frameSeed, 0, ctx->ic.jobCap, ctx->pb, kIcJobsFromFirst); });
             timed(kClsShade, [&] { launchIcCommit(s, ctx->ic, ctx->fcIC); });
             timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
             CHECK_HIP(ctx, hipGetLastError());
@@ -1191,11 +1293,11 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             b.numPaths = np;
             b.npix = np;
             timed(kClsShade,
-                  [&] { launchGenerateIC(s, ctx->ic, frameSeed, j0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, false); });
+                  [&] { launchGenerateIC(s, ctx->ic, frameSeed, j0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, kIcJobsHost); });
             setRaySeeds(fcIC.numNEE);
             int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
             if (rcs != RTX_OK) return rcs;
-            timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, j0, nj, ctx->pb, false); });
+            timed(kClsShade, [&] { launchIcReduce(s, ctx->ic, frameSeed, j0, nj, ctx->pb, kIcJobsHost); });
             if (keepSub) {
                 CHECK_HIP(ctx, hipMemcpyAsync(ctx->keepSub + (size_t)j0 * kIcStrata, ctx->pb.out, np * sizeof(float4),
                                               hipMemcpyDeviceToDevice, s));

@@ -66,7 +66,7 @@ struct rtx_ctx {
     size_t eventsUsed = 0;
     hipEvent_t tStart = nullptr, tStop = nullptr;
     int cuCount = 256;
-    rtx_tuning tuning{1, 14, 8, 1u << 22, 20, {0, 0, 0}};
+    rtx_tuning tuning{1, 14, 8, 1u << 22, 20, 1, {0, 0}};
     // irradiance cache (rtx_ic_create)
     IcBuffers ic{};
     std::vector<void *> icAllocs;

@@ -56,6 +56,15 @@ RTX_DI int icLevel(float c0, float r) {
     return level;
 }
 
+// The jobs [j0, jEnd) of an IcJobRange other than kIcJobsHost.
+RTX_DI void icJobRange(const IcBuffers &ic, int range, uint32_t firstJob, uint32_t &j0, uint32_t &jEnd) {
+    switch (range) {
+    case kIcJobsUpdates: j0 = 0, jEnd = ic.frame[3]; break;
+    case kIcJobsCreations: j0 = ic.frame[3], jEnd = ic.frame[4]; break;
+    default: j0 = firstJob, jEnd = ic.frame[4]; break;
+    }
+}
+
 // raytrace.rgen:806-813
 RTX_DI bool isICCapable(int type, bool onGlossy) {
     if (type == RTX_MAT_DIFFUSE || type == RTX_MAT_LIGHT) return true;

@@ -417,25 +417,37 @@ RTX_DI float sampleLights(const DevScene &S, Rng &r, V3 origin, V3 normal, V3 &l
         return light.sampleProb * lightDist * lightDist / (cosThetaLight * light.area);
     }
     // area light
-    const rtx_instance &in = S.instances[light.instanceIndex];
-    int model = in.modelIndex;
     int iRandomTri = r.integer(RTX_SIZE_TRI_RANDOM - 1);
-    const rtx_face_sample ts = S.triTable[(size_t)iLight * RTX_SIZE_TRI_RANDOM + iRandomTri];
-    const uint32_t *idx = S.indices + S.indexBase[model] + 3 * ts.index;
-    const rtx_vertex *vb = S.vertices + S.vertexBase[model];
-    const rtx_vertex &v0 = vb[idx[0]], &v1 = vb[idx[1]], &v2 = vb[idx[2]];
     float rx = r.next();
     float ry = r.next();
     float sqrtx = sqrtf(rx);
     float b0 = 1.0f - sqrtx, b1 = sqrtx * (1.0f - ry), b2 = ry * sqrtx;
-    V3 pos = v3(v0.pos) * b0 + v3(v1.pos) * b1 + v3(v2.pos) * b2;
-    V3 nrm = v3(v0.normal) * b0 + v3(v1.normal) * b1 + v3(v2.normal) * b2;
-    pos = xform(in.transform, pos, 1.0f);
-    nrm = normalize(xform(in.normalTransform, nrm, 0.0f));
+    V3 pos, nrm;
+    if (S.emitTris) { // the face table's entry holds the face (one load level instead of five)
+        const float4 *et = S.emitTris + 6 * ((size_t)iLight * RTX_SIZE_TRI_RANDOM + iRandomTri);
+        const rtx_instance &in = S.lightInst[iLight];
+        const float4 p0 = et[0], p1 = et[1], p2 = et[2];
+        pos = v3(p0) * b0 + v3(p1) * b1 + v3(p2) * b2;
+        nrm = v3(et[3]) * b0 + v3(et[4]) * b1 + v3(et[5]) * b2;
+        pos = xform(in.transform, pos, 1.0f);
+        nrm = normalize(xform(in.normalTransform, nrm, 0.0f));
+        lightColor = v3(p0.w, p1.w, p2.w);
+    } else {
+        const rtx_instance &in = S.instances[light.instanceIndex];
+        int model = in.modelIndex;
+        const rtx_face_sample ts = S.triTable[(size_t)iLight * RTX_SIZE_TRI_RANDOM + iRandomTri];
+        const uint32_t *idx = S.indices + S.indexBase[model] + 3 * ts.index;
+        const rtx_vertex *vb = S.vertices + S.vertexBase[model];
+        const rtx_vertex &v0 = vb[idx[0]], &v1 = vb[idx[1]], &v2 = vb[idx[2]];
+        pos = v3(v0.pos) * b0 + v3(v1.pos) * b1 + v3(v2.pos) * b2;
+        nrm = v3(v0.normal) * b0 + v3(v1.normal) * b1 + v3(v2.normal) * b2;
+        pos = xform(in.transform, pos, 1.0f);
+        nrm = normalize(xform(in.normalTransform, nrm, 0.0f));
+        lightColor = v3(S.materials[v0.materialIndex].lightColor);
+    }
     V3 toLight = pos - origin;
     lightDist = length(toLight);
     lightDir = toLight / lightDist;
-    lightColor = v3(S.materials[v0.materialIndex].lightColor);
     float cosThetaLight = dot(-lightDir, nrm);
     if (cosThetaLight < 0.0f) cosThetaLight = -cosThetaLight;
     return light.sampleProb * lightDist * lightDist / cosThetaLight / light.area;

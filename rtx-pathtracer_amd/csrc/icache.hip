@@ -114,10 +114,14 @@ __global__ __launch_bounds__(256) void kIcCandScatter(PathBuffers pb, uint32_t n
 
 // The jobs of the frame: update elections served in order by the header's
 // nextUpdateSlot rule (updateIrradianceCache, raytrace.rgen:1334-1345; at
-// most maxEntries update jobs a frame), then the kept candidates.
-__global__ __launch_bounds__(256) void kIcPlan(IcBuffers ic) {
+// most maxEntries update jobs a frame), then the kept candidates.  `part`
+// (IcPlanPart) plans both, or only the updates (frame[3] = frame[4] = their
+// count), or only the creations behind the frame[3] updates planned before.
+__global__ __launch_bounds__(256) void kIcPlan(IcBuffers ic, int part) {
     __shared__ uint32_t numUpd;
-    if (threadIdx.x == 0) {
+    if (part == kIcPlanCreations) {
+        if (threadIdx.x == 0) numUpd = ic.frame[3];
+    } else if (threadIdx.x == 0) {
         const uint32_t elections = ic.frame[0];
         const uint32_t count = min(ic.hdr->nextCacheSlot, ic.hdr->maxCaches);
         uint32_t u = 0;
@@ -135,8 +139,10 @@ __global__ __launch_bounds__(256) void kIcPlan(IcBuffers ic) {
             ++u;
         }
         numUpd = u;
+        if (part == kIcPlanUpdates) ic.frame[3] = ic.frame[4] = u;
     }
     __syncthreads();
+    if (part == kIcPlanUpdates) return;
     const uint32_t u = numUpd;
     const uint32_t c = min(ic.frame[1], ic.frame[2]);
     for (uint32_t i = threadIdx.x; i < c; i += 256) {
@@ -158,11 +164,14 @@ RTX_DI V3 sphericalToCartesian(float theta, float phi) { // shaders/transform.gl
 // sub-path results (radiance, first-hit distance in w) of each job of the
 // sub-batch, in the reference's loop order.
 __global__ __launch_bounds__(64) void kIcReduce(IcBuffers ic, uint32_t frameSeed, uint32_t firstJob,
-                                                uint32_t numJobs, PathBuffers pb, int devCount) {
-    // devCount: the frame's job count in device memory (ic.frame[4]) bounds numJobs
-    if (devCount) {
-        const uint32_t all = ic.frame[4];
-        numJobs = min(numJobs, all > firstJob ? all - firstJob : 0u);
+                                                uint32_t numJobs, PathBuffers pb, int jobRange, uint32_t outBase) {
+    // jobRange (IcJobRange): the jobs, from device memory unless kIcJobsHost;
+    // job firstJob + t's sub-path results are at pb.out[outBase + 200 t ..]
+    if (jobRange != kIcJobsHost) {
+        uint32_t j0, jEnd;
+        icJobRange(ic, jobRange, firstJob, j0, jEnd);
+        firstJob = j0;
+        numJobs = min(numJobs, jEnd > j0 ? jEnd - j0 : 0u);
     }
     // One 64-lane block per job: the lanes compute every stratum's terms
     // (trigonometry, neighbour differences) in parallel, then lane 0 adds
@@ -174,7 +183,7 @@ __global__ __launch_bounds__(64) void kIcReduce(IcBuffers ic, uint32_t frameSeed
     if (t >= numJobs) return; // uniform per block
     const uint32_t job = firstJob + t;
     const V3 normal = v3(ic.jobs[2 * job + 1]);
-    const float4 *sub = pb.out + (size_t)t * kIcStrata;
+    const float4 *sub = pb.out + outBase + (size_t)t * kIcStrata;
     for (int q = (int)threadIdx.x; q < kIcStrata; q += 64) smp[q] = sub[q];
     __syncthreads();
     for (int q = (int)threadIdx.x; q < kIcStrata; q += 64) {
@@ -439,14 +448,14 @@ void launchIcCollect(hipStream_t s, IcBuffers ic, PathBuffers pb, uint32_t numPa
     hipLaunchKernelGGL(kIcCandScan, dim3(1), dim3(1024), 0, s, ic.blockSums, nblk, ic.frame);
     hipLaunchKernelGGL(kIcCandScatter, dim3(nblk), dim3(256), 0, s, pb, numPaths, ic.blockSums, ic);
 }
-void launchIcPlan(hipStream_t s, IcBuffers ic) {
-    hipLaunchKernelGGL(kIcPlan, dim3(1), dim3(256), 0, s, ic);
+void launchIcPlan(hipStream_t s, IcBuffers ic, int part) {
+    hipLaunchKernelGGL(kIcPlan, dim3(1), dim3(256), 0, s, ic, part);
 }
 void launchIcReduce(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numJobs,
-                    PathBuffers pb, bool devCount) {
+                    PathBuffers pb, int jobRange, uint32_t outBase) {
     if (numJobs == 0) return;
-    hipLaunchKernelGGL(kIcReduce, dim3(numJobs), dim3(64), 0, s, ic, frameSeed, firstJob, numJobs, pb,
-                       devCount ? 1 : 0);
+    hipLaunchKernelGGL(kIcReduce, dim3(numJobs), dim3(64), 0, s, ic, frameSeed, firstJob, numJobs, pb, jobRange,
+                       outBase);
 }
 void launchIcCommit(hipStream_t s, IcBuffers ic, const FrameConst *fc) {
     hipLaunchKernelGGL(kIcCommit, dim3(1), dim3(64), 0, s, ic, fc);

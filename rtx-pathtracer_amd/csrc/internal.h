@@ -27,6 +27,13 @@ struct DevScene {
     const rtx_light *lights;
     const int32_t *lightIndex;
     const rtx_face_sample *triTable;
+    // Area-light sampling without the table -> index -> vertex -> material
+    // chain: per face-table entry (light row x RTX_SIZE_TRI_RANDOM) the face's
+    // three positions (w: light colour r, g, b) and three normals, model
+    // space, and per light row the instance it transforms them by.  Null when
+    // the table would exceed kEmitTableBytes (then the chain is walked).
+    const float4 *emitTris;
+    const rtx_instance *lightInst;
     const rtx_sphere *spheres;
     uint32_t numTris;
     uint32_t numSpheres;
@@ -43,6 +50,8 @@ struct DevScene {
     uint32_t alphaTris;         // triangles whose material has a diffuse texture (any-hit alpha test)
     const float *srgbToLinear;  // 256 entries
 };
+
+constexpr size_t kEmitTableBytes = size_t(256) << 20;
 
 // Bit of a triangle's id (tris[3k].w) marking it for the any-hit alpha test
 // (raytrace.rahit:21-47); ids stay below 2^27.
@@ -327,19 +336,36 @@ void launchQueryP(hipStream_t s, const DevScene &S, const rtx_ray *rays, uint32_
                   uint32_t *fetch, int blocks);
 void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, Queue q, PathBuffers pb,
                         Counters *cnt, int qsel, int blocks);
+// A second class of paths in the same wavefront: ids >= split use (fc, b).
+struct PathClass2 {
+    const FrameConst *fc;
+    BatchDesc b;
+    uint32_t split;
+};
 void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const DevGuiding &g, const FrameConst *fc,
                  BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt,
-                 int qsel, int blocks);
-// devCount: numPaths (numJobs) is an upper bound and the frame's job count in
-// device memory (ic.frame[4]) decides, so no host round trip sizes the launch
+                 int qsel, int blocks, const PathClass2 *second = nullptr);
+// Which jobs of the frame's list a sub-path launch or a reduction covers:
+// [firstJob, firstJob + count) as the host says, or from device memory (the
+// plan's counts, ic.frame[3] updates and ic.frame[4] jobs in all), so that no
+// host round trip sizes the launch; then the host's count is an upper bound.
+enum IcJobRange { kIcJobsHost = 0, kIcJobsFromFirst = 1, kIcJobsUpdates = 2, kIcJobsCreations = 3 };
+// (icJobRange, dev_icache.h, reads the range on the device.)
+// pidBase / qBase: the sub-paths' first path id and extend-queue entry (after
+// other paths already queued, which keep their entries)
 void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numPaths,
-                      PathBuffers pb, Queue q0, Counters *cnt, int qsel, bool devCount);
+                      PathBuffers pb, Queue q0, Counters *cnt, int qsel, int jobRange, uint32_t pidBase = 0,
+                      uint32_t qBase = 0);
 // irradiance-cache frame bookkeeping (icache.hip)
 void launchIcFrameBegin(hipStream_t s, IcBuffers ic);
 void launchIcCollect(hipStream_t s, IcBuffers ic, PathBuffers pb, uint32_t numPaths);
-void launchIcPlan(hipStream_t s, IcBuffers ic);
+// The frame's job list (kIcPlan): update elections, then kept candidates.
+// Planned in one go after the frame, or in two parts: the updates (known
+// once the camera paths are generated) and then the creations.
+enum IcPlanPart { kIcPlanAll = 0, kIcPlanUpdates = 1, kIcPlanCreations = 2 };
+void launchIcPlan(hipStream_t s, IcBuffers ic, int part = kIcPlanAll);
 void launchIcReduce(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numJobs,
-                    PathBuffers pb, bool devCount);
+                    PathBuffers pb, int jobRange, uint32_t outBase = 0);
 void launchIcCommit(hipStream_t s, IcBuffers ic, const FrameConst *fc);
 void launchIcIndex(hipStream_t s, IcBuffers ic, float c0);
 void launchIcInit(hipStream_t s, IcBuffers ic);
@@ -347,7 +373,7 @@ void launchIcInit(hipStream_t s, IcBuffers ic);
 // with FrameConst::segments a path with camera samples left or splits queued
 // goes on in qnext (count extend[qsel ^ 1]) instead.
 void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, const int32_t *done,
-                  Counters *cnt, Queue qnext, int qsel, int blocks);
+                  Counters *cnt, Queue qnext, int qsel, int blocks, const PathClass2 *second = nullptr);
 // accum may be NULL (irradiance-cache prepare frames): then only the kept paths are written
 // estimate != NULL (pc.storeEstimate): each pixel's last frame result is stored there too
 void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,

@@ -42,6 +42,22 @@ RTX_DI int waveAppend(bool pred, uint32_t *counter) {
     return pred ? (int)(base + (uint32_t)__popcll(below)) : -1;
 }
 
+// Two appends whose atomics are in flight together: the slots of `a` lanes
+// in queue ca and of `b` lanes in queue cb (-1 for the others).
+RTX_DI void waveAppend2(bool a, uint32_t *ca, bool b, uint32_t *cb, int &ia, int &ib) {
+    const uint64_t ma = __ballot(a), mb = __ballot(b);
+    const int lane = (int)__lane_id();
+    const int la = ma ? __ffsll((unsigned long long)ma) - 1 : 0, lb = mb ? __ffsll((unsigned long long)mb) - 1 : 0;
+    uint32_t basea = 0, baseb = 0;
+    if (ma && lane == la) basea = atomicAdd(ca, (uint32_t)__popcll(ma));
+    if (mb && lane == lb) baseb = atomicAdd(cb, (uint32_t)__popcll(mb));
+    basea = __shfl(basea, la);
+    baseb = __shfl(baseb, lb);
+    const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    ia = a ? (int)(basea + (uint32_t)__popcll(ma & below)) : -1;
+    ib = b ? (int)(baseb + (uint32_t)__popcll(mb & below)) : -1;
+}
+
 // Runs body(k) once for every distinct value k of `key` among the lanes
 // with `active`, with exactly the lanes holding that key active and k
 // wave-uniform (read with readlane, so per-key data are scalar loads).  The
@@ -257,16 +273,20 @@ __global__ __launch_bounds__(256) void kInstallSlot(SlotDesc sd, SlotDesc *slots
 // addNextDirectLights off.
 __global__ __launch_bounds__(256) void kGenerateIC(IcBuffers ic, uint32_t frameSeed, uint32_t firstJob,
                                                    uint32_t numPaths, PathBuffers pb, Queue q0, Counters *cnt,
-                                                   int qsel, int devCount) {
-    // devCount: the frame's job count in device memory (ic.frame[4]) bounds numPaths
-    if (devCount) {
-        const uint32_t all = ic.frame[4];
-        numPaths = min(numPaths, (all > firstJob ? all - firstJob : 0u) * (uint32_t)kIcStrata);
+                                                   int qsel, int jobRange, uint32_t pidBase, uint32_t qBase) {
+    // jobRange: which jobs of the frame's list, from device memory when not
+    // kIcJobsHost (see IcJobRange); numPaths bounds the paths either way
+    uint32_t j0 = firstJob;
+    if (jobRange != kIcJobsHost) {
+        uint32_t jEnd;
+        icJobRange(ic, jobRange, firstJob, j0, jEnd);
+        numPaths = min(numPaths, (jEnd > j0 ? jEnd - j0 : 0u) * (uint32_t)kIcStrata);
     }
-    uint32_t pid = blockIdx.x * 256 + threadIdx.x;
-    if (pid == 0) cnt->extend[qsel] = numPaths;
-    if (pid >= numPaths) return;
-    const uint32_t job = firstJob + pid / kIcStrata, st = pid % kIcStrata;
+    const uint32_t local = blockIdx.x * 256 + threadIdx.x;
+    if (local == 0) cnt->extend[qsel] = qBase + numPaths; // after the qBase entries already queued
+    if (local >= numPaths) return;
+    const uint32_t pid = pidBase + local, qi = qBase + local;
+    const uint32_t job = j0 + local / kIcStrata, st = local % kIcStrata;
     const uint32_t k = st / kIcStrataTheta, j = st % kIcStrataTheta;
     float4 o4 = ic.jobs[2 * job], n4 = ic.jobs[2 * job + 1];
     V3 n = v3(n4);
@@ -274,11 +294,11 @@ __global__ __launch_bounds__(256) void kGenerateIC(IcBuffers ic, uint32_t frameS
     float phi = 2.0f * kPi * ((float)k + col.next()) / (float)kIcStrataPhi;
     Rng rng{tea((job * kIcStrataPhi + k) * kIcStrataTheta + j, frameSeed ^ kIcStratumKey)};
     float theta = asinf(sqrtf(((float)j + rng.next()) / (float)kIcStrataTheta));
-    V3 local = v3(sinf(theta) * cosf(phi), sinf(theta) * sinf(phi), cosf(theta));
-    V3 d = toWorld(local, n);
-    q0.id[pid] = (int)pid;
-    q0.org[pid] = f4(v3(o4), kTMin);
-    q0.dir[pid] = f4(d, kTMax);
+    V3 local3 = v3(sinf(theta) * cosf(phi), sinf(theta) * sinf(phi), cosf(theta));
+    V3 d = toWorld(local3, n);
+    q0.id[qi] = (int)pid;
+    q0.org[qi] = f4(v3(o4), kTMin);
+    q0.dir[qi] = f4(d, kTMax);
     pb.rng[pid] = rng.s;
     pb.hash[pid] = 2166136261u;
     pb.frameSeed[pid] = frameSeed;
@@ -549,11 +569,13 @@ __global__ __launch_bounds__(256) void kProbeResolveM(MultiArgs ma, Queue q, Pat
 // pendShadow.w: 0 no shadow ray, -1 traced and occluded, +1 traced and visible.
 // pendProbe.w (int bits): -2 no probe ray, else the probe's hit id (-1 = miss).
 // NEE slot j of path pid is pid * stride + j (stride >= numNEE: FrameConst::neeStride).
-RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint32_t stride, uint32_t &hash) {
+// (sh0, pr0): slot 0's records, which the caller may have fetched already.
+RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint32_t stride, uint32_t &hash,
+                     float4 sh0, float4 pr0) {
     V3 s = v3(0.0f);
     for (uint32_t j = 0; j < numNEE; ++j) {
-        float4 sh = pb.pendShadow[pid * stride + j];
-        float4 pr = pb.pendProbe[pid * stride + j];
+        float4 sh = j == 0 ? sh0 : pb.pendShadow[pid * stride + j];
+        float4 pr = j == 0 ? pr0 : pb.pendProbe[pid * stride + j];
         if (sh.w != 0.0f) hash = sigFold(hash, sh.w > 0.0f ? kSigVisible : kSigOccluded);
         int probePrim = __float_as_int(pr.w);
         if (probePrim != -2) hash = sigFold(hash, kSigProbe ^ (uint32_t)probePrim);
@@ -562,6 +584,9 @@ RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint3
         s = s + term;
     }
     return s;
+}
+RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint32_t stride, uint32_t &hash) {
+    return pendingSum(pb, pid, numNEE, stride, hash, pb.pendShadow[pid * stride], pb.pendProbe[pid * stride]);
 }
 
 // ---------------------------------------------------------------- ADRRS
@@ -662,8 +687,20 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
         hash = pb.hash[pid];
         if (record) grec = pb.grec[pid];
         if (segments) task = pb.task[pid];
+        // slot 0 of the pending NEE records, fetched with the state rather than
+        // after the flags that say whether it is needed (the buffers always hold it)
+        const float4 sh0 = pb.pendShadow[pid * fc.neeStride], pr0 = pb.pendProbe[pid * fc.neeStride];
+        // The hit's geometry and material depend on the queue entry alone:
+        // their loads go out before the path state is used, so that the two
+        // chains of dependent loads overlap instead of following each other.
+        const int prim = __float_as_int(hit.y);
+        if (!splitV && prim >= 0) {
+            h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
+            m = loadMat(&S.materials[h.matIndex]);
+            applyTextures(S, m, h.u, h.v);
+        }
         if (flags & kFlagPending) {
-            V3 nee = pendingSum(pb, pid, numNEE, fc.neeStride, hash);
+            V3 nee = pendingSum(pb, pid, numNEE, fc.neeStride, hash, sh0, pr0);
             if (flags & kFlagPendTask) { // a split vertex's NEE joins main()'s result
                 float4 tr = pb.taskRes[pid];
                 pb.taskRes[pid] = f4(v3(tr) + nee, tr.w);
@@ -674,7 +711,6 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
                 res = res + nee;
             }
         }
-        int prim = __float_as_int(hit.y);
         if (splitV) {
             // the next queued split: its vertex, throughput and depth; raytrace(...,
             // addFirstHitLight = false, ...) from there (raytrace.rgen:1684-1703)
@@ -721,14 +757,13 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
             }
             isDone = true;
         } else {
-            h = hitInfo(S, prim, hit.x, hit.z, hit.w, o, d);
-            m = loadMat(&S.materials[h.matIndex]);
-            applyTextures(S, m, h.u, h.v);
             if (depth == 1) firstT = hit.x;
+#ifdef RTX_DEVICE_DEBUG
             if (b.firstPath + (pid - b.pathBase) == fc.debugPath)
                 printf("[gpu] depth %d prim %d t %.9g mat %d seed %u thr %.6g %.6g %.6g d %.6g %.6g %.6g n %.6g %.6g %.6g ff %d res %.6g\n",
                        depth, prim, hit.x, m.type, rng.s, thr.x, thr.y, thr.z, d.x, d.y, d.z, h.normal.x,
                        h.normal.y, h.normal.z, (int)h.frontFace, res.x);
+#endif
             if (m.type == RTX_MAT_LIGHT && (addDirect || addNext)) {
                 res = res + thr * m.lightColor;
                 if (record) updateSamples(pb, pid, task.w, grec.x, m.lightColor);
@@ -878,13 +913,13 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
             }
             if (probeZero) pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-2));
         }
-        int si = waveAppend(wantShadow, &cnt->shadow);
+        int si, pi;
+        waveAppend2(wantShadow, &cnt->shadow, wantProbe, &cnt->probe, si, pi);
         if (wantShadow) {
             qs.id[si] = (int)slot;
             qs.org[si] = f4(h.worldPos, kTMin);
             qs.dir[si] = f4(lightDir, lightDist * (1.0f - 0.0001f));
         }
-        int pi = waveAppend(wantProbe, &cnt->probe);
         if (wantProbe) {
             qp.id[pi] = (int)slot;
             qp.org[pi] = f4(h.worldPos, kTMin);
@@ -911,10 +946,12 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
     if (icHit) isDone = true;
     if (active && !isDone) {
         float pdf = getNewDirection(rng, G, gp, m, h.worldPos, h.normal, wi, h.frontFace, newDir);
+#ifdef RTX_DEVICE_DEBUG
         if (b.firstPath + (pid - b.pathBase) == fc.debugPath) {
             V3 ev = evalBsdf(m, h.normal, wi, newDir, h.frontFace);
             printf("[gpu]   continue pdf %.9g eval %.9g new %.6g %.6g %.6g\n", pdf, ev.x, newDir.x, newDir.y, newDir.z);
         }
+#endif
         // NaN guard beyond the reference (which stops on pdf <= 0 only): a NaN
         // pdf or BSDF ratio ends the path instead of poisoning the pixel.
         V3 change = v3(0.0f);
@@ -986,13 +1023,13 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
             pb.grec[pid] = grec;
         }
     }
-    int qi = waveAppend(cont, nextCount);
+    int qi, di;
+    waveAppend2(cont, nextCount, isDone, &cnt->done[qsel], qi, di);
     if (cont) {
         qout.id[qi] = (int)pid;
         qout.org[qi] = f4(h.worldPos, kTMin);
         qout.dir[qi] = f4(newDir, kTMax);
     }
-    int di = waveAppend(isDone, &cnt->done[qsel]);
     if (isDone) done[di] = (int)pid;
 }
 
@@ -1004,6 +1041,25 @@ __global__ __launch_bounds__(kShadeBlock, 2) void kShade(DevScene S, DevIC IC, D
     for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
         const uint32_t i = base + threadIdx.x;
         shadeLane(S, IC, G, *fcp, b, pb, qin, qout, qs, qp, done, cnt, qsel, i, i < n);
+    }
+}
+
+// Two classes of paths in one wavefront: ids below `split` are the class of
+// (fc, b), the others of (fc2, b2) (a frame's camera paths and the cache's
+// update sub-paths).  A wave takes one pass per class among its lanes.
+__global__ __launch_bounds__(kShadeBlock, 2) void kShade2(DevScene S, DevIC IC, DevGuiding G,
+                                                       const FrameConst *__restrict__ fcp, BatchDesc b,
+                                                       const FrameConst *__restrict__ fcp2, BatchDesc b2,
+                                                       uint32_t split, PathBuffers pb, Queue qin, Queue qout,
+                                                       Queue qs, Queue qp, int32_t *done, Counters *cnt, int qsel) {
+    const uint32_t n = cnt->extend[qsel];
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        const uint32_t cls = active && ((uint32_t)qin.id[i] & ~kSplitVertex) >= split ? 1u : 0u;
+        forEachKey(active, cls, [&](uint32_t k) {
+            shadeLane(S, IC, G, k ? *fcp2 : *fcp, k ? b2 : b, pb, qin, qout, qs, qp, done, cnt, qsel, i, true);
+        });
     }
 }
 
@@ -1145,6 +1201,21 @@ __global__ __launch_bounds__(256) void kFinish(const FrameConst *__restrict__ fc
     }
 }
 
+// kFinish over two classes of paths (kShade2).
+__global__ __launch_bounds__(256) void kFinish2(const FrameConst *__restrict__ fcp, BatchDesc b,
+                                                const FrameConst *__restrict__ fcp2, BatchDesc b2, uint32_t split,
+                                                PathBuffers pb, const int32_t *done, Counters *cnt, Queue qn,
+                                                int qsel) {
+    const uint32_t n = cnt->done[qsel ^ 1];
+    finishClears(cnt);
+    for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        const uint32_t cls = active && (uint32_t)done[i] >= split ? 1u : 0u;
+        forEachKey(active, cls, [&](uint32_t k) { (void)finishLane(k ? *fcp2 : *fcp, k ? b2 : b, pb, done, cnt, qn, qsel, i, true); });
+    }
+}
+
 // The same over the serving pool, one pass per slot among a wave's lanes;
 // the paths that end leave their slot's live count (one atomic per pass).
 __global__ __launch_bounds__(256) void kFinishM(MultiArgs ma, PathBuffers pb, const int32_t *done, Counters *cnt,
@@ -1255,10 +1326,11 @@ void launchGenerate(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffer
                        electCounter);
 }
 void launchGenerateIC(hipStream_t s, IcBuffers ic, uint32_t frameSeed, uint32_t firstJob, uint32_t numPaths,
-                      PathBuffers pb, Queue q0, Counters *cnt, int qsel, bool devCount) {
+                      PathBuffers pb, Queue q0, Counters *cnt, int qsel, int jobRange, uint32_t pidBase,
+                      uint32_t qBase) {
     if (numPaths == 0) return;
     hipLaunchKernelGGL(kGenerateIC, dim3(gridFor(numPaths, 256)), dim3(256), 0, s, ic, frameSeed, firstJob, numPaths,
-                       pb, q0, cnt, qsel, devCount ? 1 : 0);
+                       pb, q0, cnt, qsel, jobRange, pidBase, qBase);
 }
 template <int LDS>
 static void launchTracePL(hipStream_t s, const DevScene &S, Queue qe, int esel, Queue qs, Queue qp, PathBuffers pb,
@@ -1287,13 +1359,21 @@ void launchProbeResolve(hipStream_t s, const DevScene &S, const FrameConst *fc, 
 }
 void launchShade(hipStream_t s, const DevScene &S, const DevIC &ic, const DevGuiding &g, const FrameConst *fc,
                  BatchDesc b, PathBuffers pb, Queue qin, Queue qout, Queue qs, Queue qp, int32_t *done, Counters *cnt,
-                 int qsel, int blocks) {
-    hipLaunchKernelGGL(kShade, dim3(blocks), dim3(kShadeBlock), 0, s, S, ic, g, fc, b, pb, qin, qout, qs, qp, done,
-                       cnt, qsel);
+                 int qsel, int blocks, const PathClass2 *second) {
+    if (second)
+        hipLaunchKernelGGL(kShade2, dim3(blocks), dim3(kShadeBlock), 0, s, S, ic, g, fc, b, second->fc, second->b,
+                           second->split, pb, qin, qout, qs, qp, done, cnt, qsel);
+    else
+        hipLaunchKernelGGL(kShade, dim3(blocks), dim3(kShadeBlock), 0, s, S, ic, g, fc, b, pb, qin, qout, qs, qp, done,
+                           cnt, qsel);
 }
 void launchFinish(hipStream_t s, const FrameConst *fc, BatchDesc b, PathBuffers pb, const int32_t *done,
-                  Counters *cnt, Queue qnext, int qsel, int blocks) {
-    hipLaunchKernelGGL(kFinish, dim3(blocks), dim3(256), 0, s, fc, b, pb, done, cnt, qnext, qsel);
+                  Counters *cnt, Queue qnext, int qsel, int blocks, const PathClass2 *second) {
+    if (second)
+        hipLaunchKernelGGL(kFinish2, dim3(blocks), dim3(256), 0, s, fc, b, second->fc, second->b, second->split, pb,
+                           done, cnt, qnext, qsel);
+    else
+        hipLaunchKernelGGL(kFinish, dim3(blocks), dim3(256), 0, s, fc, b, pb, done, cnt, qnext, qsel);
 }
 void launchAccumulate(hipStream_t s, BatchDesc b, PathBuffers pb, float4 *accum, float4 *keepRgb, uint2 *keepSig,
                       bool keep, float4 *estimate) {

@@ -569,7 +569,7 @@ class Renderer:
 
     def set_tuning(self, **kw):
         """Launch-shape knobs (traceBlocksPerCU, shadeBlocksPerCU, batchPaths,
-        ldsStackEntries); fields left out keep their current values."""
+        ldsStackEntries, cacheUpkeep); fields left out keep their current values."""
         t = abi.Tuning()
         for k, v in kw.items():
             setattr(t, k, v)

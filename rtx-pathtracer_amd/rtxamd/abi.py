@@ -189,7 +189,7 @@ class FrameStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("traversalKernel", C.c_int32), ("traceBlocksPerCU", C.c_int32),
                 ("shadeBlocksPerCU", C.c_int32), ("batchPaths", C.c_uint32), ("ldsStackEntries", C.c_int32),
-                ("_reserved", C.c_int32 * 3)]
+                ("cacheUpkeep", C.c_int32), ("_reserved", C.c_int32 * 2)]
 
 
 class AutotuneResult(C.Structure):

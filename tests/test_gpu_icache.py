@@ -301,3 +301,37 @@ def test_room_cache_image_relmse(renderer):
         f"mean {img.mean():.5f} vs {ref.mean():.5f}")
     assert e <= 3 * e32 + 1e-6
     ds.close()
+
+
+def test_update_jobs_in_the_frame_equal_jobs_after_it(renderer):
+    """rtx_tuning.cacheUpkeep 1 (a frame's update jobs run their sub-paths in
+    the frame's own bounces, beside its camera paths; the creation jobs after
+    it) and 2 (every job after the frame) leave the same cache and the same
+    image, bit for bit: six prepare frames that create and update records,
+    then eight cache frames with many update elections on the Cornell box
+    and on the room."""
+    for scene, w, h in [(abi.SCENE_CORNELL, 96, 64), (abi.SCENE_ROOM, 160, 90)]:
+        hs = rtxamd.HostScene.generate(scene, 1)
+        ds = renderer.upload(hs)
+        out = []
+        for mode in (2, 1):
+            renderer.set_tuning(cacheUpkeep=mode)
+            renderer.ic_create(2000)
+            base = dict(maxDepth=8, enableMIS=1, useIrradianceCache=1)
+            renderer.render(ds, scenes.params(hs, w, h, 6, isIrradiancePrepareFrame=1, irradianceCreateProb=0.02,
+                                              irradianceUpdateProb=0.01, sample_offset=10, **base))
+            mid = renderer.ic_read()
+            renderer.render(ds, scenes.params(hs, w, h, 8, irradianceCreateProb=0.002, irradianceUpdateProb=0.02,
+                                              sample_offset=40, **base))
+            out.append((mid, renderer.ic_read(), renderer.read_image(w, h)))
+        renderer.set_tuning(cacheUpkeep=1)
+        (m2, s2, i2), (m1, s1, i1) = out
+        upd = int((s1.entries["numUpdates"] > 1).sum())
+        log(f"scene {scene}: {m1.count} records after the prepare frames, {s1.count} after the cache frames, "
+            f"{upd} of them updated")
+        assert m1.count > 50 and upd > 20
+        for a, b in [(m1, m2), (s1, s2)]:
+            assert (a.hdr.nextCacheSlot, a.hdr.nextUpdateSlot) == (b.hdr.nextCacheSlot, b.hdr.nextUpdateSlot)
+            assert a.entries.tobytes() == b.entries.tobytes() and a.spheres.tobytes() == b.spheres.tobytes()
+        assert np.array_equal(i1.view(np.uint32), i2.view(np.uint32)) and i1[..., :3].mean() > 0
+        ds.close()
