@@ -174,8 +174,11 @@ def relmse_leg(rtxamd, abi, r, a):
     subtracted).  Plates (the guiding configuration's plates-and-lights scene)
     at 640x360: plain vs guided sampling (2^8 regions of vMF mixtures, 6
     recording frames with a GPU EM update each inside the budget, then guided
-    frames); reference `relmse_ref_spp` / 2 samples per half.  Each method gets
-    `relmse_seconds` of wall time (the plates a tenth of it)."""
+    frames); reference `relmse_ref_spp` / 2 samples per half.  Spot room (a
+    closed room lit only through a bright spot on its ceiling, the case
+    guiding is for): the same plain vs guided comparison at 640x360.  Each
+    method gets `relmse_seconds` of wall time (plates and spot room a tenth of
+    it)."""
     pc = dict(maxDepth=12, enableMIS=1, enableNEE=1, usePowerHeuristic=1)
     RESET = abi.RTX_FRAME_RESET_ACCUM
 
@@ -293,6 +296,29 @@ def relmse_leg(rtxamd, abi, r, a):
     ds.close()
     out["plates"] = plates
     log(f"relmse plates: plain {e_plain:.4g}, guided {e_g:.4g}")
+
+    # ---- spot room: plain vs guided where the light arrives through a bright spot
+    hs = rtxamd.HostScene.generate(abi.SCENE_SPOT, 1)
+    ds = r.upload(hs)
+    cam = hs.camera(W, H)
+    t0 = time.perf_counter()
+    ref, var = reference(ds, cam, W, H, half)
+    ref_s = time.perf_counter() - t0
+    spot = {"scene": f"closed room lit through a ceiling spot, {W}x{H}, depth 12, NEE+MIS",
+            "reference": f"2 x {half} spp plain (independent seeds), FP64 accumulation, its variance "
+                         f"subtracted ({ref_s:.1f} s)", "budget_s": T}
+    state["spp"] = 0
+    n, sec = equal_time(plain_call, T)
+    e_plain = relmse(r.read_image(W, H), ref, ref_var=var)
+    spot["plain"] = {"spp": state["spp"], "seconds": round(sec, 3), "relmse": e_plain}
+    bounds = ds.bounds()
+    n, sec = equal_time(guided_call, T)
+    e_g = relmse(r.read_image(W, H), ref, ref_var=var)
+    spot["guided"] = {"spp": state["spp"] - 6, "seconds": round(sec, 3), "relmse": e_g,
+                      "improvement": round(e_plain / e_g, 3)}
+    ds.close()
+    out["spot"] = spot
+    log(f"relmse spot room: plain {e_plain:.4g}, guided {e_g:.4g}")
     return out
 
 
@@ -353,9 +379,32 @@ def main():
     ap.add_argument("--relmse-seconds", type=float, default=2.0, help="equal-time budget per method (room)")
     ap.add_argument("--relmse-ref-spp", type=int, default=1024,
                     help="reference samples per pixel (the room renders a quarter of them at 1080p)")
+    ap.add_argument("--nrc-mlp", type=int, default=0, metavar="N",
+                    help="only run the learned cache's network on N random queries (rocprofv3 MFMA timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 byte counters)")
     a = ap.parse_args()
+    if a.nrc_mlp:
+        # the learned cache's network alone (rtx_nrc_infer / rtx_nrc_train on
+        # random inputs), for rocprofv3 kernel times of its MFMA kernels
+        import rtxamd
+        r = rtxamd.Renderer(0)
+        r.nrc_create([-1.0, -1.0, -1.0], [1.0, 1.0, 1.0], seed=3)
+        rng = np.random.default_rng(1)
+        x = rng.uniform(-1, 1, (a.nrc_mlp, 16)).astype(np.float32)
+        x[:, 3] = rng.uniform(0.05, 1.0, a.nrc_mlp)  # roughness
+        for c in (7, 11, 15):
+            x[:, c] = 0.0
+        for _ in range(5):
+            r.nrc_infer(x)
+        B = 16384
+        t = rng.uniform(0, 1, (B, 3)).astype(np.float32)
+        losses = [r.nrc_train(x[:B], t) for _ in range(8)]
+        r.close()
+        print(json.dumps({"nrc_queries": a.nrc_mlp, "infer_calls": 5, "train_rows": B, "train_steps": 8,
+                          "flop_per_query": 2 * (5 * 64 * 64 + 64 * 3), "loss_first": losses[0],
+                          "loss_last": losses[-1]}))
+        return
     if a.calibrate_copy:
         import torch
         n = a.calibrate_copy * (1 << 20) // 4
@@ -474,6 +523,10 @@ def main():
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
+    # what the HBM itself carries per launch (rocprofv3 FETCH/WRITE of the same
+    # workload) at the launch duration measured here: the traversal's
+    # algorithmic bytes are mostly served by L2, so this is far below `achieved`
+    hbm_gbs = round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1) if traffic else None
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -484,6 +537,10 @@ def main():
         "nodes_per_ray": round(tr_nodes / max(1, tr_rays), 2),
         "tris_per_ray": round(tr_prims / max(1, tr_rays), 2),
         "all_traversal_frac": round(all_bytes / (trav_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "hbm_traffic_gbs": hbm_gbs,
+        "note": "achieved = algorithmic bytes (every node and triangle a ray reads, as if from HBM) per launch "
+                "over the launch time; the kernel is bound by dependent-load latency and VALU issue, with "
+                "the BVH served mostly from L2 (hbm_traffic_gbs is what HBM carries)",
     }
     mrays = all_rays / (ms_per_step * 1e3)
 

@@ -26,6 +26,8 @@ typedef struct rtxh_scene rtxh_scene;
 #define RTXH_SCENE_CORNELL 1 /* 5 walls + 2 boxes (34 tris) + quad light + dielectric sphere */
 #define RTXH_SCENE_ROOM 2    /* 20x10x20 m room, exactly `triangles` triangles, 4 area lights */
 #define RTXH_SCENE_PLATES 3  /* plates-and-sphere-lights test (veachMIS-like) */
+#define RTXH_SCENE_SPOT 4    /* closed diffuse room lit only through a spot on its ceiling (a
+                                lamp on a black box under it): the case path guiding is for */
 
 /* Camera placement of a scene (SceneLoader origin/target/upDir/vfov,
  * src/SceneLoader.h:147-150). */

@@ -55,6 +55,7 @@ int rtxh_scene_generate(int kind, uint64_t seed, uint32_t triangles, rtxh_scene 
         case RTXH_SCENE_CORNELL: ok = rtxh::generateCornell(seed, s->scene, err); break;
         case RTXH_SCENE_ROOM: ok = rtxh::generateRoom(seed, triangles, s->scene, err); break;
         case RTXH_SCENE_PLATES: ok = rtxh::generatePlates(seed, s->scene, err); break;
+        case RTXH_SCENE_SPOT: ok = rtxh::generateSpot(seed, s->scene, err); break;
         default: err = "unknown scene kind " + std::to_string(kind);
         }
     } catch (const std::exception &e) {

@@ -161,4 +161,33 @@ bool generatePlates(uint64_t /*seed*/, Scene &s, std::string &err) {
     return true;
 }
 
+// A closed diffuse room (6 x 6 x 3, albedo 0.8) whose only light is a small
+// square lamp (radiance 50) lying on top of a black box under the ceiling:
+// the lamp lights a spot on the ceiling and the upper walls, and everything
+// below the box sees it only through them, so BSDF sampling finds the bright
+// spot rarely and a guide that learns its direction pays off.
+bool generateSpot(uint64_t /*seed*/, Scene &s, std::string &err) {
+    (void)err;
+    int wall = s.addMaterial(makeDiffuse({0.8f, 0.8f, 0.8f}));
+    int black = s.addMaterial(makeDiffuse({0.0f, 0.0f, 0.0f}));
+    int lamp = s.addMaterial(makeLight({0.0f, 0.0f, 0.0f}, {50.0f, 50.0f, 50.0f}));
+    Model light;
+    addQuad(light, {-0.2f, -0.2f, 2.51f}, {0.2f, -0.2f, 2.51f}, {0.2f, 0.2f, 2.51f}, {-0.2f, 0.2f, 2.51f}, lamp);
+    s.addModelInstance(std::move(light));
+    Model shade;
+    addBox(shade, {-0.21f, -0.21f, 2.3f}, {0.21f, 0.21f, 2.505f}, black);
+    s.addModelInstance(std::move(shade));
+    Model room;
+    addQuad(room, {-3, -3, 0}, {3, -3, 0}, {3, 3, 0}, {-3, 3, 0}, wall);  // floor (+z)
+    addQuad(room, {-3, -3, 3}, {-3, 3, 3}, {3, 3, 3}, {3, -3, 3}, wall);  // ceiling (-z)
+    addQuad(room, {-3, -3, 0}, {-3, 3, 0}, {-3, 3, 3}, {-3, -3, 3}, wall);
+    addQuad(room, {3, -3, 0}, {3, -3, 3}, {3, 3, 3}, {3, 3, 0}, wall);
+    addQuad(room, {-3, 3, 0}, {3, 3, 0}, {3, 3, 3}, {-3, 3, 3}, wall);
+    addQuad(room, {-3, -3, 0}, {-3, -3, 3}, {3, -3, 3}, {3, -3, 0}, wall);
+    s.addModelInstance(std::move(room));
+    s.view = rtxh_view{{0.0f, -2.8f, 1.5f}, {0.0f, 0.0f, 1.1f}, {0.0f, 0.0f, 1.0f}, 70.0f};
+    s.buildLightTables();
+    return true;
+}
+
 } // namespace rtxh

@@ -5,7 +5,7 @@
 // regions over the scene box; the options switch their use on (the
 // reference's UI toggles, off by default as in src/RayTracingApp.h:97-120).
 // Scenes are Mitsuba XML, JSON or OBJ files (rtxh_scene_load) or the
-// procedural scenes gen:cornell, gen:room[:TRIANGLES] and gen:plates.
+// procedural scenes gen:cornell, gen:room[:TRIANGLES], gen:plates and gen:spot.
 // Frames run through the reference's per-frame host logic (rtx_schedule_*):
 // cache prepare frames, the guiding cadence, progressive accumulation, and
 // eval-by-samples, which stops after --spp samples per pixel.  Each scene is
@@ -88,6 +88,7 @@ static bool parseArgs(int argc, char **argv, Options &o) {
 static int loadScene(const std::string &name, uint32_t seed, rtxh_scene **hs) {
     if (name == "gen:cornell") return rtxh_scene_generate(RTXH_SCENE_CORNELL, seed, 0, hs);
     if (name == "gen:plates") return rtxh_scene_generate(RTXH_SCENE_PLATES, seed, 0, hs);
+    if (name == "gen:spot") return rtxh_scene_generate(RTXH_SCENE_SPOT, seed, 0, hs);
     if (name.rfind("gen:room", 0) == 0) {
         uint32_t tris = name.size() > 9 ? (uint32_t)std::strtoul(name.c_str() + 9, nullptr, 10) : 0;
         return rtxh_scene_generate(RTXH_SCENE_ROOM, seed, tris, hs);

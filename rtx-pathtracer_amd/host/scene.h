@@ -114,6 +114,7 @@ void addColumn(Model &m, Vec3 base, float radius, float height, int sides, int m
 bool generateCornell(uint64_t seed, Scene &s, std::string &err);
 bool generateRoom(uint64_t seed, uint32_t triangles, Scene &s, std::string &err);
 bool generatePlates(uint64_t seed, Scene &s, std::string &err);
+bool generateSpot(uint64_t seed, Scene &s, std::string &err);
 bool loadMitsubaXml(const std::string &path, Scene &s, std::string &err);
 bool loadJsonScene(const std::string &path, Scene &s, std::string &err);
 // By extension: ".xml" Mitsuba, ".json" the reference's own format.

@@ -31,6 +31,7 @@ DISTRIBUTION_BECKMANN, DISTRIBUTION_GGX = 0, 1
 SCENE_CORNELL = 1
 SCENE_ROOM = 2
 SCENE_PLATES = 3
+SCENE_SPOT = 4
 
 F3 = C.c_float * 3
 M4 = C.c_float * 16

@@ -65,27 +65,12 @@ def material_zoo(point_light=True, distribution=0):
     return s.finalize()
 
 
-def spot_room(radiance=50.0, albedo=0.8):
-    """A closed diffuse room (6 x 6 x 3) whose only light is a small square
-    lamp lying on top of a black box under the ceiling: the lamp lights a spot
-    on the ceiling and the upper walls, and everything below the box sees it
-    only through them.  BSDF sampling finds the bright spot rarely; a guide
-    that learns its direction lowers the variance there (the classic case for
-    path guiding)."""
-    s = rtxamd.HostScene.new()
-    wall = s.add_material(rtxamd.diffuse((albedo,) * 3))
-    black = s.add_material(rtxamd.diffuse((0.0, 0.0, 0.0)))
-    lamp = s.add_material(rtxamd.light((radiance,) * 3))
-    p, i = quad((-0.2, -0.2, 2.51), (0.2, -0.2, 2.51), (0.2, 0.2, 2.51), (-0.2, 0.2, 2.51))
-    s.add_mesh(p, i, lamp)
-    p, i = box((-0.21, -0.21, 2.3), (0.21, 0.21, 2.505))
-    s.add_mesh(p, i, black)
-    for q in [quad((-3, -3, 0), (3, -3, 0), (3, 3, 0), (-3, 3, 0)), quad((-3, -3, 3), (-3, 3, 3), (3, 3, 3), (3, -3, 3)),
-              quad((-3, -3, 0), (-3, 3, 0), (-3, 3, 3), (-3, -3, 3)), quad((3, -3, 0), (3, -3, 3), (3, 3, 3), (3, 3, 0)),
-              quad((-3, 3, 0), (3, 3, 0), (3, 3, 3), (-3, 3, 3)), quad((-3, -3, 0), (-3, -3, 3), (3, -3, 3), (3, -3, 0))]:
-        s.add_mesh(*q, wall)
-    s.set_view((0.0, -2.8, 1.5), (0.0, 0.0, 1.1), (0.0, 0.0, 1.0), 70.0)
-    return s.finalize()
+def spot_room():
+    """The closed room lit only through a spot on its ceiling (a lamp on a
+    black box under it; host generator RTXH_SCENE_SPOT): BSDF sampling finds
+    the bright spot rarely, a guide that learns its direction lowers the
+    variance (the classic case for path guiding)."""
+    return rtxamd.HostScene.generate(abi.SCENE_SPOT)
 
 
 def emitter_over_plane(side=1.0, height=1.0, radiance=5.0, albedo=0.5):
