@@ -1,6 +1,7 @@
 """The frame schedule on the device: rtx_schedule_frame renders exactly the
 frames its plan describes (the same images as planning with
-begin_frame/end_frame and calling rtx_render by hand), the ADRRS sequence
+begin_frame/end_frame and calling rtx_render by hand, and the oracle's
+images for the same plan, frame by frame), the ADRRS sequence
 (prepare frames, estimate frame, ADRRS frames) runs end to end with the
 estimate frame in the average as in the reference, and the guiding cadence
 skips the update after the last recording frame."""
@@ -53,6 +54,47 @@ def test_scheduled_frames_equal_planned_renders(renderer, cornell):
         assert samples(renderer) == 3  # prepare frames are not accumulated
         assert s.state().frames == 6
     assert np.array_equal(images[0], images[1])
+    ds.close()
+
+
+def test_scheduled_cache_frames_match_oracle(renderer, cornell):
+    """The schedule's frames on the device against the oracle following the
+    same plan frame by frame from the same empty cache (orc_ic_frame: the
+    frame and its upkeep): three prepare frames, then five cache frames.  The
+    device accumulates exactly the cache frames, and its image's RelMSE
+    against the FP64 oracle's average is at most 3x that of the reference
+    logic in FP32 (+1e-6), the bar of the per-path parity tests carried to the
+    image (each run grows its own cache, so FP32 branch flips move records)."""
+    from oracle.binding import OracleScene
+    from parity import relmse
+    hs = cornell
+    orc = OracleScene(abi, hs.desc())
+    ds = renderer.upload(hs)
+    fp = base(hs, useIrradianceCache=1, maxDepth=6, enableMIS=1, irradianceCreateProb=0.01,
+              irradianceUpdateProb=0.01)
+    renderer.ic_create(2000)
+    s = rtxamd.Schedule(fp.pc, prepareFrames=3)
+    st64, st32 = rtxamd.IcState.empty(2000), rtxamd.IcState.empty(2000)
+    acc64, acc32, n = np.zeros((W * H, 3)), np.zeros((W * H, 3)), 0
+    for _ in range(8):
+        f = s.begin_frame(fp)
+        renderer.render(ds, f)
+        s.end_frame()
+        rgb64 = orc.ic_frame(f, st64, prec=64)[0]
+        rgb32 = orc.ic_frame(f, st32, prec=32)[0]
+        if not f.pc.isIrradiancePrepareFrame:
+            acc64 += rgb64
+            acc32 += rgb32
+            n += 1
+    assert n == 5 and samples(renderer) == n
+    img = renderer.read_image(W, H)[..., :3].astype(np.float64)
+    ref = (acc64 / n).reshape(H, W, 3)
+    e, e32 = relmse(img, ref), relmse((acc32 / n).reshape(H, W, 3), ref)
+    dev = renderer.ic_read()
+    log(f"scheduled cache frames: records device {dev.count} / oracle {st64.count}; image relmse vs FP64 "
+        f"{e:.3e} (reference logic in FP32 {e32:.3e}), mean {img.mean():.5f} vs {ref.mean():.5f}")
+    assert abs(dev.count - st64.count) <= max(3, 0.05 * st64.count)
+    assert e <= 3 * e32 + 1e-6
     ds.close()
 
 
