@@ -662,8 +662,11 @@ int rtx_trace_closest_noalloc(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ra
  * ends (no NEE there; the cache holds everything the tracer would add from
  * that vertex on, emission at the vertex excluded).  About trainingPaths
  * paths per frame (a regular pixel pattern that moves every frame) continue
- * unchanged instead; what they add from that vertex on, divided by their
- * throughput there, is the training target of the vertex.  After each frame
+ * instead, for trainSuffixBounces more non-specular vertices, where they end
+ * into the cache themselves (or to their end when it is 0); what they add
+ * from the training vertex on, the cache's prediction at their end included,
+ * divided by their throughput at the training vertex, is the vertex's
+ * training target.  After each frame
  * the cache takes trainSteps Adam steps on trainBatch of these targets each
  * (relative L2 loss, Mueller et al. 2021), so a frame reads the cache as the
  * previous frames left it.  Not combined with the irradiance cache, ADRRS,
@@ -682,6 +685,12 @@ typedef struct rtx_nrc_config {
     float sceneMin[3];         /* box the positions are encoded in              */
     float sceneMax[3];
     uint64_t seed;             /* parameter initialisation                      */
+    uint32_t trainSuffixBounces; /* a training path ends into the cache this many
+                                    non-specular vertices past its training vertex,
+                                    its target completed there by the cache's own
+                                    prediction (self-training, Mueller et al. 2021);
+                                    0: it runs to its end (1)                     */
+    uint32_t reserved;
 } rtx_nrc_config;
 
 /* One cache query: the vertex's position, the direction towards the previous

@@ -1034,6 +1034,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.splitOnFirst = p->pc.splitOnFirst ? 1u : 0u;
     fc.nrc = nrc ? 1u : 0u;
     fc.nrcDepth = nrc ? ctx->nrcCfg.terminateBounces + 1u : 0u;
+    fc.nrcSuffix = nrc ? ctx->nrcCfg.trainSuffixBounces : 0u;
     fc.nrcTrainEvery = nrcEvery;
     fc.nrcTrainSlots = nrcSlots;
     CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));
@@ -1183,7 +1184,9 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             if (nrc) // the paths that ended into the learned cache: throughput x cache
                 timed(kClsShade, [&] {
                     launchNrcInfer(s, ctx->nrc, ctx->pb.nrcQuery, b.numPaths, ctx->pb.nrcQueryCount, nullptr,
-                                   ctx->pb.nrcQueryAux, ctx->pb.out, 2 * ctx->cuCount);
+                                   ctx->pb.nrcQueryAux, ctx->pb.out, 2 * ctx->cuCount,
+                                   NrcSelfTrain{ctx->pb.thr, ctx->pb.nrcPath,
+                                                fc.nrcSuffix ? ctx->pb.nrcTrainTarget : nullptr, kFlagNrcTrainEnd});
                 });
             timed(kClsShade, [&] {
                 launchAccumulate(s, b, ctx->pb, accumulate ? ctx->accum : nullptr, ctx->keepRgb, ctx->keepSig, keep,
@@ -1976,6 +1979,7 @@ void rtx_nrc_config_defaults(rtx_nrc_config *c) {
     c->trainSteps = 4;
     c->terminateBounces = 2;
     c->trainingPaths = 65536;
+    c->trainSuffixBounces = 1;
     for (int a = 0; a < 3; ++a) {
         c->sceneMin[a] = -1.0f;
         c->sceneMax[a] = 1.0f;
@@ -1991,9 +1995,9 @@ int rtx_nrc_create(rtx_ctx *ctx, const rtx_nrc_config *c) try {
     if (!(c->baseResolution >= 2.0f) || !(c->maxResolution >= c->baseResolution) || !(c->maxResolution <= 1e6f))
         return fail(ctx, RTX_E_INVALID, "need 2 <= baseResolution <= maxResolution <= 1e6");
     if (c->trainBatch < 256 || c->trainBatch > 65536 || c->trainSteps < 1 || c->trainSteps > 64 ||
-        c->trainingPaths < 1 || c->terminateBounces > 64)
+        c->trainingPaths < 1 || c->terminateBounces > 64 || c->trainSuffixBounces > 64)
         return fail(ctx, RTX_E_INVALID, "trainBatch in [256, 65536], trainSteps in [1, 64], trainingPaths >= 1, "
-                                        "terminateBounces <= 64");
+                                        "terminateBounces and trainSuffixBounces <= 64");
     if (!(c->learningRate > 0.0f) || !(c->beta1 >= 0.0f && c->beta1 < 1.0f) || !(c->beta2 >= 0.0f && c->beta2 < 1.0f) ||
         !(c->epsilon >= 0.0f))
         return fail(ctx, RTX_E_INVALID, "Adam: learningRate > 0, beta1 and beta2 in [0, 1), epsilon >= 0");

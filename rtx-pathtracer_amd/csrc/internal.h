@@ -109,6 +109,7 @@ struct FrameConst {
     // first non-specular vertex of depth >= nrcDepth; pixels with
     // (pixel + frame offset) % nrcTrainEvery == 0 are training paths
     uint32_t nrc, nrcDepth, nrcTrainEvery, nrcTrainSlots;
+    uint32_t nrcSuffix; // a training path ends into the cache this many non-specular vertices past its training vertex (0: never)
 };
 
 // ------------------------------------------------------------ path guiding
@@ -295,6 +296,7 @@ constexpr uint32_t kFlagCandShift = 19;           // bits 19-21 irradiance-cache
 constexpr uint32_t kFlagCandMask = 7u << kFlagCandShift;
 constexpr uint32_t kFlagPendTask = 1u << 22;      // the pending NEE terms belong to main()'s result (a split vertex)
 constexpr uint32_t kFlagNrcTrain = 1u << 23;      // a training path past its training vertex
+constexpr uint32_t kFlagNrcTrainEnd = 1u << 24;   // ... that ended into the cache (its target adds the prediction)
 // Extend-queue id bit: the entry is a queued split to resume (its ray is
 // empty); kShade takes the vertex from the path's split record.
 constexpr uint32_t kSplitVertex = 1u << 31;

@@ -793,8 +793,18 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
                         ti[2] = f4(h.normal, 0.0f);
                         ti[3] = f4(alb, 0.0f);
                         pb.nrcPath[2 * (size_t)pid] = f4(thr, __int_as_float(slot));
-                        pb.nrcPath[2 * (size_t)pid + 1] = f4(res, 0.0f);
+                        pb.nrcPath[2 * (size_t)pid + 1] = f4(res, __int_as_float(depth));
                         flags |= kFlagNrcTrain;
+                    }
+                } else if (fc.nrcSuffix && (flags & kFlagNrcTrain) && !(flags & kFlagNrcTrainEnd)) {
+                    // self-training: the suffix ends into the cache nrcSuffix vertices past the
+                    // training vertex; the cache's prediction there completes the target
+                    const int d0 = __float_as_int(pb.nrcPath[2 * (size_t)pid + 1].w);
+                    if (depth >= d0 + (int)fc.nrcSuffix) {
+                        nrcQ = true;
+                        isDone = true;
+                        nee = false;
+                        flags |= kFlagNrcTrainEnd;
                     }
                 }
                 if ((useIC || (useADRRS && depth > 1)) && isICCapable(m.type, icOnGlossy)) { // raytrace.rgen:1080-1139
@@ -1012,7 +1022,7 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
         uint32_t nf = (uint32_t)depth | ((uint32_t)followCount << kFlagFollowCountShift) |
                       (follow ? kFlagFollow : 0u) | (addNext ? kFlagAddNext : 0u) |
                       (doNee ? kFlagPending : 0u) | (doNee && pendTask ? kFlagPendTask : 0u) |
-                      (ncand << kFlagCandShift) | (flags & kFlagNrcTrain);
+                      (ncand << kFlagCandShift) | (flags & (kFlagNrcTrain | kFlagNrcTrainEnd));
         if (taskDirty) pb.task[pid] = task;
         pb.rng[pid] = rng.s;
         pb.hash[pid] = hash;

@@ -56,9 +56,18 @@ struct NrcBuffers {
     float lr, beta1, beta2, eps;
 };
 
+// Self-training suffixes (inference inside a render): a query whose path has
+// endFlag in pathThr[pid].w also adds its term, over the throughput at the
+// training vertex (trainPath[2 pid].xyz; .w = its target slot), to target.
+struct NrcSelfTrain {
+    const float4 *pathThr;
+    const float4 *trainPath;
+    float4 *target;   // NULL: no self-training
+    uint32_t endFlag;
+};
 // n rows, or *count rows (at most n) when count is given
 void launchNrcInfer(hipStream_t s, const NrcBuffers &nb, const float4 *in, uint32_t n, const uint32_t *count,
-                    float4 *out, const float4 *aux, float4 *pathOut, int blocks);
+                    float4 *out, const float4 *aux, float4 *pathOut, int blocks, NrcSelfTrain st = NrcSelfTrain{});
 // Rows (queries, training vertices) are rtx_nrc_input records, 4 float4 each:
 // (pos, roughness), (dir, -), (normal, -), (albedo, -).
 // Forward with saved activations, the loss gradient, the backward pass, the

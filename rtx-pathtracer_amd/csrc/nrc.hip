@@ -184,7 +184,7 @@ __device__ void mlpWave(const uint16_t *wf, uint16_t *act, uint16_t *acts, uint3
 template <bool TRAIN>
 __global__ __launch_bounds__(kBlock) void kNrcForward(NrcBuffers nb, const float4 *in, uint32_t nHost,
                                                       const uint32_t *count, float4 *out, const float4 *aux,
-                                                      float4 *pathOut) {
+                                                      float4 *pathOut, NrcSelfTrain st) {
     __shared__ __align__(16) uint16_t wf[kNrcLayers * kNrcWidth * kPitch];
     __shared__ __align__(16) uint16_t act[kNrcRowsPerBlock * kPitch];
     const uint32_t n = count ? min(*count, nHost) : nHost;
@@ -216,10 +216,21 @@ __global__ __launch_bounds__(kBlock) void kNrcForward(NrcBuffers nb, const float
                     if (out) out[row] = rgb;
                     if (pathOut) {
                         const float4 a = aux[row];
-                        float4 &po = pathOut[__float_as_uint(a.w)];
-                        po.x += a.x * fmaxf(rgb.x, 0.0f);
-                        po.y += a.y * fmaxf(rgb.y, 0.0f);
-                        po.z += a.z * fmaxf(rgb.z, 0.0f);
+                        const uint32_t pid = __float_as_uint(a.w);
+                        float4 &po = pathOut[pid];
+                        const float px = fmaxf(rgb.x, 0.0f), py = fmaxf(rgb.y, 0.0f), pz = fmaxf(rgb.z, 0.0f);
+                        po.x += a.x * px;
+                        po.y += a.y * py;
+                        po.z += a.z * pz;
+                        // a self-training suffix's end: the target of its training
+                        // vertex gets the same term over the throughput there
+                        if (st.target && (__float_as_uint(st.pathThr[pid].w) & st.endFlag)) {
+                            const float4 tp = st.trainPath[2 * (size_t)pid];
+                            float4 &t = st.target[__float_as_int(tp.w)];
+                            t.x += tp.x != 0.0f ? a.x * px / tp.x : 0.0f;
+                            t.y += tp.y != 0.0f ? a.y * py / tp.y : 0.0f;
+                            t.z += tp.z != 0.0f ? a.z * pz / tp.z : 0.0f;
+                        }
                     }
                 }
             }
@@ -425,11 +436,11 @@ __global__ __launch_bounds__(256) void kNrcZero(float *p, size_t n) {
 } // namespace
 
 void launchNrcInfer(hipStream_t s, const NrcBuffers &nb, const float4 *in, uint32_t n, const uint32_t *count,
-                    float4 *out, const float4 *aux, float4 *pathOut, int blocks) {
+                    float4 *out, const float4 *aux, float4 *pathOut, int blocks, NrcSelfTrain st) {
     if (n == 0) return;
     const int tiles = (int)((n + kNrcRowsPerBlock - 1) / kNrcRowsPerBlock);
     hipLaunchKernelGGL(kNrcForward<false>, dim3(tiles < blocks ? tiles : blocks), dim3(kBlock), 0, s, nb, in, n,
-                       count, out, aux, pathOut);
+                       count, out, aux, pathOut, st);
 }
 
 void launchNrcGradients(hipStream_t s, NrcBuffers nb, const float4 *in, const float4 *target, uint32_t n,
@@ -440,7 +451,7 @@ void launchNrcGradients(hipStream_t s, NrcBuffers nb, const float4 *in, const fl
     const uint32_t chunks = (n + kChunk - 1) / kChunk;
     const uint32_t tiles = chunks * (kChunk / kNrcRowsPerBlock);
     hipLaunchKernelGGL(kNrcForward<true>, dim3(tiles), dim3(kBlock), 0, s, nb, in, n, (const uint32_t *)nullptr,
-                       (float4 *)nullptr, (const float4 *)nullptr, (float4 *)nullptr);
+                       (float4 *)nullptr, (const float4 *)nullptr, (float4 *)nullptr, NrcSelfTrain{});
     hipLaunchKernelGGL(kNrcBackward, dim3(tiles), dim3(kBlock), 0, s, nb, target, n, invCount, lossOut);
     hipLaunchKernelGGL(kNrcWeightGrad, dim3(chunks, kNrcLayers), dim3(kBlock), 0, s, nb);
     hipLaunchKernelGGL(kNrcReduceGrad, dim3((kNrcMlpParams + 255) / 256), dim3(256), 0, s, nb, chunks);

@@ -159,6 +159,9 @@ def test_training_lowers_the_loss(cache):
     assert relmse(pred, f(x)) < 0.05
 
 
+RAYS = {}
+
+
 def test_training_paths_are_the_plain_paths():
     """With one training path per pixel no path ends into the cache: the frame
     is the plain tracer's, path for path."""
@@ -167,7 +170,8 @@ def test_training_paths_are_the_plain_paths():
     with rtxamd.Renderer(0) as r:
         ds = r.upload(hs)
         info = ds.info()
-        r.nrc_create(list(info.boundsMin), list(info.boundsMax), trainingPaths=W * H, trainBatch=4096, trainSteps=1)
+        r.nrc_create(list(info.boundsMin), list(info.boundsMax), trainingPaths=W * H, trainBatch=4096, trainSteps=1,
+                     trainSuffixBounces=0)
         out = {}
         for flags in (0, abi.RTX_FRAME_NRC):
             fp = scenes.params(hs, W, H, 2, maxDepth=8, enableMIS=1,
@@ -180,12 +184,16 @@ def test_training_paths_are_the_plain_paths():
     assert np.array_equal(out[0][1], out[abi.RTX_FRAME_NRC][1])
 
 
-def test_trained_cache_renders_close_to_plain():
+@pytest.mark.parametrize("suffix", [0, 1])
+def test_trained_cache_renders_close_to_plain(suffix):
     """Cornell box, paths ending into the cache after 2 bounces: after 256
     training frames (4 steps each) the cache's 16-spp image has the plain
     image's mean within 10%, and its per-pixel error against a 1024-spp plain
     image is of the order of the plain tracer's at the same sample count
-    (the equal-time comparison on the room is in the benchmark)."""
+    (the equal-time comparison on the room is in the benchmark); with the
+    training suffixes ending into the cache one vertex past their training
+    vertex (self-training) as with full-length ones, while a frame traces
+    fewer rays."""
     hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL, 1)
     W = H = 96
     with rtxamd.Renderer(0) as r:
@@ -197,8 +205,8 @@ def test_trained_cache_renders_close_to_plain():
         r.render(ds, scenes.params(hs, W, H, 16, sample_offset=5000, **pc))
         plain16 = r.read_image(W, H)[..., :3]
         r.nrc_create(list(info.boundsMin), list(info.boundsMax), trainingPaths=4096, trainBatch=1024,
-                     trainSteps=4, seed=3)
-        r.render(ds, scenes.params(hs, W, H, 256, sample_offset=10000, flags=abi.RTX_FRAME_NRC, **pc))
+                     trainSteps=4, seed=3, trainSuffixBounces=suffix)
+        st = r.render(ds, scenes.params(hs, W, H, 256, sample_offset=10000, flags=abi.RTX_FRAME_NRC, **pc))
         ni = r.nrc_info()
         r.render(ds, scenes.params(hs, W, H, 16, sample_offset=20000, flags=abi.RTX_FRAME_NRC | abi.RTX_FRAME_RESET_ACCUM,
                                    **pc))
@@ -206,8 +214,12 @@ def test_trained_cache_renders_close_to_plain():
         ds.close()
     e_plain, e_nrc = relmse(plain16, ref), relmse(nrc16, ref)
     m_ref, m_nrc = ref.mean(), nrc16.mean()
-    log(f"nrc render: {ni.steps} steps, last loss {ni.lastLoss:.4f}; 16 spp relmse plain {e_plain:.4f} "
-        f"cache {e_nrc:.4f}; mean {m_nrc:.4f} vs {m_ref:.4f}")
+    rays = st.extendRays / 256
+    log(f"nrc render (suffix {suffix}): {ni.steps} steps, last loss {ni.lastLoss:.4f}, {rays:.0f} extend rays a "
+        f"training frame; 16 spp relmse plain {e_plain:.4f} cache {e_nrc:.4f}; mean {m_nrc:.4f} vs {m_ref:.4f}")
+    RAYS[suffix] = rays
+    if len(RAYS) == 2:
+        assert RAYS[1] < RAYS[0]
     assert np.isfinite(nrc16).all()
     assert ni.steps == 256 * 4
     assert abs(m_nrc - m_ref) < 0.1 * m_ref
