@@ -11,9 +11,9 @@
 // device, so the stream never drains while jobs run.
 //
 // Host side: one thread.  Per bounce it admits queued jobs whose images fit
-// the framebuffer arena (in arrival order), gives free slots the next batch
-// of the earliest admitted job that has one (install + generate, appended
-// to the next bounce's extend queue), launches the bounce, and reads the
+// the framebuffer arena (in arrival order), gives each free slot the next
+// batch of the admitted job holding the fewest slots (install + generate,
+// appended to the next bounce's extend queue), launches the bounce, and reads the
 // ring of an earlier bounce: a slot whose count reached zero holds a
 // finished batch.  Finished batches are added to their job's image in batch
 // order (then the slot is free), and a job whose batches are all added gets
@@ -32,6 +32,7 @@
 #include <new>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "abi_guard.h"
@@ -307,16 +308,27 @@ void Loop::fillSlots() {
         admit();
         order = sv_->running;
     }
+    // A free slot goes to the admitted job holding the fewest slots (the
+    // earliest of those): every admitted job keeps batches in flight, so a
+    // small job is not held behind a large one that arrived first.
+    std::unordered_map<const Job *, uint32_t> held;
+    for (const Slot &s : sv_->slots)
+        if (s.job) held[s.job]++;
     for (uint32_t k = 0; k < sv_->slots.size(); ++k) {
         Slot &s = sv_->slots[k];
         if (s.job) continue;
         Job *j = nullptr;
-        for (Job *c : order)
-            if (c->nextBatch < c->numBatches) {
+        uint32_t best = UINT32_MAX;
+        for (Job *c : order) {
+            if (c->nextBatch >= c->numBatches) continue;
+            const uint32_t h = held[c];
+            if (h < best) {
+                best = h;
                 j = c;
-                break;
             }
+        }
         if (!j) break;
+        held[j]++;
         if (fresh_) { // the pool was idle: every counter starts at zero
             launchFillU32(sv_->stream, reinterpret_cast<uint32_t *>(sv_->cnt), sizeof(Counters) / 4, 0u);
             qsel_ = 0;
