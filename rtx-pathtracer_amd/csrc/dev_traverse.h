@@ -124,11 +124,28 @@ RTX_DI bool isectSphere(const rtx_sphere &s, V3 o, V3 d, float tmin, float tmax,
 }
 
 // Stack with LDS_DEPTH entries in LDS (column per thread, stride BLOCK)
-// and the remainder in a private array.
+// and the remainder in a private array of kStackCapacity - LDS_DEPTH
+// entries.  The private array is a separate local of the caller, not a
+// member: a dynamically indexed array inside the struct keeps the whole
+// struct in scratch, sp included, and every push and pop then waits on a
+// scratch load of sp.
+// The two pointers carry their address spaces (3: LDS, 5: private), so that
+// a pop, which reads one or the other, compiles to ds_read and scratch_load
+// rather than to a flat load of a selected pointer.
+typedef __attribute__((address_space(3))) int LdsInt;
+typedef __attribute__((address_space(5))) int PrivInt;
+
 template <int BLOCK, int LDS_DEPTH> struct HybridStack {
-    int *lds;
+    static constexpr int kSpill = kStackCapacity - LDS_DEPTH;
+    LdsInt *lds;
+    PrivInt *priv;
     int sp;
-    int priv[kStackCapacity - LDS_DEPTH];
+    // ldsColumn: this thread's column of the block's LDS stack (stride BLOCK);
+    // spill: kSpill private entries.
+    RTX_DI void bind(int *ldsColumn, int *spill) {
+        lds = (LdsInt *)ldsColumn;
+        priv = (PrivInt *)spill;
+    }
     RTX_DI void push(int x) {
         if (sp < LDS_DEPTH) lds[sp * BLOCK] = x;
         else priv[sp - LDS_DEPTH] = x;
@@ -140,9 +157,11 @@ template <int BLOCK, int LDS_DEPTH> struct HybridStack {
     }
 };
 
+// Whole stack in a private array (kStackCapacity entries, the caller's).
 struct PrivateStack {
+    PrivInt *priv;
     int sp;
-    int priv[kStackCapacity];
+    RTX_DI PrivateStack(int *spill) : priv((PrivInt *)spill), sp(0) {}
     RTX_DI void push(int x) { priv[sp++] = x; }
     RTX_DI int pop() { return priv[--sp]; }
 };

@@ -372,7 +372,8 @@ RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue 
         reportBounce(poll, ne, np, ns, cnt->done[esel ^ 1]);
     }
     HybridStack<kTraceBlock, LDS> st;
-    st.lds = stackMem + threadIdx.x;
+    int spill[HybridStack<kTraceBlock, LDS>::kSpill];
+    st.bind(stackMem + threadIdx.x, spill);
     VisitCount vc, vk[3];
     // the queues' arrays as plain pointers, read in per-queue branches (a
     // select among them becomes a table in scratch and flat loads)
@@ -472,7 +473,8 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceM(MultiArgs ma, Queue qe, i
     if (blockIdx.x == 0 && poll)
         for (uint32_t k = threadIdx.x; k < ma.numSlots; k += kTraceBlock) poll[4 + k] = ma.slotAlive[k];
     HybridStack<kTraceBlock, kLdsStack> st;
-    st.lds = stackMem + threadIdx.x;
+    int spill[HybridStack<kTraceBlock, kLdsStack>::kSpill];
+    st.bind(stackMem + threadIdx.x, spill);
     VisitCount vc;
     LaneScene ls{};
     const float4 *eOrg = qe.org, *eDir = qe.dir, *pOrg = qp.org, *pDir = qp.dir, *sOrg = qs.org, *sDir = qs.dir;
@@ -892,7 +894,8 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
                         // The reference returns from NEE here if the shadow ray is
                         // unoccluded (raytrace.rgen:653-655), skipping the probe's draws:
                         // resolve visibility now to keep the random stream in step.
-                        PrivateStack ps;
+                        int spill[kStackCapacity];
+                        PrivateStack ps(spill);
                         VisitCount vc0;
                         TravRay sr = makeTravRay(h.worldPos, lightDir);
                         float tm = lightDist * (1.0f - 0.0001f), bu, bv;
@@ -1302,7 +1305,8 @@ __global__ __launch_bounds__(kTraceBlock) void kQueryP(DevScene S, const rtx_ray
                                                       uint32_t *fetch) {
     __shared__ int stackMem[kLdsStack * kTraceBlock];
     HybridStack<kTraceBlock, kLdsStack> st;
-    st.lds = stackMem + threadIdx.x;
+    int spill[HybridStack<kTraceBlock, kLdsStack>::kSpill];
+    st.bind(stackMem + threadIdx.x, spill);
     VisitCount vc;
     persistentTrace<ANY ? kHitAny : kHitClosest, false, true>( // alpha-tested triangles with randomUInt 0
         S, n, fetch, st, vc,
