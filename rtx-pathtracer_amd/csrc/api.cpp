@@ -1708,7 +1708,8 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) try {
     auto &L = ctx->icAllocs;
     ic.maxEntries = maxEntries;
     ic.nb = 1024;
-    while (ic.nb < 4u * maxEntries) ic.nb <<= 1;
+    // records file up to 27 items (about 12 on surfaces): 16 buckets per record, up to 2^26
+    while (ic.nb < 16u * maxEntries && ic.nb < (1u << 26)) ic.nb <<= 1;
     ic.itemCap = 27u * maxEntries;
     ic.bucketBits = 0;
     while ((1u << ic.bucketBits) < ic.nb) ic.bucketBits++;
@@ -1727,7 +1728,6 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) try {
     if (e == hipSuccess) e = devAlloc(&ic.slotIdx, ic.itemCap, L);
     if (e == hipSuccess) e = devAlloc(&ic.slotIdxB, ic.itemCap, L);
     if (e == hipSuccess) e = devAlloc(&ic.sortHist, (size_t)((ic.itemCap + kSortBlock - 1) / kSortBlock) * kDigits, L);
-    if (e == hipSuccess) e = devAlloc(&ic.slotItems, ic.itemCap, L);
     if (e == hipSuccess) e = devAlloc(&ic.frame, 8, L);
     if (e == hipSuccess) e = devAlloc(&ic.cand, 2 * (size_t)maxEntries, L);
     if (e == hipSuccess) e = devAlloc(&ic.jobs, 2 * (size_t)ic.jobCap, L);

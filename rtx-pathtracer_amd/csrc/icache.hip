@@ -357,10 +357,10 @@ __global__ __launch_bounds__(256) void kIcIndexClear(IcBuffers ic, float c0) {
 // of them despite rounding; at most 3 cells per axis.
 RTX_DI void icCells(const rtx_sphere &sp, float c0, int &level, int32_t lo[3], int32_t hi[3]) {
     const float r = sp.radius;
-    level = icLevel(c0, r);
-    const float cs = icCellSize(c0, level);
     const float m = r * 1.0001f + 1e-6f * (1.0f + fmaxf(fabsf(sp.center[0]), fmaxf(fabsf(sp.center[1]),
                                                                                   fabsf(sp.center[2]))));
+    level = icLevel(c0, m);
+    const float cs = icCellSize(c0, level);
     for (int a = 0; a < 3; ++a) {
         lo[a] = icCellCoord(sp.center[a] - m, cs);
         hi[a] = icCellCoord(sp.center[a] + m, cs);
@@ -368,40 +368,51 @@ RTX_DI void icCells(const rtx_sphere &sp, float c0, int &level, int32_t lo[3], i
     }
 }
 
-// Pass 1 counts the (record, cell) items per bucket; pass 2 writes record
-// i's items into its 27 fixed slots in cell order (unused slots keyed nb),
-// so that a stable sort by bucket leaves each bucket's items in record order
-// and lookups add the same records in the same order every time.
+RTX_DI bool icFiled(const IcBuffers &ic, uint32_t i, uint32_t count, rtx_sphere &sp) {
+    if (i >= count) return false;
+    sp = ic.spheres[i];
+    return sp.radius > 0.0f;
+}
+
+// The occupied levels (meta[0]), which the bucket ranges depend on.
+__global__ __launch_bounds__(256) void kIcIndexLevels(IcBuffers ic, float c0) {
+    const uint32_t count = min(ic.hdr->nextCacheSlot, ic.hdr->maxCaches);
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    rtx_sphere sp;
+    if (i >= ic.maxEntries || !icFiled(ic, i, count, sp)) return;
+    int level;
+    int32_t lo[3], hi[3];
+    icCells(sp, c0, level, lo, hi);
+    atomicOr(&ic.meta[0], 1u << level);
+}
+
+// Pass 1 counts the (record, bucket) items per bucket; pass 2 writes record
+// i's bucket keys into its 27 fixed slots in cell order (unused slots keyed
+// nb), so that a stable sort by bucket leaves each bucket's items in record
+// order and lookups add the same records in the same order every time.  A
+// cell whose bucket an earlier cell of the record already has files nothing.
 template <bool SCATTER> __global__ __launch_bounds__(256) void kIcIndexFill(IcBuffers ic, float c0) {
     const uint32_t count = min(ic.hdr->nextCacheSlot, ic.hdr->maxCaches);
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= ic.maxEntries) return;
+    const uint32_t mask = ic.meta[0];
     uint32_t k = 0;
-    const rtx_sphere sp = ic.spheres[i];
-    if (i < count && sp.radius > 0.0f) {
+    uint32_t filed[27];
+    rtx_sphere sp;
+    if (icFiled(ic, i, count, sp)) {
         int level;
         int32_t lo[3], hi[3];
         icCells(sp, c0, level, lo, hi);
-        if (!SCATTER) atomicOr(&ic.meta[0], 1u << level);
         for (int32_t x = lo[0]; x <= hi[0]; ++x)
             for (int32_t y = lo[1]; y <= hi[1]; ++y)
                 for (int32_t z = lo[2]; z <= hi[2]; ++z) {
-                    const uint64_t key = icKey(level, x, y, z);
-                    const uint32_t b = icBucket(key, ic.nb);
-                    if (!SCATTER) {
-                        atomicAdd(&ic.bucketCount[b], 1u);
-                    } else {
-                        IcItem it;
-                        it.center[0] = sp.center[0];
-                        it.center[1] = sp.center[1];
-                        it.center[2] = sp.center[2];
-                        it.radius = sp.radius;
-                        it.entry = i;
-                        it.pad = 0;
-                        it.key = key;
-                        ic.slotKeys[27 * (size_t)i + k] = b;
-                        ic.slotItems[27 * (size_t)i + k] = it;
-                    }
+                    const uint32_t b = icBucket(icKey(level, x, y, z), ic.nb, mask, level);
+                    bool dup = false;
+                    for (uint32_t q = 0; q < k; ++q) dup |= filed[q] == b;
+                    if (dup) continue;
+                    filed[k] = b;
+                    if (!SCATTER) atomicAdd(&ic.bucketCount[b], 1u);
+                    else ic.slotKeys[27 * (size_t)i + k] = b;
                     ++k;
                 }
     }
@@ -411,10 +422,26 @@ template <bool SCATTER> __global__ __launch_bounds__(256) void kIcIndexFill(IcBu
     }
 }
 
-// items[j] = the item of the j-th slot in bucket order, for the meta[1] items in use.
+// items[j] = the item of the j-th slot in bucket order (record order[j] / 27),
+// for the meta[1] items in use.
 __global__ __launch_bounds__(256) void kIcIndexGather(IcBuffers ic, const uint32_t *order) {
     const uint32_t n = ic.meta[1];
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) ic.items[j] = ic.slotItems[order[j]];
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+        const uint32_t r = order[j] / 27u;
+        const rtx_sphere sp = ic.spheres[r];
+        const rtx_cache_data &e = ic.entries[r];
+        IcItem it;
+        for (int a = 0; a < 3; ++a) {
+            it.center[a] = sp.center[a];
+            it.normal[a] = e.normal[a];
+            it.color[a] = e.color[a];
+        }
+        it.radius = sp.radius;
+        it.harmonicR = e.harmonicR;
+        it.entry = r;
+        it.pad[0] = it.pad[1] = it.pad[2] = it.pad[3] = 0u;
+        ic.items[j] = it;
+    }
 }
 
 __global__ __launch_bounds__(1024) void kIcIndexScan(IcBuffers ic) {
@@ -465,6 +492,7 @@ void launchIcInit(hipStream_t s, IcBuffers ic) {
 }
 void launchIcIndex(hipStream_t s, IcBuffers ic, float c0) {
     hipLaunchKernelGGL(kIcIndexClear, dim3(blocksFor(ic.nb, 256)), dim3(256), 0, s, ic, c0);
+    hipLaunchKernelGGL(kIcIndexLevels, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
     hipLaunchKernelGGL(kIcIndexFill<false>, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
     hipLaunchKernelGGL(kIcIndexScan, dim3(1), dim3(1024), 0, s, ic);
     hipLaunchKernelGGL(kIcIndexFill<true>, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);

@@ -183,19 +183,21 @@ void launchGuideUpdate(hipStream_t s, GuideFitBuffers g, const rtx_directional_d
 void launchGuideToVmm(hipStream_t s, GuideFitBuffers g, bool parallax, rtx_vmm_theta_wide *vmmsOut);
 
 // ------------------------------------------------------ irradiance cache
-// One (record, grid cell) pair of the lookup index, 32 bytes: the record's
-// sphere rides along so that the key and containment tests of a lookup need
-// no second load; only records that contain the point read their entry.
-// (Carrying the whole entry as well, 96 bytes, measured slower: every item a
-// lookup passes over then costs three times the cache footprint.)
+// One (record, grid cell) pair of the lookup index, 64 bytes.  A lookup's
+// scan reads the first 16 (the sphere); a record whose sphere contains the
+// point then reads the next 32 (normal, harmonic radius, colour) from the
+// same cache line, and its gradients, when they are used, from its entry.
+// The item is the same for all of a record's cells.
 struct alignas(16) IcItem {
     float center[3];
     float radius;
-    uint64_t key;
+    float normal[3];
+    float harmonicR;
+    float color[3];
     uint32_t entry;
-    uint32_t pad;
+    uint32_t pad[4];
 };
-static_assert(sizeof(IcItem) == 32, "index item layout");
+static_assert(sizeof(IcItem) == 64, "index item layout");
 
 // What lookups read (dev_icache.h).
 struct DevIC {
@@ -204,6 +206,7 @@ struct DevIC {
     const uint32_t *bucketStart; // nb + 1
     const IcItem *items;
     const uint32_t *meta;        // [0] occupied levels, [1] items, [2] level-0 cell size (float bits)
+                                 // [3] slots of the records in use
     uint32_t nb;
 };
 
@@ -216,12 +219,11 @@ struct IcBuffers {
     uint32_t nb, itemCap;
     uint32_t *bucketCount, *bucketStart, *cursor, *meta;
     IcItem *items;
-    // the index build's sort: 27 fixed slots per record (bucket keys and
-    // items, unused slots keyed nb); (key, slot) pairs are sorted stably by
-    // bucket (keys, slot ids and their ping-pong copies, digit histograms),
-    // then the items gathered into `items`
+    // the index build's sort: 27 fixed slots per record (bucket keys,
+    // unused slots keyed nb); (key, slot) pairs are sorted stably by bucket
+    // (keys, slot ids and their ping-pong copies, digit histograms), then
+    // each sorted slot's item is written from its record (slot / 27)
     uint32_t *slotKeys, *slotKeysB, *slotIdx, *slotIdxB, *sortHist;
-    IcItem *slotItems;
     uint32_t bucketBits; // log2(nb)
     // frame: [0] update elections, [1] creation candidates, [2] free slots at
     // frame start, [3] update jobs, [4] jobs
