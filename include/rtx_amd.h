@@ -690,7 +690,11 @@ typedef struct rtx_nrc_config {
                                     its target completed there by the cache's own
                                     prediction (self-training, Mueller et al. 2021);
                                     0: it runs to its end (1)                     */
-    uint32_t reserved;
+    float emaDecay;            /* inference reads an exponential moving average
+                                  of the parameters, e = d e + (1 - d) p after
+                                  every Adam step (from e = p at creation and at
+                                  rtx_nrc_set_params); 0: the parameters
+                                  themselves (0.99; in [0, 1))                    */
 } rtx_nrc_config;
 
 /* One cache query: the vertex's position, the direction towards the previous
@@ -727,10 +731,11 @@ int rtx_nrc_create(rtx_ctx *ctx, const rtx_nrc_config *config);
 int rtx_nrc_destroy(rtx_ctx *ctx);
 int rtx_nrc_get_info(rtx_ctx *ctx, rtx_nrc_info *info);
 /* FP32 master parameters: the table [level][entry][feature], then the MLP.
- * set_params restarts Adam. */
+ * set_params restarts Adam and the moving average (at the parameters set). */
 int rtx_nrc_get_params(rtx_ctx *ctx, float *grid, size_t gridCount, float *mlp, size_t mlpCount);
 int rtx_nrc_set_params(rtx_ctx *ctx, const float *grid, size_t gridCount, const float *mlp, size_t mlpCount);
-/* The cache's RGB prediction for n queries (rgb: 3 floats each). */
+/* The cache's RGB prediction for n queries (rgb: 3 floats each), from the
+ * parameters' moving average (rtx_nrc_config.emaDecay). */
 int rtx_nrc_infer(rtx_ctx *ctx, const rtx_nrc_input *in, size_t n, float *rgb);
 /* The loss gradients of n <= trainBatch rows against targets (3 floats each)
  * without changing the parameters; grid/mlp as in get_params, either may be

@@ -1980,6 +1980,7 @@ void rtx_nrc_config_defaults(rtx_nrc_config *c) {
     c->terminateBounces = 2;
     c->trainingPaths = 65536;
     c->trainSuffixBounces = 1;
+    c->emaDecay = 0.99f;
     for (int a = 0; a < 3; ++a) {
         c->sceneMin[a] = -1.0f;
         c->sceneMax[a] = 1.0f;
@@ -2001,6 +2002,7 @@ int rtx_nrc_create(rtx_ctx *ctx, const rtx_nrc_config *c) try {
     if (!(c->learningRate > 0.0f) || !(c->beta1 >= 0.0f && c->beta1 < 1.0f) || !(c->beta2 >= 0.0f && c->beta2 < 1.0f) ||
         !(c->epsilon >= 0.0f))
         return fail(ctx, RTX_E_INVALID, "Adam: learningRate > 0, beta1 and beta2 in [0, 1), epsilon >= 0");
+    if (!(c->emaDecay >= 0.0f && c->emaDecay < 1.0f)) return fail(ctx, RTX_E_INVALID, "emaDecay must be in [0, 1)");
     for (int a = 0; a < 3; ++a)
         if (!(c->sceneMax[a] > c->sceneMin[a])) return fail(ctx, RTX_E_INVALID, "sceneMax must exceed sceneMin");
     CHECK_HIP(ctx, hipSetDevice(ctx->device));
@@ -2033,6 +2035,16 @@ int rtx_nrc_create(rtx_ctx *ctx, const rtx_nrc_config *c) try {
     CHECK_HIP(ctx, devAlloc(&nb.gridBf16, (size_t)c->levels * T, L));
     CHECK_HIP(ctx, devAlloc(&nb.wFwd, (size_t)kNrcLayers * kNrcWidth * kNrcWidth, L));
     CHECK_HIP(ctx, devAlloc(&nb.wBwd, (size_t)kNrcLayers * kNrcWidth * kNrcWidth, L));
+    nb.emaDecay = c->emaDecay;
+    if (nb.emaDecay > 0.0f) {
+        CHECK_HIP(ctx, devAlloc(&nb.ema, total, L));
+        CHECK_HIP(ctx, devAlloc(&nb.gridBf16Ema, (size_t)c->levels * T, L));
+        CHECK_HIP(ctx, devAlloc(&nb.wFwdEma, (size_t)kNrcLayers * kNrcWidth * kNrcWidth, L));
+    } else {
+        nb.ema = nb.params;
+        nb.gridBf16Ema = nb.gridBf16;
+        nb.wFwdEma = nb.wFwd;
+    }
     CHECK_HIP(ctx, devAlloc(&nb.acts, (size_t)kNrcLayers * nb.rowsCap * 64, L));
     CHECK_HIP(ctx, devAlloc(&nb.dz, (size_t)kNrcLayers * nb.rowsCap * 64, L));
     CHECK_HIP(ctx, devAlloc(&nb.pred, (size_t)nb.rowsCap * 4, L));
@@ -2042,6 +2054,8 @@ int rtx_nrc_create(rtx_ctx *ctx, const rtx_nrc_config *c) try {
     hipStream_t s = ctx->stream;
     CHECK_HIP(ctx, hipMemsetAsync(nb.wFwd, 0, (size_t)kNrcLayers * kNrcWidth * kNrcWidth * 2, s));
     CHECK_HIP(ctx, hipMemsetAsync(nb.wBwd, 0, (size_t)kNrcLayers * kNrcWidth * kNrcWidth * 2, s));
+    if (nb.wFwdEma != nb.wFwd)
+        CHECK_HIP(ctx, hipMemsetAsync(nb.wFwdEma, 0, (size_t)kNrcLayers * kNrcWidth * kNrcWidth * 2, s));
     CHECK_HIP(ctx, hipMemsetAsync(nb.acts, 0, (size_t)kNrcLayers * nb.rowsCap * 64 * 2, s));
     CHECK_HIP(ctx, hipMemsetAsync(nb.dz, 0, (size_t)kNrcLayers * nb.rowsCap * 64 * 2, s));
     CHECK_HIP(ctx, hipMemsetAsync(ctx->nrcLoss, 0, sizeof(float), s));
@@ -2065,7 +2079,7 @@ int rtx_nrc_create(rtx_ctx *ctx, const rtx_nrc_config *c) try {
     ctx->nrc = nb;
     int rc = nrcResetOptimizer(ctx);
     if (rc != RTX_OK) return rc;
-    launchNrcPackWeights(s, ctx->nrc);
+    CHECK_HIP(ctx, launchNrcEmaReset(s, ctx->nrc));
     CHECK_HIP(ctx, hipGetLastError());
     CHECK_HIP(ctx, hipStreamSynchronize(s));
     ctx->nrcCfg = *c;
@@ -2145,7 +2159,7 @@ int rtx_nrc_set_params(rtx_ctx *ctx, const float *grid, size_t gridCount, const 
     }
     int rc = nrcResetOptimizer(ctx);
     if (rc != RTX_OK) return rc;
-    launchNrcPackWeights(ctx->stream, ctx->nrc);
+    CHECK_HIP(ctx, launchNrcEmaReset(ctx->stream, ctx->nrc));
     CHECK_HIP(ctx, hipGetLastError());
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RTX_OK;

@@ -45,6 +45,13 @@ struct NrcBuffers {
     uint32_t *gridBf16;         // [levels][T] two bf16 features packed
     uint16_t *wFwd;             // [layer][out][64] bf16 (out-major: forward B operand)
     uint16_t *wBwd;             // [layer][in][64] bf16 (in-major: backward B operand; layer 5 out padded to 32)
+    // inference's parameters: their moving average (FP32, from 0 at the
+    // optimizer's reset) and its bias-corrected bf16 copies; with emaDecay 0
+    // these alias params, gridBf16 and wFwd
+    float *ema;
+    uint32_t *gridBf16Ema;
+    uint16_t *wFwdEma;
+    float emaDecay;
     // training batch work (rows = trainBatch)
     uint16_t *acts;             // [6][rows][64] bf16: the input of every layer
     uint16_t *dz;               // [6][rows][64] bf16: d loss / d pre-activation of every layer
@@ -77,6 +84,8 @@ void launchNrcGradients(hipStream_t s, NrcBuffers nb, const float4 *in, const fl
                         float invCount, float *lossOut, int cuCount);
 void launchNrcAdam(hipStream_t s, NrcBuffers nb);
 void launchNrcPackWeights(hipStream_t s, NrcBuffers nb);
+// The moving average restarted at the parameters (then pack the weights).
+hipError_t launchNrcEmaReset(hipStream_t s, NrcBuffers nb);
 void launchNrcZeroGrads(hipStream_t s, NrcBuffers nb);
 void launchNrcZero(hipStream_t s, float *p, size_t n); // a fill kernel
 

@@ -396,9 +396,14 @@ __global__ __launch_bounds__(256) void kNrcGridGrad(NrcBuffers nb, const float4 
 
 // ----------------------------------------------------------------- Adam
 // Kingma and Ba (2015), bias-corrected: p -= lr m_hat / (sqrt(v_hat) + eps).
-// Clears the gradient and refreshes the table's bf16 copy.
-__global__ __launch_bounds__(256) void kNrcAdam(NrcBuffers nb, size_t total, float c1, float c2) {
+// Clears the gradient and refreshes the table's bf16 copy; then the moving
+// average inference reads, e = d e + (1 - d) p from e = 0, whose table copy
+// is bias-corrected the same way: e / (1 - d^t) (c3) after t steps.
+__global__ __launch_bounds__(256) void kNrcAdam(NrcBuffers nb, size_t total, float c1, float c2, float c3) {
     uint16_t *tab = reinterpret_cast<uint16_t *>(nb.gridBf16);
+    uint16_t *tabEma = reinterpret_cast<uint16_t *>(nb.gridBf16Ema);
+    const bool ema = nb.ema != nb.params;
+    const float d = nb.emaDecay;
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
         const float g = nb.grads[i];
         const float m = nb.beta1 * nb.adamM[i] + (1.0f - nb.beta1) * g;
@@ -409,13 +414,22 @@ __global__ __launch_bounds__(256) void kNrcAdam(NrcBuffers nb, size_t total, flo
         nb.params[i] = p;
         nb.grads[i] = 0.0f;
         if (i < nb.gridParams) tab[i] = f2bf(p);
+        if (ema) {
+            const float e = d * nb.ema[i] + (1.0f - d) * p;
+            nb.ema[i] = e;
+            if (i < nb.gridParams) tabEma[i] = f2bf(e * c3);
+        }
     }
 }
 
 // The MLP's bf16 copies: forward [layer][out][in], backward [layer][in][out]
 // (the output layer's 16 padded outputs extend to 32 zero columns there),
 // and the table's bf16 copy when all is set (after rtx_nrc_set_params).
-__global__ __launch_bounds__(256) void kNrcPack(NrcBuffers nb, int withTable) {
+// The moving average's copies are ema * c3, or the parameters themselves
+// before the first step (c3 = 0).
+__global__ __launch_bounds__(256) void kNrcPack(NrcBuffers nb, int withTable, float c3) {
+    const float *emaSrc = c3 > 0.0f ? nb.ema : nb.params;
+    const float emaScale = c3 > 0.0f ? c3 : 1.0f;
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i < (size_t)kNrcMlpParams) {
         const uint32_t layer = (uint32_t)(i / (kNrcWidth * kNrcWidth)), rem = (uint32_t)(i % (kNrcWidth * kNrcWidth));
@@ -423,10 +437,14 @@ __global__ __launch_bounds__(256) void kNrcPack(NrcBuffers nb, int withTable) {
         const uint16_t v = f2bf(nb.params[nb.gridParams + i]);
         nb.wFwd[layer * kWLayer + o * kNrcWidth + k] = v;
         nb.wBwd[layer * kWLayer + k * kNrcWidth + o] = v;
+        if (nb.ema != nb.params)
+            nb.wFwdEma[layer * kWLayer + o * kNrcWidth + k] = f2bf(emaSrc[nb.gridParams + i] * emaScale);
     }
     if (withTable)
-        for (size_t j = i; j < nb.gridParams; j += (size_t)gridDim.x * 256)
+        for (size_t j = i; j < nb.gridParams; j += (size_t)gridDim.x * 256) {
             reinterpret_cast<uint16_t *>(nb.gridBf16)[j] = f2bf(nb.params[j]);
+            if (nb.ema != nb.params) reinterpret_cast<uint16_t *>(nb.gridBf16Ema)[j] = f2bf(emaSrc[j] * emaScale);
+        }
 }
 
 __global__ __launch_bounds__(256) void kNrcZero(float *p, size_t n) {
@@ -439,7 +457,10 @@ void launchNrcInfer(hipStream_t s, const NrcBuffers &nb, const float4 *in, uint3
                     float4 *out, const float4 *aux, float4 *pathOut, int blocks, NrcSelfTrain st) {
     if (n == 0) return;
     const int tiles = (int)((n + kNrcRowsPerBlock - 1) / kNrcRowsPerBlock);
-    hipLaunchKernelGGL(kNrcForward<false>, dim3(tiles < blocks ? tiles : blocks), dim3(kBlock), 0, s, nb, in, n,
+    NrcBuffers v = nb; // inference reads the moving average's copies
+    v.gridBf16 = nb.gridBf16Ema;
+    v.wFwd = nb.wFwdEma;
+    hipLaunchKernelGGL(kNrcForward<false>, dim3(tiles < blocks ? tiles : blocks), dim3(kBlock), 0, s, v, in, n,
                        count, out, aux, pathOut, st);
 }
 
@@ -465,12 +486,21 @@ void launchNrcAdam(hipStream_t s, NrcBuffers nb) {
     const double t = (double)nb.step;
     const float c1 = (float)(1.0 / (1.0 - std::pow((double)nb.beta1, t)));
     const float c2 = (float)(1.0 / (1.0 - std::pow((double)nb.beta2, t)));
-    hipLaunchKernelGGL(kNrcAdam, dim3(4096), dim3(256), 0, s, nb, total, c1, c2);
-    hipLaunchKernelGGL(kNrcPack, dim3((kNrcMlpParams + 255) / 256), dim3(256), 0, s, nb, 0);
+    const float c3 = nb.emaDecay > 0.0f ? (float)(1.0 / (1.0 - std::pow((double)nb.emaDecay, t))) : 1.0f;
+    hipLaunchKernelGGL(kNrcAdam, dim3(4096), dim3(256), 0, s, nb, total, c1, c2, c3);
+    hipLaunchKernelGGL(kNrcPack, dim3((kNrcMlpParams + 255) / 256), dim3(256), 0, s, nb, 0, c3);
 }
 
 void launchNrcPackWeights(hipStream_t s, NrcBuffers nb) {
-    hipLaunchKernelGGL(kNrcPack, dim3(4096), dim3(256), 0, s, nb, 1);
+    const double t = (double)nb.step;
+    const float c3 = nb.emaDecay > 0.0f && nb.step > 0 ? (float)(1.0 / (1.0 - std::pow((double)nb.emaDecay, t))) : 0.0f;
+    hipLaunchKernelGGL(kNrcPack, dim3(4096), dim3(256), 0, s, nb, 1, c3);
+}
+
+hipError_t launchNrcEmaReset(hipStream_t s, NrcBuffers nb) {
+    if (nb.ema != nb.params) launchNrcZero(s, nb.ema, nb.gridParams + kNrcMlpParams);
+    launchNrcPackWeights(s, nb); // nb.step is 0 here: the copies are the parameters'
+    return hipGetLastError();
 }
 
 void launchNrcZero(hipStream_t s, float *p, size_t n) {

@@ -245,7 +245,7 @@ class NrcConfig(C.Structure):
                 ("maxResolution", C.c_float), ("learningRate", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("epsilon", C.c_float), ("trainBatch", C.c_uint32), ("trainSteps", C.c_uint32),
                 ("terminateBounces", C.c_uint32), ("trainingPaths", C.c_uint32), ("sceneMin", F3),
-                ("sceneMax", F3), ("seed", C.c_uint64), ("trainSuffixBounces", C.c_uint32), ("reserved", C.c_uint32)]
+                ("sceneMax", F3), ("seed", C.c_uint64), ("trainSuffixBounces", C.c_uint32), ("emaDecay", C.c_float)]
 
 
 class NrcInput(C.Structure):

@@ -8,6 +8,7 @@ FP32 restatement (tests/nrc_reference.py), through the C ABI:
 * the loss and every parameter gradient (the MLP's six matrices and the
   table) against autograd;
 * one Adam step against its formula;
+* inference reads the bias-corrected moving average of the parameters;
 * training on a smooth function lowers the loss tenfold;
 * in a render, training paths are the plain tracer's own paths (a frame
   whose every path trains equals the plain render bit for bit), and a
@@ -136,10 +137,45 @@ def test_adam_step_matches_formula(cache):
         assert np.array_equal(p1[g == 0], p0[g == 0])
 
 
+def test_inference_reads_the_moving_average(cache):
+    """After t Adam steps from rtx_nrc_set_params, inference evaluates the
+    bias-corrected moving average of the parameters after each step,
+    sum_j (1 - d) d^(t - j) p_j / (1 - d^t), with d the default emaDecay."""
+    d = 0.99
+    rng = np.random.default_rng(6)
+    _random_params(cache, rng)
+    n = 2048
+    x = random_inputs(n, LO, HI, rng)
+    target = rng.uniform(0.0, 2.0, (n, 3)).astype(np.float32)
+    q = random_inputs(4000, LO, HI, rng)
+    ema_g, ema_m, traj = 0.0, 0.0, []
+    for t in range(1, 4):
+        cache.nrc_train(x, target)
+        g, m = cache.nrc_params()
+        traj.append((g.copy(), m.copy()))
+        ema_g = d * ema_g + (1 - d) * g.astype(np.float64)
+        ema_m = d * ema_m + (1 - d) * m.astype(np.float64)
+    corr = 1.0 / (1.0 - d ** 3)
+    ref = Reference(cache.nrc_info())
+    got = cache.nrc_infer(q)
+    qt = torch.from_numpy(q)
+    want = ref.forward(qt, torch.from_numpy((ema_g * corr).astype(np.float32)),
+                       torch.from_numpy((ema_m * corr).astype(np.float32))).detach().numpy()
+    last = ref.forward(qt, torch.from_numpy(traj[-1][0]), torch.from_numpy(traj[-1][1])).detach().numpy()
+    err = np.abs(got - want).max() / np.abs(want).max()
+    err_last = np.abs(got - last).max() / np.abs(last).max()
+    log(f"nrc moving average: max |gpu - torch(ema)| / max = {err:.2e}; against the last parameters {err_last:.2e}")
+    assert err < 1e-2
+    assert np.corrcoef(got.reshape(-1), want.reshape(-1))[0, 1] > 0.9999
+
+
 def test_training_lowers_the_loss(cache):
     """200 Adam steps on a smooth radiance-like function of position, normal
-    and albedo lower the loss at least tenfold."""
-    cache.nrc_create(LO, HI, seed=7)
+    and albedo lower the loss at least tenfold, and the trained parameters
+    fit it (inference from the parameters themselves: emaDecay 0; a moving
+    average over a window of about 100 steps would still hold parameters
+    from the steep start of training)."""
+    cache.nrc_create(LO, HI, seed=7, emaDecay=0.0)
     rng = np.random.default_rng(5)
 
     def f(x):
