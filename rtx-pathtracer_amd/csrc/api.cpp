@@ -1152,6 +1152,15 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     for (uint64_t f0 = 0; f0 < total; f0 += frameLen) {
         const uint64_t fEnd = std::min(total, f0 + frameLen);
         if (icFrames) timed(kClsShade, [&] { launchIcFrameBegin(s, ctx->ic); });
+        // every training slot starts the frame without a target (w = 0) and
+        // with zero inputs: a slot no path of this frame claims still enters
+        // the training batch, where its weight gradient is its (zero) delta
+        // times its inputs' activations, so those must be finite
+        if (nrc)
+            timed(kClsShade, [&] {
+                launchNrcZero(s, &ctx->pb.nrcTrainTarget[0].x, 4 * (size_t)nrcSlots);
+                launchNrcZero(s, &ctx->pb.nrcTrainIn[0].x, 16 * (size_t)nrcSlots);
+            });
         for (uint64_t first = f0; first < fEnd; first += batch) {
             BatchDesc b{};
             b.firstPath = first;
