@@ -42,7 +42,7 @@ int fail(rtx_ctx *ctx, int code, const std::string &msg) {
 
 template <class T> hipError_t devAlloc(T **p, size_t count, std::vector<void *> &list) {
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    hipError_t e = devMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
     if (e == hipSuccess) {
         list.push_back(q);
         *p = (T *)q;
@@ -462,7 +462,7 @@ int rtx_ctx_create(int device, rtx_ctx **out) try {
     if (!ctx) return RTX_E_MEMORY;
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&ctx->cnt, sizeof(Counters)) != hipSuccess || hipMalloc(&ctx->fc, sizeof(FrameConst)) != hipSuccess ||
+        devMalloc(&ctx->cnt, sizeof(Counters)) != hipSuccess || devMalloc(&ctx->fc, sizeof(FrameConst)) != hipSuccess ||
         hipEventCreate(&ctx->tStart) != hipSuccess || hipEventCreate(&ctx->tStop) != hipSuccess ||
         hipHostMalloc((void **)&ctx->pollRing, kPollSlots * 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         rtx_ctx_destroy(ctx);
@@ -845,8 +845,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         if (ctx->accum) (void)hipFree(ctx->accum);
         if (ctx->resolved) (void)hipFree(ctx->resolved);
         ctx->accum = ctx->resolved = nullptr;
-        CHECK_HIP(ctx, hipMalloc(&ctx->accum, npix * sizeof(float4)));
-        CHECK_HIP(ctx, hipMalloc(&ctx->resolved, npix * sizeof(float4)));
+        CHECK_HIP(ctx, devMalloc(&ctx->accum, npix * sizeof(float4)));
+        CHECK_HIP(ctx, devMalloc(&ctx->resolved, npix * sizeof(float4)));
         ctx->accW = p->width;
         ctx->accH = p->height;
     }
@@ -909,7 +909,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         if (ctx->estimate) (void)hipFree(ctx->estimate);
         ctx->estimate = nullptr;
         ctx->estW = ctx->estH = 0;
-        CHECK_HIP(ctx, hipMalloc(&ctx->estimate, npix * sizeof(float4)));
+        CHECK_HIP(ctx, devMalloc(&ctx->estimate, npix * sizeof(float4)));
         CHECK_HIP(ctx, hipMemsetAsync(ctx->estimate, 0, npix * sizeof(float4), ctx->stream));
         ctx->estW = p->width;
         ctx->estH = p->height;
@@ -918,7 +918,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
         if (ctx->candBuf) (void)hipFree(ctx->candBuf);
         ctx->candBuf = nullptr;
         ctx->candCap = 0;
-        CHECK_HIP(ctx, hipMalloc(&ctx->candBuf, bufPaths * kMaxNewCacheEntries * 2 * sizeof(float4)));
+        CHECK_HIP(ctx, devMalloc(&ctx->candBuf, bufPaths * kMaxNewCacheEntries * 2 * sizeof(float4)));
         ctx->candCap = bufPaths;
     }
     ctx->pb.cand = ctx->candBuf;
@@ -962,7 +962,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             if (ctx->dirData) (void)hipFree(ctx->dirData);
             ctx->dirData = nullptr;
             ctx->dirCap = 0;
-            CHECK_HIP(ctx, hipMalloc(&ctx->dirData, slots * sizeof(rtx_directional_data)));
+            CHECK_HIP(ctx, devMalloc(&ctx->dirData, slots * sizeof(rtx_directional_data)));
             ctx->dirCap = slots;
         }
         ctx->pb.dirData = ctx->dirData;
@@ -977,8 +977,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             ctx->keepRgb = nullptr;
             ctx->keepSig = nullptr;
             ctx->keepCap = 0;
-            CHECK_HIP(ctx, hipMalloc(&ctx->keepRgb, total * sizeof(float4)));
-            CHECK_HIP(ctx, hipMalloc(&ctx->keepSig, total * sizeof(uint2)));
+            CHECK_HIP(ctx, devMalloc(&ctx->keepRgb, total * sizeof(float4)));
+            CHECK_HIP(ctx, devMalloc(&ctx->keepSig, total * sizeof(uint2)));
             ctx->keepCap = total;
         }
         ctx->keptPaths = total;
@@ -1290,8 +1290,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
                 ctx->keepSub = nullptr;
                 ctx->keepSubSig = nullptr;
                 ctx->keepSubCap = 0;
-                CHECK_HIP(ctx, hipMalloc(&ctx->keepSub, need * sizeof(float4)));
-                CHECK_HIP(ctx, hipMalloc(&ctx->keepSubSig, need * sizeof(uint2)));
+                CHECK_HIP(ctx, devMalloc(&ctx->keepSub, need * sizeof(float4)));
+                CHECK_HIP(ctx, devMalloc(&ctx->keepSubSig, need * sizeof(uint2)));
                 ctx->keepSubCap = need;
             }
             ctx->keptJobs = numJobs;
@@ -1742,7 +1742,7 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) try {
     if (e == hipSuccess) e = devAlloc(&ic.jobs, 2 * (size_t)ic.jobCap, L);
     if (e == hipSuccess) e = devAlloc(&ic.jobRes, 10 * (size_t)ic.jobCap, L);
     if (e == hipSuccess) e = devAlloc(&ic.blockSums, ic.blockCap, L);
-    if (e == hipSuccess && !ctx->fcIC) e = hipMalloc(&ctx->fcIC, sizeof(FrameConst));
+    if (e == hipSuccess && !ctx->fcIC) e = devMalloc(&ctx->fcIC, sizeof(FrameConst));
     if (e != hipSuccess) {
         icFree(ctx);
         return fail(ctx, RTX_E_DEVICE, std::string("irradiance cache allocation: ") + hipGetErrorString(e));
@@ -1776,7 +1776,7 @@ int rtx_estimate_write(rtx_ctx *ctx, const float *rgba, uint32_t width, uint32_t
         if (ctx->estimate) (void)hipFree(ctx->estimate);
         ctx->estimate = nullptr;
         ctx->estW = ctx->estH = 0;
-        CHECK_HIP(ctx, hipMalloc(&ctx->estimate, npix * sizeof(float4)));
+        CHECK_HIP(ctx, devMalloc(&ctx->estimate, npix * sizeof(float4)));
         ctx->estW = width;
         ctx->estH = height;
     }

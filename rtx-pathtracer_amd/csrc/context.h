@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -12,6 +13,22 @@
 #include "nrc.h"
 
 using namespace rtx;
+
+// hipMalloc for the library's device buffers.  Developer aid: with
+// RTX_POISON_ALLOC set in the environment every new buffer is filled with
+// 0xFF bytes (NaN floats, -1 integers) before use, so that a read of memory
+// no kernel or copy has written shows up in the results instead of reading
+// the zeros that fresh pages happen to hold.
+inline hipError_t devMalloc(void **p, size_t bytes) {
+    static const bool poison = std::getenv("RTX_POISON_ALLOC") != nullptr;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess && poison) {
+        e = hipMemset(*p, 0xFF, bytes);
+        if (e == hipSuccess) e = hipDeviceSynchronize(); // before any stream's use of it
+    }
+    return e;
+}
+template <class T> hipError_t devMalloc(T **p, size_t bytes) { return devMalloc(reinterpret_cast<void **>(p), bytes); }
 
 struct rtx_scene {
     DevScene view{};

@@ -165,7 +165,7 @@ namespace {
 
 template <class T> hipError_t poolAlloc(rtx_server *sv, T **p, size_t count) {
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    hipError_t e = devMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
     if (e == hipSuccess) {
         sv->allocs.push_back(q);
         sv->poolBytes += std::max<size_t>(count, 1) * sizeof(T);
@@ -587,7 +587,7 @@ int rtx_server_create(rtx_ctx *ctx, const rtx_server_config *c, rtx_server **out
         q->seedDiv = q == &sv->qs || q == &sv->qp ? cfg.maxNumNEE : 1u;
     }
     void *fb = nullptr;
-    if ((e = hipMalloc(&fb, cfg.framebufferBytes)) != hipSuccess) return fail(e, "hipMalloc (framebuffers)");
+    if ((e = devMalloc(&fb, cfg.framebufferBytes)) != hipSuccess) return fail(e, "hipMalloc (framebuffers)");
     sv->allocs.push_back(fb);
     sv->fbBase = static_cast<char *>(fb);
     sv->arena.init(cfg.framebufferBytes);

@@ -260,3 +260,23 @@ def test_trained_cache_renders_close_to_plain(suffix):
     assert ni.steps == 256 * 4
     assert abs(m_nrc - m_ref) < 0.1 * m_ref
     assert e_nrc < 2 * e_plain
+
+
+def test_render_reads_no_unwritten_memory():
+    """The cache renders above again in a fresh process whose library fills
+    every new device buffer with 0xFF bytes (RTX_POISON_ALLOC): a read of
+    memory that no kernel or copy has written - such as a training slot that
+    no path of the frame claims - then brings NaN inputs or targets into
+    training and fails their image checks, where fresh zeroed pages would
+    hide it."""
+    import os
+    import subprocess
+    import sys
+    env = dict(os.environ, RTX_POISON_ALLOC="1")
+    args = [sys.executable, "-m", "pytest", __file__, "-q", "-s", "-m", "gpu", "-k", "renders_close_to_plain",
+            "-p", "no:cacheprovider"]
+    r = subprocess.run(args, env=env, capture_output=True, text=True, timeout=110)
+    for line in r.stdout.splitlines():
+        if "nrc render" in line or "passed" in line or "failed" in line:
+            log("poisoned: " + line.strip(". "))
+    assert r.returncode == 0, r.stdout[-4000:]
