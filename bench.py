@@ -176,9 +176,11 @@ def relmse_leg(rtxamd, abi, r, a):
     recording frames with a GPU EM update each inside the budget, then guided
     frames); reference `relmse_ref_spp` / 2 samples per half.  Spot room (a
     closed room lit only through a bright spot on its ceiling, the case
-    guiding is for): the same plain vs guided comparison at 640x360.  Each
-    method gets `relmse_seconds` of wall time (plates and spot room a tenth of
-    it)."""
+    guiding is for): the same plain vs guided comparison at 640x360, then the
+    learned cache alone and with guided sampling (the guide's 6 recording
+    frames and the cache's training inside the budget).  Each method gets
+    `relmse_seconds` of wall time on the room, a tenth of it on the plates
+    and half of it on the spot room."""
     pc = dict(maxDepth=12, enableMIS=1, enableNEE=1, usePowerHeuristic=1)
     RESET = abi.RTX_FRAME_RESET_ACCUM
 
@@ -298,6 +300,7 @@ def relmse_leg(rtxamd, abi, r, a):
     log(f"relmse plates: plain {e_plain:.4g}, guided {e_g:.4g}")
 
     # ---- spot room: plain vs guided where the light arrives through a bright spot
+    T = a.relmse_seconds / 2
     hs = rtxamd.HostScene.generate(abi.SCENE_SPOT, 1)
     ds = r.upload(hs)
     cam = hs.camera(W, H)
@@ -316,9 +319,31 @@ def relmse_leg(rtxamd, abi, r, a):
     e_g = relmse(r.read_image(W, H), ref, ref_var=var)
     spot["guided"] = {"spp": state["spp"] - 6, "seconds": round(sec, 3), "relmse": e_g,
                       "improvement": round(e_plain / e_g, 3)}
+    sinfo = ds.info()
+
+    def spot_nrc_call(first, guided):
+        if first:
+            if guided:
+                r.guiding_init(bounds, 8)
+                r.render(ds, rtxamd.frame_params(cam, W, H, 6, seed=1, flags=RESET, updateGuiding=1, useGuiding=1,
+                                                 **pc))
+            r.nrc_create(list(sinfo.boundsMin), list(sinfo.boundsMax), seed=5)
+            state["spp"] = 6 if guided else 0
+        r.render(ds, rtxamd.frame_params(cam, W, H, chunk, seed=1, sample_offset=state["spp"],
+                                         flags=abi.RTX_FRAME_NRC | (RESET if first else 0),
+                                         useGuiding=1 if guided else 0, **pc))
+        state["spp"] += chunk
+
+    for key, guided in (("learned_cache", False), ("guided_learned_cache", True)):
+        n, sec = equal_time(lambda first: spot_nrc_call(first, guided), T)
+        e_n = relmse(r.read_image(W, H), ref, ref_var=var)
+        spot[key] = {"spp": state["spp"] - (6 if guided else 0), "seconds": round(sec, 3), "relmse": e_n,
+                     "improvement": round(e_plain / e_n, 3), "adam_steps": r.nrc_info().steps}
+        r.nrc_destroy()
     ds.close()
     out["spot"] = spot
-    log(f"relmse spot room: plain {e_plain:.4g}, guided {e_g:.4g}")
+    log(f"relmse spot room: plain {e_plain:.4g}, guided {e_g:.4g}, learned {spot['learned_cache']['relmse']:.4g}, "
+        f"guided + learned {spot['guided_learned_cache']['relmse']:.4g}")
     return out
 
 

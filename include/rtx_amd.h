@@ -669,8 +669,10 @@ int rtx_trace_closest_noalloc(rtx_ctx *ctx, const rtx_scene *scene, const rtx_ra
  * training target.  After each frame
  * the cache takes trainSteps Adam steps on trainBatch of these targets each
  * (relative L2 loss, Mueller et al. 2021), so a frame reads the cache as the
- * previous frames left it.  Not combined with the irradiance cache, ADRRS,
- * guiding, splitting or several samples per frame (RTX_E_UNSUPPORTED). */
+ * previous frames left it.  Guided sampling (pc.useGuiding, with a guide
+ * trained by earlier recording frames) combines with it; the irradiance
+ * cache, ADRRS, guiding updates (pc.updateGuiding), splitting and several
+ * samples per frame do not (RTX_E_UNSUPPORTED). */
 typedef struct rtx_nrc_config {
     uint32_t levels;           /* hash-grid levels (16; at most 16)             */
     uint32_t log2TableSize;    /* entries per level, log2 (19)                  */

@@ -826,9 +826,10 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     const bool nrc = (p->flags & RTX_FRAME_NRC) != 0;
     if (nrc) {
         if (!ctx->nrcValid) return fail(ctx, RTX_E_STATE, "RTX_FRAME_NRC needs rtx_nrc_create first");
-        if (icFrames || splitting || p->pc.useGuiding || p->pc.updateGuiding || p->pc.samplesPerPixel > 1)
+        // guided sampling (useGuiding) combines with it; recording for the guide's update does not
+        if (icFrames || splitting || p->pc.updateGuiding || p->pc.samplesPerPixel > 1)
             return fail(ctx, RTX_E_UNSUPPORTED, "the learned radiance cache with the irradiance cache, ADRRS, "
-                                                "splitting, guiding or several samples per frame");
+                                                "splitting, guiding updates or several samples per frame");
     }
     if ((p->pc.useGuiding || p->pc.updateGuiding) && ctx->guide.numRegions == 0)
         return fail(ctx, RTX_E_STATE, "useGuiding/updateGuiding need guiding regions (rtx_guiding_set)");
