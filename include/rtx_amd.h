@@ -361,8 +361,11 @@ typedef struct rtx_frame_params {
 typedef struct rtx_frame_stats {
     uint64_t paths;         /* camera paths traced (= samples)            */
     uint64_t extendRays;    /* closest-hit rays (camera + continuation)   */
-    uint64_t shadowRays;    /* any-hit NEE shadow rays                    */
-    uint64_t probeRays;     /* closest-hit MIS probe rays                 */
+    uint64_t shadowRays;    /* any-hit rays: NEE shadow rays, and the
+                               occlusion rays of MIS probes that found their
+                               emitter first (scenes with at most 64
+                               emissive triangles and no environment map) */
+    uint64_t probeRays;     /* closest-hit MIS probe rays (other scenes)  */
     uint64_t nodeVisits;    /* BVH node fetches (RTX_FRAME_COUNT_VISITS)  */
     uint64_t primVisits;    /* primitive fetches (RTX_FRAME_COUNT_VISITS) */
     double msTotal;         /* device time of the whole render (HIP events) */

@@ -1064,7 +1064,10 @@ template <class R> struct Tracer {
             R pdfMat = sampleBSDF(m, wi, normal, frontFace, bsdfDir);
             if (pdfMat > R(0)) {
                 HitInfo<R> pinfo = trace(origin, bsdfDir, C::TMIN, C::TMAX);
-                primHash = (primHash ^ (kSigProbe ^ (uint32_t)pinfo.prim)) * 16777619u;
+                // the signature records the emitter the probe sees, or -1 (a miss or
+                // another surface first)
+                const bool seesLight = !pinfo.isMiss && S.mats[pinfo.matIndex].type == RTX_MAT_LIGHT;
+                primHash = (primHash ^ (kSigProbe ^ (uint32_t)(seesLight ? pinfo.prim : -1))) * 16777619u;
                 if (pinfo.isMiss) {
                     lightColor = pinfo.missColor;
                     pdfLights = C::INV_2PI / (R)S.lights.size();

@@ -190,9 +190,10 @@ int ensureBatchBuffers(rtx_ctx *ctx, size_t paths, uint32_t numNEE) {
         CHECK_HIP(ctx, devAlloc(&ctx->q[k].hit, cap, L));
         ctx->q[k].aux = nullptr;
     }
-    CHECK_HIP(ctx, devAlloc(&ctx->qs.id, cap * nee, L));
-    CHECK_HIP(ctx, devAlloc(&ctx->qs.org, cap * nee, L));
-    CHECK_HIP(ctx, devAlloc(&ctx->qs.dir, cap * nee, L));
+    // shadow queue: a shadow ray and a probe's occlusion ray per NEE slot
+    CHECK_HIP(ctx, devAlloc(&ctx->qs.id, 2 * cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->qs.org, 2 * cap * nee, L));
+    CHECK_HIP(ctx, devAlloc(&ctx->qs.dir, 2 * cap * nee, L));
     ctx->qs.aux = nullptr;
     ctx->qs.hit = nullptr;
     CHECK_HIP(ctx, devAlloc(&ctx->qp.id, cap * nee, L));
@@ -581,6 +582,20 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) try
         tris[3 * k + 1] = make_float4(v[3], v[4], v[5], 0.0f);
         tris[3 * k + 2] = make_float4(v[6], v[7], v[8], 0.0f);
     }
+    // the emissive triangles (in BVH order), for probe rays that look for
+    // their light first: kept only while few and none is alpha-tested
+    std::vector<float4> lightTris;
+    bool lightTrisOk = true;
+    for (size_t k = 0; k < bvh.primOrder.size() && lightTrisOk; ++k) {
+        const uint32_t p = bvh.primOrder[k];
+        const rtx_instance &in = d->instances[primRef[p].x];
+        const rtx_model_desc &md = d->models[in.modelIndex];
+        const int mat = md.vertices[md.indices[3 * (size_t)primRef[p].y]].materialIndex;
+        if (d->materials[mat].type != RTX_MAT_LIGHT) continue;
+        if (alpha[p] || lightTris.size() >= 3 * (size_t)kProbeLightMax) lightTrisOk = false;
+        for (int j = 0; j < 3; ++j) lightTris.push_back(tris[3 * k + j]);
+    }
+    if (!lightTrisOk) lightTris.clear();
     std::vector<rtx_vertex> verts;
     std::vector<uint32_t> idx, vbase(std::max<uint32_t>(d->modelCount, 1)), ibase(std::max<uint32_t>(d->modelCount, 1));
     for (uint32_t m = 0; m < d->modelCount; ++m) {
@@ -616,6 +631,8 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) try
     rtx_sphere *spD = nullptr;
     if (e == hipSuccess) e = upload(&nodes, reinterpret_cast<const float4 *>(bvh.nodes.data()), bvh.nodes.size() * 8);
     if (e == hipSuccess) e = upload(&trisD, tris.data(), tris.size());
+    float4 *lightTrisD = nullptr;
+    if (e == hipSuccess && lightTrisOk && !lightTris.empty()) e = upload(&lightTrisD, lightTris.data(), lightTris.size());
     // per world triangle: its instance and the global indices of its vertices
     std::vector<int4> triRef((size_t)numTris);
     for (size_t p = 0; p < (size_t)numTris; ++p) {
@@ -709,6 +726,9 @@ int rtx_scene_create(rtx_ctx *ctx, const rtx_scene_desc *d, rtx_scene **out) try
     S.triTable = ftD;
     S.emitTris = emitD;
     S.lightInst = linstD;
+    S.lightTris = lightTrisOk ? lightTrisD : nullptr;
+    S.numLightTris = lightTrisOk ? (uint32_t)(lightTris.size() / 3) : 0u;
+    S.lightFirstOk = lightTrisOk ? 1u : 0u;
     S.spheres = spD;
     S.numTris = (uint32_t)numTris;
     S.numSpheres = d->sphereCount;
@@ -1036,6 +1056,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.nrc = nrc ? 1u : 0u;
     fc.nrcDepth = nrc ? ctx->nrcCfg.terminateBounces + 1u : 0u;
     fc.nrcSuffix = nrc ? ctx->nrcCfg.trainSuffixBounces : 0u;
+    fc.probeFirstLight = 1u; // taken where the scene has few emitters and no environment map
     fc.nrcTrainEvery = nrcEvery;
     fc.nrcTrainSlots = nrcSlots;
     CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));

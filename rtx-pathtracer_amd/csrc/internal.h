@@ -34,6 +34,14 @@ struct DevScene {
     // the table would exceed kEmitTableBytes (then the chain is walked).
     const float4 *emitTris;
     const rtx_instance *lightInst;
+    // The emissive triangles, world space, 3 float4 each as in `tris` (w of
+    // the first: the primitive id), for MIS probe rays that look for their
+    // light before they trace (FrameConst::probeFirstLight); lightFirstOk is
+    // 0 when there are more than kProbeLightMax of them or one is
+    // alpha-tested (emissive spheres are found among `spheres`).
+    const float4 *lightTris;
+    uint32_t numLightTris;
+    uint32_t lightFirstOk;
     const rtx_sphere *spheres;
     uint32_t numTris;
     uint32_t numSpheres;
@@ -52,6 +60,11 @@ struct DevScene {
 };
 
 constexpr size_t kEmitTableBytes = size_t(256) << 20;
+// Emissive triangles up to which probe rays look for their light first.
+constexpr uint32_t kProbeLightMax = 64;
+// Bit of a shadow-queue id marking a probe's occlusion ray (the rest: its
+// NEE slot); its visibility goes to pendProbe instead of pendShadow.
+constexpr int32_t kProbeOccBit = 1 << 30;
 
 // Bit of a triangle's id (tris[3k].w) marking it for the any-hit alpha test
 // (raytrace.rahit:21-47); ids stay below 2^27.
@@ -110,6 +123,7 @@ struct FrameConst {
     // (pixel + frame offset) % nrcTrainEvery == 0 are training paths
     uint32_t nrc, nrcDepth, nrcTrainEvery, nrcTrainSlots;
     uint32_t nrcSuffix; // a training path ends into the cache this many non-specular vertices past its training vertex (0: never)
+    uint32_t probeFirstLight; // MIS probe rays find their light first and trace only an occlusion ray to it
 };
 
 // ------------------------------------------------------------ path guiding

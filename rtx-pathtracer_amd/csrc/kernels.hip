@@ -328,8 +328,9 @@ RTX_DI void reportBounce(uint32_t *poll, uint32_t ne, uint32_t np, uint32_t ns, 
 // light-side pdf (pdfLight, raytrace.rgen:334-339) and the BSDF-side weight.
 // base = throughput * f * cos / (numNEE * pdfMat), base.w = pdfMat.
 RTX_DI V3 probeContribution(const DevScene &S, bool power, V3 o, V3 d, int prim, float t, float bu, float bv,
-                            float4 base) {
+                            float4 base, bool &emitter) {
     float pdfMat = base.w;
+    emitter = false;
     if (prim < 0) {
         V3 lightColor = envColor(S, normalize(d));
         float pdfL = kInv2Pi / (float)S.lightCount;
@@ -339,6 +340,7 @@ RTX_DI V3 probeContribution(const DevScene &S, bool power, V3 o, V3 d, int prim,
     HitInfo hi = hitInfo(S, prim, t, bu, bv, o, d);
     const rtx_material &ms = S.materials[hi.matIndex];
     if (ms.type != RTX_MAT_LIGHT) return v3(0.0f);
+    emitter = true;
     int iLight = hi.isSphere ? S.spheres[hi.instanceIndex].iLight : S.instances[hi.instanceIndex].iLight;
     const rtx_light &light = S.lights[iLight];
     float cosL = dot(-d, hi.normal);
@@ -346,6 +348,45 @@ RTX_DI V3 probeContribution(const DevScene &S, bool power, V3 o, V3 d, int prim,
     float h = misWeight(power, pdfMat, pdfL);
     if (isnan(h)) return v3(0.0f);
     return v3(base) * v3(ms.lightColor) * h;
+}
+
+// The nearest emitter along (o, d) within [tmin, kTMax]: the emissive
+// spheres and S.lightTris, with the closest-hit traversal's rule (least t,
+// ties to the smaller primitive id) and its intersection tests on the same
+// ray set-up, so the hit is the one the traversal would report if nothing
+// else came first.  Returns the primitive id (tHit, bu, bv set) or -1.
+RTX_DI int nearestLight(const DevScene &S, V3 o, V3 d, float tmin, float &tHit, float &bu, float &bv) {
+    const TravRay r = makeTravRay(o, d);
+    int best = -1;
+    float tmax = kTMax;
+    bu = bv = 0.0f;
+    for (uint32_t k = 0; k < S.numSpheres; ++k) {
+        const rtx_sphere &sp = S.spheres[k];
+        if (S.materials[sp.materialIndex].type != RTX_MAT_LIGHT) continue;
+        float t;
+        if (isectSphere(sp, r.o, r.d, tmin, tmax, t)) {
+            const int id = (int)(S.numTris + k);
+            if (best < 0 || t < tmax || id < best) {
+                tmax = t;
+                best = id;
+            }
+        }
+    }
+    for (uint32_t k = 0; k < S.numLightTris; ++k) {
+        const float4 *tp = S.lightTris + 3 * (size_t)k;
+        float t, u, v;
+        if (isectTri(r, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) {
+            const int id = __float_as_int(tp[0].w) & ~kAlphaBit;
+            if (best < 0 || t < tmax || id < best) {
+                tmax = t;
+                best = id;
+                bu = u;
+                bv = v;
+            }
+        }
+    }
+    tHit = tmax;
+    return best;
 }
 
 // --------------------------------------------------- persistent traversal
@@ -380,7 +421,7 @@ RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue 
     const float4 *eOrg = qe.org, *eDir = qe.dir, *pOrg = qp.org, *pDir = qp.dir, *sOrg = qs.org, *sDir = qs.dir;
     const int *eId = qe.id, *pId = qp.id, *sId = qs.id;
     float4 *eHit = qe.hit, *pHit = qp.hit;
-    float4 *pendShadow = pb.pendShadow;
+    float4 *pendShadow = pb.pendShadow, *pendProbe = pb.pendProbe;
     const uint32_t *pathSeed = qe.pathSeed;
     const uint32_t secDiv = qs.seedDiv; // probe and shadow entries: path * numNEE + j
     persistentTrace<kHitLane, COUNT, ALPHA>(
@@ -408,14 +449,22 @@ RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue 
             L.tmax = d.w;
             L.any = shadow;
             if (ALPHA) {
-                id &= 0x7fffffffu;
+                id &= ext ? 0x7fffffffu : (uint32_t)~kProbeOccBit;
                 L.randomUInt = pathSeed[ext ? id : id / secDiv];
             }
         },
         [&](const LaneRay &L) {
             const bool ext = L.idx < ne;
             if (L.any) {
-                if (L.best < 0) pendShadow[sId[L.idx - nep]].w = 1.0f; // raytrace.shadow.rmiss:7-10
+                if (L.best < 0) { // raytrace.shadow.rmiss:7-10
+                    const int sid = sId[L.idx - nep];
+                    if (sid & kProbeOccBit) { // a probe's light is visible: its pending record becomes the light's id
+                        float4 &p = pendProbe[sid & ~kProbeOccBit];
+                        p.w = __int_as_float(-3 - __float_as_int(p.w));
+                    } else {
+                        pendShadow[sid].w = 1.0f;
+                    }
+                }
             } else if (ext) {
                 eHit[L.idx] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv);
             } else {
@@ -538,8 +587,9 @@ __global__ __launch_bounds__(256) void kProbeResolve(DevScene S, const FrameCons
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         float4 o4 = q.org[i], d4 = q.dir[i], base = q.aux[i], hit = q.hit[i];
         int prim = __float_as_int(hit.y);
-        pb.pendProbe[q.id[i]] = f4(probeContribution(S, power, v3(o4), v3(d4), prim, hit.x, hit.z, hit.w, base),
-                                   __int_as_float(prim));
+        bool emitter;
+        const V3 c = probeContribution(S, power, v3(o4), v3(d4), prim, hit.x, hit.z, hit.w, base, emitter);
+        pb.pendProbe[q.id[i]] = f4(c, __int_as_float(emitter ? prim : -1));
     }
 }
 
@@ -560,16 +610,20 @@ __global__ __launch_bounds__(256) void kProbeResolveM(MultiArgs ma, Queue q, Pat
             const SlotDesc &sd = ma.slots[k];
             float4 o4 = q.org[i], d4 = q.dir[i], base4 = q.aux[i], hit = q.hit[i];
             int prim = __float_as_int(hit.y);
-            pb.pendProbe[id] = f4(probeContribution(sd.S, sd.fc.pc.usePowerHeuristic != 0, v3(o4), v3(d4), prim,
-                                                    hit.x, hit.z, hit.w, base4),
-                                  __int_as_float(prim));
+            bool emitter;
+            const V3 c = probeContribution(sd.S, sd.fc.pc.usePowerHeuristic != 0, v3(o4), v3(d4), prim, hit.x, hit.z,
+                                           hit.w, base4, emitter);
+            pb.pendProbe[id] = f4(c, __int_as_float(emitter ? prim : -1));
         });
     }
 }
 
 // Pending NEE terms of the previous vertex, in NEE-sample order.
 // pendShadow.w: 0 no shadow ray, -1 traced and occluded, +1 traced and visible.
-// pendProbe.w (int bits): -2 no probe ray, else the probe's hit id (-1 = miss).
+// pendProbe.w (int bits): -2 no probe ray; the id of the emitter the probe
+// sees, or -1 when it sees none (its term is then zero or the environment's);
+// -3 - id while the occlusion ray to emitter `id` has not shown it visible
+// (the term does not count, and the path signature records -1).
 // NEE slot j of path pid is pid * stride + j (stride >= numNEE: FrameConst::neeStride).
 // (sh0, pr0): slot 0's records, which the caller may have fetched already.
 RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint32_t stride, uint32_t &hash,
@@ -580,8 +634,12 @@ RTX_DI V3 pendingSum(const PathBuffers &pb, uint32_t pid, uint32_t numNEE, uint3
         float4 pr = j == 0 ? pr0 : pb.pendProbe[pid * stride + j];
         if (sh.w != 0.0f) hash = sigFold(hash, sh.w > 0.0f ? kSigVisible : kSigOccluded);
         int probePrim = __float_as_int(pr.w);
-        if (probePrim != -2) hash = sigFold(hash, kSigProbe ^ (uint32_t)probePrim);
         V3 term = v3(pr);
+        if (probePrim <= -3) { // its light was occluded
+            probePrim = -1;
+            term = v3(0.0f);
+        }
+        if (probePrim != -2) hash = sigFold(hash, kSigProbe ^ (uint32_t)probePrim);
         if (sh.w > 0.0f) term = v3(sh) + term;
         s = s + term;
     }
@@ -871,12 +929,18 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
         } // not a split vertex
     }
     // ---- next event estimation, raytrace.rgen:601-731 (numNEE samples, multipleNEE 871-877)
+    // A probe ray (the BSDF-sampled half of MIS) adds only if the first thing
+    // it hits is an emitter.  Where the scene has few emitters and no
+    // environment, the probe finds its nearest emitter here and traces only
+    // an occlusion ray to it, in the shadow queue, instead of a closest-hit
+    // ray through the whole scene.
+    const bool lightFirst = fc.probeFirstLight && S.lightFirstOk && S.env == nullptr;
     const V3 wi = -d;
     const V3 thrNee = thr; // the throughput NEE terms of this vertex carry
     for (uint32_t j = 0; j < numNEE; ++j) {
-        bool wantShadow = false, wantProbe = false;
+        bool wantShadow = false, wantProbe = false, wantOcc = false;
         V3 lightDir = v3(0.0f), bsdfDir = v3(0.0f), probeBase = v3(0.0f);
-        float lightDist = 0.0f, pdfMatProbe = 0.0f;
+        float lightDist = 0.0f, pdfMatProbe = 0.0f, occMax = 0.0f;
         const uint32_t slot = pid * fc.neeStride + j;
         if (doNee) {
             V3 lightColor;
@@ -925,9 +989,32 @@ RTX_DI void shadeLane(const DevScene &S, const DevIC &IC, const DevGuiding &G, c
                 }
             }
             if (probeZero) pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-2));
+            if (wantProbe && lightFirst) {
+                float tl, lu, lv;
+                const int lp = nearestLight(S, h.worldPos, bsdfDir, kTMin, tl, lu, lv);
+                wantProbe = false;
+                if (lp >= 0) {
+                    bool emitter;
+                    const V3 c = probeContribution(S, power, h.worldPos, bsdfDir, lp, tl, lu, lv,
+                                                   f4(probeBase, pdfMatProbe), emitter);
+                    pb.pendProbe[slot] = f4(c, __int_as_float(-3 - lp)); // counts once shown visible
+                    wantOcc = true;
+                    occMax = __int_as_float(__float_as_int(tl) - 1); // below the light: t < tl
+                } else {
+                    pb.pendProbe[slot] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+                }
+            }
         }
         int si, pi;
         waveAppend2(wantShadow, &cnt->shadow, wantProbe, &cnt->probe, si, pi);
+        if (lightFirst) {
+            const int oi = waveAppend(wantOcc, &cnt->shadow);
+            if (wantOcc) {
+                qs.id[oi] = (int)slot | kProbeOccBit;
+                qs.org[oi] = f4(h.worldPos, kTMin);
+                qs.dir[oi] = f4(bsdfDir, occMax);
+            }
+        }
         if (wantShadow) {
             qs.id[si] = (int)slot;
             qs.org[si] = f4(h.worldPos, kTMin);
