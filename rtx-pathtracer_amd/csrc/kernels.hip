@@ -529,7 +529,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceM(MultiArgs ma, Queue qe, i
     const float4 *eOrg = qe.org, *eDir = qe.dir, *pOrg = qp.org, *pDir = qp.dir, *sOrg = qs.org, *sDir = qs.dir;
     const int *eId = qe.id, *pId = qp.id, *sId = qs.id;
     float4 *eHit = qe.hit, *pHit = qp.hit;
-    float4 *pendShadow = pb.pendShadow;
+    float4 *pendShadow = pb.pendShadow, *pendProbe = pb.pendProbe;
     const uint32_t *frameSeed = pb.frameSeed;
     persistentTrace<kHitLane, false, ALPHA>(
         ls, nep + ns, &cnt->fetch[1], st, vc,
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceM(MultiArgs ma, Queue qe, i
                 d = pDir[j];
                 id = (uint32_t)pId[j];
             }
-            const uint32_t pid = ext ? (id & ~kSplitVertex) : id / ma.neeStride;
+            const uint32_t pid = ext ? (id & ~kSplitVertex) : (id & ~(uint32_t)kProbeOccBit) / ma.neeStride;
             const DevScene &sc = ma.slots[pid >> ma.slotShift].S;
             pointAt(ls, sc, &sc);
             L.r = makeTravRay(v3(o), v3(d));
@@ -562,7 +562,15 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceM(MultiArgs ma, Queue qe, i
         },
         [&](const LaneRay &L) {
             if (L.any) {
-                if (L.best < 0) pendShadow[sId[L.idx - nep]].w = 1.0f; // raytrace.shadow.rmiss:7-10
+                if (L.best < 0) { // raytrace.shadow.rmiss:7-10
+                    const int sid = sId[L.idx - nep];
+                    if (sid & kProbeOccBit) { // a probe's light is visible
+                        float4 &p = pendProbe[sid & ~kProbeOccBit];
+                        p.w = __int_as_float(-3 - __float_as_int(p.w));
+                    } else {
+                        pendShadow[sid].w = 1.0f;
+                    }
+                }
             } else if (L.idx < ne) {
                 eHit[L.idx] = make_float4(L.tmax, __int_as_float(L.best), L.bu, L.bv);
             } else {

@@ -233,6 +233,7 @@ FrameConst jobFrame(const rtx_job_desc &d, uint32_t neeStride) {
     fc.useADRRS = 0;
     fc.splitOnFirst = 0;
     fc.nrc = 0;
+    fc.probeFirstLight = 1; // taken for slots whose scene has few emitters and no environment map
     return fc;
 }
 
@@ -569,9 +570,9 @@ int rtx_server_create(rtx_ctx *ctx, const rtx_server_config *c, rtx_server **out
         al(&q.dir, P);
         al(&q.hit, P);
     }
-    al(&sv->qs.id, PN);
-    al(&sv->qs.org, PN);
-    al(&sv->qs.dir, PN);
+    al(&sv->qs.id, 2 * PN); // a shadow ray and a probe's occlusion ray per NEE slot
+    al(&sv->qs.org, 2 * PN);
+    al(&sv->qs.dir, 2 * PN);
     al(&sv->qp.id, PN);
     al(&sv->qp.org, PN);
     al(&sv->qp.dir, PN);
