@@ -1057,6 +1057,7 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
     fc.nrcDepth = nrc ? ctx->nrcCfg.terminateBounces + 1u : 0u;
     fc.nrcSuffix = nrc ? ctx->nrcCfg.trainSuffixBounces : 0u;
     fc.probeFirstLight = 1u; // taken where the scene has few emitters and no environment map
+    const bool probesLightFirst = scene->view.lightFirstOk && scene->view.env == nullptr;
     fc.nrcTrainEvery = nrcEvery;
     fc.nrcTrainSlots = nrcSlots;
     CHECK_HIP(ctx, hipMemcpyAsync(ctx->fc, &fc, sizeof(fc), hipMemcpyHostToDevice, ctx->stream));
@@ -1154,7 +1155,8 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
                              traceBlocks, ctx->tuning.ldsStackEntries);
             });
             CHECK_HIP(ctx, hipEventRecord(ctx->pollEv[it % kPollSlots], s));
-            timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, qsel, shadeBlocks); });
+            if (!probesLightFirst) // otherwise every probe traced an occlusion ray, and the probe queue is empty
+                timed(kClsProbe, [&] { launchProbeResolve(s, S, fcd, ctx->qp, ctx->pb, ctx->cnt, qsel, shadeBlocks); });
             timed(kClsShade, [&] {
                 launchFinish(s, fcd, b, ctx->pb, ctx->done, ctx->cnt, ctx->q[qsel ^ 1], qsel, shadeBlocks, second);
             });

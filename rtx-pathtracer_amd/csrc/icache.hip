@@ -444,23 +444,31 @@ __global__ __launch_bounds__(256) void kIcIndexGather(IcBuffers ic, const uint32
     }
 }
 
-__global__ __launch_bounds__(1024) void kIcIndexScan(IcBuffers ic) {
+// The bucket starts, an exclusive scan of the counts, in three passes over
+// blocks of 1024 buckets: each block's total (kIcBucketSums), a scan of the
+// totals (kScanInPlace), and each block's own scan plus its offset
+// (kIcBucketStarts, whose last block also writes the item count).
+__global__ __launch_bounds__(1024) void kIcBucketSums(IcBuffers ic) {
     __shared__ uint32_t sh[1024 / 64 + 1];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < ic.nb; base += 1024) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < ic.nb ? ic.bucketCount[i] : 0u;
-        uint32_t total;
-        uint32_t ex = blockScanExclusive<1024>(v, sh, total);
-        if (i < ic.nb) ic.bucketStart[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0) {
-        ic.bucketStart[ic.nb] = carry;
-        ic.meta[1] = carry;
+    const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+    uint32_t total;
+    blockScanExclusive<1024>(i < ic.nb ? ic.bucketCount[i] : 0u, sh, total);
+    if (threadIdx.x == 0) ic.blockSums[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(1024) void kIcBucketStarts(IcBuffers ic) {
+    __shared__ uint32_t sh[1024 / 64 + 1];
+    const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+    uint32_t total;
+    const uint32_t ex = blockScanExclusive<1024>(i < ic.nb ? ic.bucketCount[i] : 0u, sh, total);
+    const uint32_t base = ic.blockSums[blockIdx.x];
+    if (i < ic.nb) ic.bucketStart[i] = base + ex;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        ic.bucketStart[ic.nb] = base + total;
+        ic.meta[1] = base + total;
         ic.meta[3] = 27u * min(ic.hdr->nextCacheSlot, ic.hdr->maxCaches); // slots of the records in use
     }
 }
+
 
 // ------------------------------------------------------------- launchers
 static int blocksFor(size_t n, int block) { return (int)((n + block - 1) / block); }
@@ -494,7 +502,10 @@ void launchIcIndex(hipStream_t s, IcBuffers ic, float c0) {
     hipLaunchKernelGGL(kIcIndexClear, dim3(blocksFor(ic.nb, 256)), dim3(256), 0, s, ic, c0);
     hipLaunchKernelGGL(kIcIndexLevels, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
     hipLaunchKernelGGL(kIcIndexFill<false>, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
-    hipLaunchKernelGGL(kIcIndexScan, dim3(1), dim3(1024), 0, s, ic);
+    const uint32_t nblk = (ic.nb + 1023) / 1024;
+    hipLaunchKernelGGL(kIcBucketSums, dim3(nblk), dim3(1024), 0, s, ic);
+    hipLaunchKernelGGL(kScanInPlace<0>, dim3(1), dim3(1024), 0, s, ic.blockSums, nblk, (uint32_t *)nullptr);
+    hipLaunchKernelGGL(kIcBucketStarts, dim3(nblk), dim3(1024), 0, s, ic);
     hipLaunchKernelGGL(kIcIndexFill<true>, dim3(blocksFor(ic.maxEntries, 256)), dim3(256), 0, s, ic, c0);
     // stable LSD radix sort of (bucket, slot) over the slots of the records in
     // use (meta[3]; keys below 2^(bucketBits + 1): nb itself marks unused

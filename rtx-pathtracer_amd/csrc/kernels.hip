@@ -411,6 +411,10 @@ RTX_DI void traceBody(const DevScene &S, const Queue &qe, int esel, const Queue 
         atomicAdd(&cnt->probeRays, (unsigned long long)np);
         atomicAdd(&cnt->shadowRays, (unsigned long long)ns);
         reportBounce(poll, ne, np, ns, cnt->done[esel ^ 1]);
+        // the lists the bounce's kFinish and kShade append to: their previous
+        // contents (two bounces back) are consumed
+        cnt->extend[esel ^ 1] = 0;
+        cnt->done[esel] = 0;
     }
     HybridStack<kTraceBlock, LDS> st;
     int spill[HybridStack<kTraceBlock, LDS>::kSpill];
@@ -518,6 +522,10 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceM(MultiArgs ma, Queue qe, i
         atomicAdd(&cnt->probeRays, (unsigned long long)np);
         atomicAdd(&cnt->shadowRays, (unsigned long long)ns);
         reportBounce(poll, ne, np, ns, cnt->done[esel ^ 1]);
+        // the lists the bounce's kFinish and kShade append to: their previous
+        // contents (two bounces back) are consumed
+        cnt->extend[esel ^ 1] = 0;
+        cnt->done[esel] = 0;
     }
     if (blockIdx.x == 0 && poll)
         for (uint32_t k = threadIdx.x; k < ma.numSlots; k += kTraceBlock) poll[4 + k] = ma.slotAlive[k];
@@ -580,16 +588,10 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceM(MultiArgs ma, Queue qe, i
 }
 
 // The BSDF-side half of the MIS probe (raytrace.rgen:684-726) for probe rays
-// whose closest hits the trace kernels wrote.  Also clears, for bounce qsel,
-// the counters of the next bounce's extend queue and of this bounce's done
-// list: their previous contents (two bounces back) are consumed, and the
-// kernels that append to them (kFinish, kShade) run after this one.
+// whose closest hits the trace kernels wrote (probes that found their
+// emitter first never enter this queue).
 __global__ __launch_bounds__(256) void kProbeResolve(DevScene S, const FrameConst *__restrict__ fcp, Queue q,
                                                      PathBuffers pb, Counters *cnt, int qsel) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        cnt->extend[qsel ^ 1] = 0;
-        cnt->done[qsel] = 0;
-    }
     const uint32_t n = cnt->probe;
     const bool power = fcp->pc.usePowerHeuristic != 0;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -604,10 +606,6 @@ __global__ __launch_bounds__(256) void kProbeResolve(DevScene S, const FrameCons
 // The same over the serving pool: probe entry ids are NEE slots, path * neeStride + j.
 __global__ __launch_bounds__(256) void kProbeResolveM(MultiArgs ma, Queue q, PathBuffers pb, Counters *cnt,
                                                       int qsel) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        cnt->extend[qsel ^ 1] = 0;
-        cnt->done[qsel] = 0;
-    }
     const uint32_t n = cnt->probe;
     for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
         const uint32_t i = base + threadIdx.x;
