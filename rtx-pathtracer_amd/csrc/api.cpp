@@ -1262,19 +1262,27 @@ int rtx_render(rtx_ctx *ctx, const rtx_scene *scene, const rtx_frame_params *p, 
             timed(kClsShade, [&] {
                 launchIcReduce(s, ctx->ic, frameSeed, 0, ctx->ic.maxEntries, ctx->pb, kIcJobsUpdates, (uint32_t)batch);
             });
-            timed(kClsShade, [&] { launchIcPlan(s, ctx->ic, kIcPlanCreations); });
-            BatchDesc b{};
-            b.numPaths = np;
-            b.npix = np;
-            timed(kClsShade, [&] {
-                launchGenerateIC(s, ctx->ic, frameSeed, 0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, kIcJobsCreations);
-            });
-            setRaySeeds(neeCap);
-            int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
-            if (rcs != RTX_OK) return rcs;
-            timed(kClsShade, [&] {
-                launchIcReduce(s, ctx->ic, frameSeed, 0, ctx->ic.maxEntries, ctx->pb, kIcJobsCreations);
-            });
+            // Once an earlier commit left every record slot in use (the flag
+            // kIcCommit sets in pinned memory; a value not yet seen reads as 0)
+            // the frame has no creation job, and its creation loop would run
+            // empty bounces: it is skipped (the plan's update part already set
+            // the job count to the updates').
+            const bool cacheFull = *reinterpret_cast<volatile uint32_t *>(ctx->ic.fullHost) != 0u;
+            if (!cacheFull) {
+                timed(kClsShade, [&] { launchIcPlan(s, ctx->ic, kIcPlanCreations); });
+                BatchDesc b{};
+                b.numPaths = np;
+                b.npix = np;
+                timed(kClsShade, [&] {
+                    launchGenerateIC(s, ctx->ic, frameSeed, 0, np, ctx->pb, ctx->q[0], ctx->cnt, 0, kIcJobsCreations);
+                });
+                setRaySeeds(neeCap);
+                int rcs = bounces(ctx->fcIC, b, maxIterIC, false);
+                if (rcs != RTX_OK) return rcs;
+                timed(kClsShade, [&] {
+                    launchIcReduce(s, ctx->ic, frameSeed, 0, ctx->ic.maxEntries, ctx->pb, kIcJobsCreations);
+                });
+            }
             timed(kClsShade, [&] { launchIcCommit(s, ctx->ic, ctx->fcIC); });
             timed(kClsShade, [&] { launchIcIndex(s, ctx->ic, c0); });
             CHECK_HIP(ctx, hipGetLastError());
@@ -1726,6 +1734,7 @@ int rtx_guiding_read_samples(rtx_ctx *ctx, rtx_directional_data *out, size_t cou
 // ------------------------------------------------------- irradiance cache
 static void icFree(rtx_ctx *ctx) {
     freeAll(ctx->icAllocs);
+    if (ctx->ic.fullHost) (void)hipHostFree(ctx->ic.fullHost);
     ctx->ic = IcBuffers{};
     ctx->icValid = false;
 }
@@ -1766,6 +1775,8 @@ int rtx_ic_create(rtx_ctx *ctx, uint32_t maxEntries) try {
     if (e == hipSuccess) e = devAlloc(&ic.jobs, 2 * (size_t)ic.jobCap, L);
     if (e == hipSuccess) e = devAlloc(&ic.jobRes, 10 * (size_t)ic.jobCap, L);
     if (e == hipSuccess) e = devAlloc(&ic.blockSums, ic.blockCap, L);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ic.fullHost, sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) *ic.fullHost = 0u;
     if (e == hipSuccess && !ctx->fcIC) e = devMalloc(&ctx->fcIC, sizeof(FrameConst));
     if (e != hipSuccess) {
         icFree(ctx);
@@ -1783,6 +1794,7 @@ int rtx_ic_reset(rtx_ctx *ctx) try {
     launchIcInit(ctx->stream, ctx->ic);
     CHECK_HIP(ctx, hipGetLastError());
     CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *ctx->ic.fullHost = 0u;
     ctx->icValid = true;
     ctx->keptJobs = ctx->keptUpdates = 0;
     return RTX_OK;
@@ -1853,6 +1865,7 @@ int rtx_ic_write(rtx_ctx *ctx, const rtx_cache_header *hdr, const rtx_cache_data
     CHECK_HIP(ctx, hipMemcpy(ctx->ic.hdr, hdr, sizeof(rtx_cache_header), hipMemcpyHostToDevice));
     CHECK_HIP(ctx, hipMemcpy(ctx->ic.entries, entries, n * sizeof(rtx_cache_data), hipMemcpyHostToDevice));
     CHECK_HIP(ctx, hipMemcpy(ctx->ic.spheres, spheres, n * sizeof(rtx_sphere), hipMemcpyHostToDevice));
+    *ctx->ic.fullHost = 0u; // the next commit tells again
     return RTX_OK;
 } catch (...) {
     return rtx::abiFail(ctx);

@@ -268,8 +268,7 @@ RTX_DI void put3(float *d, V3 v) {
 
 // Updates (raytrace.rgen:1347-1380) and creations (1383-1421, slots kept
 // below maxCaches) in job order.
-__global__ void kIcCommit(IcBuffers ic, const FrameConst *__restrict__ fcp) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+RTX_DI void commitJobs(IcBuffers ic, const FrameConst *__restrict__ fcp) {
     const rtx_push_constant &pc = fcp->pc;
     const uint32_t numJobs = ic.frame[4];
     for (uint32_t j = 0; j < numJobs; ++j) {
@@ -311,6 +310,12 @@ __global__ void kIcCommit(IcBuffers ic, const FrameConst *__restrict__ fcp) {
             sp.radius = pc.irradianceA * h;
         }
     }
+}
+
+__global__ void kIcCommit(IcBuffers ic, const FrameConst *__restrict__ fcp) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    commitJobs(ic, fcp);
+    if (ic.fullHost) *reinterpret_cast<volatile uint32_t *>(ic.fullHost) = ic.hdr->nextCacheSlot >= ic.hdr->maxCaches;
 }
 
 // The freshly created cache (IrradianceCache::createBuffers,

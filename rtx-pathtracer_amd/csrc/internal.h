@@ -248,6 +248,10 @@ struct IcBuffers {
     float *jobRes;               // 10 per job: color, rot, trans, harmonicR
     uint32_t *blockSums;
     uint32_t blockCap;
+    // pinned host word: 1 once a commit has left every record slot in use
+    // (the cache then takes no creation job again until it is reset or
+    // written); 0 before, or not yet known
+    uint32_t *fullHost;
 };
 constexpr int kMaxNewCacheEntries = 5; // MAX_NEW_IRRADIANCE_ENTRIES, raytrace.rgen:65
 constexpr int kIcStrataPhi = 20;       // N, raytrace.rgen:1232
