@@ -365,3 +365,31 @@ def test_ray_counts(renderer):
     st = renderer.render(ds, fp)
     assert st.extendRays == st.paths and 0 < st.shadowRays <= 2 * st.paths and st.probeRays == 0
     ds.close()
+
+
+def test_light_first_probes_equal_traced_probes(renderer):
+    """MIS probes that find their emitter in the shading kernel (the Cornell
+    box: two emissive triangles, no environment map) against probes traced
+    to their closest hit (the same box with a black environment map that is
+    not a light, which turns light-first probes off): every path has the
+    same signature and the same radiance to FP32 rounding."""
+    hs = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
+    he = rtxamd.HostScene.generate(abi.SCENE_CORNELL)
+    he.set_environment(np.zeros((8, 16, 4), np.float32), add_light=False)
+    n = 48 * 48 * 4
+    out = []
+    for h in (hs, he):
+        ds = renderer.upload(h)
+        st = renderer.render(ds, scenes.params(h, 48, 48, 4, maxDepth=6, enableMIS=1,
+                                               flags=abi.RTX_FRAME_KEEP_PATHS | abi.RTX_FRAME_RESET_ACCUM))
+        rgb, sig = renderer.read_paths(n)
+        out.append((rgb, sig, st.probeRays, st.shadowRays))
+        ds.close()
+    (a, sa, pa, za), (b, sb, pb, zb) = out
+    exact = np.all(a == b, axis=1).mean()
+    rel = np.abs(a - b).max() / max(np.abs(b).max(), 1e-6)
+    log(f"light-first probes: {za} any-hit rays, 0 probes traced; traced probes: {pb} probes, {zb} shadow rays; "
+        f"paths bit-equal {exact:.4f}, max relative difference {rel:.2e}")
+    assert pa == 0 and pb > 0 and za > zb
+    assert np.array_equal(sa, sb)
+    assert rel <= 1e-6
