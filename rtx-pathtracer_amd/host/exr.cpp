@@ -6,6 +6,7 @@
 // UINT channels.
 #include <zlib.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -129,23 +130,32 @@ bool readExr(const std::string &path, std::vector<float> &rgba, uint32_t &w, uin
         return false;
     }
     std::vector<char> data((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    // Every read is checked against the end of the file: a malformed or
+    // truncated file is an error, never a read past the buffer.
     size_t p = 0;
-    auto need = [&](size_t n) { return p + n <= data.size(); };
-    auto rd32 = [&]() {
-        int32_t v;
+    auto need = [&](size_t n) { return p <= data.size() && n <= data.size() - p; };
+    auto rd32 = [&](int32_t &v) {
+        if (!need(4)) return false;
         std::memcpy(&v, &data[p], 4);
         p += 4;
-        return v;
+        return true;
     };
-    if (!need(8) || rd32() != 20000630) {
-        err = "not an OpenEXR file";
+    auto rdStr = [&](size_t limit, std::string &s) {
+        const size_t lim = std::min(limit, data.size());
+        if (p >= lim) return false;
+        const void *z = std::memchr(&data[p], 0, lim - p);
+        if (!z) return false;
+        s.assign(&data[p], (size_t)((const char *)z - &data[p]));
+        p += s.size() + 1;
+        return true;
+    };
+    auto bad = [&](const char *what) {
+        err = what;
         return false;
-    }
-    int32_t version = rd32();
-    if (version & 0x200) {
-        err = "tiled EXR not supported";
-        return false;
-    }
+    };
+    int32_t magic = 0, version = 0;
+    if (!rd32(magic) || magic != 20000630 || !rd32(version)) return bad("not an OpenEXR file");
+    if (version & 0x200) return bad("tiled EXR not supported");
     struct Ch {
         std::string name;
         int type;
@@ -154,35 +164,38 @@ bool readExr(const std::string &path, std::vector<float> &rgba, uint32_t &w, uin
     int compression = -1;
     int32_t dw[4] = {0, 0, -1, -1};
     while (need(1) && data[p] != 0) {
-        std::string name(&data[p]);
-        p += name.size() + 1;
-        std::string type(&data[p]);
-        p += type.size() + 1;
-        int32_t size = rd32();
-        if (!need((size_t)size)) break;
-        size_t end = p + (size_t)size;
+        std::string name, type;
+        int32_t size = 0;
+        if (!rdStr(data.size(), name) || !rdStr(data.size(), type) || !rd32(size) || size < 0 ||
+            !need((size_t)size))
+            return bad("truncated EXR header");
+        const size_t end = p + (size_t)size;
         if (name == "channels") {
             while (p < end && data[p] != 0) {
-                std::string cn(&data[p]);
-                p += cn.size() + 1;
-                int32_t t = rd32();
+                std::string cn;
+                int32_t t = 0;
+                if (!rdStr(end, cn) || !rd32(t) || p + 12 > end) return bad("bad EXR channel list");
+                if (t < 0 || t > 2) return bad("bad EXR channel type");
                 p += 4 + 8;
                 chans.push_back({cn, t});
             }
         } else if (name == "compression") {
+            if (size < 1) return bad("bad EXR compression attribute");
             compression = (unsigned char)data[p];
         } else if (name == "dataWindow") {
-            for (int i = 0; i < 4; ++i) dw[i] = rd32();
+            if (size < 16) return bad("bad EXR dataWindow attribute");
+            for (int i = 0; i < 4; ++i) rd32(dw[i]);
         }
         p = end;
     }
+    if (!need(1)) return bad("truncated EXR header");
     p++; // end of header
-    if (dw[2] < dw[0] || dw[3] < dw[1] || chans.empty()) {
-        err = "bad EXR header";
-        return false;
-    }
-    w = (uint32_t)(dw[2] - dw[0] + 1);
-    h = (uint32_t)(dw[3] - dw[1] + 1);
+    const int64_t w64 = (int64_t)dw[2] - dw[0] + 1, h64 = (int64_t)dw[3] - dw[1] + 1;
+    if (w64 <= 0 || h64 <= 0 || chans.empty()) return bad("bad EXR header");
+    // 2^28 pixels (4 GiB of RGBA floats) is far above any environment map.
+    if (w64 * h64 > (int64_t(1) << 28)) return bad("EXR image too large");
+    w = (uint32_t)w64;
+    h = (uint32_t)h64;
     int linesPerBlock;
     if (compression == 0 || compression == 2) linesPerBlock = 1;
     else if (compression == 3) linesPerBlock = 16;
@@ -191,11 +204,8 @@ bool readExr(const std::string &path, std::vector<float> &rgba, uint32_t &w, uin
         return false;
     }
     uint32_t blocks = (h + linesPerBlock - 1) / linesPerBlock;
+    if (!need((size_t)blocks * 8)) return bad("truncated EXR");
     std::vector<uint64_t> offsets(blocks);
-    if (!need((size_t)blocks * 8)) {
-        err = "truncated EXR";
-        return false;
-    }
     std::memcpy(offsets.data(), &data[p], (size_t)blocks * 8);
     std::vector<int> bpp(chans.size());
     size_t pixelBytes = 0;
@@ -208,30 +218,25 @@ bool readExr(const std::string &path, std::vector<float> &rgba, uint32_t &w, uin
     for (size_t i = 0; i < (size_t)w * h; ++i) rgba[i * 4 + 3] = 1.0f;
     std::vector<unsigned char> raw, tmp;
     for (uint32_t b = 0; b < blocks; ++b) {
+        if (offsets[b] > data.size()) return bad("bad EXR block offset");
         p = (size_t)offsets[b];
-        if (!need(8)) {
-            err = "truncated EXR block";
-            return false;
-        }
-        int32_t y0 = rd32() - dw[1];
-        int32_t size = rd32();
-        if (!need((size_t)size)) {
-            err = "truncated EXR block";
-            return false;
-        }
+        int32_t y = 0, size = 0;
+        if (!rd32(y) || !rd32(size) || size < 0 || !need((size_t)size)) return bad("truncated EXR block");
+        const int64_t y0 = (int64_t)y - dw[1];
+        if (y0 < 0 || y0 >= (int64_t)h) return bad("EXR block outside the data window");
         uint32_t lines = std::min<uint32_t>((uint32_t)linesPerBlock, h - (uint32_t)y0);
         size_t expect = (size_t)lines * w * pixelBytes;
         raw.resize(expect);
-        if (compression == 0 || (size_t)size == expect) {
+        if ((size_t)size == expect) {
             std::memcpy(raw.data(), &data[p], expect);
+        } else if (compression == 0) {
+            return bad("EXR block of the wrong size");
         } else {
             tmp.resize(expect);
             uLongf outLen = (uLongf)expect;
             if (uncompress(tmp.data(), &outLen, (const Bytef *)&data[p], (uLong)size) != Z_OK ||
-                outLen != expect) {
-                err = "zlib error in EXR block";
-                return false;
-            }
+                outLen != expect)
+                return bad("zlib error in EXR block");
             for (size_t i = 1; i < expect; ++i) tmp[i] = (unsigned char)(tmp[i - 1] + tmp[i] - 128);
             size_t half = (expect + 1) / 2;
             for (size_t i = 0; i < expect; ++i)

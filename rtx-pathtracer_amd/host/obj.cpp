@@ -263,7 +263,8 @@ bool objParseDouble(const char *s, const char *end, double *out) {
             }
             int digits = 0;
             while (p != end && isDigit(*p)) {
-                exponent = exponent * 10 + (*p - '0');
+                // Held below 10^6: any larger exponent already gives inf or 0.
+                if (exponent < 100000) exponent = exponent * 10 + (*p - '0');
                 ++p;
                 ++digits;
             }

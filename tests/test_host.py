@@ -133,6 +133,35 @@ def test_exr_roundtrip(tmp_path):
         rtxamd.read_exr(tmp_path / "missing.exr")
 
 
+def test_exr_reader_rejects_malformed_files(tmp_path):
+    """Truncated files, a negative attribute size, a block outside the data
+    window and an uncompressed block of the wrong size are errors, never
+    reads past the file (each of these read out of bounds before)."""
+    img = np.arange(3 * 2 * 4, dtype=np.float32).reshape(2, 3, 4)
+    good = tmp_path / "good.exr"
+    rtxamd.write_exr(good, img)
+    data = good.read_bytes()
+    bad = tmp_path / "bad.exr"
+
+    def rejects(b):
+        bad.write_bytes(bytes(b))
+        with pytest.raises(rtxamd.RtxError):
+            rtxamd.read_exr(bad)
+
+    for n in range(len(data)):
+        rejects(data[:n])
+    i = data.index(b"channels\0chlist\0") + len(b"channels\0chlist\0")
+    rejects(data[:i] + (-1).to_bytes(4, "little", signed=True) + data[i + 4:])
+    # two blocks of one line (line number, byte count, 3 pixels of R, G, B
+    # floats) after a table of two 8-byte offsets
+    table = len(data) - 2 * (8 + 3 * 3 * 4) - 2 * 8
+    first = int.from_bytes(data[table:table + 8], "little")
+    for y in (2, -1, 1 << 30):
+        rejects(data[:first] + y.to_bytes(4, "little", signed=True) + data[first + 4:])
+    rejects(data[:first + 4] + (35).to_bytes(4, "little") + data[first + 8:])
+    rtxamd.read_exr(good)
+
+
 def test_reference_exr_reads_if_mounted():
     """Mitsuba's ZIP/HALF ground-truth EXRs decode (CPU only, skipped when the
     reference is not mounted)."""

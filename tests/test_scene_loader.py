@@ -163,9 +163,10 @@ def test_obj_out_of_range_index_is_an_error(tmp_path):
 
 def test_obj_numbers(tmp_path):
     """tinyobjloader's number reader: signs, leading dots, exponents; text
-    that is not a number reads as 0."""
-    vals = ["1e-3", ".5", "-.25", "+3", "1.5e+2", "123.456", "nan", "-0", "2.5E-1"]
-    expect = [1e-3, 0.5, -0.25, 3.0, 150.0, 123.456, 0.0, 0.0, 0.25]
+    that is not a number reads as 0; an exponent too long for an int
+    (undefined behaviour in tinyobjloader) reads as 0 or inf."""
+    vals = ["1e-3", ".5", "-.25", "+3", "1.5e+2", "123.456", "nan", "-0", "2.5E-1", "7e-99999999999"]
+    expect = [1e-3, 0.5, -0.25, 3.0, 150.0, 123.456, 0.0, 0.0, 0.25, 0.0]
     lines = [f"v {v} 0 0" for v in vals] + ["vn 0 0 1"] + [f"f {i + 1}//1 {i + 1}//1 {i + 1}//1" for i in range(len(vals))]
     m = models(load_obj(tmp_path, "\n".join(lines) + "\n"))[0]
     got = sorted(set(m["pos"][:, 0].tolist()))
